@@ -1,0 +1,203 @@
+"""ctypes binding of the CPU oracle (liborc.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The functions mirror the reference's
+``c_channel`` calls (channel/channel.i) on numpy arrays.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib_path():
+    return os.path.join(_HERE, "liborc.so")
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE, "-B"], check=True)
+
+
+def load():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = lib_path()
+    if not os.path.exists(path):
+        build()
+    L = ctypes.CDLL(path)
+    c = ctypes
+    dp = c.POINTER(c.c_double)
+    L.orc_init.argtypes = [c.c_int, c.c_int, c.c_int, c.c_double, c.c_double,
+                           c.c_double, c.c_double, c.c_double, c.c_int]
+    L.orc_init.restype = c.c_int
+    L.orc_primal.argtypes = [c.c_int, c.c_void_p]
+    L.orc_primal.restype = c.c_int
+    L.orc_tangent.argtypes = [c.c_int, c.c_int, c.c_void_p, c.c_int]
+    L.orc_tangent.restype = c.c_int
+    L.orc_adjoint.argtypes = [c.c_int, c.c_int, c.c_void_p, c.c_int, c.c_double]
+    L.orc_adjoint.restype = c.c_int
+    L.orc_ddt_project.argtypes = [c.c_int, c.c_void_p]
+    L.orc_ddt_project.restype = c.c_double
+    L.orc_spec2phys.argtypes = [c.c_void_p, c.c_void_p]
+    L.orc_getsoln.argtypes = [c.c_int]
+    L.orc_getsoln.restype = c.c_void_p
+    L.orc_getincresoln.argtypes = [c.c_int]
+    L.orc_getincresoln.restype = c.c_void_p
+    L.orc_quad.argtypes = [c.c_void_p, c.c_void_p]
+    L.orc_get_array.argtypes = [c.c_char_p, dp, c.c_int]
+    L.orc_get_array.restype = c.c_int
+    L.orc_get_scalar.argtypes = [c.c_char_p]
+    L.orc_get_scalar.restype = c.c_double
+    L.orc_inv2d.argtypes = [c.c_void_p, c.c_void_p]
+    L.orc_fwd2d.argtypes = [c.c_void_p, c.c_void_p]
+    L.orc_band_solve.argtypes = [c.c_void_p, c.c_void_p, c.c_int, c.c_int]
+    L.orc_pass1_state.argtypes = [c.c_void_p, c.c_int]
+    L.orc_pass1_state.restype = c.c_int
+    L.orc_pass2_state.argtypes = [c.c_void_p, c.c_void_p]
+    L.orc_coef_to_gauss.argtypes = [c.c_void_p, c.c_void_p]
+    L.orc_get_state.argtypes = [c.c_void_p, c.c_void_p]
+    L.orc_set_faithful_cost.argtypes = [c.c_int]
+    L.orc_srand.argtypes = [c.c_uint]
+    L.orc_rand.restype = c.c_int
+    for f in ("orc_qpts", "orc_dimR", "orc_dimQ", "orc_Nx", "orc_Nz",
+              "orc_nsteps", "orc_last_status"):
+        getattr(L, f).restype = c.c_int
+    _LIB = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check_c(a):
+    assert a.dtype == np.complex128 and a.flags.c_contiguous
+    return a
+
+
+class Oracle:
+    """One simulation (the oracle, like the reference, is process-global)."""
+
+    def __init__(self, Nx, Ny, Nz, Lx, Lz, Re, flux, dt, nsteps):
+        self.L = load()
+        rc = self.L.orc_init(Nx, Ny, Nz, Lx, Lz, Re, flux, dt, nsteps)
+        if rc != 0:
+            raise ValueError("oracle init failed")
+        self.Nx, self.Ny, self.Nz = Nx, Ny, Nz
+        self.qpts = self.L.orc_qpts()
+        self.dimR = self.L.orc_dimR()
+        self.dimQ = self.L.orc_dimQ()
+        self.nsteps = nsteps
+        self.cshape = (Nz, 2, self.dimR, Nx // 2)
+        self.ushape = (Nz, 5, self.qpts, Nx // 2)
+
+    def primal(self, ru_steps, C=None):
+        if C is None:
+            return self.L.orc_primal(ru_steps, None)
+        return self.L.orc_primal(ru_steps, _ptr(_check_c(C)))
+
+    def tangent(self, start, end, IC, inhomo):
+        return self.L.orc_tangent(start, end, _ptr(_check_c(IC)), inhomo)
+
+    def adjoint(self, start, end, AC, inhomo, strength):
+        return self.L.orc_adjoint(start, end, _ptr(_check_c(AC)), inhomo, strength)
+
+    def ddt_project(self, i, v):
+        return self.L.orc_ddt_project(i, _ptr(_check_c(v)))
+
+    def spec2phys(self, C):
+        flow = np.zeros((3, 3 * self.Nx // 2, self.qpts, 3 * self.Nz // 2))
+        self.L.orc_spec2phys(_ptr(_check_c(np.ascontiguousarray(C))), _ptr(flow))
+        return flow
+
+    def _view(self, p):
+        n = int(np.prod(self.cshape))
+        buf = (ctypes.c_double * (2 * n)).from_address(p)
+        return np.frombuffer(buf, dtype=np.complex128).reshape(self.cshape)
+
+    def getsoln(self, i):
+        p = self.L.orc_getsoln(i)
+        assert p
+        return self._view(p).copy()
+
+    def getincresoln(self, i):
+        p = self.L.orc_getincresoln(i)
+        assert p
+        return self._view(p).copy()
+
+    def quad(self):
+        y = np.zeros(self.qpts)
+        w = np.zeros(self.qpts)
+        self.L.orc_quad(_ptr(y), _ptr(w))
+        return y, w
+
+    def array(self, name):
+        out = np.zeros(1 << 20)
+        n = self.L.orc_get_array(name.encode(), out.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double)), out.size)
+        assert n >= 0, name
+        return out[:n].copy()
+
+    def scalar(self, name):
+        return self.L.orc_get_scalar(name.encode())
+
+    def inv2d(self, spec):
+        spec = np.ascontiguousarray(spec, dtype=np.complex128)
+        phys = np.zeros((3 * self.Nz // 2, 3 * self.Nx // 2))
+        self.L.orc_inv2d(_ptr(spec), _ptr(phys))
+        return phys
+
+    def fwd2d(self, phys):
+        phys = np.ascontiguousarray(phys, dtype=np.float64)
+        spec = np.zeros((self.Nz, self.Nx // 2), complex)
+        self.L.orc_fwd2d(_ptr(phys), _ptr(spec))
+        return spec
+
+    def pass1(self, U, check_cfl=0):
+        U = np.ascontiguousarray(U, dtype=np.complex128).copy()
+        rc = self.L.orc_pass1_state(_ptr(U), check_cfl)
+        return rc, U
+
+    def pass2(self, U, AU):
+        U = np.ascontiguousarray(U, dtype=np.complex128).copy()
+        AU = np.ascontiguousarray(AU, dtype=np.complex128)
+        self.L.orc_pass2_state(_ptr(U), _ptr(AU))
+        return U
+
+    def coef_to_gauss(self, C):
+        U = np.zeros(self.ushape, complex)
+        self.L.orc_coef_to_gauss(_ptr(_check_c(np.ascontiguousarray(C))), _ptr(U))
+        return U
+
+    def state(self):
+        C = np.zeros(self.cshape, complex)
+        U = np.zeros(self.ushape, complex)
+        self.L.orc_get_state(_ptr(C), _ptr(U))
+        return C, U
+
+    def last_status(self):
+        return self.L.orc_last_status()
+
+    def destroy(self):
+        self.L.orc_destroy()
+
+
+def band_solve(A, b, dl):
+    """Solve the stride-2 banded system (A: N x (2dl+1)) with the oracle."""
+    L = load()
+    A = np.ascontiguousarray(A, dtype=np.float64).copy()
+    b = np.ascontiguousarray(b, dtype=np.complex128).copy()
+    L.orc_band_solve(_ptr(A), _ptr(b), b.shape[0], dl)
+    return b
+
+
+def glibc_rand_sequence(n, seed=1):
+    L = load()
+    L.orc_srand(seed)
+    return [L.orc_rand() for _ in range(n)]

@@ -1,0 +1,1395 @@
+/*
+ * refcpu.c -- CPU restatement of the shadow-channel DNS path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity oracle for the HIP implementation.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * this library.  Nothing in the product links it.
+ *
+ * Each routine names the reference file:line it restates.  The restatement
+ * keeps the reference's semantics, including its quirks (the right-hand side
+ * carried between Runge-Kutta stages, real-part-only mean-mode forcing,
+ * mass-only flux correction, untouched alpha rows dimQ..dimR-1, Nyquist
+ * zeroing, the stage-k storage index of MC/MIC, the LU lag of the adjoint and
+ * the forward difference of ddt_project at step 0).
+ *
+ * Pinning (tests/test_oracle_*.py): Gauss nodes/weights against numpy's
+ * leggauss; the bases against numpy Legendre polynomials; the band Gram
+ * matrices against dense quadrature; the padded transforms against numpy.fft;
+ * the nonlinear products against numpy on the physical grid; the band solver
+ * against dense solves; the linear stage map against dense Galerkin matrices
+ * built independently; the tangent against central finite differences of the
+ * primal and its linearity; the adjoint against the tangent by duality; the
+ * laminar Orr-Sommerfeld growth rate against Orszag (1971).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "refcpu.h"
+
+typedef orc_cplx cx;
+
+#define CFL_LIMIT 2.8
+#define MIN(a, b) ((a) < (b) ? (a) : (b))
+#define MAX(a, b) ((a) > (b) ? (a) : (b))
+
+/* slots of U (channel/minChnl.h:46-56) */
+enum { S_U = 0, S_V = 1, S_W = 2, S_DU = 3, S_DW = 4 };
+/* after the nonlinear pass: Hz -> slot 0, Hx -> slot 1, Hy -> slot 2 */
+enum { S_HZ = 0, S_HX = 1, S_HY = 2 };
+
+/* ------------------------------------------------------------------ */
+/* FFT: generic mixed-radix decimation in time, exp(sign*2*pi*i*j*k/n)  */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int n, nf, f[40];
+    cx *twp, *twm;      /* exp(+2 pi i j/n), exp(-2 pi i j/n) */
+    cx *buf;
+} fftplan;
+
+static void plan_make(fftplan *p, int n)
+{
+    int m = n, d = 2;
+    p->n = n;
+    p->nf = 0;
+    while (m > 1) {
+        while (m % d == 0) { p->f[p->nf++] = d; m /= d; }
+        ++d;
+    }
+    p->twp = (cx *) malloc(sizeof(cx) * n);
+    p->twm = (cx *) malloc(sizeof(cx) * n);
+    p->buf = (cx *) malloc(sizeof(cx) * n);
+    for (int j = 0; j < n; ++j) {
+        double a = 2.0 * M_PI * (double) j / (double) n;
+        p->twp[j].re = cos(a);
+        p->twp[j].im = sin(a);
+        p->twm[j].re = cos(a);
+        p->twm[j].im = -sin(a);
+    }
+}
+
+static void plan_free(fftplan *p)
+{
+    free(p->twp); free(p->twm); free(p->buf);
+    memset(p, 0, sizeof(*p));
+}
+
+static void fft_rec(const fftplan *p, int n, int fi, const cx *in, int is,
+                    cx *out, const cx *tw)
+{
+    if (n == 1) { out[0] = in[0]; return; }
+    int r = p->f[fi], m = n / r, step = p->n / n;
+    for (int q = 0; q < r; ++q)
+        fft_rec(p, m, fi + 1, in + q * is, is * r, out + q * m, tw);
+    cx t[64];
+    cx *tb = r <= 64 ? t : (cx *) malloc(sizeof(cx) * r);
+    for (int k = 0; k < m; ++k) {
+        for (int q = 0; q < r; ++q) {
+            cx a = out[q * m + k];
+            cx w = tw[((long) q * k * step) % p->n];
+            tb[q].re = a.re * w.re - a.im * w.im;
+            tb[q].im = a.re * w.im + a.im * w.re;
+        }
+        for (int s = 0; s < r; ++s) {
+            double sr = 0.0, si = 0.0;
+            for (int q = 0; q < r; ++q) {
+                cx w = tw[((long) ((q * s) % r) * m * step) % p->n];
+                sr += tb[q].re * w.re - tb[q].im * w.im;
+                si += tb[q].re * w.im + tb[q].im * w.re;
+            }
+            out[k + s * m].re = sr;
+            out[k + s * m].im = si;
+        }
+    }
+    if (tb != t) free(tb);
+}
+
+/* in-place transform of x (contiguous), sign +1 (backward) or -1 (forward) */
+static void fft_c(fftplan *p, cx *x, int sign)
+{
+    fft_rec(p, p->n, 0, x, 1, p->buf, sign > 0 ? p->twp : p->twm);
+    memcpy(x, p->buf, sizeof(cx) * p->n);
+}
+
+/* ------------------------------------------------------------------ */
+/* global solver state (the reference keeps one simulation per process) */
+/* ------------------------------------------------------------------ */
+static struct {
+    int ok;
+    int Nx, Ny, Nz, Mx, qpts, dimR, dimQ, nsteps, Px, Pz;
+    double Lx, Lz, dt, re, flux, cfl1, cfl3;
+    double *y, *w, *cfl2, *Kx, *Kz, *K2;
+    double *Q, *Qp, *Qpp, *R, *Rp;          /* qpts x dim   */
+    double *Qw, *Qpw, *Qppw, *Rw, *Rpw;     /* dim x qpts   */
+    double *Qs, *Qps, *Qpps;                /* dimQ x 9     */
+    double *Rs, *Rps;                       /* dimR x 5     */
+    double *Rp0, *Rpp0;
+    double *addm;                           /* Rs^{-1} Rw 1 */
+    cx *C, *U, *IC, *IU, *AC, *AU, *LU;
+    cx *MC, *MIC;
+    cx *Uxb, *Uzb;
+    cx *Fa, *Fb, *TM;                       /* dimR x Mx    */
+    double *band;                           /* dimR x 9 x Mx */
+    /* pass work */
+    cx *spec;                               /* 12 x Nz x Mx */
+    double *phys;                           /* 12 x Pz x Px */
+    cx *zcol, *xrow;                        /* Pz x Mx, Px  */
+    fftplan pz, px;
+    int faithful, status;
+} G;
+
+static int rand_pos = -1;
+static unsigned rand_ring[31];
+
+/* glibc random() TYPE_3 additive feedback generator, as seeded by
+   srand(seed): r0 = seed, r_i = 16807 r_{i-1} mod (2^31-1) for i < 31,
+   r_i = r_{i-31} for i = 31..33, r_i = r_{i-3} + r_{i-31} (mod 2^32)
+   afterwards; the first 310 outputs are discarded and rand() returns
+   r_i >> 1.  rand_ring holds r_{i-31} .. r_{i-1}. */
+void orc_srand(unsigned seed)
+{
+    unsigned r[34];
+    long long v = seed == 0 ? 1 : seed;
+    r[0] = (unsigned) v;
+    for (int i = 1; i < 31; ++i) {
+        v = (16807LL * v) % 2147483647LL;
+        if (v < 0) v += 2147483647LL;
+        r[i] = (unsigned) v;
+    }
+    for (int i = 31; i < 34; ++i) r[i] = r[i - 31];
+    for (int i = 0; i < 31; ++i) rand_ring[i] = r[i + 3];
+    rand_pos = 0;
+    for (int i = 0; i < 310; ++i) (void) orc_rand();
+}
+
+int orc_rand(void)
+{
+    if (rand_pos < 0) orc_srand(1);
+    unsigned v = rand_ring[0] + rand_ring[28];
+    memmove(rand_ring, rand_ring + 1, sizeof(unsigned) * 30);
+    rand_ring[30] = v;
+    return (int) (v >> 1);
+}
+
+/* index helpers */
+static inline long ci(int z, int v, int i, int x)
+{ return (((long) z * 2 + v) * G.dimR + i) * G.Mx + x; }
+static inline long ui(int z, int s, int j, int x)
+{ return (((long) z * 5 + s) * G.qpts + j) * G.Mx + x; }
+static inline long csize(void) { return (long) G.Nz * 2 * G.dimR * G.Mx; }
+static inline long usize(void) { return (long) G.Nz * 5 * G.qpts * G.Mx; }
+
+/* ------------------------------------------------------------------ */
+/* setup: channel/LegendreSetup.c:25-527, channel/RsQs.c:55-261,       */
+/*        channel/waveNumsAndCFL.c:22-96                                */
+/* ------------------------------------------------------------------ */
+static double legendre_p(int n, double x)
+{
+    double p0 = 1.0, p1 = x;
+    if (n == 0) return 1.0;
+    for (int k = 1; k < n; ++k) {
+        double p2 = ((2.0 * k + 1.0) * x * p1 - k * p0) / (k + 1.0);
+        p0 = p1;
+        p1 = p2;
+    }
+    return p1;
+}
+
+/* Gauss-Legendre nodes in ascending order, mirror-symmetric
+   (channel/LegendreSetup.c:566-643; Newton iteration instead of the
+   bracketing secant, same roots) and weights (:732-745). */
+static void gauss_nodes(int n, double *x, double *w)
+{
+    int half = n / 2;
+    for (int m = 0; m < half; ++m) {
+        double t = -cos(M_PI * (m + 0.75) / (n + 0.5));
+        for (int it = 0; it < 100; ++it) {
+            double p = legendre_p(n, t), pm = legendre_p(n - 1, t);
+            double dp = n * (pm - t * p) / (1.0 - t * t);
+            double dx = p / dp;
+            t -= dx;
+            if (fabs(dx) <= 1e-16 * fabs(t) || dx == 0.0) break;
+        }
+        x[m] = t;
+        x[n - 1 - m] = -t;
+    }
+    if (n % 2 == 1) x[half] = 0.0;
+    for (int k = 0; k < (n + 1) / 2; ++k) {
+        double v1 = n * (legendre_p(n - 1, x[k]) - x[k] * legendre_p(n, x[k]));
+        w[k] = 2.0 * (1.0 - x[k] * x[k]) / (v1 * v1);
+        w[n - 1 - k] = w[k];
+    }
+}
+
+/* banded Gram matrix D (n x (2dl+1)), D[i][dl+m] = sum_k A[i][k] B[k][i+2m],
+   computed for |m| < nd (diagonals beyond are structurally zero) and
+   mirrored (channel/LegendreSetup.c:698-720). A is dim x qpts (weighted),
+   B is qpts x dim. */
+static void gram_band(double *D, int n, int dl, int nd, const double *A,
+                      const double *B, int dim)
+{
+    int W = 2 * dl + 1;
+    memset(D, 0, sizeof(double) * n * W);
+    for (int m = 0; m < nd; ++m) {
+        for (int i = 0; i + 2 * m < n; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < G.qpts; ++k)
+                s += A[(long) i * G.qpts + k] * B[(long) k * dim + i + 2 * m];
+            D[i * W + dl + m] = s;
+            if (m > 0) D[(i + 2 * m) * W + dl - m] = s;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* stride-2 band LU without pivoting (channel/mvOps.c:29-144)           */
+/* ------------------------------------------------------------------ */
+static void band_factor(double *A, int N, int dl)
+{
+    int W = 2 * dl + 1;
+    for (int i = 2; i < N; ++i) {
+        int mmax = MIN(dl, i / 2);
+        for (int m = mmax; m >= 1; --m) {
+            double v = A[i * W + dl - m];
+            for (int k = 1; m + k <= mmax; ++k)
+                v -= A[i * W + dl - (m + k)] * A[(i - 2 * (m + k)) * W + dl + k];
+            A[i * W + dl - m] = v / A[(i - 2 * m) * W + dl];
+        }
+        int umax = MIN(dl, (N - i + 1) / 2);
+        for (int u = 0; u < umax; ++u) {
+            double v = A[i * W + dl + u];
+            int kmax = MIN(i / 2, MIN(dl, dl - u));
+            for (int k = 1; k <= kmax; ++k)
+                v -= A[i * W + dl - k] * A[(i - 2 * k) * W + dl + u + k];
+            A[i * W + dl + u] = v;
+        }
+    }
+}
+
+/* forward/back substitution with factors from band_factor; b has stride bs */
+static void band_subst(const double *A, cx *b, long bs, int N, int dl)
+{
+    int W = 2 * dl + 1;
+    for (int i = 2; i < N; ++i) {
+        int mmax = MIN(dl, i / 2);
+        for (int m = mmax; m >= 1; --m) {
+            double l = A[i * W + dl - m];
+            b[i * bs].re -= l * b[(i - 2 * m) * bs].re;
+            b[i * bs].im -= l * b[(i - 2 * m) * bs].im;
+        }
+    }
+    for (int i = N - 1; i >= 0; --i) {
+        int jmax = MIN(dl, (N - i + 1) / 2 - 1);
+        for (int j = 1; j <= jmax; ++j) {
+            double u = A[i * W + dl + j];
+            b[i * bs].re -= u * b[(i + 2 * j) * bs].re;
+            b[i * bs].im -= u * b[(i + 2 * j) * bs].im;
+        }
+        b[i * bs].re /= A[i * W + dl];
+        b[i * bs].im /= A[i * W + dl];
+    }
+}
+
+void orc_band_solve(double *A, orc_cplx *b, int N, int dl)
+{
+    band_factor(A, N, dl);
+    band_subst(A, b, 1, N, dl);
+}
+
+/* out[i] = sum_m A[i][dl+m] y[i+2m], rows i < N (channel/mvOps.c:198-237) */
+static void band_mult(const double *A, const cx *y, long ys, cx *out, long os,
+                      int N, int dl)
+{
+    int W = 2 * dl + 1;
+    for (int i = 0; i < N; ++i) {
+        double sr = 0.0, si = 0.0;
+        int m0 = -MIN(dl, i / 2);
+        int m1 = MIN(dl, (N - i + 1) / 2 - 1);
+        for (int m = m0; m <= m1; ++m) {
+            double a = A[i * W + dl + m];
+            sr += a * y[(i + 2 * m) * ys].re;
+            si += a * y[(i + 2 * m) * ys].im;
+        }
+        out[i * os].re = sr;
+        out[i * os].im = si;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* operators (channel/project.c:55-71,88-93,140-151,163-172)            */
+/* ------------------------------------------------------------------ */
+/* v operator: -(K2 Qs + Qps) + coef (K2^2 Qs + 2 K2 Qps + Qpps) */
+static void op_alpha(double *M, double K2, double coef)
+{
+    double s = K2 * K2;
+    for (int i = 0; i < G.dimQ; ++i)
+        for (int j = 0; j < 9; ++j) {
+            int e = i * 9 + j;
+            M[e] = -(K2 * G.Qs[e] + G.Qps[e]) +
+                   coef * (s * G.Qs[e] + 2. * K2 * G.Qps[e] + G.Qpps[e]);
+        }
+}
+
+/* omega_y operator: Rs + coef (Rps + K2 Rs) */
+static void op_beta(double *M, double K2, double coef)
+{
+    for (int i = 0; i < G.dimR; ++i)
+        for (int j = 0; j < 5; ++j) {
+            int e = i * 5 + j;
+            M[e] = G.Rs[e] + coef * (G.Rps[e] + K2 * G.Rs[e]);
+        }
+}
+
+/* mean-mode operator: Rs + coef Rps */
+static void op_mean(double *M, double coef)
+{
+    for (int i = 0; i < G.dimR; ++i)
+        for (int j = 0; j < 5; ++j) {
+            int e = i * 5 + j;
+            M[e] = G.Rs[e] + coef * G.Rps[e];
+        }
+}
+
+/* RK3/CN coefficients: forward (channel/project.c:44-47),
+   adjoint (channel/adjproject.c:34-37) */
+static const double FA[3] = { 1. / 3., -1. / 2, 1. / 3. };
+static const double FB[3] = { 1. / 6, 2. / 3, 0. };
+static const double FC[3] = { 1. / 2, 1. / 3., 1., };
+static const double FD[3] = { 0., -1. / 6, -2. / 3 };
+static const double BA[3] = { 1. / 3., -1. / 2, 1. / 3. };
+static const double BB[3] = { 2. / 3, 1. / 6, 0. };
+static const double BC[3] = { 1., 1. / 3., 1. / 2., };
+static const double BD[3] = { 0., -2. / 3, -1. / 6 };
+
+/* ------------------------------------------------------------------ */
+/* padded 2-D transforms (channel/pass1.c:52-114, 312-371)              */
+/* ------------------------------------------------------------------ */
+/* spectral plane S[kz][kx] (row stride ss, only rows != Nz/2) -> phys[zp][n] */
+static void inv2d(const cx *S, long ss, double *phys)
+{
+    int Nz = G.Nz, Mx = G.Mx, Pz = G.Pz, Px = G.Px;
+    cx *col = G.zcol;      /* Pz x Mx, column-major per kx: col[x*Pz + zp] */
+    for (int x = 0; x < Mx; ++x) {
+        cx *c = col + (long) x * Pz;
+        memset(c, 0, sizeof(cx) * Pz);
+        for (int z = 0; z < Nz / 2; ++z) c[z] = S[z * ss + x];
+        for (int z = Nz / 2 + 1; z < Nz; ++z) c[z + Nz / 2] = S[z * ss + x];
+        fft_c(&G.pz, c, +1);
+    }
+    cx *row = G.xrow;
+    for (int zp = 0; zp < Pz; ++zp) {
+        memset(row, 0, sizeof(cx) * Px);
+        row[0].re = col[zp].re;        /* c2r ignores Im at kx = 0 */
+        for (int k = 1; k < Mx; ++k) {
+            cx v = col[(long) k * Pz + zp];
+            row[k] = v;
+            row[Px - k].re = v.re;
+            row[Px - k].im = -v.im;
+        }
+        fft_c(&G.px, row, +1);
+        for (int n = 0; n < Px; ++n) phys[(long) zp * Px + n] = row[n].re;
+    }
+}
+
+/* phys[zp][n] -> spectral plane S[kz][kx], scaled by 1/(Px Pz), rows != Nz/2 */
+static void fwd2d(const double *phys, cx *S, long ss)
+{
+    int Nz = G.Nz, Mx = G.Mx, Pz = G.Pz, Px = G.Px;
+    cx *col = G.zcol;
+    cx *row = G.xrow;
+    double norm = 1.0 / ((3. * G.Nx / 2.) * (3. * G.Nz / 2.));
+    for (int zp = 0; zp < Pz; ++zp) {
+        for (int n = 0; n < Px; ++n) {
+            row[n].re = phys[(long) zp * Px + n];
+            row[n].im = 0.0;
+        }
+        fft_c(&G.px, row, -1);
+        for (int k = 0; k < Mx; ++k) col[(long) k * Pz + zp] = row[k];
+    }
+    for (int x = 0; x < Mx; ++x) {
+        cx *c = col + (long) x * Pz;
+        fft_c(&G.pz, c, -1);
+        for (int z = 0; z < Nz / 2; ++z) {
+            S[z * ss + x].re = norm * c[z].re;
+            S[z * ss + x].im = norm * c[z].im;
+        }
+        for (int z = Nz / 2 + 1; z < Nz; ++z) {
+            S[z * ss + x].re = norm * c[z + Nz / 2].re;
+            S[z * ss + x].im = norm * c[z + Nz / 2].im;
+        }
+    }
+}
+
+void orc_inv2d(const orc_cplx *spec, double *phys) { inv2d(spec, G.Mx, phys); }
+
+void orc_fwd2d(const double *phys, orc_cplx *spec)
+{
+    fwd2d(phys, spec, G.Mx);
+    memset(spec + (long) (G.Nz / 2) * G.Mx, 0, sizeof(cx) * G.Mx);
+}
+
+/* load u, v, w, omega_z, omega_x, omega_y of plane y into spec[0..5]
+   (channel/pass1.c:52-104: curl in spectral space) */
+static void load_curl(const cx *Ua, int y, cx *sp)
+{
+    int Nz = G.Nz, Mx = G.Mx;
+    long fs = (long) Nz * Mx;
+    for (int z = 0; z < Nz; ++z) {
+        if (z == Nz / 2) continue;
+        double kz = G.Kz[z];
+        for (int x = 0; x < Mx; ++x) {
+            double kx = G.Kx[x];
+            cx u = Ua[ui(z, S_U, y, x)], v = Ua[ui(z, S_V, y, x)];
+            cx w = Ua[ui(z, S_W, y, x)], du = Ua[ui(z, S_DU, y, x)];
+            cx dw = Ua[ui(z, S_DW, y, x)];
+            long o = (long) z * Mx + x;
+            sp[0 * fs + o] = u;
+            sp[1 * fs + o] = v;
+            sp[2 * fs + o] = w;
+            sp[3 * fs + o].re = -(du.re + kx * v.im);      /* omega_z */
+            sp[3 * fs + o].im = -du.im + kx * v.re;
+            sp[4 * fs + o].re = dw.re + kz * v.im;         /* omega_x */
+            sp[4 * fs + o].im = dw.im - kz * v.re;
+            sp[5 * fs + o].re = kx * w.im - kz * u.im;     /* omega_y */
+            sp[5 * fs + o].im = kz * u.re - kx * w.re;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* pass1: H = u x omega (+ tangent IH) (channel/pass1.c:13-376)         */
+/* ------------------------------------------------------------------ */
+static int pass1(cx *Ua, cx *IUa, int check_cfl)
+{
+    int Nz = G.Nz, Mx = G.Mx, Pz = G.Pz, Px = G.Px;
+    long fs = (long) Nz * Mx, ps = (long) Pz * Px;
+    for (int y = 0; y < G.qpts; ++y) {
+        load_curl(Ua, y, G.spec);
+        for (int f = 0; f < 6; ++f) inv2d(G.spec + f * fs, Mx, G.phys + f * ps);
+        double *u = G.phys, *v = u + ps, *w = v + ps;
+        double *oz = w + ps, *ox = oz + ps, *oy = ox + ps;
+        if (check_cfl) {
+            double cfl = 0.0;
+            for (long p = 0; p < ps; ++p) {
+                double t = fabs(u[p]) * G.cfl1 + fabs(v[p]) * G.cfl2[y] +
+                           fabs(w[p]) * G.cfl3;
+                if (t > cfl) cfl = t;
+            }
+            cfl *= G.dt;
+            if (cfl >= CFL_LIMIT) {
+                printf("Pass1:  CFL[%d]:%e, Max:%e dt:%e.  CFL condition violated.\n",
+                       y, cfl, CFL_LIMIT, G.dt);
+                return -1;
+            }
+        }
+        if (IUa) {
+            double *Iu = G.phys + 6 * ps, *Iv = Iu + ps, *Iw = Iv + ps;
+            double *Ioz = Iw + ps, *Iox = Ioz + ps, *Ioy = Iox + ps;
+            load_curl(IUa, y, G.spec + 6 * fs);
+            for (int f = 0; f < 6; ++f)
+                inv2d(G.spec + (6 + f) * fs, Mx, G.phys + (6 + f) * ps);
+            for (long p = 0; p < ps; ++p) {
+                double hx = Iv[p] * oz[p] - Iw[p] * oy[p] + v[p] * Ioz[p] - w[p] * Ioy[p];
+                double hy = Iw[p] * ox[p] - Iu[p] * oz[p] + w[p] * Iox[p] - u[p] * Ioz[p];
+                double hz = Iu[p] * oy[p] - Iv[p] * ox[p] + u[p] * Ioy[p] - v[p] * Iox[p];
+                Iu[p] = hz;
+                Iv[p] = hx;
+                Iw[p] = hy;
+            }
+        }
+        for (long p = 0; p < ps; ++p) {
+            double hx = v[p] * oz[p] - w[p] * oy[p];
+            double hy = w[p] * ox[p] - u[p] * oz[p];
+            double hz = u[p] * oy[p] - v[p] * ox[p];
+            u[p] = hz;
+            v[p] = hx;
+            w[p] = hy;
+        }
+        long zs = 5L * G.qpts * Mx;
+        for (int s = 0; s < 3; ++s)
+            fwd2d(G.phys + s * ps, Ua + ui(0, s, y, 0), zs);
+        if (IUa) {
+            for (int s = 0; s < 3; ++s)
+                fwd2d(G.phys + (6 + s) * ps, IUa + ui(0, s, y, 0), zs);
+        }
+    }
+    return 0;
+}
+
+/* scratch U used when only the cost of the tangent side is wanted */
+static cx *g_scratchU;
+
+int orc_pass1_state(orc_cplx *Ua, int check_cfl)
+{
+    return pass1(Ua, NULL, check_cfl);
+}
+
+/* ------------------------------------------------------------------ */
+/* pass2: adjoint convective operator (channel/pass2.c:23-570)          */
+/* ------------------------------------------------------------------ */
+static void pass2(cx *Ua, const cx *AUa, int incremental_cost)
+{
+    int Nz = G.Nz, Mx = G.Mx, Pz = G.Pz, Px = G.Px;
+    long fs = (long) Nz * Mx, ps = (long) Pz * Px;
+    cx *out = G.spec + 9 * fs;     /* 6 transformed fields in spec[9..14]? */
+    (void) out;
+    for (int y = 0; y < G.qpts; ++y) {
+        load_curl(Ua, y, G.spec);
+        for (int s = 0; s < 3; ++s) {
+            for (int z = 0; z < Nz; ++z) {
+                if (z == Nz / 2) continue;
+                for (int x = 0; x < Mx; ++x)
+                    G.spec[(6 + s) * fs + (long) z * Mx + x] = AUa[ui(z, s, y, x)];
+            }
+        }
+        for (int f = 0; f < 9; ++f) inv2d(G.spec + f * fs, Mx, G.phys + f * ps);
+        if (incremental_cost) {
+            /* the reference also transforms the incremental side (IU, IAU)
+               and discards the result here; repeat the work for cost only */
+            for (int f = 0; f < 9; ++f) inv2d(G.spec + f * fs, Mx, G.phys + 9 * ps);
+        }
+        double *u = G.phys, *v = u + ps, *w = v + ps;
+        double *oz = w + ps, *ox = oz + ps, *oy = ox + ps;
+        double *lx = oy + ps, *ly = lx + ps, *lz = ly + ps;
+        for (long p = 0; p < ps; ++p) {
+            double sx = v[p] * lz[p] - w[p] * ly[p];
+            double sy = w[p] * lx[p] - u[p] * lz[p];
+            double sz = u[p] * ly[p] - v[p] * lx[p];
+            double tx = ly[p] * oz[p] - lz[p] * oy[p];
+            double ty = lz[p] * ox[p] - lx[p] * oz[p];
+            double tz = lx[p] * oy[p] - ly[p] * ox[p];
+            u[p] = sx; v[p] = sy; w[p] = sz;
+            oz[p] = tx; ox[p] = ty; oy[p] = tz;
+        }
+        /* transformed S_x,S_y,S_z,T_x,T_y,T_z into spec[0..5] */
+        for (int f = 0; f < 6; ++f) fwd2d(G.phys + f * ps, G.spec + f * fs, Mx);
+        if (incremental_cost)
+            for (int f = 0; f < 6; ++f) fwd2d(G.phys + f * ps, G.spec + 9 * fs, Mx);
+        const cx *Sx = G.spec, *Sy = Sx + fs, *Sz = Sy + fs;
+        const cx *Tx = Sz + fs, *Ty = Tx + fs, *Tz = Ty + fs;
+        for (int z = 0; z < Nz; ++z) {
+            if (z == Nz / 2) continue;
+            double kz = G.Kz[z];
+            for (int x = 0; x < Mx; ++x) {
+                double kx = G.Kx[x];
+                long o = (long) z * Mx + x;
+                cx *hx = Ua + ui(z, S_HX, y, x), *hy = Ua + ui(z, S_HY, y, x);
+                cx *hz = Ua + ui(z, S_HZ, y, x);
+                Ua[ui(z, S_DU, y, x)] = Sx[o];
+                Ua[ui(z, S_DW, y, x)] = Sz[o];
+                hx->re = -Tx[o].re - kz * Sy[o].im;
+                hx->im = -Tx[o].im + kz * Sy[o].re;
+                hy->re = -Ty[o].re + kz * Sx[o].im - kx * Sz[o].im;
+                hy->im = -Ty[o].im - kz * Sx[o].re + kx * Sz[o].re;
+                hz->re = -Tz[o].re + kx * Sy[o].im;
+                hz->im = -Tz[o].im - kx * Sy[o].re;
+            }
+        }
+    }
+}
+
+void orc_pass2_state(orc_cplx *Ua, const orc_cplx *AUa) { pass2(Ua, AUa, 0); }
+
+/* ------------------------------------------------------------------ */
+/* coefficients -> Gauss points (channel/initAlphaBeta2.c:12-240)       */
+/* ------------------------------------------------------------------ */
+static void rebuild_mode(const cx *Ca, cx *Ua, int z, int x)
+{
+    int q = G.qpts, nQ = G.dimQ, nR = G.dimR;
+    double kx = G.Kx[x], kz = G.Kz[z], k2 = G.K2[z * G.Mx + x];
+    for (int i = 0; i < q; ++i) {
+        double vr = 0, vi = 0, fr = 0, fi = 0, qr = 0, qi = 0;
+        double gr = 0, gi = 0, pr = 0, pi = 0;
+        for (int j = 0; j < nQ; ++j) {
+            cx a = Ca[ci(z, 0, j, x)];
+            double cq = G.Q[i * nQ + j], cp = G.Qp[i * nQ + j], c2 = G.Qpp[i * nQ + j];
+            vr += cq * a.re; vi += cq * a.im;
+            fr -= cp * a.re; fi -= cp * a.im;
+            qr += c2 * a.re; qi += c2 * a.im;
+        }
+        for (int j = 0; j < nR; ++j) {
+            cx b = Ca[ci(z, 1, j, x)];
+            double cr = G.R[i * nR + j], cp = G.Rp[i * nR + j];
+            gr += cr * b.re; gi += cr * b.im;
+            pr += cp * b.re; pi += cp * b.im;
+        }
+        cx *o = Ua;
+        o[ui(z, S_V, i, x)].re = vr;
+        o[ui(z, S_V, i, x)].im = vi;
+        o[ui(z, S_U, i, x)].re = (kx * fi + kz * gi) / k2;
+        o[ui(z, S_U, i, x)].im = -(kx * fr + kz * gr) / k2;
+        o[ui(z, S_W, i, x)].re = (-kx * gi + kz * fi) / k2;
+        o[ui(z, S_W, i, x)].im = (kx * gr - kz * fr) / k2;
+        o[ui(z, S_DU, i, x)].re = (-kx * qi + kz * pi) / k2;
+        o[ui(z, S_DU, i, x)].im = (kx * qr - kz * pr) / k2;
+        o[ui(z, S_DW, i, x)].re = -(kx * pi + kz * qi) / k2;
+        o[ui(z, S_DW, i, x)].im = (kx * pr + kz * qr) / k2;
+    }
+}
+
+static void rebuild_mean(const cx *Ca, cx *Ua)
+{
+    int q = G.qpts, nR = G.dimR;
+    for (int i = 0; i < q; ++i) {
+        double ur = 0, uim = 0, dr = 0, di = 0, wr = 0, wi = 0, er = 0, ei = 0;
+        for (int j = 0; j < nR; ++j) {
+            cx a = Ca[ci(0, 0, j, 0)], b = Ca[ci(0, 1, j, 0)];
+            double r = G.R[i * nR + j], rp = G.Rp[i * nR + j];
+            ur += r * a.re; uim += r * a.im;
+            dr += rp * a.re; di += rp * a.im;
+            wr += r * b.re; wi += r * b.im;
+            er += rp * b.re; ei += rp * b.im;
+        }
+        Ua[ui(0, S_U, i, 0)].re = ur;  Ua[ui(0, S_U, i, 0)].im = uim;
+        Ua[ui(0, S_DU, i, 0)].re = dr; Ua[ui(0, S_DU, i, 0)].im = di;
+        Ua[ui(0, S_V, i, 0)].re = 0;   Ua[ui(0, S_V, i, 0)].im = 0;
+        Ua[ui(0, S_W, i, 0)].re = wr;  Ua[ui(0, S_W, i, 0)].im = wi;
+        Ua[ui(0, S_DW, i, 0)].re = er; Ua[ui(0, S_DW, i, 0)].im = ei;
+    }
+}
+
+/* wall derivatives of the state (channel/project.c:336-349) */
+static void wall_accumulate(const cx *Ca, int z, int x)
+{
+    double kx = G.Kx[x], kz = G.Kz[z], k2 = G.K2[z * G.Mx + x];
+    cx *ub = G.Uxb + z * G.Mx + x, *wb = G.Uzb + z * G.Mx + x;
+    for (int j = 0; j < G.dimR; ++j) {
+        cx b = Ca[ci(z, 1, j, x)];
+        ub->re += G.Rp0[j] * b.im * kz / k2;
+        ub->im += -G.Rp0[j] * b.re * kz / k2;
+        wb->re += -G.Rp0[j] * b.im * kx / k2;
+        wb->im += G.Rp0[j] * b.re * kx / k2;
+    }
+}
+
+static void wall_mean(const cx *Ca)
+{
+    memset(G.Uxb, 0, sizeof(cx) * G.Nz * G.Mx);
+    memset(G.Uzb, 0, sizeof(cx) * G.Nz * G.Mx);
+    for (int j = 0; j < G.dimR; ++j) {
+        cx a = Ca[ci(0, 0, j, 0)], b = Ca[ci(0, 1, j, 0)];
+        G.Uxb[0].re += G.Rp0[j] * a.re;
+        G.Uxb[0].im += G.Rp0[j] * a.im;
+        G.Uzb[0].re += G.Rp0[j] * b.re;
+        G.Uzb[0].im += G.Rp0[j] * b.im;
+    }
+}
+
+/* initAlphaBeta2 / incre_initAlphaBeta2 / adj_initAlphaBeta2 */
+static void coef_to_gauss(const cx *Ca, cx *Ua, int wall)
+{
+    rebuild_mean(Ca, Ua);
+    if (wall) wall_mean(Ca);
+    for (int z = 0; z < G.Nz; ++z)
+        for (int x = (z == 0 ? 1 : 0); x < G.Mx; ++x) {
+            rebuild_mode(Ca, Ua, z, x);
+            if (wall) wall_accumulate(Ca, z, x);
+        }
+}
+
+void orc_coef_to_gauss(const orc_cplx *Ca, orc_cplx *Ua) { coef_to_gauss(Ca, Ua, 0); }
+
+/* ------------------------------------------------------------------ */
+/* Galerkin projection of the nonlinear term (channel/project.c:95-115, */
+/* 176-191; channel/adjproject.c:108-134, 213-234)                      */
+/* ------------------------------------------------------------------ */
+static void forcing_modes(const cx *Ua, int z, int x0, int adj)
+{
+    int q = G.qpts, nQ = G.dimQ, nR = G.dimR, Mx = G.Mx;
+    double kz = G.Kz[z];
+    for (int x = x0; x < Mx; ++x) {
+        double kx = G.Kx[x], k2 = G.K2[z * Mx + x];
+        for (int i = 0; i < nQ; ++i) {
+            double fr = 0, fi = 0;
+            for (int j = 0; j < q; ++j) {
+                cx hx = Ua[ui(z, S_HX, j, x)], hy = Ua[ui(z, S_HY, j, x)];
+                cx hz = Ua[ui(z, S_HZ, j, x)];
+                double a = G.Qpw[i * q + j], b = G.Qw[i * q + j];
+                double tr = a * (-kx * hx.im - kz * hz.im) - k2 * (b * hy.re);
+                double ti = a * (kx * hx.re + kz * hz.re) - k2 * (b * hy.im);
+                if (adj) {
+                    cx sx = Ua[ui(z, S_DU, j, x)], sz = Ua[ui(z, S_DW, j, x)];
+                    double c = G.Qppw[i * q + j];
+                    tr += c * (kz * sx.im - kx * sz.im);
+                    ti += c * (-kz * sx.re + kx * sz.re);
+                }
+                fr += tr;
+                fi += ti;
+            }
+            G.Fa[i * Mx + x].re = fr;
+            G.Fa[i * Mx + x].im = fi;
+        }
+        for (int i = 0; i < nR; ++i) {
+            double fr = 0, fi = 0;
+            for (int j = 0; j < q; ++j) {
+                cx hx = Ua[ui(z, S_HX, j, x)], hz = Ua[ui(z, S_HZ, j, x)];
+                double a = G.Rw[i * q + j];
+                double tr = a * (-kz * hx.im + kx * hz.im);
+                double ti = a * (kz * hx.re - kx * hz.re);
+                if (adj) {
+                    cx sx = Ua[ui(z, S_DU, j, x)], sz = Ua[ui(z, S_DW, j, x)];
+                    double c = G.Rpw[i * q + j];
+                    tr += c * (-kz * sz.im - kx * sx.im);
+                    ti += c * (kz * sz.re + kx * sx.re);
+                }
+                fr += tr;
+                fi += ti;
+            }
+            G.Fb[i * Mx + x].re = fr;
+            G.Fb[i * Mx + x].im = fi;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* one Runge-Kutta stage for the modes of row z, x >= x0                */
+/* project (channel/project.c:24-405), increproject                     */
+/* (channel/increproject.c:17-500), adjproject (channel/adjproject.c)   */
+/*   Ca:   coefficient array carried between stages (RHS between stages)*/
+/*   Ua:   nonlinear term in (H slots; S_x,S_z in DU/DW for adjoint)     */
+/*   LUa:  adjoint only: nonlinear term for the next-stage explicit part*/
+/*   store: MC/MIC slot or NULL                                          */
+/* ------------------------------------------------------------------ */
+static void stage_modes(cx *Ca, const cx *Ua, const cx *LUa, cx *Uout,
+                        cx *store, int k, int z, int x0, int adj, int wall)
+{
+    int Mx = G.Mx, nQ = G.dimQ, nR = G.dimR;
+    const double *a = adj ? BA : FA, *b = adj ? BB : FB;
+    const double *c = adj ? BC : FC, *d = adj ? BD : FD;
+    double dt = G.dt, re = G.re;
+    double *M = G.band;
+    /* alpha */
+    if (k == 0) {
+        for (int x = x0; x < Mx; ++x) {
+            op_alpha(M, G.K2[z * Mx + x], re * a[0] * dt);
+            band_mult(M, Ca + ci(z, 0, 0, x), Mx, G.TM + x, Mx, nQ, 4);
+        }
+        for (int i = 0; i < nQ; ++i)
+            for (int x = x0; x < Mx; ++x) Ca[ci(z, 0, i, x)] = G.TM[i * Mx + x];
+    }
+    forcing_modes(Ua, z, x0, adj);
+    for (int i = 0; i < nQ; ++i)
+        for (int x = x0; x < Mx; ++x) {
+            Ca[ci(z, 0, i, x)].re += dt * c[k] * G.Fa[i * Mx + x].re;
+            Ca[ci(z, 0, i, x)].im += dt * c[k] * G.Fa[i * Mx + x].im;
+        }
+    for (int x = x0; x < Mx; ++x) {
+        op_alpha(M, G.K2[z * Mx + x], -(re * b[k] * dt));
+        band_factor(M, nQ, 4);
+        band_subst(M, Ca + ci(z, 0, 0, x), Mx, nQ, 4);
+    }
+    /* beta */
+    if (k == 0) {
+        for (int x = x0; x < Mx; ++x) {
+            op_beta(M, G.K2[z * Mx + x], -(re * a[0] * dt));
+            band_mult(M, Ca + ci(z, 1, 0, x), Mx, G.TM + x, Mx, nR, 2);
+        }
+        for (int i = 0; i < nR; ++i)
+            for (int x = x0; x < Mx; ++x) Ca[ci(z, 1, i, x)] = G.TM[i * Mx + x];
+    }
+    for (int i = 0; i < nR; ++i)
+        for (int x = x0; x < Mx; ++x) {
+            Ca[ci(z, 1, i, x)].re += dt * c[k] * G.Fb[i * Mx + x].re;
+            Ca[ci(z, 1, i, x)].im += dt * c[k] * G.Fb[i * Mx + x].im;
+        }
+    for (int x = x0; x < Mx; ++x) {
+        op_beta(M, G.K2[z * Mx + x], re * b[k] * dt);
+        band_factor(M, nR, 2);
+        band_subst(M, Ca + ci(z, 1, 0, x), Mx, nR, 2);
+    }
+    if (store) {
+        for (int v = 0; v < 2; ++v)
+            for (int i = 0; i < nR; ++i)
+                for (int x = x0; x < Mx; ++x) store[ci(z, v, i, x)] = Ca[ci(z, v, i, x)];
+    }
+    /* velocities at Gauss points from the solved coefficients */
+    for (int x = x0; x < Mx; ++x) rebuild_mode(Ca, Uout, z, x);
+    if (wall)
+        for (int x = x0; x < Mx; ++x) wall_accumulate(Ca, z, x);
+    /* explicit part of the next stage */
+    if (k != 2) {
+        if (adj) forcing_modes(LUa, z, x0, 1);
+        for (int x = x0; x < Mx; ++x) {
+            op_alpha(M, G.K2[z * Mx + x], re * a[k + 1] * dt);
+            band_mult(M, Ca + ci(z, 0, 0, x), Mx, G.TM + x, Mx, nQ, 4);
+        }
+        for (int i = 0; i < nQ; ++i)
+            for (int x = x0; x < Mx; ++x) {
+                Ca[ci(z, 0, i, x)].re = G.TM[i * Mx + x].re + dt * d[k + 1] * G.Fa[i * Mx + x].re;
+                Ca[ci(z, 0, i, x)].im = G.TM[i * Mx + x].im + dt * d[k + 1] * G.Fa[i * Mx + x].im;
+            }
+        for (int x = x0; x < Mx; ++x) {
+            op_beta(M, G.K2[z * Mx + x], -(re * a[k + 1] * dt));
+            band_mult(M, Ca + ci(z, 1, 0, x), Mx, G.TM + x, Mx, nR, 2);
+        }
+        for (int i = 0; i < nR; ++i)
+            for (int x = x0; x < Mx; ++x) {
+                Ca[ci(z, 1, i, x)].re = G.TM[i * Mx + x].re + dt * d[k + 1] * G.Fb[i * Mx + x].re;
+                Ca[ci(z, 1, i, x)].im = G.TM[i * Mx + x].im + dt * d[k + 1] * G.Fb[i * Mx + x].im;
+            }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* the (kx,kz) = (0,0) mode: project0 (channel/project0.c:22-271),      */
+/* increproject0 (channel/increproject0.c:14-312),                      */
+/* adjproject0 (channel/adjproject0.c:8-310)                            */
+/*   mode: 0 primal (flux target G.flux), 1 tangent (target 0 or 2),    */
+/*         2 adjoint (target 0, correction through the LHS matrix)      */
+/* ------------------------------------------------------------------ */
+static void mean_rhs(const cx *Ua, int adj, double *fa, double *fb)
+{
+    int q = G.qpts, nR = G.dimR;
+    for (int i = 0; i < nR; ++i) {
+        double sa = 0, sb = 0;
+        for (int j = 0; j < q; ++j) {
+            if (adj) {
+                sa += G.Rw[i * q + j] * Ua[ui(0, S_HX, j, 0)].re +
+                      G.Rpw[i * q + j] * Ua[ui(0, S_DW, j, 0)].re;
+                sb += G.Rw[i * q + j] * Ua[ui(0, S_HZ, j, 0)].re -
+                      G.Rpw[i * q + j] * Ua[ui(0, S_DU, j, 0)].re;
+            } else {
+                sa += G.Rw[i * q + j] * Ua[ui(0, S_HX, j, 0)].re;
+                sb += G.Rw[i * q + j] * Ua[ui(0, S_HZ, j, 0)].re;
+            }
+        }
+        fa[i] = sa;
+        fb[i] = sb;
+    }
+}
+
+static void stage_mean(cx *Ca, const cx *Ua, const cx *LUa, cx *Uout,
+                       cx *store, int k, int mode, double target, int wall)
+{
+    int nR = G.dimR, q = G.qpts;
+    int adj = (mode == 2);
+    const double *a = adj ? BA : FA, *b = adj ? BB : FB;
+    const double *c = adj ? BC : FC, *d = adj ? BD : FD;
+    double dt = G.dt, re = G.re;
+    double M[5 * 1024], add[1024];
+    double fa[1024], fb[1024];
+    cx tm[1024];
+    cx *al = Ca + ci(0, 0, 0, 0), *be = Ca + ci(0, 1, 0, 0);
+    long s = G.Mx;
+    if (k == 0) {
+        op_mean(M, -(re * a[0] * dt));
+        band_mult(M, al, s, tm, 1, nR, 2);
+        for (int i = 0; i < nR; ++i) al[i * s] = tm[i];
+        band_mult(M, be, s, tm, 1, nR, 2);
+        for (int i = 0; i < nR; ++i) be[i * s] = tm[i];
+    }
+    /* add = (Rs or Rs + re b dt Rps)^{-1} (Rw 1) */
+    for (int i = 0; i < nR; ++i) {
+        double t = 0;
+        for (int j = 0; j < q; ++j) t += G.Rw[i * q + j];
+        add[i] = t;
+    }
+    if (adj) op_mean(M, re * b[k] * dt);
+    else memcpy(M, G.Rs, sizeof(double) * nR * 5);
+    {
+        cx tmp[1024];
+        for (int i = 0; i < nR; ++i) { tmp[i].re = add[i]; tmp[i].im = 0; }
+        band_factor(M, nR, 2);
+        band_subst(M, tmp, 1, nR, 2);
+        for (int i = 0; i < nR; ++i) add[i] = tmp[i].re;
+    }
+    mean_rhs(Ua, adj, fa, fb);
+    op_mean(M, re * b[k] * dt);
+    band_factor(M, nR, 2);
+    for (int i = 0; i < nR; ++i) al[i * s].re += dt * c[k] * fa[i];
+    band_subst(M, al, s, nR, 2);
+    for (int i = 0; i < nR; ++i) be[i * s].re += dt * c[k] * fb[i];
+    band_subst(M, be, s, nR, 2);
+    double flux_t = 0;
+    for (int i = 0; i < nR; ++i)
+        for (int j = 0; j < q; ++j) flux_t += G.Rw[i * q + j] * al[i * s].re;
+    flux_t = target - flux_t;
+    for (int j = 0; j < nR; ++j) {
+        al[j * s].re = al[j * s].re + flux_t * add[j] * 0.5;
+        al[j * s].im = al[j * s].im + flux_t * 0.0 * 0.5;
+    }
+    rebuild_mean(Ca, Uout);
+    if (wall) wall_mean(Ca);
+    if (store)
+        for (int v = 0; v < 2; ++v)
+            for (int i = 0; i < nR; ++i) store[ci(0, v, i, 0)] = Ca[ci(0, v, i, 0)];
+    if (k != 2) {
+        if (adj) mean_rhs(LUa, 1, fa, fb);
+        op_mean(M, -(re * a[k + 1] * dt));
+        band_mult(M, al, s, tm, 1, nR, 2);
+        for (int i = 0; i < nR; ++i) {
+            al[i * s].re = tm[i].re + dt * d[k + 1] * fa[i];
+            al[i * s].im = tm[i].im + dt * d[k + 1] * 0.0;
+        }
+        band_mult(M, be, s, tm, 1, nR, 2);
+        for (int i = 0; i < nR; ++i) {
+            be[i * s].re = tm[i].re + dt * d[k + 1] * fb[i];
+            be[i * s].im = tm[i].im + dt * d[k + 1] * 0.0;
+        }
+    }
+}
+
+/* one full stage over all modes of an array (the loops of
+   channel/channel.c:456-470, 546-559, 668-679) */
+static void stage_all(cx *Ca, cx *Ua, const cx *LUa, cx *Uout, cx *storebase,
+                      int k, int mode, double target)
+{
+    int Nz = G.Nz;
+    int adj = (mode == 2), wall = (mode == 0);
+    stage_mean(Ca, Ua, LUa, Uout, storebase, k, mode, target, wall);
+    stage_modes(Ca, Ua, LUa, Uout, storebase, k, 0, 1, adj, wall);
+    for (int z = 1; z < Nz; ++z) {
+        if (z == Nz / 2) {
+            for (int s = 0; s < 5; ++s)
+                memset(Uout + ui(z, s, 0, 0), 0, sizeof(cx) * G.qpts * G.Mx);
+            memset(Ca + ci(z, 0, 0, 0), 0, sizeof(cx) * 2 * G.dimR * G.Mx);
+            continue;
+        }
+        stage_modes(Ca, Ua, LUa, Uout, storebase, k, z, 0, adj, wall);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* initial projection from Gauss points (channel/initAlphaBeta.c:24-266)*/
+/* ------------------------------------------------------------------ */
+static void gauss_to_coef(void)
+{
+    int q = G.qpts, nQ = G.dimQ, nR = G.dimR, Mx = G.Mx;
+    double M[9 * 1024];
+    /* mean mode: alpha from u, beta from w, R-basis mass solves */
+    for (int v = 0; v < 2; ++v) {
+        int slot = v == 0 ? S_U : S_W;
+        for (int i = 0; i < nR; ++i) {
+            double sr = 0, si = 0;
+            for (int j = 0; j < q; ++j) {
+                sr += G.Rw[i * q + j] * G.U[ui(0, slot, j, 0)].re;
+                si += G.Rw[i * q + j] * G.U[ui(0, slot, j, 0)].im;
+            }
+            G.C[ci(0, v, i, 0)].re = sr;
+            G.C[ci(0, v, i, 0)].im = si;
+        }
+        memcpy(M, G.Rs, sizeof(double) * nR * 5);
+        band_factor(M, nR, 2);
+        band_subst(M, G.C + ci(0, v, 0, 0), Mx, nR, 2);
+    }
+    for (int z = 0; z < G.Nz; ++z) {
+        int x0 = z == 0 ? 1 : 0;
+        double kz = G.Kz[z];
+        /* alpha from v with the Q mass matrix (rows dimQ.. untouched) */
+        for (int i = 0; i < nQ; ++i)
+            for (int x = x0; x < Mx; ++x) {
+                double sr = 0, si = 0;
+                for (int j = 0; j < q; ++j) {
+                    sr += G.Qw[i * q + j] * G.U[ui(z, S_V, j, x)].re;
+                    si += G.Qw[i * q + j] * G.U[ui(z, S_V, j, x)].im;
+                }
+                G.C[ci(z, 0, i, x)].re = sr;
+                G.C[ci(z, 0, i, x)].im = si;
+            }
+        memcpy(M, G.Qs, sizeof(double) * nQ * 9);
+        band_factor(M, nQ, 4);
+        for (int x = x0; x < Mx; ++x) band_subst(M, G.C + ci(z, 0, 0, x), Mx, nQ, 4);
+        /* beta from omega_y = i kz u - i kx w */
+        for (int i = 0; i < nR; ++i)
+            for (int x = x0; x < Mx; ++x) {
+                double kx = G.Kx[x], sr = 0, si = 0;
+                for (int j = 0; j < q; ++j) {
+                    cx u = G.U[ui(z, S_U, j, x)], w = G.U[ui(z, S_W, j, x)];
+                    sr += G.Rw[i * q + j] * (kx * w.im - kz * u.im);
+                    si += G.Rw[i * q + j] * (kz * u.re - kx * w.re);
+                }
+                G.C[ci(z, 1, i, x)].re = sr;
+                G.C[ci(z, 1, i, x)].im = si;
+            }
+        memcpy(M, G.Rs, sizeof(double) * nR * 5);
+        band_factor(M, nR, 2);
+        for (int x = x0; x < Mx; ++x) band_subst(M, G.C + ci(z, 1, 0, x), Mx, nR, 2);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* public API: channel/channel.c                                        */
+/* ------------------------------------------------------------------ */
+static void freeall(void)
+{
+    void **ptrs[] = {
+        (void **) &G.y, (void **) &G.w, (void **) &G.cfl2, (void **) &G.Kx,
+        (void **) &G.Kz, (void **) &G.K2, (void **) &G.Q, (void **) &G.Qp,
+        (void **) &G.Qpp, (void **) &G.R, (void **) &G.Rp, (void **) &G.Qw,
+        (void **) &G.Qpw, (void **) &G.Qppw, (void **) &G.Rw, (void **) &G.Rpw,
+        (void **) &G.Qs, (void **) &G.Qps, (void **) &G.Qpps, (void **) &G.Rs,
+        (void **) &G.Rps, (void **) &G.Rp0, (void **) &G.Rpp0, (void **) &G.addm,
+        (void **) &G.C, (void **) &G.U, (void **) &G.IC, (void **) &G.IU,
+        (void **) &G.AC, (void **) &G.AU, (void **) &G.LU, (void **) &G.MC,
+        (void **) &G.MIC, (void **) &G.Uxb, (void **) &G.Uzb, (void **) &G.Fa,
+        (void **) &G.Fb, (void **) &G.TM, (void **) &G.band, (void **) &G.spec,
+        (void **) &G.phys, (void **) &G.zcol, (void **) &G.xrow,
+        (void **) &g_scratchU };
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) {
+        free(*ptrs[i]);
+        *ptrs[i] = NULL;
+    }
+    if (G.pz.n) plan_free(&G.pz);
+    if (G.px.n) plan_free(&G.px);
+    G.ok = 0;
+}
+
+void orc_destroy(void) { freeall(); }
+
+int orc_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re,
+             double flux, double dt, int nsteps)
+{
+    int faithful = G.faithful;
+    if (G.ok) freeall();
+    memset(&G, 0, sizeof(G));
+    G.faithful = faithful;
+    /* channel/channel.c:274-312 */
+    if (Nx % 4 != 0 || Nz % 2 != 0 || Ny - 4 < 0 || nsteps < 0) return 1;
+    G.Nx = Nx; G.Ny = Ny; G.Nz = Nz; G.Lx = Lx; G.Lz = Lz;
+    G.dt = dt; G.nsteps = nsteps; G.flux = flux;
+    G.re = 1. / Re;
+    G.qpts = 3 * Ny / 2;
+    G.dimR = Ny - 2;
+    G.dimQ = Ny - 4;
+    G.Mx = Nx / 2;
+    G.Px = 3 * Nx / 2;
+    G.Pz = 3 * Nz / 2;
+    if (G.dimR > 1000) return 1;
+    int q = G.qpts, nQ = G.dimQ, nR = G.dimR, Mx = G.Mx;
+#define ALLOC(p, n) do { (p) = calloc((size_t) (n), sizeof(*(p))); if (!(p)) { freeall(); return 1; } } while (0)
+    ALLOC(G.y, q); ALLOC(G.w, q); ALLOC(G.cfl2, q);
+    ALLOC(G.Kx, Mx); ALLOC(G.Kz, Nz); ALLOC(G.K2, Nz * Mx);
+    ALLOC(G.Q, q * nQ); ALLOC(G.Qp, q * nQ); ALLOC(G.Qpp, q * nQ);
+    ALLOC(G.R, q * nR); ALLOC(G.Rp, q * nR);
+    ALLOC(G.Qw, nQ * q); ALLOC(G.Qpw, nQ * q); ALLOC(G.Qppw, nQ * q);
+    ALLOC(G.Rw, nR * q); ALLOC(G.Rpw, nR * q);
+    ALLOC(G.Qs, nQ * 9); ALLOC(G.Qps, nQ * 9); ALLOC(G.Qpps, nQ * 9);
+    ALLOC(G.Rs, nR * 5); ALLOC(G.Rps, nR * 5);
+    ALLOC(G.Rp0, nR); ALLOC(G.Rpp0, nR); ALLOC(G.addm, nR);
+    ALLOC(G.C, csize()); ALLOC(G.IC, csize()); ALLOC(G.AC, csize());
+    ALLOC(G.U, usize()); ALLOC(G.IU, usize()); ALLOC(G.AU, usize());
+    ALLOC(G.LU, usize()); ALLOC(g_scratchU, usize());
+    ALLOC(G.MC, csize() * (3L * nsteps + 1));
+    ALLOC(G.MIC, csize() * (3L * nsteps + 1));
+    ALLOC(G.Uxb, Nz * Mx); ALLOC(G.Uzb, Nz * Mx);
+    ALLOC(G.Fa, nR * Mx); ALLOC(G.Fb, nR * Mx); ALLOC(G.TM, nR * Mx);
+    ALLOC(G.band, nR * 9 * Mx);
+    ALLOC(G.spec, 16L * Nz * Mx);
+    ALLOC(G.phys, 16L * G.Pz * G.Px);
+    ALLOC(G.zcol, (long) G.Pz * Mx);
+    ALLOC(G.xrow, G.Px);
+#undef ALLOC
+    plan_make(&G.pz, G.Pz);
+    plan_make(&G.px, G.Px);
+
+    /* quadrature, Legendre matrix, bases (channel/LegendreSetup.c:353-400) */
+    gauss_nodes(q, G.y, G.w);
+    double *P = (double *) malloc(sizeof(double) * q * nR);
+    for (int m = 0; m < q; ++m) {
+        P[m * nR + 0] = 1.0;
+        if (nR > 1) P[m * nR + 1] = G.y[m];
+        for (int n = 1; n < nR - 1; ++n)
+            P[m * nR + n + 1] = ((2. * n + 1.) * (G.y[m] * P[m * nR + n]) -
+                                 n * P[m * nR + n - 1]) / (double) (n + 1);
+    }
+    for (int i = 0; i < q; ++i) {
+        double yy = G.y[i], om = 1.0 - yy * yy, y2 = yy * yy;
+        for (int j = 0; j < nR; ++j) {
+            G.R[i * nR + j] = om * P[i * nR + j];
+            G.Rp[i * nR + j] = (j == 0) ? -2.0 * yy
+                : j * P[i * nR + j - 1] - (j + 2.0) * (yy * P[i * nR + j]);
+        }
+        for (int j = 0; j < nQ; ++j) {
+            G.Q[i * nQ + j] = om * G.R[i * nR + j];
+            G.Qp[i * nQ + j] = (j == 0) ? -4.0 * yy * om
+                : om * (j * P[i * nR + j - 1] - (j + 4.0) * (yy * P[i * nR + j]));
+            G.Qpp[i * nQ + j] = (j == 0) ? 12.0 * y2 - 4.0
+                : (12.0 + 7.0 * j + j * j) * (y2 * P[i * nR + j]) -
+                  (4.0 + j * j + j) * P[i * nR + j] - 6.0 * j * (yy * P[i * nR + j - 1]);
+        }
+    }
+    free(P);
+    for (int j = 0; j < nR; ++j) {
+        double sg = (j % 2 == 0) ? 1.0 : -1.0;     /* (-1)^j */
+        G.Rp0[j] = 2.0 * sg;                       /* R_j'(-1)  */
+        G.Rpp0[j] = (j == 0) ? -2.0 : -2.0 * sg * (1.0 + j + (double) j * j);
+    }
+    /* weighted transposes Mw[i][j] = w[j] M[j][i] (:756-765) */
+    for (int i = 0; i < nQ; ++i)
+        for (int j = 0; j < q; ++j) {
+            G.Qw[i * q + j] = G.w[j] * G.Q[j * nQ + i];
+            G.Qpw[i * q + j] = G.w[j] * G.Qp[j * nQ + i];
+            G.Qppw[i * q + j] = G.w[j] * G.Qpp[j * nQ + i];
+        }
+    for (int i = 0; i < nR; ++i)
+        for (int j = 0; j < q; ++j) {
+            G.Rw[i * q + j] = G.w[j] * G.R[j * nR + i];
+            G.Rpw[i * q + j] = G.w[j] * G.Rp[j * nR + i];
+        }
+    /* band Gram matrices: nonzero diagonals only (:387-396) */
+    gram_band(G.Qs, nQ, 4, 5, G.Qw, G.Q, nQ);
+    gram_band(G.Qps, nQ, 4, 4, G.Qpw, G.Qp, nQ);
+    gram_band(G.Qpps, nQ, 4, 3, G.Qppw, G.Qpp, nQ);
+    gram_band(G.Rs, nR, 2, 3, G.Rw, G.R, nR);
+    gram_band(G.Rps, nR, 2, 2, G.Rpw, G.Rp, nR);
+    /* wavenumbers (channel/waveNumsAndCFL.c:45-59) */
+    for (int i = 0; i < Mx; ++i) G.Kx[i] = i * (2. * M_PI / Lx);
+    for (int i = 0; i < Nz / 2 + 1; ++i) G.Kz[i] = i * (2. * M_PI / Lz);
+    for (int i = Nz / 2 + 1; i < Nz; ++i) G.Kz[i] = (i - Nz) * (2. * M_PI / Lz);
+    for (int i = 0; i < Nz; ++i)
+        for (int j = 0; j < Mx; ++j) G.K2[i * Mx + j] = G.Kz[i] * G.Kz[i] + G.Kx[j] * G.Kx[j];
+    /* CFL constants (:65-96) */
+    G.cfl1 = M_PI / (Lx / Nx);
+    G.cfl3 = M_PI / (Lz / Nz);
+    for (int j = 1; j < q - 1; ++j) G.cfl2[j] = 2. * M_PI / (G.y[j + 1] - G.y[j - 1]);
+    G.cfl2[0] = G.cfl2[q - 1] = M_PI / (G.y[0] + 1.0);
+    G.ok = 1;
+    return 0;
+}
+
+/* laminar Poiseuille + rand() perturbation start (channel/channel.c:409-422) */
+static void laminar_start(void)
+{
+    memset(G.U, 0, sizeof(cx) * usize());
+    for (int i = 0; i < G.qpts; ++i) {
+        G.U[ui(0, S_U, i, 0)].re = (1.0 - G.y[i] * G.y[i]) * G.flux * 3. / 4;
+        G.U[ui(0, S_U, i, 0)].im = 0.0;
+    }
+    if (G.Nz > 1 && G.Mx > 1)
+        for (int i = 0; i < G.qpts; ++i) {
+            G.U[ui(1, S_U, i, 1)].re = (orc_rand() / (double) 2147483647 - 0.5);
+            G.U[ui(1, S_U, i, 1)].im = (orc_rand() / (double) 2147483647 - 0.5);
+        }
+    gauss_to_coef();
+}
+
+int orc_primal(int ru_steps, orc_cplx *Cg)
+{
+    if (!G.ok || ru_steps < 0) return -2;
+    long cs = csize();
+    if (Cg) {
+        memcpy(G.C, Cg, sizeof(cx) * cs);
+        coef_to_gauss(G.C, G.U, 1);
+    } else {
+        laminar_start();
+    }
+    memset(G.MC, 0, sizeof(cx) * cs * (3L * G.nsteps + 1));
+    memcpy(G.MC, G.C, sizeof(cx) * cs);
+    G.status = 0;
+    for (int n = 0; n < G.nsteps + ru_steps; ++n) {
+        for (int k = 0; k < 3; ++k) {
+            if (pass1(G.U, G.faithful ? g_scratchU : NULL, k == 0) != 0) {
+                printf("Pass1 failure\n");
+                G.status = -1;
+                goto done;
+            }
+            int count = (n - ru_steps) * 3 + k + 1;
+            stage_all(G.C, G.U, NULL, G.U, count >= 0 ? G.MC + cs * count : NULL,
+                      k, 0, G.flux);
+        }
+    }
+done:
+    if (Cg) memcpy(Cg, G.C, sizeof(cx) * cs);
+    return 0;
+}
+
+int orc_tangent(int start, int end, orc_cplx *ICg, int inhomo)
+{
+    if (!G.ok) return -2;
+    if (!(end > start) || start < 0 || end > G.nsteps || !ICg) return -3;
+    long cs = csize();
+    memcpy(G.C, G.MC + cs * (3L * start), sizeof(cx) * cs);
+    memcpy(G.IC, ICg, sizeof(cx) * cs);
+    coef_to_gauss(G.C, G.U, 1);
+    coef_to_gauss(G.IC, G.IU, 0);
+    memcpy(G.MIC + cs * (3L * start), G.IC, sizeof(cx) * cs);
+    G.status = 0;
+    for (int n = start; n < end; ++n) {
+        for (int k = 0; k < 3; ++k) {
+            if (pass1(G.U, G.IU, k == 0) != 0) {
+                printf("Pass1 failure\n");
+                G.status = -1;
+                goto done;
+            }
+            int count = n * 3 + k + 1;
+            stage_all(G.IC, G.IU, NULL, G.IU, G.MIC + cs * count, k, 1,
+                      inhomo ? 2.0 : 0.0);
+            stage_all(G.C, G.U, NULL, G.U, NULL, k, 0, G.flux);
+        }
+    }
+done:
+    memcpy(ICg, G.IC, sizeof(cx) * cs);
+    return 0;
+}
+
+static void spec2phys_impl(const cx *Cin, double *flow)
+{
+    int Px = G.Px, Pz = G.Pz, q = G.qpts;
+    long fs = (long) G.Nz * G.Mx, ps = (long) Pz * Px;
+    memcpy(G.C, Cin, sizeof(cx) * csize());
+    coef_to_gauss(G.C, G.U, 1);
+    for (int y = 0; y < q; ++y) {
+        for (int s = 0; s < 3; ++s) {
+            for (int z = 0; z < G.Nz; ++z)
+                for (int x = 0; x < G.Mx; ++x)
+                    G.spec[s * fs + (long) z * G.Mx + x] = G.U[ui(z, s, y, x)];
+            inv2d(G.spec + s * fs, G.Mx, G.phys + s * ps);
+        }
+        for (int s = 0; s < 3; ++s)
+            for (int zp = 0; zp < Pz; ++zp)
+                for (int x = 0; x < Px; ++x)
+                    flow[(long) s * Px * q * Pz + ((long) x * q + y) * Pz + zp] =
+                        G.phys[s * ps + (long) zp * Px + x];
+    }
+}
+
+void orc_spec2phys(const orc_cplx *Cin, double *flow) { spec2phys_impl(Cin, flow); }
+
+double orc_ddt_project(int i_step, orc_cplx *v)
+{
+    if (!G.ok || i_step < 0 || i_step > G.nsteps || G.nsteps < 1) return NAN;
+    long cs = csize();
+    long fl = 3L * G.Px * G.qpts * G.Pz;
+    cx *dC = (cx *) calloc(cs, sizeof(cx));
+    double *du = (double *) calloc(fl, sizeof(double));
+    double *vp = (double *) calloc(fl, sizeof(double));
+    int ip, im;
+    if (i_step == 0) { ip = 3 * (i_step + 1); im = 3 * i_step; }
+    else { ip = 3 * i_step; im = 3 * (i_step - 1); }
+    const cx *Mp = G.MC + cs * ip, *Mm = G.MC + cs * im;
+    for (long e = 0; e < cs; ++e) {
+        dC[e].re = (1.0 / G.dt) * (Mp[e].re - Mm[e].re);
+        dC[e].im = (1.0 / G.dt) * (Mp[e].im - Mm[e].im);
+    }
+    spec2phys_impl(dC, du);
+    spec2phys_impl(v, vp);
+    double num = 0.0, den = 0.0;
+    long fX = 0, fY = (long) G.Px * G.qpts * G.Pz, fZ = 2 * fY;
+    for (int y = 0; y < G.qpts; ++y)
+        for (int z = 0; z < G.Pz; ++z)
+            for (int x = 0; x < G.Px; ++x) {
+                long i = ((long) x * G.qpts + y) * G.Pz + z;
+                num += du[i + fX] * vp[i + fX] * G.w[y];
+                den += du[i + fX] * du[i + fX] * G.w[y];
+                num += du[i + fY] * vp[i + fY] * G.w[y];
+                den += du[i + fY] * du[i + fY] * G.w[y];
+                num += du[i + fZ] * vp[i + fZ] * G.w[y];
+                den += du[i + fZ] * du[i + fZ] * G.w[y];
+            }
+    for (long e = 0; e < cs; ++e) {
+        v[e].re -= (num / den) * dC[e].re;
+        v[e].im -= (num / den) * dC[e].im;
+    }
+    free(dC); free(du); free(vp);
+    return num / den;
+}
+
+int orc_adjoint(int start, int end, orc_cplx *ACg, int inhomo, double strength)
+{
+    (void) inhomo;
+    if (!G.ok) return -2;
+    if (!(end < start) || end < 0 || start > G.nsteps || !ACg) return -3;
+    long cs = csize(), us = usize();
+    cx *tmp = (cx *) malloc(sizeof(cx) * cs);
+    memcpy(G.C, G.MC + cs * (3L * start), sizeof(cx) * cs);
+    memcpy(G.AC, ACg, sizeof(cx) * cs);
+    coef_to_gauss(G.AC, G.AU, 0);
+    strength *= G.dt;
+    for (int n = start; n > end; --n) {
+        memcpy(tmp, G.MIC + cs * (3L * n), sizeof(cx) * cs);
+        orc_ddt_project(n, tmp);
+        for (long e = 0; e < cs; ++e) {
+            G.AC[e].re -= 0.5 * strength * tmp[e].re;
+            G.AC[e].im -= 0.5 * strength * tmp[e].im;
+        }
+        for (int k = 0; k < 3; ++k) {
+            int count = n * 3 - k - 1;
+            if (k < 2) {
+                memcpy(G.C, G.MC + cs * (count - 1), sizeof(cx) * cs);
+                coef_to_gauss(G.C, G.U, 1);
+                pass2(G.U, G.AU, G.faithful);
+                memcpy(G.LU, G.U, sizeof(cx) * us);
+            }
+            memcpy(G.C, G.MC + cs * count, sizeof(cx) * cs);
+            coef_to_gauss(G.C, G.U, 1);
+            pass2(G.U, G.AU, G.faithful);
+            stage_all(G.AC, G.U, G.LU, G.AU, NULL, k, 2, 0.0);
+        }
+        memcpy(tmp, G.MIC + cs * (3L * (n - 1)), sizeof(cx) * cs);
+        orc_ddt_project(n - 1, tmp);
+        for (long e = 0; e < cs; ++e) {
+            G.AC[e].re -= 0.5 * strength * tmp[e].re;
+            G.AC[e].im -= 0.5 * strength * tmp[e].im;
+        }
+    }
+    memcpy(ACg, G.AC, sizeof(cx) * cs);
+    free(tmp);
+    return 0;
+}
+
+const orc_cplx *orc_getsoln(int i)
+{
+    if (!G.ok || i < 0 || i > G.nsteps) return NULL;
+    return G.MC + csize() * 3L * i;
+}
+
+const orc_cplx *orc_getincresoln(int i)
+{
+    if (!G.ok || i < 0 || i > G.nsteps) return NULL;
+    return G.MIC + csize() * 3L * i;
+}
+
+void orc_quad(double *y, double *w)
+{
+    memcpy(y, G.y, sizeof(double) * G.qpts);
+    memcpy(w, G.w, sizeof(double) * G.qpts);
+}
+
+int orc_qpts(void) { return G.qpts; }
+int orc_dimR(void) { return G.dimR; }
+int orc_dimQ(void) { return G.dimQ; }
+int orc_Nx(void) { return G.Nx; }
+int orc_Nz(void) { return G.Nz; }
+int orc_nsteps(void) { return G.nsteps; }
+int orc_last_status(void) { return G.status; }
+void orc_set_faithful_cost(int on) { G.faithful = on; }
+
+void orc_get_state(orc_cplx *Cout, orc_cplx *Uout)
+{
+    if (Cout) memcpy(Cout, G.C, sizeof(cx) * csize());
+    if (Uout) memcpy(Uout, G.U, sizeof(cx) * usize());
+}
+
+int orc_get_array(const char *name, double *out, int cap)
+{
+    struct { const char *n; double *p; long len; } t[] = {
+        { "Q", G.Q, (long) G.qpts * G.dimQ }, { "Qp", G.Qp, (long) G.qpts * G.dimQ },
+        { "Qpp", G.Qpp, (long) G.qpts * G.dimQ }, { "R", G.R, (long) G.qpts * G.dimR },
+        { "Rp", G.Rp, (long) G.qpts * G.dimR }, { "Qw", G.Qw, (long) G.qpts * G.dimQ },
+        { "Qpw", G.Qpw, (long) G.qpts * G.dimQ }, { "Qppw", G.Qppw, (long) G.qpts * G.dimQ },
+        { "Rw", G.Rw, (long) G.qpts * G.dimR }, { "Rpw", G.Rpw, (long) G.qpts * G.dimR },
+        { "Qs", G.Qs, 9L * G.dimQ }, { "Qps", G.Qps, 9L * G.dimQ },
+        { "Qpps", G.Qpps, 9L * G.dimQ }, { "Rs", G.Rs, 5L * G.dimR },
+        { "Rps", G.Rps, 5L * G.dimR }, { "Rp0", G.Rp0, G.dimR },
+        { "Rpp0", G.Rpp0, G.dimR }, { "Kx", G.Kx, G.Mx }, { "Kz", G.Kz, G.Nz },
+        { "cfl2", G.cfl2, G.qpts }, { "y", G.y, G.qpts }, { "w", G.w, G.qpts },
+    };
+    if (!G.ok) return -1;
+    for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); ++i)
+        if (strcmp(name, t[i].n) == 0) {
+            if (t[i].len > cap) return -1;
+            memcpy(out, t[i].p, sizeof(double) * t[i].len);
+            return (int) t[i].len;
+        }
+    return -1;
+}
+
+double orc_get_scalar(const char *name)
+{
+    if (!strcmp(name, "cfl1")) return G.cfl1;
+    if (!strcmp(name, "cfl3")) return G.cfl3;
+    if (!strcmp(name, "re")) return G.re;
+    if (!strcmp(name, "flux")) return G.flux;
+    if (!strcmp(name, "dt")) return G.dt;
+    return NAN;
+}

@@ -1,0 +1,88 @@
+/*
+ * refcpu -- CPU restatement of the shadow-channel DNS path (TEST INFRASTRUCTURE).
+ *
+ * This library is the parity oracle for the HIP implementation.  Only the
+ * test suite, the smoke check and the CPU-baseline leg of the benchmark load
+ * it; the product never links it.
+ *
+ * It restates, in FP64 and single-threaded, the reference's Fourier-Fourier-
+ * Legendre-Galerkin channel solver: setup (channel/LegendreSetup.c,
+ * channel/RsQs.c, channel/waveNumsAndCFL.c), the nonlinear terms
+ * (channel/pass1.c, channel/pass2.c), the Galerkin/Helmholtz solves
+ * (channel/project*.c, channel/increproject*.c, channel/adjproject*.c,
+ * channel/mvOps.c), the coefficient<->Gauss-point transforms
+ * (channel/initAlphaBeta*.c, channel/spec2phys.c) and the drivers
+ * (channel/channel.c: init, primal, tangent, adjoint, ddt_project).
+ *
+ * Layouts (complex = {re, im} doubles, x fastest):
+ *   C  [Nz][2][dimR][Nx/2]      alpha (v, Q basis) and beta (omega_y, R basis)
+ *   U  [Nz][5][qpts][Nx/2]      u, v, w, du/dy, dw/dy at Gauss points
+ *   flow [3][3Nx/2][qpts][3Nz/2] physical velocity (spec2phys)
+ */
+#ifndef REFCPU_H
+#define REFCPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } orc_cplx;
+
+int    orc_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re,
+                double flux, double dt, int nsteps);
+void   orc_destroy(void);
+/* C_given == NULL: laminar profile + rand() perturbation start. */
+int    orc_primal(int ru_steps, orc_cplx *C_given);
+int    orc_tangent(int start_step, int end_step, orc_cplx *IC, int inhomo);
+int    orc_adjoint(int start_step, int end_step, orc_cplx *AC, int inhomo,
+                   double strength);
+double orc_ddt_project(int i_step, orc_cplx *IC);
+void   orc_spec2phys(const orc_cplx *C, double *flow);
+const orc_cplx *orc_getsoln(int i_step);
+const orc_cplx *orc_getincresoln(int i_step);
+void   orc_quad(double *y, double *w);
+int    orc_qpts(void);
+int    orc_dimR(void);
+int    orc_dimQ(void);
+int    orc_Nx(void);
+int    orc_Nz(void);
+int    orc_nsteps(void);
+/* 0 after a clean run, -1 after a CFL violation truncated the last run. */
+int    orc_last_status(void);
+/* 1: also compute the terms the reference computes and discards (tangent
+   side of pass1 in the primal, incremental side of pass2); 0: skip them.
+   Results are identical either way; only the cost changes. */
+void   orc_set_faithful_cost(int on);
+/* Reset the process-global rand() stream (glibc TYPE_3 generator). */
+void   orc_srand(unsigned seed);
+int    orc_rand(void);
+
+/* ---- internals exposed for pinning tests ---- */
+/* name: "Q","Qp","Qpp","R","Rp","Qw","Qpw","Qppw","Rw","Rpw" (row-major),
+   "Qs","Qps","Qpps" (dimQ x 9), "Rs","Rps" (dimR x 5), "Rp0","Rpp0",
+   "Kx","Kz","cfl2","y","w".  Returns the element count, or -1. */
+int    orc_get_array(const char *name, double *out, int cap);
+double orc_get_scalar(const char *name);
+/* One padded 2-D inverse transform of a spectral plane [Nz][Nx/2] into a
+   physical plane [3Nz/2][3Nx/2] (z rows, x fastest), as pass1 does. */
+void   orc_inv2d(const orc_cplx *spec, double *phys);
+/* Forward transform of a physical plane, scaled and truncated to [Nz][Nx/2]
+   (row Nz/2 set to zero). */
+void   orc_fwd2d(const double *phys, orc_cplx *spec);
+/* Solve a stride-2 banded system in place (no pivoting): A is N x (2dl+1),
+   entry A[i][dl+m] is the matrix element (i, i+2m).  A is overwritten. */
+void   orc_band_solve(double *A, orc_cplx *b, int N, int dl);
+/* Run the state nonlinear term on a user U: U[z][0..2] <- H at all planes. */
+int    orc_pass1_state(orc_cplx *U, int check_cfl);
+/* Adjoint nonlinear term: U (state velocities) and AU (adjoint velocities)
+   in; U[z][HZ,HX,HY] <- adjoint H, U[z][DX] <- S_x, U[z][DZ] <- S_z. */
+void   orc_pass2_state(orc_cplx *U, const orc_cplx *AU);
+/* C -> U at Gauss points (initAlphaBeta2). */
+void   orc_coef_to_gauss(const orc_cplx *C, orc_cplx *U);
+/* Copy the current working state C / U of the solver. */
+void   orc_get_state(orc_cplx *C, orc_cplx *U);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
