@@ -1,0 +1,135 @@
+"""Benchmark: forward+adjoint steps per second of the channel DNS at
+64x64 Fourier modes x 97 Legendre modes (96x145x96 dealiased grid) on one GPU.
+
+One step of the workload is one primal RK3/CN step followed by one adjoint
+step (strength 1, with the two trapezoid ddt_project forcings per step), the
+unit the least-squares-shadowing drivers repeat per time step
+(drivers/wrapper.py:14-30).  Timed: K primal steps then the K adjoint steps
+back over the same window, with the state resident in device memory; the
+host only launches.  Prints one JSON line.
+
+Box: Lx = 2 pi, Lz = pi, Re = 3000, dt = 0.004, meanU = 2/3 (flux 4/3),
+synthetic seeded state (laminar profile + k^-2 perturbation, amplitude 0.02).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "shadow-channel_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+METRIC = "fwd+adjoint steps/s at 64x64x97 vs plain FP64 GPU; grad err vs FP64/FD"
+CFG = dict(Nx=64, Ny=97, Nz=64, Lx=2 * np.pi, Lz=np.pi, Re=3000.0, flux=4.0 / 3.0, dt=0.004)
+HBM_PEAK_GBS = 8000.0
+CLASS_NAMES = {1: "z-inverse FFT (k_zinv)", 2: "x transform + products (k_xmix)",
+               3: "z-forward FFT (k_zfwd)", 4: "y-transform to Gauss points (k_rebuild)",
+               5: "Galerkin projection (k_force)", 6: "band solves (k_solve)"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def fwd_adj(cc, C0, nsteps):
+    C = C0.copy()
+    cc.c_primal(0, C)
+    AC = np.zeros_like(C0)
+    cc.c_adjoint(nsteps, 0, AC, 0, 1.0)
+    return C, AC
+
+
+def cpu_baseline():
+    """The oracle (single-threaded FP64 restatement, computing the reference's
+    discarded terms as well) on a bounded sample: one primal step and one
+    adjoint step of the same 64x64x97 workload."""
+    import pyorc
+    from channel.synthetic import synthetic_state
+    c = CFG
+    L = pyorc.load()
+    L.orc_set_faithful_cost(1)
+    o = pyorc.Oracle(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"], 1)
+    C0 = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=0)
+    t0 = time.perf_counter()
+    o.primal(0, C0.copy())
+    AC = np.zeros_like(C0)
+    o.adjoint(1, 0, AC, 0, 1.0)
+    t = time.perf_counter() - t0
+    L.orc_set_faithful_cost(0)
+    o.destroy()
+    return {"value": 1.0 / t, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": "1 primal + 1 adjoint step of the 64x64x97 workload, oracle (FP64 "
+                      "restatement, single thread, incl. the reference's discarded "
+                      "tangent/incremental-adjoint transforms), %.1f s" % t}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--roofline-class", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the solver runs one simulation per device and "
+              "multi-GPU sharding is not part of this build (--gpus must be 1)",
+              file=sys.stderr)
+        return 2
+    import c_channel as cc
+    from channel.synthetic import synthetic_state
+    c = CFG
+    K, W = args.steps, args.warmup
+    C0 = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=0)
+
+    # warm-up: a window of W steps
+    if W > 0:
+        assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
+                         c["dt"], W) == 0
+        fwd_adj(cc, C0, W)
+        cc.c_destroy()
+    assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
+                     c["dt"], K) == 0
+    log("timing %d fwd+adj steps" % K)
+    cc.sc_prof_enable(args.roofline_class)
+    cc.sc_synchronize()
+    t0 = time.perf_counter()
+    C, AC = fwd_adj(cc, C0, K)
+    cc.sc_synchronize()
+    elapsed = time.perf_counter() - t0
+    n, ms, nbytes = cc.sc_prof_read()
+    cc.sc_prof_enable(0)
+    assert cc.last_status() == 0 and np.all(np.isfinite(C)) and np.all(np.isfinite(AC))
+    cc.c_destroy()
+    achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    roof = {"kernel": CLASS_NAMES.get(args.roofline_class, str(args.roofline_class)),
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "launches": n,
+            "avg_launch_us": round(1e3 * ms / n, 2) if n else None,
+            "bytes_per_launch": round(nbytes / n) if n else None}
+    line = {
+        "metric": METRIC, "value": round(K / elapsed, 3), "unit": "steps/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(1e3 * elapsed / K, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "fwd+adj step: 1 primal RK3/CN step + 1 adjoint step "
+                               "(strength 1), state resident in HBM",
+                   "grid": "64x64x97 (96x145x96 dealiased)", "Lx": "2pi", "Lz": "pi",
+                   "Re": c["Re"], "dt": c["dt"], "meanU": c["flux"] / 2,
+                   "window_steps": K, "precision": "fp64"},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle, 1 thread)")
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

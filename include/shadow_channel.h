@@ -1,0 +1,118 @@
+/*
+ * shadow_channel.h -- C ABI of the MI355X channel-DNS library
+ * (libshadowchannel.so).
+ *
+ * These are the functions the reference's SWIG module `c_channel` binds
+ * (channel/channel.i), with the numpy typemaps expanded into plain pointers
+ * and dimensions.  Arrays are host memory, C-contiguous, complex values laid
+ * out as {double re, im} (fftw_complex, channel/mcomplex.h:19-22):
+ *   C, IC, AC   [Nz][2][Ny-2][Nx/2]
+ *   flow        [3][3Nx/2][qpts][3Nz/2]        (qpts = 3Ny/2)
+ * One simulation per process, as in the reference (all state is global;
+ * calls are not re-entrant).  Where the reference aborts on a failed
+ * assert, these functions return SC_EINVAL and leave the state unchanged.
+ */
+#ifndef SHADOW_CHANNEL_H
+#define SHADOW_CHANNEL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } sc_complex;
+
+#define SC_OK 0
+#define SC_EINIT 1        /* c_init: bad grid (EXIT_FAILURE in the reference) */
+#define SC_ENOTREADY -2   /* no successful c_init */
+#define SC_EINVAL -3      /* a dimension or step-range assert would fail */
+#define SC_EIO -4         /* file could not be read or written */
+#define SC_EDEVICE -10    /* HIP runtime error (see sc_last_error) */
+
+/* channel/channel.i:17-23 -> init (channel/channel.c:207-391).
+   Returns 0, or 1 if Nx % 4, Nz % 2 or Ny > 4 fail. */
+int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re,
+           double flux, double dt, int nstep);
+
+/* channel/channel.i:26-31 -> destroy (channel/channel.c:52-199) */
+void c_destroy(void);
+
+/* channel/channel.i:34-53 -> primal (channel/channel.c:399-478).
+   All four dims 0 (or C_given NULL): laminar profile + rand() start.
+   Otherwise C_given is the initial state and receives the final state. */
+int c_primal(int ru_steps, sc_complex *C_given,
+             int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_dup);
+
+/* channel/channel.i:56-71 -> tangent (channel/channel.c:486-587).
+   Requires 0 <= start_step < end_step <= nsteps. */
+int c_tangent(int start_step, int end_step, sc_complex *IC_given,
+              int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_dup, int inhomo);
+
+/* channel/channel.i:74-89 -> adjoint (channel/channel.c:596-696).
+   Requires 0 <= end_step < start_step <= nsteps. */
+int c_adjoint(int start_step, int end_step, sc_complex *AC_given,
+              int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_dup,
+              int inhomo, double strength);
+
+/* channel/channel.i:92-106 -> ddt_project (channel/channel.c:704-780).
+   Returns num/den (NaN on an invalid call). */
+double c_ddt_project(int i_step, sc_complex *IC_given,
+                     int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_dup);
+
+/* channel/channel.i:109-143: state at step i (MC[3i] / MIC[3i]).  The
+   pointer is library-owned host memory refreshed by each call for that
+   step, valid until c_destroy. */
+int c_getsoln(int i_step, sc_complex **MC_ptr,
+              int *Nz_ptr, int *Nvar_ptr, int *Ny_ptr, int *Nx_ptr);
+int c_getincresoln(int i_step, sc_complex **MIC_ptr,
+                   int *Nz_ptr, int *Nvar_ptr, int *Ny_ptr, int *Nx_ptr);
+
+/* channel/channel.i:146-169 -> spec2phys (channel/spec2phys.c:10-90) */
+int c_spec2phys(sc_complex *C_ptr, int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_dup,
+                double *flow_ptr, int Nd_flow, int Nx_flow, int Ny_flow, int Nz_flow);
+
+/* channel/channel.i:172-203 -> read_solution / save_solution
+   (channel/channel.c:791-1028).  Datasets alpha, beta, ialpha, ibeta as
+   [dimR][Nz][Nx/2] complex plus Nx, Ny, Nz, dt, Re, mpg, in the container
+   format described in INTEGRATION.md. */
+int c_read_solution(const char *filename, sc_complex *C_ptr,
+                    int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_dup);
+int c_save_solution(const char *filename, sc_complex *C_ptr,
+                    int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_dup);
+
+/* channel/channel.i:206-218: Gauss points and weights (library-owned) */
+int c_quad(double **y_ptr, int *qpts_ptr1, double **w_ptr, int *qpts_ptr2);
+
+/* channel/channel.i:221-229 */
+int c_qpts(void);
+int c_dimR(void);
+int c_dimQ(void);
+int c_Nx(void);
+int c_Nz(void);
+int c_nsteps(void);
+
+/* ---- additions without a counterpart in the reference ---- */
+/* Text of the last error. */
+const char *sc_last_error(void);
+/* 0 after a clean run; -1 when a CFL violation truncated the last primal or
+   tangent run (the reference prints "Pass1 failure" and returns). */
+int sc_last_status(void);
+/* Reseed the process-global rand() stream used by the laminar start
+   (glibc TYPE_3 generator; the reference calls the C library rand()). */
+void sc_srand(unsigned seed);
+/* Kernel timing for benchmarks: with kernel_class > 0, every launch of that
+   class is bracketed by HIP events on its stream; sc_prof_read returns the
+   number of launches and the summed milliseconds, and clears the record.
+   Classes: 1 z-inverse FFT, 2 x transform/products, 3 z-forward FFT,
+   4 y-transform to Gauss points, 5 Galerkin projection, 6 band solves. */
+int sc_prof_enable(int kernel_class);
+int sc_prof_read(double *total_ms);
+/* Algorithmic bytes (compulsory reads + writes, each element once) of the
+   launches recorded since sc_prof_enable. */
+double sc_prof_bytes(void);
+/* Wait until all work queued by the library has finished. */
+int sc_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

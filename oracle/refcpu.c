@@ -74,41 +74,56 @@ static void plan_free(fftplan *p)
     memset(p, 0, sizeof(*p));
 }
 
-static void fft_rec(const fftplan *p, int n, int fi, const cx *in, int is,
-                    cx *out, const cx *tw)
+/* Stockham autosort pass of radix r: x (length n) -> y, Ns = product of the
+   radices already applied; twiddle exp(sign 2 pi i q k / (Ns r)) */
+static void fft_pass(const fftplan *p, int r, int Ns, const cx *x, cx *y, const cx *tw)
 {
-    if (n == 1) { out[0] = in[0]; return; }
-    int r = p->f[fi], m = n / r, step = p->n / n;
-    for (int q = 0; q < r; ++q)
-        fft_rec(p, m, fi + 1, in + q * is, is * r, out + q * m, tw);
-    cx t[64];
-    cx *tb = r <= 64 ? t : (cx *) malloc(sizeof(cx) * r);
-    for (int k = 0; k < m; ++k) {
+    int n = p->n, nb = n / r, step = n / (Ns * r);
+    cx v[64];
+    cx w_r[64];
+    for (int q = 0; q < r; ++q) w_r[q] = tw[(q * (n / r)) % n];   /* exp(sign 2 pi i q / r) */
+    for (int j = 0; j < nb; ++j) {
+        int k = j % Ns;
         for (int q = 0; q < r; ++q) {
-            cx a = out[q * m + k];
-            cx w = tw[((long) q * k * step) % p->n];
-            tb[q].re = a.re * w.re - a.im * w.im;
-            tb[q].im = a.re * w.im + a.im * w.re;
+            cx a = x[j + q * nb];
+            if (q > 0 && k > 0) {
+                cx w = tw[q * k * step];
+                cx t = { a.re * w.re - a.im * w.im, a.re * w.im + a.im * w.re };
+                a = t;
+            }
+            v[q] = a;
+        }
+        int base = (j / Ns) * Ns * r + k;
+        if (r == 2) {
+            y[base].re = v[0].re + v[1].re; y[base].im = v[0].im + v[1].im;
+            y[base + Ns].re = v[0].re - v[1].re; y[base + Ns].im = v[0].im - v[1].im;
+            continue;
         }
         for (int s = 0; s < r; ++s) {
             double sr = 0.0, si = 0.0;
             for (int q = 0; q < r; ++q) {
-                cx w = tw[((long) ((q * s) % r) * m * step) % p->n];
-                sr += tb[q].re * w.re - tb[q].im * w.im;
-                si += tb[q].re * w.im + tb[q].im * w.re;
+                cx w = w_r[(q * s) % r];
+                sr += v[q].re * w.re - v[q].im * w.im;
+                si += v[q].re * w.im + v[q].im * w.re;
             }
-            out[k + s * m].re = sr;
-            out[k + s * m].im = si;
+            y[base + s * Ns].re = sr;
+            y[base + s * Ns].im = si;
         }
     }
-    if (tb != t) free(tb);
 }
 
 /* in-place transform of x (contiguous), sign +1 (backward) or -1 (forward) */
 static void fft_c(fftplan *p, cx *x, int sign)
 {
-    fft_rec(p, p->n, 0, x, 1, p->buf, sign > 0 ? p->twp : p->twm);
-    memcpy(x, p->buf, sizeof(cx) * p->n);
+    const cx *tw = sign > 0 ? p->twp : p->twm;
+    cx *a = x, *b = p->buf;
+    int Ns = 1;
+    for (int f = 0; f < p->nf; ++f) {
+        fft_pass(p, p->f[f], Ns, a, b, tw);
+        Ns *= p->f[f];
+        cx *t = a; a = b; b = t;
+    }
+    if (a != x) memcpy(x, a, sizeof(cx) * p->n);
 }
 
 /* ------------------------------------------------------------------ */
@@ -593,37 +608,50 @@ void orc_pass2_state(orc_cplx *Ua, const orc_cplx *AUa) { pass2(Ua, AUa, 0); }
 /* ------------------------------------------------------------------ */
 /* coefficients -> Gauss points (channel/initAlphaBeta2.c:12-240)       */
 /* ------------------------------------------------------------------ */
-static void rebuild_mode(const cx *Ca, cx *Ua, int z, int x)
+/* all modes x >= x0 of row z: Gauss-point values from the coefficients
+   (channel/initAlphaBeta2.c:96-208), sums over j in ascending order */
+static void rebuild_row(const cx *Ca, cx *Ua, int z, int x0)
 {
-    int q = G.qpts, nQ = G.dimQ, nR = G.dimR;
-    double kx = G.Kx[x], kz = G.Kz[z], k2 = G.K2[z * G.Mx + x];
+    int q = G.qpts, nQ = G.dimQ, nR = G.dimR, Mx = G.Mx;
+    double kz = G.Kz[z];
+    cx acc[5][512];
     for (int i = 0; i < q; ++i) {
-        double vr = 0, vi = 0, fr = 0, fi = 0, qr = 0, qi = 0;
-        double gr = 0, gi = 0, pr = 0, pi = 0;
+        for (int m = 0; m < 5; ++m)
+            for (int x = x0; x < Mx; ++x) acc[m][x].re = acc[m][x].im = 0.0;
         for (int j = 0; j < nQ; ++j) {
-            cx a = Ca[ci(z, 0, j, x)];
             double cq = G.Q[i * nQ + j], cp = G.Qp[i * nQ + j], c2 = G.Qpp[i * nQ + j];
-            vr += cq * a.re; vi += cq * a.im;
-            fr -= cp * a.re; fi -= cp * a.im;
-            qr += c2 * a.re; qi += c2 * a.im;
+            const cx *a = Ca + ci(z, 0, j, 0);
+            for (int x = x0; x < Mx; ++x) {
+                acc[0][x].re += cq * a[x].re; acc[0][x].im += cq * a[x].im;
+                acc[1][x].re -= cp * a[x].re; acc[1][x].im -= cp * a[x].im;
+                acc[2][x].re += c2 * a[x].re; acc[2][x].im += c2 * a[x].im;
+            }
         }
         for (int j = 0; j < nR; ++j) {
-            cx b = Ca[ci(z, 1, j, x)];
             double cr = G.R[i * nR + j], cp = G.Rp[i * nR + j];
-            gr += cr * b.re; gi += cr * b.im;
-            pr += cp * b.re; pi += cp * b.im;
+            const cx *b = Ca + ci(z, 1, j, 0);
+            for (int x = x0; x < Mx; ++x) {
+                acc[3][x].re += cr * b[x].re; acc[3][x].im += cr * b[x].im;
+                acc[4][x].re += cp * b[x].re; acc[4][x].im += cp * b[x].im;
+            }
         }
-        cx *o = Ua;
-        o[ui(z, S_V, i, x)].re = vr;
-        o[ui(z, S_V, i, x)].im = vi;
-        o[ui(z, S_U, i, x)].re = (kx * fi + kz * gi) / k2;
-        o[ui(z, S_U, i, x)].im = -(kx * fr + kz * gr) / k2;
-        o[ui(z, S_W, i, x)].re = (-kx * gi + kz * fi) / k2;
-        o[ui(z, S_W, i, x)].im = (kx * gr - kz * fr) / k2;
-        o[ui(z, S_DU, i, x)].re = (-kx * qi + kz * pi) / k2;
-        o[ui(z, S_DU, i, x)].im = (kx * qr - kz * pr) / k2;
-        o[ui(z, S_DW, i, x)].re = -(kx * pi + kz * qi) / k2;
-        o[ui(z, S_DW, i, x)].im = (kx * pr + kz * qr) / k2;
+        for (int x = x0; x < Mx; ++x) {
+            double kx = G.Kx[x], k2 = G.K2[z * Mx + x];
+            double vr = acc[0][x].re, vi = acc[0][x].im, fr = acc[1][x].re, fi = acc[1][x].im;
+            double qr = acc[2][x].re, qi = acc[2][x].im, gr = acc[3][x].re, gi = acc[3][x].im;
+            double pr = acc[4][x].re, pi = acc[4][x].im;
+            cx *o = Ua;
+            o[ui(z, S_V, i, x)].re = vr;
+            o[ui(z, S_V, i, x)].im = vi;
+            o[ui(z, S_U, i, x)].re = (kx * fi + kz * gi) / k2;
+            o[ui(z, S_U, i, x)].im = -(kx * fr + kz * gr) / k2;
+            o[ui(z, S_W, i, x)].re = (-kx * gi + kz * fi) / k2;
+            o[ui(z, S_W, i, x)].im = (kx * gr - kz * fr) / k2;
+            o[ui(z, S_DU, i, x)].re = (-kx * qi + kz * pi) / k2;
+            o[ui(z, S_DU, i, x)].im = (kx * qr - kz * pr) / k2;
+            o[ui(z, S_DW, i, x)].re = -(kx * pi + kz * qi) / k2;
+            o[ui(z, S_DW, i, x)].im = (kx * pr + kz * qr) / k2;
+        }
     }
 }
 
@@ -680,11 +708,11 @@ static void coef_to_gauss(const cx *Ca, cx *Ua, int wall)
 {
     rebuild_mean(Ca, Ua);
     if (wall) wall_mean(Ca);
-    for (int z = 0; z < G.Nz; ++z)
-        for (int x = (z == 0 ? 1 : 0); x < G.Mx; ++x) {
-            rebuild_mode(Ca, Ua, z, x);
-            if (wall) wall_accumulate(Ca, z, x);
-        }
+    for (int z = 0; z < G.Nz; ++z) {
+        rebuild_row(Ca, Ua, z, z == 0 ? 1 : 0);
+        if (wall)
+            for (int x = (z == 0 ? 1 : 0); x < G.Mx; ++x) wall_accumulate(Ca, z, x);
+    }
 }
 
 void orc_coef_to_gauss(const orc_cplx *Ca, orc_cplx *Ua) { coef_to_gauss(Ca, Ua, 0); }
@@ -697,46 +725,49 @@ static void forcing_modes(const cx *Ua, int z, int x0, int adj)
 {
     int q = G.qpts, nQ = G.dimQ, nR = G.dimR, Mx = G.Mx;
     double kz = G.Kz[z];
-    for (int x = x0; x < Mx; ++x) {
-        double kx = G.Kx[x], k2 = G.K2[z * Mx + x];
-        for (int i = 0; i < nQ; ++i) {
-            double fr = 0, fi = 0;
-            for (int j = 0; j < q; ++j) {
-                cx hx = Ua[ui(z, S_HX, j, x)], hy = Ua[ui(z, S_HY, j, x)];
-                cx hz = Ua[ui(z, S_HZ, j, x)];
-                double a = G.Qpw[i * q + j], b = G.Qw[i * q + j];
-                double tr = a * (-kx * hx.im - kz * hz.im) - k2 * (b * hy.re);
-                double ti = a * (kx * hx.re + kz * hz.re) - k2 * (b * hy.im);
+    for (int i = 0; i < nQ; ++i)
+        for (int x = x0; x < Mx; ++x) G.Fa[i * Mx + x].re = G.Fa[i * Mx + x].im = 0.0;
+    for (int i = 0; i < nR; ++i)
+        for (int x = x0; x < Mx; ++x) G.Fb[i * Mx + x].re = G.Fb[i * Mx + x].im = 0.0;
+    for (int i = 0; i < nQ; ++i) {
+        cx *F = G.Fa + i * Mx;
+        for (int j = 0; j < q; ++j) {
+            double a = G.Qpw[i * q + j], b = G.Qw[i * q + j];
+            double c = adj ? G.Qppw[i * q + j] : 0.0;
+            const cx *hx = Ua + ui(z, S_HX, j, 0), *hy = Ua + ui(z, S_HY, j, 0);
+            const cx *hz = Ua + ui(z, S_HZ, j, 0);
+            const cx *sx = Ua + ui(z, S_DU, j, 0), *sz = Ua + ui(z, S_DW, j, 0);
+            for (int x = x0; x < Mx; ++x) {
+                double kx = G.Kx[x], k2 = G.K2[z * Mx + x];
+                double tr = a * (-kx * hx[x].im - kz * hz[x].im) - k2 * (b * hy[x].re);
+                double ti = a * (kx * hx[x].re + kz * hz[x].re) - k2 * (b * hy[x].im);
                 if (adj) {
-                    cx sx = Ua[ui(z, S_DU, j, x)], sz = Ua[ui(z, S_DW, j, x)];
-                    double c = G.Qppw[i * q + j];
-                    tr += c * (kz * sx.im - kx * sz.im);
-                    ti += c * (-kz * sx.re + kx * sz.re);
+                    tr += c * (kz * sx[x].im - kx * sz[x].im);
+                    ti += c * (-kz * sx[x].re + kx * sz[x].re);
                 }
-                fr += tr;
-                fi += ti;
+                F[x].re += tr;
+                F[x].im += ti;
             }
-            G.Fa[i * Mx + x].re = fr;
-            G.Fa[i * Mx + x].im = fi;
         }
-        for (int i = 0; i < nR; ++i) {
-            double fr = 0, fi = 0;
-            for (int j = 0; j < q; ++j) {
-                cx hx = Ua[ui(z, S_HX, j, x)], hz = Ua[ui(z, S_HZ, j, x)];
-                double a = G.Rw[i * q + j];
-                double tr = a * (-kz * hx.im + kx * hz.im);
-                double ti = a * (kz * hx.re - kx * hz.re);
+    }
+    for (int i = 0; i < nR; ++i) {
+        cx *F = G.Fb + i * Mx;
+        for (int j = 0; j < q; ++j) {
+            double a = G.Rw[i * q + j];
+            double c = adj ? G.Rpw[i * q + j] : 0.0;
+            const cx *hx = Ua + ui(z, S_HX, j, 0), *hz = Ua + ui(z, S_HZ, j, 0);
+            const cx *sx = Ua + ui(z, S_DU, j, 0), *sz = Ua + ui(z, S_DW, j, 0);
+            for (int x = x0; x < Mx; ++x) {
+                double kx = G.Kx[x];
+                double tr = a * (-kz * hx[x].im + kx * hz[x].im);
+                double ti = a * (kz * hx[x].re - kx * hz[x].re);
                 if (adj) {
-                    cx sx = Ua[ui(z, S_DU, j, x)], sz = Ua[ui(z, S_DW, j, x)];
-                    double c = G.Rpw[i * q + j];
-                    tr += c * (-kz * sz.im - kx * sx.im);
-                    ti += c * (kz * sz.re + kx * sx.re);
+                    tr += c * (-kz * sz[x].im - kx * sx[x].im);
+                    ti += c * (kz * sz[x].re + kx * sx[x].re);
                 }
-                fr += tr;
-                fi += ti;
+                F[x].re += tr;
+                F[x].im += ti;
             }
-            G.Fb[i * Mx + x].re = fr;
-            G.Fb[i * Mx + x].im = fi;
         }
     }
 }
@@ -803,7 +834,7 @@ static void stage_modes(cx *Ca, const cx *Ua, const cx *LUa, cx *Uout,
                 for (int x = x0; x < Mx; ++x) store[ci(z, v, i, x)] = Ca[ci(z, v, i, x)];
     }
     /* velocities at Gauss points from the solved coefficients */
-    for (int x = x0; x < Mx; ++x) rebuild_mode(Ca, Uout, z, x);
+    rebuild_row(Ca, Uout, z, x0);
     if (wall)
         for (int x = x0; x < Mx; ++x) wall_accumulate(Ca, z, x);
     /* explicit part of the next stage */

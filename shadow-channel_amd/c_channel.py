@@ -1,0 +1,228 @@
+"""c_channel: the reference's SWIG module (channel/channel.i) over the C ABI of
+libshadowchannel.so (include/shadow_channel.h).
+
+Same function names and Python-visible behaviour as the SWIG build with
+numpy.i typemaps:
+  * INPLACE_ARRAY4 arguments must be C-contiguous complex128 (float64 for the
+    flow array) numpy arrays of rank 4, else TypeError, as numpy.i raises;
+  * c_getsoln / c_getincresoln / c_quad return numpy views of library-owned
+    memory (valid until c_destroy; c_getsoln refreshes the view of that step);
+  * where the C side of the reference asserts (and would abort the process),
+    an AssertionError is raised instead.
+The library is the only compute path: importing this module without a built
+libshadowchannel.so raises ImportError.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libshadowchannel.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("libshadowchannel.so is not built (run __graft_entry__.build() "
+                      "or make -C shadow-channel_amd/csrc)")
+
+_L = ctypes.CDLL(LIB_PATH)
+_c = ctypes
+_i, _d, _vp = _c.c_int, _c.c_double, _c.c_void_p
+_ip = _c.POINTER(_c.c_int)
+_dpp = _c.POINTER(_c.POINTER(_c.c_double))
+
+_L.c_init.argtypes = [_i, _i, _i, _d, _d, _d, _d, _d, _i]
+_L.c_init.restype = _i
+_L.c_destroy.argtypes = []
+_L.c_destroy.restype = None
+_L.c_primal.argtypes = [_i, _vp, _i, _i, _i, _i]
+_L.c_primal.restype = _i
+_L.c_tangent.argtypes = [_i, _i, _vp, _i, _i, _i, _i, _i]
+_L.c_tangent.restype = _i
+_L.c_adjoint.argtypes = [_i, _i, _vp, _i, _i, _i, _i, _i, _d]
+_L.c_adjoint.restype = _i
+_L.c_ddt_project.argtypes = [_i, _vp, _i, _i, _i, _i]
+_L.c_ddt_project.restype = _d
+_L.c_getsoln.argtypes = [_i, _c.POINTER(_vp), _ip, _ip, _ip, _ip]
+_L.c_getsoln.restype = _i
+_L.c_getincresoln.argtypes = [_i, _c.POINTER(_vp), _ip, _ip, _ip, _ip]
+_L.c_getincresoln.restype = _i
+_L.c_spec2phys.argtypes = [_vp, _i, _i, _i, _i, _vp, _i, _i, _i, _i]
+_L.c_spec2phys.restype = _i
+_L.c_read_solution.argtypes = [_c.c_char_p, _vp, _i, _i, _i, _i]
+_L.c_read_solution.restype = _i
+_L.c_save_solution.argtypes = [_c.c_char_p, _vp, _i, _i, _i, _i]
+_L.c_save_solution.restype = _i
+_L.c_quad.argtypes = [_dpp, _ip, _dpp, _ip]
+_L.c_quad.restype = _i
+for _f in ("c_qpts", "c_dimR", "c_dimQ", "c_Nx", "c_Nz", "c_nsteps"):
+    getattr(_L, _f).argtypes = []
+    getattr(_L, _f).restype = _i
+_L.sc_last_error.restype = _c.c_char_p
+_L.sc_last_status.restype = _i
+_L.sc_srand.argtypes = [_c.c_uint]
+_L.sc_prof_enable.argtypes = [_i]
+_L.sc_prof_enable.restype = _i
+_L.sc_prof_read.argtypes = [_c.POINTER(_d)]
+_L.sc_prof_read.restype = _i
+_L.sc_prof_bytes.argtypes = []
+_L.sc_prof_bytes.restype = _d
+_L.sc_synchronize.argtypes = []
+_L.sc_synchronize.restype = _i
+
+SC_EINVAL = -3
+SC_ENOTREADY = -2
+
+
+def _inplace(a, dtype, name):
+    if not isinstance(a, np.ndarray):
+        raise TypeError("%s must be a numpy array" % name)
+    if a.dtype != dtype:
+        raise TypeError("%s must have dtype %s" % (name, np.dtype(dtype).name))
+    if a.ndim != 4:
+        raise TypeError("%s must have 4 dimensions" % name)
+    if not a.flags.c_contiguous or not a.flags.writeable:
+        raise TypeError("%s must be C-contiguous and writeable" % name)
+    return a
+
+
+def _check(rc, what):
+    if rc == 0:
+        return
+    msg = "%s failed (%d): %s" % (what, rc, last_error())
+    if rc in (SC_EINVAL, SC_ENOTREADY):
+        raise AssertionError(msg)
+    raise RuntimeError(msg)
+
+
+def last_error():
+    e = _L.sc_last_error()
+    return e.decode() if e else ""
+
+
+def last_status():
+    """0 after a clean run, -1 after a CFL violation truncated the last run."""
+    return _L.sc_last_status()
+
+
+def c_init(Nx, Ny, Nz, Lx, Lz, Re, flux, dt, nstep):
+    return _L.c_init(int(Nx), int(Ny), int(Nz), float(Lx), float(Lz), float(Re),
+                     float(flux), float(dt), int(nstep))
+
+
+def c_destroy():
+    _L.c_destroy()
+
+
+def c_primal(ru_steps, C_given):
+    C = _inplace(C_given, np.complex128, "C_given")
+    ptr = C.ctypes.data if C.size else None
+    _check(_L.c_primal(int(ru_steps), ptr, *C.shape), "c_primal")
+
+
+def c_tangent(start_step, end_step, IC_given, inhomo):
+    IC = _inplace(IC_given, np.complex128, "IC_given")
+    _check(_L.c_tangent(int(start_step), int(end_step), IC.ctypes.data, *IC.shape, int(inhomo)),
+           "c_tangent")
+
+
+def c_adjoint(start_step, end_step, AC_given, inhomo, strength):
+    AC = _inplace(AC_given, np.complex128, "AC_given")
+    _check(_L.c_adjoint(int(start_step), int(end_step), AC.ctypes.data, *AC.shape, int(inhomo),
+                        float(strength)), "c_adjoint")
+
+
+def c_ddt_project(i_step, IC_given):
+    IC = _inplace(IC_given, np.complex128, "IC_given")
+    r = _L.c_ddt_project(int(i_step), IC.ctypes.data, *IC.shape)
+    if np.isnan(r) and last_error():
+        pass
+    return r
+
+
+def _view(fn, i_step):
+    p = _vp()
+    dims = [_i() for _ in range(4)]
+    _check(fn(int(i_step), _c.byref(p), *[_c.byref(x) for x in dims]), fn.__name__)
+    shape = tuple(x.value for x in dims)
+    n = int(np.prod(shape))
+    buf = (_c.c_double * (2 * n)).from_address(p.value)
+    return np.frombuffer(buf, dtype=np.complex128).reshape(shape)
+
+
+def c_getsoln(i_step):
+    return _view(_L.c_getsoln, i_step)
+
+
+def c_getincresoln(i_step):
+    return _view(_L.c_getincresoln, i_step)
+
+
+def c_spec2phys(C_ptr, flow_ptr):
+    C = _inplace(C_ptr, np.complex128, "C")
+    F = _inplace(flow_ptr, np.float64, "flow")
+    _check(_L.c_spec2phys(C.ctypes.data, *C.shape, F.ctypes.data, *F.shape), "c_spec2phys")
+
+
+def c_read_solution(filename, C_ptr):
+    C = _inplace(C_ptr, np.complex128, "C")
+    _check(_L.c_read_solution(str(filename).encode(), C.ctypes.data, *C.shape),
+           "c_read_solution")
+
+
+def c_save_solution(filename, C_ptr):
+    C = _inplace(C_ptr, np.complex128, "C")
+    _check(_L.c_save_solution(str(filename).encode(), C.ctypes.data, *C.shape),
+           "c_save_solution")
+
+
+def c_quad():
+    y = _c.POINTER(_c.c_double)()
+    w = _c.POINTER(_c.c_double)()
+    n1, n2 = _i(), _i()
+    _check(_L.c_quad(_c.byref(y), _c.byref(n1), _c.byref(w), _c.byref(n2)), "c_quad")
+    return (np.ctypeslib.as_array(y, shape=(n1.value,)),
+            np.ctypeslib.as_array(w, shape=(n2.value,)))
+
+
+def c_qpts():
+    return _L.c_qpts()
+
+
+def c_dimR():
+    return _L.c_dimR()
+
+
+def c_dimQ():
+    return _L.c_dimQ()
+
+
+def c_Nx():
+    return _L.c_Nx()
+
+
+def c_Nz():
+    return _L.c_Nz()
+
+
+def c_nsteps():
+    return _L.c_nsteps()
+
+
+def sc_srand(seed):
+    _L.sc_srand(int(seed))
+
+
+def sc_prof_enable(kernel_class):
+    return _L.sc_prof_enable(int(kernel_class))
+
+
+def sc_prof_read():
+    """(launches, milliseconds, algorithmic bytes) recorded since sc_prof_enable."""
+    nbytes = _L.sc_prof_bytes()
+    ms = _d(0.0)
+    n = _L.sc_prof_read(_c.byref(ms))
+    return n, ms.value, nbytes
+
+
+def sc_synchronize():
+    _check(_L.sc_synchronize(), "sc_synchronize")

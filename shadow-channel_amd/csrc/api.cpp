@@ -1,0 +1,730 @@
+// C ABI of the MI355X channel-DNS library and its time-stepping drivers.
+//
+// The drivers follow channel/channel.c: primal (:399-478), tangent (:486-587),
+// adjoint (:596-696), ddt_project (:704-780), with the stage loops over all
+// Fourier modes replaced by batched kernels.  State stays on the device for
+// the whole run; the host only launches.  Two deliberate changes of
+// mechanism, with identical results:
+//  * the tangent and the adjoint rebuild the base velocity of each stage from
+//    the stored stage coefficients MC (the reference re-integrates the base
+//    in the tangent; MC holds exactly the coefficients that integration
+//    produces, so the rebuilt velocities are the same);
+//  * the terms the reference computes and discards (the tangent side of the
+//    nonlinear pass in the primal, the incremental-adjoint side of the
+//    adjoint pass) are not computed.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+
+#include "../../include/shadow_channel.h"
+#include "sc_internal.h"
+
+namespace sc {
+
+Context &ctx()
+{
+    static Context c;
+    return c;
+}
+
+std::string &last_error()
+{
+    static std::string e;
+    return e;
+}
+
+// ---------------------------------------------------------------- timing
+void prof_begin(Context &c, hipStream_t s, int id, double bytes)
+{
+    if (c.prof_class != id) return;
+    c.prof_bytes += bytes;
+    if (c.ev_used + 2 > c.ev_pool.size()) {
+        for (int i = 0; i < 256; ++i) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            c.ev_pool.push_back(e);
+        }
+    }
+    (void) hipEventRecord(c.ev_pool[c.ev_used], s);
+}
+
+void prof_end(Context &c, hipStream_t s, int id)
+{
+    if (c.prof_class != id || c.ev_used + 2 > c.ev_pool.size()) return;
+    (void) hipEventRecord(c.ev_pool[c.ev_used + 1], s);
+    c.ev_used += 2;
+}
+
+// ---------------------------------------------------------------- rand()
+// glibc random() TYPE_3: r_i = r_{i-3} + r_{i-31} (mod 2^32) after the
+// srand() seeding r_0 = seed, r_i = 16807 r_{i-1} mod (2^31 - 1) (i < 31),
+// r_31..33 = r_0..2 and 310 discarded outputs; rand() = r_i >> 1.
+static unsigned g_ring[31];
+static bool g_seeded = false;
+
+static int glibc_rand();
+
+static void glibc_srand(unsigned seed)
+{
+    unsigned r[34];
+    long long v = seed == 0 ? 1 : seed;
+    r[0] = (unsigned) v;
+    for (int i = 1; i < 31; ++i) {
+        v = (16807LL * v) % 2147483647LL;
+        if (v < 0) v += 2147483647LL;
+        r[i] = (unsigned) v;
+    }
+    for (int i = 31; i < 34; ++i) r[i] = r[i - 31];
+    for (int i = 0; i < 31; ++i) g_ring[i] = r[i + 3];
+    g_seeded = true;
+    for (int i = 0; i < 310; ++i) glibc_rand();
+}
+
+static int glibc_rand()
+{
+    if (!g_seeded) glibc_srand(1);
+    unsigned v = g_ring[0] + g_ring[28];
+    memmove(g_ring, g_ring + 1, sizeof(unsigned) * 30);
+    g_ring[30] = v;
+    return (int) (v >> 1);
+}
+
+// ---------------------------------------------------------------- helpers
+static void free_all(Context &c)
+{
+    void *ptrs[] = {c.dKx, c.dKz, c.dcfl2, c.dW, c.dQ, c.dQp, c.dQpp, c.dR, c.dRp, c.dQw,
+                    c.dQpw, c.dQppw, c.dRw, c.dRpw, c.dQs, c.dQps, c.dQpps, c.dRs, c.dRps,
+                    c.dRowsum, c.C, c.IC, c.AC, c.U, c.IU, c.AU, c.LU, c.Ut, c.Ut2, c.MC,
+                    c.MIC, c.Csol, c.Ctmp, c.Ctmp2, c.Fa, c.Fb, c.FLa, c.FLb, c.W1, c.W2,
+                    c.SP, c.flowdev, c.cflmax, c.dotpart, c.scal, c.abort_flag, c.pz.tw,
+                    c.px.tw};
+    for (void *p : ptrs)
+        if (p) (void) hipFree(p);
+    for (int a = 0; a < 2; ++a)
+        for (int k = 0; k < 3; ++k) {
+            if (c.dLa[a][k]) (void) hipFree(c.dLa[a][k]);
+            if (c.dLb[a][k]) (void) hipFree(c.dLb[a][k]);
+            if (c.dLm[a][k]) (void) hipFree(c.dLm[a][k]);
+            if (c.dAdd[a][k]) (void) hipFree(c.dAdd[a][k]);
+        }
+    for (hipEvent_t e : c.ev_pool) (void) hipEventDestroy(e);
+    if (c.stream) (void) hipStreamDestroy(c.stream);
+    c = Context();
+}
+
+static int dims_ok(const Context &c, int Nz, int Nvar, int Ny, int Nx)
+{
+    if (Nx != c.d.Mx || Ny != c.d.dimR || Nz != c.d.Nz || Nvar != 2) {
+        last_error() = "array shape must be (Nz, 2, dimR, Nx/2)";
+        return 0;
+    }
+    return 1;
+}
+
+static cplx *mc(Context &c, long slot) { return c.MC + c.csize * slot; }
+static cplx *mic(Context &c, long slot) { return c.MIC + c.csize * slot; }
+
+static FieldSpec velocity_and_vorticity(const cplx *Ub)
+{
+    FieldSpec fs;
+    fs.nf = 6;
+    for (int s = 0; s < 3; ++s) {
+        fs.src[s] = Ub;
+        fs.kind[s] = FK_SLOT;
+        fs.slot[s] = s;
+    }
+    fs.src[3] = fs.src[4] = fs.src[5] = Ub;
+    fs.kind[3] = FK_OMEGA_Z;
+    fs.kind[4] = FK_OMEGA_X;
+    fs.kind[5] = FK_OMEGA_Y;
+    fs.slot[3] = fs.slot[4] = fs.slot[5] = 0;
+    return fs;
+}
+
+// H = u x omega into the H slots of Ub (channel/pass1.c, state side)
+static void pass1_state(Context &c, cplx *Ub, int check_cfl)
+{
+    if (check_cfl) (void) hipMemsetAsync(c.cflmax, 0, sizeof(double) * c.d.qpts, c.stream);
+    launch_zinv(c, velocity_and_vorticity(Ub), c.W1);
+    launch_xmix(c, XM_PRIMAL, c.W1, c.W2, check_cfl, nullptr, nullptr);
+    if (check_cfl) launch_cfl_check(c);
+    launch_zfwd(c, c.W2, 3, Ub, ZF_USLOTS);
+}
+
+// IH = u x I omega + I u x omega into the H slots of IUb (channel/pass1.c)
+static void pass1_tangent(Context &c, const cplx *Ub, cplx *IUb, int check_cfl)
+{
+    FieldSpec a = velocity_and_vorticity(Ub), b = velocity_and_vorticity(IUb);
+    for (int f = 0; f < 6; ++f) {
+        a.src[6 + f] = b.src[f];
+        a.kind[6 + f] = b.kind[f];
+        a.slot[6 + f] = b.slot[f];
+    }
+    a.nf = 12;
+    if (check_cfl) (void) hipMemsetAsync(c.cflmax, 0, sizeof(double) * c.d.qpts, c.stream);
+    launch_zinv(c, a, c.W1);
+    launch_xmix(c, XM_TANGENT, c.W1, c.W2, check_cfl, nullptr, nullptr);
+    if (check_cfl) launch_cfl_check(c);
+    launch_zfwd(c, c.W2, 3, IUb, ZF_USLOTS);
+}
+
+// adjoint convective operator (channel/pass2.c, adjoint side) into Udst
+static void pass2_adjoint(Context &c, const cplx *Ub, const cplx *AUb, cplx *Udst)
+{
+    FieldSpec a = velocity_and_vorticity(Ub);
+    for (int s = 0; s < 3; ++s) {
+        a.src[6 + s] = AUb;
+        a.kind[6 + s] = FK_SLOT;
+        a.slot[6 + s] = s;
+    }
+    a.nf = 9;
+    launch_zinv(c, a, c.W1);
+    launch_xmix(c, XM_ADJOINT, c.W1, c.W2, 0, nullptr, nullptr);
+    launch_zfwd(c, c.W2, 6, c.SP, ZF_SCRATCH);
+    launch_assemble_adj(c, c.SP, Udst);
+}
+
+// v -= (num/den) dC/dt with the Gauss-weighted physical inner products
+// (channel/channel.c:704-780); v is a device array.  Result in scal[2].
+static void ddt_project_dev(Context &c, int i, cplx *v)
+{
+    const Dims &d = c.d;
+    long ip = i == 0 ? 3 : 3L * i, im = i == 0 ? 0 : 3L * (i - 1);
+    launch_diff_scale(c, c.Ctmp2, mc(c, ip), mc(c, im), 1.0 / d.dt, c.csize);
+    launch_rebuild(c, c.Ctmp2, c.Ut);
+    launch_rebuild(c, v, c.Ut2);
+    FieldSpec fs;
+    fs.nf = 6;
+    for (int s = 0; s < 3; ++s) {
+        fs.src[s] = c.Ut;
+        fs.src[3 + s] = c.Ut2;
+        fs.kind[s] = fs.kind[3 + s] = FK_SLOT;
+        fs.slot[s] = fs.slot[3 + s] = s;
+    }
+    launch_zinv(c, fs, c.W1);
+    launch_xmix(c, XM_DOT, c.W1, nullptr, 0, nullptr, c.dotpart);
+    int nparts = d.qpts * (d.Pz / xmix_zblock(c, 6));
+    launch_ddt_reduce(c, nparts);
+    launch_ddt_apply(c, v, c.Ctmp2, c.csize);
+}
+
+static int check_abort(Context &c, const char *what)
+{
+    int flag = 0;
+    SC_HIP(hipMemcpy(&flag, c.abort_flag, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) {
+        int y = flag - 1;
+        double m = 0.0;
+        (void) hipMemcpy(&m, c.cflmax + y, sizeof(double), hipMemcpyDeviceToHost);
+        printf("Pass1:  CFL[%d]:%e, Max:%e dt:%e.  CFL condition violated.\n", y, m * c.d.dt,
+               2.8, c.d.dt);
+        printf("%s failure\n", what);
+        c.status = -1;
+    }
+    return flag;
+}
+
+// host restatement of the initial projection (channel/initAlphaBeta.c:24-266)
+static void host_gauss_to_coef(Context &c, const std::vector<cplx> &U, std::vector<cplx> &C)
+{
+    const Dims &d = c.d;
+    const int q = d.qpts, nQ = d.dimQ, nR = d.dimR, Mx = d.Mx;
+    auto ui = [&](int z, int s, int j, int x) { return (((size_t) z * 5 + s) * q + j) * Mx + x; };
+    auto ci = [&](int z, int v, int i, int x) { return (((size_t) z * 2 + v) * nR + i) * Mx + x; };
+    std::vector<double> Ms(c.Rs), Mq(c.Qs);
+    band_factor(Ms.data(), nR, 2);
+    band_factor(Mq.data(), nQ, 4);
+    auto subst = [&](const std::vector<double> &A, int N, int dl, size_t base) {
+        int W = 2 * dl + 1;
+        for (int i = 2; i < N; ++i)
+            for (int m = std::min(dl, i / 2); m >= 1; --m) {
+                double l = A[(size_t) i * W + dl - m];
+                C[base + (size_t) i * Mx].x -= l * C[base + (size_t) (i - 2 * m) * Mx].x;
+                C[base + (size_t) i * Mx].y -= l * C[base + (size_t) (i - 2 * m) * Mx].y;
+            }
+        for (int i = N - 1; i >= 0; --i) {
+            int jmax = std::min(dl, (N - i + 1) / 2 - 1);
+            for (int j = 1; j <= jmax; ++j) {
+                double u = A[(size_t) i * W + dl + j];
+                C[base + (size_t) i * Mx].x -= u * C[base + (size_t) (i + 2 * j) * Mx].x;
+                C[base + (size_t) i * Mx].y -= u * C[base + (size_t) (i + 2 * j) * Mx].y;
+            }
+            C[base + (size_t) i * Mx].x /= A[(size_t) i * W + dl];
+            C[base + (size_t) i * Mx].y /= A[(size_t) i * W + dl];
+        }
+    };
+    for (int v = 0; v < 2; ++v) {
+        int slot = v == 0 ? 0 : 2;
+        for (int i = 0; i < nR; ++i) {
+            double sr = 0, si = 0;
+            for (int j = 0; j < q; ++j) {
+                sr += c.Rw[(size_t) i * q + j] * U[ui(0, slot, j, 0)].x;
+                si += c.Rw[(size_t) i * q + j] * U[ui(0, slot, j, 0)].y;
+            }
+            C[ci(0, v, i, 0)] = make_double2(sr, si);
+        }
+        subst(Ms, nR, 2, ci(0, v, 0, 0));
+    }
+    for (int z = 0; z < d.Nz; ++z) {
+        int x0 = z == 0 ? 1 : 0;
+        double kz = c.Kz[z];
+        for (int x = x0; x < Mx; ++x) {
+            double kx = c.Kx[x];
+            for (int i = 0; i < nQ; ++i) {
+                double sr = 0, si = 0;
+                for (int j = 0; j < q; ++j) {
+                    sr += c.Qw[(size_t) i * q + j] * U[ui(z, 1, j, x)].x;
+                    si += c.Qw[(size_t) i * q + j] * U[ui(z, 1, j, x)].y;
+                }
+                C[ci(z, 0, i, x)] = make_double2(sr, si);
+            }
+            subst(Mq, nQ, 4, ci(z, 0, 0, x));
+            for (int i = 0; i < nR; ++i) {
+                double sr = 0, si = 0;
+                for (int j = 0; j < q; ++j) {
+                    cplx u = U[ui(z, 0, j, x)], w = U[ui(z, 2, j, x)];
+                    sr += c.Rw[(size_t) i * q + j] * (kx * w.y - kz * u.y);
+                    si += c.Rw[(size_t) i * q + j] * (kz * u.x - kx * w.x);
+                }
+                C[ci(z, 1, i, x)] = make_double2(sr, si);
+            }
+            subst(Ms, nR, 2, ci(z, 1, 0, x));
+        }
+    }
+}
+
+static int stream_sync(Context &c)
+{
+    SC_HIP(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+}  // namespace sc
+
+using namespace sc;
+
+extern "C" {
+
+const char *sc_last_error(void) { return last_error().c_str(); }
+int sc_last_status(void) { return ctx().status; }
+void sc_srand(unsigned seed) { glibc_srand(seed); }
+
+int sc_prof_enable(int kernel_class)
+{
+    Context &c = ctx();
+    c.prof_class = kernel_class;
+    c.ev_used = 0;
+    c.prof_bytes = 0.0;
+    return 0;
+}
+
+double sc_prof_bytes(void) { return ctx().prof_bytes; }
+
+int sc_synchronize(void)
+{
+    Context &c = ctx();
+    if (!c.stream) return SC_ENOTREADY;
+    SC_HIP(hipStreamSynchronize(c.stream));
+    return SC_OK;
+}
+
+int sc_prof_read(double *total_ms)
+{
+    Context &c = ctx();
+    if (c.stream) (void) hipStreamSynchronize(c.stream);
+    double tot = 0.0;
+    int n = 0;
+    for (size_t i = 0; i + 1 < c.ev_used; i += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c.ev_pool[i], c.ev_pool[i + 1]) == hipSuccess) {
+            tot += ms;
+            ++n;
+        }
+    }
+    c.ev_used = 0;
+    if (total_ms) *total_ms = tot;
+    return n;
+}
+
+void c_destroy(void)
+{
+    Context &c = ctx();
+    if (c.stream) (void) hipStreamSynchronize(c.stream);
+    free_all(c);
+}
+
+int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux, double dt,
+           int nstep)
+{
+    Context &c = ctx();
+    if (c.ok || c.stream) c_destroy();
+    printf("Nx,Ny,Nz,Lx  |  Lz,dt,nsteps  |  flux,Re\n%d %d %d %f  |  %f %f %d |  %f %f\n", Nx,
+           Ny, Nz, Lx, Lz, dt, nstep, flux, Re);
+    if (Nx % 4 != 0 || Nz % 2 != 0 || Ny - 4 < 0 || Nx <= 0 || Nz <= 0 || nstep < 0) {
+        last_error() = "grid requires Nx % 4 == 0, Nz % 2 == 0, Ny >= 4, nsteps >= 0";
+        return SC_EINIT;
+    }
+    Dims &d = c.d;
+    d.Nx = Nx; d.Ny = Ny; d.Nz = Nz; d.Lx = Lx; d.Lz = Lz; d.dt = dt; d.nsteps = nstep;
+    d.flux = flux;
+    d.re = 1. / Re;
+    d.qpts = 3 * Ny / 2;
+    d.dimR = Ny - 2;
+    d.dimQ = Ny - 4;
+    d.Mx = Nx / 2;
+    d.Px = 3 * Nx / 2;
+    d.Pz = 3 * Nz / 2;
+    if (d.dimR > 512 || 6 * d.Px > 3584 || d.Pz > 4096) {
+        last_error() = "grid too large: need Ny <= 514 and Nx <= 398";
+        return SC_EINIT;
+    }
+    if (hipStreamCreate(&c.stream) != hipSuccess) {
+        last_error() = "no HIP device";
+        return SC_EINIT;
+    }
+    if (setup_host(c) != 0) {
+        free_all(c);
+        return SC_EINIT;
+    }
+    c.csize = (size_t) d.Nz * 2 * d.dimR * d.Mx;
+    c.usize = (size_t) d.Nz * 5 * d.qpts * d.Mx;
+    const size_t cb = sizeof(cplx) * c.csize, ub = sizeof(cplx) * c.usize;
+    const size_t fb = sizeof(cplx) * (size_t) d.Nz * d.dimR * d.Mx;
+    const size_t mcb = cb * (3 * (size_t) nstep + 1);
+    struct { void **p; size_t n; } al[] = {
+        {(void **) &c.C, cb}, {(void **) &c.IC, cb}, {(void **) &c.AC, cb},
+        {(void **) &c.Csol, cb}, {(void **) &c.Ctmp, cb}, {(void **) &c.Ctmp2, cb},
+        {(void **) &c.U, ub}, {(void **) &c.IU, ub}, {(void **) &c.AU, ub},
+        {(void **) &c.LU, ub}, {(void **) &c.Ut, ub}, {(void **) &c.Ut2, ub},
+        {(void **) &c.MC, mcb}, {(void **) &c.MIC, mcb},
+        {(void **) &c.Fa, fb}, {(void **) &c.Fb, fb}, {(void **) &c.FLa, fb}, {(void **) &c.FLb, fb},
+        {(void **) &c.W1, sizeof(cplx) * (size_t) d.qpts * MAXF * d.Pz * d.Mx},
+        {(void **) &c.W2, sizeof(cplx) * (size_t) d.qpts * 6 * d.Pz * d.Mx},
+        {(void **) &c.SP, sizeof(cplx) * 6 * (size_t) d.Nz * d.qpts * d.Mx},
+        {(void **) &c.flowdev, sizeof(double) * 3 * (size_t) d.Px * d.qpts * d.Pz},
+        {(void **) &c.cflmax, sizeof(double) * d.qpts},
+        {(void **) &c.dotpart, sizeof(double) * 2 * (size_t) d.qpts * d.Pz},
+        {(void **) &c.scal, sizeof(double) * 8},
+        {(void **) &c.abort_flag, sizeof(int)},
+    };
+    for (auto &e : al) {
+        if (hipMalloc(e.p, e.n) != hipSuccess || hipMemset(*e.p, 0, e.n) != hipSuccess) {
+            last_error() = "device allocation failed";
+            free_all(c);
+            return SC_EINIT;
+        }
+    }
+    c.views.assign(nstep + 1, {});
+    c.iviews.assign(nstep + 1, {});
+    c.ok = true;
+    return SC_OK;
+}
+
+int c_primal(int ru_steps, sc_complex *C_given, int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_dup)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    const Dims &d = c.d;
+    bool given = C_given && !(Nz_dup == 0 && Nvar_dup == 0 && Ny_dup == 0 && Nx_dup == 0);
+    if (given && !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup)) return SC_EINVAL;
+    if (ru_steps < 0) return SC_EINVAL;
+    const size_t cb = sizeof(cplx) * c.csize;
+    if (given) {
+        SC_HIP(hipMemcpyAsync(c.C, C_given, cb, hipMemcpyHostToDevice, c.stream));
+        launch_rebuild(c, c.C, c.U);
+    } else {
+        // laminar Poiseuille + rand() perturbation at Gauss points; C by
+        // projection, U left as set (channel/channel.c:409-422)
+        std::vector<cplx> Uh(c.usize, make_double2(0.0, 0.0)), Ch(c.csize);
+        SC_HIP(hipMemcpy(Ch.data(), c.C, cb, hipMemcpyDeviceToHost));
+        const int q = d.qpts, Mx = d.Mx;
+        for (int i = 0; i < q; ++i)
+            Uh[(size_t) i * Mx] = make_double2((1.0 - c.y[i] * c.y[i]) * d.flux * 3. / 4, 0.0);
+        for (int i = 0; i < q; ++i) {
+            double re = glibc_rand() / (double) 2147483647 - 0.5;
+            double im = glibc_rand() / (double) 2147483647 - 0.5;
+            if (d.Nz > 1) Uh[(((size_t) 1 * 5 + 0) * q + i) * Mx + 1] = make_double2(re, im);
+        }
+        host_gauss_to_coef(c, Uh, Ch);
+        SC_HIP(hipMemcpy(c.C, Ch.data(), cb, hipMemcpyHostToDevice));
+        SC_HIP(hipMemcpy(c.U, Uh.data(), sizeof(cplx) * c.usize, hipMemcpyHostToDevice));
+    }
+    SC_HIP(hipMemsetAsync(c.MC, 0, cb * (3 * (size_t) d.nsteps + 1), c.stream));
+    SC_HIP(hipMemcpyAsync(c.MC, c.C, cb, hipMemcpyDeviceToDevice, c.stream));
+    SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
+    c.status = 0;
+    const int total = d.nsteps + ru_steps;
+    for (int n = 0; n < total; ++n) {
+        for (int k = 0; k < 3; ++k) {
+            pass1_state(c, c.U, k == 0);
+            long count = (long) (n - ru_steps) * 3 + k + 1;
+            cplx *Cs = count >= 0 ? mc(c, count) : c.Csol;
+            launch_force(c, c.U, 0);
+            launch_solve(c, c.C, Cs, k, 0, c.Fa, c.Fb);
+            launch_mean(c, c.C, Cs, c.U, nullptr, c.U, k, 0, d.flux);
+            launch_rebuild(c, Cs, c.U, false);
+        }
+        if ((n & 63) == 63) {
+            int flag = 0;
+            SC_HIP(hipMemcpy(&flag, c.abort_flag, sizeof(int), hipMemcpyDeviceToHost));
+            if (flag) break;
+        }
+    }
+    int rc = stream_sync(c);
+    if (rc) return rc;
+    check_abort(c, "Pass1");
+    if (given) SC_HIP(hipMemcpy(C_given, c.C, cb, hipMemcpyDeviceToHost));
+    return SC_OK;
+}
+
+int c_tangent(int start, int end, sc_complex *IC_given, int Nz_dup, int Nvar_dup, int Ny_dup,
+              int Nx_dup, int inhomo)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    const Dims &d = c.d;
+    if (!IC_given || !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup)) return SC_EINVAL;
+    if (!(end > start) || start < 0 || end > d.nsteps) {
+        last_error() = "tangent requires 0 <= start_step < end_step <= nsteps";
+        return SC_EINVAL;
+    }
+    const size_t cb = sizeof(cplx) * c.csize;
+    SC_HIP(hipMemcpyAsync(c.IC, IC_given, cb, hipMemcpyHostToDevice, c.stream));
+    SC_HIP(hipMemcpyAsync(mic(c, 3L * start), c.IC, cb, hipMemcpyDeviceToDevice, c.stream));
+    SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
+    launch_rebuild(c, c.IC, c.IU);
+    c.status = 0;
+    const double target = inhomo ? 2.0 : 0.0;
+    for (int n = start; n < end; ++n) {
+        for (int k = 0; k < 3; ++k) {
+            launch_rebuild(c, mc(c, 3L * n + k), c.U);
+            pass1_tangent(c, c.U, c.IU, k == 0);
+            long count = 3L * n + k + 1;
+            cplx *Cs = mic(c, count);
+            launch_force(c, c.IU, 0);
+            launch_solve(c, c.IC, Cs, k, 0, c.Fa, c.Fb);
+            launch_mean(c, c.IC, Cs, c.IU, nullptr, c.IU, k, 1, target);
+            launch_rebuild(c, Cs, c.IU, false);
+        }
+        if (((n - start) & 63) == 63) {
+            int flag = 0;
+            SC_HIP(hipMemcpy(&flag, c.abort_flag, sizeof(int), hipMemcpyDeviceToHost));
+            if (flag) break;
+        }
+    }
+    int rc = stream_sync(c);
+    if (rc) return rc;
+    check_abort(c, "Pass1");
+    SC_HIP(hipMemcpy(IC_given, c.IC, cb, hipMemcpyDeviceToHost));
+    return SC_OK;
+}
+
+int c_adjoint(int start, int end, sc_complex *AC_given, int Nz_dup, int Nvar_dup, int Ny_dup,
+              int Nx_dup, int inhomo, double strength)
+{
+    (void) inhomo;   // unused by the reference as well (channel/channel.c:596-696)
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    const Dims &d = c.d;
+    if (!AC_given || !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup)) return SC_EINVAL;
+    if (!(end < start) || end < 0 || start > d.nsteps) {
+        last_error() = "adjoint requires 0 <= end_step < start_step <= nsteps";
+        return SC_EINVAL;
+    }
+    const size_t cb = sizeof(cplx) * c.csize;
+    SC_HIP(hipMemcpyAsync(c.AC, AC_given, cb, hipMemcpyHostToDevice, c.stream));
+    SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
+    launch_rebuild(c, c.AC, c.AU);
+    const double s = strength * d.dt;
+    for (int n = start; n > end; --n) {
+        SC_HIP(hipMemcpyAsync(c.Ctmp, mic(c, 3L * n), cb, hipMemcpyDeviceToDevice, c.stream));
+        ddt_project_dev(c, n, c.Ctmp);
+        launch_axpy_c(c, c.AC, c.Ctmp, 0.5 * s, c.csize);
+        for (int k = 0; k < 3; ++k) {
+            long count = 3L * n - k - 1;
+            if (k < 2) {
+                launch_rebuild(c, mc(c, count - 1), c.U);
+                pass2_adjoint(c, c.U, c.AU, c.LU);
+            }
+            launch_rebuild(c, mc(c, count), c.U);
+            pass2_adjoint(c, c.U, c.AU, c.U);
+            launch_force(c, c.U, 1);
+            if (k < 2) launch_force(c, c.LU, 1, c.FLa, c.FLb);
+            launch_solve(c, c.AC, c.Csol, k, 1, c.FLa, c.FLb);
+            launch_mean(c, c.AC, c.Csol, c.U, c.LU, c.AU, k, 2, 0.0);
+            launch_rebuild(c, c.Csol, c.AU, false);
+        }
+        SC_HIP(hipMemcpyAsync(c.Ctmp, mic(c, 3L * (n - 1)), cb, hipMemcpyDeviceToDevice, c.stream));
+        ddt_project_dev(c, n - 1, c.Ctmp);
+        launch_axpy_c(c, c.AC, c.Ctmp, 0.5 * s, c.csize);
+    }
+    int rc = stream_sync(c);
+    if (rc) return rc;
+    SC_HIP(hipMemcpy(AC_given, c.AC, cb, hipMemcpyDeviceToHost));
+    return SC_OK;
+}
+
+double c_ddt_project(int i_step, sc_complex *IC_given, int Nz_dup, int Nvar_dup, int Ny_dup,
+                     int Nx_dup)
+{
+    Context &c = ctx();
+    if (!c.ok || !IC_given || !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup) || i_step < 0 ||
+        i_step > c.d.nsteps || c.d.nsteps < 1)
+        return NAN;
+    const size_t cb = sizeof(cplx) * c.csize;
+    if (hipMemcpyAsync(c.Ctmp, IC_given, cb, hipMemcpyHostToDevice, c.stream) != hipSuccess) return NAN;
+    SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream)) ;
+    ddt_project_dev(c, i_step, c.Ctmp);
+    double scal[3] = {0, 0, 0};
+    if (hipMemcpyAsync(IC_given, c.Ctmp, cb, hipMemcpyDeviceToHost, c.stream) != hipSuccess) return NAN;
+    if (hipMemcpyAsync(scal, c.scal, sizeof(scal), hipMemcpyDeviceToHost, c.stream) != hipSuccess) return NAN;
+    if (hipStreamSynchronize(c.stream) != hipSuccess) return NAN;
+    return scal[2];
+}
+
+static int get_view(int i_step, bool incre, sc_complex **ptr, int *Nz, int *Nvar, int *Ny, int *Nx)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (i_step < 0 || i_step > c.d.nsteps) {
+        last_error() = "step index out of range";
+        return SC_EINVAL;
+    }
+    auto &v = incre ? c.iviews[i_step] : c.views[i_step];
+    v.resize(c.csize);
+    const cplx *src = incre ? mic(c, 3L * i_step) : mc(c, 3L * i_step);
+    SC_HIP(hipStreamSynchronize(c.stream));
+    SC_HIP(hipMemcpy(v.data(), src, sizeof(cplx) * c.csize, hipMemcpyDeviceToHost));
+    *ptr = (sc_complex *) v.data();
+    *Nz = c.d.Nz;
+    *Nvar = 2;
+    *Ny = c.d.dimR;
+    *Nx = c.d.Mx;
+    return SC_OK;
+}
+
+int c_getsoln(int i_step, sc_complex **MC_ptr, int *Nz_ptr, int *Nvar_ptr, int *Ny_ptr, int *Nx_ptr)
+{
+    return get_view(i_step, false, MC_ptr, Nz_ptr, Nvar_ptr, Ny_ptr, Nx_ptr);
+}
+
+int c_getincresoln(int i_step, sc_complex **MIC_ptr, int *Nz_ptr, int *Nvar_ptr, int *Ny_ptr,
+                   int *Nx_ptr)
+{
+    return get_view(i_step, true, MIC_ptr, Nz_ptr, Nvar_ptr, Ny_ptr, Nx_ptr);
+}
+
+int c_spec2phys(sc_complex *C_ptr, int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_dup,
+                double *flow_ptr, int Nd_flow, int Nx_flow, int Ny_flow, int Nz_flow)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    const Dims &d = c.d;
+    if (!C_ptr || !flow_ptr || !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup) || Nd_flow != 3 ||
+        Nx_flow != d.Px || Ny_flow != d.qpts || Nz_flow != d.Pz) {
+        last_error() = "spec2phys: flow must be (3, 3Nx/2, qpts, 3Nz/2)";
+        return SC_EINVAL;
+    }
+    SC_HIP(hipMemcpyAsync(c.Ctmp, C_ptr, sizeof(cplx) * c.csize, hipMemcpyHostToDevice, c.stream));
+    SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
+    launch_rebuild(c, c.Ctmp, c.Ut);
+    FieldSpec fs;
+    fs.nf = 3;
+    for (int s = 0; s < 3; ++s) {
+        fs.src[s] = c.Ut;
+        fs.kind[s] = FK_SLOT;
+        fs.slot[s] = s;
+    }
+    launch_zinv(c, fs, c.W1);
+    launch_xmix(c, XM_PHYS, c.W1, nullptr, 0, c.flowdev, nullptr);
+    SC_HIP(hipMemcpyAsync(flow_ptr, c.flowdev, sizeof(double) * 3 * (size_t) d.Px * d.qpts * d.Pz,
+                          hipMemcpyDeviceToHost, c.stream));
+    SC_HIP(hipStreamSynchronize(c.stream));
+    return SC_OK;
+}
+
+// Container for save/read_solution: magic "SCSOL001", int32 Nx, Ny, Nz, dimR,
+// float64 dt, Re, mpg, then alpha, beta, ialpha, ibeta as [dimR][Nz][Nx/2]
+// complex128 (the dataset layout of channel/channel.c:889-1028).
+static const char MAGIC[8] = {'S', 'C', 'S', 'O', 'L', '0', '0', '1'};
+
+int c_save_solution(const char *filename, sc_complex *C_ptr, int Nz_dup, int Nvar_dup, int Ny_dup,
+                    int Nx_dup)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!filename || !C_ptr || !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup)) return SC_EINVAL;
+    const Dims &d = c.d;
+    std::vector<cplx> IC(c.csize);
+    SC_HIP(hipMemcpy(IC.data(), c.IC, sizeof(cplx) * c.csize, hipMemcpyDeviceToHost));
+    std::ofstream f(filename, std::ios::binary);
+    if (!f) return SC_EIO;
+    int32_t hdr[4] = {d.Nx, d.Ny, d.Nz, d.dimR};
+    double par[3] = {d.dt, d.re, 0.0};
+    f.write(MAGIC, 8);
+    f.write((const char *) hdr, sizeof(hdr));
+    f.write((const char *) par, sizeof(par));
+    const cplx *src[2] = {(const cplx *) C_ptr, IC.data()};
+    for (int a = 0; a < 2; ++a)
+        for (int v = 0; v < 2; ++v)
+            for (int y = 0; y < d.dimR; ++y)
+                for (int z = 0; z < d.Nz; ++z)
+                    f.write((const char *) (src[a] + (((size_t) z * 2 + v) * d.dimR + y) * d.Mx),
+                            sizeof(cplx) * d.Mx);
+    return f ? SC_OK : SC_EIO;
+}
+
+int c_read_solution(const char *filename, sc_complex *C_ptr, int Nz_dup, int Nvar_dup, int Ny_dup,
+                    int Nx_dup)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!filename || !C_ptr || !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup)) return SC_EINVAL;
+    const Dims &d = c.d;
+    std::ifstream f(filename, std::ios::binary);
+    char magic[8];
+    int32_t hdr[4];
+    double par[3];
+    if (!f.read(magic, 8) || memcmp(magic, MAGIC, 8) != 0 || !f.read((char *) hdr, sizeof(hdr)) ||
+        !f.read((char *) par, sizeof(par))) {
+        last_error() = "not a solution file";
+        return SC_EIO;
+    }
+    if (hdr[0] != d.Nx || hdr[2] != d.Nz || hdr[3] != d.dimR) {
+        last_error() = "solution file grid does not match c_init";
+        return SC_EIO;
+    }
+    std::vector<cplx> IC(c.csize);
+    cplx *dst[2] = {(cplx *) C_ptr, IC.data()};
+    for (int a = 0; a < 2; ++a)
+        for (int v = 0; v < 2; ++v)
+            for (int y = 0; y < d.dimR; ++y)
+                for (int z = 0; z < d.Nz; ++z)
+                    if (!f.read((char *) (dst[a] + (((size_t) z * 2 + v) * d.dimR + y) * d.Mx),
+                                sizeof(cplx) * d.Mx))
+                        return SC_EIO;
+    SC_HIP(hipMemcpy(c.C, C_ptr, sizeof(cplx) * c.csize, hipMemcpyHostToDevice));
+    SC_HIP(hipMemcpy(c.IC, IC.data(), sizeof(cplx) * c.csize, hipMemcpyHostToDevice));
+    return SC_OK;
+}
+
+int c_quad(double **y_ptr, int *qpts_ptr1, double **w_ptr, int *qpts_ptr2)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    *y_ptr = c.y.data();
+    *w_ptr = c.w.data();
+    *qpts_ptr1 = *qpts_ptr2 = c.d.qpts;
+    return SC_OK;
+}
+
+int c_qpts(void) { return ctx().d.qpts; }
+int c_dimR(void) { return ctx().d.dimR; }
+int c_dimQ(void) { return ctx().d.dimQ; }
+int c_Nx(void) { return ctx().d.Nx; }
+int c_Nz(void) { return ctx().d.Nz; }
+int c_nsteps(void) { return ctx().d.nsteps; }
+
+}  // extern "C"

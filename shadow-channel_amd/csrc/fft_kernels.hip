@@ -1,0 +1,585 @@
+// Dealiased x/z transforms of the nonlinear terms, one launch per direction.
+//
+// The reference does, per Gauss plane y and field, an unnormalised backward
+// c2c along z over the Nx/2 stored columns of a 3/2-padded array, then a c2r
+// along x, the pointwise products, and the reverse with a 1/(Px Pz) scale and
+// truncation (channel/pass1.c:107-114, 312-371; channel/pass2.c:128-221,
+// 417-472; channel/spec2phys.c:67-76).  Here the three phases are separate
+// batched kernels over all planes at once:
+//   zinv : (y, field, kx block) -> length-Pz inverse transforms in LDS, with
+//          the spectral curl fused into the load and the zero padding implied
+//   xmix : (y, block of z rows) -> pairs of real rows packed into one complex
+//          length-Px inverse transform, products, CFL maximum, forward
+//          transforms of packed product pairs, unpacking of the kx < Nx/2 bins
+//   zfwd : (y, field, kx block) -> forward length-Pz transforms, scale,
+//          truncation to the stored kz rows
+// The FFTs are Stockham passes of radix 4, 2, 3, 5 run in place in LDS: every
+// thread holds its butterflies' inputs in registers across a barrier.
+#include "sc_internal.h"
+
+namespace sc {
+
+constexpr int BLK = 256;
+// complex values staged in LDS by one x-kernel block (keeps static + dynamic
+// LDS of that kernel under 64 KiB)
+constexpr int XMIX_TILE = 3584;
+
+struct PlanDev {
+    int n, nf;
+    int f[MAX_FACTORS];
+    const double2 *tw;
+};
+
+static PlanDev to_dev(const FftPlan &p)
+{
+    PlanDev d;
+    d.n = p.n;
+    d.nf = p.nf;
+    for (int i = 0; i < MAX_FACTORS; ++i) d.f[i] = i < p.nf ? p.f[i] : 0;
+    d.tw = p.tw;
+    return d;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// in-place radix-r DFT of v[0..r-1], exp(sign 2 pi i q s / r)
+__device__ __forceinline__ void small_dft(double2 *v, int r, double sign)
+{
+    if (r == 2) {
+        double2 a = v[0], b = v[1];
+        v[0] = make_double2(a.x + b.x, a.y + b.y);
+        v[1] = make_double2(a.x - b.x, a.y - b.y);
+    } else if (r == 4) {
+        double2 a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
+        double2 s02 = make_double2(a0.x + a2.x, a0.y + a2.y);
+        double2 d02 = make_double2(a0.x - a2.x, a0.y - a2.y);
+        double2 s13 = make_double2(a1.x + a3.x, a1.y + a3.y);
+        double2 d13 = make_double2(a1.x - a3.x, a1.y - a3.y);
+        // w d13 with w = sign * i
+        double2 wd = make_double2(-sign * d13.y, sign * d13.x);
+        v[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
+        v[1] = make_double2(d02.x + wd.x, d02.y + wd.y);
+        v[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
+        v[3] = make_double2(d02.x - wd.x, d02.y - wd.y);
+    } else if (r == 3) {
+        const double h = 0.86602540378443864676;   // sqrt(3)/2
+        double2 a0 = v[0], a1 = v[1], a2 = v[2];
+        double2 t1 = make_double2(a1.x + a2.x, a1.y + a2.y);
+        double2 t2 = make_double2(a0.x - 0.5 * t1.x, a0.y - 0.5 * t1.y);
+        double2 t3 = make_double2(sign * h * (a1.x - a2.x), sign * h * (a1.y - a2.y));
+        v[0] = make_double2(a0.x + t1.x, a0.y + t1.y);
+        v[1] = make_double2(t2.x - t3.y, t2.y + t3.x);
+        v[2] = make_double2(t2.x + t3.y, t2.y - t3.x);
+    } else if (r == 5) {
+        const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+        const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+        double2 a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3], a4 = v[4];
+        double2 b1 = make_double2(a1.x + a4.x, a1.y + a4.y);
+        double2 b2 = make_double2(a2.x + a3.x, a2.y + a3.y);
+        double2 d1 = make_double2(a1.x - a4.x, a1.y - a4.y);
+        double2 d2 = make_double2(a2.x - a3.x, a2.y - a3.y);
+        double2 t1 = make_double2(a0.x + c1 * b1.x + c2 * b2.x, a0.y + c1 * b1.y + c2 * b2.y);
+        double2 t2 = make_double2(a0.x + c2 * b1.x + c1 * b2.x, a0.y + c2 * b1.y + c1 * b2.y);
+        double2 u1 = make_double2(sign * (s1 * d1.x + s2 * d2.x), sign * (s1 * d1.y + s2 * d2.y));
+        double2 u2 = make_double2(sign * (s2 * d1.x - s1 * d2.x), sign * (s2 * d1.y - s1 * d2.y));
+        v[0] = make_double2(a0.x + b1.x + b2.x, a0.y + b1.y + b2.y);
+        v[1] = make_double2(t1.x - u1.y, t1.y + u1.x);
+        v[4] = make_double2(t1.x + u1.y, t1.y - u1.x);
+        v[2] = make_double2(t2.x - u2.y, t2.y + u2.x);
+        v[3] = make_double2(t2.x + u2.y, t2.y - u2.x);
+    }
+}
+
+// One Stockham pass of radix R over nseq sequences of length n (in place in
+// LDS).  Each thread keeps its butterflies' inputs in registers across the
+// barrier; a block of 256 threads holds at most 4096 complex values, so a
+// thread owns at most ceil(16/R) butterflies.
+template <int R>
+__device__ __forceinline__ void fft_pass(double2 *data, int nseq, int stride, int n, int Ns,
+                                         const double2 *tw, double sign)
+{
+    constexpr int NB = (16 + R - 1) / R;
+    const int nb = n / R;
+    const int total = nseq * nb;
+    const int twstep = n / (Ns * R);
+    double2 v[NB][R];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        int b = threadIdx.x + i * BLK;
+        if (b < total) {
+            int s = b / nb, j = b - s * nb;
+            const double2 *src = data + s * stride;
+            int k = j % Ns;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                double2 a = src[j + q * nb];
+                if (q > 0) {
+                    double2 w = tw[q * k * twstep];
+                    w.y *= sign;
+                    a = cmul(a, w);
+                }
+                v[i][q] = a;
+            }
+            small_dft(v[i], R, sign);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        int b = threadIdx.x + i * BLK;
+        if (b < total) {
+            int s = b / nb, j = b - s * nb;
+            double2 *dst = data + s * stride;
+            int k = j % Ns;
+            int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) dst[base + q * Ns] = v[i][q];
+        }
+    }
+    __syncthreads();
+}
+
+// In-place FFT of nseq sequences of length p.n, sequence s at data[s*stride],
+// nseq * p.n <= 4096.  All threads of the block must call it.
+// sign = +1: backward (exp(+i)), -1: forward.
+__device__ void lds_fft(double2 *data, int nseq, int stride, const PlanDev &p, double sign)
+{
+    int Ns = 1;
+    for (int pass = 0; pass < p.nf; ++pass) {
+        const int r = p.f[pass];
+        if (r == 4) fft_pass<4>(data, nseq, stride, p.n, Ns, p.tw, sign);
+        else if (r == 2) fft_pass<2>(data, nseq, stride, p.n, Ns, p.tw, sign);
+        else if (r == 3) fft_pass<3>(data, nseq, stride, p.n, Ns, p.tw, sign);
+        else fft_pass<5>(data, nseq, stride, p.n, Ns, p.tw, sign);
+        Ns *= r;
+    }
+}
+
+struct ZinvArgs {
+    FieldSpec fs;
+    const double *Kx, *Kz;
+    int Nz, Mx, qpts, Pz, XB;
+    const int *abort_flag;
+};
+
+// grid (ceil(Mx/XB), nf, qpts)
+__global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, cplx *W1)
+{
+    extern __shared__ double2 lds[];
+    if (*a.abort_flag) return;
+    const int x0 = blockIdx.x * a.XB, f = blockIdx.y, y = blockIdx.z;
+    const int XB = min(a.XB, a.Mx - x0);
+    const int Nz = a.Nz, Pz = a.Pz, Mx = a.Mx;
+    const cplx *src = a.fs.src[f];
+    const int kind = a.fs.kind[f], slot = a.fs.slot[f];
+    const size_t zs = (size_t) 5 * a.qpts * Mx;        // stride of kz in U
+    const size_t ss = (size_t) a.qpts * Mx;            // stride of slot in U
+    for (int e = threadIdx.x; e < XB * Pz; e += blockDim.x) {
+        int zp = e / XB, xl = e - zp * XB, x = x0 + xl;
+        int kz = zp < Nz / 2 ? zp : (zp > Nz ? zp - Nz / 2 : -1);
+        double2 val = make_double2(0.0, 0.0);
+        if (kz >= 0) {
+            const cplx *b = src + kz * zs + (size_t) y * Mx + x;
+            double kx = a.Kx[x], kzv = a.Kz[kz];
+            if (kind == FK_SLOT) {
+                val = b[slot * ss];
+            } else if (kind == FK_OMEGA_Z) {           // -du/dy + i kx v
+                double2 du = b[3 * ss], v = b[1 * ss];
+                val = make_double2(-(du.x + kx * v.y), -du.y + kx * v.x);
+            } else if (kind == FK_OMEGA_X) {           // dw/dy - i kz v
+                double2 dw = b[4 * ss], v = b[1 * ss];
+                val = make_double2(dw.x + kzv * v.y, dw.y - kzv * v.x);
+            } else {                                   // i kz u - i kx w
+                double2 u = b[0], w = b[2 * ss];
+                val = make_double2(kx * w.y - kzv * u.y, kzv * u.x - kx * w.x);
+            }
+        }
+        lds[xl * Pz + zp] = val;
+    }
+    __syncthreads();
+    lds_fft(lds, XB, Pz, pz, +1.0);
+    cplx *dst = W1 + (((size_t) y * gridDim.y + f) * Pz) * Mx;
+    for (int e = threadIdx.x; e < XB * Pz; e += blockDim.x) {
+        int zp = e / XB, xl = e - zp * XB;
+        dst[(size_t) zp * Mx + x0 + xl] = lds[xl * Pz + zp];
+    }
+}
+
+struct XmixArgs {
+    int mode, nf, nout, Mx, Px, Pz, qpts, ZB, check_cfl;
+    double cfl1, cfl3;
+    const double *cfl2, *W;
+    double *cflmax;        // [qpts], as ordered bit patterns
+    double *flow;          // XM_PHYS output
+    double *dotpart;       // XM_DOT partials [qpts][nzb][2]
+    const int *abort_flag;
+};
+
+__device__ __forceinline__ void atomic_max_nonneg(double *addr, double v)
+{
+    atomicMax((unsigned long long *) addr, (unsigned long long) __double_as_longlong(v));
+}
+
+template <int MODE> struct XmixShape;
+template <> struct XmixShape<XM_PRIMAL> { static constexpr int NF = 6, NOUT = 3; };
+template <> struct XmixShape<XM_TANGENT> { static constexpr int NF = 12, NOUT = 3; };
+template <> struct XmixShape<XM_ADJOINT> { static constexpr int NF = 9, NOUT = 6; };
+template <> struct XmixShape<XM_PHYS> { static constexpr int NF = 3, NOUT = 0; };
+template <> struct XmixShape<XM_DOT> { static constexpr int NF = 6, NOUT = 0; };
+
+// grid (Pz/ZB, qpts); LDS: ZB * nseq * Px complex, nseq = ceil(nf/2)
+template <int MODE>
+__global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx *W1, cplx *W2)
+{
+    extern __shared__ double2 lds[];
+    __shared__ double red[2][BLK];
+    if (*a.abort_flag) return;
+    constexpr int nf = XmixShape<MODE>::NF;
+    constexpr int NOUT = XmixShape<MODE>::NOUT;
+    constexpr int nseq = (nf + 1) / 2;
+    const int zb = blockIdx.x, y = blockIdx.y;
+    const int Px = a.Px, Mx = a.Mx, Pz = a.Pz, ZB = a.ZB;
+    const int z0 = zb * ZB;
+    // load packed pairs with Hermitian extension (c2r uses Re only at kx=0)
+    const int tot = ZB * nseq * Px;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        int n = e % Px, t = e / Px, s = t % nseq, r = t / nseq;
+        int zp = z0 + r;
+        const cplx *X = W1 + (((size_t) y * nf + 2 * s) * Pz + zp) * Mx;
+        const cplx *Y = (2 * s + 1 < nf) ? X + (size_t) Pz * Mx : nullptr;
+        double2 val = make_double2(0.0, 0.0);
+        if (n == 0) {
+            val.x = X[0].x;
+            val.y = Y ? Y[0].x : 0.0;
+        } else if (n < Mx) {
+            double2 xv = X[n], yv = Y ? Y[n] : make_double2(0.0, 0.0);
+            val = make_double2(xv.x - yv.y, xv.y + yv.x);
+        } else if (n > Px - Mx) {
+            int k = Px - n;
+            double2 xv = X[k], yv = Y ? Y[k] : make_double2(0.0, 0.0);
+            val = make_double2(xv.x + yv.y, -xv.y + yv.x);
+        }
+        lds[(r * nseq + s) * Px + n] = val;
+    }
+    __syncthreads();
+    lds_fft(lds, ZB * nseq, Px, px, +1.0);
+    // pointwise stage: read all fields of a point, compute, then (after a
+    // barrier) write the packed outputs compactly: row r, pair t at sequence
+    // r*nso + t
+    constexpr int nso = (NOUT + 1) / 2;
+    constexpr int MAXP = (XMIX_TILE / nseq + BLK - 1) / BLK;   // points per thread
+    constexpr int NO = NOUT > 0 ? NOUT : 1;
+    double outv[MAXP][NO];
+    double cmax = 0.0, acc0 = 0.0, acc1 = 0.0;
+    const int npt = ZB * Px;
+#pragma unroll
+    for (int i = 0; i < MAXP; ++i) {
+        int e = threadIdx.x + i * BLK;
+        if (e >= npt) break;
+        int n = e % Px, r = e / Px;
+        const double2 *row = lds + (size_t) r * nseq * Px + n;
+        double f[12];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            if (s < nseq) {
+                double2 v = row[s * Px];
+                f[2 * s] = v.x;
+                f[2 * s + 1] = v.y;
+            } else {
+                f[2 * s] = 0.0;
+                f[2 * s + 1] = 0.0;
+            }
+        }
+        double u = f[0], v = f[1], w = f[2], oz = f[3], ox = f[4], oy = f[5];
+        if (a.check_cfl) {
+            double t = fabs(u) * a.cfl1 + fabs(v) * a.cfl2[y] + fabs(w) * a.cfl3;
+            cmax = fmax(cmax, t);
+        }
+        if constexpr (MODE == XM_PRIMAL) {
+            outv[i][1] = v * oz - w * oy;           // Hx
+            outv[i][2] = w * ox - u * oz;           // Hy
+            outv[i][0] = u * oy - v * ox;           // Hz
+        } else if constexpr (MODE == XM_TANGENT) {
+            double Iu = f[6], Iv = f[7], Iw = f[8], Ioz = f[9], Iox = f[10], Ioy = f[11];
+            outv[i][1] = Iv * oz - Iw * oy + v * Ioz - w * Ioy;
+            outv[i][2] = Iw * ox - Iu * oz + w * Iox - u * Ioz;
+            outv[i][0] = Iu * oy - Iv * ox + u * Ioy - v * Iox;
+        } else if constexpr (MODE == XM_ADJOINT) {
+            double lx = f[6], ly = f[7], lz = f[8];
+            outv[i][0] = v * lz - w * ly;           // Sx
+            outv[i][1] = w * lx - u * lz;           // Sy
+            outv[i][2] = u * ly - v * lx;           // Sz
+            outv[i][3] = ly * oz - lz * oy;         // Tx
+            outv[i][4] = lz * ox - lx * oz;         // Ty
+            outv[i][5] = lx * oy - ly * ox;         // Tz
+        } else if constexpr (MODE == XM_PHYS) {
+            int zp = z0 + r;
+            size_t plane = (size_t) Px * a.qpts * Pz;
+            size_t o = ((size_t) n * a.qpts + y) * Pz + zp;
+            a.flow[o] = u;
+            a.flow[plane + o] = v;
+            a.flow[2 * plane + o] = w;
+        } else {   // XM_DOT: f[0..2] = du/dt, f[3..5] = v
+            acc0 += f[0] * f[3] + f[1] * f[4] + f[2] * f[5];
+            acc1 += f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+        }
+    }
+    if (a.check_cfl) {
+        red[0][threadIdx.x] = cmax;
+        __syncthreads();
+        for (int o = BLK / 2; o > 0; o >>= 1) {
+            if (threadIdx.x < o) red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) atomic_max_nonneg(&a.cflmax[y], red[0][0]);
+    }
+    if constexpr (MODE == XM_DOT) {
+        red[0][threadIdx.x] = acc0;
+        red[1][threadIdx.x] = acc1;
+        __syncthreads();
+        for (int o = BLK / 2; o > 0; o >>= 1) {
+            if (threadIdx.x < o) {
+                red[0][threadIdx.x] += red[0][threadIdx.x + o];
+                red[1][threadIdx.x] += red[1][threadIdx.x + o];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            size_t slot = ((size_t) y * gridDim.x + zb) * 2;
+            a.dotpart[slot] = red[0][0] * a.W[y];
+            a.dotpart[slot + 1] = red[1][0] * a.W[y];
+        }
+        return;
+    }
+    if constexpr (MODE == XM_PHYS) return;
+    if constexpr (NOUT > 0) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < MAXP; ++i) {
+        int e = threadIdx.x + i * BLK;
+        if (e >= npt) break;
+        int n = e % Px, r = e / Px;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            if (t < nso) {
+                double re = outv[i][2 * t < NO ? 2 * t : 0];
+                double im = (2 * t + 1 < NOUT) ? outv[i][2 * t + 1 < NO ? 2 * t + 1 : 0] : 0.0;
+                lds[(size_t) (r * nso + t) * Px + n] = make_double2(re, im);
+            }
+        }
+    }
+    __syncthreads();
+    lds_fft(lds, ZB * nso, Px, px, -1.0);
+    // unpack two real transforms from each packed pair, bins kx < Mx
+    const int tot2 = ZB * nso * Mx;
+    for (int e = threadIdx.x; e < tot2; e += blockDim.x) {
+        int k = e % Mx, t = (e / Mx) % nso, r = e / (Mx * nso);
+        const double2 *Z = lds + (size_t) (r * nso + t) * Px;
+        double2 zk = Z[k], zm = Z[k == 0 ? 0 : Px - k];
+        double2 X = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+        double2 Yv = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+        int zp = z0 + r;
+        int g0 = 2 * t, g1 = 2 * t + 1;
+        W2[(((size_t) y * NOUT + g0) * Pz + zp) * Mx + k] = X;
+        if (g1 < NOUT) W2[(((size_t) y * NOUT + g1) * Pz + zp) * Mx + k] = Yv;
+    }
+    }
+}
+
+struct ZfwdArgs {
+    int nout, Nz, Mx, Pz, qpts, XB, dst_kind;
+    double norm;
+    const int *abort_flag;
+};
+
+// grid (ceil(Mx/XB), nout, qpts)
+__global__ __launch_bounds__(BLK) void k_zfwd(ZfwdArgs a, PlanDev pz, const cplx *W2, cplx *dst)
+{
+    extern __shared__ double2 lds[];
+    if (*a.abort_flag) return;
+    const int x0 = blockIdx.x * a.XB, g = blockIdx.y, y = blockIdx.z;
+    const int XB = min(a.XB, a.Mx - x0);
+    const int Nz = a.Nz, Pz = a.Pz, Mx = a.Mx;
+    const cplx *src = W2 + (((size_t) y * a.nout + g) * Pz) * Mx;
+    for (int e = threadIdx.x; e < XB * Pz; e += blockDim.x) {
+        int zp = e / XB, xl = e - zp * XB;
+        lds[xl * Pz + zp] = src[(size_t) zp * Mx + x0 + xl];
+    }
+    __syncthreads();
+    lds_fft(lds, XB, Pz, pz, -1.0);
+    for (int e = threadIdx.x; e < XB * (Nz - 1); e += blockDim.x) {
+        int r = e / XB, xl = e - r * XB;
+        int kz = r < Nz / 2 ? r : r + 1;             // skip the Nyquist row
+        int zp = kz < Nz / 2 ? kz : kz + Nz / 2;
+        double2 v = lds[xl * Pz + zp];
+        v.x *= a.norm;
+        v.y *= a.norm;
+        size_t o;
+        if (a.dst_kind == ZF_USLOTS)
+            o = (((size_t) kz * 5 + g) * a.qpts + y) * Mx + x0 + xl;
+        else
+            o = (((size_t) g * Nz + kz) * a.qpts + y) * Mx + x0 + xl;
+        dst[o] = v;
+    }
+}
+
+// adjoint assembly (channel/pass2.c:474-566):
+// Hz = -Tz - i kx Sy, Hx = -Tx + i kz Sy, Hy = -Ty - i kz Sx + i kx Sz
+__global__ void k_assemble_adj(const cplx *SP, cplx *U, const double *Kx, const double *Kz,
+                               int Nz, int qpts, int Mx, const int *abort_flag)
+{
+    if (*abort_flag) return;
+    size_t n = (size_t) Nz * qpts * Mx;
+    for (size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x; e < n;
+         e += (size_t) gridDim.x * blockDim.x) {
+        int x = e % Mx;
+        int y = (e / Mx) % qpts;
+        int z = e / ((size_t) Mx * qpts);
+        if (z == Nz / 2) continue;
+        double kx = Kx[x], kz = Kz[z];
+        double2 Sx = SP[e], Sy = SP[n + e], Sz = SP[2 * n + e];
+        double2 Tx = SP[3 * n + e], Ty = SP[4 * n + e], Tz = SP[5 * n + e];
+        size_t b = ((size_t) z * 5 * qpts + y) * Mx + x, ss = (size_t) qpts * Mx;
+        U[b + 0 * ss] = make_double2(-Tz.x + kx * Sy.y, -Tz.y - kx * Sy.x);
+        U[b + 1 * ss] = make_double2(-Tx.x - kz * Sy.y, -Tx.y + kz * Sy.x);
+        U[b + 2 * ss] = make_double2(-Ty.x + kz * Sx.y - kx * Sz.y, -Ty.y - kz * Sx.x + kx * Sz.x);
+        U[b + 3 * ss] = Sx;
+        U[b + 4 * ss] = Sz;
+    }
+}
+
+// ---------------------------------------------------------------- launches
+int xmix_zblock(Context &c, int nf);
+
+static int pick_block(int Mx, int P, int nper)
+{
+    // columns per block so that the LDS tile stays <= 4096 complex
+    int xb = Mx;
+    while (xb > 1 && (size_t) xb * P * nper > 4096) xb = (xb + 1) / 2;
+    return xb;
+}
+
+void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1)
+{
+    const Dims &d = c.d;
+    ZinvArgs a;
+    a.fs = fs;
+    a.Kx = c.dKx;
+    a.Kz = c.dKz;
+    a.Nz = d.Nz;
+    a.Mx = d.Mx;
+    a.qpts = d.qpts;
+    a.Pz = d.Pz;
+    a.XB = pick_block(d.Mx, d.Pz, 1);
+    a.abort_flag = c.abort_flag;
+    dim3 grid((d.Mx + a.XB - 1) / a.XB, fs.nf, d.qpts);
+    size_t lds = sizeof(double2) * a.XB * d.Pz;
+    const double plane = 16.0 * (double) d.qpts * d.Mx;
+    int slots = 0;
+    for (int f = 0; f < fs.nf; ++f) {
+        bool seen = false;
+        for (int g = 0; g < f; ++g) seen |= fs.src[g] == fs.src[f];
+        if (!seen) slots += fs.nf >= 6 ? 5 : 3;
+    }
+    if (fs.nf == 9) slots = 8;            // adjoint: 5 state slots + 3 adjoint
+    if (fs.nf == 6 && fs.kind[3] == FK_SLOT) slots = 6;   // two velocity triples
+    double bytes = plane * ((double) slots * (d.Nz - 1) + (double) fs.nf * d.Pz);
+    prof_begin(c, c.stream, 1, bytes);
+    hipLaunchKernelGGL(k_zinv, grid, dim3(BLK), lds, c.stream, a, to_dev(c.pz), W1);
+    prof_end(c, c.stream, 1);
+}
+
+void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
+                 double *flow, double *dot_out)
+{
+    const Dims &d = c.d;
+    XmixArgs a;
+    a.mode = mode;
+    a.nf = mode == XM_PRIMAL ? 6 : mode == XM_TANGENT ? 12 : mode == XM_ADJOINT ? 9
+         : mode == XM_PHYS ? 3 : 6;
+    a.nout = mode == XM_ADJOINT ? 6 : (mode == XM_PRIMAL || mode == XM_TANGENT) ? 3 : 0;
+    a.Mx = d.Mx;
+    a.Px = d.Px;
+    a.Pz = d.Pz;
+    a.qpts = d.qpts;
+    int nseq = (a.nf + 1) / 2;
+    int zb = xmix_zblock(c, a.nf);
+    a.ZB = zb;
+    a.check_cfl = check_cfl;
+    a.cfl1 = d.cfl1;
+    a.cfl3 = d.cfl3;
+    a.cfl2 = c.dcfl2;
+    a.W = c.dW;
+    a.cflmax = c.cflmax;
+    a.flow = flow;
+    a.dotpart = dot_out;
+    a.abort_flag = c.abort_flag;
+    dim3 grid(d.Pz / zb, d.qpts);
+    size_t lds = sizeof(double2) * zb * nseq * d.Px;
+    PlanDev pd = to_dev(c.px);
+    const double rows = 16.0 * (double) d.qpts * d.Pz * d.Mx;
+    double bytes = rows * (a.nf + a.nout);
+    if (mode == XM_PHYS) bytes += 8.0 * 3 * d.Px * d.qpts * d.Pz;
+    prof_begin(c, c.stream, 2, bytes);
+    switch (mode) {
+    case XM_PRIMAL:
+        hipLaunchKernelGGL(k_xmix<XM_PRIMAL>, grid, dim3(BLK), lds, c.stream, a, pd, W1, W2);
+        break;
+    case XM_TANGENT:
+        hipLaunchKernelGGL(k_xmix<XM_TANGENT>, grid, dim3(BLK), lds, c.stream, a, pd, W1, W2);
+        break;
+    case XM_ADJOINT:
+        hipLaunchKernelGGL(k_xmix<XM_ADJOINT>, grid, dim3(BLK), lds, c.stream, a, pd, W1, W2);
+        break;
+    case XM_PHYS:
+        hipLaunchKernelGGL(k_xmix<XM_PHYS>, grid, dim3(BLK), lds, c.stream, a, pd, W1, W2);
+        break;
+    default:
+        hipLaunchKernelGGL(k_xmix<XM_DOT>, grid, dim3(BLK), lds, c.stream, a, pd, W1, W2);
+    }
+    prof_end(c, c.stream, 2);
+}
+
+int xmix_zblock(Context &c, int nf)
+{
+    int nseq = (nf + 1) / 2, zb = 1;
+    while (zb * 2 <= c.d.Pz && c.d.Pz % (zb * 2) == 0 &&
+           (size_t) zb * 2 * nseq * c.d.Px <= (size_t) XMIX_TILE)
+        zb *= 2;
+    return zb;
+}
+
+void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind)
+{
+    const Dims &d = c.d;
+    ZfwdArgs a;
+    a.nout = nout;
+    a.Nz = d.Nz;
+    a.Mx = d.Mx;
+    a.Pz = d.Pz;
+    a.qpts = d.qpts;
+    a.XB = pick_block(d.Mx, d.Pz, 1);
+    a.dst_kind = dst_kind;
+    a.norm = 1.0 / ((3. * d.Nx / 2.) * (3. * d.Nz / 2.));
+    a.abort_flag = c.abort_flag;
+    dim3 grid((d.Mx + a.XB - 1) / a.XB, nout, d.qpts);
+    size_t lds = sizeof(double2) * a.XB * d.Pz;
+    double bytes = 16.0 * (double) d.qpts * d.Mx * nout * (d.Pz + d.Nz - 1);
+    prof_begin(c, c.stream, 3, bytes);
+    hipLaunchKernelGGL(k_zfwd, grid, dim3(BLK), lds, c.stream, a, to_dev(c.pz), W2, dst);
+    prof_end(c, c.stream, 3);
+}
+
+void launch_assemble_adj(Context &c, const cplx *SP, cplx *Udst)
+{
+    const Dims &d = c.d;
+    size_t n = (size_t) d.Nz * d.qpts * d.Mx;
+    int blocks = (int) std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_assemble_adj, dim3(blocks), dim3(256), 0, c.stream, SP, Udst,
+                       c.dKx, c.dKz, d.Nz, d.qpts, d.Mx, c.abort_flag);
+}
+
+}  // namespace sc

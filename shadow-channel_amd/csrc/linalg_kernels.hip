@@ -1,0 +1,733 @@
+// Legendre-Galerkin kernels: y-transforms, banded Helmholtz solves, the
+// (kx,kz) = (0,0) mode, and small vector kernels.
+//
+// y-transforms (channel/initAlphaBeta2.c:96-240 rebuild, channel/project.c:
+// 95-115,176-191 and channel/adjproject.c:108-134,213-234 projections) are
+// real dense products (qpts x dim or dim x qpts) applied to the complex
+// columns of one kz row; one block computes a 32-row x 16-column tile of all
+// outputs of a row band, staging the basis tiles and the columns in LDS.
+//
+// Band solves (channel/mvOps.c:29-144, used by channel/project.c,
+// increproject.c, adjproject.c): the matrices of the implicit stage are
+// time-invariant, so their LU factors (no pivoting, same elimination order as
+// the reference) are computed once on the host.  Even and odd rows decouple
+// (bands sit on every other diagonal), so one thread solves one parity of
+// one (kz, kx, field) system.  Explicit band products are formed on the fly
+// from the Gram tables.
+#include "sc_internal.h"
+
+namespace sc {
+
+// ------------------------------------------------------------ rebuild
+struct RebuildArgs {
+    const double *Q, *Qp, *Qpp, *R, *Rp, *Kx, *Kz;
+    int Nz, Mx, qpts, dimR, dimQ;
+    const int *abort_flag;
+};
+
+constexpr int TI = 32, TX = 16, JT = 16;
+
+// U[z][0..4][i][x] from C[z][0..1][j][x]; grid (ceil(qpts/32), ceil(Mx/16), Nz)
+__global__ __launch_bounds__(256) void k_rebuild(RebuildArgs a, const cplx *C, cplx *U)
+{
+    __shared__ double As[5][TI][JT + 1];
+    __shared__ double2 Ba[JT][TX], Bb[JT][TX];
+    if (*a.abort_flag) return;
+    const int z = blockIdx.z;
+    if (z == a.Nz / 2) return;
+    const int i0 = blockIdx.x * TI, x0 = blockIdx.y * TX;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int Mx = a.Mx, q = a.qpts, nR = a.dimR, nQ = a.dimQ;
+    const cplx *Ca = C + (size_t) z * 2 * nR * Mx;
+    const cplx *Cb = Ca + (size_t) nR * Mx;
+    double2 acc[2][5];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int m = 0; m < 5; ++m) acc[r][m] = make_double2(0.0, 0.0);
+    for (int j0 = 0; j0 < nR; j0 += JT) {
+        for (int e = threadIdx.x; e < 5 * TI * JT; e += 256) {
+            int m = e / (TI * JT), rem = e - m * TI * JT;
+            int ii = rem / JT, jj = rem - ii * JT;
+            int i = i0 + ii, j = j0 + jj;
+            double v = 0.0;
+            if (i < q) {
+                if (m < 3) {
+                    if (j < nQ) {
+                        const double *M = m == 0 ? a.Q : m == 1 ? a.Qp : a.Qpp;
+                        v = M[(size_t) i * nQ + j];
+                    }
+                } else if (j < nR) {
+                    const double *M = m == 3 ? a.R : a.Rp;
+                    v = M[(size_t) i * nR + j];
+                }
+            }
+            As[m][ii][jj] = v;
+        }
+        for (int e = threadIdx.x; e < JT * TX; e += 256) {
+            int jj = e / TX, xx = e - jj * TX;
+            int j = j0 + jj, xg = x0 + xx;
+            double2 va = make_double2(0.0, 0.0), vb = va;
+            if (xg < Mx) {
+                if (j < nQ) va = Ca[(size_t) j * Mx + xg];
+                if (j < nR) vb = Cb[(size_t) j * Mx + xg];
+            }
+            Ba[jj][xx] = va;
+            Bb[jj][xx] = vb;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int jj = 0; jj < JT; ++jj) {
+            double2 ba = Ba[jj][tx], bb = Bb[jj][tx];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                int ii = ty + 16 * r;
+                double cq = As[0][ii][jj], cp = As[1][ii][jj], c2 = As[2][ii][jj];
+                double cr = As[3][ii][jj], crp = As[4][ii][jj];
+                acc[r][0].x += cq * ba.x;  acc[r][0].y += cq * ba.y;
+                acc[r][1].x += cp * ba.x;  acc[r][1].y += cp * ba.y;
+                acc[r][2].x += c2 * ba.x;  acc[r][2].y += c2 * ba.y;
+                acc[r][3].x += cr * bb.x;  acc[r][3].y += cr * bb.y;
+                acc[r][4].x += crp * bb.x; acc[r][4].y += crp * bb.y;
+            }
+        }
+        __syncthreads();
+    }
+    const int x = x0 + tx;
+    if (x >= Mx || (z == 0 && x == 0)) return;
+    const double kx = a.Kx[x], kz = a.Kz[z], k2 = kz * kz + kx * kx;
+    const size_t ss = (size_t) q * Mx;
+    cplx *Uz = U + (size_t) z * 5 * ss;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        int i = i0 + ty + 16 * r;
+        if (i >= q) continue;
+        double2 v = acc[r][0], f = make_double2(-acc[r][1].x, -acc[r][1].y);
+        double2 q2 = acc[r][2], g = acc[r][3], gp = acc[r][4];
+        size_t o = (size_t) i * Mx + x;
+        Uz[o + 1 * ss] = v;
+        Uz[o + 0 * ss] = make_double2((kx * f.y + kz * g.y) / k2, -(kx * f.x + kz * g.x) / k2);
+        Uz[o + 2 * ss] = make_double2((-kx * g.y + kz * f.y) / k2, (kx * g.x - kz * f.x) / k2);
+        Uz[o + 3 * ss] = make_double2((-kx * q2.y + kz * gp.y) / k2, (kx * q2.x - kz * gp.x) / k2);
+        Uz[o + 4 * ss] = make_double2(-(kx * gp.y + kz * q2.y) / k2, (kx * gp.x + kz * q2.x) / k2);
+    }
+}
+
+// mean mode (kx,kz) = (0,0): u = R a, du = Rp a, v = 0, w = R b, dw = Rp b
+__global__ void k_rebuild_mean(RebuildArgs a, const cplx *C, cplx *U)
+{
+    if (*a.abort_flag) return;
+    const int Mx = a.Mx, q = a.qpts, nR = a.dimR;
+    const size_t ss = (size_t) q * Mx;
+    for (int i = threadIdx.x; i < q; i += blockDim.x) {
+        double2 u = make_double2(0, 0), du = u, w = u, dw = u;
+        for (int j = 0; j < nR; ++j) {
+            double2 al = C[(size_t) j * Mx], be = C[(size_t) (nR + j) * Mx];
+            double r = a.R[(size_t) i * nR + j], rp = a.Rp[(size_t) i * nR + j];
+            u.x += r * al.x;  u.y += r * al.y;
+            du.x += rp * al.x; du.y += rp * al.y;
+            w.x += r * be.x;  w.y += r * be.y;
+            dw.x += rp * be.x; dw.y += rp * be.y;
+        }
+        size_t o = (size_t) i * Mx;
+        U[o + 0 * ss] = u;
+        U[o + 1 * ss] = make_double2(0.0, 0.0);
+        U[o + 2 * ss] = w;
+        U[o + 3 * ss] = du;
+        U[o + 4 * ss] = dw;
+    }
+}
+
+void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst, bool with_mean)
+{
+    const Dims &d = c.d;
+    RebuildArgs a{c.dQ, c.dQp, c.dQpp, c.dR, c.dRp, c.dKx, c.dKz,
+                  d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, c.abort_flag};
+    dim3 grid((d.qpts + TI - 1) / TI, (d.Mx + TX - 1) / TX, d.Nz);
+    double bytes = 16.0 * (double) (d.Nz - 1) * d.Mx * (2.0 * d.dimR + 5.0 * d.qpts);
+    prof_begin(c, c.stream, 4, bytes);
+    hipLaunchKernelGGL(k_rebuild, grid, dim3(256), 0, c.stream, a, Csrc, Udst);
+    prof_end(c, c.stream, 4);
+    if (with_mean)
+        hipLaunchKernelGGL(k_rebuild_mean, dim3(1), dim3(256), 0, c.stream, a, Csrc, Udst);
+}
+
+// ------------------------------------------------------------ projection
+struct ForceArgs {
+    const double *Qw, *Qpw, *Qppw, *Rw, *Rpw, *Kx, *Kz;
+    int Nz, Mx, qpts, dimR, dimQ, adj;
+    const int *abort_flag;
+};
+
+// Fa[z][i][x] (i < dimQ), Fb[z][i][x] (i < dimR); grid (ceil(dimR/32), ceil(Mx/16), Nz)
+__global__ __launch_bounds__(256) void k_force(ForceArgs a, const cplx *U, cplx *Fa, cplx *Fb)
+{
+    __shared__ double As[5][TI][JT + 1];
+    __shared__ double2 Bs[5][JT][TX];
+    if (*a.abort_flag) return;
+    const int z = blockIdx.z;
+    if (z == a.Nz / 2) return;
+    const int i0 = blockIdx.x * TI, x0 = blockIdx.y * TX;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int Mx = a.Mx, q = a.qpts, nR = a.dimR, nQ = a.dimQ;
+    const size_t ss = (size_t) q * Mx;
+    const cplx *Uz = U + (size_t) z * 5 * ss;
+    const double kz = a.Kz[z];
+    const int nm = a.adj ? 5 : 3;
+    double2 acc[2][5];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int m = 0; m < 5; ++m) acc[r][m] = make_double2(0.0, 0.0);
+    for (int j0 = 0; j0 < q; j0 += JT) {
+        // A tiles: 0 Qpw, 1 Qw, 2 Rw, 3 Qppw, 4 Rpw
+        for (int e = threadIdx.x; e < nm * TI * JT; e += 256) {
+            int m = e / (TI * JT), rem = e - m * TI * JT;
+            int ii = rem / JT, jj = rem - ii * JT;
+            int i = i0 + ii, j = j0 + jj;
+            double v = 0.0;
+            if (j < q) {
+                if (m == 0 || m == 1 || m == 3) {
+                    if (i < nQ) {
+                        const double *M = m == 0 ? a.Qpw : m == 1 ? a.Qw : a.Qppw;
+                        v = M[(size_t) i * q + j];
+                    }
+                } else if (i < nR) {
+                    const double *M = m == 2 ? a.Rw : a.Rpw;
+                    v = M[(size_t) i * q + j];
+                }
+            }
+            As[m][ii][jj] = v;
+        }
+        // B: 0 P = i kx Hx + i kz Hz, 1 Hy, 2 P2 = i kz Hx - i kx Hz,
+        //    3 Z = -i kz Sx + i kx Sz, 4 Z2 = i kz Sz + i kx Sx
+        for (int e = threadIdx.x; e < JT * TX; e += 256) {
+            int jj = e / TX, xx = e - jj * TX;
+            int j = j0 + jj, xg = x0 + xx;
+            double2 P = make_double2(0, 0), Y = P, P2 = P, Z = P, Z2 = P;
+            if (j < q && xg < Mx) {
+                double kx = a.Kx[xg];
+                size_t o = (size_t) j * Mx + xg;
+                double2 hz = Uz[o], hx = Uz[o + ss], hy = Uz[o + 2 * ss];
+                P = make_double2(-kx * hx.y - kz * hz.y, kx * hx.x + kz * hz.x);
+                Y = hy;
+                P2 = make_double2(-kz * hx.y + kx * hz.y, kz * hx.x - kx * hz.x);
+                if (a.adj) {
+                    double2 sx = Uz[o + 3 * ss], sz = Uz[o + 4 * ss];
+                    Z = make_double2(kz * sx.y - kx * sz.y, -kz * sx.x + kx * sz.x);
+                    Z2 = make_double2(-kz * sz.y - kx * sx.y, kz * sz.x + kx * sx.x);
+                }
+            }
+            Bs[0][jj][xx] = P;
+            Bs[1][jj][xx] = Y;
+            Bs[2][jj][xx] = P2;
+            Bs[3][jj][xx] = Z;
+            Bs[4][jj][xx] = Z2;
+        }
+        __syncthreads();
+        if (a.adj) {
+#pragma unroll 4
+            for (int jj = 0; jj < JT; ++jj) {
+                double2 b0 = Bs[0][jj][tx], b1 = Bs[1][jj][tx], b2 = Bs[2][jj][tx];
+                double2 b3 = Bs[3][jj][tx], b4 = Bs[4][jj][tx];
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    int ii = ty + 16 * r;
+                    double c0 = As[0][ii][jj], c1 = As[1][ii][jj], c2 = As[2][ii][jj];
+                    double c3 = As[3][ii][jj], c4 = As[4][ii][jj];
+                    acc[r][0].x += c0 * b0.x; acc[r][0].y += c0 * b0.y;
+                    acc[r][1].x += c1 * b1.x; acc[r][1].y += c1 * b1.y;
+                    acc[r][2].x += c2 * b2.x; acc[r][2].y += c2 * b2.y;
+                    acc[r][3].x += c3 * b3.x; acc[r][3].y += c3 * b3.y;
+                    acc[r][4].x += c4 * b4.x; acc[r][4].y += c4 * b4.y;
+                }
+            }
+        } else {
+#pragma unroll 4
+            for (int jj = 0; jj < JT; ++jj) {
+                double2 b0 = Bs[0][jj][tx], b1 = Bs[1][jj][tx], b2 = Bs[2][jj][tx];
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    int ii = ty + 16 * r;
+                    double c0 = As[0][ii][jj], c1 = As[1][ii][jj], c2 = As[2][ii][jj];
+                    acc[r][0].x += c0 * b0.x; acc[r][0].y += c0 * b0.y;
+                    acc[r][1].x += c1 * b1.x; acc[r][1].y += c1 * b1.y;
+                    acc[r][2].x += c2 * b2.x; acc[r][2].y += c2 * b2.y;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const int x = x0 + tx;
+    if (x >= Mx || (z == 0 && x == 0)) return;
+    const double kx = a.Kx[x], k2 = kz * kz + kx * kx;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        int i = i0 + ty + 16 * r;
+        if (i < nQ) {
+            double2 fa = make_double2(acc[r][0].x - k2 * acc[r][1].x + acc[r][3].x,
+                                      acc[r][0].y - k2 * acc[r][1].y + acc[r][3].y);
+            Fa[((size_t) z * nQ + i) * Mx + x] = fa;
+        }
+        if (i < nR) {
+            double2 fb = make_double2(acc[r][2].x + acc[r][4].x, acc[r][2].y + acc[r][4].y);
+            Fb[((size_t) z * nR + i) * Mx + x] = fb;
+        }
+    }
+}
+
+void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb)
+{
+    const Dims &d = c.d;
+    ForceArgs a{c.dQw, c.dQpw, c.dQppw, c.dRw, c.dRpw, c.dKx, c.dKz,
+                d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, adj, c.abort_flag};
+    dim3 grid((d.dimR + TI - 1) / TI, (d.Mx + TX - 1) / TX, d.Nz);
+    double bytes = 16.0 * (double) (d.Nz - 1) * d.Mx *
+                   ((adj ? 5.0 : 3.0) * d.qpts + d.dimQ + d.dimR);
+    prof_begin(c, c.stream, 5, bytes);
+    hipLaunchKernelGGL(k_force, grid, dim3(256), 0, c.stream, a, Usrc, Fa, Fb);
+    prof_end(c, c.stream, 5);
+}
+
+void launch_force(Context &c, const cplx *Usrc, int adj) { launch_force(c, Usrc, adj, c.Fa, c.Fb); }
+
+// ------------------------------------------------------------ band solves
+struct SolveArgs {
+    const double *Qs, *Qps, *Qpps, *Rs, *Rps, *Kx, *Kz;
+    const double *La, *Lb;             // factors of this stage
+    const cplx *Fa, *Fb;               // projection for this stage's RHS
+    const cplx *Fa2, *Fb2;             // projection for the next-stage RHS
+    int Nz, Mx, dimR, dimQ, k, has_next;
+    double dtc, dtd;                   // dt c_k, dt d_{k+1}
+    double ce0[2], cn[2];              // explicit-operator coefficients per field (stage 0, next)
+    const int *abort_flag;
+};
+
+// explicit band operator row entry A(i, i+2m)
+__device__ __forceinline__ double op_entry(const SolveArgs &a, int var, int i, int m, double k2, double coef)
+{
+    if (var == 0) {
+        int e = i * 9 + 4 + m;
+        double s = k2 * k2;
+        return -(k2 * a.Qs[e] + a.Qps[e]) + coef * (s * a.Qs[e] + 2. * k2 * a.Qps[e] + a.Qpps[e]);
+    }
+    int e = i * 5 + 2 + m;
+    return a.Rs[e] + coef * (a.Rps[e] + k2 * a.Rs[e]);
+}
+
+// grid (ceil(Nz*Mx/256), 4): blockIdx.y = field*2 + parity
+__global__ __launch_bounds__(256) void k_solve(SolveArgs a, cplx *C, cplx *Cs)
+{
+    if (*a.abort_flag) return;
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    if (col >= a.Nz * a.Mx) return;
+    const int z = col / a.Mx, x = col - z * a.Mx;
+    const int var = blockIdx.y >> 1, par = blockIdx.y & 1;
+    const int Mx = a.Mx, nR = a.dimR, nQ = a.dimQ;
+    cplx *Cc = C + ((size_t) (z * 2 + var) * nR) * Mx + x;
+    cplx *Co = Cs + ((size_t) (z * 2 + var) * nR) * Mx + x;
+    if (z == a.Nz / 2) {
+        for (int i = par; i < nR; i += 2) Cc[(size_t) i * Mx] = make_double2(0.0, 0.0);
+        return;
+    }
+    if (z == 0 && x == 0) return;
+    const int N = var == 0 ? nQ : nR, dl = var == 0 ? 4 : 2, W = 2 * dl + 1;
+    const double kx = a.Kx[x], kz = a.Kz[z], k2 = kz * kz + kx * kx;
+    const cplx *F = var == 0 ? a.Fa + ((size_t) z * nQ) * Mx + x : a.Fb + ((size_t) z * nR) * Mx + x;
+    const cplx *F2 = var == 0 ? a.Fa2 + ((size_t) z * nQ) * Mx + x : a.Fb2 + ((size_t) z * nR) * Mx + x;
+    const double *L = (var == 0 ? a.La : a.Lb) + ((size_t) z * N * W) * Mx + x;
+    // alpha rows dimQ..dimR-1 pass through unchanged (channel/project.c:133)
+    if (var == 0)
+        for (int i = nQ + ((nQ & 1) != par); i < nR; i += 2) Co[(size_t) i * Mx] = Cc[(size_t) i * Mx];
+    // right-hand side
+    for (int i = par; i < N; i += 2) {
+        double2 t;
+        if (a.k == 0) {
+            t = make_double2(0.0, 0.0);
+            int m0 = -min(dl, i / 2), m1 = min(dl, (N - i + 1) / 2 - 1);
+            for (int m = m0; m <= m1; ++m) {
+                double e = op_entry(a, var, i, m, k2, a.ce0[var]);
+                double2 yv = Cc[(size_t) (i + 2 * m) * Mx];
+                t.x += e * yv.x;
+                t.y += e * yv.y;
+            }
+        } else {
+            t = Cc[(size_t) i * Mx];
+        }
+        double2 f = F[(size_t) i * Mx];
+        t.x += a.dtc * f.x;
+        t.y += a.dtc * f.y;
+        Co[(size_t) i * Mx] = t;
+    }
+    // forward substitution with unit L
+    for (int i = par; i < N; i += 2) {
+        if (i < 2) continue;
+        double2 b = Co[(size_t) i * Mx];
+        int mmax = min(dl, i / 2);
+        for (int m = mmax; m >= 1; --m) {
+            double l = L[((size_t) i * W + dl - m) * Mx];
+            double2 p = Co[(size_t) (i - 2 * m) * Mx];
+            b.x -= l * p.x;
+            b.y -= l * p.y;
+        }
+        Co[(size_t) i * Mx] = b;
+    }
+    // back substitution
+    int ilast = (N - 1) - (((N - 1) - par) & 1);
+    for (int i = ilast; i >= 0; i -= 2) {
+        double2 b = Co[(size_t) i * Mx];
+        int jmax = min(dl, (N - i + 1) / 2 - 1);
+        for (int j = 1; j <= jmax; ++j) {
+            double u = L[((size_t) i * W + dl + j) * Mx];
+            double2 p = Co[(size_t) (i + 2 * j) * Mx];
+            b.x -= u * p.x;
+            b.y -= u * p.y;
+        }
+        double dg = L[((size_t) i * W + dl) * Mx];
+        b.x /= dg;
+        b.y /= dg;
+        Co[(size_t) i * Mx] = b;
+    }
+    // explicit part of the next stage, or the solution after the last stage
+    for (int i = par; i < N; i += 2) {
+        double2 t;
+        if (a.has_next) {
+            t = make_double2(0.0, 0.0);
+            int m0 = -min(dl, i / 2), m1 = min(dl, (N - i + 1) / 2 - 1);
+            for (int m = m0; m <= m1; ++m) {
+                double e = op_entry(a, var, i, m, k2, a.cn[var]);
+                double2 yv = Co[(size_t) (i + 2 * m) * Mx];
+                t.x += e * yv.x;
+                t.y += e * yv.y;
+            }
+            double2 f = F2[(size_t) i * Mx];
+            t.x += a.dtd * f.x;
+            t.y += a.dtd * f.y;
+        } else {
+            t = Co[(size_t) i * Mx];
+        }
+        Cc[(size_t) i * Mx] = t;
+    }
+}
+
+// coefficient tables for the stages
+static const double FA_[3] = {1. / 3., -1. / 2, 1. / 3.};
+static const double FC_[3] = {1. / 2, 1. / 3., 1.};
+static const double FD_[3] = {0., -1. / 6, -2. / 3};
+static const double BA_[3] = {1. / 3., -1. / 2, 1. / 3.};
+static const double BC_[3] = {1., 1. / 3., 1. / 2.};
+static const double BD_[3] = {0., -2. / 3, -1. / 6};
+
+void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cplx *Fa2,
+                  const cplx *Fb2)
+{
+    const Dims &d = c.d;
+    const double *A = adj ? BA_ : FA_, *Cc = adj ? BC_ : FC_, *D = adj ? BD_ : FD_;
+    SolveArgs a;
+    a.Qs = c.dQs; a.Qps = c.dQps; a.Qpps = c.dQpps; a.Rs = c.dRs; a.Rps = c.dRps;
+    a.Kx = c.dKx; a.Kz = c.dKz;
+    a.La = c.dLa[adj][k];
+    a.Lb = c.dLb[adj][k];
+    a.Fa = c.Fa; a.Fb = c.Fb;
+    a.Fa2 = Fa2; a.Fb2 = Fb2;
+    a.Nz = d.Nz; a.Mx = d.Mx; a.dimR = d.dimR; a.dimQ = d.dimQ; a.k = k;
+    a.has_next = k != 2;
+    a.dtc = d.dt * Cc[k];
+    a.dtd = k != 2 ? d.dt * D[k + 1] : 0.0;
+    // v: -(K2 Qs + Qps) + re a dt (K2^2 Qs + 2 K2 Qps + Qpps);
+    // omega_y: Rs - re a dt (Rps + K2 Rs)  (channel/project.c:55-71,140-151)
+    a.ce0[0] = d.re * A[0] * d.dt;
+    a.ce0[1] = -(d.re * A[0] * d.dt);
+    a.cn[0] = k != 2 ? d.re * A[k + 1] * d.dt : 0.0;
+    a.cn[1] = k != 2 ? -(d.re * A[k + 1] * d.dt) : 0.0;
+    a.abort_flag = c.abort_flag;
+    dim3 grid((d.Nz * d.Mx + 255) / 256, 4);
+    // RHS in, projections in, solution and next right-hand side out, factors in
+    double bytes = (double) (d.Nz - 1) * d.Mx *
+                   (16.0 * (2.0 * d.dimR) * 3.0 + 16.0 * (d.dimQ + d.dimR) * (k != 2 ? 2.0 : 1.0) +
+                    8.0 * (9.0 * d.dimQ + 5.0 * d.dimR));
+    prof_begin(c, c.stream, 6, bytes);
+    hipLaunchKernelGGL(k_solve, grid, dim3(256), 0, c.stream, a, Ccarry, Csol);
+    prof_end(c, c.stream, 6);
+}
+
+// ------------------------------------------------------------ mean mode
+// (channel/project0.c:22-271, channel/increproject0.c:14-312,
+//  channel/adjproject0.c:8-310)
+struct MeanArgs {
+    const double *Rs, *Rps, *Rw, *Rpw, *R, *Rp, *rowsum;
+    const double *Lm, *add;
+    int Mx, qpts, dimR, k, has_next, adj;
+    double dtc, dtd, ce0, cn, target;
+    const int *abort_flag;
+};
+
+constexpr int MAXR = 512;
+
+__device__ void mean_rhs(const MeanArgs &a, const cplx *Uh, double *fa, double *fb)
+{
+    const int q = a.qpts, nR = a.dimR, Mx = a.Mx;
+    const size_t ss = (size_t) q * Mx;
+    for (int i = threadIdx.x; i < nR; i += blockDim.x) {
+        double sa = 0.0, sb = 0.0;
+        for (int j = 0; j < q; ++j) {
+            size_t o = (size_t) j * Mx;
+            double rw = a.Rw[(size_t) i * q + j];
+            if (a.adj) {
+                double rpw = a.Rpw[(size_t) i * q + j];
+                sa += rw * Uh[o + 1 * ss].x + rpw * Uh[o + 4 * ss].x;
+                sb += rw * Uh[o + 0 * ss].x - rpw * Uh[o + 3 * ss].x;
+            } else {
+                sa += rw * Uh[o + 1 * ss].x;
+                sb += rw * Uh[o + 0 * ss].x;
+            }
+        }
+        fa[i] = sa;
+        fb[i] = sb;
+    }
+}
+
+// explicit mean operator Rs + coef Rps applied to v (row i)
+__device__ double2 mean_mult(const MeanArgs &a, const double2 *v, int i, double coef)
+{
+    const int N = a.dimR;
+    double2 t = make_double2(0.0, 0.0);
+    int m0 = -min(2, i / 2), m1 = min(2, (N - i + 1) / 2 - 1);
+    for (int m = m0; m <= m1; ++m) {
+        int e = i * 5 + 2 + m;
+        double c = a.Rs[e] + coef * a.Rps[e];
+        t.x += c * v[i + 2 * m].x;
+        t.y += c * v[i + 2 * m].y;
+    }
+    return t;
+}
+
+__device__ void mean_subst(const MeanArgs &a, double2 *b)
+{
+    const int N = a.dimR, dl = 2, W = 5;
+    const double *A = a.Lm;
+    for (int i = 2; i < N; ++i) {
+        int mmax = min(dl, i / 2);
+        for (int m = mmax; m >= 1; --m) {
+            double l = A[i * W + dl - m];
+            b[i].x -= l * b[i - 2 * m].x;
+            b[i].y -= l * b[i - 2 * m].y;
+        }
+    }
+    for (int i = N - 1; i >= 0; --i) {
+        int jmax = min(dl, (N - i + 1) / 2 - 1);
+        for (int j = 1; j <= jmax; ++j) {
+            double u = A[i * W + dl + j];
+            b[i].x -= u * b[i + 2 * j].x;
+            b[i].y -= u * b[i + 2 * j].y;
+        }
+        b[i].x /= A[i * W + dl];
+        b[i].y /= A[i * W + dl];
+    }
+}
+
+// one block of 256 threads
+__global__ __launch_bounds__(256) void k_mean(MeanArgs a, cplx *C, cplx *Cs, const cplx *Uh,
+                                              const cplx *Ul, cplx *Uout)
+{
+    __shared__ double2 al[MAXR], be[MAXR], tmp[MAXR];
+    __shared__ double fa[MAXR], fb[MAXR], red[256];
+    if (*a.abort_flag) return;
+    const int nR = a.dimR, Mx = a.Mx, q = a.qpts;
+    const size_t ss = (size_t) q * Mx;
+    cplx *Ca = C, *Cb = C + (size_t) nR * Mx;
+    for (int i = threadIdx.x; i < nR; i += blockDim.x) {
+        al[i] = Ca[(size_t) i * Mx];
+        be[i] = Cb[(size_t) i * Mx];
+    }
+    mean_rhs(a, Uh, fa, fb);
+    __syncthreads();
+    if (a.k == 0) {
+        for (int i = threadIdx.x; i < nR; i += blockDim.x) tmp[i] = mean_mult(a, al, i, a.ce0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < nR; i += blockDim.x) al[i] = tmp[i];
+        __syncthreads();
+        for (int i = threadIdx.x; i < nR; i += blockDim.x) tmp[i] = mean_mult(a, be, i, a.ce0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < nR; i += blockDim.x) be[i] = tmp[i];
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < nR; i += blockDim.x) {
+        al[i].x += a.dtc * fa[i];
+        be[i].x += a.dtc * fb[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) mean_subst(a, al);
+    if (threadIdx.x == 64) mean_subst(a, be);
+    __syncthreads();
+    // flux correction: alpha += (target - sum_i sum_j Rw[i][j] Re alpha_i) add / 2
+    double p = 0.0;
+    for (int i = threadIdx.x; i < nR; i += blockDim.x) p += a.rowsum[i] * al[i].x;
+    red[threadIdx.x] = p;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    double flux_t = a.target - red[0];
+    __syncthreads();
+    for (int i = threadIdx.x; i < nR; i += blockDim.x) {
+        al[i].x = al[i].x + flux_t * a.add[i] * 0.5;
+        al[i].y = al[i].y + flux_t * 0.0 * 0.5;
+        Cs[(size_t) i * Mx] = al[i];
+        Cs[(size_t) (nR + i) * Mx] = be[i];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < q; i += blockDim.x) {
+        double2 u = make_double2(0, 0), du = u, w = u, dw = u;
+        for (int j = 0; j < nR; ++j) {
+            double r = a.R[(size_t) i * nR + j], rp = a.Rp[(size_t) i * nR + j];
+            u.x += r * al[j].x;  u.y += r * al[j].y;
+            du.x += rp * al[j].x; du.y += rp * al[j].y;
+            w.x += r * be[j].x;  w.y += r * be[j].y;
+            dw.x += rp * be[j].x; dw.y += rp * be[j].y;
+        }
+        size_t o = (size_t) i * Mx;
+        Uout[o + 0 * ss] = u;
+        Uout[o + 1 * ss] = make_double2(0.0, 0.0);
+        Uout[o + 2 * ss] = w;
+        Uout[o + 3 * ss] = du;
+        Uout[o + 4 * ss] = dw;
+    }
+    if (!a.has_next) {
+        for (int i = threadIdx.x; i < nR; i += blockDim.x) {
+            Ca[(size_t) i * Mx] = al[i];
+            Cb[(size_t) i * Mx] = be[i];
+        }
+        return;
+    }
+    if (a.adj) {
+        __syncthreads();
+        mean_rhs(a, Ul, fa, fb);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nR; i += blockDim.x) {
+        double2 ta = mean_mult(a, al, i, a.cn), tb = mean_mult(a, be, i, a.cn);
+        Ca[(size_t) i * Mx] = make_double2(ta.x + a.dtd * fa[i], ta.y + a.dtd * 0.0);
+        Cb[(size_t) i * Mx] = make_double2(tb.x + a.dtd * fb[i], tb.y + a.dtd * 0.0);
+    }
+}
+
+void launch_mean(Context &c, cplx *Ccarry, cplx *Csol, const cplx *Uh, const cplx *Ul,
+                 cplx *Uout, int k, int mode, double target)
+{
+    const Dims &d = c.d;
+    const int adj = mode == 2;
+    const double *A = adj ? BA_ : FA_, *Cc = adj ? BC_ : FC_, *D = adj ? BD_ : FD_;
+    MeanArgs a;
+    a.Rs = c.dRs; a.Rps = c.dRps; a.Rw = c.dRw; a.Rpw = c.dRpw; a.R = c.dR; a.Rp = c.dRp;
+    a.rowsum = c.dRowsum;
+    a.Lm = c.dLm[adj][k];
+    a.add = c.dAdd[adj][k];
+    a.Mx = d.Mx; a.qpts = d.qpts; a.dimR = d.dimR; a.k = k; a.has_next = k != 2; a.adj = adj;
+    a.dtc = d.dt * Cc[k];
+    a.dtd = k != 2 ? d.dt * D[k + 1] : 0.0;
+    a.ce0 = -(d.re * A[0] * d.dt);
+    a.cn = k != 2 ? -(d.re * A[k + 1] * d.dt) : 0.0;
+    a.target = target;
+    a.abort_flag = c.abort_flag;
+    hipLaunchKernelGGL(k_mean, dim3(1), dim3(256), 0, c.stream, a, Ccarry, Csol, Uh,
+                       Ul ? Ul : Uh, Uout);
+}
+
+// ------------------------------------------------------------ vectors
+__global__ void k_axpy(cplx *y, const cplx *x, double a, size_t n, const int *abort_flag)
+{
+    if (*abort_flag) return;
+    for (size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x; e < n;
+         e += (size_t) gridDim.x * blockDim.x) {
+        y[e].x -= a * x[e].x;
+        y[e].y -= a * x[e].y;
+    }
+}
+
+__global__ void k_diff_scale(cplx *o, const cplx *p, const cplx *m, double s, size_t n)
+{
+    for (size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x; e < n;
+         e += (size_t) gridDim.x * blockDim.x) {
+        o[e].x = s * (p[e].x - m[e].x);
+        o[e].y = s * (p[e].y - m[e].y);
+    }
+}
+
+__global__ void k_ddt_reduce(const double *part, int nparts, double *scal)
+{
+    __shared__ double r0[256], r1[256];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 256) {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    r0[threadIdx.x] = a;
+    r1[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            r0[threadIdx.x] += r0[threadIdx.x + o];
+            r1[threadIdx.x] += r1[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        scal[0] = r0[0];
+        scal[1] = r1[0];
+        scal[2] = r0[0] / r1[0];
+    }
+}
+
+__global__ void k_ddt_apply(cplx *v, const cplx *dC, const double *scal, size_t n)
+{
+    double r = scal[2];
+    for (size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x; e < n;
+         e += (size_t) gridDim.x * blockDim.x) {
+        v[e].x -= r * dC[e].x;
+        v[e].y -= r * dC[e].y;
+    }
+}
+
+// first plane whose CFL number reaches the limit sets abort_flag = 1 + plane
+__global__ void k_cfl_check(const double *cflmax, int qpts, double dt, int *abort_flag)
+{
+    if (threadIdx.x != 0 || *abort_flag) return;
+    for (int y = 0; y < qpts; ++y) {
+        if (cflmax[y] * dt >= 2.8) {
+            *abort_flag = 1 + y;
+            return;
+        }
+    }
+}
+
+static int nblocks(size_t n) { return (int) std::min<size_t>((n + 255) / 256, 8192); }
+
+void launch_axpy_c(Context &c, cplx *y, const cplx *x, double a, size_t n)
+{
+    hipLaunchKernelGGL(k_axpy, dim3(nblocks(n)), dim3(256), 0, c.stream, y, x, a, n, c.abort_flag);
+}
+
+void launch_diff_scale(Context &c, cplx *out, const cplx *p, const cplx *m, double s, size_t n)
+{
+    hipLaunchKernelGGL(k_diff_scale, dim3(nblocks(n)), dim3(256), 0, c.stream, out, p, m, s, n);
+}
+
+void launch_ddt_reduce(Context &c, int nparts)
+{
+    hipLaunchKernelGGL(k_ddt_reduce, dim3(1), dim3(256), 0, c.stream, c.dotpart, nparts, c.scal);
+}
+
+void launch_ddt_apply(Context &c, cplx *v, const cplx *dC, size_t n)
+{
+    hipLaunchKernelGGL(k_ddt_apply, dim3(nblocks(n)), dim3(256), 0, c.stream, v, dC, c.scal, n);
+}
+
+void launch_cfl_check(Context &c)
+{
+    hipLaunchKernelGGL(k_cfl_check, dim3(1), dim3(64), 0, c.stream, c.cflmax, c.d.qpts, c.d.dt,
+                       c.abort_flag);
+}
+
+}  // namespace sc

@@ -1,0 +1,158 @@
+// Internal declarations of the MI355X channel-DNS library.
+//
+// Device layouts (complex = double2 {re, im}, kx fastest), identical to the
+// reference's arrays so that C, MC and MIC can be exchanged without reshuffle
+// (channel/main.h:81-200, channel/minChnl.h:46-72):
+//   C  [Nz][2][dimR][Mx]     alpha (Q basis), beta (R basis)
+//   U  [Nz][5][qpts][Mx]     u, v, w, du/dy, dw/dy at the Gauss points;
+//                            after a nonlinear pass slots 0..2 hold Hz, Hx, Hy
+//                            (adjoint: slots 3, 4 hold S_x, S_z)
+//   W1 [qpts][nf][Pz][Mx]    fields after the z inverse transform
+//   W2 [qpts][no][Pz][Mx]    products after the x forward transform
+// with Mx = Nx/2, Pz = 3Nz/2, Px = 3Nx/2 (3/2-rule padding).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace sc {
+
+typedef double2 cplx;
+
+constexpr int MAXF = 12;        // fields in one z-inverse launch
+constexpr int MAX_FACTORS = 24; // radix factors of an FFT length
+
+struct FftPlan {
+    int n = 0;
+    int nf = 0;
+    int f[MAX_FACTORS];
+    double2 *tw = nullptr;      // device: exp(+2 pi i m / n), m < n
+};
+
+// What the z-inverse kernel loads into field f.
+enum FieldKind : int {
+    FK_SLOT = 0,     // copy slot `slot` of `src`
+    FK_OMEGA_Z = 1,  // -du/dy + i kx v
+    FK_OMEGA_X = 2,  // dw/dy - i kz v
+    FK_OMEGA_Y = 3,  // i kz u - i kx w
+};
+
+struct FieldSpec {
+    const cplx *src[MAXF];
+    int kind[MAXF];
+    int slot[MAXF];
+    int nf;
+};
+
+// Pointwise stage of the x kernel.
+enum XMode : int {
+    XM_PRIMAL = 0,   // 6 in (u,v,w,oz,ox,oy) -> 3 out (Hz, Hx, Hy)
+    XM_TANGENT = 1,  // 12 in (base, tangent) -> 3 out (IHz, IHx, IHy)
+    XM_ADJOINT = 2,  // 9 in (u,v,w,oz,ox,oy,lx,ly,lz) -> 6 out (Sx,Sy,Sz,Tx,Ty,Tz)
+    XM_PHYS = 3,     // 3 in -> physical flow[3][Px][qpts][Pz]
+    XM_DOT = 4,      // 6 in (dudt xyz, v xyz) -> weighted dot products
+};
+
+struct Dims {
+    int Nx, Ny, Nz, Mx, qpts, dimR, dimQ, Px, Pz, nsteps;
+    double Lx, Lz, dt, re, flux, cfl1, cfl3;
+};
+
+struct Context {
+    bool ok = false;
+    Dims d{};
+    hipStream_t stream = nullptr;
+
+    // host copies of the setup (also used for the laminar start)
+    std::vector<double> y, w, cfl2, Kx, Kz;
+    std::vector<double> Q, Qp, Qpp, R, Rp, Qw, Qpw, Qppw, Rw, Rpw;
+    std::vector<double> Qs, Qps, Qpps, Rs, Rps, Rp0;
+
+    // device constants
+    double *dKx = nullptr, *dKz = nullptr, *dcfl2 = nullptr, *dW = nullptr;
+    double *dQ = nullptr, *dQp = nullptr, *dQpp = nullptr, *dR = nullptr, *dRp = nullptr;
+    double *dQw = nullptr, *dQpw = nullptr, *dQppw = nullptr, *dRw = nullptr, *dRpw = nullptr;
+    double *dQs = nullptr, *dQps = nullptr, *dQpps = nullptr, *dRs = nullptr, *dRps = nullptr;
+    double *dRowsum = nullptr;                  // sum_j Rw[i][j]
+    // LU factors of the implicit operators, per (direction, stage):
+    //   alpha: [Nz][dimQ][9][Mx]   (j<4: L(i,i-2(4-j)); j=4: U diagonal; j>4: U)
+    //   beta:  [Nz][dimR][5][Mx]
+    double *dLa[2][3] = {{nullptr}}, *dLb[2][3] = {{nullptr}};
+    // mean mode: factors [dimR][5] and flux-correction vector [dimR]
+    double *dLm[2][3] = {{nullptr}}, *dAdd[2][3] = {{nullptr}};
+
+    // state
+    cplx *C = nullptr, *IC = nullptr, *AC = nullptr;
+    cplx *U = nullptr, *IU = nullptr, *AU = nullptr, *LU = nullptr, *Ut = nullptr, *Ut2 = nullptr;
+    cplx *MC = nullptr, *MIC = nullptr;
+    cplx *Csol = nullptr, *Ctmp = nullptr, *Ctmp2 = nullptr;
+    cplx *Fa = nullptr, *Fb = nullptr;          // [Nz][dimQ][Mx], [Nz][dimR][Mx]
+    cplx *FLa = nullptr, *FLb = nullptr;        // adjoint next-stage projection
+    cplx *W1 = nullptr, *W2 = nullptr, *SP = nullptr;
+    double *flowdev = nullptr;                  // [3][Px][qpts][Pz]
+    double *cflmax = nullptr;                   // [qpts]
+    double *dotpart = nullptr;                  // partial sums
+    double *scal = nullptr;                     // [8] device scalars
+    int *abort_flag = nullptr;
+    FftPlan pz, px;
+    size_t csize = 0, usize = 0;                // element counts
+    int status = 0;                             // -1 after a CFL failure
+
+    // host views handed out by c_getsoln / c_getincresoln, per step index
+    std::vector<std::vector<cplx>> views, iviews;
+
+    // kernel timing (benchmarks)
+    int prof_class = 0;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    double prof_bytes = 0.0;        // algorithmic bytes of the recorded launches
+};
+
+Context &ctx();
+
+// ---- setup (host) ----
+int setup_host(Context &c);
+void band_factor(double *A, int N, int dl);
+
+// ---- launches ----
+void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1);
+void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
+                 double *flow, double *dot_out);
+void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind);
+void launch_assemble_adj(Context &c, const cplx *SP, cplx *Udst);
+void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst, bool with_mean = true);
+void launch_force(Context &c, const cplx *Usrc, int adj);
+void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb);
+void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cplx *Fa2,
+                  const cplx *Fb2);
+void launch_mean(Context &c, cplx *Ccarry, cplx *Csol, const cplx *Uh, const cplx *Ul,
+                 cplx *Uout, int k, int mode, double target);
+void launch_cfl_check(Context &c);
+int xmix_zblock(Context &c, int nf);
+void launch_axpy_c(Context &c, cplx *y, const cplx *x, double a, size_t n);
+void launch_diff_scale(Context &c, cplx *out, const cplx *a, const cplx *b, double s, size_t n);
+void launch_ddt_apply(Context &c, cplx *v, const cplx *dC, size_t n);
+void launch_ddt_reduce(Context &c, int nparts);
+
+// dst_kind for launch_zfwd
+enum { ZF_USLOTS = 0, ZF_SCRATCH = 1 };
+
+// timing of one kernel class inside the timed region of a benchmark
+void prof_begin(Context &c, hipStream_t s, int kernel_id, double bytes = 0.0);
+void prof_end(Context &c, hipStream_t s, int kernel_id);
+
+std::string &last_error();
+}  // namespace sc
+
+#define SC_HIP(call)                                                        \
+    do {                                                                    \
+        hipError_t e_ = (call);                                             \
+        if (e_ != hipSuccess) {                                             \
+            sc::last_error() = std::string(#call) + ": " + hipGetErrorString(e_); \
+            fprintf(stderr, "shadow-channel: %s\n", sc::last_error().c_str()); \
+            return -10;                                                     \
+        }                                                                   \
+    } while (0)

@@ -1,12 +1,26 @@
 import os
+import subprocess
 import sys
 
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-for p in (ROOT, os.path.join(ROOT, "oracle"), os.path.join(ROOT, "shadow-channel_amd")):
+PKG = os.path.join(ROOT, "shadow-channel_amd")
+for p in (ROOT, os.path.join(ROOT, "oracle"), PKG):
     if p not in sys.path:
         sys.path.insert(0, p)
+
+
+def _ensure_built():
+    """Build the oracle and the HIP library when they are missing (a fresh
+    checkout); the GPU box only receives prebuilt files."""
+    if not os.path.exists(os.path.join(ROOT, "oracle", "liborc.so")):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    if not os.path.exists(os.path.join(PKG, "libshadowchannel.so")):
+        subprocess.run(["make", "-s", "-C", os.path.join(PKG, "csrc")], check=True)
+
+
+_ensure_built()
 
 
 def pytest_configure(config):

@@ -1,0 +1,199 @@
+"""GPU parity: the HIP library (through the C ABI, via c_channel) against the
+CPU oracle on the same seeded states.
+
+Tolerances (relative L2 over the coefficient array):
+  * SHORT = 1e-10 for one to four steps.  The Legendre-Galerkin solves are
+    ill-conditioned, so any change of rounding order moves the result by a
+    few 1e-11: the oracle built with FMA contraction differs from the plain
+    oracle by 4.9e-11 after one step of the 32x32x65 case, 1.3e-11 (run-up),
+    3.1e-11 (tangent), 4.1e-11 (adjoint), 9.2e-11 (projection); see
+    tests/test_oracle_rounding_floor.py.  The device path lands at 1.2-1.5x
+    those floors.
+  * 1e-8 after 100 primal steps on the 32x32x65 minimal channel (Lx = pi,
+    Lz = 0.3 pi, Re = 2000, dt = 0.005).
+"""
+import numpy as np
+import pytest
+
+import pyorc
+from channel.synthetic import synthetic_state
+
+pytestmark = pytest.mark.gpu
+
+G0 = dict(Nx=32, Ny=65, Nz=32, Lx=np.pi, Lz=0.3 * np.pi, Re=2000.0, flux=4.0 / 3.0, dt=0.005)
+SMALL = dict(Nx=16, Ny=33, Nz=12, Lx=np.pi, Lz=0.3 * np.pi, Re=2000.0, flux=4.0 / 3.0, dt=0.005)
+
+
+@pytest.fixture(scope="module")
+def cc():
+    import c_channel
+    yield c_channel
+    c_channel.c_destroy()
+
+
+SHORT = 1e-10
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def start(cc, cfg, nsteps, seed=0, amp=0.02):
+    c = cfg
+    o = pyorc.Oracle(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"], nsteps)
+    assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"],
+                     nsteps) == 0
+    C0 = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=seed, amp=amp)
+    return o, C0
+
+
+def direction(cfg, seed):
+    c = cfg
+    return synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=seed,
+                           unit=True)
+
+
+def test_primal_one_step_minimal_channel(cc):
+    print("case: primal 1 step, 32x32x65", flush=True)
+    o, C0 = start(cc, G0, 1)
+    Co, Cg = C0.copy(), C0.copy()
+    o.primal(0, Co)
+    cc.c_primal(0, Cg)
+    assert cc.last_status() == 0
+    assert rel(Cg, Co) <= SHORT
+    assert rel(cc.c_getsoln(1), o.getsoln(1)) <= SHORT
+    np.testing.assert_array_equal(cc.c_getsoln(0), C0)
+
+
+def test_primal_hundred_steps_minimal_channel(cc):
+    print("case: primal 100 steps, 32x32x65", flush=True)
+    o, C0 = start(cc, G0, 100)
+    Co, Cg = C0.copy(), C0.copy()
+    o.primal(0, Co)
+    cc.c_primal(0, Cg)
+    assert rel(Cg, Co) <= 1e-8
+    for i in (1, 10, 50):
+        assert rel(cc.c_getsoln(i), o.getsoln(i)) <= 1e-8
+
+
+def test_primal_with_run_up(cc):
+    print("case: primal run-up 2 + 2 steps", flush=True)
+    o, C0 = start(cc, SMALL, 2, amp=0.1)
+    Co, Cg = C0.copy(), C0.copy()
+    o.primal(2, Co)
+    cc.c_primal(2, Cg)
+    assert rel(Cg, Co) <= SHORT
+    for i in range(3):
+        assert rel(cc.c_getsoln(i), o.getsoln(i)) <= SHORT
+
+
+def test_laminar_start_with_rand(cc):
+    print("case: laminar + rand() start", flush=True)
+    c = SMALL
+    o = pyorc.Oracle(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"], 3)
+    cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"], 3)
+    pyorc.load().orc_srand(1)
+    cc.sc_srand(1)
+    o.primal(0, None)
+    cc.c_primal(0, np.zeros([0, 0, 0, 0], complex))
+    for i in range(4):
+        assert rel(cc.c_getsoln(i), o.getsoln(i)) <= SHORT
+
+
+def test_tangent_homogeneous_and_inhomogeneous(cc):
+    print("case: tangent 4 steps", flush=True)
+    o, C0 = start(cc, SMALL, 4, amp=0.1)
+    o.primal(0, C0.copy())
+    cc.c_primal(0, C0.copy())
+    for inhomo in (0, 1):
+        d = direction(SMALL, 1 + inhomo)
+        Io, Ig = d.copy(), d.copy()
+        o.tangent(0, 4, Io, inhomo)
+        cc.c_tangent(0, 4, Ig, inhomo)
+        assert rel(Ig, Io) <= SHORT
+        for i in range(5):
+            assert rel(cc.c_getincresoln(i), o.getincresoln(i)) <= SHORT
+    d = direction(SMALL, 5)
+    Io, Ig = d.copy(), d.copy()
+    o.tangent(1, 3, Io, 0)
+    cc.c_tangent(1, 3, Ig, 0)
+    assert rel(Ig, Io) <= SHORT
+
+
+def test_adjoint_with_and_without_forcing(cc):
+    print("case: adjoint 4 steps", flush=True)
+    o, C0 = start(cc, SMALL, 4, amp=0.1)
+    o.primal(0, C0.copy())
+    cc.c_primal(0, C0.copy())
+    d = direction(SMALL, 3)
+    o.tangent(0, 4, d.copy(), 0)
+    cc.c_tangent(0, 4, d.copy(), 0)
+    for strength in (0.0, 1.0):
+        a = direction(SMALL, 4)
+        Ao, Ag = a.copy(), a.copy()
+        o.adjoint(4, 0, Ao, 0, strength)
+        cc.c_adjoint(4, 0, Ag, 0, strength)
+        assert rel(Ag, Ao) <= SHORT
+
+
+def test_ddt_project_and_spec2phys(cc):
+    print("case: ddt_project and spec2phys", flush=True)
+    o, C0 = start(cc, SMALL, 3, amp=0.1)
+    o.primal(0, C0.copy())
+    cc.c_primal(0, C0.copy())
+    for i in (0, 2, 3):
+        v = direction(SMALL, 10 + i)
+        vo, vg = v.copy(), v.copy()
+        ro = o.ddt_project(i, vo)
+        rg = cc.c_ddt_project(i, vg)
+        assert abs(rg - ro) <= SHORT * max(1.0, abs(ro))
+        assert rel(vg, vo) <= SHORT
+    flow = np.zeros((3, 3 * SMALL["Nx"] // 2, 3 * SMALL["Ny"] // 2, 3 * SMALL["Nz"] // 2))
+    cc.c_spec2phys(C0, flow)
+    fo = o.spec2phys(C0)
+    assert rel(flow, fo) <= SHORT
+
+
+def test_duality_on_device(cc):
+    print("case: tangent/adjoint duality", flush=True)
+    n = 5
+    o, C0 = start(cc, SMALL, n, amp=0.1)
+    cc.c_primal(0, C0.copy())
+    y, w = cc.c_quad()
+    IC0, AC0 = direction(SMALL, 20), direction(SMALL, 21)
+    IC, AC = IC0.copy(), AC0.copy()
+    cc.c_tangent(0, n, IC, 0)
+    cc.c_adjoint(n, 0, AC, 0, 0.0)
+
+    def phys(C):
+        f = np.zeros((3, 3 * SMALL["Nx"] // 2, len(y), 3 * SMALL["Nz"] // 2))
+        cc.c_spec2phys(np.ascontiguousarray(C), f)
+        return f
+
+    left = np.sum(phys(AC0) * phys(IC) * w[None, None, :, None])
+    right = np.sum(phys(AC) * phys(IC0) * w[None, None, :, None])
+    assert abs(left - right) <= 1e-10 * abs(left)
+
+
+def test_cfl_violation_stops_the_run(cc):
+    print("case: CFL violation", flush=True)
+    c = dict(SMALL)
+    c["dt"] = 5.0
+    o, C0 = start(cc, c, 3, amp=0.1)
+    Cg = C0.copy()
+    cc.c_primal(0, Cg)
+    assert cc.last_status() == -1
+    np.testing.assert_array_equal(Cg, C0)
+    assert np.all(cc.c_getsoln(1) == 0)
+
+
+def test_invalid_calls_raise(cc):
+    print("case: invalid calls", flush=True)
+    o, C0 = start(cc, SMALL, 2)
+    cc.c_primal(0, C0.copy())
+    with pytest.raises(AssertionError):
+        cc.c_tangent(1, 1, C0.copy(), 0)
+    with pytest.raises(AssertionError):
+        cc.c_adjoint(0, 1, C0.copy(), 0, 0.0)
+    with pytest.raises(AssertionError):
+        cc.c_primal(0, np.zeros((SMALL["Nz"], 2, SMALL["Ny"] - 2, 3), complex))
