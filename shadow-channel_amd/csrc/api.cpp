@@ -462,9 +462,8 @@ int c_primal(int ru_steps, sc_complex *C_given, int Nz_dup, int Nvar_dup, int Ny
             long count = (long) (n - ru_steps) * 3 + k + 1;
             cplx *Cs = count >= 0 ? mc(c, count) : c.Csol;
             launch_force(c, c.U, 0);
-            launch_solve(c, c.C, Cs, k, 0, c.Fa, c.Fb);
-            launch_mean(c, c.C, Cs, c.U, nullptr, c.U, k, 0, d.flux);
-            launch_rebuild(c, Cs, c.U, false);
+            launch_solve(c, c.C, Cs, k, 0, c.Fa, c.Fb, d.flux);
+            launch_rebuild(c, Cs, c.U);
         }
         if ((n & 63) == 63) {
             int flag = 0;
@@ -504,9 +503,8 @@ int c_tangent(int start, int end, sc_complex *IC_given, int Nz_dup, int Nvar_dup
             long count = 3L * n + k + 1;
             cplx *Cs = mic(c, count);
             launch_force(c, c.IU, 0);
-            launch_solve(c, c.IC, Cs, k, 0, c.Fa, c.Fb);
-            launch_mean(c, c.IC, Cs, c.IU, nullptr, c.IU, k, 1, target);
-            launch_rebuild(c, Cs, c.IU, false);
+            launch_solve(c, c.IC, Cs, k, 0, c.Fa, c.Fb, target);
+            launch_rebuild(c, Cs, c.IU);
         }
         if (((n - start) & 63) == 63) {
             int flag = 0;
@@ -552,9 +550,8 @@ int c_adjoint(int start, int end, sc_complex *AC_given, int Nz_dup, int Nvar_dup
             pass2_adjoint(c, c.U, c.AU, c.U);
             launch_force(c, c.U, 1);
             if (k < 2) launch_force(c, c.LU, 1, c.FLa, c.FLb);
-            launch_solve(c, c.AC, c.Csol, k, 1, c.FLa, c.FLb);
-            launch_mean(c, c.AC, c.Csol, c.U, c.LU, c.AU, k, 2, 0.0);
-            launch_rebuild(c, c.Csol, c.AU, false);
+            launch_solve(c, c.AC, c.Csol, k, 1, c.FLa, c.FLb, 0.0);
+            launch_rebuild(c, c.Csol, c.AU);
         }
         SC_HIP(hipMemcpyAsync(c.Ctmp, mic(c, 3L * (n - 1)), cb, hipMemcpyDeviceToDevice, c.stream));
         ddt_project_dev(c, n - 1, c.Ctmp);

@@ -114,21 +114,37 @@ __global__ __launch_bounds__(256) void k_rebuild(RebuildArgs a, const cplx *C, c
 }
 
 // mean mode (kx,kz) = (0,0): u = R a, du = Rp a, v = 0, w = R b, dw = Rp b
-__global__ void k_rebuild_mean(RebuildArgs a, const cplx *C, cplx *U)
+// (channel/initAlphaBeta2.c:22-68).  Block: 32 rows x 8 lanes splitting j.
+__global__ __launch_bounds__(256) void k_rebuild_mean(RebuildArgs a, const cplx *C, cplx *U)
 {
+    __shared__ double2 al[512], be[512];
     if (*a.abort_flag) return;
     const int Mx = a.Mx, q = a.qpts, nR = a.dimR;
     const size_t ss = (size_t) q * Mx;
-    for (int i = threadIdx.x; i < q; i += blockDim.x) {
-        double2 u = make_double2(0, 0), du = u, w = u, dw = u;
-        for (int j = 0; j < nR; ++j) {
-            double2 al = C[(size_t) j * Mx], be = C[(size_t) (nR + j) * Mx];
+    for (int j = threadIdx.x; j < nR; j += blockDim.x) {
+        al[j] = C[(size_t) j * Mx];
+        be[j] = C[(size_t) (nR + j) * Mx];
+    }
+    __syncthreads();
+    const int i = blockIdx.x * 32 + threadIdx.x / 8, lane = threadIdx.x % 8;
+    double2 u = make_double2(0, 0), du = u, w = u, dw = u;
+    if (i < q) {
+        for (int j = lane; j < nR; j += 8) {
             double r = a.R[(size_t) i * nR + j], rp = a.Rp[(size_t) i * nR + j];
-            u.x += r * al.x;  u.y += r * al.y;
-            du.x += rp * al.x; du.y += rp * al.y;
-            w.x += r * be.x;  w.y += r * be.y;
-            dw.x += rp * be.x; dw.y += rp * be.y;
+            u.x += r * al[j].x;  u.y += r * al[j].y;
+            du.x += rp * al[j].x; du.y += rp * al[j].y;
+            w.x += r * be[j].x;  w.y += r * be[j].y;
+            dw.x += rp * be[j].x; dw.y += rp * be[j].y;
         }
+    }
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) {
+        u.x += __shfl_xor(u.x, o, 8);   u.y += __shfl_xor(u.y, o, 8);
+        du.x += __shfl_xor(du.x, o, 8); du.y += __shfl_xor(du.y, o, 8);
+        w.x += __shfl_xor(w.x, o, 8);   w.y += __shfl_xor(w.y, o, 8);
+        dw.x += __shfl_xor(dw.x, o, 8); dw.y += __shfl_xor(dw.y, o, 8);
+    }
+    if (i < q && lane == 0) {
         size_t o = (size_t) i * Mx;
         U[o + 0 * ss] = u;
         U[o + 1 * ss] = make_double2(0.0, 0.0);
@@ -149,7 +165,8 @@ void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst, bool with_mean)
     hipLaunchKernelGGL(k_rebuild, grid, dim3(256), 0, c.stream, a, Csrc, Udst);
     prof_end(c, c.stream, 4);
     if (with_mean)
-        hipLaunchKernelGGL(k_rebuild_mean, dim3(1), dim3(256), 0, c.stream, a, Csrc, Udst);
+        hipLaunchKernelGGL(k_rebuild_mean, dim3((d.qpts + 31) / 32), dim3(256), 0, c.stream, a,
+                           Csrc, Udst);
 }
 
 // ------------------------------------------------------------ projection
@@ -209,13 +226,24 @@ __global__ __launch_bounds__(256) void k_force(ForceArgs a, const cplx *U, cplx 
                 double kx = a.Kx[xg];
                 size_t o = (size_t) j * Mx + xg;
                 double2 hz = Uz[o], hx = Uz[o + ss], hy = Uz[o + 2 * ss];
-                P = make_double2(-kx * hx.y - kz * hz.y, kx * hx.x + kz * hz.x);
-                Y = hy;
-                P2 = make_double2(-kz * hx.y + kx * hz.y, kz * hx.x - kx * hz.x);
-                if (a.adj) {
-                    double2 sx = Uz[o + 3 * ss], sz = Uz[o + 4 * ss];
-                    Z = make_double2(kz * sx.y - kx * sz.y, -kz * sx.x + kx * sz.x);
-                    Z2 = make_double2(-kz * sz.y - kx * sx.y, kz * sz.x + kx * sx.x);
+                if (z == 0 && xg == 0) {
+                    // mean mode (channel/project0.c:110-146, adjproject0.c:98-140):
+                    // Rw Re(Hx) and Rw Re(Hz) packed as one complex column,
+                    // Rpw Re(S_z) and -Rpw Re(S_x) likewise for the adjoint
+                    P2 = make_double2(hx.x, hz.x);
+                    if (a.adj) {
+                        double2 sx = Uz[o + 3 * ss], sz = Uz[o + 4 * ss];
+                        Z2 = make_double2(sz.x, -sx.x);
+                    }
+                } else {
+                    P = make_double2(-kx * hx.y - kz * hz.y, kx * hx.x + kz * hz.x);
+                    Y = hy;
+                    P2 = make_double2(-kz * hx.y + kx * hz.y, kz * hx.x - kx * hz.x);
+                    if (a.adj) {
+                        double2 sx = Uz[o + 3 * ss], sz = Uz[o + 4 * ss];
+                        Z = make_double2(kz * sx.y - kx * sz.y, -kz * sx.x + kx * sz.x);
+                        Z2 = make_double2(-kz * sz.y - kx * sx.y, kz * sz.x + kx * sx.x);
+                    }
                 }
             }
             Bs[0][jj][xx] = P;
@@ -259,12 +287,13 @@ __global__ __launch_bounds__(256) void k_force(ForceArgs a, const cplx *U, cplx 
         __syncthreads();
     }
     const int x = x0 + tx;
-    if (x >= Mx || (z == 0 && x == 0)) return;
+    if (x >= Mx) return;
+    const bool mean = z == 0 && x == 0;   // Fb(0,0) = (fa, fb) of the mean mode
     const double kx = a.Kx[x], k2 = kz * kz + kx * kx;
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
         int i = i0 + ty + 16 * r;
-        if (i < nQ) {
+        if (i < nQ && !mean) {
             double2 fa = make_double2(acc[r][0].x - k2 * acc[r][1].x + acc[r][3].x,
                                       acc[r][0].y - k2 * acc[r][1].y + acc[r][3].y);
             Fa[((size_t) z * nQ + i) * Mx + x] = fa;
@@ -292,14 +321,36 @@ void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb)
 void launch_force(Context &c, const cplx *Usrc, int adj) { launch_force(c, Usrc, adj, c.Fa, c.Fb); }
 
 // ------------------------------------------------------------ band solves
+// Stage solve of every (kz, kx) column (channel/project.c:22-151,
+// channel/increproject.c:13-194, channel/adjproject.c:9-226): the right-hand
+// side E C + dt c_k F (E applied only at the first stage, where C still holds
+// the state), the LU solve with the stage's implicit operator, then
+// C <- E' X + dt d_{k+1} F2, the next stage's carried right-hand side.  The
+// even and odd rows of the stride-2 bands decouple, so one chain is one
+// (column, field, parity).
+//
+// The LU factors are formed once at init with the pivots inverted, so a chain
+// is a pure multiply-add recurrence.  A block stages the factors and vectors
+// of SBT columns of one (field, parity) in LDS with all its threads, runs the
+// SBT chains with one thread each (the last DL solved values in registers),
+// and writes back with all its threads.  The loops over rows stay rolled: the
+// kernel fits the instruction cache at every grid size.
+//
+// The mean mode (kz = kx = 0; channel/project0.c:22-271,
+// channel/increproject0.c:14-312, channel/adjproject0.c:8-310) runs in one
+// extra block: real-part-only forcing fa = Re(Hx)-projection and
+// fb = Re(Hz)-projection, read as the pair Fb(0,0) = (fa, fb); the flux
+// correction alpha += (target - sum_i rowsum_i Re alpha_i) add / 2.
 struct SolveArgs {
     const double *Qs, *Qps, *Qpps, *Rs, *Rps, *Kx, *Kz;
-    const double *La, *Lb;             // factors of this stage
+    const double *La, *Lb;             // factors of this stage, [z][row][2DL+1][x]
+    const double *Lm, *add, *rowsum;   // mean mode: factors [row][5], flux vector, row sums
     const cplx *Fa, *Fb;               // projection for this stage's RHS
     const cplx *Fa2, *Fb2;             // projection for the next-stage RHS
     int Nz, Mx, dimR, dimQ, k, has_next;
     double dtc, dtd;                   // dt c_k, dt d_{k+1}
     double ce0[2], cn[2];              // explicit-operator coefficients per field (stage 0, next)
+    double target;                     // mean-mode flux target
     const int *abort_flag;
 };
 
@@ -315,99 +366,298 @@ __device__ __forceinline__ double op_entry(const SolveArgs &a, int var, int i, i
     return a.Rs[e] + coef * (a.Rps[e] + k2 * a.Rs[e]);
 }
 
-// grid (ceil(Nz*Mx/256), 4): blockIdx.y = field*2 + parity
+// Forward and back substitution of one chain held in LDS.  Factors of row r:
+// f[r*fr + j*fj], j = DL - m for L(r, r-m), j = DL for 1/U(r, r),
+// j = DL + u for U(r, r+u); entries that fall outside the matrix are zero.
+// v: right-hand side in, solution out, row r at v[r*vr].  The next row's
+// operands are fetched while the current row is eliminated.
+template <int DL>
+__device__ void chain_subst(const double *f, int fr, int fj, double2 *v, int vr, int nr)
+{
+    if (nr <= 0) return;
+    double2 w[DL];
+#pragma unroll
+    for (int m = 0; m < DL; ++m) w[m] = make_double2(0.0, 0.0);
+    double cf[DL], nf[DL];
+    double2 nv = v[0];
+#pragma unroll
+    for (int m = 1; m <= DL; ++m) nf[m - 1] = f[(DL - m) * fj];
+    for (int r = 0; r < nr; ++r) {
+        double2 x = nv;
+#pragma unroll
+        for (int m = 0; m < DL; ++m) cf[m] = nf[m];
+        if (r + 1 < nr) {
+            nv = v[(r + 1) * vr];
+#pragma unroll
+            for (int m = 1; m <= DL; ++m) nf[m - 1] = f[(r + 1) * fr + (DL - m) * fj];
+        }
+#pragma unroll
+        for (int m = DL; m >= 1; --m) {
+            x.x -= cf[m - 1] * w[m - 1].x;
+            x.y -= cf[m - 1] * w[m - 1].y;
+        }
+#pragma unroll
+        for (int m = DL - 1; m > 0; --m) w[m] = w[m - 1];
+        w[0] = x;
+        v[r * vr] = x;
+    }
+#pragma unroll
+    for (int m = 0; m < DL; ++m) w[m] = make_double2(0.0, 0.0);
+    double crd, nrd;
+    {
+        const int r = nr - 1;
+        nv = v[r * vr];
+        nrd = f[r * fr + DL * fj];
+#pragma unroll
+        for (int u = 1; u <= DL; ++u) nf[u - 1] = f[r * fr + (DL + u) * fj];
+    }
+    for (int r = nr - 1; r >= 0; --r) {
+        double2 x = nv;
+        crd = nrd;
+#pragma unroll
+        for (int u = 0; u < DL; ++u) cf[u] = nf[u];
+        if (r > 0) {
+            nv = v[(r - 1) * vr];
+            nrd = f[(r - 1) * fr + DL * fj];
+#pragma unroll
+            for (int u = 1; u <= DL; ++u) nf[u - 1] = f[(r - 1) * fr + (DL + u) * fj];
+        }
+#pragma unroll
+        for (int u = 1; u <= DL; ++u) {
+            x.x -= cf[u - 1] * w[u - 1].x;
+            x.y -= cf[u - 1] * w[u - 1].y;
+        }
+        x.x *= crd;
+        x.y *= crd;
+#pragma unroll
+        for (int u = DL - 1; u > 0; --u) w[u] = w[u - 1];
+        w[0] = x;
+        v[r * vr] = x;
+    }
+}
+
+struct ColumnOf {
+    int z, x;
+    bool valid, chain;
+};
+
+__device__ __forceinline__ ColumnOf column_of(const SolveArgs &a, int col)
+{
+    ColumnOf c;
+    c.valid = col < a.Nz * a.Mx;
+    c.z = c.valid ? col / a.Mx : 0;
+    c.x = c.valid ? col - c.z * a.Mx : 0;
+    c.chain = c.valid && c.z != a.Nz / 2 && !(c.z == 0 && c.x == 0);
+    return c;
+}
+
+// one (field, parity) of SBT consecutive columns; lds: [nr][W][SBT] factors,
+// then [nr][SBT] right-hand side, then [nr][SBT] stage-0 copy of C
+template <int DL, int SBT>
+__device__ void solve_block(const SolveArgs &a, int var, int par, int col0, cplx *C, cplx *Cs,
+                            double *lds)
+{
+    constexpr int W = 2 * DL + 1;
+    const int Mx = a.Mx, nR = a.dimR, nQ = a.dimQ;
+    const int N = var == 0 ? nQ : nR;
+    const int nr = (N - par + 1) / 2;
+    const int T = blockDim.x, tid = threadIdx.x;
+    double *lf = lds;
+    double2 *vec = (double2 *) (lds + (size_t) nr * W * SBT);
+    double2 *raw = vec + nr * SBT;
+    const double *Lg = var == 0 ? a.La : a.Lb;
+    const cplx *F = var == 0 ? a.Fa : a.Fb;
+    const cplx *F2 = var == 0 ? a.Fa2 : a.Fb2;
+    const double ce0 = a.ce0[var], cn = a.cn[var];
+
+    // stage the factors
+    const int nfac = nr * W * SBT;
+#pragma unroll 4
+    for (int e = tid; e < nfac; e += T) {
+        const int cc = e % SBT, q = e / SBT, r = q / W, j = q - r * W;
+        const ColumnOf co = column_of(a, col0 + cc);
+        lf[e] = co.chain ? Lg[(((size_t) co.z * N + par + 2 * r) * W + j) * Mx + co.x] : 0.0;
+    }
+    // stage the carried right-hand side (plus the projection after stage 0)
+    const int nvec = nr * SBT;
+#pragma unroll 2
+    for (int e = tid; e < nvec; e += T) {
+        const int cc = e % SBT, r = e / SBT, i = par + 2 * r;
+        const ColumnOf co = column_of(a, col0 + cc);
+        if (!co.chain) continue;
+        double2 cv = C[(((size_t) co.z * 2 + var) * nR + i) * Mx + co.x];
+        if (a.k == 0) {
+            raw[e] = cv;
+        } else {
+            double2 f = F[((size_t) co.z * N + i) * Mx + co.x];
+            cv.x += a.dtc * f.x;
+            cv.y += a.dtc * f.y;
+            vec[e] = cv;
+        }
+    }
+    __syncthreads();
+    if (a.k == 0) {
+        for (int e = tid; e < nvec; e += T) {
+            const int cc = e % SBT, r = e / SBT, i = par + 2 * r;
+            const ColumnOf co = column_of(a, col0 + cc);
+            if (!co.chain) continue;
+            const double kx = a.Kx[co.x], kz = a.Kz[co.z], k2 = kz * kz + kx * kx;
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int m = -DL; m <= DL; ++m)
+                if (r + m >= 0 && r + m < nr) {
+                    double ev = op_entry(a, var, i, m, k2, ce0);
+                    double2 y = raw[(r + m) * SBT + cc];
+                    acc.x += ev * y.x;
+                    acc.y += ev * y.y;
+                }
+            double2 f = F[((size_t) co.z * N + i) * Mx + co.x];
+            acc.x += a.dtc * f.x;
+            acc.y += a.dtc * f.y;
+            vec[e] = acc;
+        }
+        __syncthreads();
+    }
+    if (tid < SBT && column_of(a, col0 + tid).chain)
+        chain_subst<DL>(lf + tid, W * SBT, SBT, vec + tid, SBT, nr);
+    __syncthreads();
+    // write the solution and the next stage's right-hand side
+    for (int e = tid; e < nvec; e += T) {
+        const int cc = e % SBT, r = e / SBT, i = par + 2 * r;
+        const ColumnOf co = column_of(a, col0 + cc);
+        if (!co.chain) continue;
+        const size_t gi = (((size_t) co.z * 2 + var) * nR + i) * Mx + co.x;
+        const double2 sol = vec[e];
+        Cs[gi] = sol;
+        if (a.has_next) {
+            const double kx = a.Kx[co.x], kz = a.Kz[co.z], k2 = kz * kz + kx * kx;
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int m = -DL; m <= DL; ++m)
+                if (r + m >= 0 && r + m < nr) {
+                    double ev = op_entry(a, var, i, m, k2, cn);
+                    double2 y = vec[(r + m) * SBT + cc];
+                    acc.x += ev * y.x;
+                    acc.y += ev * y.y;
+                }
+            double2 f = F2[((size_t) co.z * N + i) * Mx + co.x];
+            acc.x += a.dtd * f.x;
+            acc.y += a.dtd * f.y;
+            C[gi] = acc;
+        } else {
+            C[gi] = sol;
+        }
+    }
+    // rows of v beyond dimQ are carried unchanged; the Nyquist row is zeroed
+    const int nrest = (nR - par + 1) / 2;
+    for (int e = tid; e < nrest * SBT; e += T) {
+        const int cc = e % SBT, r = e / SBT, i = par + 2 * r;
+        const ColumnOf co = column_of(a, col0 + cc);
+        if (!co.valid || (co.z == 0 && co.x == 0)) continue;
+        const size_t gi = (((size_t) co.z * 2 + var) * nR + i) * Mx + co.x;
+        if (co.z == a.Nz / 2)
+            C[gi] = make_double2(0.0, 0.0);
+        else if (var == 0 && i >= nQ)
+            Cs[gi] = C[gi];
+    }
+}
+
+// explicit mean operator Rs + coef Rps applied to v (row i)
+__device__ double2 mean_mult(const SolveArgs &a, const double2 *v, int i, double coef)
+{
+    const int N = a.dimR;
+    double2 t = make_double2(0.0, 0.0);
+    int m0 = -min(2, i / 2), m1 = min(2, (N - i + 1) / 2 - 1);
+    for (int m = m0; m <= m1; ++m) {
+        double c = op_entry(a, 1, i, m, 0.0, coef);
+        t.x += c * v[i + 2 * m].x;
+        t.y += c * v[i + 2 * m].y;
+    }
+    return t;
+}
+
+// the mean mode: alpha (streamwise mean) and beta (spanwise mean) columns
+__device__ void solve_mean(const SolveArgs &a, cplx *C, cplx *Cs, double *lds)
+{
+    const int nR = a.dimR, Mx = a.Mx, T = blockDim.x, tid = threadIdx.x;
+    double2 *al = (double2 *) lds, *be = al + nR, *tmp = be + nR;
+    double *lm = (double *) (tmp + nR), *red = lm + 5 * nR;
+    cplx *Ca = C, *Cb = C + (size_t) nR * Mx;
+    for (int i = tid; i < nR; i += T) {
+        al[i] = Ca[(size_t) i * Mx];
+        be[i] = Cb[(size_t) i * Mx];
+    }
+    for (int e = tid; e < 5 * nR; e += T) lm[e] = a.Lm[e];
+    __syncthreads();
+    if (a.k == 0) {
+        for (int i = tid; i < nR; i += T) tmp[i] = mean_mult(a, al, i, a.ce0[1]);
+        __syncthreads();
+        for (int i = tid; i < nR; i += T) al[i] = tmp[i];
+        __syncthreads();
+        for (int i = tid; i < nR; i += T) tmp[i] = mean_mult(a, be, i, a.ce0[1]);
+        __syncthreads();
+        for (int i = tid; i < nR; i += T) be[i] = tmp[i];
+        __syncthreads();
+    }
+    const cplx *Fm = a.Fb, *Fm2 = a.Fb2;   // column (0, 0): (fa, fb)
+    for (int i = tid; i < nR; i += T) {
+        double2 f = Fm[(size_t) i * Mx];
+        al[i].x += a.dtc * f.x;
+        be[i].x += a.dtc * f.y;
+    }
+    __syncthreads();
+    const int n0 = (nR + 1) / 2, n1 = nR / 2;
+    if (tid == 0) chain_subst<2>(lm, 10, 1, al, 2, n0);
+    if (tid == 64) chain_subst<2>(lm + 5, 10, 1, al + 1, 2, n1);
+    if (tid == 128) chain_subst<2>(lm, 10, 1, be, 2, n0);
+    if (tid == 192) chain_subst<2>(lm + 5, 10, 1, be + 1, 2, n1);
+    __syncthreads();
+    double p = 0.0;
+    for (int i = tid; i < nR; i += T) p += a.rowsum[i] * al[i].x;
+    red[tid] = p;
+    __syncthreads();
+    for (int o = T / 2; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    const double flux_t = a.target - red[0];
+    for (int i = tid; i < nR; i += T) {
+        al[i].x = al[i].x + flux_t * a.add[i] * 0.5;
+        al[i].y = al[i].y + flux_t * 0.0 * 0.5;
+        Cs[(size_t) i * Mx] = al[i];
+        Cs[(size_t) (nR + i) * Mx] = be[i];
+    }
+    __syncthreads();
+    for (int i = tid; i < nR; i += T) {
+        if (a.has_next) {
+            double2 f = Fm2[(size_t) i * Mx];
+            double2 ta = mean_mult(a, al, i, a.cn[1]), tb = mean_mult(a, be, i, a.cn[1]);
+            Ca[(size_t) i * Mx] = make_double2(ta.x + a.dtd * f.x, ta.y + a.dtd * 0.0);
+            Cb[(size_t) i * Mx] = make_double2(tb.x + a.dtd * f.y, tb.y + a.dtd * 0.0);
+        } else {
+            Ca[(size_t) i * Mx] = al[i];
+            Cb[(size_t) i * Mx] = be[i];
+        }
+    }
+}
+
+// grid (ceil(Nz*Mx/SBT) + 1, 4): blockIdx.y = field*2 + parity; the last
+// x-block with blockIdx.y == 0 is the mean mode.  Block of 256 threads.
+template <int SBT>
 __global__ __launch_bounds__(256) void k_solve(SolveArgs a, cplx *C, cplx *Cs)
 {
+    extern __shared__ double lds_solve[];
     if (*a.abort_flag) return;
-    const int col = blockIdx.x * 256 + threadIdx.x;
-    if (col >= a.Nz * a.Mx) return;
-    const int z = col / a.Mx, x = col - z * a.Mx;
-    const int var = blockIdx.y >> 1, par = blockIdx.y & 1;
-    const int Mx = a.Mx, nR = a.dimR, nQ = a.dimQ;
-    cplx *Cc = C + ((size_t) (z * 2 + var) * nR) * Mx + x;
-    cplx *Co = Cs + ((size_t) (z * 2 + var) * nR) * Mx + x;
-    if (z == a.Nz / 2) {
-        for (int i = par; i < nR; i += 2) Cc[(size_t) i * Mx] = make_double2(0.0, 0.0);
+    if (blockIdx.x == gridDim.x - 1) {
+        if (blockIdx.y == 0) solve_mean(a, C, Cs, lds_solve);
         return;
     }
-    if (z == 0 && x == 0) return;
-    const int N = var == 0 ? nQ : nR, dl = var == 0 ? 4 : 2, W = 2 * dl + 1;
-    const double kx = a.Kx[x], kz = a.Kz[z], k2 = kz * kz + kx * kx;
-    const cplx *F = var == 0 ? a.Fa + ((size_t) z * nQ) * Mx + x : a.Fb + ((size_t) z * nR) * Mx + x;
-    const cplx *F2 = var == 0 ? a.Fa2 + ((size_t) z * nQ) * Mx + x : a.Fb2 + ((size_t) z * nR) * Mx + x;
-    const double *L = (var == 0 ? a.La : a.Lb) + ((size_t) z * N * W) * Mx + x;
-    // alpha rows dimQ..dimR-1 pass through unchanged (channel/project.c:133)
+    const int var = blockIdx.y >> 1, par = blockIdx.y & 1;
+    const int col0 = blockIdx.x * SBT;
     if (var == 0)
-        for (int i = nQ + ((nQ & 1) != par); i < nR; i += 2) Co[(size_t) i * Mx] = Cc[(size_t) i * Mx];
-    // right-hand side
-    for (int i = par; i < N; i += 2) {
-        double2 t;
-        if (a.k == 0) {
-            t = make_double2(0.0, 0.0);
-            int m0 = -min(dl, i / 2), m1 = min(dl, (N - i + 1) / 2 - 1);
-            for (int m = m0; m <= m1; ++m) {
-                double e = op_entry(a, var, i, m, k2, a.ce0[var]);
-                double2 yv = Cc[(size_t) (i + 2 * m) * Mx];
-                t.x += e * yv.x;
-                t.y += e * yv.y;
-            }
-        } else {
-            t = Cc[(size_t) i * Mx];
-        }
-        double2 f = F[(size_t) i * Mx];
-        t.x += a.dtc * f.x;
-        t.y += a.dtc * f.y;
-        Co[(size_t) i * Mx] = t;
-    }
-    // forward substitution with unit L
-    for (int i = par; i < N; i += 2) {
-        if (i < 2) continue;
-        double2 b = Co[(size_t) i * Mx];
-        int mmax = min(dl, i / 2);
-        for (int m = mmax; m >= 1; --m) {
-            double l = L[((size_t) i * W + dl - m) * Mx];
-            double2 p = Co[(size_t) (i - 2 * m) * Mx];
-            b.x -= l * p.x;
-            b.y -= l * p.y;
-        }
-        Co[(size_t) i * Mx] = b;
-    }
-    // back substitution
-    int ilast = (N - 1) - (((N - 1) - par) & 1);
-    for (int i = ilast; i >= 0; i -= 2) {
-        double2 b = Co[(size_t) i * Mx];
-        int jmax = min(dl, (N - i + 1) / 2 - 1);
-        for (int j = 1; j <= jmax; ++j) {
-            double u = L[((size_t) i * W + dl + j) * Mx];
-            double2 p = Co[(size_t) (i + 2 * j) * Mx];
-            b.x -= u * p.x;
-            b.y -= u * p.y;
-        }
-        double dg = L[((size_t) i * W + dl) * Mx];
-        b.x /= dg;
-        b.y /= dg;
-        Co[(size_t) i * Mx] = b;
-    }
-    // explicit part of the next stage, or the solution after the last stage
-    for (int i = par; i < N; i += 2) {
-        double2 t;
-        if (a.has_next) {
-            t = make_double2(0.0, 0.0);
-            int m0 = -min(dl, i / 2), m1 = min(dl, (N - i + 1) / 2 - 1);
-            for (int m = m0; m <= m1; ++m) {
-                double e = op_entry(a, var, i, m, k2, a.cn[var]);
-                double2 yv = Co[(size_t) (i + 2 * m) * Mx];
-                t.x += e * yv.x;
-                t.y += e * yv.y;
-            }
-            double2 f = F2[(size_t) i * Mx];
-            t.x += a.dtd * f.x;
-            t.y += a.dtd * f.y;
-        } else {
-            t = Co[(size_t) i * Mx];
-        }
-        Cc[(size_t) i * Mx] = t;
-    }
+        solve_block<4, SBT>(a, 0, par, col0, C, Cs, lds_solve);
+    else
+        solve_block<2, SBT>(a, 1, par, col0, C, Cs, lds_solve);
 }
 
 // coefficient tables for the stages
@@ -418,8 +668,23 @@ static const double BA_[3] = {1. / 3., -1. / 2, 1. / 3.};
 static const double BC_[3] = {1., 1. / 3., 1. / 2.};
 static const double BD_[3] = {0., -2. / 3, -1. / 6};
 
+size_t solve_lds_bytes(int dimR, int sbt)
+{
+    const size_t nr = (size_t) (dimR + 1) / 2;
+    const size_t blk = nr * sbt * (9 * 8 + 16 + 16);
+    const size_t mean = (size_t) dimR * (3 * 16 + 5 * 8) + 256 * 8;
+    return blk > mean ? blk : mean;
+}
+
+int solve_columns_per_block(int dimR)
+{
+    for (int sbt = 8; sbt > 1; sbt >>= 1)
+        if (solve_lds_bytes(dimR, sbt) <= 48 * 1024) return sbt;
+    return 1;
+}
+
 void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cplx *Fa2,
-                  const cplx *Fb2)
+                  const cplx *Fb2, double target)
 {
     const Dims &d = c.d;
     const double *A = adj ? BA_ : FA_, *Cc = adj ? BC_ : FC_, *D = adj ? BD_ : FD_;
@@ -428,6 +693,9 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
     a.Kx = c.dKx; a.Kz = c.dKz;
     a.La = c.dLa[adj][k];
     a.Lb = c.dLb[adj][k];
+    a.Lm = c.dLm[adj][k];
+    a.add = c.dAdd[adj][k];
+    a.rowsum = c.dRowsum;
     a.Fa = c.Fa; a.Fb = c.Fb;
     a.Fa2 = Fa2; a.Fb2 = Fb2;
     a.Nz = d.Nz; a.Mx = d.Mx; a.dimR = d.dimR; a.dimQ = d.dimQ; a.k = k;
@@ -440,199 +708,26 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
     a.ce0[1] = -(d.re * A[0] * d.dt);
     a.cn[0] = k != 2 ? d.re * A[k + 1] * d.dt : 0.0;
     a.cn[1] = k != 2 ? -(d.re * A[k + 1] * d.dt) : 0.0;
+    a.target = target;
     a.abort_flag = c.abort_flag;
-    dim3 grid((d.Nz * d.Mx + 255) / 256, 4);
     // RHS in, projections in, solution and next right-hand side out, factors in
     double bytes = (double) (d.Nz - 1) * d.Mx *
                    (16.0 * (2.0 * d.dimR) * 3.0 + 16.0 * (d.dimQ + d.dimR) * (k != 2 ? 2.0 : 1.0) +
                     8.0 * (9.0 * d.dimQ + 5.0 * d.dimR));
     prof_begin(c, c.stream, 6, bytes);
-    hipLaunchKernelGGL(k_solve, grid, dim3(256), 0, c.stream, a, Ccarry, Csol);
+    const int sbt = solve_columns_per_block(d.dimR);
+    const int ncol = d.Nz * d.Mx;
+    const size_t lds = solve_lds_bytes(d.dimR, sbt);
+    dim3 grid((ncol + sbt - 1) / sbt + 1, 4);
+    if (sbt == 8)
+        hipLaunchKernelGGL(k_solve<8>, grid, dim3(256), lds, c.stream, a, Ccarry, Csol);
+    else if (sbt == 4)
+        hipLaunchKernelGGL(k_solve<4>, grid, dim3(256), lds, c.stream, a, Ccarry, Csol);
+    else if (sbt == 2)
+        hipLaunchKernelGGL(k_solve<2>, grid, dim3(256), lds, c.stream, a, Ccarry, Csol);
+    else
+        hipLaunchKernelGGL(k_solve<1>, grid, dim3(256), lds, c.stream, a, Ccarry, Csol);
     prof_end(c, c.stream, 6);
-}
-
-// ------------------------------------------------------------ mean mode
-// (channel/project0.c:22-271, channel/increproject0.c:14-312,
-//  channel/adjproject0.c:8-310)
-struct MeanArgs {
-    const double *Rs, *Rps, *Rw, *Rpw, *R, *Rp, *rowsum;
-    const double *Lm, *add;
-    int Mx, qpts, dimR, k, has_next, adj;
-    double dtc, dtd, ce0, cn, target;
-    const int *abort_flag;
-};
-
-constexpr int MAXR = 512;
-
-__device__ void mean_rhs(const MeanArgs &a, const cplx *Uh, double *fa, double *fb)
-{
-    const int q = a.qpts, nR = a.dimR, Mx = a.Mx;
-    const size_t ss = (size_t) q * Mx;
-    for (int i = threadIdx.x; i < nR; i += blockDim.x) {
-        double sa = 0.0, sb = 0.0;
-        for (int j = 0; j < q; ++j) {
-            size_t o = (size_t) j * Mx;
-            double rw = a.Rw[(size_t) i * q + j];
-            if (a.adj) {
-                double rpw = a.Rpw[(size_t) i * q + j];
-                sa += rw * Uh[o + 1 * ss].x + rpw * Uh[o + 4 * ss].x;
-                sb += rw * Uh[o + 0 * ss].x - rpw * Uh[o + 3 * ss].x;
-            } else {
-                sa += rw * Uh[o + 1 * ss].x;
-                sb += rw * Uh[o + 0 * ss].x;
-            }
-        }
-        fa[i] = sa;
-        fb[i] = sb;
-    }
-}
-
-// explicit mean operator Rs + coef Rps applied to v (row i)
-__device__ double2 mean_mult(const MeanArgs &a, const double2 *v, int i, double coef)
-{
-    const int N = a.dimR;
-    double2 t = make_double2(0.0, 0.0);
-    int m0 = -min(2, i / 2), m1 = min(2, (N - i + 1) / 2 - 1);
-    for (int m = m0; m <= m1; ++m) {
-        int e = i * 5 + 2 + m;
-        double c = a.Rs[e] + coef * a.Rps[e];
-        t.x += c * v[i + 2 * m].x;
-        t.y += c * v[i + 2 * m].y;
-    }
-    return t;
-}
-
-__device__ void mean_subst(const MeanArgs &a, double2 *b)
-{
-    const int N = a.dimR, dl = 2, W = 5;
-    const double *A = a.Lm;
-    for (int i = 2; i < N; ++i) {
-        int mmax = min(dl, i / 2);
-        for (int m = mmax; m >= 1; --m) {
-            double l = A[i * W + dl - m];
-            b[i].x -= l * b[i - 2 * m].x;
-            b[i].y -= l * b[i - 2 * m].y;
-        }
-    }
-    for (int i = N - 1; i >= 0; --i) {
-        int jmax = min(dl, (N - i + 1) / 2 - 1);
-        for (int j = 1; j <= jmax; ++j) {
-            double u = A[i * W + dl + j];
-            b[i].x -= u * b[i + 2 * j].x;
-            b[i].y -= u * b[i + 2 * j].y;
-        }
-        b[i].x /= A[i * W + dl];
-        b[i].y /= A[i * W + dl];
-    }
-}
-
-// one block of 256 threads
-__global__ __launch_bounds__(256) void k_mean(MeanArgs a, cplx *C, cplx *Cs, const cplx *Uh,
-                                              const cplx *Ul, cplx *Uout)
-{
-    __shared__ double2 al[MAXR], be[MAXR], tmp[MAXR];
-    __shared__ double fa[MAXR], fb[MAXR], red[256];
-    if (*a.abort_flag) return;
-    const int nR = a.dimR, Mx = a.Mx, q = a.qpts;
-    const size_t ss = (size_t) q * Mx;
-    cplx *Ca = C, *Cb = C + (size_t) nR * Mx;
-    for (int i = threadIdx.x; i < nR; i += blockDim.x) {
-        al[i] = Ca[(size_t) i * Mx];
-        be[i] = Cb[(size_t) i * Mx];
-    }
-    mean_rhs(a, Uh, fa, fb);
-    __syncthreads();
-    if (a.k == 0) {
-        for (int i = threadIdx.x; i < nR; i += blockDim.x) tmp[i] = mean_mult(a, al, i, a.ce0);
-        __syncthreads();
-        for (int i = threadIdx.x; i < nR; i += blockDim.x) al[i] = tmp[i];
-        __syncthreads();
-        for (int i = threadIdx.x; i < nR; i += blockDim.x) tmp[i] = mean_mult(a, be, i, a.ce0);
-        __syncthreads();
-        for (int i = threadIdx.x; i < nR; i += blockDim.x) be[i] = tmp[i];
-        __syncthreads();
-    }
-    for (int i = threadIdx.x; i < nR; i += blockDim.x) {
-        al[i].x += a.dtc * fa[i];
-        be[i].x += a.dtc * fb[i];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) mean_subst(a, al);
-    if (threadIdx.x == 64) mean_subst(a, be);
-    __syncthreads();
-    // flux correction: alpha += (target - sum_i sum_j Rw[i][j] Re alpha_i) add / 2
-    double p = 0.0;
-    for (int i = threadIdx.x; i < nR; i += blockDim.x) p += a.rowsum[i] * al[i].x;
-    red[threadIdx.x] = p;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    double flux_t = a.target - red[0];
-    __syncthreads();
-    for (int i = threadIdx.x; i < nR; i += blockDim.x) {
-        al[i].x = al[i].x + flux_t * a.add[i] * 0.5;
-        al[i].y = al[i].y + flux_t * 0.0 * 0.5;
-        Cs[(size_t) i * Mx] = al[i];
-        Cs[(size_t) (nR + i) * Mx] = be[i];
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < q; i += blockDim.x) {
-        double2 u = make_double2(0, 0), du = u, w = u, dw = u;
-        for (int j = 0; j < nR; ++j) {
-            double r = a.R[(size_t) i * nR + j], rp = a.Rp[(size_t) i * nR + j];
-            u.x += r * al[j].x;  u.y += r * al[j].y;
-            du.x += rp * al[j].x; du.y += rp * al[j].y;
-            w.x += r * be[j].x;  w.y += r * be[j].y;
-            dw.x += rp * be[j].x; dw.y += rp * be[j].y;
-        }
-        size_t o = (size_t) i * Mx;
-        Uout[o + 0 * ss] = u;
-        Uout[o + 1 * ss] = make_double2(0.0, 0.0);
-        Uout[o + 2 * ss] = w;
-        Uout[o + 3 * ss] = du;
-        Uout[o + 4 * ss] = dw;
-    }
-    if (!a.has_next) {
-        for (int i = threadIdx.x; i < nR; i += blockDim.x) {
-            Ca[(size_t) i * Mx] = al[i];
-            Cb[(size_t) i * Mx] = be[i];
-        }
-        return;
-    }
-    if (a.adj) {
-        __syncthreads();
-        mean_rhs(a, Ul, fa, fb);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < nR; i += blockDim.x) {
-        double2 ta = mean_mult(a, al, i, a.cn), tb = mean_mult(a, be, i, a.cn);
-        Ca[(size_t) i * Mx] = make_double2(ta.x + a.dtd * fa[i], ta.y + a.dtd * 0.0);
-        Cb[(size_t) i * Mx] = make_double2(tb.x + a.dtd * fb[i], tb.y + a.dtd * 0.0);
-    }
-}
-
-void launch_mean(Context &c, cplx *Ccarry, cplx *Csol, const cplx *Uh, const cplx *Ul,
-                 cplx *Uout, int k, int mode, double target)
-{
-    const Dims &d = c.d;
-    const int adj = mode == 2;
-    const double *A = adj ? BA_ : FA_, *Cc = adj ? BC_ : FC_, *D = adj ? BD_ : FD_;
-    MeanArgs a;
-    a.Rs = c.dRs; a.Rps = c.dRps; a.Rw = c.dRw; a.Rpw = c.dRpw; a.R = c.dR; a.Rp = c.dRp;
-    a.rowsum = c.dRowsum;
-    a.Lm = c.dLm[adj][k];
-    a.add = c.dAdd[adj][k];
-    a.Mx = d.Mx; a.qpts = d.qpts; a.dimR = d.dimR; a.k = k; a.has_next = k != 2; a.adj = adj;
-    a.dtc = d.dt * Cc[k];
-    a.dtd = k != 2 ? d.dt * D[k + 1] : 0.0;
-    a.ce0 = -(d.re * A[0] * d.dt);
-    a.cn = k != 2 ? -(d.re * A[k + 1] * d.dt) : 0.0;
-    a.target = target;
-    a.abort_flag = c.abort_flag;
-    hipLaunchKernelGGL(k_mean, dim3(1), dim3(256), 0, c.stream, a, Ccarry, Csol, Uh,
-                       Ul ? Ul : Uh, Uout);
 }
 
 // ------------------------------------------------------------ vectors

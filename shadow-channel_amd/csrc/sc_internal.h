@@ -127,9 +127,7 @@ void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst, bool with_mean = t
 void launch_force(Context &c, const cplx *Usrc, int adj);
 void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb);
 void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cplx *Fa2,
-                  const cplx *Fb2);
-void launch_mean(Context &c, cplx *Ccarry, cplx *Csol, const cplx *Uh, const cplx *Ul,
-                 cplx *Uout, int k, int mode, double target);
+                  const cplx *Fb2, double target);
 void launch_cfl_check(Context &c);
 int xmix_zblock(Context &c, int nf);
 void launch_axpy_c(Context &c, cplx *y, const cplx *x, double a, size_t n);

@@ -87,6 +87,21 @@ void band_factor(double *A, int N, int dl)
     }
 }
 
+// Device form of the factors: the pivot U(i, i) replaced by its reciprocal
+// and the entries that fall outside the matrix set to zero, so the device
+// substitution runs without bounds tests or divisions.
+static void device_factors(double *A, int N, int dl)
+{
+    const int W = 2 * dl + 1;
+    for (int i = 0; i < N; ++i) {
+        A[i * W + dl] = 1.0 / A[i * W + dl];
+        for (int m = 1; m <= dl; ++m) {
+            if (i - 2 * m < 0) A[i * W + dl - m] = 0.0;
+            if (i + 2 * m >= N) A[i * W + dl + m] = 0.0;
+        }
+    }
+}
+
 static void band_subst_real(const double *A, double *b, int N, int dl)
 {
     int W = 2 * dl + 1;
@@ -228,12 +243,14 @@ int setup_host(Context &c)
                         A9[e] = -(k2 * c.Qs[e] + c.Qps[e]) -
                                 coef * (s * c.Qs[e] + 2. * k2 * c.Qps[e] + c.Qpps[e]);
                     band_factor(A9.data(), nQ, 4);
+                    device_factors(A9.data(), nQ, 4);
                     for (int i = 0; i < nQ; ++i)
                         for (int j = 0; j < 9; ++j)
                             La[(((size_t) z * nQ + i) * 9 + j) * Mx + x] = A9[(size_t) i * 9 + j];
                     for (int e = 0; e < 5 * nR; ++e)
                         A5[e] = c.Rs[e] + coef * (c.Rps[e] + k2 * c.Rs[e]);
                     band_factor(A5.data(), nR, 2);
+                    device_factors(A5.data(), nR, 2);
                     for (int i = 0; i < nR; ++i)
                         for (int j = 0; j < 5; ++j)
                             Lb[(((size_t) z * nR + i) * 5 + j) * Mx + x] = A5[(size_t) i * 5 + j];
@@ -246,6 +263,7 @@ int setup_host(Context &c)
             for (int e = 0; e < 5 * nR; ++e) A5[e] = c.Rs[e] + coef * c.Rps[e];
             std::vector<double> Lm(A5);
             band_factor(Lm.data(), nR, 2);
+            device_factors(Lm.data(), nR, 2);
             // primal and tangent use the mass matrix (channel/project0.c:85-98),
             // the adjoint the stage matrix (channel/adjproject0.c:82-95)
             std::vector<double> M = dir == 0 ? c.Rs : A5;
