@@ -1,11 +1,11 @@
-// Legendre-Galerkin kernels: y-transforms, banded Helmholtz solves, the
+// Legendre-Galerkin kernels: y-transforms, banded Helmholtz solves with the
 // (kx,kz) = (0,0) mode, and small vector kernels.
 //
 // y-transforms (channel/initAlphaBeta2.c:96-240 rebuild, channel/project.c:
 // 95-115,176-191 and channel/adjproject.c:108-134,213-234 projections) are
 // real dense products (qpts x dim or dim x qpts) applied to the complex
-// columns of one kz row; one block computes a 32-row x 16-column tile of all
-// outputs of a row band, staging the basis tiles and the columns in LDS.
+// columns of every kz row; they run on the FP64 matrix cores, the complex
+// columns split into real and imaginary parts.
 //
 // Band solves (channel/mvOps.c:29-144, used by channel/project.c,
 // increproject.c, adjproject.c): the matrices of the implicit stage are
@@ -18,289 +18,274 @@
 
 namespace sc {
 
+// ------------------------------------------------------------ MFMA helpers
+// v_mfma_f64_16x16x4f64: D[16][16] += A[16][4] B[4][16].  Lane l supplies
+// A[l%16][l/16] and B[l/16][l%16]; item r of its result holds D[4r + l/16][l%16].
+typedef double mf4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ mf4 mfma_f64(double a, double b, mf4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// The 16 real columns of an n-tile are eight complex columns; real column n
+// is complex column 2(n%4) + n/8, part (n/4)%2.  Item r of lane l then holds
+// complex column 2(l/16) + r/2, part r%2: both parts of two neighbouring
+// columns sit in one lane.
+__device__ __forceinline__ int ncol_x(int n) { return 2 * (n & 3) + (n >> 3); }
+__device__ __forceinline__ int ncol_part(int n) { return (n >> 2) & 1; }
+
+__device__ __forceinline__ mf4 mf4_zero()
+{
+    mf4 z = {0.0, 0.0, 0.0, 0.0};
+    return z;
+}
+
 // ------------------------------------------------------------ rebuild
 struct RebuildArgs {
-    const double *Q, *Qp, *Qpp, *R, *Rp, *Kx, *Kz;
-    int Nz, Mx, qpts, dimR, dimQ;
+    const double *frag, *Kx, *Kz;
+    int Nz, Mx, qpts, dimR, dimQ, nks;
     const int *abort_flag;
 };
 
-constexpr int TI = 32, TX = 16, JT = 16;
-
-// U[z][0..4][i][x] from C[z][0..1][j][x]; grid (ceil(qpts/32), ceil(Mx/16), Nz)
+// U[z][0..4][y][x] from C[z][0..1][j][x]; grid (nyt, Nz), 256 threads.
+// D[n][y] = sum_j C[j][n] B_m[y][j] for the bases m = Q, Qp, Qpp (alpha
+// columns) and R, Rp (beta columns); wave w takes the n-tiles w, w+4, ... of
+// y-tile blockIdx.x.  The basis fragments of a chunk of KC_REBUILD k-steps are
+// staged in LDS (double-buffered) and shared by the four waves; the
+// coefficient fragments are prefetched one chunk ahead in registers.
+// Epilogue: u, w, du/dy, dw/dy from v', v'', omega_y, omega_y'
+// (channel/initAlphaBeta2.c:96-240); the mean column (kx, kz) = (0, 0) takes
+// u = R alpha, du/dy = Rp alpha, w = R beta, dw/dy = Rp beta, v = 0
+// (channel/initAlphaBeta2.c:22-68) from two more products in its wave.
 __global__ __launch_bounds__(256) void k_rebuild(RebuildArgs a, const cplx *C, cplx *U)
 {
-    __shared__ double As[5][TI][JT + 1];
-    __shared__ double2 Ba[JT][TX], Bb[JT][TX];
+    constexpr int KC = KC_REBUILD, CH = KC * 5 * 64, PT = CH / 256;
+    __shared__ double bs[2][CH];
     if (*a.abort_flag) return;
-    const int z = blockIdx.z;
+    const int z = blockIdx.y;
     if (z == a.Nz / 2) return;
-    const int i0 = blockIdx.x * TI, x0 = blockIdx.y * TX;
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int yt = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int Mx = a.Mx, q = a.qpts, nR = a.dimR, nQ = a.dimQ;
-    const cplx *Ca = C + (size_t) z * 2 * nR * Mx;
-    const cplx *Cb = Ca + (size_t) nR * Mx;
-    double2 acc[2][5];
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int m = 0; m < 5; ++m) acc[r][m] = make_double2(0.0, 0.0);
-    for (int j0 = 0; j0 < nR; j0 += JT) {
-        for (int e = threadIdx.x; e < 5 * TI * JT; e += 256) {
-            int m = e / (TI * JT), rem = e - m * TI * JT;
-            int ii = rem / JT, jj = rem - ii * JT;
-            int i = i0 + ii, j = j0 + jj;
-            double v = 0.0;
-            if (i < q) {
-                if (m < 3) {
-                    if (j < nQ) {
-                        const double *M = m == 0 ? a.Q : m == 1 ? a.Qp : a.Qpp;
-                        v = M[(size_t) i * nQ + j];
-                    }
-                } else if (j < nR) {
-                    const double *M = m == 3 ? a.R : a.Rp;
-                    v = M[(size_t) i * nR + j];
-                }
-            }
-            As[m][ii][jj] = v;
-        }
-        for (int e = threadIdx.x; e < JT * TX; e += 256) {
-            int jj = e / TX, xx = e - jj * TX;
-            int j = j0 + jj, xg = x0 + xx;
-            double2 va = make_double2(0.0, 0.0), vb = va;
-            if (xg < Mx) {
-                if (j < nQ) va = Ca[(size_t) j * Mx + xg];
-                if (j < nR) vb = Cb[(size_t) j * Mx + xg];
-            }
-            Ba[jj][xx] = va;
-            Bb[jj][xx] = vb;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int jj = 0; jj < JT; ++jj) {
-            double2 ba = Ba[jj][tx], bb = Bb[jj][tx];
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                int ii = ty + 16 * r;
-                double cq = As[0][ii][jj], cp = As[1][ii][jj], c2 = As[2][ii][jj];
-                double cr = As[3][ii][jj], crp = As[4][ii][jj];
-                acc[r][0].x += cq * ba.x;  acc[r][0].y += cq * ba.y;
-                acc[r][1].x += cp * ba.x;  acc[r][1].y += cp * ba.y;
-                acc[r][2].x += c2 * ba.x;  acc[r][2].y += c2 * ba.y;
-                acc[r][3].x += cr * bb.x;  acc[r][3].y += cr * bb.y;
-                acc[r][4].x += crp * bb.x; acc[r][4].y += crp * bb.y;
-            }
-        }
-        __syncthreads();
-    }
-    const int x = x0 + tx;
-    if (x >= Mx || (z == 0 && x == 0)) return;
-    const double kx = a.Kx[x], kz = a.Kz[z], k2 = kz * kz + kx * kx;
+    const int nnt = (Mx + 7) / 8, nch = a.nks / KC;
+    const int kq = lane >> 4, nl = lane & 15;
+    const double *frag = a.frag + (size_t) yt * a.nks * 5 * 64;
+    const double *Cd = (const double *) (C + (size_t) z * 2 * nR * Mx);
     const size_t ss = (size_t) q * Mx;
     cplx *Uz = U + (size_t) z * 5 * ss;
+    const double kz = a.Kz[z];
+    for (int nt0 = 0; nt0 < nnt; nt0 += 4) {
+        const int nt = nt0 + wave;
+        const bool active = nt < nnt;
+        const int xcol = nt * 8 + ncol_x(nl), part = ncol_part(nl);
+        const bool xok = active && xcol < Mx;
+        const int limA = (z == 0 && xcol == 0) ? nR : nQ;
+        const bool meanw = z == 0 && nt == 0;
+        mf4 acc[7];
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-        int i = i0 + ty + 16 * r;
-        if (i >= q) continue;
-        double2 v = acc[r][0], f = make_double2(-acc[r][1].x, -acc[r][1].y);
-        double2 q2 = acc[r][2], g = acc[r][3], gp = acc[r][4];
-        size_t o = (size_t) i * Mx + x;
-        Uz[o + 1 * ss] = v;
-        Uz[o + 0 * ss] = make_double2((kx * f.y + kz * g.y) / k2, -(kx * f.x + kz * g.x) / k2);
-        Uz[o + 2 * ss] = make_double2((-kx * g.y + kz * f.y) / k2, (kx * g.x - kz * f.x) / k2);
-        Uz[o + 3 * ss] = make_double2((-kx * q2.y + kz * gp.y) / k2, (kx * q2.x - kz * gp.x) / k2);
-        Uz[o + 4 * ss] = make_double2(-(kx * gp.y + kz * q2.y) / k2, (kx * gp.x + kz * q2.x) / k2);
-    }
-}
-
-// mean mode (kx,kz) = (0,0): u = R a, du = Rp a, v = 0, w = R b, dw = Rp b
-// (channel/initAlphaBeta2.c:22-68).  Block: 32 rows x 8 lanes splitting j.
-__global__ __launch_bounds__(256) void k_rebuild_mean(RebuildArgs a, const cplx *C, cplx *U)
-{
-    __shared__ double2 al[512], be[512];
-    if (*a.abort_flag) return;
-    const int Mx = a.Mx, q = a.qpts, nR = a.dimR;
-    const size_t ss = (size_t) q * Mx;
-    for (int j = threadIdx.x; j < nR; j += blockDim.x) {
-        al[j] = C[(size_t) j * Mx];
-        be[j] = C[(size_t) (nR + j) * Mx];
-    }
-    __syncthreads();
-    const int i = blockIdx.x * 32 + threadIdx.x / 8, lane = threadIdx.x % 8;
-    double2 u = make_double2(0, 0), du = u, w = u, dw = u;
-    if (i < q) {
-        for (int j = lane; j < nR; j += 8) {
-            double r = a.R[(size_t) i * nR + j], rp = a.Rp[(size_t) i * nR + j];
-            u.x += r * al[j].x;  u.y += r * al[j].y;
-            du.x += rp * al[j].x; du.y += rp * al[j].y;
-            w.x += r * be[j].x;  w.y += r * be[j].y;
-            dw.x += rp * be[j].x; dw.y += rp * be[j].y;
+        for (int m = 0; m < 7; ++m) acc[m] = mf4_zero();
+        double ca[KC], cb[KC], st[PT];
+        auto load_chunk = [&](int ch) {
+#pragma unroll
+            for (int s = 0; s < KC; ++s) {
+                const int j = (ch * KC + s) * 4 + kq;
+                ca[s] = (xok && j < limA) ? Cd[((size_t) j * Mx + xcol) * 2 + part] : 0.0;
+                cb[s] = (xok && j < nR) ? Cd[((size_t) (nR + j) * Mx + xcol) * 2 + part] : 0.0;
+            }
+            const double *src = frag + (size_t) ch * CH;
+#pragma unroll
+            for (int i = 0; i < PT; ++i) st[i] = src[threadIdx.x + 256 * i];
+        };
+        load_chunk(0);
+#pragma unroll
+        for (int i = 0; i < PT; ++i) bs[0][threadIdx.x + 256 * i] = st[i];
+        __syncthreads();
+        for (int ch = 0; ch < nch; ++ch) {
+            double xa[KC], xb[KC];
+#pragma unroll
+            for (int s = 0; s < KC; ++s) {
+                xa[s] = ca[s];
+                xb[s] = cb[s];
+            }
+            if (ch + 1 < nch) load_chunk(ch + 1);
+            const double *b = bs[ch & 1];
+            if (active) {
+#pragma unroll
+                for (int s = 0; s < KC; ++s) {
+                    const double b0 = b[(s * 5 + 0) * 64 + lane], b1 = b[(s * 5 + 1) * 64 + lane];
+                    const double b2 = b[(s * 5 + 2) * 64 + lane], b3 = b[(s * 5 + 3) * 64 + lane];
+                    const double b4 = b[(s * 5 + 4) * 64 + lane];
+                    acc[0] = mfma_f64(xa[s], b0, acc[0]);
+                    acc[1] = mfma_f64(xa[s], b1, acc[1]);
+                    acc[2] = mfma_f64(xa[s], b2, acc[2]);
+                    acc[3] = mfma_f64(xb[s], b3, acc[3]);
+                    acc[4] = mfma_f64(xb[s], b4, acc[4]);
+                    if (meanw) {
+                        acc[5] = mfma_f64(xa[s], b3, acc[5]);
+                        acc[6] = mfma_f64(xa[s], b4, acc[6]);
+                    }
+                }
+            }
+            if (ch + 1 < nch) {
+#pragma unroll
+                for (int i = 0; i < PT; ++i) bs[(ch + 1) & 1][threadIdx.x + 256 * i] = st[i];
+            }
+            __syncthreads();
+        }
+        const int y = yt * 16 + nl;
+        if (!active || y >= q) continue;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int x = nt * 8 + 2 * kq + h;
+            if (x >= Mx) continue;
+            const size_t o = (size_t) y * Mx + x;
+            if (z == 0 && x == 0) {
+                Uz[o + 0 * ss] = make_double2(acc[5][2 * h], acc[5][2 * h + 1]);
+                Uz[o + 1 * ss] = make_double2(0.0, 0.0);
+                Uz[o + 2 * ss] = make_double2(acc[3][2 * h], acc[3][2 * h + 1]);
+                Uz[o + 3 * ss] = make_double2(acc[6][2 * h], acc[6][2 * h + 1]);
+                Uz[o + 4 * ss] = make_double2(acc[4][2 * h], acc[4][2 * h + 1]);
+                continue;
+            }
+            const double kx = a.Kx[x], k2 = kz * kz + kx * kx;
+            const double2 v = make_double2(acc[0][2 * h], acc[0][2 * h + 1]);
+            const double2 f = make_double2(-acc[1][2 * h], -acc[1][2 * h + 1]);
+            const double2 q2 = make_double2(acc[2][2 * h], acc[2][2 * h + 1]);
+            const double2 g = make_double2(acc[3][2 * h], acc[3][2 * h + 1]);
+            const double2 gp = make_double2(acc[4][2 * h], acc[4][2 * h + 1]);
+            Uz[o + 1 * ss] = v;
+            Uz[o + 0 * ss] = make_double2((kx * f.y + kz * g.y) / k2, -(kx * f.x + kz * g.x) / k2);
+            Uz[o + 2 * ss] = make_double2((-kx * g.y + kz * f.y) / k2, (kx * g.x - kz * f.x) / k2);
+            Uz[o + 3 * ss] = make_double2((-kx * q2.y + kz * gp.y) / k2, (kx * q2.x - kz * gp.x) / k2);
+            Uz[o + 4 * ss] = make_double2(-(kx * gp.y + kz * q2.y) / k2, (kx * gp.x + kz * q2.x) / k2);
         }
     }
-#pragma unroll
-    for (int o = 4; o > 0; o >>= 1) {
-        u.x += __shfl_xor(u.x, o, 8);   u.y += __shfl_xor(u.y, o, 8);
-        du.x += __shfl_xor(du.x, o, 8); du.y += __shfl_xor(du.y, o, 8);
-        w.x += __shfl_xor(w.x, o, 8);   w.y += __shfl_xor(w.y, o, 8);
-        dw.x += __shfl_xor(dw.x, o, 8); dw.y += __shfl_xor(dw.y, o, 8);
-    }
-    if (i < q && lane == 0) {
-        size_t o = (size_t) i * Mx;
-        U[o + 0 * ss] = u;
-        U[o + 1 * ss] = make_double2(0.0, 0.0);
-        U[o + 2 * ss] = w;
-        U[o + 3 * ss] = du;
-        U[o + 4 * ss] = dw;
-    }
 }
 
-void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst, bool with_mean)
+void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst)
 {
     const Dims &d = c.d;
-    RebuildArgs a{c.dQ, c.dQp, c.dQpp, c.dR, c.dRp, c.dKx, c.dKz,
-                  d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, c.abort_flag};
-    dim3 grid((d.qpts + TI - 1) / TI, (d.Mx + TX - 1) / TX, d.Nz);
+    RebuildArgs a{c.dFragR, c.dKx, c.dKz, d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, c.nksR, c.abort_flag};
+    dim3 grid(c.nyt, d.Nz);
     double bytes = 16.0 * (double) (d.Nz - 1) * d.Mx * (2.0 * d.dimR + 5.0 * d.qpts);
     prof_begin(c, c.stream, 4, bytes);
     hipLaunchKernelGGL(k_rebuild, grid, dim3(256), 0, c.stream, a, Csrc, Udst);
     prof_end(c, c.stream, 4);
-    if (with_mean)
-        hipLaunchKernelGGL(k_rebuild_mean, dim3((d.qpts + 31) / 32), dim3(256), 0, c.stream, a,
-                           Csrc, Udst);
 }
 
 // ------------------------------------------------------------ projection
 struct ForceArgs {
-    const double *Qw, *Qpw, *Qppw, *Rw, *Rpw, *Kx, *Kz;
-    int Nz, Mx, qpts, dimR, dimQ, adj;
+    const double *frag, *Kx, *Kz;
+    int Nz, Mx, qpts, dimR, dimQ, nks, adj;
     const int *abort_flag;
 };
 
-// Fa[z][i][x] (i < dimQ), Fb[z][i][x] (i < dimR); grid (ceil(dimR/32), ceil(Mx/16), Nz)
+// Fa[z][i][x] (i < dimQ), Fb[z][i][x] (i < dimR); grid (nit, Nz), 256 threads.
+// D[n][i] = sum_y A_m[y][n] W_m[i][y] (channel/project.c:95-115,176-191,
+// channel/adjproject.c:108-134,213-234) with the pointwise combinations
+//   fa = Qpw P - k2 Qw Hy (+ Qppw Z),     P  = i kx Hx + i kz Hz
+//   fb = Rw P2 (+ Rpw Z2),                P2 = i kz Hx - i kx Hz
+//   (adjoint) Z = -i kz Sx + i kx Sz,     Z2 = i kz Sz + i kx Sx
+// summed into two accumulators.  Each lane forms its A entries from the parts
+// of Hx, Hz, Hy, Sx, Sz it needs.  Mean column (channel/project0.c:110-146,
+// channel/adjproject0.c:98-140): Fb(0,0) = (Rw Re Hx + Rpw Re Sz,
+// Rw Re Hz - Rpw Re Sx).  Wave w takes the n-tiles w, w+4, ... of i-tile
+// blockIdx.x; the weighted-basis fragments of a chunk of KC_FORCE k-steps are
+// staged in LDS (double-buffered), the field values prefetched a chunk ahead.
 __global__ __launch_bounds__(256) void k_force(ForceArgs a, const cplx *U, cplx *Fa, cplx *Fb)
 {
-    __shared__ double As[5][TI][JT + 1];
-    __shared__ double2 Bs[5][JT][TX];
+    constexpr int KC = KC_FORCE, CH = KC * 5 * 64, PT = CH / 256;
+    static_assert(CH % 256 == 0, "chunk must split over the block");
+    __shared__ double bs[2][CH];
     if (*a.abort_flag) return;
-    const int z = blockIdx.z;
+    const int z = blockIdx.y;
     if (z == a.Nz / 2) return;
-    const int i0 = blockIdx.x * TI, x0 = blockIdx.y * TX;
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int Mx = a.Mx, q = a.qpts, nR = a.dimR, nQ = a.dimQ;
+    const int it = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int Mx = a.Mx, q = a.qpts, nR = a.dimR, nQ = a.dimQ, adj = a.adj;
+    const int nnt = (Mx + 7) / 8, nch = a.nks / KC;
+    const int kq = lane >> 4, nl = lane & 15;
+    const double *frag = a.frag + (size_t) it * a.nks * 5 * 64;
     const size_t ss = (size_t) q * Mx;
-    const cplx *Uz = U + (size_t) z * 5 * ss;
+    const double *Ud = (const double *) (U + (size_t) z * 5 * ss);
     const double kz = a.Kz[z];
-    const int nm = a.adj ? 5 : 3;
-    double2 acc[2][5];
+    for (int nt0 = 0; nt0 < nnt; nt0 += 4) {
+        const int nt = nt0 + wave;
+        const bool active = nt < nnt;
+        const int xcol = nt * 8 + ncol_x(nl), part = ncol_part(nl);
+        const bool xok = active && xcol < Mx;
+        const bool mean = z == 0 && xcol == 0;
+        const double kx = xok ? a.Kx[xcol] : 0.0, k2 = kz * kz + kx * kx;
+        const double sg = part ? 1.0 : -1.0;
+        const int other = mean ? 0 : 1 - part;
+        mf4 accA = mf4_zero(), accB = mf4_zero();
+        double rhx[KC], rhz[KC], rhy[KC], rsx[KC], rsz[KC], st[PT];
+        auto load_chunk = [&](int ch) {
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int m = 0; m < 5; ++m) acc[r][m] = make_double2(0.0, 0.0);
-    for (int j0 = 0; j0 < q; j0 += JT) {
-        // A tiles: 0 Qpw, 1 Qw, 2 Rw, 3 Qppw, 4 Rpw
-        for (int e = threadIdx.x; e < nm * TI * JT; e += 256) {
-            int m = e / (TI * JT), rem = e - m * TI * JT;
-            int ii = rem / JT, jj = rem - ii * JT;
-            int i = i0 + ii, j = j0 + jj;
-            double v = 0.0;
-            if (j < q) {
-                if (m == 0 || m == 1 || m == 3) {
-                    if (i < nQ) {
-                        const double *M = m == 0 ? a.Qpw : m == 1 ? a.Qw : a.Qppw;
-                        v = M[(size_t) i * q + j];
-                    }
-                } else if (i < nR) {
-                    const double *M = m == 2 ? a.Rw : a.Rpw;
-                    v = M[(size_t) i * q + j];
-                }
+            for (int s = 0; s < KC; ++s) {
+                const int y = (ch * KC + s) * 4 + kq;
+                const bool ok = xok && y < q;
+                const size_t e = ((size_t) y * Mx + xcol) * 2;
+                rhz[s] = ok ? Ud[e + other] : 0.0;
+                rhx[s] = ok ? Ud[e + 2 * ss + other] : 0.0;
+                rhy[s] = ok ? Ud[e + 4 * ss + part] : 0.0;
+                rsx[s] = (ok && adj) ? Ud[e + 6 * ss + other] : 0.0;
+                rsz[s] = (ok && adj) ? Ud[e + 8 * ss + other] : 0.0;
             }
-            As[m][ii][jj] = v;
-        }
-        // B: 0 P = i kx Hx + i kz Hz, 1 Hy, 2 P2 = i kz Hx - i kx Hz,
-        //    3 Z = -i kz Sx + i kx Sz, 4 Z2 = i kz Sz + i kx Sx
-        for (int e = threadIdx.x; e < JT * TX; e += 256) {
-            int jj = e / TX, xx = e - jj * TX;
-            int j = j0 + jj, xg = x0 + xx;
-            double2 P = make_double2(0, 0), Y = P, P2 = P, Z = P, Z2 = P;
-            if (j < q && xg < Mx) {
-                double kx = a.Kx[xg];
-                size_t o = (size_t) j * Mx + xg;
-                double2 hz = Uz[o], hx = Uz[o + ss], hy = Uz[o + 2 * ss];
-                if (z == 0 && xg == 0) {
-                    // mean mode (channel/project0.c:110-146, adjproject0.c:98-140):
-                    // Rw Re(Hx) and Rw Re(Hz) packed as one complex column,
-                    // Rpw Re(S_z) and -Rpw Re(S_x) likewise for the adjoint
-                    P2 = make_double2(hx.x, hz.x);
-                    if (a.adj) {
-                        double2 sx = Uz[o + 3 * ss], sz = Uz[o + 4 * ss];
-                        Z2 = make_double2(sz.x, -sx.x);
-                    }
+            const double *src = frag + (size_t) ch * CH;
+#pragma unroll
+            for (int i = 0; i < PT; ++i) st[i] = src[threadIdx.x + 256 * i];
+        };
+        load_chunk(0);
+#pragma unroll
+        for (int i = 0; i < PT; ++i) bs[0][threadIdx.x + 256 * i] = st[i];
+        __syncthreads();
+        for (int ch = 0; ch < nch; ++ch) {
+            double vP[KC], vY[KC], vZ[KC], vP2[KC], vZ2[KC];
+#pragma unroll
+            for (int s = 0; s < KC; ++s) {
+                if (mean) {
+                    vP[s] = 0.0;
+                    vY[s] = 0.0;
+                    vZ[s] = 0.0;
+                    vP2[s] = part ? rhz[s] : rhx[s];
+                    vZ2[s] = part ? -rsx[s] : rsz[s];
                 } else {
-                    P = make_double2(-kx * hx.y - kz * hz.y, kx * hx.x + kz * hz.x);
-                    Y = hy;
-                    P2 = make_double2(-kz * hx.y + kx * hz.y, kz * hx.x - kx * hz.x);
-                    if (a.adj) {
-                        double2 sx = Uz[o + 3 * ss], sz = Uz[o + 4 * ss];
-                        Z = make_double2(kz * sx.y - kx * sz.y, -kz * sx.x + kx * sz.x);
-                        Z2 = make_double2(-kz * sz.y - kx * sx.y, kz * sz.x + kx * sx.x);
+                    vP[s] = sg * (kx * rhx[s] + kz * rhz[s]);
+                    vY[s] = -k2 * rhy[s];
+                    vP2[s] = sg * (kz * rhx[s] - kx * rhz[s]);
+                    vZ[s] = -sg * (kz * rsx[s] - kx * rsz[s]);
+                    vZ2[s] = sg * (kz * rsz[s] + kx * rsx[s]);
+                }
+            }
+            if (ch + 1 < nch) load_chunk(ch + 1);
+            const double *b = bs[ch & 1];
+            if (active) {
+#pragma unroll
+                for (int s = 0; s < KC; ++s) {
+                    accA = mfma_f64(vP[s], b[(s * 5 + 0) * 64 + lane], accA);
+                    accB = mfma_f64(vP2[s], b[(s * 5 + 3) * 64 + lane], accB);
+                    accA = mfma_f64(vY[s], b[(s * 5 + 1) * 64 + lane], accA);
+                    if (adj) {
+                        accA = mfma_f64(vZ[s], b[(s * 5 + 2) * 64 + lane], accA);
+                        accB = mfma_f64(vZ2[s], b[(s * 5 + 4) * 64 + lane], accB);
                     }
                 }
             }
-            Bs[0][jj][xx] = P;
-            Bs[1][jj][xx] = Y;
-            Bs[2][jj][xx] = P2;
-            Bs[3][jj][xx] = Z;
-            Bs[4][jj][xx] = Z2;
-        }
-        __syncthreads();
-        if (a.adj) {
-#pragma unroll 4
-            for (int jj = 0; jj < JT; ++jj) {
-                double2 b0 = Bs[0][jj][tx], b1 = Bs[1][jj][tx], b2 = Bs[2][jj][tx];
-                double2 b3 = Bs[3][jj][tx], b4 = Bs[4][jj][tx];
+            if (ch + 1 < nch) {
 #pragma unroll
-                for (int r = 0; r < 2; ++r) {
-                    int ii = ty + 16 * r;
-                    double c0 = As[0][ii][jj], c1 = As[1][ii][jj], c2 = As[2][ii][jj];
-                    double c3 = As[3][ii][jj], c4 = As[4][ii][jj];
-                    acc[r][0].x += c0 * b0.x; acc[r][0].y += c0 * b0.y;
-                    acc[r][1].x += c1 * b1.x; acc[r][1].y += c1 * b1.y;
-                    acc[r][2].x += c2 * b2.x; acc[r][2].y += c2 * b2.y;
-                    acc[r][3].x += c3 * b3.x; acc[r][3].y += c3 * b3.y;
-                    acc[r][4].x += c4 * b4.x; acc[r][4].y += c4 * b4.y;
-                }
+                for (int i = 0; i < PT; ++i) bs[(ch + 1) & 1][threadIdx.x + 256 * i] = st[i];
             }
-        } else {
-#pragma unroll 4
-            for (int jj = 0; jj < JT; ++jj) {
-                double2 b0 = Bs[0][jj][tx], b1 = Bs[1][jj][tx], b2 = Bs[2][jj][tx];
-#pragma unroll
-                for (int r = 0; r < 2; ++r) {
-                    int ii = ty + 16 * r;
-                    double c0 = As[0][ii][jj], c1 = As[1][ii][jj], c2 = As[2][ii][jj];
-                    acc[r][0].x += c0 * b0.x; acc[r][0].y += c0 * b0.y;
-                    acc[r][1].x += c1 * b1.x; acc[r][1].y += c1 * b1.y;
-                    acc[r][2].x += c2 * b2.x; acc[r][2].y += c2 * b2.y;
-                }
-            }
+            __syncthreads();
         }
-        __syncthreads();
-    }
-    const int x = x0 + tx;
-    if (x >= Mx) return;
-    const bool mean = z == 0 && x == 0;   // Fb(0,0) = (fa, fb) of the mean mode
-    const double kx = a.Kx[x], k2 = kz * kz + kx * kx;
+        const int i = it * 16 + nl;
+        if (!active || i >= nR) continue;
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-        int i = i0 + ty + 16 * r;
-        if (i < nQ && !mean) {
-            double2 fa = make_double2(acc[r][0].x - k2 * acc[r][1].x + acc[r][3].x,
-                                      acc[r][0].y - k2 * acc[r][1].y + acc[r][3].y);
-            Fa[((size_t) z * nQ + i) * Mx + x] = fa;
-        }
-        if (i < nR) {
-            double2 fb = make_double2(acc[r][2].x + acc[r][4].x, acc[r][2].y + acc[r][4].y);
-            Fb[((size_t) z * nR + i) * Mx + x] = fb;
+        for (int h = 0; h < 2; ++h) {
+            const int x = nt * 8 + 2 * kq + h;
+            if (x >= Mx) continue;
+            if (i < nQ && !(z == 0 && x == 0))
+                Fa[((size_t) z * nQ + i) * Mx + x] = make_double2(accA[2 * h], accA[2 * h + 1]);
+            Fb[((size_t) z * nR + i) * Mx + x] = make_double2(accB[2 * h], accB[2 * h + 1]);
         }
     }
 }
@@ -308,9 +293,9 @@ __global__ __launch_bounds__(256) void k_force(ForceArgs a, const cplx *U, cplx 
 void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb)
 {
     const Dims &d = c.d;
-    ForceArgs a{c.dQw, c.dQpw, c.dQppw, c.dRw, c.dRpw, c.dKx, c.dKz,
-                d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, adj, c.abort_flag};
-    dim3 grid((d.dimR + TI - 1) / TI, (d.Mx + TX - 1) / TX, d.Nz);
+    ForceArgs a{c.dFragF, c.dKx, c.dKz, d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, c.nksF, adj,
+                c.abort_flag};
+    dim3 grid(c.nit, d.Nz);
     double bytes = 16.0 * (double) (d.Nz - 1) * d.Mx *
                    ((adj ? 5.0 : 3.0) * d.qpts + d.dimQ + d.dimR);
     prof_begin(c, c.stream, 5, bytes);

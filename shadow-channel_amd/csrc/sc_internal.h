@@ -23,6 +23,8 @@ namespace sc {
 typedef double2 cplx;
 
 constexpr int MAXF = 12;        // fields in one z-inverse launch
+constexpr int KC_REBUILD = 8;   // MFMA k-steps per staged chunk, coefficients -> Gauss points
+constexpr int KC_FORCE = 4;     // MFMA k-steps per staged chunk, Gauss points -> coefficients
 constexpr int MAX_FACTORS = 24; // radix factors of an FFT length
 
 struct FftPlan {
@@ -77,6 +79,12 @@ struct Context {
     double *dQw = nullptr, *dQpw = nullptr, *dQppw = nullptr, *dRw = nullptr, *dRpw = nullptr;
     double *dQs = nullptr, *dQps = nullptr, *dQpps = nullptr, *dRs = nullptr, *dRps = nullptr;
     double *dRowsum = nullptr;                  // sum_j Rw[i][j]
+    // y-transform operands in the B-operand order of v_mfma_f64_16x16x4f64
+    // (lane l of k-step s, tile t: row 4s + l/16, column 16t + l%16), zero-padded:
+    //   fragR [nyt][nksR][5][64]: Q, Qp, Qpp, R, Rp at (j = 4s + l/16, y = 16t + l%16)
+    //   fragF [nit][nksF][5][64]: Qpw, Qw, Qppw, Rw, Rpw at (y = 4s + l/16, i = 16t + l%16)
+    double *dFragR = nullptr, *dFragF = nullptr;
+    int nyt = 0, nksR = 0, nit = 0, nksF = 0;
     // LU factors of the implicit operators, per (direction, stage):
     //   alpha: [Nz][dimQ][9][Mx]   (j<4: L(i,i-2(4-j)); j=4: U diagonal; j>4: U)
     //   beta:  [Nz][dimR][5][Mx]
@@ -123,7 +131,7 @@ void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
                  double *flow, double *dot_out);
 void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind);
 void launch_assemble_adj(Context &c, const cplx *SP, cplx *Udst);
-void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst, bool with_mean = true);
+void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst);
 void launch_force(Context &c, const cplx *Usrc, int adj);
 void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb);
 void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cplx *Fa2,

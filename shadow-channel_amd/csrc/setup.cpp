@@ -225,6 +225,37 @@ int setup_host(Context &c)
         return -1;
     }
 
+    // y-transform operands in MFMA fragment order, zero outside the bases
+    {
+        const int nyt = (q + 15) / 16, nksR = ((nR + 3) / 4 + KC_REBUILD - 1) / KC_REBUILD * KC_REBUILD;
+        std::vector<double> fr((size_t) nyt * nksR * 5 * 64, 0.0);
+        const std::vector<double> *B[5] = {&c.Q, &c.Qp, &c.Qpp, &c.R, &c.Rp};
+        for (int t = 0; t < nyt; ++t)
+            for (int s = 0; s < nksR; ++s)
+                for (int m = 0; m < 5; ++m)
+                    for (int l = 0; l < 64; ++l) {
+                        const int y = 16 * t + l % 16, j = 4 * s + l / 16, dim = m < 3 ? nQ : nR;
+                        if (y < q && j < dim)
+                            fr[(((size_t) t * nksR + s) * 5 + m) * 64 + l] = (*B[m])[(size_t) y * dim + j];
+                    }
+        const int nit = (nR + 15) / 16, nksF = ((q + 3) / 4 + KC_FORCE - 1) / KC_FORCE * KC_FORCE;
+        std::vector<double> ff((size_t) nit * nksF * 5 * 64, 0.0);
+        const std::vector<double> *Wt[5] = {&c.Qpw, &c.Qw, &c.Qppw, &c.Rw, &c.Rpw};
+        for (int t = 0; t < nit; ++t)
+            for (int s = 0; s < nksF; ++s)
+                for (int m = 0; m < 5; ++m)
+                    for (int l = 0; l < 64; ++l) {
+                        const int i = 16 * t + l % 16, y = 4 * s + l / 16, dim = m < 3 ? nQ : nR;
+                        if (i < dim && y < q)
+                            ff[(((size_t) t * nksF + s) * 5 + m) * 64 + l] = (*Wt[m])[(size_t) i * q + y];
+                    }
+        c.nyt = nyt; c.nksR = nksR; c.nit = nit; c.nksF = nksF;
+        if (upload(&c.dFragR, fr) || upload(&c.dFragF, ff)) {
+            last_error() = "device allocation of transform operands failed";
+            return -1;
+        }
+    }
+
     // LU factors of the implicit operators per direction and stage
     // v:      -(K2 Qs + Qps) - re b dt (K2^2 Qs + 2 K2 Qps + Qpps)
     // omega:  Rs + re b dt (Rps + K2 Rs)
