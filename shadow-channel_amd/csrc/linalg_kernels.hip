@@ -35,39 +35,77 @@ __device__ __forceinline__ mf4 mfma_f64(double a, double b, mf4 c)
 __device__ __forceinline__ int ncol_x(int n) { return 2 * (n & 3) + (n >> 3); }
 __device__ __forceinline__ int ncol_part(int n) { return (n >> 2) & 1; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs in launch order, and
+// each XCD has its own L2.  With a 1-D grid of 8*ceil(n/8) blocks, block b
+// takes logical tile (b%8)*ceil(n/8) + b/8: consecutive logical tiles (here
+// the tiles of one kz row, which read the same data) share an XCD.  Returns
+// -1 for the padding blocks.
+__device__ __forceinline__ int xcd_tile(int n)
+{
+    const int per = (n + 7) / 8;
+    const int t = (int) (blockIdx.x & 7) * per + (int) (blockIdx.x >> 3);
+    return t < n ? t : -1;
+}
+
+static inline int xcd_grid(int n) { return 8 * ((n + 7) / 8); }
+
+// wave index within the block, in a scalar register: the compiler then
+// treats everything derived from it as uniform across the wave
+__device__ __forceinline__ int wave_id()
+{
+    return __builtin_amdgcn_readfirstlane((int) (threadIdx.x >> 6));
+}
+
 __device__ __forceinline__ mf4 mf4_zero()
 {
     mf4 z = {0.0, 0.0, 0.0, 0.0};
     return z;
 }
 
+// Wavenumbers formed in registers, bit-identical to the host tables
+// (kx = x 2pi/Lx; kz = z 2pi/Lz, z - Nz above Nz/2): no global load, and so
+// no wait on the in-order memory counter behind prefetched operands.
+__device__ __forceinline__ double wave_kx(int x, double ax) { return x * ax; }
+__device__ __forceinline__ double wave_kz(int z, int Nz, double az)
+{
+    return (z <= Nz / 2 ? z : z - Nz) * az;
+}
+
 // ------------------------------------------------------------ rebuild
 struct RebuildArgs {
     const double *frag, *Kx, *Kz;
-    int Nz, Mx, qpts, dimR, dimQ, nks;
+    int Nz, Mx, qpts, dimR, dimQ, nks, nyt, nseg;
+    double ax, az;                      // 2 pi / Lx, 2 pi / Lz
     const int *abort_flag;
 };
 
-// U[z][0..4][y][x] from C[z][0..1][j][x]; grid (nyt, Nz), 256 threads.
+// U[z][0..4][y][x] from C[z][0..1][j][x]; 1-D grid over the (kz row, y-tile)
+// tiles, y-tile fastest, XCD-aware (xcd_tile); 256 threads.
 // D[n][y] = sum_j C[j][n] B_m[y][j] for the bases m = Q, Qp, Qpp (alpha
 // columns) and R, Rp (beta columns); wave w takes the n-tiles w, w+4, ... of
-// y-tile blockIdx.x.  The basis fragments of a chunk of KC_REBUILD k-steps are
+// the block's y-tile.  The basis fragments of a chunk of KC_REBUILD k-steps are
 // staged in LDS (double-buffered) and shared by the four waves; the
 // coefficient fragments are prefetched one chunk ahead in registers.
 // Epilogue: u, w, du/dy, dw/dy from v', v'', omega_y, omega_y'
-// (channel/initAlphaBeta2.c:96-240); the mean column (kx, kz) = (0, 0) takes
-// u = R alpha, du/dy = Rp alpha, w = R beta, dw/dy = Rp beta, v = 0
-// (channel/initAlphaBeta2.c:22-68) from two more products in its wave.
-__global__ __launch_bounds__(256) void k_rebuild(RebuildArgs a, const cplx *C, cplx *U)
+// (channel/initAlphaBeta2.c:96-240).
+// The Nyquist row kz = Nz/2 carries no data, so its blocks rebuild the mean
+// column (kx, kz) = (0, 0) instead: u = R alpha, du/dy = Rp alpha,
+// w = R beta, dw/dy = Rp beta, v = 0 (channel/initAlphaBeta2.c:22-68), with
+// the n rows 0..3 of one tile holding Re/Im alpha, Re/Im beta.
+__global__ __launch_bounds__(256, SC_YT_WAVES) void k_rebuild(RebuildArgs a, const cplx *C, cplx *U)
 {
     constexpr int KC = KC_REBUILD, CH = KC * 5 * 64, PT = CH / 256;
+    static_assert(CH % 256 == 0, "chunk must split over the block");
     __shared__ double bs[2][CH];
     if (*a.abort_flag) return;
-    const int z = blockIdx.y;
-    if (z == a.Nz / 2) return;
-    const int yt = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tile = xcd_tile(a.nyt * a.Nz);
+    if (tile < 0) return;
+    const int zrow = tile / a.nyt, yt = tile - zrow * a.nyt;
+    const bool meanb = zrow == a.Nz / 2;
+    const int z = meanb ? 0 : zrow;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const int Mx = a.Mx, q = a.qpts, nR = a.dimR, nQ = a.dimQ;
-    const int nnt = (Mx + 7) / 8, nch = a.nks / KC;
+    const int nnt = meanb ? 1 : (Mx + 7) / 8, nch = a.nks / KC;
     const int kq = lane >> 4, nl = lane & 15;
     const double *frag = a.frag + (size_t) yt * a.nks * 5 * 64;
     const double *Cd = (const double *) (C + (size_t) z * 2 * nR * Mx);
@@ -77,39 +115,55 @@ __global__ __launch_bounds__(256) void k_rebuild(RebuildArgs a, const cplx *C, c
     for (int nt0 = 0; nt0 < nnt; nt0 += 4) {
         const int nt = nt0 + wave;
         const bool active = nt < nnt;
-        const int xcol = nt * 8 + ncol_x(nl), part = ncol_part(nl);
-        const bool xok = active && xcol < Mx;
-        const int limA = (z == 0 && xcol == 0) ? nR : nQ;
-        const bool meanw = z == 0 && nt == 0;
-        mf4 acc[7];
+        int xcol, part;
+        bool okA, okB;
+        size_t offA, offB;
+        if (meanb) {
+            // n = 0, 1: Re, Im alpha(0,0); n = 2, 3: Re, Im beta(0,0)
+            xcol = 0;
+            part = nl & 1;
+            okA = active && nl < 4;
+            okB = false;
+            offA = (size_t) (nl >> 1) * nR * Mx * 2 + part;
+            offB = 0;
+        } else {
+            xcol = nt * 8 + ncol_x(nl);
+            part = ncol_part(nl);
+            okA = okB = active && xcol < Mx;
+            offA = (size_t) xcol * 2 + part;
+            offB = (size_t) nR * Mx * 2 + offA;
+        }
+        const int limA = meanb ? nR : nQ;
+        mf4 acc[5];
 #pragma unroll
-        for (int m = 0; m < 7; ++m) acc[m] = mf4_zero();
-        double ca[KC], cb[KC], st[PT];
-        auto load_chunk = [&](int ch) {
+        for (int m = 0; m < 5; ++m) acc[m] = mf4_zero();
+        // two register sets of coefficient fragments, used alternately
+        double ca0[KC], cb0[KC], ca1[KC], cb1[KC], st[PT];
+        auto load_coef = [&](int ch, double *ca, double *cb) {
 #pragma unroll
             for (int s = 0; s < KC; ++s) {
                 const int j = (ch * KC + s) * 4 + kq;
-                ca[s] = (xok && j < limA) ? Cd[((size_t) j * Mx + xcol) * 2 + part] : 0.0;
-                cb[s] = (xok && j < nR) ? Cd[((size_t) (nR + j) * Mx + xcol) * 2 + part] : 0.0;
+                ca[s] = (okA && j < limA) ? Cd[offA + (size_t) j * Mx * 2] : 0.0;
+                cb[s] = (okB && j < nR) ? Cd[offB + (size_t) j * Mx * 2] : 0.0;
             }
+        };
+        auto load_basis = [&](int ch) {
             const double *src = frag + (size_t) ch * CH;
 #pragma unroll
             for (int i = 0; i < PT; ++i) st[i] = src[threadIdx.x + 256 * i];
         };
-        load_chunk(0);
+        auto put_basis = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < PT; ++i) bs[0][threadIdx.x + 256 * i] = st[i];
-        __syncthreads();
-        for (int ch = 0; ch < nch; ++ch) {
-            double xa[KC], xb[KC];
+            for (int i = 0; i < PT; ++i) bs[buf][threadIdx.x + 256 * i] = st[i];
+        };
+        auto compute = [&](const double *xa, const double *xb, const double *b) {
+            if (active && meanb) {
 #pragma unroll
-            for (int s = 0; s < KC; ++s) {
-                xa[s] = ca[s];
-                xb[s] = cb[s];
-            }
-            if (ch + 1 < nch) load_chunk(ch + 1);
-            const double *b = bs[ch & 1];
-            if (active) {
+                for (int s = 0; s < KC; ++s) {
+                    acc[3] = mfma_f64(xa[s], b[(s * 5 + 3) * 64 + lane], acc[3]);
+                    acc[4] = mfma_f64(xa[s], b[(s * 5 + 4) * 64 + lane], acc[4]);
+                }
+            } else if (active) {
 #pragma unroll
                 for (int s = 0; s < KC; ++s) {
                     const double b0 = b[(s * 5 + 0) * 64 + lane], b1 = b[(s * 5 + 1) * 64 + lane];
@@ -120,33 +174,48 @@ __global__ __launch_bounds__(256) void k_rebuild(RebuildArgs a, const cplx *C, c
                     acc[2] = mfma_f64(xa[s], b2, acc[2]);
                     acc[3] = mfma_f64(xb[s], b3, acc[3]);
                     acc[4] = mfma_f64(xb[s], b4, acc[4]);
-                    if (meanw) {
-                        acc[5] = mfma_f64(xa[s], b3, acc[5]);
-                        acc[6] = mfma_f64(xa[s], b4, acc[6]);
-                    }
                 }
             }
-            if (ch + 1 < nch) {
-#pragma unroll
-                for (int i = 0; i < PT; ++i) bs[(ch + 1) & 1][threadIdx.x + 256 * i] = st[i];
+        };
+        load_coef(0, ca0, cb0);
+        load_basis(0);
+        put_basis(0);
+        __syncthreads();
+        for (int ch = 0; ch < nch; ch += 2) {
+            const bool n1 = ch + 1 < nch;
+            if (n1) {
+                load_coef(ch + 1, ca1, cb1);
+                load_basis(ch + 1);
             }
+            compute(ca0, cb0, bs[0]);
+            if (n1) put_basis(1);
+            __syncthreads();
+            if (!n1) break;
+            const bool n2 = ch + 2 < nch;
+            if (n2) {
+                load_coef(ch + 2, ca0, cb0);
+                load_basis(ch + 2);
+            }
+            compute(ca1, cb1, bs[1]);
+            if (n2) put_basis(0);
             __syncthreads();
         }
         const int y = yt * 16 + nl;
         if (!active || y >= q) continue;
+        if (meanb) {
+            // item 0 of lane group kq holds n = kq: u (kq < 2) or w (kq >= 2)
+            double *Ur = (double *) (Uz + (size_t) y * Mx) + (kq & 1);
+            const int su = kq < 2 ? 0 : 2;
+            Ur[(size_t) su * ss * 2] = acc[3][0];
+            Ur[(size_t) (su == 0 ? 3 : 4) * ss * 2] = acc[4][0];
+            Ur[(size_t) 1 * ss * 2] = 0.0;
+            continue;
+        }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int x = nt * 8 + 2 * kq + h;
-            if (x >= Mx) continue;
+            if (x >= Mx || (z == 0 && x == 0)) continue;
             const size_t o = (size_t) y * Mx + x;
-            if (z == 0 && x == 0) {
-                Uz[o + 0 * ss] = make_double2(acc[5][2 * h], acc[5][2 * h + 1]);
-                Uz[o + 1 * ss] = make_double2(0.0, 0.0);
-                Uz[o + 2 * ss] = make_double2(acc[3][2 * h], acc[3][2 * h + 1]);
-                Uz[o + 3 * ss] = make_double2(acc[6][2 * h], acc[6][2 * h + 1]);
-                Uz[o + 4 * ss] = make_double2(acc[4][2 * h], acc[4][2 * h + 1]);
-                continue;
-            }
             const double kx = a.Kx[x], k2 = kz * kz + kx * kx;
             const double2 v = make_double2(acc[0][2 * h], acc[0][2 * h + 1]);
             const double2 f = make_double2(-acc[1][2 * h], -acc[1][2 * h + 1]);
@@ -162,25 +231,211 @@ __global__ __launch_bounds__(256) void k_rebuild(RebuildArgs a, const cplx *C, c
     }
 }
 
+// Operands of one (kz row, n-tile) task of the rebuild: lane offsets into C
+// of its alpha and beta fragments.  Every lane loads (the loads carry no
+// predicate, so nothing waits on them before use): columns past Mx read the
+// last column and are never stored, rows past the basis meet zero basis
+// fragments, and in the mean row only the n rows 0..3 are stored.
+struct RebuildTask {
+    size_t offA, offB;
+};
+
+__device__ __forceinline__ RebuildTask rebuild_task(const RebuildArgs &a, int zrow, int nt, int nl)
+{
+    RebuildTask t;
+    const int Mx = a.Mx, nR = a.dimR;
+    if (zrow == a.Nz / 2) {
+        // n = 0, 1: Re, Im alpha(0,0); n = 2, 3: Re, Im beta(0,0)
+        t.offA = (size_t) ((nl >> 1) & 1) * nR * Mx * 2 + (nl & 1);
+        t.offB = t.offA;
+    } else {
+        const int xcol = min(nt * 8 + ncol_x(nl), Mx - 1);
+        t.offA = (size_t) zrow * 2 * nR * Mx * 2 + (size_t) xcol * 2 + ncol_part(nl);
+        t.offB = t.offA + (size_t) nR * Mx * 2;
+    }
+    return t;
+}
+
+// Gauss-point values of one task from its five accumulators (D[n][y] layout)
+__device__ __forceinline__ void rebuild_store(const RebuildArgs &a, int zrow, int nt, int y, int kq,
+                                              const mf4 *acc, cplx *U)
+{
+    const int Mx = a.Mx, q = a.qpts;
+    if (y >= q) return;
+    const size_t ss = (size_t) q * Mx;
+    if (zrow == a.Nz / 2) {
+        // mean column: item 0 of lane group kq holds n = kq, Re/Im of
+        // alpha (kq < 2: u, du/dy) or beta (kq >= 2: w, dw/dy)
+        double *Ur = (double *) (U + (size_t) y * Mx) + (kq & 1);
+        const int su = kq < 2 ? 0 : 2;
+        Ur[(size_t) su * ss * 2] = acc[3][0];
+        Ur[(size_t) (su == 0 ? 3 : 4) * ss * 2] = acc[4][0];
+        Ur[(size_t) 1 * ss * 2] = 0.0;
+        return;
+    }
+    cplx *Uz = U + (size_t) zrow * 5 * ss;
+    const double kz = wave_kz(zrow, a.Nz, a.az);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int x = nt * 8 + 2 * kq + h;
+        if (x >= Mx || (zrow == 0 && x == 0)) continue;
+        const size_t o = (size_t) y * Mx + x;
+        const double kx = wave_kx(x, a.ax), k2 = kz * kz + kx * kx;
+        const double2 v = make_double2(acc[0][2 * h], acc[0][2 * h + 1]);
+        const double2 f = make_double2(-acc[1][2 * h], -acc[1][2 * h + 1]);
+        const double2 q2 = make_double2(acc[2][2 * h], acc[2][2 * h + 1]);
+        const double2 g = make_double2(acc[3][2 * h], acc[3][2 * h + 1]);
+        const double2 gp = make_double2(acc[4][2 * h], acc[4][2 * h + 1]);
+        Uz[o + 1 * ss] = v;
+        Uz[o + 0 * ss] = make_double2((kx * f.y + kz * g.y) / k2, -(kx * f.x + kz * g.x) / k2);
+        Uz[o + 2 * ss] = make_double2((-kx * g.y + kz * f.y) / k2, (kx * g.x - kz * f.x) / k2);
+        Uz[o + 3 * ss] = make_double2((-kx * q2.y + kz * gp.y) / k2, (kx * q2.x - kz * gp.x) / k2);
+        Uz[o + 4 * ss] = make_double2(-(kx * gp.y + kz * q2.y) / k2, (kx * gp.x + kz * q2.x) / k2);
+    }
+}
+
+// Row-persistent rebuild for dimR <= 4*NKS: a block owns one y-tile, stages
+// its whole basis (NKS k-steps) in LDS once, and runs the kz rows of one
+// segment.  Each wave walks its (kz row, n-tile) tasks independently (load the
+// 2 NKS coefficient fragments, 5 NKS products, store); no barrier after the
+// staging.  Two blocks share a CU, so one wave's loads overlap the other's
+// products.  1-D grid over (segment, y-tile), y-tile fastest, XCD-aware.
+template <int NKS>
+__global__ __launch_bounds__(256, 2) void k_rebuild_rows(RebuildArgs a, const cplx *C, cplx *U)
+{
+    __shared__ double bs[NKS * 5 * 64];
+    if (*a.abort_flag) return;
+    const int tile = xcd_tile(a.nyt * a.nseg);
+    if (tile < 0) return;
+    const int seg = tile / a.nyt, yt = tile - seg * a.nyt;
+    const int z0 = seg * a.Nz / a.nseg, z1 = (seg + 1) * a.Nz / a.nseg;
+    const int lane = threadIdx.x & 63, wave = wave_id();
+    const int kq = lane >> 4, nl = lane & 15;
+    const int nnt = (a.Mx + 7) / 8, nR = a.dimR;
+    const double *frag = a.frag + (size_t) yt * NKS * 5 * 64;
+    {
+        // all loads in flight before the first LDS store
+        constexpr int PT = NKS * 5 * 64 / 256;
+        static_assert(NKS * 5 * 64 % 256 == 0, "basis must split over the block");
+        double st[PT];
+#pragma unroll
+        for (int i = 0; i < PT; ++i) st[i] = frag[threadIdx.x + 256 * i];
+#pragma unroll
+        for (int i = 0; i < PT; ++i) bs[threadIdx.x + 256 * i] = st[i];
+    }
+    const double *Cd = (const double *) C;
+    // tasks of this wave: rows z0..z1-1, n-tiles wave, wave+4, ... (one tile
+    // in the mean row)
+    auto ntiles = [&](int zrow) { return zrow == a.Nz / 2 ? 1 : nnt; };
+    auto advance = [&](int &zrow, int &nt) {
+        nt += 4;
+        while (zrow < z1 && nt >= ntiles(zrow)) {
+            ++zrow;
+            nt = wave;
+        }
+    };
+    auto load = [&](int zrow, int nt, double *ca, double *cb) {
+        const RebuildTask t = rebuild_task(a, zrow, nt, nl);
+#pragma unroll
+        for (int s = 0; s < NKS; ++s) {
+            const size_t j = min(s * 4 + kq, nR - 1);
+            ca[s] = Cd[t.offA + j * a.Mx * 2];
+            cb[s] = Cd[t.offB + j * a.Mx * 2];
+        }
+    };
+    auto run = [&](int zrow, int nt, const double *xa, const double *xb) {
+        mf4 acc[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) acc[m] = mf4_zero();
+        if (zrow == a.Nz / 2) {
+#pragma unroll
+            for (int s = 0; s < NKS; ++s) {
+                acc[3] = mfma_f64(xa[s], bs[(s * 5 + 3) * 64 + lane], acc[3]);
+                acc[4] = mfma_f64(xa[s], bs[(s * 5 + 4) * 64 + lane], acc[4]);
+            }
+        } else {
+            // basis fragments move LDS -> registers two k-steps at a time, one
+            // group ahead of the products that use them; the scheduling
+            // barriers keep the compiler from sinking the loads to their uses
+            double p0[10], p1[10];
+            auto ldb = [&](int s0, double *p) {
+#pragma unroll
+                for (int e = 0; e < 10; ++e) p[e] = bs[(s0 * 5 + e) * 64 + lane];
+            };
+            auto mm = [&](int s0, const double *p) {
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const int s = s0 + t;
+                    acc[0] = mfma_f64(xa[s], p[t * 5 + 0], acc[0]);
+                    acc[1] = mfma_f64(xa[s], p[t * 5 + 1], acc[1]);
+                    acc[2] = mfma_f64(xa[s], p[t * 5 + 2], acc[2]);
+                    acc[3] = mfma_f64(xb[s], p[t * 5 + 3], acc[3]);
+                    acc[4] = mfma_f64(xb[s], p[t * 5 + 4], acc[4]);
+                }
+            };
+            ldb(0, p0);
+#pragma unroll
+            for (int s0 = 0; s0 < NKS; s0 += 4) {
+                if (s0 + 2 < NKS) ldb(s0 + 2, p1);
+                __builtin_amdgcn_sched_barrier(0);
+                mm(s0, p0);
+                __builtin_amdgcn_sched_barrier(0);
+                if (s0 + 2 >= NKS) break;
+                if (s0 + 4 < NKS) ldb(s0 + 4, p0);
+                __builtin_amdgcn_sched_barrier(0);
+                mm(s0 + 2, p1);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        rebuild_store(a, zrow, nt, yt * 16 + nl, kq, acc, U);
+    };
+    __syncthreads();
+    int za = z0, nta = wave;
+    if (za < z1 && nta >= ntiles(za)) {
+        nta -= 4;
+        advance(za, nta);
+    }
+    for (; za < z1; advance(za, nta)) {
+        double ca[NKS], cb[NKS];
+        load(za, nta, ca, cb);
+        run(za, nta, ca, cb);
+    }
+}
+
 void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst)
 {
     const Dims &d = c.d;
-    RebuildArgs a{c.dFragR, c.dKx, c.dKz, d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, c.nksR, c.abort_flag};
-    dim3 grid(c.nyt, d.Nz);
+    // segments of kz rows: about two blocks per CU
+    const int nseg = std::max(1, std::min(d.Nz, 512 / c.nyt));
+    RebuildArgs a{c.dFragR, c.dKx, c.dKz, d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, c.nksR, c.nyt,
+                  nseg, 2. * M_PI / d.Lx, 2. * M_PI / d.Lz, c.abort_flag};
     double bytes = 16.0 * (double) (d.Nz - 1) * d.Mx * (2.0 * d.dimR + 5.0 * d.qpts);
     prof_begin(c, c.stream, 4, bytes);
-    hipLaunchKernelGGL(k_rebuild, grid, dim3(256), 0, c.stream, a, Csrc, Udst);
+    if (c.nksR == 8)
+        hipLaunchKernelGGL(k_rebuild_rows<8>, dim3(xcd_grid(c.nyt * nseg)), dim3(256), 0, c.stream,
+                           a, Csrc, Udst);
+    else if (c.nksR == 16)
+        hipLaunchKernelGGL(k_rebuild_rows<16>, dim3(xcd_grid(c.nyt * nseg)), dim3(256), 0, c.stream,
+                           a, Csrc, Udst);
+    else if (c.nksR == 24)
+        hipLaunchKernelGGL(k_rebuild_rows<24>, dim3(xcd_grid(c.nyt * nseg)), dim3(256), 0, c.stream,
+                           a, Csrc, Udst);
+    else
+        hipLaunchKernelGGL(k_rebuild, dim3(xcd_grid(c.nyt * d.Nz)), dim3(256), 0, c.stream, a, Csrc,
+                           Udst);
     prof_end(c, c.stream, 4);
 }
 
 // ------------------------------------------------------------ projection
 struct ForceArgs {
     const double *frag, *Kx, *Kz;
-    int Nz, Mx, qpts, dimR, dimQ, nks, adj;
+    int Nz, Mx, qpts, dimR, dimQ, nks, nit;
+    double ax, az;                      // 2 pi / Lx, 2 pi / Lz
     const int *abort_flag;
 };
 
-// Fa[z][i][x] (i < dimQ), Fb[z][i][x] (i < dimR); grid (nit, Nz), 256 threads.
+// Fa[z][i][x] (i < dimQ), Fb[z][i][x] (i < dimR); 1-D grid over the (kz row,
+// i-tile) tiles, i-tile fastest, XCD-aware (xcd_tile); 256 threads.
 // D[n][i] = sum_y A_m[y][n] W_m[i][y] (channel/project.c:95-115,176-191,
 // channel/adjproject.c:108-134,213-234) with the pointwise combinations
 //   fa = Qpw P - k2 Qw Hy (+ Qppw Z),     P  = i kx Hx + i kz Hz
@@ -189,92 +444,117 @@ struct ForceArgs {
 // summed into two accumulators.  Each lane forms its A entries from the parts
 // of Hx, Hz, Hy, Sx, Sz it needs.  Mean column (channel/project0.c:110-146,
 // channel/adjproject0.c:98-140): Fb(0,0) = (Rw Re Hx + Rpw Re Sz,
-// Rw Re Hz - Rpw Re Sx).  Wave w takes the n-tiles w, w+4, ... of i-tile
-// blockIdx.x; the weighted-basis fragments of a chunk of KC_FORCE k-steps are
+// Rw Re Hz - Rpw Re Sx).  Wave w takes the n-tiles w, w+4, ... of the
+// block's i-tile; the weighted-basis fragments of a chunk of KC_FORCE k-steps are
 // staged in LDS (double-buffered), the field values prefetched a chunk ahead.
-__global__ __launch_bounds__(256) void k_force(ForceArgs a, const cplx *U, cplx *Fa, cplx *Fb)
+template <bool ADJ>
+__global__ __launch_bounds__(256, SC_YT_WAVES) void k_force(ForceArgs a, const cplx *U, cplx *Fa, cplx *Fb)
 {
     constexpr int KC = KC_FORCE, CH = KC * 5 * 64, PT = CH / 256;
     static_assert(CH % 256 == 0, "chunk must split over the block");
     __shared__ double bs[2][CH];
     if (*a.abort_flag) return;
-    const int z = blockIdx.y;
+    const int tile = xcd_tile(a.nit * a.Nz);
+    if (tile < 0) return;
+    const int z = tile / a.nit, it = tile - z * a.nit;
     if (z == a.Nz / 2) return;
-    const int it = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int Mx = a.Mx, q = a.qpts, nR = a.dimR, nQ = a.dimQ, adj = a.adj;
+    const int lane = threadIdx.x & 63, wave = wave_id();
+    const int Mx = a.Mx, q = a.qpts, nR = a.dimR, nQ = a.dimQ;
+    constexpr bool adj = ADJ;
     const int nnt = (Mx + 7) / 8, nch = a.nks / KC;
     const int kq = lane >> 4, nl = lane & 15;
     const double *frag = a.frag + (size_t) it * a.nks * 5 * 64;
     const size_t ss = (size_t) q * Mx;
     const double *Ud = (const double *) (U + (size_t) z * 5 * ss);
-    const double kz = a.Kz[z];
+    const double kz = wave_kz(z, a.Nz, a.az);
     for (int nt0 = 0; nt0 < nnt; nt0 += 4) {
         const int nt = nt0 + wave;
         const bool active = nt < nnt;
-        const int xcol = nt * 8 + ncol_x(nl), part = ncol_part(nl);
-        const bool xok = active && xcol < Mx;
+        // columns past Mx read the last column and are never stored
+        const int xcol = min(nt * 8 + ncol_x(nl), Mx - 1), part = ncol_part(nl);
         const bool mean = z == 0 && xcol == 0;
-        const double kx = xok ? a.Kx[xcol] : 0.0, k2 = kz * kz + kx * kx;
+        const double kx = wave_kx(xcol, a.ax), k2 = kz * kz + kx * kx;
         const double sg = part ? 1.0 : -1.0;
         const int other = mean ? 0 : 1 - part;
         mf4 accA = mf4_zero(), accB = mf4_zero();
-        double rhx[KC], rhz[KC], rhy[KC], rsx[KC], rsz[KC], st[PT];
-        auto load_chunk = [&](int ch) {
+        // two register sets of field values, used alternately:
+        // r[0] Hx, r[1] Hz, r[2] Hy, r[3] Sx, r[4] Sz
+        double r0[5][KC], r1[5][KC], st[PT];
+        // unpredicated loads: Gauss points past qpts read the last point and
+        // meet zero basis fragments
+        auto load_fields = [&](int ch, double (*r)[KC]) {
 #pragma unroll
             for (int s = 0; s < KC; ++s) {
-                const int y = (ch * KC + s) * 4 + kq;
-                const bool ok = xok && y < q;
+                const int y = min((ch * KC + s) * 4 + kq, q - 1);
                 const size_t e = ((size_t) y * Mx + xcol) * 2;
-                rhz[s] = ok ? Ud[e + other] : 0.0;
-                rhx[s] = ok ? Ud[e + 2 * ss + other] : 0.0;
-                rhy[s] = ok ? Ud[e + 4 * ss + part] : 0.0;
-                rsx[s] = (ok && adj) ? Ud[e + 6 * ss + other] : 0.0;
-                rsz[s] = (ok && adj) ? Ud[e + 8 * ss + other] : 0.0;
+                r[1][s] = Ud[e + other];
+                r[0][s] = Ud[e + 2 * ss + other];
+                r[2][s] = Ud[e + 4 * ss + part];
+                if (adj) {
+                    r[3][s] = Ud[e + 6 * ss + other];
+                    r[4][s] = Ud[e + 8 * ss + other];
+                }
             }
+        };
+        auto load_basis = [&](int ch) {
             const double *src = frag + (size_t) ch * CH;
 #pragma unroll
             for (int i = 0; i < PT; ++i) st[i] = src[threadIdx.x + 256 * i];
         };
-        load_chunk(0);
+        auto put_basis = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < PT; ++i) bs[0][threadIdx.x + 256 * i] = st[i];
-        __syncthreads();
-        for (int ch = 0; ch < nch; ++ch) {
-            double vP[KC], vY[KC], vZ[KC], vP2[KC], vZ2[KC];
+            for (int i = 0; i < PT; ++i) bs[buf][threadIdx.x + 256 * i] = st[i];
+        };
+        auto compute = [&](double (*r)[KC], const double *b) {
+            if (!active) return;
 #pragma unroll
             for (int s = 0; s < KC; ++s) {
+                const double hx = r[0][s], hz = r[1][s], hy = r[2][s];
+                double vP, vY, vP2, vZ = 0.0, vZ2 = 0.0;
                 if (mean) {
-                    vP[s] = 0.0;
-                    vY[s] = 0.0;
-                    vZ[s] = 0.0;
-                    vP2[s] = part ? rhz[s] : rhx[s];
-                    vZ2[s] = part ? -rsx[s] : rsz[s];
+                    vP = 0.0;
+                    vY = 0.0;
+                    vP2 = part ? hz : hx;
+                    if (adj) vZ2 = part ? -r[3][s] : r[4][s];
                 } else {
-                    vP[s] = sg * (kx * rhx[s] + kz * rhz[s]);
-                    vY[s] = -k2 * rhy[s];
-                    vP2[s] = sg * (kz * rhx[s] - kx * rhz[s]);
-                    vZ[s] = -sg * (kz * rsx[s] - kx * rsz[s]);
-                    vZ2[s] = sg * (kz * rsz[s] + kx * rsx[s]);
-                }
-            }
-            if (ch + 1 < nch) load_chunk(ch + 1);
-            const double *b = bs[ch & 1];
-            if (active) {
-#pragma unroll
-                for (int s = 0; s < KC; ++s) {
-                    accA = mfma_f64(vP[s], b[(s * 5 + 0) * 64 + lane], accA);
-                    accB = mfma_f64(vP2[s], b[(s * 5 + 3) * 64 + lane], accB);
-                    accA = mfma_f64(vY[s], b[(s * 5 + 1) * 64 + lane], accA);
+                    vP = sg * (kx * hx + kz * hz);
+                    vY = -k2 * hy;
+                    vP2 = sg * (kz * hx - kx * hz);
                     if (adj) {
-                        accA = mfma_f64(vZ[s], b[(s * 5 + 2) * 64 + lane], accA);
-                        accB = mfma_f64(vZ2[s], b[(s * 5 + 4) * 64 + lane], accB);
+                        vZ = -sg * (kz * r[3][s] - kx * r[4][s]);
+                        vZ2 = sg * (kz * r[4][s] + kx * r[3][s]);
                     }
                 }
+                accA = mfma_f64(vP, b[(s * 5 + 0) * 64 + lane], accA);
+                accB = mfma_f64(vP2, b[(s * 5 + 3) * 64 + lane], accB);
+                accA = mfma_f64(vY, b[(s * 5 + 1) * 64 + lane], accA);
+                if (adj) {
+                    accA = mfma_f64(vZ, b[(s * 5 + 2) * 64 + lane], accA);
+                    accB = mfma_f64(vZ2, b[(s * 5 + 4) * 64 + lane], accB);
+                }
             }
-            if (ch + 1 < nch) {
-#pragma unroll
-                for (int i = 0; i < PT; ++i) bs[(ch + 1) & 1][threadIdx.x + 256 * i] = st[i];
+        };
+        load_fields(0, r0);
+        load_basis(0);
+        put_basis(0);
+        __syncthreads();
+        for (int ch = 0; ch < nch; ch += 2) {
+            const bool n1 = ch + 1 < nch;
+            if (n1) {
+                load_fields(ch + 1, r1);
+                load_basis(ch + 1);
             }
+            compute(r0, bs[0]);
+            if (n1) put_basis(1);
+            __syncthreads();
+            if (!n1) break;
+            const bool n2 = ch + 2 < nch;
+            if (n2) {
+                load_fields(ch + 2, r0);
+                load_basis(ch + 2);
+            }
+            compute(r1, bs[1]);
+            if (n2) put_basis(0);
             __syncthreads();
         }
         const int i = it * 16 + nl;
@@ -293,13 +573,16 @@ __global__ __launch_bounds__(256) void k_force(ForceArgs a, const cplx *U, cplx 
 void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb)
 {
     const Dims &d = c.d;
-    ForceArgs a{c.dFragF, c.dKx, c.dKz, d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, c.nksF, adj,
-                c.abort_flag};
-    dim3 grid(c.nit, d.Nz);
+    ForceArgs a{c.dFragF, c.dKx, c.dKz, d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, c.nksF, c.nit,
+                2. * M_PI / d.Lx, 2. * M_PI / d.Lz, c.abort_flag};
+    dim3 grid(xcd_grid(c.nit * d.Nz));
     double bytes = 16.0 * (double) (d.Nz - 1) * d.Mx *
                    ((adj ? 5.0 : 3.0) * d.qpts + d.dimQ + d.dimR);
     prof_begin(c, c.stream, 5, bytes);
-    hipLaunchKernelGGL(k_force, grid, dim3(256), 0, c.stream, a, Usrc, Fa, Fb);
+    if (adj)
+        hipLaunchKernelGGL(k_force<true>, grid, dim3(256), 0, c.stream, a, Usrc, Fa, Fb);
+    else
+        hipLaunchKernelGGL(k_force<false>, grid, dim3(256), 0, c.stream, a, Usrc, Fa, Fb);
     prof_end(c, c.stream, 5);
 }
 

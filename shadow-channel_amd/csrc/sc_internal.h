@@ -23,8 +23,19 @@ namespace sc {
 typedef double2 cplx;
 
 constexpr int MAXF = 12;        // fields in one z-inverse launch
-constexpr int KC_REBUILD = 8;   // MFMA k-steps per staged chunk, coefficients -> Gauss points
-constexpr int KC_FORCE = 4;     // MFMA k-steps per staged chunk, Gauss points -> coefficients
+// MFMA k-steps per staged chunk of the y-transforms (tuning constants)
+#ifndef SC_KC_REBUILD
+#define SC_KC_REBUILD 8
+#endif
+#ifndef SC_KC_FORCE
+#define SC_KC_FORCE 4
+#endif
+// minimum waves per SIMD the y-transform kernels are compiled for
+#ifndef SC_YT_WAVES
+#define SC_YT_WAVES 2
+#endif
+constexpr int KC_REBUILD = SC_KC_REBUILD;   // coefficients -> Gauss points
+constexpr int KC_FORCE = SC_KC_FORCE;       // Gauss points -> coefficients
 constexpr int MAX_FACTORS = 24; // radix factors of an FFT length
 
 struct FftPlan {
