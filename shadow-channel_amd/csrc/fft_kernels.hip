@@ -20,13 +20,16 @@
 namespace sc {
 
 constexpr int BLK = 256;
-// complex values staged in LDS by one x-kernel block (keeps static + dynamic
-// LDS of that kernel under 64 KiB)
-constexpr int XMIX_TILE = 3584;
+// complex values staged in LDS by one x-kernel block, data and twiddles
+// (keeps static + dynamic LDS of that kernel under 64 KiB)
+constexpr int XMIX_TILE = XMIX_LDS;
 
 struct PlanDev {
     int n, nf;
     int f[MAX_FACTORS];
+    // per pass: ceil(2^32 / Ns), Ns = product of the earlier radices, so that
+    // x / Ns = (x * nsm) >> 32 exactly for x <= 8192, Ns <= 4096
+    unsigned long long nsm[MAX_FACTORS];
     const double2 *tw;
 };
 
@@ -35,10 +38,43 @@ static PlanDev to_dev(const FftPlan &p)
     PlanDev d;
     d.n = p.n;
     d.nf = p.nf;
-    for (int i = 0; i < MAX_FACTORS; ++i) d.f[i] = i < p.nf ? p.f[i] : 0;
+    unsigned long long ns = 1;
+    for (int i = 0; i < MAX_FACTORS; ++i) {
+        d.f[i] = i < p.nf ? p.f[i] : 0;
+        d.nsm[i] = ((1ull << 32) + ns - 1) / ns;
+        if (i < p.nf) ns *= (unsigned long long) p.f[i];
+    }
     d.tw = p.tw;
     return d;
 }
+
+__device__ __forceinline__ int magic_div(int x, unsigned long long m)
+{
+    return (int) (((unsigned long long) (unsigned) x * m) >> 32);
+}
+
+// (index / L, index % L) of the elements threadIdx.x, threadIdx.x + BLK, ...
+// of a row-major walk with rows of length L, without divisions in the walk
+struct Walk {
+    int q, r, dq, dr, L;
+    __device__ __forceinline__ Walk(int L_) : L(L_)
+    {
+        // one division per walk
+        q = (int) threadIdx.x / L;
+        r = (int) threadIdx.x - q * L;
+        dq = BLK / L;
+        dr = BLK - dq * L;
+    }
+    __device__ __forceinline__ void next()
+    {
+        q += dq;
+        r += dr;
+        if (r >= L) {
+            r -= L;
+            ++q;
+        }
+    }
+};
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 {
@@ -93,77 +129,153 @@ __device__ __forceinline__ void small_dft(double2 *v, int r, double sign)
     }
 }
 
-// One Stockham pass of radix R over nseq sequences of length n (in place in
-// LDS).  Each thread keeps its butterflies' inputs in registers across the
-// barrier; a block of 256 threads holds at most 4096 complex values, so a
-// thread owns at most ceil(16/R) butterflies.
+// One Stockham pass of radix R over nseq sequences of length n, in place in
+// LDS: element e of sequence s at data[s*sstride + e*estride].  Lanes run
+// over the sequences fastest, so with estride = nseq (sequences interleaved)
+// every access of a pass is a contiguous run, and with sstride odd and
+// estride = 1 neighbouring lanes fall in different banks.  Each thread keeps
+// its butterflies' inputs in registers across the barrier; a block of 256
+// threads holds at most 4096 complex values, so a thread owns at most
+// ceil(16/R) butterflies.
 template <int R>
-__device__ __forceinline__ void fft_pass(double2 *data, int nseq, int stride, int n, int Ns,
-                                         const double2 *tw, double sign)
+__device__ __forceinline__ void fft_pass(double2 *data, int nseq, int sstride, int estride, int n,
+                                         int Ns, unsigned long long nsm, const double2 *tw,
+                                         double sign)
 {
     constexpr int NB = (16 + R - 1) / R;
     const int nb = n / R;
-    const int total = nseq * nb;
-    const int twstep = n / (Ns * R);
+    const int twstep = magic_div(nb, nsm);      // n / (Ns R)
     double2 v[NB][R];
+    int off[NB];                                // output position of the butterfly
+    Walk w(nseq);                               // w.q = butterfly index j, w.r = sequence s
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-        int b = threadIdx.x + i * BLK;
-        if (b < total) {
-            int s = b / nb, j = b - s * nb;
-            const double2 *src = data + s * stride;
-            int k = j % Ns;
+        const int j = w.q, s = w.r;
+        off[i] = -1;
+        if (j < nb) {
+            const double2 *src = data + s * sstride;
+            const int jq = magic_div(j, nsm), k = j - jq * Ns;
+            off[i] = s * sstride + (jq * Ns * R + k) * estride;
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                double2 a = src[j + q * nb];
+                double2 a = src[(j + q * nb) * estride];
                 if (q > 0) {
-                    double2 w = tw[q * k * twstep];
-                    w.y *= sign;
-                    a = cmul(a, w);
+                    double2 t = tw[q * k * twstep];
+                    t.y *= sign;
+                    a = cmul(a, t);
                 }
                 v[i][q] = a;
             }
             small_dft(v[i], R, sign);
         }
+        w.next();
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-        int b = threadIdx.x + i * BLK;
-        if (b < total) {
-            int s = b / nb, j = b - s * nb;
-            double2 *dst = data + s * stride;
-            int k = j % Ns;
-            int base = (j / Ns) * Ns * R + k;
+        if (off[i] >= 0) {
+            double2 *dst = data + off[i];
 #pragma unroll
-            for (int q = 0; q < R; ++q) dst[base + q * Ns] = v[i][q];
+            for (int q = 0; q < R; ++q) dst[q * Ns * estride] = v[i][q];
         }
     }
     __syncthreads();
 }
 
-// In-place FFT of nseq sequences of length p.n, sequence s at data[s*stride],
-// nseq * p.n <= 4096.  All threads of the block must call it.
-// sign = +1: backward (exp(+i)), -1: forward.
-__device__ void lds_fft(double2 *data, int nseq, int stride, const PlanDev &p, double sign)
+// tw: the plan's twiddles exp(2 pi i m / n), m < n, staged in LDS
+__device__ __forceinline__ void lds_fft(double2 *data, int nseq, int sstride, int estride,
+                                        const PlanDev &p, const double2 *tw, double sign)
 {
     int Ns = 1;
     for (int pass = 0; pass < p.nf; ++pass) {
         const int r = p.f[pass];
-        if (r == 4) fft_pass<4>(data, nseq, stride, p.n, Ns, p.tw, sign);
-        else if (r == 2) fft_pass<2>(data, nseq, stride, p.n, Ns, p.tw, sign);
-        else if (r == 3) fft_pass<3>(data, nseq, stride, p.n, Ns, p.tw, sign);
-        else fft_pass<5>(data, nseq, stride, p.n, Ns, p.tw, sign);
+        const unsigned long long m = p.nsm[pass];
+        if (r == 4) fft_pass<4>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign);
+        else if (r == 2) fft_pass<2>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign);
+        else if (r == 3) fft_pass<3>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign);
+        else fft_pass<5>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign);
         Ns *= r;
     }
+}
+
+// copy the plan's twiddle table into LDS (visible after the caller's barrier)
+__device__ __forceinline__ void stage_twiddles(const PlanDev &p, double2 *twl)
+{
+    for (int m = threadIdx.x; m < p.n; m += BLK) twl[m] = p.tw[m];
 }
 
 struct ZinvArgs {
     FieldSpec fs;
     const double *Kx, *Kz;
     int Nz, Mx, qpts, Pz, XB;
+    double ax, az;                 // 2 pi / Lx, 2 pi / Lz
     const int *abort_flag;
 };
+
+constexpr int MAXE = 4096 / BLK;   // LDS elements per thread of a 4096-value tile
+
+// Stage a (Pz x XB) tile of one field into LDS in its memory layout
+// [zp][x], zero rows for the padding, the spectral curl formed on the fly.
+// All loads of a thread are issued before the first LDS store: indices are
+// clamped into the array and the padding masked arithmetically, so no load
+// sits behind a branch.
+template <int KIND>
+__device__ __forceinline__ void zinv_stage(const ZinvArgs &a, const cplx *src, int slot, int y,
+                                           int x0, int XB, double2 *lds)
+{
+    const int Nz = a.Nz, Pz = a.Pz, Mx = a.Mx, total = XB * Pz;
+    const size_t zs = (size_t) 5 * a.qpts * Mx;        // stride of kz in U
+    const size_t ss = (size_t) a.qpts * Mx;            // stride of slot in U
+    double2 p[MAXE], q[MAXE];
+    bool ok[MAXE];
+    int kzi[MAXE], xi[MAXE];
+    Walk wk(XB);                                       // (zp, xl)
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+        const int e = threadIdx.x + i * BLK;
+        const int zp = min(wk.q, Pz - 1), xl = wk.r;
+        wk.next();
+        const int kz = zp < Nz / 2 ? zp : (zp > Nz ? zp - Nz / 2 : -1);
+        ok[i] = e < total && kz >= 0;
+        kzi[i] = max(kz, 0);
+        xi[i] = x0 + xl;
+        const cplx *b = src + kzi[i] * zs + (size_t) y * Mx + xi[i];
+        if (KIND == FK_SLOT) {
+            p[i] = b[slot * ss];
+        } else if (KIND == FK_OMEGA_Z) {
+            p[i] = b[3 * ss];
+            q[i] = b[1 * ss];
+        } else if (KIND == FK_OMEGA_X) {
+            p[i] = b[4 * ss];
+            q[i] = b[1 * ss];
+        } else {
+            p[i] = b[0];
+            q[i] = b[2 * ss];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+        const int e = threadIdx.x + i * BLK;
+        if (e >= total) break;
+        const double kx = xi[i] * a.ax;
+        const int kz = kzi[i];
+        const double kzv = (kz <= Nz / 2 ? kz : kz - Nz) * a.az;
+        double2 val;
+        if (KIND == FK_SLOT) {
+            val = p[i];
+        } else if (KIND == FK_OMEGA_Z) {            // -du/dy + i kx v
+            const double2 du = p[i], v = q[i];
+            val = make_double2(-(du.x + kx * v.y), -du.y + kx * v.x);
+        } else if (KIND == FK_OMEGA_X) {            // dw/dy - i kz v
+            const double2 dw = p[i], v = q[i];
+            val = make_double2(dw.x + kzv * v.y, dw.y - kzv * v.x);
+        } else {                                    // i kz u - i kx w
+            const double2 u = p[i], w = q[i];
+            val = make_double2(kx * w.y - kzv * u.y, kzv * u.x - kx * w.x);
+        }
+        lds[e] = ok[i] ? val : make_double2(0.0, 0.0);
+    }
+}
 
 // grid (ceil(Mx/XB), nf, qpts)
 __global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, cplx *W1)
@@ -172,40 +284,22 @@ __global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, cplx *W1)
     if (*a.abort_flag) return;
     const int x0 = blockIdx.x * a.XB, f = blockIdx.y, y = blockIdx.z;
     const int XB = min(a.XB, a.Mx - x0);
-    const int Nz = a.Nz, Pz = a.Pz, Mx = a.Mx;
+    const int Pz = a.Pz, Mx = a.Mx;
     const cplx *src = a.fs.src[f];
     const int kind = a.fs.kind[f], slot = a.fs.slot[f];
-    const size_t zs = (size_t) 5 * a.qpts * Mx;        // stride of kz in U
-    const size_t ss = (size_t) a.qpts * Mx;            // stride of slot in U
-    for (int e = threadIdx.x; e < XB * Pz; e += blockDim.x) {
-        int zp = e / XB, xl = e - zp * XB, x = x0 + xl;
-        int kz = zp < Nz / 2 ? zp : (zp > Nz ? zp - Nz / 2 : -1);
-        double2 val = make_double2(0.0, 0.0);
-        if (kz >= 0) {
-            const cplx *b = src + kz * zs + (size_t) y * Mx + x;
-            double kx = a.Kx[x], kzv = a.Kz[kz];
-            if (kind == FK_SLOT) {
-                val = b[slot * ss];
-            } else if (kind == FK_OMEGA_Z) {           // -du/dy + i kx v
-                double2 du = b[3 * ss], v = b[1 * ss];
-                val = make_double2(-(du.x + kx * v.y), -du.y + kx * v.x);
-            } else if (kind == FK_OMEGA_X) {           // dw/dy - i kz v
-                double2 dw = b[4 * ss], v = b[1 * ss];
-                val = make_double2(dw.x + kzv * v.y, dw.y - kzv * v.x);
-            } else {                                   // i kz u - i kx w
-                double2 u = b[0], w = b[2 * ss];
-                val = make_double2(kx * w.y - kzv * u.y, kzv * u.x - kx * w.x);
-            }
-        }
-        lds[xl * Pz + zp] = val;
-    }
+    double2 *twl = lds + a.XB * Pz;
+    stage_twiddles(pz, twl);
+    if (kind == FK_SLOT) zinv_stage<FK_SLOT>(a, src, slot, y, x0, XB, lds);
+    else if (kind == FK_OMEGA_Z) zinv_stage<FK_OMEGA_Z>(a, src, slot, y, x0, XB, lds);
+    else if (kind == FK_OMEGA_X) zinv_stage<FK_OMEGA_X>(a, src, slot, y, x0, XB, lds);
+    else zinv_stage<FK_OMEGA_Y>(a, src, slot, y, x0, XB, lds);
     __syncthreads();
-    lds_fft(lds, XB, Pz, pz, +1.0);
+    // columns interleaved in LDS exactly as in memory: no transpose
+    lds_fft(lds, XB, 1, XB, pz, twl, +1.0);
     cplx *dst = W1 + (((size_t) y * gridDim.y + f) * Pz) * Mx;
-    for (int e = threadIdx.x; e < XB * Pz; e += blockDim.x) {
-        int zp = e / XB, xl = e - zp * XB;
-        dst[(size_t) zp * Mx + x0 + xl] = lds[xl * Pz + zp];
-    }
+    Walk wk(XB);
+    for (int e = threadIdx.x; e < XB * Pz; e += blockDim.x, wk.next())
+        dst[(size_t) wk.q * Mx + x0 + wk.r] = lds[e];
 }
 
 struct XmixArgs {
@@ -230,7 +324,9 @@ template <> struct XmixShape<XM_ADJOINT> { static constexpr int NF = 9, NOUT = 6
 template <> struct XmixShape<XM_PHYS> { static constexpr int NF = 3, NOUT = 0; };
 template <> struct XmixShape<XM_DOT> { static constexpr int NF = 6, NOUT = 0; };
 
-// grid (Pz/ZB, qpts); LDS: ZB * nseq * Px complex, nseq = ceil(nf/2)
+// grid (Pz/ZB, qpts); LDS: ZB * nseq sequences of Px complex at a stride of
+// Px + 1 (odd, so lanes on neighbouring sequences use different banks),
+// nseq = ceil(nf/2)
 template <int MODE>
 __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx *W1, cplx *W2)
 {
@@ -242,30 +338,52 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
     constexpr int nseq = (nf + 1) / 2;
     const int zb = blockIdx.x, y = blockIdx.y;
     const int Px = a.Px, Mx = a.Mx, Pz = a.Pz, ZB = a.ZB;
+    const int SP = Px + 1;
     const int z0 = zb * ZB;
-    // load packed pairs with Hermitian extension (c2r uses Re only at kx=0)
+    double2 *twl = lds + ZB * nseq * SP;
+    stage_twiddles(px, twl);
+    // load packed pairs with Hermitian extension (c2r uses Re only at kx=0):
+    // bin n < Mx takes X[n] + i Y[n], bin n > Px - Mx the conjugates at
+    // Px - n, the bins between are zero.  Loads unpredicated (index clamped
+    // into the row), all issued before the LDS stores.
     const int tot = ZB * nseq * Px;
-    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-        int n = e % Px, t = e / Px, s = t % nseq, r = t / nseq;
-        int zp = z0 + r;
-        const cplx *X = W1 + (((size_t) y * nf + 2 * s) * Pz + zp) * Mx;
-        const cplx *Y = (2 * s + 1 < nf) ? X + (size_t) Pz * Mx : nullptr;
-        double2 val = make_double2(0.0, 0.0);
-        if (n == 0) {
-            val.x = X[0].x;
-            val.y = Y ? Y[0].x : 0.0;
-        } else if (n < Mx) {
-            double2 xv = X[n], yv = Y ? Y[n] : make_double2(0.0, 0.0);
+    constexpr int MAXL = (XMIX_TILE + BLK - 1) / BLK;
+    double2 lx[MAXL], ly[MAXL];
+    Walk wl(Px);                                       // (packed row t, bin n)
+#pragma unroll
+    for (int i = 0; i < MAXL; ++i) {
+        const int t = min(wl.q, ZB * nseq - 1), n = wl.r;
+        wl.next();
+        const int s = t % nseq, r = t / nseq;
+        const int k = min(n < Mx ? n : Px - n, Mx - 1);
+        const cplx *X = W1 + (((size_t) y * nf + 2 * s) * Pz + z0 + r) * Mx;
+        const int sy = min(2 * s + 1, nf - 1) - 2 * s;     // 1, or 0 for a lone field
+        lx[i] = X[k];
+        ly[i] = X[(size_t) sy * Pz * Mx + k];
+    }
+    wl = Walk(Px);
+#pragma unroll
+    for (int i = 0; i < MAXL; ++i) {
+        const int e = threadIdx.x + i * BLK;
+        if (e >= tot) break;
+        const int t = wl.q, n = wl.r;
+        wl.next();
+        const int s = t % nseq, r = t / nseq;
+        const bool hasy = 2 * s + 1 < nf;
+        double2 xv = lx[i], yv = hasy ? ly[i] : make_double2(0.0, 0.0);
+        double2 val;
+        if (n == 0)
+            val = make_double2(xv.x, yv.x);
+        else if (n < Mx)
             val = make_double2(xv.x - yv.y, xv.y + yv.x);
-        } else if (n > Px - Mx) {
-            int k = Px - n;
-            double2 xv = X[k], yv = Y ? Y[k] : make_double2(0.0, 0.0);
+        else if (n > Px - Mx)
             val = make_double2(xv.x + yv.y, -xv.y + yv.x);
-        }
-        lds[(r * nseq + s) * Px + n] = val;
+        else
+            val = make_double2(0.0, 0.0);
+        lds[(r * nseq + s) * SP + n] = val;
     }
     __syncthreads();
-    lds_fft(lds, ZB * nseq, Px, px, +1.0);
+    lds_fft(lds, ZB * nseq, SP, 1, px, twl, +1.0);
     // pointwise stage: read all fields of a point, compute, then (after a
     // barrier) write the packed outputs compactly: row r, pair t at sequence
     // r*nso + t
@@ -275,17 +393,19 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
     double outv[MAXP][NO];
     double cmax = 0.0, acc0 = 0.0, acc1 = 0.0;
     const int npt = ZB * Px;
+    Walk wp(Px);                                       // (z row r, point n)
 #pragma unroll
     for (int i = 0; i < MAXP; ++i) {
         int e = threadIdx.x + i * BLK;
         if (e >= npt) break;
-        int n = e % Px, r = e / Px;
-        const double2 *row = lds + (size_t) r * nseq * Px + n;
+        const int n = wp.r, r = wp.q;
+        wp.next();
+        const double2 *row = lds + (size_t) r * nseq * SP + n;
         double f[12];
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
             if (s < nseq) {
-                double2 v = row[s * Px];
+                double2 v = row[s * SP];
                 f[2 * s] = v.x;
                 f[2 * s + 1] = v.y;
             } else {
@@ -357,27 +477,30 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
     if constexpr (MODE == XM_PHYS) return;
     if constexpr (NOUT > 0) {
     __syncthreads();
+    wp = Walk(Px);
 #pragma unroll
     for (int i = 0; i < MAXP; ++i) {
         int e = threadIdx.x + i * BLK;
         if (e >= npt) break;
-        int n = e % Px, r = e / Px;
+        const int n = wp.r, r = wp.q;
+        wp.next();
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
             if (t < nso) {
                 double re = outv[i][2 * t < NO ? 2 * t : 0];
                 double im = (2 * t + 1 < NOUT) ? outv[i][2 * t + 1 < NO ? 2 * t + 1 : 0] : 0.0;
-                lds[(size_t) (r * nso + t) * Px + n] = make_double2(re, im);
+                lds[(size_t) (r * nso + t) * SP + n] = make_double2(re, im);
             }
         }
     }
     __syncthreads();
-    lds_fft(lds, ZB * nso, Px, px, -1.0);
+    lds_fft(lds, ZB * nso, SP, 1, px, twl, -1.0);
     // unpack two real transforms from each packed pair, bins kx < Mx
     const int tot2 = ZB * nso * Mx;
-    for (int e = threadIdx.x; e < tot2; e += blockDim.x) {
-        int k = e % Mx, t = (e / Mx) % nso, r = e / (Mx * nso);
-        const double2 *Z = lds + (size_t) (r * nso + t) * Px;
+    Walk wu(Mx);                                       // (packed row, bin k)
+    for (int e = threadIdx.x; e < tot2; e += blockDim.x, wu.next()) {
+        const int k = wu.r, t = wu.q % nso, r = wu.q / nso;
+        const double2 *Z = lds + (size_t) (r * nso + t) * SP;
         double2 zk = Z[k], zm = Z[k == 0 ? 0 : Px - k];
         double2 X = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
         double2 Yv = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
@@ -404,17 +527,33 @@ __global__ __launch_bounds__(BLK) void k_zfwd(ZfwdArgs a, PlanDev pz, const cplx
     const int XB = min(a.XB, a.Mx - x0);
     const int Nz = a.Nz, Pz = a.Pz, Mx = a.Mx;
     const cplx *src = W2 + (((size_t) y * a.nout + g) * Pz) * Mx;
-    for (int e = threadIdx.x; e < XB * Pz; e += blockDim.x) {
-        int zp = e / XB, xl = e - zp * XB;
-        lds[xl * Pz + zp] = src[(size_t) zp * Mx + x0 + xl];
+    double2 *twl = lds + a.XB * Pz;
+    stage_twiddles(pz, twl);
+    {
+        // all loads issued before the LDS stores
+        const int total = XB * Pz;
+        double2 v[MAXE];
+        Walk wk(XB);
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i) {
+            const int zp = min(wk.q, Pz - 1), xl = wk.r;
+            wk.next();
+            v[i] = src[(size_t) zp * Mx + x0 + xl];
+        }
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i) {
+            const int e = threadIdx.x + i * BLK;
+            if (e < total) lds[e] = v[i];
+        }
     }
     __syncthreads();
-    lds_fft(lds, XB, Pz, pz, -1.0);
-    for (int e = threadIdx.x; e < XB * (Nz - 1); e += blockDim.x) {
-        int r = e / XB, xl = e - r * XB;
+    lds_fft(lds, XB, 1, XB, pz, twl, -1.0);
+    Walk wt(XB);
+    for (int e = threadIdx.x; e < XB * (Nz - 1); e += blockDim.x, wt.next()) {
+        const int r = wt.q, xl = wt.r;
         int kz = r < Nz / 2 ? r : r + 1;             // skip the Nyquist row
         int zp = kz < Nz / 2 ? kz : kz + Nz / 2;
-        double2 v = lds[xl * Pz + zp];
+        double2 v = lds[zp * XB + xl];
         v.x *= a.norm;
         v.y *= a.norm;
         size_t o;
@@ -456,9 +595,10 @@ int xmix_zblock(Context &c, int nf);
 
 static int pick_block(int Mx, int P, int nper)
 {
-    // columns per block so that the LDS tile stays <= 4096 complex
+    // columns per block so that the LDS tile plus the P twiddles stay <= 4096
+    // complex
     int xb = Mx;
-    while (xb > 1 && (size_t) xb * P * nper > 4096) xb = (xb + 1) / 2;
+    while (xb > 1 && (size_t) xb * P * nper + P > 4096) xb = (xb + 1) / 2;
     return xb;
 }
 
@@ -474,9 +614,11 @@ void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1)
     a.qpts = d.qpts;
     a.Pz = d.Pz;
     a.XB = pick_block(d.Mx, d.Pz, 1);
+    a.ax = 2. * M_PI / d.Lx;
+    a.az = 2. * M_PI / d.Lz;
     a.abort_flag = c.abort_flag;
     dim3 grid((d.Mx + a.XB - 1) / a.XB, fs.nf, d.qpts);
-    size_t lds = sizeof(double2) * a.XB * d.Pz;
+    size_t lds = sizeof(double2) * (a.XB * d.Pz + d.Pz);
     const double plane = 16.0 * (double) d.qpts * d.Mx;
     int slots = 0;
     for (int f = 0; f < fs.nf; ++f) {
@@ -518,7 +660,7 @@ void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
     a.dotpart = dot_out;
     a.abort_flag = c.abort_flag;
     dim3 grid(d.Pz / zb, d.qpts);
-    size_t lds = sizeof(double2) * zb * nseq * d.Px;
+    size_t lds = sizeof(double2) * (zb * nseq * (d.Px + 1) + d.Px);
     PlanDev pd = to_dev(c.px);
     const double rows = 16.0 * (double) d.qpts * d.Pz * d.Mx;
     double bytes = rows * (a.nf + a.nout);
@@ -547,7 +689,7 @@ int xmix_zblock(Context &c, int nf)
 {
     int nseq = (nf + 1) / 2, zb = 1;
     while (zb * 2 <= c.d.Pz && c.d.Pz % (zb * 2) == 0 &&
-           (size_t) zb * 2 * nseq * c.d.Px <= (size_t) XMIX_TILE)
+           (size_t) zb * 2 * nseq * (c.d.Px + 1) + c.d.Px <= (size_t) XMIX_TILE)
         zb *= 2;
     return zb;
 }
@@ -566,7 +708,7 @@ void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind)
     a.norm = 1.0 / ((3. * d.Nx / 2.) * (3. * d.Nz / 2.));
     a.abort_flag = c.abort_flag;
     dim3 grid((d.Mx + a.XB - 1) / a.XB, nout, d.qpts);
-    size_t lds = sizeof(double2) * a.XB * d.Pz;
+    size_t lds = sizeof(double2) * (a.XB * d.Pz + d.Pz);
     double bytes = 16.0 * (double) d.qpts * d.Mx * nout * (d.Pz + d.Nz - 1);
     prof_begin(c, c.stream, 3, bytes);
     hipLaunchKernelGGL(k_zfwd, grid, dim3(BLK), lds, c.stream, a, to_dev(c.pz), W2, dst);
