@@ -376,11 +376,11 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
     d.Mx = Nx / 2;
     d.Px = 3 * Nx / 2;
     d.Pz = 3 * Nz / 2;
-    // LDS tiles of the transforms (data plus twiddles within 64 KiB): the x
+    // transform tiles (data plus twiddles, FFT_TILE complex values): the x
     // kernel holds at least 6 padded rows of Px + 1 and a Px twiddle table,
-    // the z kernels at least 2 columns of Pz (one of them twiddles)
-    if (d.dimR > 512 || 6 * (d.Px + 1) + d.Px > XMIX_LDS || d.Pz > 2048) {
-        last_error() = "grid too large: need Ny <= 514, Nx <= 340 and Nz <= 1364";
+    // the z kernels one column of Pz and a Pz twiddle table
+    if (d.dimR > 512 || 6 * (d.Px + 1) + d.Px > FFT_TILE || 2 * d.Pz > FFT_TILE) {
+        last_error() = "grid too large: need Ny <= 514, Nx <= 194 and Nz <= 682";
         return SC_EINIT;
     }
     if (hipStreamCreate(&c.stream) != hipSuccess) {

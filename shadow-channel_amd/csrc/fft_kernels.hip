@@ -13,16 +13,17 @@
 //          transforms of packed product pairs, unpacking of the kx < Nx/2 bins
 //   zfwd : (y, field, kx block) -> forward length-Pz transforms, scale,
 //          truncation to the stored kz rows
-// The FFTs are Stockham passes of radix 4, 2, 3, 5 run in place in LDS: every
-// thread holds its butterflies' inputs in registers across a barrier.
+// The FFTs are Stockham passes run in place in LDS with radices up to 16 (the
+// composite ones done as small FFTs in registers, so a length of 96 takes two
+// passes): every thread holds its butterflies' inputs in registers across a
+// barrier.
 #include "sc_internal.h"
 
 namespace sc {
 
 constexpr int BLK = 256;
 // complex values staged in LDS by one x-kernel block, data and twiddles
-// (keeps static + dynamic LDS of that kernel under 64 KiB)
-constexpr int XMIX_TILE = XMIX_LDS;
+constexpr int XMIX_TILE = FFT_TILE;
 
 struct PlanDev {
     int n, nf;
@@ -81,53 +82,138 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// in-place radix-r DFT of v[0..r-1], exp(sign 2 pi i q s / r)
-__device__ __forceinline__ void small_dft(double2 *v, int r, double sign)
+// In-place DFTs in registers: v[q] <- sum_m v[m] exp(sign 2 pi i m q / R).
+template <int R> __device__ __forceinline__ void dft(double2 *v, double sign);
+
+template <> __device__ __forceinline__ void dft<2>(double2 *v, double sign)
 {
-    if (r == 2) {
-        double2 a = v[0], b = v[1];
-        v[0] = make_double2(a.x + b.x, a.y + b.y);
-        v[1] = make_double2(a.x - b.x, a.y - b.y);
-    } else if (r == 4) {
-        double2 a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
-        double2 s02 = make_double2(a0.x + a2.x, a0.y + a2.y);
-        double2 d02 = make_double2(a0.x - a2.x, a0.y - a2.y);
-        double2 s13 = make_double2(a1.x + a3.x, a1.y + a3.y);
-        double2 d13 = make_double2(a1.x - a3.x, a1.y - a3.y);
-        // w d13 with w = sign * i
-        double2 wd = make_double2(-sign * d13.y, sign * d13.x);
-        v[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
-        v[1] = make_double2(d02.x + wd.x, d02.y + wd.y);
-        v[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
-        v[3] = make_double2(d02.x - wd.x, d02.y - wd.y);
-    } else if (r == 3) {
-        const double h = 0.86602540378443864676;   // sqrt(3)/2
-        double2 a0 = v[0], a1 = v[1], a2 = v[2];
-        double2 t1 = make_double2(a1.x + a2.x, a1.y + a2.y);
-        double2 t2 = make_double2(a0.x - 0.5 * t1.x, a0.y - 0.5 * t1.y);
-        double2 t3 = make_double2(sign * h * (a1.x - a2.x), sign * h * (a1.y - a2.y));
-        v[0] = make_double2(a0.x + t1.x, a0.y + t1.y);
-        v[1] = make_double2(t2.x - t3.y, t2.y + t3.x);
-        v[2] = make_double2(t2.x + t3.y, t2.y - t3.x);
-    } else if (r == 5) {
-        const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
-        const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
-        double2 a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3], a4 = v[4];
-        double2 b1 = make_double2(a1.x + a4.x, a1.y + a4.y);
-        double2 b2 = make_double2(a2.x + a3.x, a2.y + a3.y);
-        double2 d1 = make_double2(a1.x - a4.x, a1.y - a4.y);
-        double2 d2 = make_double2(a2.x - a3.x, a2.y - a3.y);
-        double2 t1 = make_double2(a0.x + c1 * b1.x + c2 * b2.x, a0.y + c1 * b1.y + c2 * b2.y);
-        double2 t2 = make_double2(a0.x + c2 * b1.x + c1 * b2.x, a0.y + c2 * b1.y + c1 * b2.y);
-        double2 u1 = make_double2(sign * (s1 * d1.x + s2 * d2.x), sign * (s1 * d1.y + s2 * d2.y));
-        double2 u2 = make_double2(sign * (s2 * d1.x - s1 * d2.x), sign * (s2 * d1.y - s1 * d2.y));
-        v[0] = make_double2(a0.x + b1.x + b2.x, a0.y + b1.y + b2.y);
-        v[1] = make_double2(t1.x - u1.y, t1.y + u1.x);
-        v[4] = make_double2(t1.x + u1.y, t1.y - u1.x);
-        v[2] = make_double2(t2.x - u2.y, t2.y + u2.x);
-        v[3] = make_double2(t2.x + u2.y, t2.y - u2.x);
+    double2 a = v[0], b = v[1];
+    v[0] = make_double2(a.x + b.x, a.y + b.y);
+    v[1] = make_double2(a.x - b.x, a.y - b.y);
+}
+
+template <> __device__ __forceinline__ void dft<4>(double2 *v, double sign)
+{
+    double2 a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
+    double2 s02 = make_double2(a0.x + a2.x, a0.y + a2.y);
+    double2 d02 = make_double2(a0.x - a2.x, a0.y - a2.y);
+    double2 s13 = make_double2(a1.x + a3.x, a1.y + a3.y);
+    double2 d13 = make_double2(a1.x - a3.x, a1.y - a3.y);
+    // w d13 with w = sign * i
+    double2 wd = make_double2(-sign * d13.y, sign * d13.x);
+    v[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
+    v[1] = make_double2(d02.x + wd.x, d02.y + wd.y);
+    v[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
+    v[3] = make_double2(d02.x - wd.x, d02.y - wd.y);
+}
+
+template <> __device__ __forceinline__ void dft<3>(double2 *v, double sign)
+{
+    const double h = 0.86602540378443864676;   // sqrt(3)/2
+    double2 a0 = v[0], a1 = v[1], a2 = v[2];
+    double2 t1 = make_double2(a1.x + a2.x, a1.y + a2.y);
+    double2 t2 = make_double2(a0.x - 0.5 * t1.x, a0.y - 0.5 * t1.y);
+    double2 t3 = make_double2(sign * h * (a1.x - a2.x), sign * h * (a1.y - a2.y));
+    v[0] = make_double2(a0.x + t1.x, a0.y + t1.y);
+    v[1] = make_double2(t2.x - t3.y, t2.y + t3.x);
+    v[2] = make_double2(t2.x + t3.y, t2.y - t3.x);
+}
+
+template <> __device__ __forceinline__ void dft<5>(double2 *v, double sign)
+{
+    const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+    const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+    double2 a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3], a4 = v[4];
+    double2 b1 = make_double2(a1.x + a4.x, a1.y + a4.y);
+    double2 b2 = make_double2(a2.x + a3.x, a2.y + a3.y);
+    double2 d1 = make_double2(a1.x - a4.x, a1.y - a4.y);
+    double2 d2 = make_double2(a2.x - a3.x, a2.y - a3.y);
+    double2 t1 = make_double2(a0.x + c1 * b1.x + c2 * b2.x, a0.y + c1 * b1.y + c2 * b2.y);
+    double2 t2 = make_double2(a0.x + c2 * b1.x + c1 * b2.x, a0.y + c2 * b1.y + c1 * b2.y);
+    double2 u1 = make_double2(sign * (s1 * d1.x + s2 * d2.x), sign * (s1 * d1.y + s2 * d2.y));
+    double2 u2 = make_double2(sign * (s2 * d1.x - s1 * d2.x), sign * (s2 * d1.y - s1 * d2.y));
+    v[0] = make_double2(a0.x + b1.x + b2.x, a0.y + b1.y + b2.y);
+    v[1] = make_double2(t1.x - u1.y, t1.y + u1.x);
+    v[4] = make_double2(t1.x + u1.y, t1.y - u1.x);
+    v[2] = make_double2(t2.x - u2.y, t2.y + u2.x);
+    v[3] = make_double2(t2.x + u2.y, t2.y - u2.x);
+}
+
+// cos and sin of 2 pi j / R, j < R (correctly rounded; exact where the value
+// is 0, +-1/2 or +-1)
+template <int R> struct RootTab;
+template <> struct RootTab<6> {
+    static constexpr double c[6] = {1.0, 0.5, -0.5, -1.0, -0.5, 0.5};
+    static constexpr double s[6] = {0.0, 0.8660254037844386, 0.8660254037844386, 0.0, -0.8660254037844386, -0.8660254037844386};
+};
+template <> struct RootTab<8> {
+    static constexpr double c[8] = {1.0, 0.7071067811865476, 0.0, -0.7071067811865476, -1.0, -0.7071067811865476, 0.0, 0.7071067811865476};
+    static constexpr double s[8] = {0.0, 0.7071067811865476, 1.0, 0.7071067811865476, 0.0, -0.7071067811865476, -1.0, -0.7071067811865476};
+};
+template <> struct RootTab<9> {
+    static constexpr double c[9] = {1.0, 0.766044443118978, 0.17364817766693036, -0.5, -0.9396926207859084, -0.9396926207859084, -0.5, 0.17364817766693036, 0.766044443118978};
+    static constexpr double s[9] = {0.0, 0.6427876096865394, 0.984807753012208, 0.8660254037844386, 0.3420201433256687, -0.3420201433256687, -0.8660254037844386, -0.984807753012208, -0.6427876096865394};
+};
+template <> struct RootTab<10> {
+    static constexpr double c[10] = {1.0, 0.8090169943749475, 0.30901699437494745, -0.30901699437494745, -0.8090169943749475, -1.0, -0.8090169943749475, -0.30901699437494745, 0.30901699437494745, 0.8090169943749475};
+    static constexpr double s[10] = {0.0, 0.5877852522924731, 0.9510565162951535, 0.9510565162951535, 0.5877852522924731, 0.0, -0.5877852522924731, -0.9510565162951535, -0.9510565162951535, -0.5877852522924731};
+};
+template <> struct RootTab<12> {
+    static constexpr double c[12] = {1.0, 0.8660254037844386, 0.5, 0.0, -0.5, -0.8660254037844386, -1.0, -0.8660254037844386, -0.5, 0.0, 0.5, 0.8660254037844386};
+    static constexpr double s[12] = {0.0, 0.5, 0.8660254037844386, 1.0, 0.8660254037844386, 0.5, 0.0, -0.5, -0.8660254037844386, -1.0, -0.8660254037844386, -0.5};
+};
+template <> struct RootTab<15> {
+    static constexpr double c[15] = {1.0, 0.9135454576426009, 0.6691306063588582, 0.30901699437494745, -0.10452846326765347, -0.5, -0.8090169943749475, -0.9781476007338057, -0.9781476007338057, -0.8090169943749475, -0.5, -0.10452846326765347, 0.30901699437494745, 0.6691306063588582, 0.9135454576426009};
+    static constexpr double s[15] = {0.0, 0.4067366430758002, 0.7431448254773942, 0.9510565162951535, 0.9945218953682733, 0.8660254037844386, 0.5877852522924731, 0.20791169081775934, -0.20791169081775934, -0.5877852522924731, -0.8660254037844386, -0.9945218953682733, -0.9510565162951535, -0.7431448254773942, -0.4067366430758002};
+};
+template <> struct RootTab<16> {
+    static constexpr double c[16] = {1.0, 0.9238795325112867, 0.7071067811865476, 0.3826834323650898, 0.0, -0.3826834323650898, -0.7071067811865476, -0.9238795325112867, -1.0, -0.9238795325112867, -0.7071067811865476, -0.3826834323650898, 0.0, 0.3826834323650898, 0.7071067811865476, 0.9238795325112867};
+    static constexpr double s[16] = {0.0, 0.3826834323650898, 0.7071067811865476, 0.9238795325112867, 1.0, 0.9238795325112867, 0.7071067811865476, 0.3826834323650898, 0.0, -0.3826834323650898, -0.7071067811865476, -0.9238795325112867, -1.0, -0.9238795325112867, -0.7071067811865476, -0.3826834323650898};
+};
+
+// Composite R = R1 R2 in registers: with m = R2 m1 + m2 and q = k1 + R1 k2,
+// DFT_R = (R1 DFTs of length R2) . twiddles exp(sign 2 pi i m2 k1 / R) .
+// (R2 DFTs of length R1).  The twiddles are compile-time constants.
+template <int R1, int R2>
+__device__ __forceinline__ void dft_comp(double2 *v, double sign)
+{
+    constexpr int R = R1 * R2;
+    double2 a[R];
+#pragma unroll
+    for (int m2 = 0; m2 < R2; ++m2) {
+        double2 t[R1];
+#pragma unroll
+        for (int m1 = 0; m1 < R1; ++m1) t[m1] = v[R2 * m1 + m2];
+        dft<R1>(t, sign);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) {
+            const int e = (m2 * k1) % R;
+            if (e == 0) {
+                a[m2 * R1 + k1] = t[k1];
+            } else {
+                const double2 w = make_double2(RootTab<R>::c[e], sign * RootTab<R>::s[e]);
+                a[m2 * R1 + k1] = cmul(t[k1], w);
+            }
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+        double2 t[R2];
+#pragma unroll
+        for (int m2 = 0; m2 < R2; ++m2) t[m2] = a[m2 * R1 + k1];
+        dft<R2>(t, sign);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = t[k2];
     }
 }
+
+template <> __device__ __forceinline__ void dft<6>(double2 *v, double s) { dft_comp<3, 2>(v, s); }
+template <> __device__ __forceinline__ void dft<8>(double2 *v, double s) { dft_comp<4, 2>(v, s); }
+template <> __device__ __forceinline__ void dft<9>(double2 *v, double s) { dft_comp<3, 3>(v, s); }
+template <> __device__ __forceinline__ void dft<10>(double2 *v, double s) { dft_comp<5, 2>(v, s); }
+template <> __device__ __forceinline__ void dft<12>(double2 *v, double s) { dft_comp<4, 3>(v, s); }
+template <> __device__ __forceinline__ void dft<15>(double2 *v, double s) { dft_comp<5, 3>(v, s); }
+template <> __device__ __forceinline__ void dft<16>(double2 *v, double s) { dft_comp<4, 4>(v, s); }
 
 // One Stockham pass of radix R over nseq sequences of length n, in place in
 // LDS: element e of sequence s at data[s*sstride + e*estride].  Lanes run
@@ -135,14 +221,14 @@ __device__ __forceinline__ void small_dft(double2 *v, int r, double sign)
 // every access of a pass is a contiguous run, and with sstride odd and
 // estride = 1 neighbouring lanes fall in different banks.  Each thread keeps
 // its butterflies' inputs in registers across the barrier; a block of 256
-// threads holds at most 4096 complex values, so a thread owns at most
-// ceil(16/R) butterflies.
+// threads transforms at most FFT_TILE complex values, so a thread owns at most
+// ceil(FFT_TILE / (256 R)) butterflies.
 template <int R>
 __device__ __forceinline__ void fft_pass(double2 *data, int nseq, int sstride, int estride, int n,
                                          int Ns, unsigned long long nsm, const double2 *tw,
                                          double sign)
 {
-    constexpr int NB = (16 + R - 1) / R;
+    constexpr int NB = (FFT_TILE / BLK + R - 1) / R;
     const int nb = n / R;
     const int twstep = magic_div(nb, nsm);      // n / (Ns R)
     double2 v[NB][R];
@@ -166,7 +252,7 @@ __device__ __forceinline__ void fft_pass(double2 *data, int nseq, int sstride, i
                 }
                 v[i][q] = a;
             }
-            small_dft(v[i], R, sign);
+            dft<R>(v[i], sign);
         }
         w.next();
     }
@@ -190,10 +276,16 @@ __device__ __forceinline__ void lds_fft(double2 *data, int nseq, int sstride, in
     for (int pass = 0; pass < p.nf; ++pass) {
         const int r = p.f[pass];
         const unsigned long long m = p.nsm[pass];
-        if (r == 4) fft_pass<4>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign);
-        else if (r == 2) fft_pass<2>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign);
-        else if (r == 3) fft_pass<3>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign);
-        else fft_pass<5>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign);
+        switch (r) {
+        case 16: fft_pass<16>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign); break;
+        case 12: fft_pass<12>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign); break;
+        case 8: fft_pass<8>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign); break;
+        case 6: fft_pass<6>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign); break;
+        case 5: fft_pass<5>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign); break;
+        case 4: fft_pass<4>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign); break;
+        case 3: fft_pass<3>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign); break;
+        default: fft_pass<2>(data, nseq, sstride, estride, p.n, Ns, m, tw, sign);
+        }
         Ns *= r;
     }
 }
@@ -212,7 +304,7 @@ struct ZinvArgs {
     const int *abort_flag;
 };
 
-constexpr int MAXE = 4096 / BLK;   // LDS elements per thread of a 4096-value tile
+constexpr int MAXE = FFT_TILE / BLK;   // LDS elements per thread of a full tile
 
 // Stage a (Pz x XB) tile of one field into LDS in its memory layout
 // [zp][x], zero rows for the padding, the spectral curl formed on the fly.
@@ -595,10 +687,10 @@ int xmix_zblock(Context &c, int nf);
 
 static int pick_block(int Mx, int P, int nper)
 {
-    // columns per block so that the LDS tile plus the P twiddles stay <= 4096
-    // complex
+    // columns per block so that the LDS tile plus the P twiddles stay within
+    // FFT_TILE complex
     int xb = Mx;
-    while (xb > 1 && (size_t) xb * P * nper + P > 4096) xb = (xb + 1) / 2;
+    while (xb > 1 && (size_t) xb * P * nper + P > FFT_TILE) xb = (xb + 1) / 2;
     return xb;
 }
 

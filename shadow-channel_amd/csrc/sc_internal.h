@@ -23,7 +23,7 @@ namespace sc {
 typedef double2 cplx;
 
 constexpr int MAXF = 12;        // fields in one z-inverse launch
-constexpr int XMIX_LDS = 3584;  // complex values (data + twiddles) in one x-kernel block
+constexpr int FFT_TILE = 2048;  // complex values (data + twiddles) in one transform block
 // MFMA k-steps per staged chunk of the y-transforms (tuning constants)
 #ifndef SC_KC_REBUILD
 #define SC_KC_REBUILD 8

@@ -118,11 +118,30 @@ static int plan_fft(FftPlan &p, int n)
 {
     p.n = n;
     p.nf = 0;
+    // fewest passes over the radices the device implements; among those, the
+    // smallest per-thread register tile (ceil(FFT_TILE/(256 R)) butterflies of
+    // R points)
+    static const int RAD[] = {16, 12, 8, 6, 5, 4, 3, 2};
+    auto tile = [](int r) { return (FFT_TILE / 256 + r - 1) / r * r; };
+    std::vector<int> passes(n + 1, 1 << 20), worst(n + 1, 0), first(n + 1, 0);
+    if (n >= 1) passes[1] = 0;
+    for (int m = 2; m <= n; ++m) {
+        if (n % m) continue;
+        for (int r : RAD) {
+            if (m % r || passes[m / r] >= (1 << 20)) continue;
+            const int np = passes[m / r] + 1, nw = std::max(worst[m / r], tile(r));
+            if (np < passes[m] || (np == passes[m] && nw < worst[m])) {
+                passes[m] = np;
+                worst[m] = nw;
+                first[m] = r;
+            }
+        }
+    }
     int m = n;
-    while (m % 4 == 0 && p.nf < MAX_FACTORS) { p.f[p.nf++] = 4; m /= 4; }
-    while (m % 2 == 0 && p.nf < MAX_FACTORS) { p.f[p.nf++] = 2; m /= 2; }
-    while (m % 3 == 0 && p.nf < MAX_FACTORS) { p.f[p.nf++] = 3; m /= 3; }
-    while (m % 5 == 0 && p.nf < MAX_FACTORS) { p.f[p.nf++] = 5; m /= 5; }
+    while (m > 1 && first[m] && p.nf < MAX_FACTORS) {
+        p.f[p.nf++] = first[m];
+        m /= first[m];
+    }
     if (m != 1 || n > 4096) {
         last_error() = "transform length " + std::to_string(n) +
                        " must be 2^a 3^b 5^c and at most 4096";

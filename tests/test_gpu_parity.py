@@ -9,6 +9,8 @@ Tolerances (relative L2 over the coefficient array):
     3.1e-11 (tangent), 4.1e-11 (adjoint), 9.2e-11 (projection); see
     tests/test_oracle_rounding_floor.py.  The device path lands at 1.2-1.5x
     those floors.
+  * PROJ = 3e-10 for the time-derivative projection (ddt_project), whose own
+    FMA/plain floor is 9.2e-11 (a difference quotient of stored states).
   * 1e-8 after 100 primal steps on the 32x32x65 minimal channel (Lx = pi,
     Lz = 0.3 pi, Re = 2000, dt = 0.005).
 """
@@ -32,6 +34,7 @@ def cc():
 
 
 SHORT = 1e-10
+PROJ = 3e-10
 
 
 def rel(a, b):
@@ -146,8 +149,8 @@ def test_ddt_project_and_spec2phys(cc):
         vo, vg = v.copy(), v.copy()
         ro = o.ddt_project(i, vo)
         rg = cc.c_ddt_project(i, vg)
-        assert abs(rg - ro) <= SHORT * max(1.0, abs(ro))
-        assert rel(vg, vo) <= SHORT
+        assert abs(rg - ro) <= PROJ * max(1.0, abs(ro))
+        assert rel(vg, vo) <= PROJ
     flow = np.zeros((3, 3 * SMALL["Nx"] // 2, 3 * SMALL["Ny"] // 2, 3 * SMALL["Nz"] // 2))
     cc.c_spec2phys(C0, flow)
     fo = o.spec2phys(C0)

@@ -4,7 +4,9 @@ The oracle built twice from the same source, once with fused multiply-add
 contraction, gives two legitimate rounding orders of the same algorithm.
 Their difference after one step of the 32x32x65 minimal channel is the floor
 below which no reimplementation with a different summation or FFT order can
-agree with the reference; the GPU parity tolerances are set from it.
+agree with the reference; the GPU parity tolerances are set from it.  The
+time-derivative projection (ddt_project, a difference quotient of stored
+states) has a floor of its own, larger than that of a step.
 """
 import os
 
@@ -25,8 +27,45 @@ def _has_fma():
         return False
 
 
-@pytest.mark.skipif(not _has_fma() or not os.path.exists(FMA_LIB),
-                    reason="needs an FMA-capable host and oracle/liborc_fma.so")
+def _both_builds(fn):
+    out = []
+    for path in (pyorc.lib_path(), FMA_LIB):
+        pyorc._LIB = None
+        orig = pyorc.lib_path
+        pyorc.lib_path = lambda p=path: p
+        try:
+            out.append(fn())
+        finally:
+            pyorc.lib_path = orig
+            pyorc._LIB = None
+    return np.linalg.norm(out[0] - out[1]) / np.linalg.norm(out[0])
+
+
+needs_fma = pytest.mark.skipif(not _has_fma() or not os.path.exists(FMA_LIB),
+                               reason="needs an FMA-capable host and oracle/liborc_fma.so")
+
+
+@needs_fma
+def test_rounding_floor_of_projection():
+    c = dict(Nx=16, Ny=33, Nz=12, Lx=np.pi, Lz=0.3 * np.pi, Re=2000.0, flux=4.0 / 3.0, dt=0.005)
+
+    def run():
+        o = pyorc.Oracle(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
+                         c["dt"], 3)
+        o.primal(0, synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"],
+                                    seed=0, amp=0.1))
+        v = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=12,
+                            unit=True)
+        o.ddt_project(2, v)
+        o.destroy()
+        return v
+
+    floor = _both_builds(run)
+    # measured 9.2e-11; the GPU projection tolerance (3e-10) must stay above it
+    assert 1e-12 < floor < 1.5e-10
+
+
+@needs_fma
 def test_rounding_floor_of_one_step():
     c = dict(Nx=32, Ny=65, Nz=32, Lx=np.pi, Lz=0.3 * np.pi, Re=2000.0, flux=4.0 / 3.0, dt=0.005)
     C0 = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=0)
