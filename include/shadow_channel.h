@@ -109,6 +109,12 @@ int sc_prof_read(double *total_ms);
 /* Algorithmic bytes (compulsory reads + writes, each element once) of the
    launches recorded since sc_prof_enable. */
 double sc_prof_bytes(void);
+/* Re-issue the last recorded launch of the enabled class reps times back to
+   back; returns the average milliseconds per launch (events around the whole
+   batch, so launch latency overlaps the previous launch) and that launch's
+   algorithmic bytes.  The replayed kernels overwrite their outputs: use after
+   the results are read back. */
+int sc_prof_replay(int reps, double *avg_ms, double *bytes);
 /* Wait until all work queued by the library has finished. */
 int sc_synchronize(void);
 

@@ -66,6 +66,8 @@ _L.sc_prof_read.argtypes = [_c.POINTER(_d)]
 _L.sc_prof_read.restype = _i
 _L.sc_prof_bytes.argtypes = []
 _L.sc_prof_bytes.restype = _d
+_L.sc_prof_replay.argtypes = [_i, _c.POINTER(_d), _c.POINTER(_d)]
+_L.sc_prof_replay.restype = _i
 _L.sc_synchronize.argtypes = []
 _L.sc_synchronize.restype = _i
 
@@ -222,6 +224,14 @@ def sc_prof_read():
     ms = _d(0.0)
     n = _L.sc_prof_read(_c.byref(ms))
     return n, ms.value, nbytes
+
+
+def sc_prof_replay(reps):
+    """(average milliseconds, algorithmic bytes) of the last recorded launch,
+    re-issued reps times back to back (overwrites that kernel's outputs)."""
+    ms, nb = _d(0.0), _d(0.0)
+    _check(_L.sc_prof_replay(int(reps), _c.byref(ms), _c.byref(nb)), "sc_prof_replay")
+    return ms.value, nb.value
 
 
 def sc_synchronize():

@@ -40,6 +40,7 @@ void prof_begin(Context &c, hipStream_t s, int id, double bytes)
 {
     if (c.prof_class != id) return;
     c.prof_bytes += bytes;
+    c.last_bytes = bytes;
     if (c.ev_used + 2 > c.ev_pool.size()) {
         for (int i = 0; i < 256; ++i) {
             hipEvent_t e;
@@ -50,9 +51,14 @@ void prof_begin(Context &c, hipStream_t s, int id, double bytes)
     (void) hipEventRecord(c.ev_pool[c.ev_used], s);
 }
 
-void prof_end(Context &c, hipStream_t s, int id)
+void prof_end(Context &c, hipStream_t s, int id, const std::function<void()> &relaunch)
 {
-    if (c.prof_class != id || c.ev_used + 2 > c.ev_pool.size()) return;
+    if (c.prof_class != id) return;
+    if (relaunch) {
+        c.replay = relaunch;
+        c.replay_bytes = c.last_bytes;
+    }
+    if (c.ev_used + 2 > c.ev_pool.size()) return;
     (void) hipEventRecord(c.ev_pool[c.ev_used + 1], s);
     c.ev_used += 2;
 }
@@ -317,7 +323,30 @@ int sc_prof_enable(int kernel_class)
     c.prof_class = kernel_class;
     c.ev_used = 0;
     c.prof_bytes = 0.0;
+    c.replay = nullptr;
     return 0;
+}
+
+int sc_prof_replay(int reps, double *avg_ms, double *bytes)
+{
+    Context &c = ctx();
+    if (!c.stream) return SC_ENOTREADY;
+    if (!c.replay || reps < 1 || !avg_ms || !bytes) return SC_EINVAL;
+    *bytes = c.replay_bytes;
+    hipEvent_t e0, e1;
+    SC_HIP(hipEventCreate(&e0));
+    SC_HIP(hipEventCreate(&e1));
+    c.replay();   // warm the code and caches
+    SC_HIP(hipEventRecord(e0, c.stream));
+    for (int i = 0; i < reps; ++i) c.replay();
+    SC_HIP(hipEventRecord(e1, c.stream));
+    SC_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    SC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void) hipEventDestroy(e0);
+    (void) hipEventDestroy(e1);
+    *avg_ms = ms / reps;
+    return SC_OK;
 }
 
 double sc_prof_bytes(void) { return ctx().prof_bytes; }

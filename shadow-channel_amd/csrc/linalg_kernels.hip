@@ -410,20 +410,22 @@ void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst)
     RebuildArgs a{c.dFragR, c.dKx, c.dKz, d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, c.nksR, c.nyt,
                   nseg, 2. * M_PI / d.Lx, 2. * M_PI / d.Lz, c.abort_flag};
     double bytes = 16.0 * (double) (d.Nz - 1) * d.Mx * (2.0 * d.dimR + 5.0 * d.qpts);
-    prof_begin(c, c.stream, 4, bytes);
-    if (c.nksR == 8)
-        hipLaunchKernelGGL(k_rebuild_rows<8>, dim3(xcd_grid(c.nyt * nseg)), dim3(256), 0, c.stream,
-                           a, Csrc, Udst);
-    else if (c.nksR == 16)
-        hipLaunchKernelGGL(k_rebuild_rows<16>, dim3(xcd_grid(c.nyt * nseg)), dim3(256), 0, c.stream,
-                           a, Csrc, Udst);
-    else if (c.nksR == 24)
-        hipLaunchKernelGGL(k_rebuild_rows<24>, dim3(xcd_grid(c.nyt * nseg)), dim3(256), 0, c.stream,
-                           a, Csrc, Udst);
-    else
-        hipLaunchKernelGGL(k_rebuild, dim3(xcd_grid(c.nyt * d.Nz)), dim3(256), 0, c.stream, a, Csrc,
-                           Udst);
-    prof_end(c, c.stream, 4);
+    const hipStream_t st = c.stream;
+    const int nks = c.nksR;
+    const dim3 grows(xcd_grid(c.nyt * nseg)), gchunk(xcd_grid(c.nyt * d.Nz));
+    auto go = [=]() {
+        if (nks == 8)
+            hipLaunchKernelGGL(k_rebuild_rows<8>, grows, dim3(256), 0, st, a, Csrc, Udst);
+        else if (nks == 16)
+            hipLaunchKernelGGL(k_rebuild_rows<16>, grows, dim3(256), 0, st, a, Csrc, Udst);
+        else if (nks == 24)
+            hipLaunchKernelGGL(k_rebuild_rows<24>, grows, dim3(256), 0, st, a, Csrc, Udst);
+        else
+            hipLaunchKernelGGL(k_rebuild, gchunk, dim3(256), 0, st, a, Csrc, Udst);
+    };
+    prof_begin(c, st, 4, bytes);
+    go();
+    prof_end(c, st, 4, go);
 }
 
 // ------------------------------------------------------------ projection
@@ -578,12 +580,16 @@ void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb)
     dim3 grid(xcd_grid(c.nit * d.Nz));
     double bytes = 16.0 * (double) (d.Nz - 1) * d.Mx *
                    ((adj ? 5.0 : 3.0) * d.qpts + d.dimQ + d.dimR);
-    prof_begin(c, c.stream, 5, bytes);
-    if (adj)
-        hipLaunchKernelGGL(k_force<true>, grid, dim3(256), 0, c.stream, a, Usrc, Fa, Fb);
-    else
-        hipLaunchKernelGGL(k_force<false>, grid, dim3(256), 0, c.stream, a, Usrc, Fa, Fb);
-    prof_end(c, c.stream, 5);
+    const hipStream_t st = c.stream;
+    auto go = [=]() {
+        if (adj)
+            hipLaunchKernelGGL(k_force<true>, grid, dim3(256), 0, st, a, Usrc, Fa, Fb);
+        else
+            hipLaunchKernelGGL(k_force<false>, grid, dim3(256), 0, st, a, Usrc, Fa, Fb);
+    };
+    prof_begin(c, st, 5, bytes);
+    go();
+    prof_end(c, st, 5, go);
 }
 
 void launch_force(Context &c, const cplx *Usrc, int adj) { launch_force(c, Usrc, adj, c.Fa, c.Fb); }
@@ -982,20 +988,24 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
     double bytes = (double) (d.Nz - 1) * d.Mx *
                    (16.0 * (2.0 * d.dimR) * 3.0 + 16.0 * (d.dimQ + d.dimR) * (k != 2 ? 2.0 : 1.0) +
                     8.0 * (9.0 * d.dimQ + 5.0 * d.dimR));
-    prof_begin(c, c.stream, 6, bytes);
     const int sbt = solve_columns_per_block(d.dimR);
     const int ncol = d.Nz * d.Mx;
     const size_t lds = solve_lds_bytes(d.dimR, sbt);
-    dim3 grid((ncol + sbt - 1) / sbt + 1, 4);
-    if (sbt == 8)
-        hipLaunchKernelGGL(k_solve<8>, grid, dim3(256), lds, c.stream, a, Ccarry, Csol);
-    else if (sbt == 4)
-        hipLaunchKernelGGL(k_solve<4>, grid, dim3(256), lds, c.stream, a, Ccarry, Csol);
-    else if (sbt == 2)
-        hipLaunchKernelGGL(k_solve<2>, grid, dim3(256), lds, c.stream, a, Ccarry, Csol);
-    else
-        hipLaunchKernelGGL(k_solve<1>, grid, dim3(256), lds, c.stream, a, Ccarry, Csol);
-    prof_end(c, c.stream, 6);
+    const dim3 grid((ncol + sbt - 1) / sbt + 1, 4);
+    const hipStream_t st = c.stream;
+    auto go = [=]() {
+        if (sbt == 8)
+            hipLaunchKernelGGL(k_solve<8>, grid, dim3(256), lds, st, a, Ccarry, Csol);
+        else if (sbt == 4)
+            hipLaunchKernelGGL(k_solve<4>, grid, dim3(256), lds, st, a, Ccarry, Csol);
+        else if (sbt == 2)
+            hipLaunchKernelGGL(k_solve<2>, grid, dim3(256), lds, st, a, Ccarry, Csol);
+        else
+            hipLaunchKernelGGL(k_solve<1>, grid, dim3(256), lds, st, a, Ccarry, Csol);
+    };
+    prof_begin(c, st, 6, bytes);
+    go();
+    prof_end(c, st, 6, go);
 }
 
 // ------------------------------------------------------------ vectors

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -129,6 +130,8 @@ struct Context {
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     double prof_bytes = 0.0;        // algorithmic bytes of the recorded launches
+    std::function<void()> replay;   // the last recorded launch, re-issuable
+    double last_bytes = 0.0, replay_bytes = 0.0;
 };
 
 Context &ctx();
@@ -160,7 +163,10 @@ enum { ZF_USLOTS = 0, ZF_SCRATCH = 1 };
 
 // timing of one kernel class inside the timed region of a benchmark
 void prof_begin(Context &c, hipStream_t s, int kernel_id, double bytes = 0.0);
-void prof_end(Context &c, hipStream_t s, int kernel_id);
+// relaunch: the same launch again (kept for sc_prof_replay when kernel_id is
+// the recorded class)
+void prof_end(Context &c, hipStream_t s, int kernel_id,
+              const std::function<void()> &relaunch = std::function<void()>());
 
 std::string &last_error();
 }  // namespace sc
