@@ -117,6 +117,14 @@ double sc_prof_bytes(void);
 int sc_prof_replay(int reps, double *avg_ms, double *bytes);
 /* Wait until all work queued by the library has finished. */
 int sc_synchronize(void);
+/* The primal keeps, per stored stage state, its velocity and vorticity after
+   the z transform, so that the tangent and adjoint need not rebuild and
+   transform that state again (capacity traded for recomputation; the results
+   are bitwise the same).  c_init allocates it when it fits in half of the
+   free device memory and it is enabled (default 1); the setting applies to
+   the next c_init.  sc_field_cache_active reports whether it is allocated. */
+int sc_set_field_cache(int enable);
+int sc_field_cache_active(void);
 
 #ifdef __cplusplus
 }

@@ -70,6 +70,10 @@ _L.sc_prof_replay.argtypes = [_i, _c.POINTER(_d), _c.POINTER(_d)]
 _L.sc_prof_replay.restype = _i
 _L.sc_synchronize.argtypes = []
 _L.sc_synchronize.restype = _i
+_L.sc_set_field_cache.argtypes = [_i]
+_L.sc_set_field_cache.restype = _i
+_L.sc_field_cache_active.argtypes = []
+_L.sc_field_cache_active.restype = _i
 
 SC_EINVAL = -3
 SC_ENOTREADY = -2
@@ -232,6 +236,16 @@ def sc_prof_replay(reps):
     ms, nb = _d(0.0), _d(0.0)
     _check(_L.sc_prof_replay(int(reps), _c.byref(ms), _c.byref(nb)), "sc_prof_replay")
     return ms.value, nb.value
+
+
+def sc_set_field_cache(enable):
+    """Enable (default) or disable the transformed-field cache of the next
+    c_init (results are the same either way)."""
+    _L.sc_set_field_cache(int(bool(enable)))
+
+
+def sc_field_cache_active():
+    return bool(_L.sc_field_cache_active())
 
 
 def sc_synchronize():

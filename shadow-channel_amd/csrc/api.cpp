@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <fstream>
 #include <string>
 
@@ -63,6 +64,9 @@ void prof_end(Context &c, hipStream_t s, int id, const std::function<void()> &re
     c.ev_used += 2;
 }
 
+// whether c_init allocates the transformed-field cache (sc_set_field_cache)
+static int g_field_cache = 1;
+
 // ---------------------------------------------------------------- rand()
 // glibc random() TYPE_3: r_i = r_{i-3} + r_{i-31} (mod 2^32) after the
 // srand() seeding r_0 = seed, r_i = 16807 r_{i-1} mod (2^31 - 1) (i < 31),
@@ -105,7 +109,7 @@ static void free_all(Context &c)
                     c.dRowsum, c.C, c.IC, c.AC, c.U, c.IU, c.AU, c.LU, c.Ut, c.Ut2, c.MC,
                     c.MIC, c.Csol, c.Ctmp, c.Ctmp2, c.Fa, c.Fb, c.FLa, c.FLb, c.W1, c.W2,
                     c.SP, c.flowdev, c.cflmax, c.dotpart, c.scal, c.abort_flag, c.pz.tw,
-                    c.px.tw, c.dFragR, c.dFragF};
+                    c.px.tw, c.dFragR, c.dFragF, c.wcache};
     for (void *p : ptrs)
         if (p) (void) hipFree(p);
     for (int a = 0; a < 2; ++a)
@@ -149,45 +153,81 @@ static FieldSpec velocity_and_vorticity(const cplx *Ub)
     return fs;
 }
 
-// H = u x omega into the H slots of Ub (channel/pass1.c, state side)
-static void pass1_state(Context &c, cplx *Ub, int check_cfl)
+// cached transformed fields of stored state j, or null when not available
+static cplx *wslot(Context &c, long j)
 {
+    if (!c.wcache || j < 0 || j >= (long) c.wvalid.size()) return nullptr;
+    return c.wcache + (size_t) j * c.wslot;
+}
+
+static const cplx *wcached(Context &c, long j)
+{
+    cplx *p = wslot(c, j);
+    return p && c.wvalid[j] ? p : nullptr;
+}
+
+// H = u x omega into the H slots of Ub (channel/pass1.c, state side).  With
+// wdst the transformed fields are kept there (the cache slot of this state).
+static void pass1_state(Context &c, cplx *Ub, int check_cfl, cplx *wdst = nullptr)
+{
+    cplx *w = wdst ? wdst : c.W1;
     if (check_cfl) (void) hipMemsetAsync(c.cflmax, 0, sizeof(double) * c.d.qpts, c.stream);
-    launch_zinv(c, velocity_and_vorticity(Ub), c.W1);
-    launch_xmix(c, XM_PRIMAL, c.W1, c.W2, check_cfl, nullptr, nullptr);
+    launch_zinv(c, velocity_and_vorticity(Ub), w);
+    launch_xmix(c, XM_PRIMAL, w, c.W2, check_cfl, nullptr, nullptr);
     if (check_cfl) launch_cfl_check(c);
     launch_zfwd(c, c.W2, 3, Ub, ZF_USLOTS);
 }
 
-// IH = u x I omega + I u x omega into the H slots of IUb (channel/pass1.c)
-static void pass1_tangent(Context &c, const cplx *Ub, cplx *IUb, int check_cfl)
+// IH = u x I omega + I u x omega into the H slots of IUb (channel/pass1.c).
+// Base fields from wbase (cached) when given, else transformed from Ub.
+static void pass1_tangent(Context &c, const cplx *Ub, cplx *IUb, int check_cfl,
+                          const cplx *wbase = nullptr)
 {
-    FieldSpec a = velocity_and_vorticity(Ub), b = velocity_and_vorticity(IUb);
-    for (int f = 0; f < 6; ++f) {
-        a.src[6 + f] = b.src[f];
-        a.kind[6 + f] = b.kind[f];
-        a.slot[6 + f] = b.slot[f];
-    }
-    a.nf = 12;
     if (check_cfl) (void) hipMemsetAsync(c.cflmax, 0, sizeof(double) * c.d.qpts, c.stream);
-    launch_zinv(c, a, c.W1);
-    launch_xmix(c, XM_TANGENT, c.W1, c.W2, check_cfl, nullptr, nullptr);
+    if (wbase) {
+        launch_zinv(c, velocity_and_vorticity(IUb), c.W1);
+        launch_xmix(c, XM_TANGENT, wbase, c.W2, check_cfl, nullptr, nullptr, c.W1, 6);
+    } else {
+        FieldSpec a = velocity_and_vorticity(Ub), b = velocity_and_vorticity(IUb);
+        for (int f = 0; f < 6; ++f) {
+            a.src[6 + f] = b.src[f];
+            a.kind[6 + f] = b.kind[f];
+            a.slot[6 + f] = b.slot[f];
+        }
+        a.nf = 12;
+        launch_zinv(c, a, c.W1);
+        launch_xmix(c, XM_TANGENT, c.W1, c.W2, check_cfl, nullptr, nullptr);
+    }
     if (check_cfl) launch_cfl_check(c);
     launch_zfwd(c, c.W2, 3, IUb, ZF_USLOTS);
 }
 
-// adjoint convective operator (channel/pass2.c, adjoint side) into Udst
-static void pass2_adjoint(Context &c, const cplx *Ub, const cplx *AUb, cplx *Udst)
+// adjoint convective operator (channel/pass2.c, adjoint side) into Udst.
+// State fields from wbase (cached) when given, else transformed from Ub.
+static void pass2_adjoint(Context &c, const cplx *Ub, const cplx *AUb, cplx *Udst,
+                          const cplx *wbase = nullptr)
 {
-    FieldSpec a = velocity_and_vorticity(Ub);
+    FieldSpec lam;
+    lam.nf = 3;
     for (int s = 0; s < 3; ++s) {
-        a.src[6 + s] = AUb;
-        a.kind[6 + s] = FK_SLOT;
-        a.slot[6 + s] = s;
+        lam.src[s] = AUb;
+        lam.kind[s] = FK_SLOT;
+        lam.slot[s] = s;
     }
-    a.nf = 9;
-    launch_zinv(c, a, c.W1);
-    launch_xmix(c, XM_ADJOINT, c.W1, c.W2, 0, nullptr, nullptr);
+    if (wbase) {
+        launch_zinv(c, lam, c.W1);
+        launch_xmix(c, XM_ADJOINT, wbase, c.W2, 0, nullptr, nullptr, c.W1, 6);
+    } else {
+        FieldSpec a = velocity_and_vorticity(Ub);
+        for (int s = 0; s < 3; ++s) {
+            a.src[6 + s] = lam.src[s];
+            a.kind[6 + s] = lam.kind[s];
+            a.slot[6 + s] = lam.slot[s];
+        }
+        a.nf = 9;
+        launch_zinv(c, a, c.W1);
+        launch_xmix(c, XM_ADJOINT, c.W1, c.W2, 0, nullptr, nullptr);
+    }
     launch_zfwd(c, c.W2, 6, c.SP, ZF_SCRATCH);
     launch_assemble_adj(c, c.SP, Udst);
 }
@@ -316,6 +356,14 @@ extern "C" {
 const char *sc_last_error(void) { return last_error().c_str(); }
 int sc_last_status(void) { return ctx().status; }
 void sc_srand(unsigned seed) { glibc_srand(seed); }
+
+int sc_set_field_cache(int enable)
+{
+    g_field_cache = enable ? 1 : 0;
+    return SC_OK;
+}
+
+int sc_field_cache_active(void) { return ctx().wcache != nullptr; }
 
 int sc_prof_enable(int kernel_class)
 {
@@ -448,6 +496,22 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
             return SC_EINIT;
         }
     }
+    // transformed-field cache of the stored trajectory, if it fits in half of
+    // the free memory (otherwise the tangent and adjoint recompute)
+    {
+        const size_t per = (size_t) d.qpts * 6 * d.Pz * d.Mx;
+        const size_t need = sizeof(cplx) * per * 3 * (size_t) nstep;
+        size_t freeb = 0, totalb = 0;
+        if (g_field_cache && need > 0 && hipMemGetInfo(&freeb, &totalb) == hipSuccess &&
+            need <= freeb / 2 &&
+            hipMalloc(&c.wcache, need) == hipSuccess) {
+            c.wslot = per;
+            c.wvalid.assign(3 * (size_t) nstep, 0);
+        } else {
+            c.wcache = nullptr;
+            c.wvalid.clear();
+        }
+    }
     c.views.assign(nstep + 1, {});
     c.iviews.assign(nstep + 1, {});
     c.ok = true;
@@ -487,10 +551,17 @@ int c_primal(int ru_steps, sc_complex *C_given, int Nz_dup, int Nvar_dup, int Ny
     SC_HIP(hipMemcpyAsync(c.MC, c.C, cb, hipMemcpyDeviceToDevice, c.stream));
     SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
     c.status = 0;
+    std::fill(c.wvalid.begin(), c.wvalid.end(), 0);
     const int total = d.nsteps + ru_steps;
     for (int n = 0; n < total; ++n) {
         for (int k = 0; k < 3; ++k) {
-            pass1_state(c, c.U, k == 0);
+            // the stage works on stored state j = 3(n - ru) + k; its fields
+            // equal those of rebuild(MC[j]) except for state 0 after a
+            // laminar start or a run-up (MC[0] is the state before either)
+            const long j = (long) (n - ru_steps) * 3 + k;
+            cplx *w = (j > 0 || (j == 0 && given && ru_steps == 0)) ? wslot(c, j) : nullptr;
+            pass1_state(c, c.U, k == 0, w);
+            if (w) c.wvalid[j] = 1;
             long count = (long) (n - ru_steps) * 3 + k + 1;
             cplx *Cs = count >= 0 ? mc(c, count) : c.Csol;
             launch_force(c, c.U, 0);
@@ -505,7 +576,8 @@ int c_primal(int ru_steps, sc_complex *C_given, int Nz_dup, int Nvar_dup, int Ny
     }
     int rc = stream_sync(c);
     if (rc) return rc;
-    check_abort(c, "Pass1");
+    // after an abort the queued launches did nothing: no cached field is valid
+    if (check_abort(c, "Pass1")) std::fill(c.wvalid.begin(), c.wvalid.end(), 0);
     if (given) SC_HIP(hipMemcpy(C_given, c.C, cb, hipMemcpyDeviceToHost));
     return SC_OK;
 }
@@ -530,8 +602,9 @@ int c_tangent(int start, int end, sc_complex *IC_given, int Nz_dup, int Nvar_dup
     const double target = inhomo ? 2.0 : 0.0;
     for (int n = start; n < end; ++n) {
         for (int k = 0; k < 3; ++k) {
-            launch_rebuild(c, mc(c, 3L * n + k), c.U);
-            pass1_tangent(c, c.U, c.IU, k == 0);
+            const cplx *w = wcached(c, 3L * n + k);
+            if (!w) launch_rebuild(c, mc(c, 3L * n + k), c.U);
+            pass1_tangent(c, c.U, c.IU, k == 0, w);
             long count = 3L * n + k + 1;
             cplx *Cs = mic(c, count);
             launch_force(c, c.IU, 0);
@@ -575,11 +648,13 @@ int c_adjoint(int start, int end, sc_complex *AC_given, int Nz_dup, int Nvar_dup
         for (int k = 0; k < 3; ++k) {
             long count = 3L * n - k - 1;
             if (k < 2) {
-                launch_rebuild(c, mc(c, count - 1), c.U);
-                pass2_adjoint(c, c.U, c.AU, c.LU);
+                const cplx *w = wcached(c, count - 1);
+                if (!w) launch_rebuild(c, mc(c, count - 1), c.U);
+                pass2_adjoint(c, c.U, c.AU, c.LU, w);
             }
-            launch_rebuild(c, mc(c, count), c.U);
-            pass2_adjoint(c, c.U, c.AU, c.U);
+            const cplx *w = wcached(c, count);
+            if (!w) launch_rebuild(c, mc(c, count), c.U);
+            pass2_adjoint(c, c.U, c.AU, c.U, w);
             launch_force(c, c.U, 1);
             if (k < 2) launch_force(c, c.LU, 1, c.FLa, c.FLb);
             launch_solve(c, c.AC, c.Csol, k, 1, c.FLa, c.FLb, 0.0);

@@ -396,6 +396,8 @@ __global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, cplx *W1)
 
 struct XmixArgs {
     int mode, nf, nout, Mx, Px, Pz, qpts, ZB, check_cfl;
+    int nf1;                   // fields 0..nf1-1 from the first source, the rest from W1b
+    const cplx *W1b;
     double cfl1, cfl3;
     const double *cfl2, *W;
     double *cflmax;        // [qpts], as ordered bit patterns
@@ -448,7 +450,9 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
         wl.next();
         const int s = t % nseq, r = t / nseq;
         const int k = min(n < Mx ? n : Px - n, Mx - 1);
-        const cplx *X = W1 + (((size_t) y * nf + 2 * s) * Pz + z0 + r) * Mx;
+        const int f0 = 2 * s, nf1 = a.nf1;
+        const cplx *X = f0 < nf1 ? W1 + (((size_t) y * nf1 + f0) * Pz + z0 + r) * Mx
+                                 : a.W1b + (((size_t) y * (nf - nf1) + f0 - nf1) * Pz + z0 + r) * Mx;
         const int sy = min(2 * s + 1, nf - 1) - 2 * s;     // 1, or 0 for a lone field
         lx[i] = X[k];
         ly[i] = X[(size_t) sy * Pz * Mx + k];
@@ -730,7 +734,7 @@ void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1)
 }
 
 void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
-                 double *flow, double *dot_out)
+                 double *flow, double *dot_out, const cplx *W1b, int nf1)
 {
     const Dims &d = c.d;
     XmixArgs a;
@@ -738,6 +742,8 @@ void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
     a.nf = mode == XM_PRIMAL ? 6 : mode == XM_TANGENT ? 12 : mode == XM_ADJOINT ? 9
          : mode == XM_PHYS ? 3 : 6;
     a.nout = mode == XM_ADJOINT ? 6 : (mode == XM_PRIMAL || mode == XM_TANGENT) ? 3 : 0;
+    a.nf1 = W1b ? nf1 : a.nf;
+    a.W1b = W1b ? W1b : W1;
     a.Mx = d.Mx;
     a.Px = d.Px;
     a.Pz = d.Pz;

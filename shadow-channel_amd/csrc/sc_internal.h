@@ -113,6 +113,14 @@ struct Context {
     cplx *Fa = nullptr, *Fb = nullptr;          // [Nz][dimQ][Mx], [Nz][dimR][Mx]
     cplx *FLa = nullptr, *FLb = nullptr;        // adjoint next-stage projection
     cplx *W1 = nullptr, *W2 = nullptr, *SP = nullptr;
+    // Per stage state j = 3n + k of the stored trajectory: the six
+    // velocity/vorticity fields after the z inverse transform, as written by
+    // the primal's first nonlinear pass ([qpts][6][Pz][Mx] each).  The
+    // tangent and adjoint read them instead of rebuilding and transforming
+    // MC[j] again.  Allocated when it fits in half of the free HBM.
+    cplx *wcache = nullptr;
+    size_t wslot = 0;                           // complex values per state
+    std::vector<char> wvalid;                   // entry j holds MC[j]'s fields
     double *flowdev = nullptr;                  // [3][Px][qpts][Pz]
     double *cflmax = nullptr;                   // [qpts]
     double *dotpart = nullptr;                  // partial sums
@@ -142,8 +150,10 @@ void band_factor(double *A, int N, int dl);
 
 // ---- launches ----
 void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1);
+// Fields 0..nf1-1 come from W1 ([qpts][nf1][Pz][Mx]), the rest from W1b
+// ([qpts][nf-nf1][Pz][Mx]); W1b null: all fields from W1.
 void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
-                 double *flow, double *dot_out);
+                 double *flow, double *dot_out, const cplx *W1b = nullptr, int nf1 = 0);
 void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind);
 void launch_assemble_adj(Context &c, const cplx *SP, cplx *Udst);
 void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst);

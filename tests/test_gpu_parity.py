@@ -157,6 +157,36 @@ def test_ddt_project_and_spec2phys(cc):
     assert rel(flow, fo) <= SHORT
 
 
+def test_field_cache_is_bitwise(cc):
+    """The tangent and adjoint read the primal's transformed velocity and
+    vorticity from the per-state cache instead of rebuilding and transforming
+    MC[j]: both paths must give identical bits, including state 0 after a
+    run-up (where the cache entry is not used)."""
+    print("case: transformed-field cache on/off", flush=True)
+    c, n = SMALL, 3
+    out = {}
+    try:
+        for enable in (True, False):
+            cc.sc_set_field_cache(enable)
+            for ru in (0, 2):
+                assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
+                                 c["dt"], n) == 0
+                assert cc.sc_field_cache_active() == enable
+                C = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"],
+                                    seed=0, amp=0.1)
+                cc.c_primal(ru, C)
+                I = direction(c, 5)
+                cc.c_tangent(0, n, I, 0)
+                A = direction(c, 6)
+                cc.c_adjoint(n, 0, A, 0, 1.0)
+                out[(enable, ru)] = (C, I, A)
+    finally:
+        cc.sc_set_field_cache(True)
+    for ru in (0, 2):
+        for a, b in zip(out[(True, ru)], out[(False, ru)]):
+            np.testing.assert_array_equal(a, b)
+
+
 def test_duality_on_device(cc):
     print("case: tangent/adjoint duality", flush=True)
     n = 5
