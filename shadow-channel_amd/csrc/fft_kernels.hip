@@ -21,7 +21,7 @@
 
 namespace sc {
 
-constexpr int BLK = 256;
+constexpr int BLK = FFT_BLOCK;   // threads per block of the transform kernels
 // complex values staged in LDS by one x-kernel block, data and twiddles
 constexpr int XMIX_TILE = FFT_TILE;
 
@@ -220,9 +220,9 @@ template <> __device__ __forceinline__ void dft<16>(double2 *v, double s) { dft_
 // over the sequences fastest, so with estride = nseq (sequences interleaved)
 // every access of a pass is a contiguous run, and with sstride odd and
 // estride = 1 neighbouring lanes fall in different banks.  Each thread keeps
-// its butterflies' inputs in registers across the barrier; a block of 256
+// its butterflies' inputs in registers across the barrier; a block of BLK
 // threads transforms at most FFT_TILE complex values, so a thread owns at most
-// ceil(FFT_TILE / (256 R)) butterflies.
+// ceil(FFT_TILE / (BLK R)) butterflies.
 template <int R>
 __device__ __forceinline__ void fft_pass(double2 *data, int nseq, int sstride, int estride, int n,
                                          int Ns, unsigned long long nsm, const double2 *tw,
@@ -296,6 +296,108 @@ __device__ __forceinline__ void stage_twiddles(const PlanDev &p, double2 *twl)
     for (int m = threadIdx.x; m < p.n; m += BLK) twl[m] = p.tw[m];
 }
 
+// ---------------------------------------------------------------- fixed lengths
+// Transform lengths with a compile-time plan (k_zinv<PZ>, k_xmix<MODE, PX>,
+// k_zfwd<PZ> with a nonzero length): radices, strides, the tile shape and all
+// index divisions fold into constants.  The same rule as the host planner:
+// fewest passes over the radices below, then the smallest per-thread tile.
+struct PBest {
+    int passes, worst, first;
+};
+
+constexpr int FIX_RADICES[] = {16, 12, 8, 6, 5, 4, 3, 2};
+
+constexpr int fix_tile(int r) { return (FFT_TILE / BLK + r - 1) / r * r; }
+
+constexpr PBest plan_best(int m)
+{
+    if (m == 1) return PBest{0, 0, 0};
+    PBest b{1 << 20, 0, 0};
+    for (int r : FIX_RADICES) {
+        if (m % r) continue;
+        const PBest sub = plan_best(m / r);
+        if (sub.passes >= (1 << 20)) continue;
+        const int np = sub.passes + 1, nw = sub.worst > fix_tile(r) ? sub.worst : fix_tile(r);
+        if (np < b.passes || (np == b.passes && nw < b.worst)) b = PBest{np, nw, r};
+    }
+    return b;
+}
+
+// radix of pass p of the length-n plan
+constexpr int plan_radix(int n, int p)
+{
+    int m = n;
+    for (int i = 0; i < p; ++i) m /= plan_best(m).first;
+    return plan_best(m).first;
+}
+
+// one pass over NSEQ sequences, element e of sequence s at s*SSTR + e*ESTR
+template <int R, int N, int NS, int NSEQ, int SSTR, int ESTR>
+__device__ __forceinline__ void fft_pass_c(double2 *data, const double2 *tw, double sign)
+{
+    constexpr int NBS = N / R;                       // butterflies per sequence
+    constexpr int TOTAL = NSEQ * NBS;
+    constexpr int NBT = (TOTAL + BLK - 1) / BLK;     // butterflies per thread
+    constexpr int TWS = N / (NS * R);
+    double2 v[NBT][R];
+    int off[NBT];
+#pragma unroll
+    for (int i = 0; i < NBT; ++i) {
+        const int b = threadIdx.x + i * BLK;
+        const int j = b / NSEQ, s = b - j * NSEQ;
+        off[i] = -1;
+        if (b < TOTAL) {
+            const double2 *src = data + s * SSTR;
+            const int jq = j / NS, k = j - jq * NS;
+            off[i] = s * SSTR + (jq * NS * R + k) * ESTR;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                double2 a = src[(j + q * NBS) * ESTR];
+                if (q > 0 && NS > 1) {
+                    double2 t = tw[q * k * TWS];
+                    t.y *= sign;
+                    a = cmul(a, t);
+                }
+                v[i][q] = a;
+            }
+            dft<R>(v[i], sign);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NBT; ++i) {
+        if (off[i] >= 0) {
+            double2 *dst = data + off[i];
+#pragma unroll
+            for (int q = 0; q < R; ++q) dst[q * NS * ESTR] = v[i][q];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int NSEQ, int SSTR, int ESTR, int P = 0, int NS = 1>
+__device__ __forceinline__ void lds_fft_c(double2 *data, const double2 *tw, double sign)
+{
+    if constexpr (P < plan_best(N).passes) {
+        constexpr int R = plan_radix(N, P);
+        fft_pass_c<R, N, NS, NSEQ, SSTR, ESTR>(data, tw, sign);
+        lds_fft_c<N, NSEQ, SSTR, ESTR, P + 1, NS * R>(data, tw, sign);
+    }
+}
+
+// columns per block of the fixed-length z kernels (the launch needs Mx % ZXB == 0)
+constexpr int ZXB = 16;
+static_assert(ZXB * 96 + 96 <= FFT_TILE, "z tile of the length-96 kernels");
+
+// rows per block of the fixed-length x kernel: the largest power of 2 with
+// the tile (rows * pairs * (PX + 1) + PX twiddles) within FFT_TILE
+constexpr int xmix_rows(int px, int nseq)
+{
+    int zb = 1;
+    while (2 * zb * nseq * (px + 1) + px <= FFT_TILE) zb *= 2;
+    return zb;
+}
+
 struct ZinvArgs {
     FieldSpec fs;
     const double *Kx, *Kz;
@@ -305,92 +407,107 @@ struct ZinvArgs {
 };
 
 constexpr int MAXE = FFT_TILE / BLK;   // LDS elements per thread of a full tile
+constexpr int SBAT = 8;                // stage-in loads in flight per thread and batch
+static_assert(MAXE % SBAT == 0, "stage-in batches must tile the per-thread elements");
 
 // Stage a (Pz x XB) tile of one field into LDS in its memory layout
 // [zp][x], zero rows for the padding, the spectral curl formed on the fly.
-// All loads of a thread are issued before the first LDS store: indices are
+// A batch's loads are all issued before its first LDS store: indices are
 // clamped into the array and the padding masked arithmetically, so no load
 // sits behind a branch.
-template <int KIND>
+template <int KIND, int PZ>
 __device__ __forceinline__ void zinv_stage(const ZinvArgs &a, const cplx *src, int slot, int y,
                                            int x0, int XB, double2 *lds)
 {
-    const int Nz = a.Nz, Pz = a.Pz, Mx = a.Mx, total = XB * Pz;
+    // PZ > 0: the tile is ZXB x PZ and every thread stages NE values in one batch
+    constexpr int NE = PZ > 0 ? (ZXB * PZ + BLK - 1) / BLK : MAXE;
+    constexpr int SB = PZ > 0 ? NE : SBAT;
+    const int Pz = PZ > 0 ? PZ : a.Pz, Nz = PZ > 0 ? 2 * PZ / 3 : a.Nz, Mx = a.Mx;
+    const int total = XB * Pz;
     const size_t zs = (size_t) 5 * a.qpts * Mx;        // stride of kz in U
     const size_t ss = (size_t) a.qpts * Mx;            // stride of slot in U
-    double2 p[MAXE], q[MAXE];
-    bool ok[MAXE];
-    int kzi[MAXE], xi[MAXE];
     Walk wk(XB);                                       // (zp, xl)
+#pragma unroll 1
+    for (int i0 = 0; i0 < NE; i0 += SB) {
+        if ((int) threadIdx.x + i0 * BLK >= total) break;
+        double2 p[SB], q[SB];
+        bool ok[SB];
+        int kzi[SB], xi[SB];
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i) {
-        const int e = threadIdx.x + i * BLK;
-        const int zp = min(wk.q, Pz - 1), xl = wk.r;
-        wk.next();
-        const int kz = zp < Nz / 2 ? zp : (zp > Nz ? zp - Nz / 2 : -1);
-        ok[i] = e < total && kz >= 0;
-        kzi[i] = max(kz, 0);
-        xi[i] = x0 + xl;
-        const cplx *b = src + kzi[i] * zs + (size_t) y * Mx + xi[i];
-        if (KIND == FK_SLOT) {
-            p[i] = b[slot * ss];
-        } else if (KIND == FK_OMEGA_Z) {
-            p[i] = b[3 * ss];
-            q[i] = b[1 * ss];
-        } else if (KIND == FK_OMEGA_X) {
-            p[i] = b[4 * ss];
-            q[i] = b[1 * ss];
-        } else {
-            p[i] = b[0];
-            q[i] = b[2 * ss];
+        for (int i = 0; i < SB; ++i) {
+            const int e = threadIdx.x + (i0 + i) * BLK;
+            const int zp = min(wk.q, Pz - 1), xl = wk.r;
+            wk.next();
+            const int kz = zp < Nz / 2 ? zp : (zp > Nz ? zp - Nz / 2 : -1);
+            ok[i] = e < total && kz >= 0;
+            kzi[i] = max(kz, 0);
+            xi[i] = x0 + xl;
+            const cplx *b = src + kzi[i] * zs + (size_t) y * Mx + xi[i];
+            if (KIND == FK_SLOT) {
+                p[i] = b[slot * ss];
+            } else if (KIND == FK_OMEGA_Z) {
+                p[i] = b[3 * ss];
+                q[i] = b[1 * ss];
+            } else if (KIND == FK_OMEGA_X) {
+                p[i] = b[4 * ss];
+                q[i] = b[1 * ss];
+            } else {
+                p[i] = b[0];
+                q[i] = b[2 * ss];
+            }
         }
-    }
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i) {
-        const int e = threadIdx.x + i * BLK;
-        if (e >= total) break;
-        const double kx = xi[i] * a.ax;
-        const int kz = kzi[i];
-        const double kzv = (kz <= Nz / 2 ? kz : kz - Nz) * a.az;
-        double2 val;
-        if (KIND == FK_SLOT) {
-            val = p[i];
-        } else if (KIND == FK_OMEGA_Z) {            // -du/dy + i kx v
-            const double2 du = p[i], v = q[i];
-            val = make_double2(-(du.x + kx * v.y), -du.y + kx * v.x);
-        } else if (KIND == FK_OMEGA_X) {            // dw/dy - i kz v
-            const double2 dw = p[i], v = q[i];
-            val = make_double2(dw.x + kzv * v.y, dw.y - kzv * v.x);
-        } else {                                    // i kz u - i kx w
-            const double2 u = p[i], w = q[i];
-            val = make_double2(kx * w.y - kzv * u.y, kzv * u.x - kx * w.x);
+        for (int i = 0; i < SB; ++i) {
+            const int e = threadIdx.x + (i0 + i) * BLK;
+            if (e >= total) break;
+            const double kx = xi[i] * a.ax;
+            const int kz = kzi[i];
+            const double kzv = (kz <= Nz / 2 ? kz : kz - Nz) * a.az;
+            double2 val;
+            if (KIND == FK_SLOT) {
+                val = p[i];
+            } else if (KIND == FK_OMEGA_Z) {            // -du/dy + i kx v
+                const double2 du = p[i], v = q[i];
+                val = make_double2(-(du.x + kx * v.y), -du.y + kx * v.x);
+            } else if (KIND == FK_OMEGA_X) {            // dw/dy - i kz v
+                const double2 dw = p[i], v = q[i];
+                val = make_double2(dw.x + kzv * v.y, dw.y - kzv * v.x);
+            } else {                                    // i kz u - i kx w
+                const double2 u = p[i], w = q[i];
+                val = make_double2(kx * w.y - kzv * u.y, kzv * u.x - kx * w.x);
+            }
+            lds[e] = ok[i] ? val : make_double2(0.0, 0.0);
         }
-        lds[e] = ok[i] ? val : make_double2(0.0, 0.0);
     }
 }
 
-// grid (ceil(Mx/XB), nf, qpts)
+// grid (ceil(Mx/XB), nf, qpts); PZ > 0: Pz == PZ, XB == ZXB and Mx % ZXB == 0
+template <int PZ>
 __global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, cplx *W1)
 {
     extern __shared__ double2 lds[];
     if (*a.abort_flag) return;
-    const int x0 = blockIdx.x * a.XB, f = blockIdx.y, y = blockIdx.z;
-    const int XB = min(a.XB, a.Mx - x0);
-    const int Pz = a.Pz, Mx = a.Mx;
+    const int XB0 = PZ > 0 ? ZXB : a.XB;
+    const int x0 = blockIdx.x * XB0, f = blockIdx.y, y = blockIdx.z;
+    const int XB = PZ > 0 ? ZXB : min(a.XB, a.Mx - x0);
+    const int Pz = PZ > 0 ? PZ : a.Pz, Mx = a.Mx;
     const cplx *src = a.fs.src[f];
     const int kind = a.fs.kind[f], slot = a.fs.slot[f];
-    double2 *twl = lds + a.XB * Pz;
+    double2 *twl = lds + XB0 * Pz;
     stage_twiddles(pz, twl);
-    if (kind == FK_SLOT) zinv_stage<FK_SLOT>(a, src, slot, y, x0, XB, lds);
-    else if (kind == FK_OMEGA_Z) zinv_stage<FK_OMEGA_Z>(a, src, slot, y, x0, XB, lds);
-    else if (kind == FK_OMEGA_X) zinv_stage<FK_OMEGA_X>(a, src, slot, y, x0, XB, lds);
-    else zinv_stage<FK_OMEGA_Y>(a, src, slot, y, x0, XB, lds);
+    if (kind == FK_SLOT) zinv_stage<FK_SLOT, PZ>(a, src, slot, y, x0, XB, lds);
+    else if (kind == FK_OMEGA_Z) zinv_stage<FK_OMEGA_Z, PZ>(a, src, slot, y, x0, XB, lds);
+    else if (kind == FK_OMEGA_X) zinv_stage<FK_OMEGA_X, PZ>(a, src, slot, y, x0, XB, lds);
+    else zinv_stage<FK_OMEGA_Y, PZ>(a, src, slot, y, x0, XB, lds);
     __syncthreads();
     // columns interleaved in LDS exactly as in memory: no transpose
-    lds_fft(lds, XB, 1, XB, pz, twl, +1.0);
+    if constexpr (PZ > 0)
+        lds_fft_c<PZ, ZXB, 1, ZXB>(lds, twl, +1.0);
+    else
+        lds_fft(lds, XB, 1, XB, pz, twl, +1.0);
     cplx *dst = W1 + (((size_t) y * gridDim.y + f) * Pz) * Mx;
     Walk wk(XB);
-    for (int e = threadIdx.x; e < XB * Pz; e += blockDim.x, wk.next())
+    for (int e = threadIdx.x; e < XB * Pz; e += BLK, wk.next())
         dst[(size_t) wk.q * Mx + x0 + wk.r] = lds[e];
 }
 
@@ -420,8 +537,8 @@ template <> struct XmixShape<XM_DOT> { static constexpr int NF = 6, NOUT = 0; };
 
 // grid (Pz/ZB, qpts); LDS: ZB * nseq sequences of Px complex at a stride of
 // Px + 1 (odd, so lanes on neighbouring sequences use different banks),
-// nseq = ceil(nf/2)
-template <int MODE>
+// nseq = ceil(nf/2).  PX > 0: Px == PX and ZB == xmix_rows(PX, nseq).
+template <int MODE, int PX>
 __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx *W1, cplx *W2)
 {
     extern __shared__ double2 lds[];
@@ -430,8 +547,11 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
     constexpr int nf = XmixShape<MODE>::NF;
     constexpr int NOUT = XmixShape<MODE>::NOUT;
     constexpr int nseq = (nf + 1) / 2;
+    constexpr int nso = (NOUT + 1) / 2;
+    constexpr int ZBC = PX > 0 ? xmix_rows(PX, nseq) : 1;
     const int zb = blockIdx.x, y = blockIdx.y;
-    const int Px = a.Px, Mx = a.Mx, Pz = a.Pz, ZB = a.ZB;
+    const int Px = PX > 0 ? PX : a.Px, Mx = PX > 0 ? PX / 3 : a.Mx, Pz = a.Pz;
+    const int ZB = PX > 0 ? ZBC : a.ZB;
     const int SP = Px + 1;
     const int z0 = zb * ZB;
     double2 *twl = lds + ZB * nseq * SP;
@@ -441,50 +561,58 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
     // Px - n, the bins between are zero.  Loads unpredicated (index clamped
     // into the row), all issued before the LDS stores.
     const int tot = ZB * nseq * Px;
-    constexpr int MAXL = (XMIX_TILE + BLK - 1) / BLK;
-    double2 lx[MAXL], ly[MAXL];
+    constexpr int MAXL = PX > 0 ? (ZBC * nseq * PX + BLK - 1) / BLK : (XMIX_TILE + BLK - 1) / BLK;
+    constexpr int LB = PX > 0 ? MAXL : 4;              // loads in flight per batch
     Walk wl(Px);                                       // (packed row t, bin n)
+#pragma unroll 1
+    for (int i0 = 0; i0 < MAXL; i0 += LB) {
+        if ((int) threadIdx.x + i0 * BLK >= tot) break;
+        double2 lx[LB], ly[LB];
+        int tn[LB];
 #pragma unroll
-    for (int i = 0; i < MAXL; ++i) {
-        const int t = min(wl.q, ZB * nseq - 1), n = wl.r;
-        wl.next();
-        const int s = t % nseq, r = t / nseq;
-        const int k = min(n < Mx ? n : Px - n, Mx - 1);
-        const int f0 = 2 * s, nf1 = a.nf1;
-        const cplx *X = f0 < nf1 ? W1 + (((size_t) y * nf1 + f0) * Pz + z0 + r) * Mx
-                                 : a.W1b + (((size_t) y * (nf - nf1) + f0 - nf1) * Pz + z0 + r) * Mx;
-        const int sy = min(2 * s + 1, nf - 1) - 2 * s;     // 1, or 0 for a lone field
-        lx[i] = X[k];
-        ly[i] = X[(size_t) sy * Pz * Mx + k];
-    }
-    wl = Walk(Px);
+        for (int i = 0; i < LB; ++i) {
+            const int t = min(wl.q, ZB * nseq - 1), n = wl.r;
+            tn[i] = wl.q * Px + n;
+            wl.next();
+            const int s = t % nseq, r = t / nseq;
+            const int k = min(n < Mx ? n : Px - n, Mx - 1);
+            const int f0 = 2 * s, nf1 = a.nf1;
+            const cplx *X = f0 < nf1 ? W1 + (((size_t) y * nf1 + f0) * Pz + z0 + r) * Mx
+                                     : a.W1b + (((size_t) y * (nf - nf1) + f0 - nf1) * Pz + z0 + r) * Mx;
+            const int sy = min(2 * s + 1, nf - 1) - 2 * s;     // 1, or 0 for a lone field
+            lx[i] = X[k];
+            ly[i] = X[(size_t) sy * Pz * Mx + k];
+        }
 #pragma unroll
-    for (int i = 0; i < MAXL; ++i) {
-        const int e = threadIdx.x + i * BLK;
-        if (e >= tot) break;
-        const int t = wl.q, n = wl.r;
-        wl.next();
-        const int s = t % nseq, r = t / nseq;
-        const bool hasy = 2 * s + 1 < nf;
-        double2 xv = lx[i], yv = hasy ? ly[i] : make_double2(0.0, 0.0);
-        double2 val;
-        if (n == 0)
-            val = make_double2(xv.x, yv.x);
-        else if (n < Mx)
-            val = make_double2(xv.x - yv.y, xv.y + yv.x);
-        else if (n > Px - Mx)
-            val = make_double2(xv.x + yv.y, -xv.y + yv.x);
-        else
-            val = make_double2(0.0, 0.0);
-        lds[(r * nseq + s) * SP + n] = val;
+        for (int i = 0; i < LB; ++i) {
+            const int e = threadIdx.x + (i0 + i) * BLK;
+            if (e >= tot) break;
+            const int t = tn[i] / Px, n = tn[i] - t * Px;
+            const int s = t % nseq, r = t / nseq;
+            const bool hasy = 2 * s + 1 < nf;
+            double2 xv = lx[i], yv = hasy ? ly[i] : make_double2(0.0, 0.0);
+            double2 val;
+            if (n == 0)
+                val = make_double2(xv.x, yv.x);
+            else if (n < Mx)
+                val = make_double2(xv.x - yv.y, xv.y + yv.x);
+            else if (n > Px - Mx)
+                val = make_double2(xv.x + yv.y, -xv.y + yv.x);
+            else
+                val = make_double2(0.0, 0.0);
+            lds[(r * nseq + s) * SP + n] = val;
+        }
     }
     __syncthreads();
-    lds_fft(lds, ZB * nseq, SP, 1, px, twl, +1.0);
+    if constexpr (PX > 0)
+        lds_fft_c<PX, ZBC * nseq, PX + 1, 1>(lds, twl, +1.0);
+    else
+        lds_fft(lds, ZB * nseq, SP, 1, px, twl, +1.0);
     // pointwise stage: read all fields of a point, compute, then (after a
     // barrier) write the packed outputs compactly: row r, pair t at sequence
     // r*nso + t
-    constexpr int nso = (NOUT + 1) / 2;
-    constexpr int MAXP = (XMIX_TILE / nseq + BLK - 1) / BLK;   // points per thread
+    constexpr int MAXP = PX > 0 ? (ZBC * PX + BLK - 1) / BLK          // points per thread
+                                : (XMIX_TILE / nseq + BLK - 1) / BLK;
     constexpr int NO = NOUT > 0 ? NOUT : 1;
     double outv[MAXP][NO];
     double cmax = 0.0, acc0 = 0.0, acc1 = 0.0;
@@ -590,11 +718,14 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
         }
     }
     __syncthreads();
-    lds_fft(lds, ZB * nso, SP, 1, px, twl, -1.0);
+    if constexpr (PX > 0)
+        lds_fft_c<PX, ZBC * nso, PX + 1, 1>(lds, twl, -1.0);
+    else
+        lds_fft(lds, ZB * nso, SP, 1, px, twl, -1.0);
     // unpack two real transforms from each packed pair, bins kx < Mx
     const int tot2 = ZB * nso * Mx;
     Walk wu(Mx);                                       // (packed row, bin k)
-    for (int e = threadIdx.x; e < tot2; e += blockDim.x, wu.next()) {
+    for (int e = threadIdx.x; e < tot2; e += BLK, wu.next()) {
         const int k = wu.r, t = wu.q % nso, r = wu.q / nso;
         const double2 *Z = lds + (size_t) (r * nso + t) * SP;
         double2 zk = Z[k], zm = Z[k == 0 ? 0 : Px - k];
@@ -614,38 +745,49 @@ struct ZfwdArgs {
     const int *abort_flag;
 };
 
-// grid (ceil(Mx/XB), nout, qpts)
+// grid (ceil(Mx/XB), nout, qpts); PZ > 0: Pz == PZ, XB == ZXB and Mx % ZXB == 0
+template <int PZ>
 __global__ __launch_bounds__(BLK) void k_zfwd(ZfwdArgs a, PlanDev pz, const cplx *W2, cplx *dst)
 {
     extern __shared__ double2 lds[];
     if (*a.abort_flag) return;
-    const int x0 = blockIdx.x * a.XB, g = blockIdx.y, y = blockIdx.z;
-    const int XB = min(a.XB, a.Mx - x0);
-    const int Nz = a.Nz, Pz = a.Pz, Mx = a.Mx;
+    const int XB0 = PZ > 0 ? ZXB : a.XB;
+    const int x0 = blockIdx.x * XB0, g = blockIdx.y, y = blockIdx.z;
+    const int XB = PZ > 0 ? ZXB : min(a.XB, a.Mx - x0);
+    const int Nz = PZ > 0 ? 2 * PZ / 3 : a.Nz, Pz = PZ > 0 ? PZ : a.Pz, Mx = a.Mx;
     const cplx *src = W2 + (((size_t) y * a.nout + g) * Pz) * Mx;
-    double2 *twl = lds + a.XB * Pz;
+    double2 *twl = lds + XB0 * Pz;
     stage_twiddles(pz, twl);
     {
-        // all loads issued before the LDS stores
+        // a batch's loads all issued before its LDS stores
+        constexpr int NE = PZ > 0 ? (ZXB * PZ + BLK - 1) / BLK : MAXE;
+        constexpr int SB = PZ > 0 ? NE : SBAT;
         const int total = XB * Pz;
-        double2 v[MAXE];
         Walk wk(XB);
+#pragma unroll 1
+        for (int i0 = 0; i0 < NE; i0 += SB) {
+            if ((int) threadIdx.x + i0 * BLK >= total) break;
+            double2 v[SB];
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i) {
-            const int zp = min(wk.q, Pz - 1), xl = wk.r;
-            wk.next();
-            v[i] = src[(size_t) zp * Mx + x0 + xl];
-        }
+            for (int i = 0; i < SB; ++i) {
+                const int zp = min(wk.q, Pz - 1), xl = wk.r;
+                wk.next();
+                v[i] = src[(size_t) zp * Mx + x0 + xl];
+            }
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i) {
-            const int e = threadIdx.x + i * BLK;
-            if (e < total) lds[e] = v[i];
+            for (int i = 0; i < SB; ++i) {
+                const int e = threadIdx.x + (i0 + i) * BLK;
+                if (e < total) lds[e] = v[i];
+            }
         }
     }
     __syncthreads();
-    lds_fft(lds, XB, 1, XB, pz, twl, -1.0);
+    if constexpr (PZ > 0)
+        lds_fft_c<PZ, ZXB, 1, ZXB>(lds, twl, -1.0);
+    else
+        lds_fft(lds, XB, 1, XB, pz, twl, -1.0);
     Walk wt(XB);
-    for (int e = threadIdx.x; e < XB * (Nz - 1); e += blockDim.x, wt.next()) {
+    for (int e = threadIdx.x; e < XB * (Nz - 1); e += BLK, wt.next()) {
         const int r = wt.q, xl = wt.r;
         int kz = r < Nz / 2 ? r : r + 1;             // skip the Nyquist row
         int zp = kz < Nz / 2 ? kz : kz + Nz / 2;
@@ -689,6 +831,16 @@ __global__ void k_assemble_adj(const cplx *SP, cplx *U, const double *Kx, const 
 // ---------------------------------------------------------------- launches
 int xmix_zblock(Context &c, int nf);
 
+// fixed-length instances of the z kernels: Pz of 48 or 96, Nx/2 a multiple of ZXB
+typedef void (*ZinvKernel)(ZinvArgs, PlanDev, cplx *);
+typedef void (*ZfwdKernel)(ZfwdArgs, PlanDev, const cplx *, cplx *);
+
+static int z_fixed(const Dims &d)
+{
+    if (d.Mx % ZXB) return 0;
+    return d.Pz == 96 ? 96 : d.Pz == 48 ? 48 : 0;
+}
+
 static int pick_block(int Mx, int P, int nper)
 {
     // columns per block so that the LDS tile plus the P twiddles stay within
@@ -709,7 +861,8 @@ void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1)
     a.Mx = d.Mx;
     a.qpts = d.qpts;
     a.Pz = d.Pz;
-    a.XB = pick_block(d.Mx, d.Pz, 1);
+    const int fixed = z_fixed(d);
+    a.XB = fixed ? ZXB : pick_block(d.Mx, d.Pz, 1);
     a.ax = 2. * M_PI / d.Lx;
     a.az = 2. * M_PI / d.Lz;
     a.abort_flag = c.abort_flag;
@@ -727,10 +880,33 @@ void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1)
     double bytes = plane * ((double) slots * (d.Nz - 1) + (double) fs.nf * d.Pz);
     const hipStream_t st = c.stream;
     const PlanDev pd = to_dev(c.pz);
-    auto go = [=]() { hipLaunchKernelGGL(k_zinv, grid, dim3(BLK), lds, st, a, pd, W1); };
+    const ZinvKernel kern = fixed == 96 ? k_zinv<96> : fixed == 48 ? k_zinv<48> : k_zinv<0>;
+    auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, W1); };
     prof_begin(c, st, 1, bytes);
     go();
     prof_end(c, st, 1, go);
+}
+
+typedef void (*XmixKernel)(XmixArgs, PlanDev, const cplx *, cplx *);
+
+template <int MODE>
+static XmixKernel xmix_kernel_mode(int Px, int nseq, int zb)
+{
+    // the fixed-length instance needs the row block it was compiled for
+    if (Px == 96 && zb == xmix_rows(96, nseq)) return k_xmix<MODE, 96>;
+    if (Px == 48 && zb == xmix_rows(48, nseq)) return k_xmix<MODE, 48>;
+    return k_xmix<MODE, 0>;
+}
+
+static XmixKernel xmix_kernel(int mode, int Px, int nseq, int zb)
+{
+    switch (mode) {
+    case XM_PRIMAL: return xmix_kernel_mode<XM_PRIMAL>(Px, nseq, zb);
+    case XM_TANGENT: return xmix_kernel_mode<XM_TANGENT>(Px, nseq, zb);
+    case XM_ADJOINT: return xmix_kernel_mode<XM_ADJOINT>(Px, nseq, zb);
+    case XM_PHYS: return xmix_kernel_mode<XM_PHYS>(Px, nseq, zb);
+    default: return xmix_kernel_mode<XM_DOT>(Px, nseq, zb);
+    }
 }
 
 void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
@@ -767,35 +943,12 @@ void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
     double bytes = rows * (a.nf + a.nout);
     if (mode == XM_PHYS) bytes += 8.0 * 3 * d.Px * d.qpts * d.Pz;
     const hipStream_t st = c.stream;
-    auto go = [=]() {
-        switch (mode) {
-        case XM_PRIMAL:
-            hipLaunchKernelGGL(k_xmix<XM_PRIMAL>, grid, dim3(BLK), lds, st, a, pd, W1, W2);
-            break;
-        case XM_TANGENT:
-            hipLaunchKernelGGL(k_xmix<XM_TANGENT>, grid, dim3(BLK), lds, st, a, pd, W1, W2);
-            break;
-        case XM_ADJOINT:
-            hipLaunchKernelGGL(k_xmix<XM_ADJOINT>, grid, dim3(BLK), lds, st, a, pd, W1, W2);
-            break;
-        case XM_PHYS:
-            hipLaunchKernelGGL(k_xmix<XM_PHYS>, grid, dim3(BLK), lds, st, a, pd, W1, W2);
-            break;
-        default:
-            hipLaunchKernelGGL(k_xmix<XM_DOT>, grid, dim3(BLK), lds, st, a, pd, W1, W2);
-        }
-    };
+    const XmixKernel kern = xmix_kernel(mode, d.Px, nseq, zb);
+    auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, W1, W2); };
     // the CFL maximum accumulates across launches, so a replay skips it
     XmixArgs ar = a;
     ar.check_cfl = 0;
-    auto again = [=]() {
-        if (mode == XM_ADJOINT)
-            hipLaunchKernelGGL(k_xmix<XM_ADJOINT>, grid, dim3(BLK), lds, st, ar, pd, W1, W2);
-        else if (mode == XM_PRIMAL)
-            hipLaunchKernelGGL(k_xmix<XM_PRIMAL>, grid, dim3(BLK), lds, st, ar, pd, W1, W2);
-        else if (mode == XM_TANGENT)
-            hipLaunchKernelGGL(k_xmix<XM_TANGENT>, grid, dim3(BLK), lds, st, ar, pd, W1, W2);
-    };
+    auto again = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, ar, pd, W1, W2); };
     prof_begin(c, st, 2, bytes);
     go();
     // replays only the modes whose launches are side-effect free
@@ -823,7 +976,8 @@ void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind)
     a.Mx = d.Mx;
     a.Pz = d.Pz;
     a.qpts = d.qpts;
-    a.XB = pick_block(d.Mx, d.Pz, 1);
+    const int fixed = z_fixed(d);
+    a.XB = fixed ? ZXB : pick_block(d.Mx, d.Pz, 1);
     a.dst_kind = dst_kind;
     a.norm = 1.0 / ((3. * d.Nx / 2.) * (3. * d.Nz / 2.));
     a.abort_flag = c.abort_flag;
@@ -832,7 +986,8 @@ void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind)
     double bytes = 16.0 * (double) d.qpts * d.Mx * nout * (d.Pz + d.Nz - 1);
     const hipStream_t st = c.stream;
     const PlanDev pd = to_dev(c.pz);
-    auto go = [=]() { hipLaunchKernelGGL(k_zfwd, grid, dim3(BLK), lds, st, a, pd, W2, dst); };
+    const ZfwdKernel kern = fixed == 96 ? k_zfwd<96> : fixed == 48 ? k_zfwd<48> : k_zfwd<0>;
+    auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, W2, dst); };
     prof_begin(c, st, 3, bytes);
     go();
     prof_end(c, st, 3, go);

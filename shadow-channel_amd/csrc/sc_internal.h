@@ -25,6 +25,7 @@ typedef double2 cplx;
 
 constexpr int MAXF = 12;        // fields in one z-inverse launch
 constexpr int FFT_TILE = 2048;  // complex values (data + twiddles) in one transform block
+constexpr int FFT_BLOCK = 256;  // threads of a transform block
 // MFMA k-steps per staged chunk of the y-transforms (tuning constants)
 #ifndef SC_KC_REBUILD
 #define SC_KC_REBUILD 8

@@ -119,10 +119,10 @@ static int plan_fft(FftPlan &p, int n)
     p.n = n;
     p.nf = 0;
     // fewest passes over the radices the device implements; among those, the
-    // smallest per-thread register tile (ceil(FFT_TILE/(256 R)) butterflies of
-    // R points)
+    // smallest per-thread register tile (ceil(FFT_TILE/(FFT_BLOCK R))
+    // butterflies of R points)
     static const int RAD[] = {16, 12, 8, 6, 5, 4, 3, 2};
-    auto tile = [](int r) { return (FFT_TILE / 256 + r - 1) / r * r; };
+    auto tile = [](int r) { return (FFT_TILE / FFT_BLOCK + r - 1) / r * r; };
     std::vector<int> passes(n + 1, 1 << 20), worst(n + 1, 0), first(n + 1, 0);
     if (n >= 1) passes[1] = 0;
     for (int m = 2; m <= n; ++m) {
