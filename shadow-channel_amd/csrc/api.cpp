@@ -171,7 +171,6 @@ static const cplx *wcached(Context &c, long j)
 static void pass1_state(Context &c, cplx *Ub, int check_cfl, cplx *wdst = nullptr)
 {
     cplx *w = wdst ? wdst : c.W1;
-    if (check_cfl) (void) hipMemsetAsync(c.cflmax, 0, sizeof(double) * c.d.qpts, c.stream);
     launch_zinv(c, velocity_and_vorticity(Ub), w);
     launch_xmix(c, XM_PRIMAL, w, c.W2, check_cfl, nullptr, nullptr);
     if (check_cfl) launch_cfl_check(c);
@@ -183,7 +182,6 @@ static void pass1_state(Context &c, cplx *Ub, int check_cfl, cplx *wdst = nullpt
 static void pass1_tangent(Context &c, const cplx *Ub, cplx *IUb, int check_cfl,
                           const cplx *wbase = nullptr)
 {
-    if (check_cfl) (void) hipMemsetAsync(c.cflmax, 0, sizeof(double) * c.d.qpts, c.stream);
     if (wbase) {
         launch_zinv(c, velocity_and_vorticity(IUb), c.W1);
         launch_xmix(c, XM_TANGENT, wbase, c.W2, check_cfl, nullptr, nullptr, c.W1, 6);
@@ -232,15 +230,17 @@ static void pass2_adjoint(Context &c, const cplx *Ub, const cplx *AUb, cplx *Uds
     launch_assemble_adj(c, c.SP, Udst);
 }
 
-// v -= (num/den) dC/dt with the Gauss-weighted physical inner products
-// (channel/channel.c:704-780); v is a device array.  Result in scal[2].
-static void ddt_project_dev(Context &c, int i, cplx *v)
+// vout = vin - (num/den) dC/dt with the Gauss-weighted physical inner
+// products (channel/channel.c:704-780); device arrays, vout may be vin.
+// Result in scal[2].  With acc, also acc -= a vout.
+static void ddt_project_dev(Context &c, int i, const cplx *vin, cplx *vout,
+                            cplx *acc = nullptr, double a = 0.0)
 {
     const Dims &d = c.d;
     long ip = i == 0 ? 3 : 3L * i, im = i == 0 ? 0 : 3L * (i - 1);
     launch_diff_scale(c, c.Ctmp2, mc(c, ip), mc(c, im), 1.0 / d.dt, c.csize);
     launch_rebuild(c, c.Ctmp2, c.Ut);
-    launch_rebuild(c, v, c.Ut2);
+    launch_rebuild(c, vin, c.Ut2);
     FieldSpec fs;
     fs.nf = 6;
     for (int s = 0; s < 3; ++s) {
@@ -253,7 +253,7 @@ static void ddt_project_dev(Context &c, int i, cplx *v)
     launch_xmix(c, XM_DOT, c.W1, nullptr, 0, nullptr, c.dotpart);
     int nparts = d.qpts * (d.Pz / xmix_zblock(c, 6));
     launch_ddt_reduce(c, nparts);
-    launch_ddt_apply(c, v, c.Ctmp2, c.csize);
+    launch_ddt_apply(c, vout, vin, c.Ctmp2, c.csize, acc, a);
 }
 
 static int check_abort(Context &c, const char *what)
@@ -263,7 +263,7 @@ static int check_abort(Context &c, const char *what)
     if (flag) {
         int y = flag - 1;
         double m = 0.0;
-        (void) hipMemcpy(&m, c.cflmax + y, sizeof(double), hipMemcpyDeviceToHost);
+        (void) hipMemcpy(&m, c.cflmax + c.d.qpts, sizeof(double), hipMemcpyDeviceToHost);
         printf("Pass1:  CFL[%d]:%e, Max:%e dt:%e.  CFL condition violated.\n", y, m * c.d.dt,
                2.8, c.d.dt);
         printf("%s failure\n", what);
@@ -484,7 +484,7 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
         {(void **) &c.W2, sizeof(cplx) * (size_t) d.qpts * 6 * d.Pz * d.Mx},
         {(void **) &c.SP, sizeof(cplx) * 6 * (size_t) d.Nz * d.qpts * d.Mx},
         {(void **) &c.flowdev, sizeof(double) * 3 * (size_t) d.Px * d.qpts * d.Pz},
-        {(void **) &c.cflmax, sizeof(double) * d.qpts},
+        {(void **) &c.cflmax, sizeof(double) * (d.qpts + 1)},
         {(void **) &c.dotpart, sizeof(double) * 2 * (size_t) d.qpts * d.Pz},
         {(void **) &c.scal, sizeof(double) * 8},
         {(void **) &c.abort_flag, sizeof(int)},
@@ -550,6 +550,8 @@ int c_primal(int ru_steps, sc_complex *C_given, int Nz_dup, int Nvar_dup, int Ny
     SC_HIP(hipMemsetAsync(c.MC, 0, cb * (3 * (size_t) d.nsteps + 1), c.stream));
     SC_HIP(hipMemcpyAsync(c.MC, c.C, cb, hipMemcpyDeviceToDevice, c.stream));
     SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
+    // each CFL check clears the maxima it read; a run starts from zero
+    SC_HIP(hipMemsetAsync(c.cflmax, 0, sizeof(double) * c.d.qpts, c.stream));
     c.status = 0;
     std::fill(c.wvalid.begin(), c.wvalid.end(), 0);
     const int total = d.nsteps + ru_steps;
@@ -597,6 +599,8 @@ int c_tangent(int start, int end, sc_complex *IC_given, int Nz_dup, int Nvar_dup
     SC_HIP(hipMemcpyAsync(c.IC, IC_given, cb, hipMemcpyHostToDevice, c.stream));
     SC_HIP(hipMemcpyAsync(mic(c, 3L * start), c.IC, cb, hipMemcpyDeviceToDevice, c.stream));
     SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
+    // each CFL check clears the maxima it read; a run starts from zero
+    SC_HIP(hipMemsetAsync(c.cflmax, 0, sizeof(double) * c.d.qpts, c.stream));
     launch_rebuild(c, c.IC, c.IU);
     c.status = 0;
     const double target = inhomo ? 2.0 : 0.0;
@@ -642,9 +646,12 @@ int c_adjoint(int start, int end, sc_complex *AC_given, int Nz_dup, int Nvar_dup
     launch_rebuild(c, c.AC, c.AU);
     const double s = strength * d.dt;
     for (int n = start; n > end; --n) {
-        SC_HIP(hipMemcpyAsync(c.Ctmp, mic(c, 3L * n), cb, hipMemcpyDeviceToDevice, c.stream));
-        ddt_project_dev(c, n, c.Ctmp);
-        launch_axpy_c(c, c.AC, c.Ctmp, 0.5 * s, c.csize);
+        // The projection of MIC[3n] at step n was made at the end of the
+        // previous iteration (the reference computes it twice): reuse it.
+        if (n == start)
+            ddt_project_dev(c, n, mic(c, 3L * n), c.Ctmp, c.AC, 0.5 * s);
+        else
+            launch_axpy_c(c, c.AC, c.Ctmp, 0.5 * s, c.csize);
         for (int k = 0; k < 3; ++k) {
             long count = 3L * n - k - 1;
             if (k < 2) {
@@ -660,9 +667,7 @@ int c_adjoint(int start, int end, sc_complex *AC_given, int Nz_dup, int Nvar_dup
             launch_solve(c, c.AC, c.Csol, k, 1, c.FLa, c.FLb, 0.0);
             launch_rebuild(c, c.Csol, c.AU);
         }
-        SC_HIP(hipMemcpyAsync(c.Ctmp, mic(c, 3L * (n - 1)), cb, hipMemcpyDeviceToDevice, c.stream));
-        ddt_project_dev(c, n - 1, c.Ctmp);
-        launch_axpy_c(c, c.AC, c.Ctmp, 0.5 * s, c.csize);
+        ddt_project_dev(c, n - 1, mic(c, 3L * (n - 1)), c.Ctmp, c.AC, 0.5 * s);
     }
     int rc = stream_sync(c);
     if (rc) return rc;
@@ -680,7 +685,7 @@ double c_ddt_project(int i_step, sc_complex *IC_given, int Nz_dup, int Nvar_dup,
     const size_t cb = sizeof(cplx) * c.csize;
     if (hipMemcpyAsync(c.Ctmp, IC_given, cb, hipMemcpyHostToDevice, c.stream) != hipSuccess) return NAN;
     SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream)) ;
-    ddt_project_dev(c, i_step, c.Ctmp);
+    ddt_project_dev(c, i_step, c.Ctmp, c.Ctmp);
     double scal[3] = {0, 0, 0};
     if (hipMemcpyAsync(IC_given, c.Ctmp, cb, hipMemcpyDeviceToHost, c.stream) != hipSuccess) return NAN;
     if (hipMemcpyAsync(scal, c.scal, sizeof(scal), hipMemcpyDeviceToHost, c.stream) != hipSuccess) return NAN;

@@ -1053,26 +1053,46 @@ __global__ void k_ddt_reduce(const double *part, int nparts, double *scal)
     }
 }
 
-__global__ void k_ddt_apply(cplx *v, const cplx *dC, const double *scal, size_t n)
+// vout = vin - r dC (r = scal[2]); with acc, also acc -= a vout
+__global__ void k_ddt_apply(cplx *vout, const cplx *vin, const cplx *dC, const double *scal,
+                            size_t n, cplx *acc, double a)
 {
     double r = scal[2];
     for (size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x; e < n;
          e += (size_t) gridDim.x * blockDim.x) {
-        v[e].x -= r * dC[e].x;
-        v[e].y -= r * dC[e].y;
+        cplx v = vin[e];
+        v.x -= r * dC[e].x;
+        v.y -= r * dC[e].y;
+        vout[e] = v;
+        if (acc) {
+            acc[e].x -= a * v.x;
+            acc[e].y -= a * v.y;
+        }
     }
 }
 
-// first plane whose CFL number reaches the limit sets abort_flag = 1 + plane
-__global__ void k_cfl_check(const double *cflmax, int qpts, double dt, int *abort_flag)
+// The first plane whose CFL number reaches the limit sets abort_flag =
+// 1 + plane and leaves its maximum in cflmax[qpts]; the per-plane maxima are
+// cleared for the next check.  One block of CFL_THREADS.
+constexpr int CFL_THREADS = 256;
+__global__ __launch_bounds__(CFL_THREADS) void k_cfl_check(double *cflmax, int qpts, double dt,
+                                                           int *abort_flag)
 {
-    if (threadIdx.x != 0 || *abort_flag) return;
-    for (int y = 0; y < qpts; ++y) {
-        if (cflmax[y] * dt >= 2.8) {
-            *abort_flag = 1 + y;
-            return;
-        }
+    __shared__ int first;
+    if (threadIdx.x == 0) first = qpts;
+    __syncthreads();
+    const bool live = *abort_flag == 0;
+    for (int y = threadIdx.x; y < qpts; y += CFL_THREADS) {
+        if (live && cflmax[y] * dt >= 2.8) atomicMin(&first, y);
     }
+    __syncthreads();
+    const int y0 = first;
+    if (y0 < qpts && threadIdx.x == 0) {
+        cflmax[qpts] = cflmax[y0];
+        *abort_flag = 1 + y0;
+    }
+    __syncthreads();
+    for (int y = threadIdx.x; y < qpts; y += CFL_THREADS) cflmax[y] = 0.0;
 }
 
 static int nblocks(size_t n) { return (int) std::min<size_t>((n + 255) / 256, 8192); }
@@ -1092,15 +1112,17 @@ void launch_ddt_reduce(Context &c, int nparts)
     hipLaunchKernelGGL(k_ddt_reduce, dim3(1), dim3(256), 0, c.stream, c.dotpart, nparts, c.scal);
 }
 
-void launch_ddt_apply(Context &c, cplx *v, const cplx *dC, size_t n)
+void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *dC, size_t n,
+                      cplx *acc, double a)
 {
-    hipLaunchKernelGGL(k_ddt_apply, dim3(nblocks(n)), dim3(256), 0, c.stream, v, dC, c.scal, n);
+    hipLaunchKernelGGL(k_ddt_apply, dim3(nblocks(n)), dim3(256), 0, c.stream, vout, vin, dC,
+                       c.scal, n, acc, a);
 }
 
 void launch_cfl_check(Context &c)
 {
-    hipLaunchKernelGGL(k_cfl_check, dim3(1), dim3(64), 0, c.stream, c.cflmax, c.d.qpts, c.d.dt,
-                       c.abort_flag);
+    hipLaunchKernelGGL(k_cfl_check, dim3(1), dim3(CFL_THREADS), 0, c.stream, c.cflmax, c.d.qpts,
+                       c.d.dt, c.abort_flag);
 }
 
 }  // namespace sc

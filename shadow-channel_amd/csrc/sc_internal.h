@@ -123,7 +123,9 @@ struct Context {
     size_t wslot = 0;                           // complex values per state
     std::vector<char> wvalid;                   // entry j holds MC[j]'s fields
     double *flowdev = nullptr;                  // [3][Px][qpts][Pz]
-    double *cflmax = nullptr;                   // [qpts]
+    // [qpts + 1]: per-plane CFL maxima of the running check (zero between
+    // checks), then the maximum of the plane that stopped a run
+    double *cflmax = nullptr;
     double *dotpart = nullptr;                  // partial sums
     double *scal = nullptr;                     // [8] device scalars
     int *abort_flag = nullptr;
@@ -166,7 +168,9 @@ void launch_cfl_check(Context &c);
 int xmix_zblock(Context &c, int nf);
 void launch_axpy_c(Context &c, cplx *y, const cplx *x, double a, size_t n);
 void launch_diff_scale(Context &c, cplx *out, const cplx *a, const cplx *b, double s, size_t n);
-void launch_ddt_apply(Context &c, cplx *v, const cplx *dC, size_t n);
+// vout = vin - (num/den) dC; with acc, also acc -= a vout
+void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *dC, size_t n,
+                      cplx *acc = nullptr, double a = 0.0);
 void launch_ddt_reduce(Context &c, int nparts);
 
 // dst_kind for launch_zfwd
