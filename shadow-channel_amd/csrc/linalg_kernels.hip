@@ -622,6 +622,8 @@ struct SolveArgs {
     const cplx *Fa, *Fb;               // projection for this stage's RHS
     const cplx *Fa2, *Fb2;             // projection for the next-stage RHS
     int Nz, Mx, dimR, dimQ, k, has_next;
+    int nblk;                          // column blocks of SBT columns
+    int nyq_mean;                      // the mean mode runs in the Nyquist blocks' first (1) or in an extra block (0)
     double dtc, dtd;                   // dt c_k, dt d_{k+1}
     double ce0[2], cn[2];              // explicit-operator coefficients per field (stage 0, next)
     double target;                     // mean-mode flux target
@@ -643,71 +645,160 @@ __device__ __forceinline__ double op_entry(const SolveArgs &a, int var, int i, i
 // Forward and back substitution of one chain held in LDS.  Factors of row r:
 // f[r*fr + j*fj], j = DL - m for L(r, r-m), j = DL for 1/U(r, r),
 // j = DL + u for U(r, r+u); entries that fall outside the matrix are zero.
-// v: right-hand side in, solution out, row r at v[r*vr].  The next row's
-// operands are fetched while the current row is eliminated.
+// v: right-hand side in, solution out, row r at v[r*vr].
+//
+// Forward sweep, row s: x_s = v_s - L(s,DL) x_{s-DL} - ... - L(s,1) x_{s-1},
+// the terms applied farthest first (the reference's order).  It runs skewed:
+// the iteration that produces x_{r-1} applies it to the rows r .. r+DL-1 at
+// once (term m = k+1 of row r+k), so the DL multiply-adds of an iteration are
+// independent and only one of them waits for the previous iteration.  Every
+// row still sees its terms in the same order, so the result is the same as
+// the row-by-row sweep.  P[k] holds row r+k with its terms DL .. k+2 applied.
+//
+// Operands stream from LDS through two register buffers of two rows each,
+// used in turn by a body of four rows: the loads for rows r+4, r+5 go into
+// the buffer rows r, r+1 have just used, issued before the arithmetic of rows
+// r+2, r+3 (a scheduling barrier keeps them there), with no register copies
+// that would wait for them.  Row indices are clamped into the chain; values
+// of rows past the end are never stored.
+template <int DL>
+struct FwdOps {
+    double c[2][DL];   // c[i][k] = L(r+i+k, k+1)
+    double2 vn[2];     // right-hand side of row r+i+DL-1
+};
+
+template <int DL>
+__device__ __forceinline__ void fwd_load(FwdOps<DL> &o, const double *f, int fr, int fj,
+                                         const double2 *v, int vr, int r, int nr)
+{
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int k = 0; k < DL; ++k) o.c[i][k] = f[min(r + i + k, nr - 1) * fr + (DL - 1 - k) * fj];
+        o.vn[i] = v[min(r + i + DL - 1, nr - 1) * vr];
+    }
+}
+
+template <int DL>
+__device__ __forceinline__ void fwd_step(const double *c, double2 vn, double2 *P, double2 &xm1,
+                                         double2 *v, int vr, int r)
+{
+    double2 x = P[0];
+    x.x -= c[0] * xm1.x;
+    x.y -= c[0] * xm1.y;
+#pragma unroll
+    for (int k = 1; k < DL - 1; ++k) {
+        P[k - 1].x = P[k].x - c[k] * xm1.x;
+        P[k - 1].y = P[k].y - c[k] * xm1.y;
+    }
+    P[DL - 2].x = vn.x - c[DL - 1] * xm1.x;
+    P[DL - 2].y = vn.y - c[DL - 1] * xm1.y;
+    v[r * vr] = x;
+    xm1 = x;
+}
+
+template <int DL>
+__device__ __forceinline__ void chain_forward(const double *f, int fr, int fj, double2 *v, int vr,
+                                              int nr)
+{
+    double2 P[DL - 1];
+#pragma unroll
+    for (int k = 0; k < DL - 1; ++k) P[k] = v[min(k, nr - 1) * vr];
+    double2 xm1 = make_double2(0.0, 0.0);
+    FwdOps<DL> X, Y;
+    fwd_load<DL>(X, f, fr, fj, v, vr, 0, nr);
+    fwd_load<DL>(Y, f, fr, fj, v, vr, 2, nr);
+    int r = 0;
+    for (; r + 4 <= nr; r += 4) {
+        fwd_step<DL>(X.c[0], X.vn[0], P, xm1, v, vr, r);
+        fwd_step<DL>(X.c[1], X.vn[1], P, xm1, v, vr, r + 1);
+        fwd_load<DL>(X, f, fr, fj, v, vr, r + 4, nr);
+        __builtin_amdgcn_sched_barrier(0);
+        fwd_step<DL>(Y.c[0], Y.vn[0], P, xm1, v, vr, r + 2);
+        fwd_step<DL>(Y.c[1], Y.vn[1], P, xm1, v, vr, r + 3);
+        fwd_load<DL>(Y, f, fr, fj, v, vr, r + 6, nr);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // fewer than four rows left: X holds rows r, r+1, Y rows r+2, r+3
+    if (r < nr) fwd_step<DL>(X.c[0], X.vn[0], P, xm1, v, vr, r);
+    if (r + 1 < nr) fwd_step<DL>(X.c[1], X.vn[1], P, xm1, v, vr, r + 1);
+    if (r + 2 < nr) fwd_step<DL>(Y.c[0], Y.vn[0], P, xm1, v, vr, r + 2);
+}
+
+// Backward sweep, row r: x_r = (v_r - U(r,1) x_{r+1} - ... - U(r,DL) x_{r+DL})
+// / U(r,r), nearest term first (the reference's order).  Same two-buffer
+// stream of operands as the forward sweep, rows descending.
+template <int DL>
+struct BwdOps {
+    double u[2][DL + 1];   // U(r-i, r-i+1..r-i+DL), then 1/U(r-i, r-i)
+    double2 v[2];          // right-hand side of row r-i
+};
+
+template <int DL>
+__device__ __forceinline__ void bwd_load(BwdOps<DL> &o, const double *f, int fr, int fj,
+                                         const double2 *v, int vr, int r)
+{
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int rr = max(r - i, 0);
+#pragma unroll
+        for (int k = 1; k <= DL; ++k) o.u[i][k - 1] = f[rr * fr + (DL + k) * fj];
+        o.u[i][DL] = f[rr * fr + DL * fj];
+        o.v[i] = v[rr * vr];
+    }
+}
+
+template <int DL>
+__device__ __forceinline__ void bwd_step(const double *u, double2 x, double2 *w, double2 *v,
+                                         int vr, int r)
+{
+#pragma unroll
+    for (int k = 1; k <= DL; ++k) {
+        x.x -= u[k - 1] * w[k - 1].x;
+        x.y -= u[k - 1] * w[k - 1].y;
+    }
+    x.x *= u[DL];
+    x.y *= u[DL];
+#pragma unroll
+    for (int m = DL - 1; m > 0; --m) w[m] = w[m - 1];
+    w[0] = x;
+    v[r * vr] = x;
+}
+
+template <int DL>
+__device__ __forceinline__ void chain_backward(const double *f, int fr, int fj, double2 *v, int vr,
+                                               int nr)
+{
+    double2 w[DL];
+#pragma unroll
+    for (int m = 0; m < DL; ++m) w[m] = make_double2(0.0, 0.0);
+    BwdOps<DL> X, Y;
+    bwd_load<DL>(X, f, fr, fj, v, vr, nr - 1);
+    bwd_load<DL>(Y, f, fr, fj, v, vr, nr - 3);
+    int r = nr - 1;
+    // rows below r - 3 still hold forward values until their own step
+    for (; r >= 3; r -= 4) {
+        bwd_step<DL>(X.u[0], X.v[0], w, v, vr, r);
+        bwd_step<DL>(X.u[1], X.v[1], w, v, vr, r - 1);
+        bwd_load<DL>(X, f, fr, fj, v, vr, r - 4);
+        __builtin_amdgcn_sched_barrier(0);
+        bwd_step<DL>(Y.u[0], Y.v[0], w, v, vr, r - 2);
+        bwd_step<DL>(Y.u[1], Y.v[1], w, v, vr, r - 3);
+        bwd_load<DL>(Y, f, fr, fj, v, vr, r - 6);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // fewer than four rows left: X holds rows r, r-1, Y rows r-2, r-3
+    if (r >= 0) bwd_step<DL>(X.u[0], X.v[0], w, v, vr, r);
+    if (r >= 1) bwd_step<DL>(X.u[1], X.v[1], w, v, vr, r - 1);
+    if (r >= 2) bwd_step<DL>(Y.u[0], Y.v[0], w, v, vr, r - 2);
+}
+
 template <int DL>
 __device__ void chain_subst(const double *f, int fr, int fj, double2 *v, int vr, int nr)
 {
     if (nr <= 0) return;
-    double2 w[DL];
-#pragma unroll
-    for (int m = 0; m < DL; ++m) w[m] = make_double2(0.0, 0.0);
-    double cf[DL], nf[DL];
-    double2 nv = v[0];
-#pragma unroll
-    for (int m = 1; m <= DL; ++m) nf[m - 1] = f[(DL - m) * fj];
-    for (int r = 0; r < nr; ++r) {
-        double2 x = nv;
-#pragma unroll
-        for (int m = 0; m < DL; ++m) cf[m] = nf[m];
-        if (r + 1 < nr) {
-            nv = v[(r + 1) * vr];
-#pragma unroll
-            for (int m = 1; m <= DL; ++m) nf[m - 1] = f[(r + 1) * fr + (DL - m) * fj];
-        }
-#pragma unroll
-        for (int m = DL; m >= 1; --m) {
-            x.x -= cf[m - 1] * w[m - 1].x;
-            x.y -= cf[m - 1] * w[m - 1].y;
-        }
-#pragma unroll
-        for (int m = DL - 1; m > 0; --m) w[m] = w[m - 1];
-        w[0] = x;
-        v[r * vr] = x;
-    }
-#pragma unroll
-    for (int m = 0; m < DL; ++m) w[m] = make_double2(0.0, 0.0);
-    double crd, nrd;
-    {
-        const int r = nr - 1;
-        nv = v[r * vr];
-        nrd = f[r * fr + DL * fj];
-#pragma unroll
-        for (int u = 1; u <= DL; ++u) nf[u - 1] = f[r * fr + (DL + u) * fj];
-    }
-    for (int r = nr - 1; r >= 0; --r) {
-        double2 x = nv;
-        crd = nrd;
-#pragma unroll
-        for (int u = 0; u < DL; ++u) cf[u] = nf[u];
-        if (r > 0) {
-            nv = v[(r - 1) * vr];
-            nrd = f[(r - 1) * fr + DL * fj];
-#pragma unroll
-            for (int u = 1; u <= DL; ++u) nf[u - 1] = f[(r - 1) * fr + (DL + u) * fj];
-        }
-#pragma unroll
-        for (int u = 1; u <= DL; ++u) {
-            x.x -= cf[u - 1] * w[u - 1].x;
-            x.y -= cf[u - 1] * w[u - 1].y;
-        }
-        x.x *= crd;
-        x.y *= crd;
-#pragma unroll
-        for (int u = DL - 1; u > 0; --u) w[u] = w[u - 1];
-        w[0] = x;
-        v[r * vr] = x;
-    }
+    chain_forward<DL>(f, fr, fj, v, vr, nr);
+    chain_backward<DL>(f, fr, fj, v, vr, nr);
 }
 
 struct ColumnOf {
@@ -725,57 +816,86 @@ __device__ __forceinline__ ColumnOf column_of(const SolveArgs &a, int col)
     return c;
 }
 
-// one (field, parity) of SBT consecutive columns; lds: [nr][W][SBT] factors,
-// then [nr][SBT] right-hand side, then [nr][SBT] stage-0 copy of C
-template <int DL, int SBT>
-__device__ void solve_block(const SolveArgs &a, int var, int par, int col0, cplx *C, cplx *Cs,
+// rows of one thread in a block of SOLVE_THREADS: r = tid / SBT + t * (SOLVE_THREADS / SBT),
+// t < VR (a chain has at most VR * SOLVE_THREADS / SBT rows)
+constexpr int SOLVE_THREADS = 256;
+constexpr int SOLVE_VR_MAX = 4;
+// factor values in flight per thread and batch (16-byte loads)
+constexpr int SOLVE_FBAT = 8;
+
+// One (field, parity) of SBT consecutive columns.  The factors of the block
+// are one contiguous image [nr][W][SBT] (laid out so at init) copied to LDS
+// with 16-byte loads; every thread owns the column tid % SBT, so its rows are
+// tid / SBT + t * (SOLVE_THREADS / SBT).  lds: [nr][W][SBT] factors, then
+// [nr][SBT] right-hand side, then [nr][SBT] stage-0 copy of C.
+template <int DL, int SBT, int VR>
+__device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx *C, cplx *Cs,
                             double *lds)
 {
     constexpr int W = 2 * DL + 1;
+    constexpr int RSTEP = SOLVE_THREADS / SBT;
     const int Mx = a.Mx, nR = a.dimR, nQ = a.dimQ;
     const int N = var == 0 ? nQ : nR;
-    const int nr = (N - par + 1) / 2;
-    const int T = blockDim.x, tid = threadIdx.x;
+    const int nr = (N - par + 1) / 2, nrmax = (N + 1) / 2;
+    const int tid = threadIdx.x;
     double *lf = lds;
     double2 *vec = (double2 *) (lds + (size_t) nr * W * SBT);
     double2 *raw = vec + nr * SBT;
-    const double *Lg = var == 0 ? a.La : a.Lb;
     const cplx *F = var == 0 ? a.Fa : a.Fb;
     const cplx *F2 = var == 0 ? a.Fa2 : a.Fb2;
     const double ce0 = a.ce0[var], cn = a.cn[var];
 
-    // stage the factors
-    const int nfac = nr * W * SBT;
-#pragma unroll 4
-    for (int e = tid; e < nfac; e += T) {
-        const int cc = e % SBT, q = e / SBT, r = q / W, j = q - r * W;
-        const ColumnOf co = column_of(a, col0 + cc);
-        lf[e] = co.chain ? Lg[(((size_t) co.z * N + par + 2 * r) * W + j) * Mx + co.x] : 0.0;
+    const int cc = tid % SBT, rt = tid / SBT;
+    const ColumnOf co = column_of(a, blk * SBT + cc);
+    const double kx = a.Kx[co.x], kz = a.Kz[co.z], k2 = kz * kz + kx * kx;
+    const size_t cbase = (((size_t) co.z * 2 + var) * nR + par) * Mx + co.x;   // C row par
+    const size_t fbase = ((size_t) co.z * N + par) * Mx + co.x;                 // F row par
+    const size_t rs = (size_t) 2 * Mx;                                          // next chain row
+
+    // right-hand side rows and the next stage's projection, loads first
+    double2 cv[VR], fv[VR], f2v[VR];
+#pragma unroll
+    for (int t = 0; t < VR; ++t) {
+        const int r = min(rt + t * RSTEP, nr - 1);
+        cv[t] = C[cbase + r * rs];
+        fv[t] = F[fbase + r * rs];
+        f2v[t] = a.has_next ? F2[fbase + r * rs] : make_double2(0.0, 0.0);
     }
-    // stage the carried right-hand side (plus the projection after stage 0)
-    const int nvec = nr * SBT;
-#pragma unroll 2
-    for (int e = tid; e < nvec; e += T) {
-        const int cc = e % SBT, r = e / SBT, i = par + 2 * r;
-        const ColumnOf co = column_of(a, col0 + cc);
-        if (!co.chain) continue;
-        double2 cv = C[(((size_t) co.z * 2 + var) * nR + i) * Mx + co.x];
+    // the factor image: contiguous, 16-byte loads, batched
+    {
+        const double2 *src = (const double2 *) (var == 0 ? a.La : a.Lb) +
+                             ((size_t) blk * 2 + par) * nrmax * W * SBT / 2;
+        double2 *dst = (double2 *) lf;
+        const int n2 = nr * W * SBT / 2;
+        for (int b0 = 0; b0 < n2; b0 += SOLVE_FBAT * SOLVE_THREADS) {
+            double2 t[SOLVE_FBAT];
+#pragma unroll
+            for (int i = 0; i < SOLVE_FBAT; ++i) t[i] = src[min(b0 + tid + i * SOLVE_THREADS, n2 - 1)];
+#pragma unroll
+            for (int i = 0; i < SOLVE_FBAT; ++i)
+                if (b0 + tid + i * SOLVE_THREADS < n2) dst[b0 + tid + i * SOLVE_THREADS] = t[i];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < VR; ++t) {
+        const int r = rt + t * RSTEP;
+        if (r >= nr || !co.chain) continue;
         if (a.k == 0) {
-            raw[e] = cv;
+            raw[r * SBT + cc] = cv[t];
         } else {
-            double2 f = F[((size_t) co.z * N + i) * Mx + co.x];
-            cv.x += a.dtc * f.x;
-            cv.y += a.dtc * f.y;
-            vec[e] = cv;
+            double2 v = cv[t];
+            v.x += a.dtc * fv[t].x;
+            v.y += a.dtc * fv[t].y;
+            vec[r * SBT + cc] = v;
         }
     }
     __syncthreads();
     if (a.k == 0) {
-        for (int e = tid; e < nvec; e += T) {
-            const int cc = e % SBT, r = e / SBT, i = par + 2 * r;
-            const ColumnOf co = column_of(a, col0 + cc);
-            if (!co.chain) continue;
-            const double kx = a.Kx[co.x], kz = a.Kz[co.z], k2 = kz * kz + kx * kx;
+        // E C + dt c_0 F with the stage-0 explicit operator
+#pragma unroll
+        for (int t = 0; t < VR; ++t) {
+            const int r = rt + t * RSTEP, i = par + 2 * r;
+            if (r >= nr || !co.chain) continue;
             double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
             for (int m = -DL; m <= DL; ++m)
@@ -785,26 +905,28 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int col0, cplx
                     acc.x += ev * y.x;
                     acc.y += ev * y.y;
                 }
-            double2 f = F[((size_t) co.z * N + i) * Mx + co.x];
-            acc.x += a.dtc * f.x;
-            acc.y += a.dtc * f.y;
-            vec[e] = acc;
+            acc.x += a.dtc * fv[t].x;
+            acc.y += a.dtc * fv[t].y;
+            vec[r * SBT + cc] = acc;
         }
         __syncthreads();
     }
-    if (tid < SBT && column_of(a, col0 + tid).chain)
-        chain_subst<DL>(lf + tid, W * SBT, SBT, vec + tid, SBT, nr);
+    // The chains run in one wave of the block; blocks resident on one CU
+    // differ in blockIdx.y, so taking wave blockIdx.y % 4 spreads their chain
+    // waves over the four SIMDs instead of stacking them on the first.
+    const int cw = blockIdx.y & 3, cl = tid & 63;
+    if ((tid >> 6) == cw && cl < SBT && co.chain)
+        chain_subst<DL>(lf + cl, W * SBT, SBT, vec + cl, SBT, nr);
     __syncthreads();
     // write the solution and the next stage's right-hand side
-    for (int e = tid; e < nvec; e += T) {
-        const int cc = e % SBT, r = e / SBT, i = par + 2 * r;
-        const ColumnOf co = column_of(a, col0 + cc);
-        if (!co.chain) continue;
-        const size_t gi = (((size_t) co.z * 2 + var) * nR + i) * Mx + co.x;
-        const double2 sol = vec[e];
+#pragma unroll
+    for (int t = 0; t < VR; ++t) {
+        const int r = rt + t * RSTEP, i = par + 2 * r;
+        if (r >= nr || !co.chain) continue;
+        const size_t gi = cbase + r * rs;
+        const double2 sol = vec[r * SBT + cc];
         Cs[gi] = sol;
         if (a.has_next) {
-            const double kx = a.Kx[co.x], kz = a.Kz[co.z], k2 = kz * kz + kx * kx;
             double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
             for (int m = -DL; m <= DL; ++m)
@@ -814,9 +936,8 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int col0, cplx
                     acc.x += ev * y.x;
                     acc.y += ev * y.y;
                 }
-            double2 f = F2[((size_t) co.z * N + i) * Mx + co.x];
-            acc.x += a.dtd * f.x;
-            acc.y += a.dtd * f.y;
+            acc.x += a.dtd * f2v[t].x;
+            acc.y += a.dtd * f2v[t].y;
             C[gi] = acc;
         } else {
             C[gi] = sol;
@@ -824,11 +945,10 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int col0, cplx
     }
     // rows of v beyond dimQ are carried unchanged; the Nyquist row is zeroed
     const int nrest = (nR - par + 1) / 2;
-    for (int e = tid; e < nrest * SBT; e += T) {
-        const int cc = e % SBT, r = e / SBT, i = par + 2 * r;
-        const ColumnOf co = column_of(a, col0 + cc);
-        if (!co.valid || (co.z == 0 && co.x == 0)) continue;
-        const size_t gi = (((size_t) co.z * 2 + var) * nR + i) * Mx + co.x;
+    if (!co.valid || (co.z == 0 && co.x == 0)) return;
+    for (int r = rt; r < nrest; r += RSTEP) {
+        const int i = par + 2 * r;
+        const size_t gi = cbase + r * rs;
         if (co.z == a.Nz / 2)
             C[gi] = make_double2(0.0, 0.0);
         else if (var == 0 && i >= nQ)
@@ -850,34 +970,57 @@ __device__ double2 mean_mult(const SolveArgs &a, const double2 *v, int i, double
     return t;
 }
 
-// the mean mode: alpha (streamwise mean) and beta (spanwise mean) columns
+// the mean mode: alpha (streamwise mean) and beta (spanwise mean) columns.
+// Every global load (state, both forcings, factors) is issued up front; the
+// two explicit products of a pass run together.  Rows per thread:
+// i = tid + MEAN_ROWS_STEP t, t < MEAN_ROWS (dimR <= 512).
+constexpr int MEAN_ROWS = 2;
 __device__ void solve_mean(const SolveArgs &a, cplx *C, cplx *Cs, double *lds)
 {
-    const int nR = a.dimR, Mx = a.Mx, T = blockDim.x, tid = threadIdx.x;
-    double2 *al = (double2 *) lds, *be = al + nR, *tmp = be + nR;
-    double *lm = (double *) (tmp + nR), *red = lm + 5 * nR;
+    const int nR = a.dimR, Mx = a.Mx, T = SOLVE_THREADS, tid = threadIdx.x;
+    double2 *al = (double2 *) lds, *be = al + nR, *ta = be + nR, *tb = ta + nR;
+    double *lm = (double *) (tb + nR), *red = lm + 5 * nR;
     cplx *Ca = C, *Cb = C + (size_t) nR * Mx;
-    for (int i = tid; i < nR; i += T) {
-        al[i] = Ca[(size_t) i * Mx];
-        be[i] = Cb[(size_t) i * Mx];
+    const cplx *Fm = a.Fb, *Fm2 = a.Fb2;   // column (0, 0): (fa, fb)
+    double2 fr[MEAN_ROWS], f2r[MEAN_ROWS];
+#pragma unroll
+    for (int t = 0; t < MEAN_ROWS; ++t) {
+        const int i = tid + T * t;
+        if (i < nR) {
+            al[i] = Ca[(size_t) i * Mx];
+            be[i] = Cb[(size_t) i * Mx];
+            fr[t] = Fm[(size_t) i * Mx];
+            f2r[t] = a.has_next ? Fm2[(size_t) i * Mx] : make_double2(0.0, 0.0);
+        }
     }
     for (int e = tid; e < 5 * nR; e += T) lm[e] = a.Lm[e];
     __syncthreads();
     if (a.k == 0) {
-        for (int i = tid; i < nR; i += T) tmp[i] = mean_mult(a, al, i, a.ce0[1]);
+#pragma unroll
+        for (int t = 0; t < MEAN_ROWS; ++t) {
+            const int i = tid + T * t;
+            if (i < nR) {
+                ta[i] = mean_mult(a, al, i, a.ce0[1]);
+                tb[i] = mean_mult(a, be, i, a.ce0[1]);
+            }
+        }
         __syncthreads();
-        for (int i = tid; i < nR; i += T) al[i] = tmp[i];
-        __syncthreads();
-        for (int i = tid; i < nR; i += T) tmp[i] = mean_mult(a, be, i, a.ce0[1]);
-        __syncthreads();
-        for (int i = tid; i < nR; i += T) be[i] = tmp[i];
-        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < MEAN_ROWS; ++t) {
+            const int i = tid + T * t;
+            if (i < nR) {
+                al[i] = ta[i];
+                be[i] = tb[i];
+            }
+        }
     }
-    const cplx *Fm = a.Fb, *Fm2 = a.Fb2;   // column (0, 0): (fa, fb)
-    for (int i = tid; i < nR; i += T) {
-        double2 f = Fm[(size_t) i * Mx];
-        al[i].x += a.dtc * f.x;
-        be[i].x += a.dtc * f.y;
+#pragma unroll
+    for (int t = 0; t < MEAN_ROWS; ++t) {
+        const int i = tid + T * t;
+        if (i < nR) {
+            al[i].x += a.dtc * fr[t].x;
+            be[i].x += a.dtc * fr[t].y;
+        }
     }
     __syncthreads();
     const int n0 = (nR + 1) / 2, n1 = nR / 2;
@@ -902,12 +1045,15 @@ __device__ void solve_mean(const SolveArgs &a, cplx *C, cplx *Cs, double *lds)
         Cs[(size_t) (nR + i) * Mx] = be[i];
     }
     __syncthreads();
-    for (int i = tid; i < nR; i += T) {
+#pragma unroll
+    for (int t = 0; t < MEAN_ROWS; ++t) {
+        const int i = tid + T * t;
+        if (i >= nR) continue;
         if (a.has_next) {
-            double2 f = Fm2[(size_t) i * Mx];
-            double2 ta = mean_mult(a, al, i, a.cn[1]), tb = mean_mult(a, be, i, a.cn[1]);
-            Ca[(size_t) i * Mx] = make_double2(ta.x + a.dtd * f.x, ta.y + a.dtd * 0.0);
-            Cb[(size_t) i * Mx] = make_double2(tb.x + a.dtd * f.y, tb.y + a.dtd * 0.0);
+            const double2 f = f2r[t];
+            double2 xa = mean_mult(a, al, i, a.cn[1]), xb = mean_mult(a, be, i, a.cn[1]);
+            Ca[(size_t) i * Mx] = make_double2(xa.x + a.dtd * f.x, xa.y + a.dtd * 0.0);
+            Cb[(size_t) i * Mx] = make_double2(xb.x + a.dtd * f.y, xb.y + a.dtd * 0.0);
         } else {
             Ca[(size_t) i * Mx] = al[i];
             Cb[(size_t) i * Mx] = be[i];
@@ -915,23 +1061,42 @@ __device__ void solve_mean(const SolveArgs &a, cplx *C, cplx *Cs, double *lds)
     }
 }
 
-// grid (ceil(Nz*Mx/SBT) + 1, 4): blockIdx.y = field*2 + parity; the last
-// x-block with blockIdx.y == 0 is the mean mode.  Block of 256 threads.
+// The Nyquist row kz = Nz/2 carries no data: a block whose columns all lie
+// in it only clears its C rows.
 template <int SBT>
-__global__ __launch_bounds__(256) void k_solve(SolveArgs a, cplx *C, cplx *Cs)
+__device__ void nyquist_clear(const SolveArgs &a, int var, int par, int blk, cplx *C)
+{
+    const int nR = a.dimR, Mx = a.Mx, nrest = (nR - par + 1) / 2;
+    const int x0 = blk * SBT - (a.Nz / 2) * Mx;
+    const size_t base = (((size_t) (a.Nz / 2) * 2 + var) * nR + par) * Mx + x0;
+    for (int e = threadIdx.x; e < nrest * SBT; e += SOLVE_THREADS)
+        C[base + (size_t) (e / SBT) * 2 * Mx + e % SBT] = make_double2(0.0, 0.0);
+}
+
+// grid (nblk, 4), or (nblk + 1, 4) when the mean mode needs a block of its
+// own: blockIdx.y = field*2 + parity.  With Mx % SBT == 0 the blocks of the
+// Nyquist row hold no chains, and the first of them with blockIdx.y == 0 runs
+// the mean mode (so the grid is exactly the column blocks); otherwise the
+// extra x-block nblk with blockIdx.y == 0 does.
+template <int SBT, int VR>
+__global__ __launch_bounds__(SOLVE_THREADS, 4) void k_solve(SolveArgs a, cplx *C, cplx *Cs)
 {
     extern __shared__ double lds_solve[];
     if (*a.abort_flag) return;
-    if (blockIdx.x == gridDim.x - 1) {
+    const int var = blockIdx.y >> 1, par = blockIdx.y & 1, blk = blockIdx.x;
+    if (blk == a.nblk) {
         if (blockIdx.y == 0) solve_mean(a, C, Cs, lds_solve);
         return;
     }
-    const int var = blockIdx.y >> 1, par = blockIdx.y & 1;
-    const int col0 = blockIdx.x * SBT;
+    if (a.nyq_mean && blk * SBT / a.Mx == a.Nz / 2) {
+        if (blockIdx.y == 0 && blk * SBT == (a.Nz / 2) * a.Mx) solve_mean(a, C, Cs, lds_solve);
+        nyquist_clear<SBT>(a, var, par, blk, C);
+        return;
+    }
     if (var == 0)
-        solve_block<4, SBT>(a, 0, par, col0, C, Cs, lds_solve);
+        solve_block<4, SBT, VR>(a, 0, par, blk, C, Cs, lds_solve);
     else
-        solve_block<2, SBT>(a, 1, par, col0, C, Cs, lds_solve);
+        solve_block<2, SBT, VR>(a, 1, par, blk, C, Cs, lds_solve);
 }
 
 // coefficient tables for the stages
@@ -946,15 +1111,20 @@ size_t solve_lds_bytes(int dimR, int sbt)
 {
     const size_t nr = (size_t) (dimR + 1) / 2;
     const size_t blk = nr * sbt * (9 * 8 + 16 + 16);
-    const size_t mean = (size_t) dimR * (3 * 16 + 5 * 8) + 256 * 8;
+    const size_t mean = (size_t) dimR * (4 * 16 + 5 * 8) + SOLVE_THREADS * 8;
     return blk > mean ? blk : mean;
 }
 
+// Columns per solve block: 8, 4 or 2 (an even count keeps every block's
+// factor image a whole number of 16-byte pairs), within 64 KB of LDS and with
+// a row slot for every chain row; 0 when none fits.
 int solve_columns_per_block(int dimR)
 {
-    for (int sbt = 8; sbt > 1; sbt >>= 1)
-        if (solve_lds_bytes(dimR, sbt) <= 48 * 1024) return sbt;
-    return 1;
+    const int nr = (dimR + 1) / 2;
+    for (int sbt = 8; sbt >= 2; sbt >>= 1)
+        if (solve_lds_bytes(dimR, sbt) <= 64 * 1024 && nr <= SOLVE_VR_MAX * (SOLVE_THREADS / sbt))
+            return sbt;
+    return 0;
 }
 
 void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cplx *Fa2,
@@ -988,20 +1158,21 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
     double bytes = (double) (d.Nz - 1) * d.Mx *
                    (16.0 * (2.0 * d.dimR) * 3.0 + 16.0 * (d.dimQ + d.dimR) * (k != 2 ? 2.0 : 1.0) +
                     8.0 * (9.0 * d.dimQ + 5.0 * d.dimR));
-    const int sbt = solve_columns_per_block(d.dimR);
+    const int sbt = c.sbt;
     const int ncol = d.Nz * d.Mx;
     const size_t lds = solve_lds_bytes(d.dimR, sbt);
-    const dim3 grid((ncol + sbt - 1) / sbt + 1, 4);
+    a.nblk = (ncol + sbt - 1) / sbt;
+    a.nyq_mean = d.Mx % sbt == 0;
+    const dim3 grid(a.nblk + (a.nyq_mean ? 0 : 1), 4);
     const hipStream_t st = c.stream;
+    // row slots per thread: the chains have at most (dimR + 1) / 2 rows
+    const bool two = (d.dimR + 1) / 2 <= 2 * (SOLVE_THREADS / sbt);
+    typedef void (*SolveKernel)(SolveArgs, cplx *, cplx *);
+    const SolveKernel kern = sbt == 8 ? (two ? k_solve<8, 2> : k_solve<8, 4>)
+                           : sbt == 4 ? (two ? k_solve<4, 2> : k_solve<4, 4>)
+                                      : (two ? k_solve<2, 2> : k_solve<2, 4>);
     auto go = [=]() {
-        if (sbt == 8)
-            hipLaunchKernelGGL(k_solve<8>, grid, dim3(256), lds, st, a, Ccarry, Csol);
-        else if (sbt == 4)
-            hipLaunchKernelGGL(k_solve<4>, grid, dim3(256), lds, st, a, Ccarry, Csol);
-        else if (sbt == 2)
-            hipLaunchKernelGGL(k_solve<2>, grid, dim3(256), lds, st, a, Ccarry, Csol);
-        else
-            hipLaunchKernelGGL(k_solve<1>, grid, dim3(256), lds, st, a, Ccarry, Csol);
+        hipLaunchKernelGGL(kern, grid, dim3(SOLVE_THREADS), lds, st, a, Ccarry, Csol);
     };
     prof_begin(c, st, 6, bytes);
     go();

@@ -99,10 +99,15 @@ struct Context {
     //   fragF [nit][nksF][5][64]: Qpw, Qw, Qppw, Rw, Rpw at (y = 4s + l/16, i = 16t + l%16)
     double *dFragR = nullptr, *dFragF = nullptr;
     int nyt = 0, nksR = 0, nit = 0, nksF = 0;
-    // LU factors of the implicit operators, per (direction, stage):
-    //   alpha: [Nz][dimQ][9][Mx]   (j<4: L(i,i-2(4-j)); j=4: U diagonal; j>4: U)
-    //   beta:  [Nz][dimR][5][Mx]
+    // LU factors of the implicit operators, per (direction, stage), as the
+    // LDS images of the solve's blocks of sbt columns (col = z Mx + x):
+    //   alpha: [nblk][2 parities][(dimQ+1)/2 chain rows][9][sbt]
+    //   beta:  [nblk][2 parities][(dimR+1)/2 chain rows][5][sbt]
+    // (j<DL: L(i,i-2(DL-j)); j=DL: 1/U(i,i); j>DL: U(i,i+2(j-DL)); chain row
+    // r of parity p is matrix row p + 2r; zero outside the matrix and for the
+    // columns without a chain)
     double *dLa[2][3] = {{nullptr}}, *dLb[2][3] = {{nullptr}};
+    int sbt = 0;                                // columns per solve block
     // mean mode: factors [dimR][5] and flux-correction vector [dimR]
     double *dLm[2][3] = {{nullptr}}, *dAdd[2][3] = {{nullptr}};
 
@@ -165,6 +170,7 @@ void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb);
 void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cplx *Fa2,
                   const cplx *Fb2, double target);
 void launch_cfl_check(Context &c);
+int solve_columns_per_block(int dimR);
 int xmix_zblock(Context &c, int nf);
 void launch_axpy_c(Context &c, cplx *y, const cplx *x, double a, size_t n);
 void launch_diff_scale(Context &c, cplx *out, const cplx *a, const cplx *b, double s, size_t n);

@@ -280,14 +280,28 @@ int setup_host(Context &c)
     // omega:  Rs + re b dt (Rps + K2 Rs)
     // mean:   Rs + re b dt Rps
     std::vector<double> A9(9 * (size_t) nQ), A5(5 * (size_t) nR);
+    const int sbt = solve_columns_per_block(nR);
+    if (sbt <= 0) {
+        last_error() = "no solve block shape for dimR " + std::to_string(nR);
+        return -1;
+    }
+    c.sbt = sbt;
+    const size_t nblk = ((size_t) Nz * Mx + sbt - 1) / sbt;
+    const size_t rqa = (nQ + 1) / 2, rqb = (nR + 1) / 2;
+    // position of factor j of matrix row i of column col in the block images
+    auto image = [&](size_t rmax, int W, int col, int i, int j) {
+        const size_t b = col / sbt, cc = col % sbt, par = i & 1, r = i >> 1;
+        return (((b * 2 + par) * rmax + r) * W + j) * sbt + cc;
+    };
     for (int dir = 0; dir < 2; ++dir)
         for (int k = 0; k < 3; ++k) {
             const double bk = dir == 0 ? FWD_B[k] : ADJ_B[k];
             const double coef = d.re * bk * d.dt;
-            std::vector<double> La((size_t) Nz * nQ * 9 * Mx), Lb((size_t) Nz * nR * 5 * Mx);
+            std::vector<double> La(nblk * 2 * rqa * 9 * sbt, 0.0), Lb(nblk * 2 * rqb * 5 * sbt, 0.0);
             for (int z = 0; z < Nz; ++z)
                 for (int x = 0; x < Mx; ++x) {
                     if ((z == 0 && x == 0) || z == Nz / 2) continue;
+                    const int col = z * Mx + x;
                     double k2 = c.Kz[z] * c.Kz[z] + c.Kx[x] * c.Kx[x], s = k2 * k2;
                     for (int e = 0; e < 9 * nQ; ++e)
                         A9[e] = -(k2 * c.Qs[e] + c.Qps[e]) -
@@ -296,14 +310,14 @@ int setup_host(Context &c)
                     device_factors(A9.data(), nQ, 4);
                     for (int i = 0; i < nQ; ++i)
                         for (int j = 0; j < 9; ++j)
-                            La[(((size_t) z * nQ + i) * 9 + j) * Mx + x] = A9[(size_t) i * 9 + j];
+                            La[image(rqa, 9, col, i, j)] = A9[(size_t) i * 9 + j];
                     for (int e = 0; e < 5 * nR; ++e)
                         A5[e] = c.Rs[e] + coef * (c.Rps[e] + k2 * c.Rs[e]);
                     band_factor(A5.data(), nR, 2);
                     device_factors(A5.data(), nR, 2);
                     for (int i = 0; i < nR; ++i)
                         for (int j = 0; j < 5; ++j)
-                            Lb[(((size_t) z * nR + i) * 5 + j) * Mx + x] = A5[(size_t) i * 5 + j];
+                            Lb[image(rqb, 5, col, i, j)] = A5[(size_t) i * 5 + j];
                 }
             if (upload(&c.dLa[dir][k], La) || upload(&c.dLb[dir][k], Lb)) {
                 last_error() = "device allocation of factors failed";
