@@ -793,8 +793,12 @@ __device__ __forceinline__ void chain_backward(const double *f, int fr, int fj, 
     if (r >= 2) bwd_step<DL>(Y.u[0], Y.v[0], w, v, vr, r - 2);
 }
 
+// inlined at every call site: the pointers then stay LDS pointers (a call
+// would pass them as generic addresses, and every access would go through
+// the flat path)
 template <int DL>
-__device__ void chain_subst(const double *f, int fr, int fj, double2 *v, int vr, int nr)
+__device__ __forceinline__ void chain_subst(const double *f, int fr, int fj, double2 *v, int vr,
+                                            int nr)
 {
     if (nr <= 0) return;
     chain_forward<DL>(f, fr, fj, v, vr, nr);
@@ -845,6 +849,14 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx 
     const cplx *F2 = var == 0 ? a.Fa2 : a.Fb2;
     const double ce0 = a.ce0[var], cn = a.cn[var];
 
+    __shared__ int chain_wave[1];
+    if (tid == 0) chain_wave[0] = 0;
+    __syncthreads();
+    {
+        // HW_ID: SIMD in bits 5:4, workgroup slot in bits 19:16
+        const unsigned hw = __builtin_amdgcn_s_getreg(4 | (0 << 6) | (31 << 11));
+        if ((tid & 63) == 0 && ((hw >> 4) & 3) == ((hw >> 16) & 3)) chain_wave[0] = tid >> 6;
+    }
     const int cc = tid % SBT, rt = tid / SBT;
     const ColumnOf co = column_of(a, blk * SBT + cc);
     const double kx = a.Kx[co.x], kz = a.Kz[co.z], k2 = kz * kz + kx * kx;
@@ -911,11 +923,11 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx 
         }
         __syncthreads();
     }
-    // The chains run in one wave of the block; blocks resident on one CU
-    // differ in blockIdx.y, so taking wave blockIdx.y % 4 spreads their chain
-    // waves over the four SIMDs instead of stacking them on the first.
-    const int cw = blockIdx.y & 3, cl = tid & 63;
-    if ((tid >> 6) == cw && cl < SBT && co.chain)
+    // The chains run in one wave of the block: the wave whose SIMD equals the
+    // block's workgroup slot on the CU (mod 4), so that the blocks resident on
+    // one CU run their chains on different SIMDs (wave 0 when no wave matches).
+    const int cl = tid & 63;
+    if ((tid >> 6) == chain_wave[0] && cl < SBT && co.chain)
         chain_subst<DL>(lf + cl, W * SBT, SBT, vec + cl, SBT, nr);
     __syncthreads();
     // write the solution and the next stage's right-hand side
