@@ -624,6 +624,7 @@ struct SolveArgs {
     int Nz, Mx, dimR, dimQ, k, has_next;
     int nblk;                          // column blocks of SBT columns
     int nyq_mean;                      // the mean mode runs in the Nyquist blocks' first (1) or in an extra block (0)
+    int mirror;                        // factor images of rows z > Nz/2 are those of row Nz - z
     double dtc, dtd;                   // dt c_k, dt d_{k+1}
     double ce0[2], cn[2];              // explicit-operator coefficients per field (stage 0, next)
     double target;                     // mean-mode flux target
@@ -875,8 +876,14 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx 
     }
     // the factor image: contiguous, 16-byte loads, batched
     {
+        // factor image: that of the mirror row Nz - z for z > Nz/2
+        int fb = blk;
+        if (a.mirror) {
+            const int z = blk * SBT / Mx;
+            if (z > a.Nz / 2) fb = blk - (2 * z - a.Nz) * (Mx / SBT);
+        }
         const double2 *src = (const double2 *) (var == 0 ? a.La : a.Lb) +
-                             ((size_t) blk * 2 + par) * nrmax * W * SBT / 2;
+                             ((size_t) fb * 2 + par) * nrmax * W * SBT / 2;
         double2 *dst = (double2 *) lf;
         const int n2 = nr * W * SBT / 2;
         for (int b0 = 0; b0 < n2; b0 += SOLVE_FBAT * SOLVE_THREADS) {
@@ -1174,6 +1181,7 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
     const int ncol = d.Nz * d.Mx;
     const size_t lds = solve_lds_bytes(d.dimR, sbt);
     a.nblk = (ncol + sbt - 1) / sbt;
+    a.mirror = c.factor_mirror;
     a.nyq_mean = d.Mx % sbt == 0;
     const dim3 grid(a.nblk + (a.nyq_mean ? 0 : 1), 4);
     const hipStream_t st = c.stream;

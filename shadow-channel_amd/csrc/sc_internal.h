@@ -108,6 +108,7 @@ struct Context {
     // columns without a chain)
     double *dLa[2][3] = {{nullptr}}, *dLb[2][3] = {{nullptr}};
     int sbt = 0;                                // columns per solve block
+    bool factor_mirror = false;                 // images stored for kz rows <= Nz/2 only
     // mean mode: factors [dimR][5] and flux-correction vector [dimR]
     double *dLm[2][3] = {{nullptr}}, *dAdd[2][3] = {{nullptr}};
 

@@ -286,7 +286,14 @@ int setup_host(Context &c)
         return -1;
     }
     c.sbt = sbt;
-    const size_t nblk = ((size_t) Nz * Mx + sbt - 1) / sbt;
+    // Columns kz and Nz - kz have the same k^2 (Kz[Nz - z] = -Kz[z] exactly),
+    // so the same factors: when the blocks follow the kz rows (Mx % sbt ==
+    // 0), only the rows z <= Nz/2 are stored and the solve reads the mirror
+    // row's image for z > Nz/2.
+    const bool mirror = Mx % sbt == 0;
+    c.factor_mirror = mirror;
+    const int zrows = mirror ? Nz / 2 + 1 : Nz;
+    const size_t nblk = ((size_t) zrows * Mx + sbt - 1) / sbt;
     const size_t rqa = (nQ + 1) / 2, rqb = (nR + 1) / 2;
     // position of factor j of matrix row i of column col in the block images
     auto image = [&](size_t rmax, int W, int col, int i, int j) {
@@ -298,7 +305,7 @@ int setup_host(Context &c)
             const double bk = dir == 0 ? FWD_B[k] : ADJ_B[k];
             const double coef = d.re * bk * d.dt;
             std::vector<double> La(nblk * 2 * rqa * 9 * sbt, 0.0), Lb(nblk * 2 * rqb * 5 * sbt, 0.0);
-            for (int z = 0; z < Nz; ++z)
+            for (int z = 0; z < zrows; ++z)
                 for (int x = 0; x < Mx; ++x) {
                     if ((z == 0 && x == 0) || z == Nz / 2) continue;
                     const int col = z * Mx + x;
