@@ -168,12 +168,12 @@ static const cplx *wcached(Context &c, long j)
 
 // H = u x omega into the H slots of Ub (channel/pass1.c, state side).  With
 // wdst the transformed fields are kept there (the cache slot of this state).
-static void pass1_state(Context &c, cplx *Ub, int check_cfl, cplx *wdst = nullptr)
+static void pass1_state(Context &c, cplx *Ub, int check_cfl, cplx *wdst = nullptr, int slot = 0)
 {
     cplx *w = wdst ? wdst : c.W1;
     launch_zinv(c, velocity_and_vorticity(Ub), w);
     launch_xmix(c, XM_PRIMAL, w, c.W2, check_cfl, nullptr, nullptr);
-    if (check_cfl) launch_cfl_check(c);
+    if (check_cfl) launch_cfl_check(c, slot);
     launch_zfwd(c, c.W2, 3, Ub, ZF_USLOTS);
 }
 
@@ -196,7 +196,7 @@ static void pass1_tangent(Context &c, const cplx *Ub, cplx *IUb, int check_cfl,
         launch_zinv(c, a, c.W1);
         launch_xmix(c, XM_TANGENT, c.W1, c.W2, check_cfl, nullptr, nullptr);
     }
-    if (check_cfl) launch_cfl_check(c);
+    if (check_cfl) launch_cfl_check(c, 0);
     launch_zfwd(c, c.W2, 3, IUb, ZF_USLOTS);
 }
 
@@ -487,7 +487,7 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
         {(void **) &c.cflmax, sizeof(double) * (d.qpts + 1)},
         {(void **) &c.dotpart, sizeof(double) * 2 * (size_t) d.qpts * d.Pz},
         {(void **) &c.scal, sizeof(double) * 8},
-        {(void **) &c.abort_flag, sizeof(int)},
+        {(void **) &c.abort_flag, 2 * sizeof(int)},
     };
     for (auto &e : al) {
         if (hipMalloc(e.p, e.n) != hipSuccess || hipMemset(*e.p, 0, e.n) != hipSuccess) {
@@ -547,7 +547,9 @@ int c_primal(int ru_steps, sc_complex *C_given, int Nz_dup, int Nvar_dup, int Ny
         SC_HIP(hipMemcpy(c.C, Ch.data(), cb, hipMemcpyHostToDevice));
         SC_HIP(hipMemcpy(c.U, Uh.data(), sizeof(cplx) * c.usize, hipMemcpyHostToDevice));
     }
-    SC_HIP(hipMemsetAsync(c.MC, 0, cb * (3 * (size_t) d.nsteps + 1), c.stream));
+    // Every stored state is written by a completed run; after a CFL stop the
+    // states from the stopped step on are cleared (below), so the trajectory
+    // reads as the reference's zeroed array (channel/channel.c:426-432).
     SC_HIP(hipMemcpyAsync(c.MC, c.C, cb, hipMemcpyDeviceToDevice, c.stream));
     SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
     // each CFL check clears the maxima it read; a run starts from zero
@@ -562,7 +564,7 @@ int c_primal(int ru_steps, sc_complex *C_given, int Nz_dup, int Nvar_dup, int Ny
             // laminar start or a run-up (MC[0] is the state before either)
             const long j = (long) (n - ru_steps) * 3 + k;
             cplx *w = (j > 0 || (j == 0 && given && ru_steps == 0)) ? wslot(c, j) : nullptr;
-            pass1_state(c, c.U, k == 0, w);
+            pass1_state(c, c.U, k == 0, w, (int) std::max<long>(1, (long) (n - ru_steps) * 3 + 1));
             if (w) c.wvalid[j] = 1;
             long count = (long) (n - ru_steps) * 3 + k + 1;
             cplx *Cs = count >= 0 ? mc(c, count) : c.Csol;
@@ -578,8 +580,16 @@ int c_primal(int ru_steps, sc_complex *C_given, int Nz_dup, int Nvar_dup, int Ny
     }
     int rc = stream_sync(c);
     if (rc) return rc;
-    // after an abort the queued launches did nothing: no cached field is valid
-    if (check_abort(c, "Pass1")) std::fill(c.wvalid.begin(), c.wvalid.end(), 0);
+    // after an abort the queued launches did nothing: no cached field is
+    // valid, and the states from the stopped step on were not written
+    if (check_abort(c, "Pass1")) {
+        std::fill(c.wvalid.begin(), c.wvalid.end(), 0);
+        int slot = 1;
+        SC_HIP(hipMemcpy(&slot, c.abort_flag + 1, sizeof(int), hipMemcpyDeviceToHost));
+        const long last = 3L * d.nsteps;
+        if (slot >= 1 && slot <= last)
+            SC_HIP(hipMemset(mc(c, slot), 0, cb * (size_t) (last - slot + 1)));
+    }
     if (given) SC_HIP(hipMemcpy(C_given, c.C, cb, hipMemcpyDeviceToHost));
     return SC_OK;
 }

@@ -1262,12 +1262,13 @@ __global__ void k_ddt_apply(cplx *vout, const cplx *vin, const cplx *dC, const d
     }
 }
 
-// The first plane whose CFL number reaches the limit sets abort_flag =
-// 1 + plane and leaves its maximum in cflmax[qpts]; the per-plane maxima are
-// cleared for the next check.  One block of CFL_THREADS.
+// The first plane whose CFL number reaches the limit sets abort_flag[0] =
+// 1 + plane, abort_flag[1] = slot (the first stored state the stopped step
+// would have written) and leaves its maximum in cflmax[qpts]; the per-plane
+// maxima are cleared for the next check.  One block of CFL_THREADS.
 constexpr int CFL_THREADS = 256;
 __global__ __launch_bounds__(CFL_THREADS) void k_cfl_check(double *cflmax, int qpts, double dt,
-                                                           int *abort_flag)
+                                                           int *abort_flag, int slot)
 {
     __shared__ int first;
     if (threadIdx.x == 0) first = qpts;
@@ -1280,7 +1281,8 @@ __global__ __launch_bounds__(CFL_THREADS) void k_cfl_check(double *cflmax, int q
     const int y0 = first;
     if (y0 < qpts && threadIdx.x == 0) {
         cflmax[qpts] = cflmax[y0];
-        *abort_flag = 1 + y0;
+        abort_flag[1] = slot;
+        abort_flag[0] = 1 + y0;
     }
     __syncthreads();
     for (int y = threadIdx.x; y < qpts; y += CFL_THREADS) cflmax[y] = 0.0;
@@ -1310,10 +1312,10 @@ void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *dC, s
                        c.scal, n, acc, a);
 }
 
-void launch_cfl_check(Context &c)
+void launch_cfl_check(Context &c, int slot)
 {
     hipLaunchKernelGGL(k_cfl_check, dim3(1), dim3(CFL_THREADS), 0, c.stream, c.cflmax, c.d.qpts,
-                       c.d.dt, c.abort_flag);
+                       c.d.dt, c.abort_flag, slot);
 }
 
 }  // namespace sc

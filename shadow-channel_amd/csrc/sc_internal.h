@@ -134,7 +134,7 @@ struct Context {
     double *cflmax = nullptr;
     double *dotpart = nullptr;                  // partial sums
     double *scal = nullptr;                     // [8] device scalars
-    int *abort_flag = nullptr;
+    int *abort_flag = nullptr;                  // [0] CFL stop (1 + plane), [1] first unwritten slot
     FftPlan pz, px;
     size_t csize = 0, usize = 0;                // element counts
     int status = 0;                             // -1 after a CFL failure
@@ -170,7 +170,8 @@ void launch_force(Context &c, const cplx *Usrc, int adj);
 void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb);
 void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cplx *Fa2,
                   const cplx *Fb2, double target);
-void launch_cfl_check(Context &c);
+// slot: first stored state of the step being checked (kept when it stops)
+void launch_cfl_check(Context &c, int slot);
 int solve_columns_per_block(int dimR);
 int xmix_zblock(Context &c, int nf);
 void launch_axpy_c(Context &c, cplx *y, const cplx *x, double a, size_t n);
