@@ -77,6 +77,8 @@ struct RebuildArgs {
     int Nz, Mx, qpts, dimR, dimQ, nks, nyt, nseg;
     double ax, az;                      // 2 pi / Lx, 2 pi / Lz
     const int *abort_flag;
+    unsigned *queue;                    // [8][nyt][2] task and exit counters, zero between launches
+    int bpq;                            // blocks per (XCD, y-tile) queue
 };
 
 // U[z][0..4][y][x] from C[z][0..1][j][x]; 1-D grid over the (kz row, y-tile)
@@ -280,35 +282,40 @@ __device__ __forceinline__ void rebuild_store(const RebuildArgs &a, int zrow, in
         const int x = nt * 8 + 2 * kq + h;
         if (x >= Mx || (zrow == 0 && x == 0)) continue;
         const size_t o = (size_t) y * Mx + x;
-        const double kx = wave_kx(x, a.ax), k2 = kz * kz + kx * kx;
+        // one reciprocal per column instead of eight divisions
+        const double kx = wave_kx(x, a.ax), r2 = 1.0 / (kz * kz + kx * kx);
         const double2 v = make_double2(acc[0][2 * h], acc[0][2 * h + 1]);
         const double2 f = make_double2(-acc[1][2 * h], -acc[1][2 * h + 1]);
         const double2 q2 = make_double2(acc[2][2 * h], acc[2][2 * h + 1]);
         const double2 g = make_double2(acc[3][2 * h], acc[3][2 * h + 1]);
         const double2 gp = make_double2(acc[4][2 * h], acc[4][2 * h + 1]);
         Uz[o + 1 * ss] = v;
-        Uz[o + 0 * ss] = make_double2((kx * f.y + kz * g.y) / k2, -(kx * f.x + kz * g.x) / k2);
-        Uz[o + 2 * ss] = make_double2((-kx * g.y + kz * f.y) / k2, (kx * g.x - kz * f.x) / k2);
-        Uz[o + 3 * ss] = make_double2((-kx * q2.y + kz * gp.y) / k2, (kx * q2.x - kz * gp.x) / k2);
-        Uz[o + 4 * ss] = make_double2(-(kx * gp.y + kz * q2.y) / k2, (kx * gp.x + kz * q2.x) / k2);
+        Uz[o + 0 * ss] = make_double2((kx * f.y + kz * g.y) * r2, -(kx * f.x + kz * g.x) * r2);
+        Uz[o + 2 * ss] = make_double2((-kx * g.y + kz * f.y) * r2, (kx * g.x - kz * f.x) * r2);
+        Uz[o + 3 * ss] = make_double2((-kx * q2.y + kz * gp.y) * r2, (kx * q2.x - kz * gp.x) * r2);
+        Uz[o + 4 * ss] = make_double2(-(kx * gp.y + kz * q2.y) * r2, (kx * gp.x + kz * q2.x) * r2);
     }
 }
 
-// Row-persistent rebuild for dimR <= 4*NKS: a block owns one y-tile, stages
-// its whole basis (NKS k-steps) in LDS once, and runs the kz rows of one
-// segment.  Each wave walks its (kz row, n-tile) tasks independently (load the
-// 2 NKS coefficient fragments, 5 NKS products, store); no barrier after the
-// staging.  Two blocks share a CU, so one wave's loads overlap the other's
-// products.  1-D grid over (segment, y-tile), y-tile fastest, XCD-aware.
+// Rebuild for dimR <= 4*NKS with dynamic task queues: a block owns one
+// y-tile and stages its whole basis (NKS k-steps) in LDS once; its waves then
+// take (kz row, n-tile) tasks (load the 2 NKS coefficient fragments, 5 NKS
+// products, store) from the queue of their (XCD, y-tile) until it is empty,
+// so the products spread evenly over the SIMDs.  Blocks are dispatched
+// round-robin over the 8 XCDs (block b on XCD b % 8), and the queue of XCD x
+// holds the kz rows [x Nz/8, (x+1) Nz/8): the coefficient rows of an XCD's
+// tasks stay in its L2.  A queue is a task counter and an exit counter; the
+// last wave of the queue to exit clears both, so they are zero between
+// launches.  Two blocks share a CU.  Grid: 8 * nyt * bpq blocks.
 template <int NKS>
 __global__ __launch_bounds__(256, 2) void k_rebuild_rows(RebuildArgs a, const cplx *C, cplx *U)
 {
     __shared__ double bs[NKS * 5 * 64];
     if (*a.abort_flag) return;
-    const int tile = xcd_tile(a.nyt * a.nseg);
-    if (tile < 0) return;
-    const int seg = tile / a.nyt, yt = tile - seg * a.nyt;
-    const int z0 = seg * a.Nz / a.nseg, z1 = (seg + 1) * a.Nz / a.nseg;
+    const int xcd = blockIdx.x & 7, ib = blockIdx.x >> 3;
+    const int yt = ib % a.nyt;
+    const int z0 = xcd * a.Nz / 8, z1 = (xcd + 1) * a.Nz / 8;
+    unsigned *qc = a.queue + ((size_t) xcd * a.nyt + yt) * 2;
     const int lane = threadIdx.x & 63, wave = wave_id();
     const int kq = lane >> 4, nl = lane & 15;
     const int nnt = (a.Mx + 7) / 8, nR = a.dimR;
@@ -324,81 +331,147 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_rows(RebuildArgs a, const cp
         for (int i = 0; i < PT; ++i) bs[threadIdx.x + 256 * i] = st[i];
     }
     const double *Cd = (const double *) C;
-    // tasks of this wave: rows z0..z1-1, n-tiles wave, wave+4, ... (one tile
-    // in the mean row)
-    auto ntiles = [&](int zrow) { return zrow == a.Nz / 2 ? 1 : nnt; };
-    auto advance = [&](int &zrow, int &nt) {
-        nt += 4;
-        while (zrow < z1 && nt >= ntiles(zrow)) {
-            ++zrow;
-            nt = wave;
+    // task t of the queue: rows z0..z1-1 in order, nnt n-tiles each (one in
+    // the mean row)
+    const int zm = a.Nz / 2;
+    const bool has_mean = zm >= z0 && zm < z1;
+    const int ntask = (z1 - z0) * nnt - (has_mean ? nnt - 1 : 0);
+    auto task_of = [&](int t, int &zrow, int &nt) {
+        const int tm = has_mean ? (zm - z0) * nnt : ntask;   // first task of the mean row
+        if (t < tm) {
+            zrow = z0 + t / nnt;
+            nt = t % nnt;
+        } else if (t == tm) {
+            zrow = zm;
+            nt = 0;
+        } else {
+            const int u = t - tm - 1;
+            zrow = zm + 1 + u / nnt;
+            nt = u % nnt;
         }
     };
-    auto load = [&](int zrow, int nt, double *ca, double *cb) {
-        const RebuildTask t = rebuild_task(a, zrow, nt, nl);
+    auto grab = [&]() {
+        unsigned t0 = 0;
+        if (lane == 0) t0 = atomicAdd(&qc[0], 1u);
+        return __builtin_amdgcn_readfirstlane((int) t0);
+    };
+    // The coefficient fragments of a task move in two halves, one register
+    // buffer each: the loads of the second half are issued before the
+    // products of the first, those of the next task's first half before the
+    // products of the second.
+    constexpr int CK = NKS / 2, NCH = 2;
+    static_assert(CK % 4 == 0, "a half must be whole groups of four k-steps");
+    double ca[NCH][CK], cb[NCH][CK];
+    // row j = 4 s + kq of k-step s: a lane base (row kq) plus a uniform step;
+    // only the last k-step can pass row dimR - 1 (4 (NKS-1) < dimR), so only
+    // it is clamped
+    const size_t rstep = (size_t) 4 * a.Mx * 2;
+    auto load = [&](const RebuildTask &t, int ch) {
+        const double *pa = Cd + t.offA + (size_t) kq * a.Mx * 2;
+        const double *pb = Cd + t.offB + (size_t) kq * a.Mx * 2;
 #pragma unroll
-        for (int s = 0; s < NKS; ++s) {
-            const size_t j = min(s * 4 + kq, nR - 1);
-            ca[s] = Cd[t.offA + j * a.Mx * 2];
-            cb[s] = Cd[t.offB + j * a.Mx * 2];
+        for (int s = 0; s < CK; ++s) {
+            const int ks = ch * CK + s;
+            if (ks == NKS - 1) {
+                const size_t j = min(ks * 4 + kq, nR - 1);
+                ca[ch][s] = Cd[t.offA + j * a.Mx * 2];
+                cb[ch][s] = Cd[t.offB + j * a.Mx * 2];
+            } else {
+                ca[ch][s] = pa[ks * rstep];
+                cb[ch][s] = pb[ks * rstep];
+            }
         }
     };
-    auto run = [&](int zrow, int nt, const double *xa, const double *xb) {
+    // products of chunk ch (basis fragments LDS -> registers two k-steps at a
+    // time, one group ahead; the scheduling barriers keep the compiler from
+    // sinking those loads to their uses)
+    auto chunk = [&](bool mean, int ch, mf4 *acc) {
+        const double *xa = ca[ch], *xb = cb[ch];
+        if (mean) {
+#pragma unroll
+            for (int t = 0; t < CK; ++t) {
+                const int s = ch * CK + t;
+                acc[3] = mfma_f64(xa[t], bs[(s * 5 + 3) * 64 + lane], acc[3]);
+                acc[4] = mfma_f64(xa[t], bs[(s * 5 + 4) * 64 + lane], acc[4]);
+            }
+            return;
+        }
+        double p0[10], p1[10];
+        auto ldb = [&](int s0, double *p) {
+#pragma unroll
+            for (int e = 0; e < 10; ++e) p[e] = bs[(s0 * 5 + e) * 64 + lane];
+        };
+        auto mm = [&](int t0, const double *p) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int t = t0 + u;
+                acc[0] = mfma_f64(xa[t], p[u * 5 + 0], acc[0]);
+                acc[1] = mfma_f64(xa[t], p[u * 5 + 1], acc[1]);
+                acc[2] = mfma_f64(xa[t], p[u * 5 + 2], acc[2]);
+                acc[3] = mfma_f64(xb[t], p[u * 5 + 3], acc[3]);
+                acc[4] = mfma_f64(xb[t], p[u * 5 + 4], acc[4]);
+            }
+        };
+        const int sb = ch * CK;
+        ldb(sb, p0);
+#pragma unroll
+        for (int t0 = 0; t0 < CK; t0 += 4) {
+            if (t0 + 2 < CK) ldb(sb + t0 + 2, p1);
+            __builtin_amdgcn_sched_barrier(0);
+            mm(t0, p0);
+            __builtin_amdgcn_sched_barrier(0);
+            if (t0 + 2 >= CK) break;
+            if (t0 + 4 < CK) ldb(sb + t0 + 4, p0);
+            __builtin_amdgcn_sched_barrier(0);
+            mm(t0 + 2, p1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+    __syncthreads();
+    (void) wave;
+    int t = grab();
+    if (t >= ntask) {
+        t = ntask;
+    } else {
+        int zr, nt;
+        task_of(t, zr, nt);
+        load(rebuild_task(a, zr, nt, nl), 0);
+    }
+    while (t < ntask) {
+        int zr, nt;
+        task_of(t, zr, nt);
+        const RebuildTask cur = rebuild_task(a, zr, nt, nl);
+        const bool mean = zr == zm;
         mf4 acc[5];
 #pragma unroll
         for (int m = 0; m < 5; ++m) acc[m] = mf4_zero();
-        if (zrow == a.Nz / 2) {
+        int tn = ntask;
 #pragma unroll
-            for (int s = 0; s < NKS; ++s) {
-                acc[3] = mfma_f64(xa[s], bs[(s * 5 + 3) * 64 + lane], acc[3]);
-                acc[4] = mfma_f64(xa[s], bs[(s * 5 + 4) * 64 + lane], acc[4]);
-            }
-        } else {
-            // basis fragments move LDS -> registers two k-steps at a time, one
-            // group ahead of the products that use them; the scheduling
-            // barriers keep the compiler from sinking the loads to their uses
-            double p0[10], p1[10];
-            auto ldb = [&](int s0, double *p) {
-#pragma unroll
-                for (int e = 0; e < 10; ++e) p[e] = bs[(s0 * 5 + e) * 64 + lane];
-            };
-            auto mm = [&](int s0, const double *p) {
-#pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    const int s = s0 + t;
-                    acc[0] = mfma_f64(xa[s], p[t * 5 + 0], acc[0]);
-                    acc[1] = mfma_f64(xa[s], p[t * 5 + 1], acc[1]);
-                    acc[2] = mfma_f64(xa[s], p[t * 5 + 2], acc[2]);
-                    acc[3] = mfma_f64(xb[s], p[t * 5 + 3], acc[3]);
-                    acc[4] = mfma_f64(xb[s], p[t * 5 + 4], acc[4]);
+        for (int ch = 0; ch < NCH; ++ch) {
+            if (ch + 1 < NCH) {
+                load(cur, ch + 1);
+            } else {
+                tn = grab();
+                if (tn < ntask) {
+                    int zn, ntn;
+                    task_of(tn, zn, ntn);
+                    load(rebuild_task(a, zn, ntn, nl), 0);
                 }
-            };
-            ldb(0, p0);
-#pragma unroll
-            for (int s0 = 0; s0 < NKS; s0 += 4) {
-                if (s0 + 2 < NKS) ldb(s0 + 2, p1);
-                __builtin_amdgcn_sched_barrier(0);
-                mm(s0, p0);
-                __builtin_amdgcn_sched_barrier(0);
-                if (s0 + 2 >= NKS) break;
-                if (s0 + 4 < NKS) ldb(s0 + 4, p0);
-                __builtin_amdgcn_sched_barrier(0);
-                mm(s0 + 2, p1);
-                __builtin_amdgcn_sched_barrier(0);
             }
+            __builtin_amdgcn_sched_barrier(0);
+            chunk(mean, ch, acc);
+            __builtin_amdgcn_sched_barrier(0);
         }
-        rebuild_store(a, zrow, nt, yt * 16 + nl, kq, acc, U);
-    };
-    __syncthreads();
-    int za = z0, nta = wave;
-    if (za < z1 && nta >= ntiles(za)) {
-        nta -= 4;
-        advance(za, nta);
+        rebuild_store(a, zr, nt, yt * 16 + nl, kq, acc, U);
+        t = tn;
     }
-    for (; za < z1; advance(za, nta)) {
-        double ca[NKS], cb[NKS];
-        load(za, nta, ca, cb);
-        run(za, nta, ca, cb);
+    // every wave of the queue's blocks exits here once; the last clears it
+    if (lane == 0) {
+        const unsigned done = atomicAdd(&qc[1], 1u);
+        if (done == (unsigned) (a.bpq * 4 - 1)) {
+            atomicExch(&qc[0], 0u);
+            atomicExch(&qc[1], 0u);
+        }
     }
 }
 
@@ -407,12 +480,14 @@ void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst)
     const Dims &d = c.d;
     // segments of kz rows: about two blocks per CU
     const int nseg = std::max(1, std::min(d.Nz, 512 / c.nyt));
+    // queued form: blocks per (XCD, y-tile), about two blocks per CU
+    const int bpq = std::max(1, 64 / c.nyt);
     RebuildArgs a{c.dFragR, c.dKx, c.dKz, d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, c.nksR, c.nyt,
-                  nseg, 2. * M_PI / d.Lx, 2. * M_PI / d.Lz, c.abort_flag};
+                  nseg, 2. * M_PI / d.Lx, 2. * M_PI / d.Lz, c.abort_flag, c.rb_queue, bpq};
     double bytes = 16.0 * (double) (d.Nz - 1) * d.Mx * (2.0 * d.dimR + 5.0 * d.qpts);
     const hipStream_t st = c.stream;
     const int nks = c.nksR;
-    const dim3 grows(xcd_grid(c.nyt * nseg)), gchunk(xcd_grid(c.nyt * d.Nz));
+    const dim3 grows(8 * c.nyt * bpq), gchunk(xcd_grid(c.nyt * d.Nz));
     auto go = [=]() {
         if (nks == 8)
             hipLaunchKernelGGL(k_rebuild_rows<8>, grows, dim3(256), 0, st, a, Csrc, Udst);

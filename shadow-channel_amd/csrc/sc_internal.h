@@ -99,6 +99,7 @@ struct Context {
     //   fragF [nit][nksF][5][64]: Qpw, Qw, Qppw, Rw, Rpw at (y = 4s + l/16, i = 16t + l%16)
     double *dFragR = nullptr, *dFragF = nullptr;
     int nyt = 0, nksR = 0, nit = 0, nksF = 0;
+    unsigned *rb_queue = nullptr;               // rebuild task queues [8][nyt][2], zero between launches
     // LU factors of the implicit operators, per (direction, stage), as the
     // LDS images of the solve's blocks of sbt columns (col = z Mx + x):
     //   alpha: [nblk][2 parities][(dimQ+1)/2 chain rows][9][sbt]

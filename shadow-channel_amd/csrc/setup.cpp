@@ -269,6 +269,11 @@ int setup_host(Context &c)
                             ff[(((size_t) t * nksF + s) * 5 + m) * 64 + l] = (*Wt[m])[(size_t) i * q + y];
                     }
         c.nyt = nyt; c.nksR = nksR; c.nit = nit; c.nksF = nksF;
+        const size_t qbytes = sizeof(unsigned) * 2 * 8 * (size_t) nyt;
+        if (hipMalloc(&c.rb_queue, qbytes) != hipSuccess || hipMemset(c.rb_queue, 0, qbytes) != hipSuccess) {
+            last_error() = "device allocation of the rebuild queues failed";
+            return -1;
+        }
         if (upload(&c.dFragR, fr) || upload(&c.dFragF, ff)) {
             last_error() = "device allocation of transform operands failed";
             return -1;
