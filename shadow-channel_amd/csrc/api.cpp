@@ -226,8 +226,10 @@ static void pass2_adjoint(Context &c, const cplx *Ub, const cplx *AUb, cplx *Uds
         launch_zinv(c, a, c.W1);
         launch_xmix(c, XM_ADJOINT, c.W1, c.W2, 0, nullptr, nullptr);
     }
-    launch_zfwd(c, c.W2, 6, c.SP, ZF_SCRATCH);
-    launch_assemble_adj(c, c.SP, Udst);
+    if (!launch_zfwd_adj(c, c.W2, Udst)) {
+        launch_zfwd(c, c.W2, 6, c.SP, ZF_SCRATCH);
+        launch_assemble_adj(c, c.SP, Udst);
+    }
 }
 
 // vout = vin - (num/den) dC/dt with the Gauss-weighted physical inner

@@ -803,6 +803,89 @@ __global__ __launch_bounds__(BLK) void k_zfwd(ZfwdArgs a, PlanDev pz, const cplx
     }
 }
 
+// Forward z transforms of the six adjoint products with the assembly fused
+// (channel/pass2.c:417-566): a block transforms all six fields of ZA
+// columns, interleaved in LDS as 6 ZA sequences ([zp][field][x], no
+// transpose), and forms the five state slots at the stored kz rows:
+// Hz = -Tz - i kx Sy, Hx = -Tx + i kz Sy, Hy = -Ty - i kz Sx + i kx Sz, then
+// S_x, S_z (each field scaled first, as the separate transform does).
+// grid (Mx / ZA, qpts); PZ > 0: Pz == PZ and Mx % ZA == 0.
+constexpr int ZA = 4;
+struct ZadjArgs {
+    int Nz, Mx, Pz, qpts, XA;
+    double norm;
+    const double *Kx, *Kz;
+    const int *abort_flag;
+};
+
+template <int PZ>
+__global__ __launch_bounds__(BLK) void k_zfwd_adj(ZadjArgs a, PlanDev pz, const cplx *W2, cplx *U)
+{
+    extern __shared__ double2 lds[];
+    if (*a.abort_flag) return;
+    const int XA = PZ > 0 ? ZA : a.XA;
+    const int x0 = blockIdx.x * XA, y = blockIdx.y;
+    const int XB = PZ > 0 ? ZA : min(a.XA, a.Mx - x0);   // columns of this block
+    const int Nz = PZ > 0 ? 2 * PZ / 3 : a.Nz, Pz = PZ > 0 ? PZ : a.Pz, Mx = a.Mx;
+    const int NS = 6 * XA;                               // interleaved sequences
+    double2 *twl = lds + NS * Pz;
+    stage_twiddles(pz, twl);
+    {
+        // element (zp, field f, column xl) at (zp * 6 + f) * XA + xl
+        const int total = 6 * Pz * XA;
+        const cplx *src = W2 + (size_t) y * 6 * Pz * Mx;
+        Walk wk(XA);                                     // (zp * 6 + f, xl)
+        constexpr int NE = PZ > 0 ? (6 * ZA * PZ + BLK - 1) / BLK : MAXE;
+        constexpr int SB = PZ > 0 ? NE : SBAT;
+#pragma unroll 1
+        for (int i0 = 0; i0 < NE; i0 += SB) {
+            if ((int) threadIdx.x + i0 * BLK >= total) break;
+            double2 v[SB];
+#pragma unroll
+            for (int i = 0; i < SB; ++i) {
+                const int q = min(wk.q, 6 * Pz - 1), xl = min(wk.r, XB - 1);
+                wk.next();
+                const int zp = q / 6, f = q - zp * 6;
+                v[i] = src[((size_t) f * Pz + zp) * Mx + x0 + xl];
+            }
+#pragma unroll
+            for (int i = 0; i < SB; ++i) {
+                const int e = threadIdx.x + (i0 + i) * BLK;
+                if (e < total) lds[e] = v[i];
+            }
+        }
+    }
+    __syncthreads();
+    if constexpr (PZ > 0)
+        lds_fft_c<PZ, 6 * ZA, 1, 6 * ZA>(lds, twl, -1.0);
+    else
+        lds_fft(lds, NS, 1, NS, pz, twl, -1.0);
+    const size_t ss = (size_t) a.qpts * Mx;
+    Walk wt(XA);                                         // (stored kz row r, xl)
+    for (int e = threadIdx.x; e < XA * (Nz - 1); e += BLK, wt.next()) {
+        const int r = wt.q, xl = wt.r;
+        if (xl >= XB) continue;
+        const int kz = r < Nz / 2 ? r : r + 1;           // skip the Nyquist row
+        const int zp = kz < Nz / 2 ? kz : kz + Nz / 2;
+        double2 S[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            S[f] = lds[(zp * 6 + f) * XA + xl];
+            S[f].x *= a.norm;
+            S[f].y *= a.norm;
+        }
+        const int x = x0 + xl;
+        const double kx = a.Kx[x], kzv = a.Kz[kz];
+        const double2 Sx = S[0], Sy = S[1], Sz = S[2], Tx = S[3], Ty = S[4], Tz = S[5];
+        cplx *b = U + ((size_t) kz * 5 * a.qpts + y) * Mx + x;
+        b[0 * ss] = make_double2(-Tz.x + kx * Sy.y, -Tz.y - kx * Sy.x);
+        b[1 * ss] = make_double2(-Tx.x - kzv * Sy.y, -Tx.y + kzv * Sy.x);
+        b[2 * ss] = make_double2(-Ty.x + kzv * Sx.y - kx * Sz.y, -Ty.y - kzv * Sx.x + kx * Sz.x);
+        b[3 * ss] = Sx;
+        b[4 * ss] = Sz;
+    }
+}
+
 // adjoint assembly (channel/pass2.c:474-566):
 // Hz = -Tz - i kx Sy, Hx = -Tx + i kz Sy, Hy = -Ty - i kz Sx + i kx Sz
 __global__ void k_assemble_adj(const cplx *SP, cplx *U, const double *Kx, const double *Kz,
@@ -991,6 +1074,38 @@ void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind)
     prof_begin(c, st, 3, bytes);
     go();
     prof_end(c, st, 3, go);
+}
+
+// the fused transform + assembly; false when its tile does not fit (the
+// caller then transforms into scratch and assembles separately)
+bool launch_zfwd_adj(Context &c, const cplx *W2, cplx *Udst)
+{
+    const Dims &d = c.d;
+    const int fixed = z_fixed(d);
+    ZadjArgs a;
+    a.Nz = d.Nz;
+    a.Mx = d.Mx;
+    a.Pz = d.Pz;
+    a.qpts = d.qpts;
+    // generic tile: 6 XA Pz values and the twiddles within FFT_TILE
+    a.XA = fixed ? ZA : std::min(d.Mx, (FFT_TILE - d.Pz) / (6 * d.Pz));
+    if (a.XA < 1) return false;
+    a.norm = 1.0 / ((3. * d.Nx / 2.) * (3. * d.Nz / 2.));
+    a.Kx = c.dKx;
+    a.Kz = c.dKz;
+    a.abort_flag = c.abort_flag;
+    dim3 grid((d.Mx + a.XA - 1) / a.XA, d.qpts);
+    size_t lds = sizeof(double2) * (6 * (size_t) a.XA * d.Pz + d.Pz);
+    double bytes = 16.0 * (double) d.qpts * d.Mx * (6.0 * d.Pz + 5.0 * (d.Nz - 1));
+    const hipStream_t st = c.stream;
+    const PlanDev pd = to_dev(c.pz);
+    typedef void (*ZadjKernel)(ZadjArgs, PlanDev, const cplx *, cplx *);
+    const ZadjKernel kern = fixed == 96 ? k_zfwd_adj<96> : fixed == 48 ? k_zfwd_adj<48> : k_zfwd_adj<0>;
+    auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, W2, Udst); };
+    prof_begin(c, st, 3, bytes);
+    go();
+    prof_end(c, st, 3, go);
+    return true;
 }
 
 void launch_assemble_adj(Context &c, const cplx *SP, cplx *Udst)

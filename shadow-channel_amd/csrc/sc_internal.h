@@ -166,6 +166,9 @@ void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
                  double *flow, double *dot_out, const cplx *W1b = nullptr, int nf1 = 0);
 void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind);
 void launch_assemble_adj(Context &c, const cplx *SP, cplx *Udst);
+// forward z transform of the six adjoint products fused with the assembly
+// into the five state slots of Udst; false when the tile does not fit
+bool launch_zfwd_adj(Context &c, const cplx *W2, cplx *Udst);
 void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst);
 void launch_force(Context &c, const cplx *Usrc, int adj);
 void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb);

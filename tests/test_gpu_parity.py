@@ -187,6 +187,48 @@ def test_field_cache_is_bitwise(cc):
             np.testing.assert_array_equal(a, b)
 
 
+# Nx = Nz = 64 selects the fixed-length (96-point) transform kernels; Ny = 17
+# keeps the oracle fast.  The benchmark grid itself (64x64x97) is checked
+# against the oracle for one step with tolerances set by its own rounding floor
+# (tests/test_oracle_rounding_floor.py: 9.7e-11 step, 1.8e-9 adjoint).
+X96 = dict(Nx=64, Ny=17, Nz=64, Lx=2 * np.pi, Lz=np.pi, Re=3000.0, flux=4.0 / 3.0, dt=0.004)
+BENCH = dict(Nx=64, Ny=97, Nz=64, Lx=2 * np.pi, Lz=np.pi, Re=3000.0, flux=4.0 / 3.0, dt=0.004)
+
+
+def test_fixed_length_transforms(cc):
+    print("case: 96-point transforms, 64x16x64, primal/tangent/adjoint 2 steps", flush=True)
+    n = 2
+    o, C0 = start(cc, X96, n)
+    Co, Cg = C0.copy(), C0.copy()
+    o.primal(0, Co)
+    cc.c_primal(0, Cg)
+    assert rel(Cg, Co) <= SHORT
+    d = direction(X96, 3)
+    Io, Ig = d.copy(), d.copy()
+    o.tangent(0, n, Io, 0)
+    cc.c_tangent(0, n, Ig, 0)
+    assert rel(Ig, Io) <= SHORT
+    a = direction(X96, 4)
+    Ao, Ag = a.copy(), a.copy()
+    o.adjoint(n, 0, Ao, 0, 1.0)
+    cc.c_adjoint(n, 0, Ag, 0, 1.0)
+    assert rel(Ag, Ao) <= SHORT
+
+
+def test_benchmark_grid_one_step(cc):
+    print("case: benchmark grid 64x64x97, primal + adjoint 1 step", flush=True)
+    o, C0 = start(cc, BENCH, 1)
+    Co, Cg = C0.copy(), C0.copy()
+    o.primal(0, Co)
+    cc.c_primal(0, Cg)
+    assert rel(Cg, Co) <= 3e-10
+    a = direction(BENCH, 4)
+    Ao, Ag = a.copy(), a.copy()
+    o.adjoint(1, 0, Ao, 0, 1.0)
+    cc.c_adjoint(1, 0, Ag, 0, 1.0)
+    assert rel(Ag, Ao) <= 5e-9
+
+
 def test_duality_on_device(cc):
     print("case: tangent/adjoint duality", flush=True)
     n = 5

@@ -87,3 +87,34 @@ def test_rounding_floor_of_one_step():
     # measured 4.9e-11 on this container's Xeon; the GPU tolerance (1e-10)
     # must stay above it
     assert 1e-12 < floor < 1e-10
+
+
+@needs_fma
+def test_rounding_floor_at_benchmark_grid():
+    """64x64x97 channel (Lx = 2 pi, Lz = pi, Re = 3000, dt = 0.004): one primal
+    step and one adjoint step (strength 1).  The wall-normal resolution makes
+    the solves far less well conditioned than at Ny = 65: measured floors
+    9.7e-11 (step) and 1.8e-9 (adjoint); the GPU tolerances at this grid
+    (3e-10, 5e-9) stay above them."""
+    c = dict(Nx=64, Ny=97, Nz=64, Lx=2 * np.pi, Lz=np.pi, Re=3000.0, flux=4.0 / 3.0, dt=0.004)
+
+    def run(part):
+        def f():
+            o = pyorc.Oracle(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
+                             c["dt"], 1)
+            C = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=0)
+            o.primal(0, C)
+            if part == "primal":
+                o.destroy()
+                return C
+            A = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=4,
+                                unit=True)
+            o.adjoint(1, 0, A, 0, 1.0)
+            o.destroy()
+            return A
+        return f
+
+    step = _both_builds(run("primal"))
+    adj = _both_builds(run("adjoint"))
+    assert 1e-12 < step < 2e-10
+    assert 1e-11 < adj < 3e-9
