@@ -202,8 +202,10 @@ static void pass1_tangent(Context &c, const cplx *Ub, cplx *IUb, int check_cfl,
 
 // adjoint convective operator (channel/pass2.c, adjoint side) into Udst.
 // State fields from wbase (cached) when given, else transformed from Ub.
+// lam_ready: W1 already holds AUb's transformed fields (a previous call with
+// a cached state and the same AUb).
 static void pass2_adjoint(Context &c, const cplx *Ub, const cplx *AUb, cplx *Udst,
-                          const cplx *wbase = nullptr)
+                          const cplx *wbase = nullptr, bool lam_ready = false)
 {
     FieldSpec lam;
     lam.nf = 3;
@@ -213,7 +215,7 @@ static void pass2_adjoint(Context &c, const cplx *Ub, const cplx *AUb, cplx *Uds
         lam.slot[s] = s;
     }
     if (wbase) {
-        launch_zinv(c, lam, c.W1);
+        if (!lam_ready) launch_zinv(c, lam, c.W1);
         launch_xmix(c, XM_ADJOINT, wbase, c.W2, 0, nullptr, nullptr, c.W1, 6);
     } else {
         FieldSpec a = velocity_and_vorticity(Ub);
@@ -666,14 +668,17 @@ int c_adjoint(int start, int end, sc_complex *AC_given, int Nz_dup, int Nvar_dup
             launch_axpy_c(c, c.AC, c.Ctmp, 0.5 * s, c.csize);
         for (int k = 0; k < 3; ++k) {
             long count = 3L * n - k - 1;
+            // with both states cached, lambda is transformed once for the pair
+            bool lam_ready = false;
             if (k < 2) {
                 const cplx *w = wcached(c, count - 1);
                 if (!w) launch_rebuild(c, mc(c, count - 1), c.U);
                 pass2_adjoint(c, c.U, c.AU, c.LU, w);
+                lam_ready = w != nullptr;
             }
             const cplx *w = wcached(c, count);
             if (!w) launch_rebuild(c, mc(c, count), c.U);
-            pass2_adjoint(c, c.U, c.AU, c.U, w);
+            pass2_adjoint(c, c.U, c.AU, c.U, w, lam_ready && w != nullptr);
             launch_force(c, c.U, 1);
             if (k < 2) launch_force(c, c.LU, 1, c.FLa, c.FLb);
             launch_solve(c, c.AC, c.Csol, k, 1, c.FLa, c.FLb, 0.0);
