@@ -647,6 +647,189 @@ __global__ __launch_bounds__(256, SC_YT_WAVES) void k_force(ForceArgs a, const c
     }
 }
 
+// Projection with the i-tile's whole weighted basis in LDS (NKS k-steps;
+// the three bases Qpw, Qw, Rw of the primal, all five for the adjoint), for
+// NKS in {16, 28, 40}.  A block of 8 waves owns one i-tile and a contiguous
+// range of its (kz row, n-tile) tasks; wave w takes tasks w, w+8, ... of the
+// range.  No barrier after the staging: each wave streams its field values
+// through two register buffers of CK k-steps (the loads of chunk c+1, or of
+// the next task's chunk 0, issued before the products of chunk c).  The
+// products enter the accumulators in the same order as in k_force.
+// Grid: nit * bpi blocks, block b -> i-tile b / bpi.
+constexpr int FR_THREADS = 512;
+
+template <bool ADJ, int NKS>
+struct ForceRowsShape {
+    static constexpr int NB = ADJ ? 5 : 3;                   // bases staged per k-step
+    static constexpr int NF = ADJ ? 5 : 3;                   // field values per k-step
+    // chunk: the largest CK <= CKMAX with an even number of chunks
+    static constexpr int CKMAX = ADJ ? 4 : 5;
+    static constexpr int pick(int ck) { return (NKS % ck == 0 && (NKS / ck) % 2 == 0) ? ck : pick(ck - 1); }
+    static constexpr int CK = pick(CKMAX), NCH = NKS / CK;
+};
+
+template <bool ADJ, int NKS>
+__global__ __launch_bounds__(FR_THREADS) void k_force_rows(ForceArgs a, int bpi, const cplx *U,
+                                                           cplx *Fa, cplx *Fb)
+{
+    using SH = ForceRowsShape<ADJ, NKS>;
+    constexpr int NB = SH::NB, NF = SH::NF, CK = SH::CK, NCH = SH::NCH;
+    static_assert(NCH % 2 == 0 && CK >= 1, "chunks alternate between two buffers");
+    extern __shared__ double bsf[];
+    if (*a.abort_flag) return;
+    const int it = blockIdx.x / bpi, lb = blockIdx.x - it * bpi;
+    const int lane = threadIdx.x & 63, wave = wave_id();
+    const int kq = lane >> 4, nl = lane & 15;
+    const int Mx = a.Mx, q = a.qpts, nR = a.dimR, nQ = a.dimQ;
+    const int nnt = (Mx + 7) / 8;
+    const int ntask = (a.Nz - 1) * nnt;                       // every row but the Nyquist row
+    const int t0 = lb * ntask / bpi, t1 = (lb + 1) * ntask / bpi;
+    {
+        // bases of the i-tile: fragF [nit][nksF][5][64] -> [NKS][NB][64]
+        const double *frag = a.frag + (size_t) it * a.nks * 5 * 64;
+        constexpr int TOT = NKS * NB * 64, PT = (TOT + FR_THREADS - 1) / FR_THREADS;
+        double st[PT];
+#pragma unroll
+        for (int i = 0; i < PT; ++i) {
+            const int e = min((int) threadIdx.x + FR_THREADS * i, TOT - 1);
+            const int s = e / (NB * 64), r = e - s * NB * 64, m = r >> 6, l = r & 63;
+            const int mm = ADJ ? m : (m == 2 ? 3 : m);       // primal: Qpw, Qw, Rw
+            st[i] = frag[((size_t) s * 5 + mm) * 64 + l];
+        }
+#pragma unroll
+        for (int i = 0; i < PT; ++i) {
+            const int e = threadIdx.x + FR_THREADS * i;
+            if (e < TOT) bsf[e] = st[i];
+        }
+    }
+    __syncthreads();
+    // staged bases per k-step: Qpw, Qw, Qppw, Rw, Rpw (adjoint); Qpw, Qw, Rw (primal)
+    struct Task {
+        int z, nt, xcol, part;
+        size_t off;
+    };
+    auto task_of = [&](int t) {
+        Task k;
+        const int zr = t / nnt;
+        k.z = zr < a.Nz / 2 ? zr : zr + 1;
+        k.nt = t - zr * nnt;
+        k.xcol = min(k.nt * 8 + ncol_x(nl), Mx - 1);
+        k.part = ncol_part(nl);
+        k.off = (size_t) k.z * 5 * q * Mx * 2 + (size_t) k.xcol * 2;
+        return k;
+    };
+    const double *Ud = (const double *) U;
+    const size_t ss2 = (size_t) q * Mx * 2;                   // slot stride in doubles
+    double r[2][NF][CK];
+    auto load = [&](const Task &k, int ch, int buf) {
+        const bool mean = k.z == 0 && k.xcol == 0;
+        const int other = mean ? 0 : 1 - k.part;
+#pragma unroll
+        for (int s = 0; s < CK; ++s) {
+            const int y = min((ch * CK + s) * 4 + kq, q - 1);
+            const size_t e = k.off + (size_t) y * Mx * 2;
+            r[buf][1][s] = Ud[e + other];
+            r[buf][0][s] = Ud[e + ss2 + other];
+            r[buf][2][s] = Ud[e + 2 * ss2 + k.part];
+            if (ADJ) {
+                r[buf][3 % NF][s] = Ud[e + 3 * ss2 + other];
+                r[buf][4 % NF][s] = Ud[e + 4 * ss2 + other];
+            }
+        }
+    };
+    int t = t0 + wave;
+    if (t < t1) load(task_of(t), 0, 0);
+    while (t < t1) {
+        const Task k = task_of(t);
+        const bool mean = k.z == 0 && k.xcol == 0;
+        const double kz = wave_kz(k.z, a.Nz, a.az);
+        const double kx = wave_kx(k.xcol, a.ax), k2 = kz * kz + kx * kx;
+        const double sg = k.part ? 1.0 : -1.0;
+        mf4 accA = mf4_zero(), accB = mf4_zero();
+        const int tn = t + 8;
+        // products of chunk ch from buffer buf
+        auto chunk = [&](int ch, int buf) {
+            const double *bc = bsf + (size_t) ch * CK * NB * 64 + lane;
+#pragma unroll
+            for (int u = 0; u < CK; ++u) {
+                const double hx = r[buf][0][u], hz = r[buf][1][u], hy = r[buf][2][u];
+                double vP, vY, vP2, vZ = 0.0, vZ2 = 0.0;
+                if (mean) {
+                    vP = 0.0;
+                    vY = 0.0;
+                    vP2 = k.part ? hz : hx;
+                    if (ADJ) vZ2 = k.part ? -r[buf][3 % NF][u] : r[buf][4 % NF][u];
+                } else {
+                    vP = sg * (kx * hx + kz * hz);
+                    vY = -k2 * hy;
+                    vP2 = sg * (kz * hx - kx * hz);
+                    if (ADJ) {
+                        vZ = -sg * (kz * r[buf][3 % NF][u] - kx * r[buf][4 % NF][u]);
+                        vZ2 = sg * (kz * r[buf][4 % NF][u] + kx * r[buf][3 % NF][u]);
+                    }
+                }
+                const double *b = bc + u * NB * 64;
+                accA = mfma_f64(vP, b[(ADJ ? 0 : 0) * 64], accA);
+                accB = mfma_f64(vP2, b[(ADJ ? 3 : 2) * 64], accB);
+                accA = mfma_f64(vY, b[1 * 64], accA);
+                if (ADJ) {
+                    accA = mfma_f64(vZ, b[2 * 64], accA);
+                    accB = mfma_f64(vZ2, b[4 * 64], accB);
+                }
+            }
+        };
+        // two chunks per iteration, one per buffer; the loads of the chunk
+        // after next (or of the next task's first chunk) go into the buffer
+        // just used
+#pragma unroll 1
+        for (int ch = 0; ch < NCH; ch += 2) {
+            load(k, ch + 1, 1);
+            __builtin_amdgcn_sched_barrier(0);
+            chunk(ch, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            if (ch + 2 < NCH)
+                load(k, ch + 2, 0);
+            else if (tn < t1)
+                load(task_of(tn), 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            chunk(ch + 1, 1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        const int i = it * 16 + nl;
+        if (i < nR) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int x = k.nt * 8 + 2 * kq + h;
+                if (x >= Mx) continue;
+                if (i < nQ && !(k.z == 0 && x == 0))
+                    Fa[((size_t) k.z * nQ + i) * Mx + x] = make_double2(accA[2 * h], accA[2 * h + 1]);
+                Fb[((size_t) k.z * nR + i) * Mx + x] = make_double2(accB[2 * h], accB[2 * h + 1]);
+            }
+        }
+        t = tn;
+    }
+}
+
+template <bool ADJ, int NKS>
+static void launch_force_rows(const ForceArgs &a, int nit, int ntask, const cplx *U, cplx *Fa,
+                              cplx *Fb, hipStream_t st)
+{
+    using SH = ForceRowsShape<ADJ, NKS>;
+    const size_t lds = sizeof(double) * NKS * SH::NB * 64;
+    static bool attr = false;
+    if (!attr) {
+        (void) hipFuncSetAttribute((const void *) k_force_rows<ADJ, NKS>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+        attr = true;
+    }
+    // blocks per i-tile: one 8-wave block per CU in total (the registers of
+    // the wave's two field buffers allow no second block), never more than
+    // the tasks
+    const int bpi = std::max(1, std::min(ntask, 256 / nit));
+    hipLaunchKernelGGL((k_force_rows<ADJ, NKS>), dim3(nit * bpi), dim3(FR_THREADS), lds, st, a, bpi,
+                       U, Fa, Fb);
+}
+
 void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb)
 {
     const Dims &d = c.d;
@@ -656,11 +839,24 @@ void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb)
     double bytes = 16.0 * (double) (d.Nz - 1) * d.Mx *
                    ((adj ? 5.0 : 3.0) * d.qpts + d.dimQ + d.dimR);
     const hipStream_t st = c.stream;
+    const int nks = c.nksF, nit = c.nit, ntask = (d.Nz - 1) * ((d.Mx + 7) / 8);
     auto go = [=]() {
-        if (adj)
+        // staged-basis kernel for the k-step counts it is built for
+        if (nks == 40 || nks == 28 || nks == 16) {
+            if (adj) {
+                if (nks == 40) launch_force_rows<true, 40>(a, nit, ntask, Usrc, Fa, Fb, st);
+                else if (nks == 28) launch_force_rows<true, 28>(a, nit, ntask, Usrc, Fa, Fb, st);
+                else launch_force_rows<true, 16>(a, nit, ntask, Usrc, Fa, Fb, st);
+            } else {
+                if (nks == 40) launch_force_rows<false, 40>(a, nit, ntask, Usrc, Fa, Fb, st);
+                else if (nks == 28) launch_force_rows<false, 28>(a, nit, ntask, Usrc, Fa, Fb, st);
+                else launch_force_rows<false, 16>(a, nit, ntask, Usrc, Fa, Fb, st);
+            }
+        } else if (adj) {
             hipLaunchKernelGGL(k_force<true>, grid, dim3(256), 0, st, a, Usrc, Fa, Fb);
-        else
+        } else {
             hipLaunchKernelGGL(k_force<false>, grid, dim3(256), 0, st, a, Usrc, Fa, Fb);
+        }
     };
     prof_begin(c, st, 5, bytes);
     go();
