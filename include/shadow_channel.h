@@ -79,6 +79,16 @@ int c_read_solution(const char *filename, sc_complex *C_ptr,
 int c_save_solution(const char *filename, sc_complex *C_ptr,
                     int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_dup);
 
+/* channel/channel.h:17-18 -> tangent_forcing0 / tangent_forcing
+   (channel/tangent_forcing0.c:10-44, channel/tangent_forcing.c:10-75):
+   Reynolds-number sensitivity forcing of the stored state MC[count]
+   (0 <= count <= 3 nsteps).  f_a [Nz][dimQ][Nx/2] and f_b [Nz][dimR][Nx/2]
+   for the (kz,kx) != (0,0) columns (column (0,0) set to zero); f0_a, f0_b
+   [dimR] for the mean column.  The reference declares both but calls neither
+   (channel/channel.c:519-522, channel/increproject0.c:46-90). */
+int sc_tangent_forcing(int count, sc_complex *f_a, sc_complex *f_b, sc_complex *f0_a,
+                       sc_complex *f0_b);
+
 /* channel/channel.i:206-218: Gauss points and weights (library-owned) */
 int c_quad(double **y_ptr, int *qpts_ptr1, double **w_ptr, int *qpts_ptr2);
 

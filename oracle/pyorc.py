@@ -64,6 +64,7 @@ def load():
     L.orc_get_state.argtypes = [c.c_void_p, c.c_void_p]
     L.orc_set_faithful_cost.argtypes = [c.c_int]
     L.orc_srand.argtypes = [c.c_uint]
+    L.orc_tangent_forcing.argtypes = [c.c_int, c.c_void_p, c.c_void_p, c.c_void_p, c.c_void_p]
     L.orc_rand.restype = c.c_int
     for f in ("orc_qpts", "orc_dimR", "orc_dimQ", "orc_Nx", "orc_Nz",
               "orc_nsteps", "orc_last_status"):
@@ -180,6 +181,17 @@ class Oracle:
         U = np.zeros(self.ushape, complex)
         self.L.orc_get_state(_ptr(C), _ptr(U))
         return C, U
+
+    def tangent_forcing(self, count):
+        """Re-sensitivity forcing of MC[count]: (f_a [Nz][dimQ][Nx/2],
+        f_b [Nz][dimR][Nx/2], f0_a [dimR], f0_b [dimR])."""
+        nQ, nR = self.dimQ, self.dimR
+        fa = np.zeros((self.Nz, nQ, self.Nx // 2), complex)
+        fb = np.zeros((self.Nz, nR, self.Nx // 2), complex)
+        f0a = np.zeros(nR, complex)
+        f0b = np.zeros(nR, complex)
+        assert self.L.orc_tangent_forcing(count, _ptr(fa), _ptr(fb), _ptr(f0a), _ptr(f0b)) == 0
+        return fa, fb, f0a, f0b
 
     def last_status(self):
         return self.L.orc_last_status()

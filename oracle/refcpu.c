@@ -1358,6 +1358,68 @@ int orc_adjoint(int start, int end, orc_cplx *ACg, int inhomo, double strength)
     return 0;
 }
 
+/* Reynolds-number sensitivity forcing of stored state MC[count]
+ * (channel/tangent_forcing.c:10-75, channel/tangent_forcing0.c:10-44), the
+ * band products in the order of smMult / smMult0 (channel/mvOps.c:198-237):
+ *   (kz,kx) != (0,0): f_a = -re^2 (K2^2 Qs + 2 K2 Qps + Qpps) alpha,
+ *                     f_b =  re^2 (Rps + K2 Rs) beta
+ *   mean column:      f0_a = re^2 Rps alpha(0,0), f0_b = re^2 Rps beta(0,0)
+ * f_a [Nz][dimQ][Mx], f_b [Nz][dimR][Mx] (column (0,0) zero), f0 [dimR]. */
+int orc_tangent_forcing(int count, orc_cplx *fa_, orc_cplx *fb_, orc_cplx *f0a_, orc_cplx *f0b_)
+{
+    if (!G.ok || count < 0 || count > 3 * G.nsteps) return -1;
+    const int Nz = G.Nz, Mx = G.Mx, nQ = G.dimQ, nR = G.dimR;
+    const cx *S = G.MC + (long) count * csize();
+    cx *fa = (cx *) fa_, *fb = (cx *) fb_, *f0a = (cx *) f0a_, *f0b = (cx *) f0b_;
+    const double re = G.re;
+    memset(fa, 0, sizeof(cx) * (size_t) Nz * nQ * Mx);
+    memset(fb, 0, sizeof(cx) * (size_t) Nz * nR * Mx);
+    for (int z = 0; z < Nz; ++z) {
+        const cx *al = S + (long) (z * 2 + 0) * nR * Mx, *be = S + (long) (z * 2 + 1) * nR * Mx;
+        for (int x = (z == 0 ? 1 : 0); x < Mx; ++x) {
+            const double k2 = G.K2[z * Mx + x], s = k2 * k2;
+            for (int i = 0; i < nQ; ++i) {
+                double sr = 0.0, si = 0.0;
+                for (int m = -MIN(4, i / 2); m <= MIN(4, (nQ - i + 1) / 2 - 1); ++m) {
+                    const int e = i * 9 + 4 + m;
+                    const double a = -re * re * (s * G.Qs[e] + 2. * k2 * G.Qps[e] + G.Qpps[e]);
+                    sr += a * al[(long) (i + 2 * m) * Mx + x].re;
+                    si += a * al[(long) (i + 2 * m) * Mx + x].im;
+                }
+                fa[((long) z * nQ + i) * Mx + x].re = sr;
+                fa[((long) z * nQ + i) * Mx + x].im = si;
+            }
+            for (int i = 0; i < nR; ++i) {
+                double sr = 0.0, si = 0.0;
+                for (int m = -MIN(2, i / 2); m <= MIN(2, (nR - i + 1) / 2 - 1); ++m) {
+                    const int e = i * 5 + 2 + m;
+                    const double a = re * re * (G.Rps[e] + k2 * G.Rs[e]);
+                    sr += a * be[(long) (i + 2 * m) * Mx + x].re;
+                    si += a * be[(long) (i + 2 * m) * Mx + x].im;
+                }
+                fb[((long) z * nR + i) * Mx + x].re = sr;
+                fb[((long) z * nR + i) * Mx + x].im = si;
+            }
+        }
+    }
+    const cx *al0 = S, *be0 = S + (long) nR * Mx;
+    for (int i = 0; i < nR; ++i) {
+        double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+        for (int m = -MIN(2, i / 2); m <= MIN(2, (nR - i + 1) / 2 - 1); ++m) {
+            const double a = re * re * G.Rps[i * 5 + 2 + m];
+            ar += a * al0[(long) (i + 2 * m) * Mx].re;
+            ai += a * al0[(long) (i + 2 * m) * Mx].im;
+            br += a * be0[(long) (i + 2 * m) * Mx].re;
+            bi += a * be0[(long) (i + 2 * m) * Mx].im;
+        }
+        f0a[i].re = ar;
+        f0a[i].im = ai;
+        f0b[i].re = br;
+        f0b[i].im = bi;
+    }
+    return 0;
+}
+
 const orc_cplx *orc_getsoln(int i)
 {
     if (!G.ok || i < 0 || i > G.nsteps) return NULL;

@@ -37,6 +37,10 @@ int    orc_tangent(int start_step, int end_step, orc_cplx *IC, int inhomo);
 int    orc_adjoint(int start_step, int end_step, orc_cplx *AC, int inhomo,
                    double strength);
 double orc_ddt_project(int i_step, orc_cplx *IC);
+/* Re-sensitivity forcing of MC[count] (channel/tangent_forcing.c,
+   channel/tangent_forcing0.c): f_a [Nz][dimQ][Nx/2], f_b [Nz][dimR][Nx/2],
+   f0_a, f0_b [dimR]. */
+int orc_tangent_forcing(int count, orc_cplx *fa, orc_cplx *fb, orc_cplx *f0a, orc_cplx *f0b);
 void   orc_spec2phys(const orc_cplx *C, double *flow);
 const orc_cplx *orc_getsoln(int i_step);
 const orc_cplx *orc_getincresoln(int i_step);

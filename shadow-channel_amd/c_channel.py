@@ -74,6 +74,8 @@ _L.sc_set_field_cache.argtypes = [_i]
 _L.sc_set_field_cache.restype = _i
 _L.sc_field_cache_active.argtypes = []
 _L.sc_field_cache_active.restype = _i
+_L.sc_tangent_forcing.argtypes = [_i, _c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p]
+_L.sc_tangent_forcing.restype = _i
 
 SC_EINVAL = -3
 SC_ENOTREADY = -2
@@ -250,3 +252,17 @@ def sc_field_cache_active():
 
 def sc_synchronize():
     _check(_L.sc_synchronize(), "sc_synchronize")
+
+
+def sc_tangent_forcing(count):
+    """Reynolds-number sensitivity forcing of the stored state MC[count]
+    (channel/tangent_forcing.c, channel/tangent_forcing0.c): returns
+    (f_a [Nz, dimQ, Nx/2], f_b [Nz, dimR, Nx/2], f0_a [dimR], f0_b [dimR])."""
+    Nz, Mx, nQ, nR = _L.c_Nz(), _L.c_Nx() // 2, _L.c_dimQ(), _L.c_dimR()
+    fa = np.zeros((Nz, nQ, Mx), complex)
+    fb = np.zeros((Nz, nR, Mx), complex)
+    f0a = np.zeros(nR, complex)
+    f0b = np.zeros(nR, complex)
+    _check(_L.sc_tangent_forcing(int(count), fa.ctypes.data, fb.ctypes.data, f0a.ctypes.data,
+                                 f0b.ctypes.data), "sc_tangent_forcing")
+    return fa, fb, f0a, f0b

@@ -846,6 +846,28 @@ int c_quad(double **y_ptr, int *qpts_ptr1, double **w_ptr, int *qpts_ptr2)
     return SC_OK;
 }
 
+int sc_tangent_forcing(int count, sc_complex *f_a, sc_complex *f_b, sc_complex *f0_a,
+                       sc_complex *f0_b)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    const Dims &d = c.d;
+    if (!f_a || !f_b || !f0_a || !f0_b || count < 0 || count > 3 * d.nsteps) {
+        last_error() = "tangent_forcing requires 0 <= count <= 3 nsteps and four output arrays";
+        return SC_EINVAL;
+    }
+    // outputs staged in the projection buffers (same shapes) and Ctmp2
+    launch_tangent_forcing(c, mc(c, count), c.Fa, c.Fb, c.Ctmp2);
+    int rc = stream_sync(c);
+    if (rc) return rc;
+    const size_t na = (size_t) d.Nz * d.dimQ * d.Mx, nb = (size_t) d.Nz * d.dimR * d.Mx;
+    SC_HIP(hipMemcpy(f_a, c.Fa, sizeof(cplx) * na, hipMemcpyDeviceToHost));
+    SC_HIP(hipMemcpy(f_b, c.Fb, sizeof(cplx) * nb, hipMemcpyDeviceToHost));
+    SC_HIP(hipMemcpy(f0_a, c.Ctmp2, sizeof(cplx) * d.dimR, hipMemcpyDeviceToHost));
+    SC_HIP(hipMemcpy(f0_b, c.Ctmp2 + d.dimR, sizeof(cplx) * d.dimR, hipMemcpyDeviceToHost));
+    return SC_OK;
+}
+
 int c_qpts(void) { return ctx().d.qpts; }
 int c_dimR(void) { return ctx().d.dimR; }
 int c_dimQ(void) { return ctx().d.dimQ; }

@@ -1561,6 +1561,72 @@ __global__ __launch_bounds__(CFL_THREADS) void k_cfl_check(double *cflmax, int q
 
 static int nblocks(size_t n) { return (int) std::min<size_t>((n + 255) / 256, 8192); }
 
+// Reynolds-number sensitivity forcing of a stored state S = MC[count]
+// (channel/tangent_forcing.c:10-75, channel/tangent_forcing0.c:10-44), band
+// products in the order of smMult / smMult0 (channel/mvOps.c:198-237):
+//   (kz,kx) != (0,0): fa = -re^2 (K2^2 Qs + 2 K2 Qps + Qpps) alpha   [Nz][dimQ][Mx]
+//                     fb =  re^2 (Rps + K2 Rs) beta                   [Nz][dimR][Mx]
+//   mean column:      f0 = re^2 Rps alpha(0,0), re^2 Rps beta(0,0)    [2][dimR]
+// Column (0,0) of fa and fb is zero.  One thread per output element.
+__global__ void k_tangent_forcing(const cplx *S, const double *Qs, const double *Qps,
+                                  const double *Qpps, const double *Rs, const double *Rps,
+                                  const double *Kx, const double *Kz, double re, int Nz, int Mx,
+                                  int nQ, int nR, cplx *fa, cplx *fb, cplx *f0)
+{
+    const size_t na = (size_t) Nz * nQ * Mx, nb = (size_t) Nz * nR * Mx;
+    const size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+    if (e >= na + nb + 2 * (size_t) nR) return;
+    if (e >= na + nb) {
+        const int v = (int) ((e - na - nb) / nR), i = (int) (e - na - nb) - v * nR;
+        const cplx *y = S + (size_t) v * nR * Mx;                  // column (0,0) of alpha or beta
+        double sr = 0.0, si = 0.0;
+        for (int m = -min(2, i / 2); m <= min(2, (nR - i + 1) / 2 - 1); ++m) {
+            const double a = re * re * Rps[i * 5 + 2 + m];
+            sr += a * y[(size_t) (i + 2 * m) * Mx].x;
+            si += a * y[(size_t) (i + 2 * m) * Mx].y;
+        }
+        f0[e - na - nb] = make_double2(sr, si);
+        return;
+    }
+    const bool alpha = e < na;
+    const size_t r = alpha ? e : e - na;
+    const int N = alpha ? nQ : nR;
+    const int x = (int) (r % Mx), i = (int) ((r / Mx) % N), z = (int) (r / ((size_t) Mx * N));
+    cplx *out = alpha ? fa + r : fb + r;
+    if (z == 0 && x == 0) {
+        *out = make_double2(0.0, 0.0);
+        return;
+    }
+    const double k2 = Kz[z] * Kz[z] + Kx[x] * Kx[x], s = k2 * k2;
+    const cplx *y = S + ((size_t) (z * 2 + (alpha ? 0 : 1)) * nR) * Mx + x;
+    double sr = 0.0, si = 0.0;
+    if (alpha) {
+        for (int m = -min(4, i / 2); m <= min(4, (nQ - i + 1) / 2 - 1); ++m) {
+            const int b = i * 9 + 4 + m;
+            const double a = -re * re * (s * Qs[b] + 2. * k2 * Qps[b] + Qpps[b]);
+            sr += a * y[(size_t) (i + 2 * m) * Mx].x;
+            si += a * y[(size_t) (i + 2 * m) * Mx].y;
+        }
+    } else {
+        for (int m = -min(2, i / 2); m <= min(2, (nR - i + 1) / 2 - 1); ++m) {
+            const int b = i * 5 + 2 + m;
+            const double a = re * re * (Rps[b] + k2 * Rs[b]);
+            sr += a * y[(size_t) (i + 2 * m) * Mx].x;
+            si += a * y[(size_t) (i + 2 * m) * Mx].y;
+        }
+    }
+    *out = make_double2(sr, si);
+}
+
+void launch_tangent_forcing(Context &c, const cplx *S, cplx *fa, cplx *fb, cplx *f0)
+{
+    const Dims &d = c.d;
+    const size_t n = (size_t) d.Nz * (d.dimQ + d.dimR) * d.Mx + 2 * (size_t) d.dimR;
+    hipLaunchKernelGGL(k_tangent_forcing, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, c.stream,
+                       S, c.dQs, c.dQps, c.dQpps, c.dRs, c.dRps, c.dKx, c.dKz, d.re, d.Nz, d.Mx,
+                       d.dimQ, d.dimR, fa, fb, f0);
+}
+
 void launch_axpy_c(Context &c, cplx *y, const cplx *x, double a, size_t n)
 {
     hipLaunchKernelGGL(k_axpy, dim3(nblocks(n)), dim3(256), 0, c.stream, y, x, a, n, c.abort_flag);

@@ -177,6 +177,8 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
 // slot: first stored state of the step being checked (kept when it stops)
 void launch_cfl_check(Context &c, int slot);
 int solve_columns_per_block(int dimR);
+// Re-sensitivity forcing of state S: fa [Nz][dimQ][Mx], fb [Nz][dimR][Mx], f0 [2][dimR]
+void launch_tangent_forcing(Context &c, const cplx *S, cplx *fa, cplx *fb, cplx *f0);
 int xmix_zblock(Context &c, int nf);
 void launch_axpy_c(Context &c, cplx *y, const cplx *x, double a, size_t n);
 void launch_diff_scale(Context &c, cplx *out, const cplx *a, const cplx *b, double s, size_t n);

@@ -272,3 +272,18 @@ def test_invalid_calls_raise(cc):
         cc.c_adjoint(0, 1, C0.copy(), 0, 0.0)
     with pytest.raises(AssertionError):
         cc.c_primal(0, np.zeros((SMALL["Nz"], 2, SMALL["Ny"] - 2, 3), complex))
+
+
+def test_tangent_forcing(cc):
+    """Re-sensitivity forcing of stored states (channel/tangent_forcing.c,
+    channel/tangent_forcing0.c) on the device against the oracle: a direct
+    band product, so agreement is at rounding level."""
+    print("case: tangent forcing of stored states", flush=True)
+    o, C0 = start(cc, SMALL, 2, amp=0.1)
+    o.primal(0, C0.copy())
+    cc.c_primal(0, C0.copy())
+    for count in (0, 1, 4, 6):
+        got = cc.sc_tangent_forcing(count)
+        ref = o.tangent_forcing(count)
+        for g, r in zip(got, ref):
+            assert np.linalg.norm(g - r) <= 1e-12 * np.linalg.norm(r)

@@ -204,3 +204,42 @@ def test_cfl_violation_truncates_run():
     assert o.last_status() == -1
     np.testing.assert_array_equal(C, C0)
     assert np.all(o.getsoln(1) == 0)
+
+
+def test_tangent_forcing_against_dense_quadrature_operators():
+    """Re-sensitivity forcing (channel/tangent_forcing.c:10-75,
+    channel/tangent_forcing0.c:10-44) against dense operators assembled
+    independently of the band storage: Gram matrices by Gauss quadrature of
+    the basis values (exact at these degrees), f_a = -re^2 (K2^2 G_QQ +
+    2 K2 G_Q'Q' + G_Q''Q'') alpha, f_b = re^2 (G_R'R' + K2 G_RR) beta, mean
+    column re^2 G_R'R' alpha(0,0) and beta(0,0); column (0,0) of f_a, f_b zero."""
+    Nx, Ny, Nz = 16, 33, 6
+    o = pyorc.Oracle(Nx, Ny, Nz, 2.0, 1.3, 1000.0, 4.0 / 3.0, 0.01, 1)
+    C = synthetic_state(Nx, Ny, Nz, 2.0, 1.3, 4.0 / 3.0, seed=3, amp=0.2)
+    o.primal(0, C.copy())
+    q, nQ, nR, Mx = o.qpts, o.dimQ, o.dimR, Nx // 2
+    y, w = o.quad()
+    Q, Qp, Qpp = (o.array(n).reshape(q, nQ) for n in ("Q", "Qp", "Qpp"))
+    R, Rp = (o.array(n).reshape(q, nR) for n in ("R", "Rp"))
+    gram = lambda A, B: (A * w[:, None]).T @ B
+    GQ, GQp, GQpp, GR, GRp = gram(Q, Q), gram(Qp, Qp), gram(Qpp, Qpp), gram(R, R), gram(Rp, Rp)
+    Kx, Kz = o.array("Kx"), o.array("Kz")
+    re = o.scalar("re")
+    for count in (0, 3):
+        fa, fb, f0a, f0b = o.tangent_forcing(count)
+        S = o.getsoln(count // 3)            # stored state MC[count]
+        al, be = S[:, 0], S[:, 1]
+        for z in range(Nz):
+            for x in range(Mx):
+                if z == 0 and x == 0:
+                    assert np.all(fa[z, :, x] == 0) and np.all(fb[z, :, x] == 0)
+                    continue
+                k2 = Kz[z] ** 2 + Kx[x] ** 2
+                ea = -re * re * ((k2 * k2) * GQ + 2 * k2 * GQp + GQpp) @ al[z, :nQ, x]
+                eb = re * re * (GRp + k2 * GR) @ be[z, :, x]
+                assert np.linalg.norm(fa[z, :, x] - ea) <= 1e-12 * np.linalg.norm(ea) + 1e-300
+                assert np.linalg.norm(fb[z, :, x] - eb) <= 1e-12 * np.linalg.norm(eb) + 1e-300
+        for f0, v in ((f0a, al[0, :, 0]), (f0b, be[0, :, 0])):
+            e0 = re * re * GRp @ v
+            assert np.linalg.norm(f0 - e0) <= 1e-12 * np.linalg.norm(e0) + 1e-300
+    o.destroy()
