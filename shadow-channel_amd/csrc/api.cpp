@@ -243,8 +243,7 @@ static void ddt_project_dev(Context &c, int i, const cplx *vin, cplx *vout,
     const Dims &d = c.d;
     long ip = i == 0 ? 3 : 3L * i, im = i == 0 ? 0 : 3L * (i - 1);
     launch_diff_scale(c, c.Ctmp2, mc(c, ip), mc(c, im), 1.0 / d.dt, c.csize);
-    launch_rebuild(c, c.Ctmp2, c.Ut);
-    launch_rebuild(c, vin, c.Ut2);
+    launch_rebuild2(c, c.Ctmp2, c.Ut, vin, c.Ut2);
     FieldSpec fs;
     fs.nf = 6;
     for (int s = 0; s < 3; ++s) {
@@ -679,8 +678,10 @@ int c_adjoint(int start, int end, sc_complex *AC_given, int Nz_dup, int Nvar_dup
             const cplx *w = wcached(c, count);
             if (!w) launch_rebuild(c, mc(c, count), c.U);
             pass2_adjoint(c, c.U, c.AU, c.U, w, lam_ready && w != nullptr);
-            launch_force(c, c.U, 1);
-            if (k < 2) launch_force(c, c.LU, 1, c.FLa, c.FLb);
+            if (k < 2)
+                launch_force2(c, 1, c.U, c.Fa, c.Fb, c.LU, c.FLa, c.FLb);
+            else
+                launch_force(c, c.U, 1);
             launch_solve(c, c.AC, c.Csol, k, 1, c.FLa, c.FLb, 0.0);
             launch_rebuild(c, c.Csol, c.AU);
         }

@@ -79,6 +79,9 @@ struct RebuildArgs {
     const int *abort_flag;
     unsigned *queue;                    // [8][nyt][2] task and exit counters, zero between launches
     int bpq;                            // blocks per (XCD, y-tile) queue
+    int nprob;                          // 2: a second rebuild C1 -> U1 in the same launch
+    const cplx *C1;
+    cplx *U1;
 };
 
 // U[z][0..4][y][x] from C[z][0..1][j][x]; 1-D grid over the (kz row, y-tile)
@@ -240,6 +243,7 @@ __global__ __launch_bounds__(256, SC_YT_WAVES) void k_rebuild(RebuildArgs a, con
 // fragments, and in the mean row only the n rows 0..3 are stored.
 struct RebuildTask {
     size_t offA, offB;
+    const double *cd;                   // coefficient array of the task's rebuild
 };
 
 __device__ __forceinline__ RebuildTask rebuild_task(const RebuildArgs &a, int zrow, int nt, int nl)
@@ -330,14 +334,15 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_rows(RebuildArgs a, const cp
 #pragma unroll
         for (int i = 0; i < PT; ++i) bs[threadIdx.x + 256 * i] = st[i];
     }
-    const double *Cd = (const double *) C;
     // task t of the queue: rows z0..z1-1 in order, nnt n-tiles each (one in
-    // the mean row)
+    // the mean row); with two rebuilds, tasks ntask1.. belong to the second
     const int zm = a.Nz / 2;
     const bool has_mean = zm >= z0 && zm < z1;
-    const int ntask = (z1 - z0) * nnt - (has_mean ? nnt - 1 : 0);
+    const int ntask1 = (z1 - z0) * nnt - (has_mean ? nnt - 1 : 0);
+    const int ntask = ntask1 * a.nprob;
     auto task_of = [&](int t, int &zrow, int &nt) {
-        const int tm = has_mean ? (zm - z0) * nnt : ntask;   // first task of the mean row
+        if (t >= ntask1) t -= ntask1;
+        const int tm = has_mean ? (zm - z0) * nnt : ntask1;  // first task of the mean row
         if (t < tm) {
             zrow = z0 + t / nnt;
             nt = t % nnt;
@@ -367,6 +372,7 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_rows(RebuildArgs a, const cp
     // it is clamped
     const size_t rstep = (size_t) 4 * a.Mx * 2;
     auto load = [&](const RebuildTask &t, int ch) {
+        const double *Cd = t.cd;
         const double *pa = Cd + t.offA + (size_t) kq * a.Mx * 2;
         const double *pb = Cd + t.offB + (size_t) kq * a.Mx * 2;
 #pragma unroll
@@ -429,18 +435,24 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_rows(RebuildArgs a, const cp
     };
     __syncthreads();
     (void) wave;
+    // operands of task t: offsets and the coefficient array of its rebuild
+    auto task_ops = [&](int t, int zr, int nt) {
+        RebuildTask r = rebuild_task(a, zr, nt, nl);
+        r.cd = (const double *) (t >= ntask1 ? a.C1 : C);
+        return r;
+    };
     int t = grab();
     if (t >= ntask) {
         t = ntask;
     } else {
         int zr, nt;
         task_of(t, zr, nt);
-        load(rebuild_task(a, zr, nt, nl), 0);
+        load(task_ops(t, zr, nt), 0);
     }
     while (t < ntask) {
         int zr, nt;
         task_of(t, zr, nt);
-        const RebuildTask cur = rebuild_task(a, zr, nt, nl);
+        const RebuildTask cur = task_ops(t, zr, nt);
         const bool mean = zr == zm;
         mf4 acc[5];
 #pragma unroll
@@ -455,14 +467,14 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_rows(RebuildArgs a, const cp
                 if (tn < ntask) {
                     int zn, ntn;
                     task_of(tn, zn, ntn);
-                    load(rebuild_task(a, zn, ntn, nl), 0);
+                    load(task_ops(tn, zn, ntn), 0);
                 }
             }
             __builtin_amdgcn_sched_barrier(0);
             chunk(mean, ch, acc);
             __builtin_amdgcn_sched_barrier(0);
         }
-        rebuild_store(a, zr, nt, yt * 16 + nl, kq, acc, U);
+        rebuild_store(a, zr, nt, yt * 16 + nl, kq, acc, t >= ntask1 ? a.U1 : U);
         t = tn;
     }
     // every wave of the queue's blocks exits here once; the last clears it
@@ -475,32 +487,42 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_rows(RebuildArgs a, const cp
     }
 }
 
-void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst)
+// one or two rebuilds (C1 null: one); the queued kernel runs both in one
+// launch (they share the staged basis), the chunked kernel one launch each
+void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx *U1)
 {
     const Dims &d = c.d;
-    // segments of kz rows: about two blocks per CU
+    // segments of kz rows for the chunked kernel
     const int nseg = std::max(1, std::min(d.Nz, 512 / c.nyt));
     // queued form: blocks per (XCD, y-tile), about two blocks per CU
     const int bpq = std::max(1, 64 / c.nyt);
+    const int nprob = C1 ? 2 : 1;
     RebuildArgs a{c.dFragR, c.dKx, c.dKz, d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, c.nksR, c.nyt,
-                  nseg, 2. * M_PI / d.Lx, 2. * M_PI / d.Lz, c.abort_flag, c.rb_queue, bpq};
-    double bytes = 16.0 * (double) (d.Nz - 1) * d.Mx * (2.0 * d.dimR + 5.0 * d.qpts);
+                  nseg, 2. * M_PI / d.Lx, 2. * M_PI / d.Lz, c.abort_flag, c.rb_queue, bpq,
+                  nprob, C1, U1};
+    double bytes = nprob * 16.0 * (double) (d.Nz - 1) * d.Mx * (2.0 * d.dimR + 5.0 * d.qpts);
     const hipStream_t st = c.stream;
     const int nks = c.nksR;
     const dim3 grows(8 * c.nyt * bpq), gchunk(xcd_grid(c.nyt * d.Nz));
     auto go = [=]() {
         if (nks == 8)
-            hipLaunchKernelGGL(k_rebuild_rows<8>, grows, dim3(256), 0, st, a, Csrc, Udst);
+            hipLaunchKernelGGL(k_rebuild_rows<8>, grows, dim3(256), 0, st, a, C0, U0);
         else if (nks == 16)
-            hipLaunchKernelGGL(k_rebuild_rows<16>, grows, dim3(256), 0, st, a, Csrc, Udst);
+            hipLaunchKernelGGL(k_rebuild_rows<16>, grows, dim3(256), 0, st, a, C0, U0);
         else if (nks == 24)
-            hipLaunchKernelGGL(k_rebuild_rows<24>, grows, dim3(256), 0, st, a, Csrc, Udst);
+            hipLaunchKernelGGL(k_rebuild_rows<24>, grows, dim3(256), 0, st, a, C0, U0);
         else
-            hipLaunchKernelGGL(k_rebuild, gchunk, dim3(256), 0, st, a, Csrc, Udst);
+            for (int p = 0; p < nprob; ++p)
+                hipLaunchKernelGGL(k_rebuild, gchunk, dim3(256), 0, st, a, p ? C1 : C0, p ? U1 : U0);
     };
     prof_begin(c, st, 4, bytes);
     go();
     prof_end(c, st, 4, go);
+}
+
+void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst)
+{
+    launch_rebuild2(c, Csrc, Udst, nullptr, nullptr);
 }
 
 // ------------------------------------------------------------ projection
@@ -655,7 +677,9 @@ __global__ __launch_bounds__(256, SC_YT_WAVES) void k_force(ForceArgs a, const c
 // through two register buffers of CK k-steps (the loads of chunk c+1, or of
 // the next task's chunk 0, issued before the products of chunk c).  The
 // products enter the accumulators in the same order as in k_force.
-// Grid: nit * bpi blocks, block b -> i-tile b / bpi.
+// Grid: nit * bpi blocks, block b -> i-tile b / bpi.  With nprob = 2 the
+// task range covers two projections sharing the staged basis (fields U0 ->
+// Fa0, Fb0, then U1 -> Fa1, Fb1).
 constexpr int FR_THREADS = 512;
 
 template <bool ADJ, int NKS>
@@ -669,8 +693,9 @@ struct ForceRowsShape {
 };
 
 template <bool ADJ, int NKS>
-__global__ __launch_bounds__(FR_THREADS) void k_force_rows(ForceArgs a, int bpi, const cplx *U,
-                                                           cplx *Fa, cplx *Fb)
+__global__ __launch_bounds__(FR_THREADS) void k_force_rows(ForceArgs a, int bpi, int nprob,
+                                                           const cplx *U0, cplx *Fa0, cplx *Fb0,
+                                                           const cplx *U1, cplx *Fa1, cplx *Fb1)
 {
     using SH = ForceRowsShape<ADJ, NKS>;
     constexpr int NB = SH::NB, NF = SH::NF, CK = SH::CK, NCH = SH::NCH;
@@ -683,7 +708,8 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_rows(ForceArgs a, int bpi,
     const int Mx = a.Mx, q = a.qpts, nR = a.dimR, nQ = a.dimQ;
     const int nnt = (Mx + 7) / 8;
     const int ntask = (a.Nz - 1) * nnt;                       // every row but the Nyquist row
-    const int t0 = lb * ntask / bpi, t1 = (lb + 1) * ntask / bpi;
+    const int nall = ntask * nprob;
+    const int t0 = lb * nall / bpi, t1 = (lb + 1) * nall / bpi;
     {
         // bases of the i-tile: fragF [nit][nksF][5][64] -> [NKS][NB][64]
         const double *frag = a.frag + (size_t) it * a.nks * 5 * 64;
@@ -705,11 +731,13 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_rows(ForceArgs a, int bpi,
     __syncthreads();
     // staged bases per k-step: Qpw, Qw, Qppw, Rw, Rpw (adjoint); Qpw, Qw, Rw (primal)
     struct Task {
-        int z, nt, xcol, part;
+        int z, nt, xcol, part, p;
         size_t off;
     };
     auto task_of = [&](int t) {
         Task k;
+        k.p = t >= ntask;                                     // which projection
+        t -= k.p * ntask;
         const int zr = t / nnt;
         k.z = zr < a.Nz / 2 ? zr : zr + 1;
         k.nt = t - zr * nnt;
@@ -718,10 +746,10 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_rows(ForceArgs a, int bpi,
         k.off = (size_t) k.z * 5 * q * Mx * 2 + (size_t) k.xcol * 2;
         return k;
     };
-    const double *Ud = (const double *) U;
     const size_t ss2 = (size_t) q * Mx * 2;                   // slot stride in doubles
     double r[2][NF][CK];
     auto load = [&](const Task &k, int ch, int buf) {
+        const double *Ud = (const double *) (k.p ? U1 : U0);
         const bool mean = k.z == 0 && k.xcol == 0;
         const int other = mean ? 0 : 1 - k.part;
 #pragma unroll
@@ -797,6 +825,7 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_rows(ForceArgs a, int bpi,
         }
         const int i = it * 16 + nl;
         if (i < nR) {
+            cplx *Fa = k.p ? Fa1 : Fa0, *Fb = k.p ? Fb1 : Fb0;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int x = k.nt * 8 + 2 * kq + h;
@@ -811,8 +840,9 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_rows(ForceArgs a, int bpi,
 }
 
 template <bool ADJ, int NKS>
-static void launch_force_rows(const ForceArgs &a, int nit, int ntask, const cplx *U, cplx *Fa,
-                              cplx *Fb, hipStream_t st)
+static void launch_force_rows(const ForceArgs &a, int nit, int ntask, int nprob, const cplx *U0,
+                              cplx *Fa0, cplx *Fb0, const cplx *U1, cplx *Fa1, cplx *Fb1,
+                              hipStream_t st)
 {
     using SH = ForceRowsShape<ADJ, NKS>;
     const size_t lds = sizeof(double) * NKS * SH::NB * 64;
@@ -825,18 +855,22 @@ static void launch_force_rows(const ForceArgs &a, int nit, int ntask, const cplx
     // blocks per i-tile: one 8-wave block per CU in total (the registers of
     // the wave's two field buffers allow no second block), never more than
     // the tasks
-    const int bpi = std::max(1, std::min(ntask, 256 / nit));
+    const int bpi = std::max(1, std::min(ntask * nprob, 256 / nit));
     hipLaunchKernelGGL((k_force_rows<ADJ, NKS>), dim3(nit * bpi), dim3(FR_THREADS), lds, st, a, bpi,
-                       U, Fa, Fb);
+                       nprob, U0, Fa0, Fb0, U1, Fa1, Fb1);
 }
 
-void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb)
+// one or two projections (U1 null: one); the staged-basis kernel runs both in
+// one launch, the chunked kernel one launch each
+void launch_force2(Context &c, int adj, const cplx *U0, cplx *Fa0, cplx *Fb0, const cplx *U1,
+                   cplx *Fa1, cplx *Fb1)
 {
     const Dims &d = c.d;
     ForceArgs a{c.dFragF, c.dKx, c.dKz, d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, c.nksF, c.nit,
                 2. * M_PI / d.Lx, 2. * M_PI / d.Lz, c.abort_flag};
     dim3 grid(xcd_grid(c.nit * d.Nz));
-    double bytes = 16.0 * (double) (d.Nz - 1) * d.Mx *
+    const int nprob = U1 ? 2 : 1;
+    double bytes = nprob * 16.0 * (double) (d.Nz - 1) * d.Mx *
                    ((adj ? 5.0 : 3.0) * d.qpts + d.dimQ + d.dimR);
     const hipStream_t st = c.stream;
     const int nks = c.nksF, nit = c.nit, ntask = (d.Nz - 1) * ((d.Mx + 7) / 8);
@@ -844,23 +878,39 @@ void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb)
         // staged-basis kernel for the k-step counts it is built for
         if (nks == 40 || nks == 28 || nks == 16) {
             if (adj) {
-                if (nks == 40) launch_force_rows<true, 40>(a, nit, ntask, Usrc, Fa, Fb, st);
-                else if (nks == 28) launch_force_rows<true, 28>(a, nit, ntask, Usrc, Fa, Fb, st);
-                else launch_force_rows<true, 16>(a, nit, ntask, Usrc, Fa, Fb, st);
+                if (nks == 40)
+                    launch_force_rows<true, 40>(a, nit, ntask, nprob, U0, Fa0, Fb0, U1, Fa1, Fb1, st);
+                else if (nks == 28)
+                    launch_force_rows<true, 28>(a, nit, ntask, nprob, U0, Fa0, Fb0, U1, Fa1, Fb1, st);
+                else
+                    launch_force_rows<true, 16>(a, nit, ntask, nprob, U0, Fa0, Fb0, U1, Fa1, Fb1, st);
             } else {
-                if (nks == 40) launch_force_rows<false, 40>(a, nit, ntask, Usrc, Fa, Fb, st);
-                else if (nks == 28) launch_force_rows<false, 28>(a, nit, ntask, Usrc, Fa, Fb, st);
-                else launch_force_rows<false, 16>(a, nit, ntask, Usrc, Fa, Fb, st);
+                if (nks == 40)
+                    launch_force_rows<false, 40>(a, nit, ntask, nprob, U0, Fa0, Fb0, U1, Fa1, Fb1, st);
+                else if (nks == 28)
+                    launch_force_rows<false, 28>(a, nit, ntask, nprob, U0, Fa0, Fb0, U1, Fa1, Fb1, st);
+                else
+                    launch_force_rows<false, 16>(a, nit, ntask, nprob, U0, Fa0, Fb0, U1, Fa1, Fb1, st);
             }
-        } else if (adj) {
-            hipLaunchKernelGGL(k_force<true>, grid, dim3(256), 0, st, a, Usrc, Fa, Fb);
-        } else {
-            hipLaunchKernelGGL(k_force<false>, grid, dim3(256), 0, st, a, Usrc, Fa, Fb);
+            return;
+        }
+        for (int p = 0; p < nprob; ++p) {
+            const cplx *U = p ? U1 : U0;
+            cplx *Fa = p ? Fa1 : Fa0, *Fb = p ? Fb1 : Fb0;
+            if (adj)
+                hipLaunchKernelGGL(k_force<true>, grid, dim3(256), 0, st, a, U, Fa, Fb);
+            else
+                hipLaunchKernelGGL(k_force<false>, grid, dim3(256), 0, st, a, U, Fa, Fb);
         }
     };
     prof_begin(c, st, 5, bytes);
     go();
     prof_end(c, st, 5, go);
+}
+
+void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb)
+{
+    launch_force2(c, adj, Usrc, Fa, Fb, nullptr, nullptr, nullptr);
 }
 
 void launch_force(Context &c, const cplx *Usrc, int adj) { launch_force(c, Usrc, adj, c.Fa, c.Fb); }

@@ -170,8 +170,13 @@ void launch_assemble_adj(Context &c, const cplx *SP, cplx *Udst);
 // into the five state slots of Udst; false when the tile does not fit
 bool launch_zfwd_adj(Context &c, const cplx *W2, cplx *Udst);
 void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst);
+// two rebuilds in one launch where the kernel allows it
+void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx *U1);
 void launch_force(Context &c, const cplx *Usrc, int adj);
 void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb);
+// two projections in one launch where the kernel allows it
+void launch_force2(Context &c, int adj, const cplx *U0, cplx *Fa0, cplx *Fb0, const cplx *U1,
+                   cplx *Fa1, cplx *Fb1);
 void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cplx *Fa2,
                   const cplx *Fb2, double target);
 // slot: first stored state of the step being checked (kept when it stops)
