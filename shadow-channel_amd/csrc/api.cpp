@@ -109,7 +109,7 @@ static void free_all(Context &c)
                     c.dRowsum, c.C, c.IC, c.AC, c.U, c.IU, c.AU, c.LU, c.Ut, c.Ut2, c.MC,
                     c.MIC, c.Csol, c.Ctmp, c.Ctmp2, c.Fa, c.Fb, c.FLa, c.FLb, c.W1, c.W2,
                     c.SP, c.flowdev, c.cflmax, c.dotpart, c.scal, c.abort_flag, c.pz.tw,
-                    c.px.tw, c.dFragR, c.dFragF, c.wcache, c.rb_queue};
+                    c.px.tw, c.dFragR, c.dFragF, c.dFragP, c.wcache, c.rb_queue};
     for (void *p : ptrs)
         if (p) (void) hipFree(p);
     for (int a = 0; a < 2; ++a)

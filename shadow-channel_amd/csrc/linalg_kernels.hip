@@ -49,6 +49,11 @@ __device__ __forceinline__ int xcd_tile(int n)
 
 static inline int xcd_grid(int n) { return 8 * ((n + 7) / 8); }
 
+// most blocks per XCD of the parity rebuild, split over its half-domain y-tiles
+#ifndef SC_RB_BLOCKS
+#define SC_RB_BLOCKS 64
+#endif
+
 // wave index within the block, in a scalar register: the compiler then
 // treats everything derived from it as uniform across the wave
 __device__ __forceinline__ int wave_id()
@@ -487,6 +492,186 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_rows(RebuildArgs a, const cp
     }
 }
 
+// Parity form of the queued rebuild.  The Gauss points are mirror images
+// (y[q-1-i] = -y[i], exactly) and every basis column has a parity, exactly:
+// Q_j, Qpp_j and R_j that of j, Qp_j and Rp_j the opposite.  With E and O
+// the sums over the even and the odd j at a point y_i of the half domain
+// i < (q+1)/2,
+//     U(y_i) = E + O,    U(y_{q-1-i}) = s (E - O),
+// s = +1 for Q, Qpp, R and -1 for Qp, Rp.  A task covers 16 half-domain
+// points and their mirrors with the 5 NKS products that the direct form
+// spends on 16 points, so the products halve.  For odd q the middle point is
+// y = 0, where the columns of the other parity vanish exactly and E + O is
+// stored once.  k-step s < NKS/2 holds the even rows j = 8s + 2(l/16), the
+// others the odd rows; the coefficients move in four quarters (even rows,
+// then odd rows, two halves each) through two register buffers, the next
+// quarter's loads issued before the current quarter's products.  Blocks and
+// the mean task as in k_rebuild_rows (a.nyt is the number of half-domain
+// y-tiles, a.frag the parity-ordered basis), but the tasks are assigned
+// statically: wave w of the bpq blocks of an (XCD, y-tile) takes the tasks
+// w, w + 4 bpq, ..., and issues its first coefficient loads before it
+// stages the basis.
+template <int NKS>
+__global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cplx *C, cplx *U)
+{
+    __shared__ double bs[NKS * 5 * 64];
+    if (*a.abort_flag) return;
+    const int xcd = blockIdx.x & 7, ib = blockIdx.x >> 3;
+    const int yt = ib % a.nyt;
+    const int z0 = xcd * a.Nz / 8, z1 = (xcd + 1) * a.Nz / 8;
+    const int lane = threadIdx.x & 63;
+    const int kq = lane >> 4, nl = lane & 15;
+    const int nnt = (a.Mx + 7) / 8, nR = a.dimR, M2 = a.Mx * 2;
+    const int tstep = 4 * a.bpq;
+    const int zm = a.Nz / 2;
+    const bool has_mean = zm >= z0 && zm < z1;
+    const int ntask1 = (z1 - z0) * nnt - (has_mean ? nnt - 1 : 0);
+    const int ntask = ntask1 * a.nprob;
+    auto task_of = [&](int t, int &zrow, int &nt) {
+        if (t >= ntask1) t -= ntask1;
+        const int tm = has_mean ? (zm - z0) * nnt : ntask1;
+        if (t < tm) {
+            zrow = z0 + t / nnt;
+            nt = t % nnt;
+        } else if (t == tm) {
+            zrow = zm;
+            nt = 0;
+        } else {
+            const int u = t - tm - 1;
+            zrow = zm + 1 + u / nnt;
+            nt = u % nnt;
+        }
+    };
+    constexpr int NKH = NKS / 2, CK = NKS / 4;
+    static_assert(CK % 2 == 0, "a quarter must be whole pairs of k-steps");
+    double ca[2][CK], cb[2][CK];
+    // quarter ch: parity ch / 2, k-steps (ch % 2) CK .. + CK - 1 of that
+    // parity, as buffer loads over the whole coefficient array: a lane
+    // offset plus a uniform k-step offset.  Rows past dimR - 1 (whose basis
+    // fragments are zero) read the next rows of the array, or zero past its
+    // end, so no row index is clamped.
+    const unsigned cbytes = (unsigned) ((size_t) a.Nz * 2 * nR * M2 * sizeof(double));
+    auto load = [&](const RebuildTask &t, int ch, int buf) {
+        __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *) t.cd, 0, cbytes, 0x00020000);
+        const int par = ch >> 1, s0 = (ch & 1) * CK;
+        const int row = (2 * kq + par) * M2;
+        const int va = (int) (unsigned) ((t.offA + row) * 8), vb = (int) (unsigned) ((t.offB + row) * 8);
+#pragma unroll
+        for (int s = 0; s < CK; ++s) {
+            const int so = (s0 + s) * 8 * M2 * 8;
+            ca[buf][s] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, va, so, 0));
+            cb[buf][s] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, vb, so, 0));
+        }
+    };
+    auto chunk = [&](bool mean, int ch, int buf, mf4 *acc) {
+        const double *xa = ca[buf], *xb = cb[buf];
+        const int sb = (ch >> 1) * NKH + (ch & 1) * CK;
+        if (mean) {
+#pragma unroll
+            for (int t = 0; t < CK; ++t) {
+                acc[3] = mfma_f64(xa[t], bs[((sb + t) * 5 + 3) * 64 + lane], acc[3]);
+                acc[4] = mfma_f64(xa[t], bs[((sb + t) * 5 + 4) * 64 + lane], acc[4]);
+            }
+            return;
+        }
+        double p0[10], p1[10];
+        auto ldb = [&](int s0, double *p) {
+#pragma unroll
+            for (int e = 0; e < 10; ++e) p[e] = bs[(s0 * 5 + e) * 64 + lane];
+        };
+        auto mm = [&](int t0, const double *p) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int t = t0 + u;
+                acc[0] = mfma_f64(xa[t], p[u * 5 + 0], acc[0]);
+                acc[1] = mfma_f64(xa[t], p[u * 5 + 1], acc[1]);
+                acc[2] = mfma_f64(xa[t], p[u * 5 + 2], acc[2]);
+                acc[3] = mfma_f64(xb[t], p[u * 5 + 3], acc[3]);
+                acc[4] = mfma_f64(xb[t], p[u * 5 + 4], acc[4]);
+            }
+        };
+        ldb(sb, p0);
+#pragma unroll
+        for (int t0 = 0; t0 < CK; t0 += 4) {
+            if (t0 + 2 < CK) ldb(sb + t0 + 2, p1);
+            __builtin_amdgcn_sched_barrier(0);
+            mm(t0, p0);
+            __builtin_amdgcn_sched_barrier(0);
+            if (t0 + 2 >= CK) break;
+            if (t0 + 4 < CK) ldb(sb + t0 + 4, p0);
+            __builtin_amdgcn_sched_barrier(0);
+            mm(t0 + 2, p1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+    auto task_ops = [&](int t, int zr, int nt) {
+        RebuildTask r = rebuild_task(a, zr, nt, nl);
+        r.cd = (const double *) (t >= ntask1 ? a.C1 : C);
+        return r;
+    };
+    const int hq = (a.qpts + 1) / 2;
+    const int y = yt * 16 + nl, ym = a.qpts - 1 - y;
+    int t = (ib / a.nyt) * 4 + wave_id();
+    if (t < ntask) {
+        int zr, nt;
+        task_of(t, zr, nt);
+        load(task_ops(t, zr, nt), 0, 0);
+    }
+    {
+        const double *frag = a.frag + (size_t) yt * NKS * 5 * 64;
+        constexpr int PT = NKS * 5 * 64 / 256;
+        static_assert(NKS * 5 * 64 % 256 == 0, "basis must split over the block");
+        double st[PT];
+#pragma unroll
+        for (int i = 0; i < PT; ++i) st[i] = frag[threadIdx.x + 256 * i];
+#pragma unroll
+        for (int i = 0; i < PT; ++i) bs[threadIdx.x + 256 * i] = st[i];
+    }
+    __syncthreads();
+    while (t < ntask) {
+        int zr, nt;
+        task_of(t, zr, nt);
+        const RebuildTask cur = task_ops(t, zr, nt);
+        const bool mean = zr == zm;
+        mf4 ae[5], ao[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            ae[m] = mf4_zero();
+            ao[m] = mf4_zero();
+        }
+        int tn = ntask;
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) {
+            if (ch < 3) {
+                load(cur, ch + 1, (ch + 1) & 1);
+            } else {
+                tn = t + tstep;
+                if (tn < ntask) {
+                    int zn, ntn;
+                    task_of(tn, zn, ntn);
+                    load(task_ops(tn, zn, ntn), 0, 0);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            chunk(mean, ch, ch & 1, ch < 2 ? ae : ao);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (y < hq) {
+            cplx *Ud = t >= ntask1 ? a.U1 : U;
+            // values at y (E + O), then at the mirror point (s (E - O))
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                const mf4 e = ae[m], o = ao[m];
+                ae[m] = e + o;
+                ao[m] = (m == 1 || m == 4) ? o - e : e - o;
+            }
+            rebuild_store(a, zr, nt, y, kq, ae, Ud);
+            if (ym != y) rebuild_store(a, zr, nt, ym, kq, ao, Ud);
+        }
+        t = tn;
+    }
+}
+
 // one or two rebuilds (C1 null: one); the queued kernel runs both in one
 // launch (they share the staged basis), the chunked kernel one launch each
 void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx *U1)
@@ -504,8 +689,28 @@ void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx 
     const hipStream_t st = c.stream;
     const int nks = c.nksR;
     const dim3 grows(8 * c.nyt * bpq), gchunk(xcd_grid(c.nyt * d.Nz));
+    const int nksp = c.nksP;
+    const bool par = c.ypar && (nksp == 8 || nksp == 16 || nksp == 24);
+    RebuildArgs ap = a;
+    ap.frag = c.dFragP;
+    ap.nyt = c.nht;
+    ap.nks = nksp;
+    {
+        // enough waves for one task each (of one rebuild), at most about
+        // two blocks per CU
+        const int ntmax = (d.Nz + 7) / 8 * ((d.Mx + 7) / 8);
+        ap.bpq = std::max(1, std::min((ntmax + 3) / 4, SC_RB_BLOCKS / c.nht));
+    }
+    const dim3 gpar(8 * c.nht * ap.bpq);
     auto go = [=]() {
-        if (nks == 8)
+        if (par) {
+            if (nksp == 8)
+                hipLaunchKernelGGL(k_rebuild_par<8>, gpar, dim3(256), 0, st, ap, C0, U0);
+            else if (nksp == 16)
+                hipLaunchKernelGGL(k_rebuild_par<16>, gpar, dim3(256), 0, st, ap, C0, U0);
+            else
+                hipLaunchKernelGGL(k_rebuild_par<24>, gpar, dim3(256), 0, st, ap, C0, U0);
+        } else if (nks == 8)
             hipLaunchKernelGGL(k_rebuild_rows<8>, grows, dim3(256), 0, st, a, C0, U0);
         else if (nks == 16)
             hipLaunchKernelGGL(k_rebuild_rows<16>, grows, dim3(256), 0, st, a, C0, U0);

@@ -97,8 +97,14 @@ struct Context {
     // (lane l of k-step s, tile t: row 4s + l/16, column 16t + l%16), zero-padded:
     //   fragR [nyt][nksR][5][64]: Q, Qp, Qpp, R, Rp at (j = 4s + l/16, y = 16t + l%16)
     //   fragF [nit][nksF][5][64]: Qpw, Qw, Qppw, Rw, Rpw at (y = 4s + l/16, i = 16t + l%16)
-    double *dFragR = nullptr, *dFragF = nullptr;
-    int nyt = 0, nksR = 0, nit = 0, nksF = 0;
+    //   fragP [nht][nksP][5][64]: the same bases on the half domain i < (q+1)/2
+    //   (y = 16t + l%16), k-steps s < nksP/2 at the even rows j = 8s + 2(l/16),
+    //   the others at the odd rows j = 8(s - nksP/2) + 2(l/16) + 1
+    double *dFragR = nullptr, *dFragF = nullptr, *dFragP = nullptr;
+    int nyt = 0, nksR = 0, nit = 0, nksF = 0, nht = 0, nksP = 0;
+    // the Gauss points are mirror images (y[q-1-i] = -y[i]) and every basis
+    // column has the parity of its index (derivatives the opposite), exactly
+    bool ypar = false;
     unsigned *rb_queue = nullptr;               // rebuild task queues [8][nyt][2], zero between launches
     // LU factors of the implicit operators, per (direction, stage), as the
     // LDS images of the solve's blocks of sbt columns (col = z Mx + x):

@@ -269,6 +269,44 @@ int setup_host(Context &c)
                             ff[(((size_t) t * nksF + s) * 5 + m) * 64 + l] = (*Wt[m])[(size_t) i * q + y];
                     }
         c.nyt = nyt; c.nksR = nksR; c.nit = nit; c.nksF = nksF;
+        // parity of the bases on the mirrored Gauss points, checked exactly:
+        // Q, Qpp, R of column j have the parity of j, Qp and Rp the opposite
+        const int hq = (q + 1) / 2;
+        bool par = true;
+        for (int i = 0; i < hq && par; ++i) {
+            const int im = q - 1 - i;
+            par = c.y[im] == -c.y[i];
+            for (int m = 0; m < 5 && par; ++m) {
+                const int dim = m < 3 ? nQ : nR;
+                const bool deriv = m == 1 || m == 4;
+                for (int j = 0; j < dim && par; ++j) {
+                    const double b = (*B[m])[(size_t) i * dim + j];
+                    const double bm = (*B[m])[(size_t) im * dim + j];
+                    par = bm == (((j & 1) != 0) != deriv ? -b : b);
+                }
+            }
+        }
+        // k-steps per parity: whole groups of four (the kernel moves its
+        // coefficients in quarters of two-k-step pairs)
+        const int nkh = ((((nR + 1) / 2 + 3) / 4) + 3) / 4 * 4;
+        c.ypar = par;
+        c.nht = (hq + 15) / 16;
+        c.nksP = 2 * nkh;
+        std::vector<double> fp((size_t) c.nht * c.nksP * 5 * 64, 0.0);
+        for (int t = 0; t < c.nht; ++t)
+            for (int s = 0; s < c.nksP; ++s)
+                for (int m = 0; m < 5; ++m)
+                    for (int l = 0; l < 64; ++l) {
+                        const int odd = s >= nkh, so = s - odd * nkh;
+                        const int y = 16 * t + l % 16, j = 8 * so + 2 * (l / 16) + odd;
+                        const int dim = m < 3 ? nQ : nR;
+                        if (y < hq && j < dim)
+                            fp[(((size_t) t * c.nksP + s) * 5 + m) * 64 + l] = (*B[m])[(size_t) y * dim + j];
+                    }
+        if (upload(&c.dFragP, fp)) {
+            last_error() = "device allocation of transform operands failed";
+            return -1;
+        }
         const size_t qbytes = sizeof(unsigned) * 2 * 8 * (size_t) nyt;
         if (hipMalloc(&c.rb_queue, qbytes) != hipSuccess || hipMemset(c.rb_queue, 0, qbytes) != hipSuccess) {
             last_error() = "device allocation of the rebuild queues failed";
