@@ -36,11 +36,19 @@ def log(*a):
 
 
 def fwd_adj(cc, C0, nsteps):
+    """Host-array interface (c_primal / c_adjoint): includes the transfers of
+    the state and the adjoint state between host and device."""
     C = C0.copy()
     cc.c_primal(0, C)
     AC = np.zeros_like(C0)
     cc.c_adjoint(nsteps, 0, AC, 0, 1.0)
     return C, AC
+
+
+def fwd_adj_dev(cc, Cd, ACd, nsteps):
+    """Device-resident interface: the states stay in HBM."""
+    cc.sc_primal_dev(0, Cd)
+    cc.sc_adjoint_dev(nsteps, 0, ACd, 1.0)
 
 
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs
@@ -130,12 +138,24 @@ def run():
     assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
                      c["dt"], K) == 0
     log("timing %d fwd+adj steps" % K)
+    # inputs resident in HBM before the timed region
+    Cd, ACd = cc.DeviceArray(C0), cc.DeviceArray(np.zeros_like(C0))
     cc.sc_synchronize()
     t0 = time.perf_counter()
-    C, AC = fwd_adj(cc, C0, K)
+    fwd_adj_dev(cc, Cd, ACd, K)
     cc.sc_synchronize()
     elapsed = time.perf_counter() - t0
+    C, AC = Cd.download(), ACd.download()
+    Cd.free()
+    ACd.free()
     assert cc.last_status() == 0 and np.all(np.isfinite(C)) and np.all(np.isfinite(AC))
+    # the same window through the host-array interface (PCIe transfers of
+    # the states included), reported beside the resident figure
+    cc.sc_synchronize()
+    t0 = time.perf_counter()
+    Ch, ACh = fwd_adj(cc, C0, K)
+    elapsed_h = time.perf_counter() - t0
+    assert np.array_equal(Ch, C) and np.array_equal(ACh, AC)
     cc.c_destroy()
 
     # roofline of one kernel class: one fwd+adj step records its launches,
@@ -167,7 +187,8 @@ def run():
                                "(strength 1), state resident in HBM",
                    "grid": "64x64x97 (96x145x96 dealiased)", "Lx": "2pi", "Lz": "pi",
                    "Re": c["Re"], "dt": c["dt"], "meanU": c["flux"] / 2,
-                   "window_steps": K, "precision": "fp64"},
+                   "window_steps": K, "precision": "fp64",
+                   "host_array_steps_per_s": round(K / elapsed_h, 3)},
         "roofline": roof,
     }
     if not args.no_cpu_baseline:

@@ -127,6 +127,13 @@ double sc_prof_bytes(void);
 int sc_prof_replay(int reps, double *avg_ms, double *bytes);
 /* Wait until all work queued by the library has finished. */
 int sc_synchronize(void);
+/* c_primal / c_adjoint for callers that keep the state in device memory:
+   C_dev / AC_dev are device pointers to [Nz][2][dimR][Nx/2] complex arrays,
+   read at the start and overwritten with the final state (device-to-device
+   copies; no host transfer).  sc_primal_dev always starts from the given
+   state (no laminar start). */
+int sc_primal_dev(int ru_steps, sc_complex *C_dev);
+int sc_adjoint_dev(int start_step, int end_step, sc_complex *AC_dev, double strength);
 /* The primal keeps, per stored stage state, its velocity and vorticity after
    the z transform, so that the tangent and adjoint need not rebuild and
    transform that state again (capacity traded for recomputation; the results

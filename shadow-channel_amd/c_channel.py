@@ -76,6 +76,10 @@ _L.sc_field_cache_active.argtypes = []
 _L.sc_field_cache_active.restype = _i
 _L.sc_tangent_forcing.argtypes = [_i, _c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p]
 _L.sc_tangent_forcing.restype = _i
+_L.sc_primal_dev.argtypes = [_i, _c.c_void_p]
+_L.sc_primal_dev.restype = _i
+_L.sc_adjoint_dev.argtypes = [_i, _i, _c.c_void_p, _d]
+_L.sc_adjoint_dev.restype = _i
 
 SC_EINVAL = -3
 SC_ENOTREADY = -2
@@ -137,6 +141,61 @@ def c_adjoint(start_step, end_step, AC_given, inhomo, strength):
     AC = _inplace(AC_given, np.complex128, "AC_given")
     _check(_L.c_adjoint(int(start_step), int(end_step), AC.ctypes.data, *AC.shape, int(inhomo),
                         float(strength)), "c_adjoint")
+
+
+class DeviceArray:
+    """Device copy of a complex128 numpy array, allocated through the HIP
+    runtime the library itself uses (for sc_primal_dev / sc_adjoint_dev)."""
+
+    _hip = None
+
+    def __init__(self, host):
+        if DeviceArray._hip is None:
+            h = _c.CDLL("libamdhip64.so.7")      # the runtime already loaded by the library
+            h.hipMalloc.argtypes = [_c.POINTER(_vp), _c.c_size_t]
+            h.hipMemcpy.argtypes = [_vp, _vp, _c.c_size_t, _i]
+            h.hipFree.argtypes = [_vp]
+            DeviceArray._hip = h
+        host = np.ascontiguousarray(host, dtype=np.complex128)
+        self.shape, self.nbytes = host.shape, host.nbytes
+        self.ptr = _vp()
+        if DeviceArray._hip.hipMalloc(_c.byref(self.ptr), self.nbytes) != 0:
+            raise MemoryError("hipMalloc of %d bytes failed" % self.nbytes)
+        self.upload(host)
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=np.complex128)
+        assert host.nbytes == self.nbytes
+        if DeviceArray._hip.hipMemcpy(self.ptr, host.ctypes.data, self.nbytes, 1) != 0:
+            raise RuntimeError("hipMemcpy host to device failed")
+
+    def download(self):
+        out = np.empty(self.shape, dtype=np.complex128)
+        if DeviceArray._hip.hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2) != 0:
+            raise RuntimeError("hipMemcpy device to host failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            DeviceArray._hip.hipFree(self.ptr)
+            self.ptr = _vp()
+
+
+def _devptr(x):
+    return x.ptr if isinstance(x, DeviceArray) else _vp(int(x))
+
+
+def sc_primal_dev(ru_steps, C_dev):
+    """c_primal on a device-resident state: C_dev is a DeviceArray (or the
+    device address) of a [Nz][2][dimR][Nx/2] complex128 array, overwritten
+    with the final state."""
+    _check(_L.sc_primal_dev(int(ru_steps), _devptr(C_dev)), "sc_primal_dev")
+
+
+def sc_adjoint_dev(start_step, end_step, AC_dev, strength):
+    """c_adjoint on a device-resident adjoint state (see sc_primal_dev)."""
+    _check(_L.sc_adjoint_dev(int(start_step), int(end_step), _devptr(AC_dev), float(strength)),
+           "sc_adjoint_dev")
 
 
 def c_ddt_project(i_step, IC_given):

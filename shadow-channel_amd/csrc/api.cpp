@@ -109,7 +109,7 @@ static void free_all(Context &c)
                     c.dRowsum, c.C, c.IC, c.AC, c.U, c.IU, c.AU, c.LU, c.Ut, c.Ut2, c.MC,
                     c.MIC, c.Csol, c.Ctmp, c.Ctmp2, c.Fa, c.Fb, c.FLa, c.FLb, c.W1, c.W2,
                     c.SP, c.flowdev, c.cflmax, c.dotpart, c.scal, c.abort_flag, c.pz.tw,
-                    c.px.tw, c.dFragR, c.dFragF, c.dFragP, c.wcache, c.rb_queue};
+                    c.px.tw, c.dFragR, c.dFragF, c.dFragP, c.dFragQ, c.wcache, c.rb_queue};
     for (void *p : ptrs)
         if (p) (void) hipFree(p);
     for (int a = 0; a < 2; ++a)
@@ -521,17 +521,18 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
     return SC_OK;
 }
 
-int c_primal(int ru_steps, sc_complex *C_given, int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_dup)
+}   // extern "C"
+
+// primal run from a given state (host or device memory, per `in`/`out`) or
+// from the laminar start (C_given null)
+static int primal_run(int ru_steps, sc_complex *C_given, hipMemcpyKind in, hipMemcpyKind out)
 {
     Context &c = ctx();
-    if (!c.ok) return SC_ENOTREADY;
     const Dims &d = c.d;
-    bool given = C_given && !(Nz_dup == 0 && Nvar_dup == 0 && Ny_dup == 0 && Nx_dup == 0);
-    if (given && !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup)) return SC_EINVAL;
-    if (ru_steps < 0) return SC_EINVAL;
+    const bool given = C_given != nullptr;
     const size_t cb = sizeof(cplx) * c.csize;
     if (given) {
-        SC_HIP(hipMemcpyAsync(c.C, C_given, cb, hipMemcpyHostToDevice, c.stream));
+        SC_HIP(hipMemcpyAsync(c.C, C_given, cb, in, c.stream));
         launch_rebuild(c, c.C, c.U);
     } else {
         // laminar Poiseuille + rand() perturbation at Gauss points; C by
@@ -593,8 +594,28 @@ int c_primal(int ru_steps, sc_complex *C_given, int Nz_dup, int Nvar_dup, int Ny
         if (slot >= 1 && slot <= last)
             SC_HIP(hipMemset(mc(c, slot), 0, cb * (size_t) (last - slot + 1)));
     }
-    if (given) SC_HIP(hipMemcpy(C_given, c.C, cb, hipMemcpyDeviceToHost));
+    if (given) SC_HIP(hipMemcpy(C_given, c.C, cb, out));
     return SC_OK;
+}
+
+extern "C" {
+
+int c_primal(int ru_steps, sc_complex *C_given, int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_dup)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    bool given = C_given && !(Nz_dup == 0 && Nvar_dup == 0 && Ny_dup == 0 && Nx_dup == 0);
+    if (given && !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup)) return SC_EINVAL;
+    if (ru_steps < 0) return SC_EINVAL;
+    return primal_run(ru_steps, given ? C_given : nullptr, hipMemcpyHostToDevice, hipMemcpyDeviceToHost);
+}
+
+int sc_primal_dev(int ru_steps, sc_complex *C_dev)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!C_dev || ru_steps < 0) return SC_EINVAL;
+    return primal_run(ru_steps, C_dev, hipMemcpyDeviceToDevice, hipMemcpyDeviceToDevice);
 }
 
 int c_tangent(int start, int end, sc_complex *IC_given, int Nz_dup, int Nvar_dup, int Ny_dup,
@@ -641,20 +662,25 @@ int c_tangent(int start, int end, sc_complex *IC_given, int Nz_dup, int Nvar_dup
     return SC_OK;
 }
 
-int c_adjoint(int start, int end, sc_complex *AC_given, int Nz_dup, int Nvar_dup, int Ny_dup,
-              int Nx_dup, int inhomo, double strength)
+}   // extern "C"
+
+static int adjoint_range_ok(const Context &c, int start, int end)
 {
-    (void) inhomo;   // unused by the reference as well (channel/channel.c:596-696)
-    Context &c = ctx();
-    if (!c.ok) return SC_ENOTREADY;
-    const Dims &d = c.d;
-    if (!AC_given || !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup)) return SC_EINVAL;
-    if (!(end < start) || end < 0 || start > d.nsteps) {
+    if (!(end < start) || end < 0 || start > c.d.nsteps) {
         last_error() = "adjoint requires 0 <= end_step < start_step <= nsteps";
-        return SC_EINVAL;
+        return 0;
     }
+    return 1;
+}
+
+// adjoint run; AC_given in host or device memory per `in`/`out`
+static int adjoint_run(int start, int end, sc_complex *AC_given, double strength, hipMemcpyKind in,
+                       hipMemcpyKind out)
+{
+    Context &c = ctx();
+    const Dims &d = c.d;
     const size_t cb = sizeof(cplx) * c.csize;
-    SC_HIP(hipMemcpyAsync(c.AC, AC_given, cb, hipMemcpyHostToDevice, c.stream));
+    SC_HIP(hipMemcpyAsync(c.AC, AC_given, cb, in, c.stream));
     SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
     launch_rebuild(c, c.AC, c.AU);
     const double s = strength * d.dt;
@@ -689,8 +715,30 @@ int c_adjoint(int start, int end, sc_complex *AC_given, int Nz_dup, int Nvar_dup
     }
     int rc = stream_sync(c);
     if (rc) return rc;
-    SC_HIP(hipMemcpy(AC_given, c.AC, cb, hipMemcpyDeviceToHost));
+    SC_HIP(hipMemcpy(AC_given, c.AC, cb, out));
     return SC_OK;
+}
+
+extern "C" {
+
+int c_adjoint(int start, int end, sc_complex *AC_given, int Nz_dup, int Nvar_dup, int Ny_dup,
+              int Nx_dup, int inhomo, double strength)
+{
+    (void) inhomo;   // unused by the reference as well (channel/channel.c:596-696)
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!AC_given || !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup)) return SC_EINVAL;
+    if (!adjoint_range_ok(c, start, end)) return SC_EINVAL;
+    return adjoint_run(start, end, AC_given, strength, hipMemcpyHostToDevice, hipMemcpyDeviceToHost);
+}
+
+int sc_adjoint_dev(int start, int end, sc_complex *AC_dev, double strength)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!AC_dev) return SC_EINVAL;
+    if (!adjoint_range_ok(c, start, end)) return SC_EINVAL;
+    return adjoint_run(start, end, AC_dev, strength, hipMemcpyDeviceToDevice, hipMemcpyDeviceToDevice);
 }
 
 double c_ddt_project(int i_step, sc_complex *IC_given, int Nz_dup, int Nvar_dup, int Ny_dup,

@@ -1044,6 +1044,218 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_rows(ForceArgs a, int bpi,
     }
 }
 
+// Parity form of the staged-basis projection.  The weighted basis functions
+// have the parity of the Gauss-point mirror (Qw, Qppw, Rw of row i that of
+// i, Qpw and Rpw the opposite), so over a mirrored pair of points
+//     W(y) v(y) + W(-y) v(-y) = W(y) (v(y) + v(-y))  or  W(y) (v(y) - v(-y)),
+// and a projection onto 16 even and 16 odd rows i takes the half domain's
+// points once each: the products of one k-step feed both row tiles.  Each
+// lane forms the pointwise combinations (P, Hy, P2, Z, Z2 as in k_force) at
+// y and at its mirror, then their sum S and difference D:
+//     even rows:  Qpw D_P + Qw S_Y + Qppw S_Z,   Rw S_P2 + Rpw D_Z2
+//     odd rows:   Qpw S_P + Qw D_Y + Qppw D_Z,   Rw D_P2 + Rpw S_Z2
+// The middle point of an odd qpts (y = 0) is its own mirror: its staged
+// weights are halved, so that S = 2 v(0) gives the weight once, and the
+// functions that D multiplies vanish there.  A block owns one pair of row
+// tiles (even rows 2(16p + n), odd rows 2(16p + n) + 1), stages their
+// weighted bases [NKS][2][NB][64] and runs tasks as k_force_rows.  Field
+// values are read with buffer loads over the whole field array: the points
+// past the half domain, whose staged weights are zero, read values in or
+// past the array (zero past its end) and need no clamp.
+template <bool ADJ, int NKS>
+struct ForceParShape {
+    static constexpr int NB = ADJ ? 5 : 3;
+    static constexpr int NF = ADJ ? 5 : 3;
+    static constexpr int CKMAX = ADJ ? 2 : 4;
+    static constexpr int pick(int ck) { return (NKS % ck == 0 && (NKS / ck) % 2 == 0) ? ck : pick(ck - 1); }
+    static constexpr int CK = pick(CKMAX), NCH = NKS / CK;
+};
+template <bool ADJ, int NKS>
+__global__ __launch_bounds__(FR_THREADS) void k_force_par(ForceArgs a, int bpi, int nprob,
+                                                          const cplx *U0, cplx *Fa0, cplx *Fb0,
+                                                          const cplx *U1, cplx *Fa1, cplx *Fb1)
+{
+    using SH = ForceParShape<ADJ, NKS>;
+    constexpr int NB = SH::NB, NF = SH::NF, CK = SH::CK, NCH = SH::NCH;
+    static_assert(NCH % 2 == 0 && CK >= 1, "chunks alternate between two buffers");
+    extern __shared__ double bsf[];
+    if (*a.abort_flag) return;
+    const int tp = blockIdx.x / bpi, lb = blockIdx.x - tp * bpi;
+    const int lane = threadIdx.x & 63, wave = wave_id();
+    const int kq = lane >> 4, nl = lane & 15;
+    const int Mx = a.Mx, q = a.qpts, nR = a.dimR, nQ = a.dimQ;
+    const int nnt = (Mx + 7) / 8;
+    const int ntask = (a.Nz - 1) * nnt;
+    const int nall = ntask * nprob;
+    const int t0 = lb * nall / bpi, t1 = (lb + 1) * nall / bpi;
+    {
+        // fragQ [npair][NKS][2][5][64] -> [NKS][2][NB][64]
+        const double *frag = a.frag + (size_t) tp * NKS * 2 * 5 * 64;
+        constexpr int TOT = NKS * 2 * NB * 64, PT = (TOT + FR_THREADS - 1) / FR_THREADS;
+        double st[PT];
+#pragma unroll
+        for (int i = 0; i < PT; ++i) {
+            const int e = min((int) threadIdx.x + FR_THREADS * i, TOT - 1);
+            const int sp = e / (NB * 64), r = e - sp * NB * 64, m = r >> 6, l = r & 63;
+            const int mm = ADJ ? m : (m == 2 ? 3 : m);       // primal: Qpw, Qw, Rw
+            st[i] = frag[((size_t) sp * 5 + mm) * 64 + l];
+        }
+#pragma unroll
+        for (int i = 0; i < PT; ++i) {
+            const int e = threadIdx.x + FR_THREADS * i;
+            if (e < TOT) bsf[e] = st[i];
+        }
+    }
+    __syncthreads();
+    struct Task {
+        int z, nt, xcol, part, p;
+        unsigned off;                                         // bytes: row z, column xcol
+    };
+    auto task_of = [&](int t) {
+        Task k;
+        k.p = t >= ntask;
+        t -= k.p * ntask;
+        const int zr = t / nnt;
+        k.z = zr < a.Nz / 2 ? zr : zr + 1;
+        k.nt = t - zr * nnt;
+        k.xcol = min(k.nt * 8 + ncol_x(nl), Mx - 1);
+        k.part = ncol_part(nl);
+        k.off = (unsigned) (((size_t) k.z * 5 * q * Mx + k.xcol) * 16);
+        return k;
+    };
+    const unsigned ubytes = (unsigned) ((size_t) a.Nz * 5 * q * Mx * 16);
+    const unsigned rowb = (unsigned) Mx * 16, slotb = (unsigned) q * rowb;
+    // r[buf][0: y, 1: mirror][field][k-step]
+    double r[2][2][NF][CK];
+    auto load = [&](const Task &k, int ch, int buf) {
+        __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void *) (k.p ? U1 : U0), 0, ubytes, 0x00020000);
+        const bool mean = k.z == 0 && k.xcol == 0;
+        const unsigned other = k.off + (mean ? 0 : 1 - k.part) * 8, own = k.off + k.part * 8;
+        // lane offsets for the point and the part, uniform slot offsets
+        auto ld = [&](unsigned v, unsigned slot) {
+            return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int) v, (int) (slot * slotb), 0));
+        };
+#pragma unroll
+        for (int s = 0; s < CK; ++s) {
+            const int y = (ch * CK + s) * 4 + kq, ym = q - 1 - y;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const unsigned e = (unsigned) (h ? ym : y) * rowb;
+                r[buf][h][1][s] = ld(e + other, 0);
+                r[buf][h][0][s] = ld(e + other, 1);
+                r[buf][h][2][s] = ld(e + own, 2);
+                if (ADJ) {
+                    r[buf][h][3 % NF][s] = ld(e + other, 3);
+                    r[buf][h][4 % NF][s] = ld(e + other, 4);
+                }
+            }
+        }
+    };
+    int t = t0 + wave;
+    if (t < t1) load(task_of(t), 0, 0);
+    while (t < t1) {
+        const Task k = task_of(t);
+        const bool mean = k.z == 0 && k.xcol == 0;
+        const double kz = wave_kz(k.z, a.Nz, a.az);
+        const double kx = wave_kx(k.xcol, a.ax), k2 = kz * kz + kx * kx;
+        const double sg = k.part ? 1.0 : -1.0;
+        mf4 aAe = mf4_zero(), aBe = mf4_zero(), aAo = mf4_zero(), aBo = mf4_zero();
+        const int tn = t + 8;
+        auto chunk = [&](int ch, int buf) {
+            const double *bc = bsf + (size_t) ch * CK * 2 * NB * 64 + lane;
+#pragma unroll
+            for (int u = 0; u < CK; ++u) {
+                double vP[2], vY[2], vP2[2], vZ[2] = {0.0, 0.0}, vZ2[2] = {0.0, 0.0};
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const double hx = r[buf][h][0][u], hz = r[buf][h][1][u], hy = r[buf][h][2][u];
+                    if (mean) {
+                        vP[h] = 0.0;
+                        vY[h] = 0.0;
+                        vP2[h] = k.part ? hz : hx;
+                        if (ADJ) vZ2[h] = k.part ? -r[buf][h][3 % NF][u] : r[buf][h][4 % NF][u];
+                    } else {
+                        vP[h] = sg * (kx * hx + kz * hz);
+                        vY[h] = -k2 * hy;
+                        vP2[h] = sg * (kz * hx - kx * hz);
+                        if (ADJ) {
+                            vZ[h] = -sg * (kz * r[buf][h][3 % NF][u] - kx * r[buf][h][4 % NF][u]);
+                            vZ2[h] = sg * (kz * r[buf][h][4 % NF][u] + kx * r[buf][h][3 % NF][u]);
+                        }
+                    }
+                }
+                const double sP = vP[0] + vP[1], dP = vP[0] - vP[1];
+                const double sY = vY[0] + vY[1], dY = vY[0] - vY[1];
+                const double sP2 = vP2[0] + vP2[1], dP2 = vP2[0] - vP2[1];
+                const double *be = bc + u * 2 * NB * 64, *bo = be + NB * 64;
+                aAe = mfma_f64(dP, be[0], aAe);
+                aAo = mfma_f64(sP, bo[0], aAo);
+                aBe = mfma_f64(sP2, be[(ADJ ? 3 : 2) * 64], aBe);
+                aBo = mfma_f64(dP2, bo[(ADJ ? 3 : 2) * 64], aBo);
+                aAe = mfma_f64(sY, be[64], aAe);
+                aAo = mfma_f64(dY, bo[64], aAo);
+                if (ADJ) {
+                    const double sZ = vZ[0] + vZ[1], dZ = vZ[0] - vZ[1];
+                    const double sZ2 = vZ2[0] + vZ2[1], dZ2 = vZ2[0] - vZ2[1];
+                    aAe = mfma_f64(sZ, be[2 * 64], aAe);
+                    aAo = mfma_f64(dZ, bo[2 * 64], aAo);
+                    aBe = mfma_f64(dZ2, be[4 * 64], aBe);
+                    aBo = mfma_f64(sZ2, bo[4 * 64], aBo);
+                }
+            }
+        };
+#pragma unroll 1
+        for (int ch = 0; ch < NCH; ch += 2) {
+            load(k, ch + 1, 1);
+            __builtin_amdgcn_sched_barrier(0);
+            chunk(ch, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            if (ch + 2 < NCH)
+                load(k, ch + 2, 0);
+            else if (tn < t1)
+                load(task_of(tn), 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            chunk(ch + 1, 1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        cplx *Fa = k.p ? Fa1 : Fa0, *Fb = k.p ? Fb1 : Fb0;
+#pragma unroll
+        for (int par = 0; par < 2; ++par) {
+            const int i = 2 * (tp * 16 + nl) + par;
+            if (i >= nR) continue;
+            const mf4 &A = par ? aAo : aAe, &B = par ? aBo : aBe;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int x = k.nt * 8 + 2 * kq + h;
+                if (x >= Mx) continue;
+                if (i < nQ && !(k.z == 0 && x == 0))
+                    Fa[((size_t) k.z * nQ + i) * Mx + x] = make_double2(A[2 * h], A[2 * h + 1]);
+                Fb[((size_t) k.z * nR + i) * Mx + x] = make_double2(B[2 * h], B[2 * h + 1]);
+            }
+        }
+        t = tn;
+    }
+}
+
+template <bool ADJ, int NKS>
+static void launch_force_par(const ForceArgs &a, int npair, int ntask, int nprob, const cplx *U0,
+                             cplx *Fa0, cplx *Fb0, const cplx *U1, cplx *Fa1, cplx *Fb1,
+                             hipStream_t st)
+{
+    using SH = ForceParShape<ADJ, NKS>;
+    const size_t lds = sizeof(double) * NKS * 2 * SH::NB * 64;
+    static bool attr = false;
+    if (!attr) {
+        (void) hipFuncSetAttribute((const void *) k_force_par<ADJ, NKS>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+        attr = true;
+    }
+    const int bpi = std::max(1, std::min(ntask * nprob, 256 / npair));
+    hipLaunchKernelGGL((k_force_par<ADJ, NKS>), dim3(npair * bpi), dim3(FR_THREADS), lds, st, a, bpi,
+                       nprob, U0, Fa0, Fb0, U1, Fa1, Fb1);
+}
+
 template <bool ADJ, int NKS>
 static void launch_force_rows(const ForceArgs &a, int nit, int ntask, int nprob, const cplx *U0,
                               cplx *Fa0, cplx *Fb0, const cplx *U1, cplx *Fa1, cplx *Fb1,
@@ -1079,7 +1291,30 @@ void launch_force2(Context &c, int adj, const cplx *U0, cplx *Fa0, cplx *Fb0, co
                    ((adj ? 5.0 : 3.0) * d.qpts + d.dimQ + d.dimR);
     const hipStream_t st = c.stream;
     const int nks = c.nksF, nit = c.nit, ntask = (d.Nz - 1) * ((d.Mx + 7) / 8);
+    const int nksq = c.nksQ, npair = c.npair;
+    const bool par = c.ypar && (nksq == 8 || nksq == 16 || nksq == 20 || nksq == 28);
+    ForceArgs ap = a;
+    ap.frag = c.dFragQ;
+    ap.nks = nksq;
+    ap.nit = npair;
     auto go = [=]() {
+        if (par) {
+#define SC_FORCE_PAR(A, N) \
+    launch_force_par<A, N>(ap, npair, ntask, nprob, U0, Fa0, Fb0, U1, Fa1, Fb1, st)
+            if (adj) {
+                if (nksq == 8) SC_FORCE_PAR(true, 8);
+                else if (nksq == 16) SC_FORCE_PAR(true, 16);
+                else if (nksq == 20) SC_FORCE_PAR(true, 20);
+                else SC_FORCE_PAR(true, 28);
+            } else {
+                if (nksq == 8) SC_FORCE_PAR(false, 8);
+                else if (nksq == 16) SC_FORCE_PAR(false, 16);
+                else if (nksq == 20) SC_FORCE_PAR(false, 20);
+                else SC_FORCE_PAR(false, 28);
+            }
+#undef SC_FORCE_PAR
+            return;
+        }
         // staged-basis kernel for the k-step counts it is built for
         if (nks == 40 || nks == 28 || nks == 16) {
             if (adj) {

@@ -100,8 +100,11 @@ struct Context {
     //   fragP [nht][nksP][5][64]: the same bases on the half domain i < (q+1)/2
     //   (y = 16t + l%16), k-steps s < nksP/2 at the even rows j = 8s + 2(l/16),
     //   the others at the odd rows j = 8(s - nksP/2) + 2(l/16) + 1
-    double *dFragR = nullptr, *dFragF = nullptr, *dFragP = nullptr;
-    int nyt = 0, nksR = 0, nit = 0, nksF = 0, nht = 0, nksP = 0;
+    //   fragQ [npair][nksQ][2][5][64]: Qpw, Qw, Qppw, Rw, Rpw of the rows
+    //   i = 2(16p + l%16) + parity at the half-domain points y = 4s + l/16
+    //   (weights of the middle point halved)
+    double *dFragR = nullptr, *dFragF = nullptr, *dFragP = nullptr, *dFragQ = nullptr;
+    int nyt = 0, nksR = 0, nit = 0, nksF = 0, nht = 0, nksP = 0, npair = 0, nksQ = 0;
     // the Gauss points are mirror images (y[q-1-i] = -y[i]) and every basis
     // column has the parity of its index (derivatives the opposite), exactly
     bool ypar = false;

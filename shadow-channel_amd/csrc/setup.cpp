@@ -303,6 +303,38 @@ int setup_host(Context &c)
                         if (y < hq && j < dim)
                             fp[(((size_t) t * c.nksP + s) * 5 + m) * 64 + l] = (*B[m])[(size_t) y * dim + j];
                     }
+        // weighted bases of the projections for the parity form: the same
+        // parity rule (Qpw, Rpw opposite to their row), checked on the weights
+        for (int i = 0; i < hq && par; ++i) par = c.w[q - 1 - i] == c.w[i];
+        c.ypar = par;
+        c.npair = ((nR + 1) / 2 + 15) / 16;
+        // k-steps over the half domain, rounded up to a count the kernels
+        // are built for (8, 16, 20, 28; larger counts use the direct form)
+        {
+            const int need = (hq + 3) / 4;
+            c.nksQ = need;
+            for (int n : {28, 20, 16, 8})
+                if (need <= n) c.nksQ = n;
+        }
+        {
+            const std::vector<double> *Wq[5] = {&c.Qpw, &c.Qw, &c.Qppw, &c.Rw, &c.Rpw};
+            std::vector<double> fq((size_t) c.npair * c.nksQ * 2 * 5 * 64, 0.0);
+            for (int t = 0; t < c.npair; ++t)
+                for (int s = 0; s < c.nksQ; ++s)
+                    for (int pr = 0; pr < 2; ++pr)
+                        for (int m = 0; m < 5; ++m)
+                            for (int l = 0; l < 64; ++l) {
+                                const int i = 2 * (16 * t + l % 16) + pr, y = 4 * s + l / 16;
+                                const int dim = m < 3 ? nQ : nR;
+                                if (i < dim && y < hq)
+                                    fq[((((size_t) t * c.nksQ + s) * 2 + pr) * 5 + m) * 64 + l] =
+                                        (*Wq[m])[(size_t) i * q + y] * (y == q - 1 - y ? 0.5 : 1.0);
+                            }
+            if (upload(&c.dFragQ, fq)) {
+                last_error() = "device allocation of transform operands failed";
+                return -1;
+            }
+        }
         if (upload(&c.dFragP, fp)) {
             last_error() = "device allocation of transform operands failed";
             return -1;

@@ -287,3 +287,31 @@ def test_tangent_forcing(cc):
         ref = o.tangent_forcing(count)
         for g, r in zip(got, ref):
             assert np.linalg.norm(g - r) <= 1e-12 * np.linalg.norm(r)
+
+
+def test_device_resident_entry_points(cc):
+    """sc_primal_dev / sc_adjoint_dev (states in device memory) run the same
+    launches as c_primal / c_adjoint on host arrays: bitwise equal results,
+    and the primal trajectory they leave agrees with the oracle's."""
+    print("case: device-resident primal/adjoint, 32x32x65", flush=True)
+    o, C0 = start(cc, G0, 3)
+    Ch = C0.copy()
+    cc.c_primal(0, Ch)
+    ACh = direction(G0, 5)
+    AC0 = ACh.copy()
+    cc.c_adjoint(3, 0, ACh, 0, 1.0)
+    Cd, ACd = cc.DeviceArray(C0), cc.DeviceArray(AC0)
+    try:
+        cc.sc_primal_dev(0, Cd)
+        Cs = [cc.c_getsoln(i).copy() for i in range(4)]
+        cc.sc_adjoint_dev(3, 0, ACd, 1.0)
+        assert np.array_equal(Cd.download(), Ch)
+        assert np.array_equal(ACd.download(), ACh)
+    finally:
+        Cd.free()
+        ACd.free()
+    o.primal(0, C0.copy())
+    for i in range(1, 4):
+        assert rel(Cs[i], o.getsoln(i)) <= 1e-9
+    with pytest.raises(AssertionError):
+        cc.sc_adjoint_dev(3, 0, 0, 1.0)
