@@ -886,6 +886,72 @@ __global__ __launch_bounds__(BLK) void k_zfwd_adj(ZadjArgs a, PlanDev pz, const 
     }
 }
 
+// Fixed-length form of k_zfwd_adj in two phases over one 3-field tile:
+// S = (Sx, Sy, Sz) of the ZA columns is transformed first and each thread
+// keeps the values of its (stored kz row, column); then T = (Tx, Ty, Tz) is
+// transformed in the same LDS and the thread assembles its five slots.  The
+// tile is half that of k_zfwd_adj, so five blocks share a CU and the
+// (Mx/ZA) x qpts grid runs in one round; the other blocks of the CU cover a
+// block's loads.
+template <int PZ>
+__global__ __launch_bounds__(BLK) void k_zfwd_adj2(ZadjArgs a, PlanDev pz, const cplx *W2, cplx *U)
+{
+    static_assert(PZ > 0, "fixed lengths only");
+    constexpr int Pz = PZ, Nz = 2 * PZ / 3, NS = 3 * ZA, TOT = NS * Pz;
+    constexpr int NE = (TOT + BLK - 1) / BLK;
+    static_assert(ZA * (Nz - 1) <= BLK, "one output column-row per thread");
+    extern __shared__ double2 lds[];
+    if (*a.abort_flag) return;
+    const int x0 = blockIdx.x * ZA, y = blockIdx.y, Mx = a.Mx;
+    double2 *twl = lds + TOT;
+    stage_twiddles(pz, twl);
+    // element (zp, f, xl) of a phase at (zp * 3 + f) * ZA + xl; phase h
+    // holds fields 3h .. 3h + 2 of W2[y]
+    const cplx *src = W2 + (size_t) y * 6 * Pz * Mx;
+    const int r = threadIdx.x / ZA, xl = threadIdx.x - r * ZA;
+    const bool own = r < Nz - 1;
+    const int kz = r < Nz / 2 ? r : r + 1;               // skip the Nyquist row
+    const int zp = kz < Nz / 2 ? kz : kz + Nz / 2;
+    // phase h: fields 3h .. 3h + 2 into LDS, transform, this thread's values
+    auto phase = [&](int h, double2 *P) {
+        double2 v[NE];
+#pragma unroll
+        for (int i = 0; i < NE; ++i) {
+            const int e = min((int) threadIdx.x + i * BLK, TOT - 1);
+            const int q = e / ZA, c = e - q * ZA, zq = q / 3, f = q - zq * 3;
+            v[i] = src[((size_t) (3 * h + f) * Pz + zq) * Mx + x0 + c];
+        }
+#pragma unroll
+        for (int i = 0; i < NE; ++i) {
+            const int e = threadIdx.x + i * BLK;
+            if (e < TOT) lds[e] = v[i];
+        }
+        __syncthreads();
+        lds_fft_c<PZ, NS, 1, NS>(lds, twl, -1.0);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            P[f] = own ? lds[(zp * 3 + f) * ZA + xl] : make_double2(0.0, 0.0);
+            P[f].x *= a.norm;
+            P[f].y *= a.norm;
+        }
+    };
+    double2 S[3], T[3];
+    phase(0, S);
+    __syncthreads();                                     // S read before T is stored
+    phase(1, T);
+    if (!own) return;
+    const int x = x0 + xl;
+    const double kx = a.Kx[x], kzv = a.Kz[kz];
+    const double2 Sx = S[0], Sy = S[1], Sz = S[2], Tx = T[0], Ty = T[1], Tz = T[2];
+    const size_t ss = (size_t) a.qpts * Mx;
+    cplx *b = U + ((size_t) kz * 5 * a.qpts + y) * Mx + x;
+    b[0 * ss] = make_double2(-Tz.x + kx * Sy.y, -Tz.y - kx * Sy.x);
+    b[1 * ss] = make_double2(-Tx.x - kzv * Sy.y, -Tx.y + kzv * Sy.x);
+    b[2 * ss] = make_double2(-Ty.x + kzv * Sx.y - kx * Sz.y, -Ty.y - kzv * Sx.x + kx * Sz.x);
+    b[3 * ss] = Sx;
+    b[4 * ss] = Sz;
+}
+
 // adjoint assembly (channel/pass2.c:474-566):
 // Hz = -Tz - i kx Sy, Hx = -Tx + i kz Sy, Hy = -Ty - i kz Sx + i kx Sz
 __global__ void k_assemble_adj(const cplx *SP, cplx *U, const double *Kx, const double *Kz,
@@ -1095,12 +1161,13 @@ bool launch_zfwd_adj(Context &c, const cplx *W2, cplx *Udst)
     a.Kz = c.dKz;
     a.abort_flag = c.abort_flag;
     dim3 grid((d.Mx + a.XA - 1) / a.XA, d.qpts);
-    size_t lds = sizeof(double2) * (6 * (size_t) a.XA * d.Pz + d.Pz);
+    // fixed lengths: the two-phase kernel on a 3-field tile
+    size_t lds = sizeof(double2) * ((fixed ? 3 : 6) * (size_t) a.XA * d.Pz + d.Pz);
     double bytes = 16.0 * (double) d.qpts * d.Mx * (6.0 * d.Pz + 5.0 * (d.Nz - 1));
     const hipStream_t st = c.stream;
     const PlanDev pd = to_dev(c.pz);
     typedef void (*ZadjKernel)(ZadjArgs, PlanDev, const cplx *, cplx *);
-    const ZadjKernel kern = fixed == 96 ? k_zfwd_adj<96> : fixed == 48 ? k_zfwd_adj<48> : k_zfwd_adj<0>;
+    const ZadjKernel kern = fixed == 96 ? k_zfwd_adj2<96> : fixed == 48 ? k_zfwd_adj2<48> : k_zfwd_adj<0>;
     auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, W2, Udst); };
     prof_begin(c, st, 3, bytes);
     go();
