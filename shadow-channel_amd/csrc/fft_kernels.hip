@@ -892,7 +892,9 @@ __global__ __launch_bounds__(BLK) void k_zfwd_adj(ZadjArgs a, PlanDev pz, const 
 // transformed in the same LDS and the thread assembles its five slots.  The
 // tile is half that of k_zfwd_adj, so five blocks share a CU and the
 // (Mx/ZA) x qpts grid runs in one round; the other blocks of the CU cover a
-// block's loads.
+// block's loads.  1-D grid over the (y, column group) tiles, XCD-aware: the
+// column groups of a y share an XCD, so the two halves of a 128-byte line
+// (ZA = 4 columns are 64 bytes) meet in one L2.
 template <int PZ>
 __global__ __launch_bounds__(BLK) void k_zfwd_adj2(ZadjArgs a, PlanDev pz, const cplx *W2, cplx *U)
 {
@@ -902,7 +904,10 @@ __global__ __launch_bounds__(BLK) void k_zfwd_adj2(ZadjArgs a, PlanDev pz, const
     static_assert(ZA * (Nz - 1) <= BLK, "one output column-row per thread");
     extern __shared__ double2 lds[];
     if (*a.abort_flag) return;
-    const int x0 = blockIdx.x * ZA, y = blockIdx.y, Mx = a.Mx;
+    const int Mx = a.Mx, nxg = Mx / ZA;
+    const int tile = xcd_tile(nxg * a.qpts);
+    if (tile < 0) return;
+    const int y = tile / nxg, x0 = (tile - y * nxg) * ZA;
     double2 *twl = lds + TOT;
     stage_twiddles(pz, twl);
     // element (zp, f, xl) of a phase at (zp * 3 + f) * ZA + xl; phase h
@@ -1160,7 +1165,7 @@ bool launch_zfwd_adj(Context &c, const cplx *W2, cplx *Udst)
     a.Kx = c.dKx;
     a.Kz = c.dKz;
     a.abort_flag = c.abort_flag;
-    dim3 grid((d.Mx + a.XA - 1) / a.XA, d.qpts);
+    const dim3 grid = fixed ? dim3(xcd_grid(d.Mx / ZA * d.qpts)) : dim3((d.Mx + a.XA - 1) / a.XA, d.qpts);
     // fixed lengths: the two-phase kernel on a 3-field tile
     size_t lds = sizeof(double2) * ((fixed ? 3 : 6) * (size_t) a.XA * d.Pz + d.Pz);
     double bytes = 16.0 * (double) d.qpts * d.Mx * (6.0 * d.Pz + 5.0 * (d.Nz - 1));

@@ -35,19 +35,6 @@ __device__ __forceinline__ mf4 mfma_f64(double a, double b, mf4 c)
 __device__ __forceinline__ int ncol_x(int n) { return 2 * (n & 3) + (n >> 3); }
 __device__ __forceinline__ int ncol_part(int n) { return (n >> 2) & 1; }
 
-// Workgroups are dispatched round-robin over the 8 XCDs in launch order, and
-// each XCD has its own L2.  With a 1-D grid of 8*ceil(n/8) blocks, block b
-// takes logical tile (b%8)*ceil(n/8) + b/8: consecutive logical tiles (here
-// the tiles of one kz row, which read the same data) share an XCD.  Returns
-// -1 for the padding blocks.
-__device__ __forceinline__ int xcd_tile(int n)
-{
-    const int per = (n + 7) / 8;
-    const int t = (int) (blockIdx.x & 7) * per + (int) (blockIdx.x >> 3);
-    return t < n ? t : -1;
-}
-
-static inline int xcd_grid(int n) { return 8 * ((n + 7) / 8); }
 
 // most blocks per XCD of the parity rebuild, split over its half-domain y-tiles
 #ifndef SC_RB_BLOCKS

@@ -214,6 +214,20 @@ void prof_end(Context &c, hipStream_t s, int kernel_id,
 std::string &last_error();
 }  // namespace sc
 
+// Workgroups are dispatched round-robin over the 8 XCDs in launch order, and
+// each XCD has its own L2.  With a 1-D grid of 8*ceil(n/8) blocks, block b
+// takes logical tile (b%8)*ceil(n/8) + b/8: consecutive logical tiles (tiles
+// that read neighbouring data) share an XCD.  Returns -1 for the padding
+// blocks.
+__device__ __forceinline__ int xcd_tile(int n)
+{
+    const int per = (n + 7) / 8;
+    const int t = (int) (blockIdx.x & 7) * per + (int) (blockIdx.x >> 3);
+    return t < n ? t : -1;
+}
+
+static inline int xcd_grid(int n) { return 8 * ((n + 7) / 8); }
+
 #define SC_HIP(call)                                                        \
     do {                                                                    \
         hipError_t e_ = (call);                                             \
