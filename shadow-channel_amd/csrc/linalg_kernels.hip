@@ -1554,6 +1554,186 @@ __device__ __forceinline__ void chain_subst(const double *f, int fr, int fj, dou
     chain_backward<DL>(f, fr, fj, v, vr, nr);
 }
 
+// The same sweeps with NB operand buffers of two rows (prefetch distance
+// 2 NB - 2 rows): the chain waits less on LDS latency.
+template <int DL, int NB>
+__device__ __forceinline__ void chain_forward_n(const double *f, int fr, int fj, double2 *v, int vr,
+                                                int nr)
+{
+    double2 P[DL - 1];
+#pragma unroll
+    for (int k = 0; k < DL - 1; ++k) P[k] = v[min(k, nr - 1) * vr];
+    double2 xm1 = make_double2(0.0, 0.0);
+    FwdOps<DL> B[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) fwd_load<DL>(B[b], f, fr, fj, v, vr, 2 * b, nr);
+    int r = 0;
+    for (; r + 2 * NB <= nr; r += 2 * NB) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            fwd_step<DL>(B[b].c[0], B[b].vn[0], P, xm1, v, vr, r + 2 * b);
+            fwd_step<DL>(B[b].c[1], B[b].vn[1], P, xm1, v, vr, r + 2 * b + 1);
+            fwd_load<DL>(B[b], f, fr, fj, v, vr, r + 2 * b + 2 * NB, nr);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    // fewer than 2 NB rows left: B[b] holds rows r + 2b, r + 2b + 1
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        if (r + 2 * b < nr) fwd_step<DL>(B[b].c[0], B[b].vn[0], P, xm1, v, vr, r + 2 * b);
+        if (r + 2 * b + 1 < nr) fwd_step<DL>(B[b].c[1], B[b].vn[1], P, xm1, v, vr, r + 2 * b + 1);
+    }
+}
+
+// Real-valued forms of the sweeps: the real and imaginary parts of a
+// complex right-hand side are independent chains with the same factors, so
+// two lanes share a column (one part each) and every step issues half the
+// multiply-adds of the complex form.  v: one part of the vector (row r at
+// v[r*vr]); operands stream through NB buffers of two rows.
+template <int DL>
+struct FwdOpsR {
+    double c[2][DL];
+    double vn[2];
+};
+
+template <int DL>
+__device__ __forceinline__ void fwd_load_r(FwdOpsR<DL> &o, const double *f, int fr, int fj,
+                                           const double *v, int vr, int r, int nr)
+{
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int k = 0; k < DL; ++k) o.c[i][k] = f[min(r + i + k, nr - 1) * fr + (DL - 1 - k) * fj];
+        o.vn[i] = v[min(r + i + DL - 1, nr - 1) * vr];
+    }
+}
+
+template <int DL>
+__device__ __forceinline__ void fwd_step_r(const double *c, double vn, double *P, double &xm1,
+                                           double *v, int vr, int r)
+{
+    const double x = P[0] - c[0] * xm1;
+#pragma unroll
+    for (int k = 1; k < DL - 1; ++k) P[k - 1] = P[k] - c[k] * xm1;
+    P[DL - 2] = vn - c[DL - 1] * xm1;
+    v[r * vr] = x;
+    xm1 = x;
+}
+
+template <int DL, int NB>
+__device__ __forceinline__ void chain_forward_r(const double *f, int fr, int fj, double *v, int vr,
+                                                int nr)
+{
+    double P[DL - 1];
+#pragma unroll
+    for (int k = 0; k < DL - 1; ++k) P[k] = v[min(k, nr - 1) * vr];
+    double xm1 = 0.0;
+    FwdOpsR<DL> B[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) fwd_load_r<DL>(B[b], f, fr, fj, v, vr, 2 * b, nr);
+    int r = 0;
+    for (; r + 2 * NB <= nr; r += 2 * NB) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            fwd_step_r<DL>(B[b].c[0], B[b].vn[0], P, xm1, v, vr, r + 2 * b);
+            fwd_step_r<DL>(B[b].c[1], B[b].vn[1], P, xm1, v, vr, r + 2 * b + 1);
+            fwd_load_r<DL>(B[b], f, fr, fj, v, vr, r + 2 * b + 2 * NB, nr);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        if (r + 2 * b < nr) fwd_step_r<DL>(B[b].c[0], B[b].vn[0], P, xm1, v, vr, r + 2 * b);
+        if (r + 2 * b + 1 < nr) fwd_step_r<DL>(B[b].c[1], B[b].vn[1], P, xm1, v, vr, r + 2 * b + 1);
+    }
+}
+
+template <int DL>
+struct BwdOpsR {
+    double u[2][DL + 1];
+    double v[2];
+};
+
+template <int DL>
+__device__ __forceinline__ void bwd_load_r(BwdOpsR<DL> &o, const double *f, int fr, int fj,
+                                           const double *v, int vr, int r)
+{
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int rr = max(r - i, 0);
+#pragma unroll
+        for (int k = 1; k <= DL; ++k) o.u[i][k - 1] = f[rr * fr + (DL + k) * fj];
+        o.u[i][DL] = f[rr * fr + DL * fj];
+        o.v[i] = v[rr * vr];
+    }
+}
+
+template <int DL>
+__device__ __forceinline__ void bwd_step_r(const double *u, double x, double *w, double *v, int vr,
+                                           int r)
+{
+#pragma unroll
+    for (int k = 1; k <= DL; ++k) x -= u[k - 1] * w[k - 1];
+    x *= u[DL];
+#pragma unroll
+    for (int m = DL - 1; m > 0; --m) w[m] = w[m - 1];
+    w[0] = x;
+    v[r * vr] = x;
+}
+
+// Rows of the factor image whose U parts arrive while the chain runs: chunk
+// c holds the rows [nr - (c+1) cr, nr - c cr) (last rows first); ready[c]
+// counts the waves that have stored their share of it.
+struct StagedRows {
+    const int *ready;
+    int cr, nchunk, nwriters;
+};
+
+// wait until the U parts of rows >= row are in LDS (chunks in order)
+__device__ __forceinline__ void wait_rows(const StagedRows &sr, int row, int &low, int &next)
+{
+    while (row < low && next < sr.nchunk) {
+        while (__hip_atomic_load(&sr.ready[next], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) <
+               sr.nwriters)
+            __builtin_amdgcn_s_sleep(1);
+        ++next;
+        low = max(low - sr.cr, 0);
+    }
+}
+
+// real-valued backward sweep with NB operand buffers, the operand stream
+// waiting for staged rows
+template <int DL, int NB>
+__device__ __forceinline__ void chain_backward_staged(const double *f, int fr, int fj, double *v,
+                                                      int vr, int nr, const StagedRows &sr)
+{
+    int low = nr, next = 0;
+    double w[DL];
+#pragma unroll
+    for (int m = 0; m < DL; ++m) w[m] = 0.0;
+    BwdOpsR<DL> B[NB];
+    wait_rows(sr, max(nr - 2 * NB, 0), low, next);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) bwd_load_r<DL>(B[b], f, fr, fj, v, vr, nr - 1 - 2 * b);
+    int r = nr - 1;
+    // B[b] holds rows r - 2b, r - 2b - 1
+    for (; r >= 2 * NB - 1; r -= 2 * NB) {
+        wait_rows(sr, max(r - 4 * NB + 1, 0), low, next);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            bwd_step_r<DL>(B[b].u[0], B[b].v[0], w, v, vr, r - 2 * b);
+            bwd_step_r<DL>(B[b].u[1], B[b].v[1], w, v, vr, r - 2 * b - 1);
+            bwd_load_r<DL>(B[b], f, fr, fj, v, vr, r - 2 * b - 2 * NB);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        if (r - 2 * b >= 0) bwd_step_r<DL>(B[b].u[0], B[b].v[0], w, v, vr, r - 2 * b);
+        if (r - 2 * b - 1 >= 0) bwd_step_r<DL>(B[b].u[1], B[b].v[1], w, v, vr, r - 2 * b - 1);
+    }
+}
+
 struct ColumnOf {
     int z, x;
     bool valid, chain;
@@ -1598,8 +1778,11 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx 
     const cplx *F2 = var == 0 ? a.Fa2 : a.Fb2;
     const double ce0 = a.ce0[var], cn = a.cn[var];
 
+    constexpr int NCHU = 4;                 // chunks of rows whose U parts stream in
     __shared__ int chain_wave[1];
+    __shared__ int ready[NCHU];
     if (tid == 0) chain_wave[0] = 0;
+    if (tid < NCHU) ready[tid] = 0;
     __syncthreads();
     {
         // HW_ID: SIMD in bits 5:4, workgroup slot in bits 19:16
@@ -1622,25 +1805,35 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx 
         fv[t] = F[fbase + r * rs];
         f2v[t] = a.has_next ? F2[fbase + r * rs] : make_double2(0.0, 0.0);
     }
-    // the factor image: contiguous, 16-byte loads, batched
+    // The factor image ([row][W][SBT], 16-byte loads): the L parts (columns
+    // j < DL) of every row before the barrier, for the forward sweep; the U
+    // parts (j >= DL) after it, last rows first in NCHU chunks, each wave
+    // signalling a chunk once its share is in LDS, so that the backward
+    // sweep starts while the upper rows still stream in.
+    constexpr int RPR = W * SBT / 2, LPR = DL * SBT / 2, UPR = RPR - LPR;
+    int fb = blk;
+    if (a.mirror) {
+        // factor image of the mirror row Nz - z for z > Nz/2
+        const int z = blk * SBT / Mx;
+        if (z > a.Nz / 2) fb = blk - (2 * z - a.Nz) * (Mx / SBT);
+    }
+    const double2 *src = (const double2 *) (var == 0 ? a.La : a.Lb) +
+                         ((size_t) fb * 2 + par) * nrmax * RPR;
+    double2 *dst = (double2 *) lf;
     {
-        // factor image: that of the mirror row Nz - z for z > Nz/2
-        int fb = blk;
-        if (a.mirror) {
-            const int z = blk * SBT / Mx;
-            if (z > a.Nz / 2) fb = blk - (2 * z - a.Nz) * (Mx / SBT);
-        }
-        const double2 *src = (const double2 *) (var == 0 ? a.La : a.Lb) +
-                             ((size_t) fb * 2 + par) * nrmax * W * SBT / 2;
-        double2 *dst = (double2 *) lf;
-        const int n2 = nr * W * SBT / 2;
-        for (int b0 = 0; b0 < n2; b0 += SOLVE_FBAT * SOLVE_THREADS) {
+        const int nl = nr * LPR;
+        for (int b0 = 0; b0 < nl; b0 += SOLVE_FBAT * SOLVE_THREADS) {
             double2 t[SOLVE_FBAT];
+            int o[SOLVE_FBAT];
 #pragma unroll
-            for (int i = 0; i < SOLVE_FBAT; ++i) t[i] = src[min(b0 + tid + i * SOLVE_THREADS, n2 - 1)];
+            for (int i = 0; i < SOLVE_FBAT; ++i) {
+                const int e = min(b0 + tid + i * SOLVE_THREADS, nl - 1), r = e / LPR;
+                o[i] = r * RPR + (e - r * LPR);
+                t[i] = src[o[i]];
+            }
 #pragma unroll
             for (int i = 0; i < SOLVE_FBAT; ++i)
-                if (b0 + tid + i * SOLVE_THREADS < n2) dst[b0 + tid + i * SOLVE_THREADS] = t[i];
+                if (b0 + tid + i * SOLVE_THREADS < nl) dst[o[i]] = t[i];
         }
     }
 #pragma unroll
@@ -1681,9 +1874,79 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx 
     // The chains run in one wave of the block: the wave whose SIMD equals the
     // block's workgroup slot on the CU (mod 4), so that the blocks resident on
     // one CU run their chains on different SIMDs (wave 0 when no wave matches).
-    const int cl = tid & 63;
-    if ((tid >> 6) == chain_wave[0] && cl < SBT && co.chain)
-        chain_subst<DL>(lf + cl, W * SBT, SBT, vec + cl, SBT, nr);
+    // The other waves stage the U parts meanwhile, chunk by chunk, and signal.
+    const int cl = tid & 63, cw = chain_wave[0];
+    const int cr = (nr + NCHU - 1) / NCHU;
+    constexpr int NWR = SOLVE_THREADS / 64 - 1;          // staging waves
+    const StagedRows sr{ready, cr, NCHU, NWR};
+    if ((tid >> 6) == cw) {
+        // lane cl < 2 SBT: part cl / SBT (real, imaginary) of column cl % SBT
+        if (cl < 2 * SBT && co.chain && nr > 0) {
+            const int col = cl % SBT, part = cl / SBT;
+            double *vd = (double *) vec + 2 * col + part;
+            chain_forward_r<DL, 3>(lf + col, W * SBT, SBT, vd, 2 * SBT, nr);
+            chain_backward_staged<DL, 3>(lf + col, W * SBT, SBT, vd, 2 * SBT, nr, sr);
+        }
+    } else {
+        // element e of the U parts counts from the last row up (row
+        // nr - 1 - e / UPR); chunk c holds e in [c cr UPR, (c+1) cr UPR).
+        // The staging waves' threads wt take e = c cr UPR + wt + j NT of
+        // every chunk, all loads issued before the first store (chains of
+        // up to 48 rows; longer ones copy directly).
+        constexpr int NT = NWR * 64;
+        constexpr int CRMAX = (48 + NCHU - 1) / NCHU;
+        static_assert(CRMAX * UPR <= 2 * NT, "two elements per thread and chunk");
+        const int wt = tid - ((tid >> 6) > cw ? 64 : 0);
+        const int nu = nr * UPR, ce = cr * UPR;
+        const bool regs = cr <= CRMAX;
+        // a chunk's two elements per thread in named registers (arrays of
+        // them stay in scratch memory next to the signalling atomic); the
+        // loads of chunk c+1 are issued before the stores of chunk c
+        struct UC {
+            double2 v0, v1;
+            int o0, o1;
+            bool s0, s1;
+        };
+        auto ld = [&](int c) {
+            UC u;
+            const int e0 = c * ce, e1 = min(e0 + ce, nu);
+            const int ea = e0 + wt, eb = ea + NT;
+            u.s0 = ea < e1;
+            u.s1 = eb < e1;
+            const int ca = max(min(ea, e1 - 1), 0), cb = max(min(eb, e1 - 1), 0);
+            const int ra = ca / UPR, rb = cb / UPR;
+            // (no rows at all: read the image's first element, store nothing)
+            u.o0 = max((nr - 1 - ra) * RPR + LPR + (ca - ra * UPR), 0);
+            u.o1 = max((nr - 1 - rb) * RPR + LPR + (cb - rb * UPR), 0);
+            u.v0 = src[u.o0];
+            u.v1 = src[u.o1];
+            return u;
+        };
+        auto st = [&](const UC &u, int c) {
+            if (u.s0) dst[u.o0] = u.v0;
+            if (u.s1) dst[u.o1] = u.v1;
+            if (cl == 0) __hip_atomic_fetch_add(&ready[c], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        };
+        if (regs) {
+            static_assert(NCHU == 4, "four chunks below");
+            UC a0 = ld(0), a1 = ld(1);
+            st(a0, 0);
+            a0 = ld(2);
+            st(a1, 1);
+            a1 = ld(3);
+            st(a0, 2);
+            st(a1, 3);
+        } else {
+            for (int c = 0; c < NCHU; ++c) {
+                const int e0 = c * ce, e1 = min(e0 + ce, nu);
+                for (int e = e0 + wt; e < e1; e += NT) {
+                    const int rb = e / UPR, o = (nr - 1 - rb) * RPR + LPR + (e - rb * UPR);
+                    dst[o] = src[o];
+                }
+                if (cl == 0) __hip_atomic_fetch_add(&ready[c], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+    }
     __syncthreads();
     // write the solution and the next stage's right-hand side
 #pragma unroll
