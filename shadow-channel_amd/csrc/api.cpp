@@ -244,18 +244,8 @@ static void ddt_project_dev(Context &c, int i, const cplx *vin, cplx *vout,
     long ip = i == 0 ? 3 : 3L * i, im = i == 0 ? 0 : 3L * (i - 1);
     launch_diff_scale(c, c.Ctmp2, mc(c, ip), mc(c, im), 1.0 / d.dt, c.csize);
     launch_rebuild2(c, c.Ctmp2, c.Ut, vin, c.Ut2);
-    FieldSpec fs;
-    fs.nf = 6;
-    for (int s = 0; s < 3; ++s) {
-        fs.src[s] = c.Ut;
-        fs.src[3 + s] = c.Ut2;
-        fs.kind[s] = fs.kind[3 + s] = FK_SLOT;
-        fs.slot[s] = fs.slot[3 + s] = s;
-    }
-    launch_zinv(c, fs, c.W1);
-    launch_xmix(c, XM_DOT, c.W1, nullptr, 0, nullptr, c.dotpart);
-    int nparts = d.qpts * (d.Pz / xmix_zblock(c, 6));
-    launch_ddt_reduce(c, nparts);
+    // inner products from the spectra at the Gauss points (exact grid sums)
+    launch_ddt_reduce(c, launch_ddt_dot(c, c.Ut, c.Ut2));
     launch_ddt_apply(c, vout, vin, c.Ctmp2, c.csize, acc, a);
 }
 

@@ -2255,6 +2255,71 @@ __global__ void k_ddt_reduce(const double *part, int nparts, double *scal)
     }
 }
 
+// Weighted inner products of ddt_project (channel/channel.c:704-780) from the
+// Gauss-point spectra: num = <A, B>, den = <A, A> over u, v, w (slots 0..2),
+// where <f, g> = sum_y W_y sum_{x,z} f g on the dealiased Px x Pz grid.  The
+// fields are band-limited below the grid's Nyquist bins in x and z (3/2
+// rule), so by Parseval the grid sums are exact in spectral space:
+//     sum_{x,z} f g = Px Pz (sum_kz Re(Hf conj Hg) + 2 sum_{kx>0,kz} Re(F conj G)),
+// with Hf(kz) = (F(0,kz) + conj F(0,-kz)) / 2 the kx = 0 column as the
+// complex-to-real x transform uses it (real part of its z transform).  The
+// Nyquist row kz = Nz/2 carries nothing.  One partial pair per block.
+__global__ __launch_bounds__(256) void k_ddt_dot(const cplx *A, const cplx *B, const double *W,
+                                                 double scale, int Nz, int qpts, int Mx,
+                                                 double *part, const int *abort_flag)
+{
+    __shared__ double r0[256], r1[256];
+    double num = 0.0, den = 0.0;
+    if (!*abort_flag) {
+        const size_t n = (size_t) Nz * qpts * Mx, ss = (size_t) qpts * Mx;
+        for (size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x; e < n;
+             e += (size_t) gridDim.x * blockDim.x) {
+            const int x = (int) (e % Mx), y = (int) ((e / Mx) % qpts), z = (int) (e / ss);
+            if (z == Nz / 2) continue;
+            const size_t o = (size_t) z * 5 * ss + (size_t) y * Mx + x;
+            double sn = 0.0, sd = 0.0;
+            if (x > 0) {
+#pragma unroll
+                for (int s = 0; s < 3; ++s) {
+                    const cplx a = A[o + s * ss], b = B[o + s * ss];
+                    sn += a.x * b.x + a.y * b.y;
+                    sd += a.x * a.x + a.y * a.y;
+                }
+                sn *= 2.0;
+                sd *= 2.0;
+            } else {
+                const int zm = z == 0 ? 0 : Nz - z;
+                const size_t om = (size_t) zm * 5 * ss + (size_t) y * Mx;
+#pragma unroll
+                for (int s = 0; s < 3; ++s) {
+                    const cplx a = A[o + s * ss], am = A[om + s * ss];
+                    const cplx b = B[o + s * ss], bm = B[om + s * ss];
+                    const double har = 0.5 * (a.x + am.x), hai = 0.5 * (a.y - am.y);
+                    const double hbr = 0.5 * (b.x + bm.x), hbi = 0.5 * (b.y - bm.y);
+                    sn += har * hbr + hai * hbi;
+                    sd += har * har + hai * hai;
+                }
+            }
+            num += W[y] * sn;
+            den += W[y] * sd;
+        }
+    }
+    r0[threadIdx.x] = num * scale;
+    r1[threadIdx.x] = den * scale;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            r0[threadIdx.x] += r0[threadIdx.x + o];
+            r1[threadIdx.x] += r1[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = r0[0];
+        part[2 * blockIdx.x + 1] = r1[0];
+    }
+}
+
 // vout = vin - r dC (r = scal[2]); with acc, also acc -= a vout
 __global__ void k_ddt_apply(cplx *vout, const cplx *vin, const cplx *dC, const double *scal,
                             size_t n, cplx *acc, double a)
@@ -2375,6 +2440,16 @@ void launch_axpy_c(Context &c, cplx *y, const cplx *x, double a, size_t n)
 void launch_diff_scale(Context &c, cplx *out, const cplx *p, const cplx *m, double s, size_t n)
 {
     hipLaunchKernelGGL(k_diff_scale, dim3(nblocks(n)), dim3(256), 0, c.stream, out, p, m, s, n);
+}
+
+int launch_ddt_dot(Context &c, const cplx *A, const cplx *B)
+{
+    const Dims &d = c.d;
+    const size_t n = (size_t) d.Nz * d.qpts * d.Mx;
+    const int nb = (int) std::min<size_t>({(n + 255) / 256, 1024, (size_t) d.qpts * d.Pz});
+    hipLaunchKernelGGL(k_ddt_dot, dim3(nb), dim3(256), 0, c.stream, A, B, c.dW,
+                       (double) d.Px * (double) d.Pz, d.Nz, d.qpts, d.Mx, c.dotpart, c.abort_flag);
+    return nb;
 }
 
 void launch_ddt_reduce(Context &c, int nparts)

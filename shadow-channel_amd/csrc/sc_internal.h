@@ -200,6 +200,9 @@ void launch_diff_scale(Context &c, cplx *out, const cplx *a, const cplx *b, doub
 void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *dC, size_t n,
                       cplx *acc = nullptr, double a = 0.0);
 void launch_ddt_reduce(Context &c, int nparts);
+// partial inner products of ddt_project from Gauss-point spectra; returns the
+// number of partial pairs written to dotpart
+int launch_ddt_dot(Context &c, const cplx *A, const cplx *B);
 
 // dst_kind for launch_zfwd
 enum { ZF_USLOTS = 0, ZF_SCRATCH = 1 };
