@@ -560,47 +560,85 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
     // bin n < Mx takes X[n] + i Y[n], bin n > Px - Mx the conjugates at
     // Px - n, the bins between are zero.  Loads unpredicated (index clamped
     // into the row), all issued before the LDS stores.
-    const int tot = ZB * nseq * Px;
-    constexpr int MAXL = PX > 0 ? (ZBC * nseq * PX + BLK - 1) / BLK : (XMIX_TILE + BLK - 1) / BLK;
-    constexpr int LB = PX > 0 ? MAXL : 4;              // loads in flight per batch
-    Walk wl(Px);                                       // (packed row t, bin n)
-#pragma unroll 1
-    for (int i0 = 0; i0 < MAXL; i0 += LB) {
-        if ((int) threadIdx.x + i0 * BLK >= tot) break;
-        double2 lx[LB], ly[LB];
-        int tn[LB];
+    if constexpr (PX > 0) {
+        // fixed length: element e = t Px + n of packed row t = r nseq + s;
+        // every division by a constant, 32-bit offsets from two block bases
+        // (fields below nf1 in W1, the others in W1b; pairs never straddle)
+        constexpr int MX = PX / 3, NE = ZBC * nseq * PX, ML = (NE + BLK - 1) / BLK;
+        const int nf1 = a.nf1, PM = Pz * Mx;
+        const cplx *Xa = W1 + ((size_t) y * nf1 * Pz + z0) * Mx;
+        const cplx *Xb = a.W1b + ((size_t) y * (nf - nf1) * Pz + z0) * Mx;
+        double2 lx[ML], ly[ML];
 #pragma unroll
-        for (int i = 0; i < LB; ++i) {
-            const int t = min(wl.q, ZB * nseq - 1), n = wl.r;
-            tn[i] = wl.q * Px + n;
-            wl.next();
-            const int s = t % nseq, r = t / nseq;
-            const int k = min(n < Mx ? n : Px - n, Mx - 1);
-            const int f0 = 2 * s, nf1 = a.nf1;
-            const cplx *X = f0 < nf1 ? W1 + (((size_t) y * nf1 + f0) * Pz + z0 + r) * Mx
-                                     : a.W1b + (((size_t) y * (nf - nf1) + f0 - nf1) * Pz + z0 + r) * Mx;
-            const int sy = min(2 * s + 1, nf - 1) - 2 * s;     // 1, or 0 for a lone field
-            lx[i] = X[k];
-            ly[i] = X[(size_t) sy * Pz * Mx + k];
+        for (int i = 0; i < ML; ++i) {
+            const int e = min((int) threadIdx.x + i * BLK, NE - 1);
+            const int t = e / PX, n = e - t * PX, r = t / nseq, s2 = 2 * (t - r * nseq);
+            const int k = min(n < MX ? n : PX - n, MX - 1);
+            const bool first = s2 < nf1;
+            const cplx *X = first ? Xa : Xb;
+            const int off = ((first ? s2 : s2 - nf1) * Pz + r) * Mx + k;
+            lx[i] = X[off];
+            ly[i] = X[off + (s2 + 1 < nf ? PM : 0)];
         }
 #pragma unroll
-        for (int i = 0; i < LB; ++i) {
-            const int e = threadIdx.x + (i0 + i) * BLK;
-            if (e >= tot) break;
-            const int t = tn[i] / Px, n = tn[i] - t * Px;
-            const int s = t % nseq, r = t / nseq;
-            const bool hasy = 2 * s + 1 < nf;
-            double2 xv = lx[i], yv = hasy ? ly[i] : make_double2(0.0, 0.0);
-            double2 val;
+        for (int i = 0; i < ML; ++i) {
+            const int e = (int) threadIdx.x + i * BLK;
+            if (e >= NE) break;
+            const int t = e / PX, n = e - t * PX;
+            const bool hasy = 2 * (t % nseq) + 1 < nf;
+            const double2 xv = lx[i], yv = hasy ? ly[i] : make_double2(0.0, 0.0);
+            double2 val = make_double2(0.0, 0.0);
             if (n == 0)
                 val = make_double2(xv.x, yv.x);
-            else if (n < Mx)
+            else if (n < MX)
                 val = make_double2(xv.x - yv.y, xv.y + yv.x);
-            else if (n > Px - Mx)
+            else if (n > PX - MX)
                 val = make_double2(xv.x + yv.y, -xv.y + yv.x);
-            else
-                val = make_double2(0.0, 0.0);
-            lds[(r * nseq + s) * SP + n] = val;
+            lds[e + t] = val;                          // (r nseq + s) (Px + 1) + n
+        }
+    } else {
+        const int tot = ZB * nseq * Px;
+        constexpr int MAXL = (XMIX_TILE + BLK - 1) / BLK;
+        constexpr int LB = 4;                              // loads in flight per batch
+        Walk wl(Px);                                       // (packed row t, bin n)
+#pragma unroll 1
+        for (int i0 = 0; i0 < MAXL; i0 += LB) {
+            if ((int) threadIdx.x + i0 * BLK >= tot) break;
+            double2 lx[LB], ly[LB];
+            int tn[LB];
+#pragma unroll
+            for (int i = 0; i < LB; ++i) {
+                const int t = min(wl.q, ZB * nseq - 1), n = wl.r;
+                tn[i] = wl.q * Px + n;
+                wl.next();
+                const int s = t % nseq, r = t / nseq;
+                const int k = min(n < Mx ? n : Px - n, Mx - 1);
+                const int f0 = 2 * s, nf1 = a.nf1;
+                const cplx *X = f0 < nf1 ? W1 + (((size_t) y * nf1 + f0) * Pz + z0 + r) * Mx
+                                         : a.W1b + (((size_t) y * (nf - nf1) + f0 - nf1) * Pz + z0 + r) * Mx;
+                const int sy = min(2 * s + 1, nf - 1) - 2 * s;     // 1, or 0 for a lone field
+                lx[i] = X[k];
+                ly[i] = X[(size_t) sy * Pz * Mx + k];
+            }
+#pragma unroll
+            for (int i = 0; i < LB; ++i) {
+                const int e = threadIdx.x + (i0 + i) * BLK;
+                if (e >= tot) break;
+                const int t = tn[i] / Px, n = tn[i] - t * Px;
+                const int s = t % nseq, r = t / nseq;
+                const bool hasy = 2 * s + 1 < nf;
+                double2 xv = lx[i], yv = hasy ? ly[i] : make_double2(0.0, 0.0);
+                double2 val;
+                if (n == 0)
+                    val = make_double2(xv.x, yv.x);
+                else if (n < Mx)
+                    val = make_double2(xv.x - yv.y, xv.y + yv.x);
+                else if (n > Px - Mx)
+                    val = make_double2(xv.x + yv.y, -xv.y + yv.x);
+                else
+                    val = make_double2(0.0, 0.0);
+                lds[(r * nseq + s) * SP + n] = val;
+            }
         }
     }
     __syncthreads();
