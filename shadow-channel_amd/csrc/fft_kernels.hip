@@ -561,40 +561,51 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
     // Px - n, the bins between are zero.  Loads unpredicated (index clamped
     // into the row), all issued before the LDS stores.
     if constexpr (PX > 0) {
-        // fixed length: element e = t Px + n of packed row t = r nseq + s;
-        // every division by a constant, 32-bit offsets from two block bases
-        // (fields below nf1 in W1, the others in W1b; pairs never straddle)
-        constexpr int MX = PX / 3, NE = ZBC * nseq * PX, ML = (NE + BLK - 1) / BLK;
+        // fixed length: the load stage is the first pass of the inverse
+        // transform.  Thread b < NSQ NB1 takes butterfly j = b / NSQ of packed
+        // row t = b % NSQ = r nseq + s (the pass's own thread mapping), loads
+        // its R1 bins n = j + q NB1 straight from the fields (32-bit offsets
+        // from two block bases: fields below nf1 in W1, the others in W1b;
+        // pairs never straddle), applies the Hermitian extension and the
+        // radix-R1 transform in registers and writes the pass's output.
+        // Groups q whose bins all lie in the zero band are not loaded.
+        constexpr int MX = PX / 3, R1 = plan_radix(PX, 0), NB1 = PX / R1;
+        constexpr int NSQ = ZBC * nseq, NBUT = NSQ * NB1;
+        static_assert(NBUT <= BLK, "one first-pass butterfly per thread");
         const int nf1 = a.nf1, PM = Pz * Mx;
-        const cplx *Xa = W1 + ((size_t) y * nf1 * Pz + z0) * Mx;
-        const cplx *Xb = a.W1b + ((size_t) y * (nf - nf1) * Pz + z0) * Mx;
-        double2 lx[ML], ly[ML];
-#pragma unroll
-        for (int i = 0; i < ML; ++i) {
-            const int e = min((int) threadIdx.x + i * BLK, NE - 1);
-            const int t = e / PX, n = e - t * PX, r = t / nseq, s2 = 2 * (t - r * nseq);
-            const int k = min(n < MX ? n : PX - n, MX - 1);
+        const int b = threadIdx.x;
+        if (b < NBUT) {
+            const int j = b / NSQ, t = b - j * NSQ, r = t / nseq, s2 = 2 * (t - r * nseq);
             const bool first = s2 < nf1;
-            const cplx *X = first ? Xa : Xb;
-            const int off = ((first ? s2 : s2 - nf1) * Pz + r) * Mx + k;
-            lx[i] = X[off];
-            ly[i] = X[off + (s2 + 1 < nf ? PM : 0)];
-        }
+            const cplx *X = first ? W1 + ((size_t) y * nf1 * Pz + z0) * Mx
+                                  : a.W1b + ((size_t) y * (nf - nf1) * Pz + z0) * Mx;
+            X += ((first ? s2 : s2 - nf1) * Pz + r) * Mx;
+            const bool hasy = s2 + 1 < nf;
+            const int yo = hasy ? PM : 0;
+            double2 v[R1];
 #pragma unroll
-        for (int i = 0; i < ML; ++i) {
-            const int e = (int) threadIdx.x + i * BLK;
-            if (e >= NE) break;
-            const int t = e / PX, n = e - t * PX;
-            const bool hasy = 2 * (t % nseq) + 1 < nf;
-            const double2 xv = lx[i], yv = hasy ? ly[i] : make_double2(0.0, 0.0);
-            double2 val = make_double2(0.0, 0.0);
-            if (n == 0)
-                val = make_double2(xv.x, yv.x);
-            else if (n < MX)
-                val = make_double2(xv.x - yv.y, xv.y + yv.x);
-            else if (n > PX - MX)
-                val = make_double2(xv.x + yv.y, -xv.y + yv.x);
-            lds[e + t] = val;                          // (r nseq + s) (Px + 1) + n
+            for (int q = 0; q < R1; ++q) {
+                if (q * NB1 >= MX && q * NB1 + NB1 - 1 <= PX - MX) {
+                    v[q] = make_double2(0.0, 0.0);             // zero band for every j
+                    continue;
+                }
+                const int n = j + q * NB1;
+                const int k = min(n < MX ? n : PX - n, MX - 1);
+                const double2 xv = X[k], yl = X[yo + k];
+                const double2 yv = hasy ? yl : make_double2(0.0, 0.0);
+                double2 val = make_double2(0.0, 0.0);
+                if (n == 0)
+                    val = make_double2(xv.x, yv.x);
+                else if (n < MX)
+                    val = make_double2(xv.x - yv.y, xv.y + yv.x);
+                else if (n > PX - MX)
+                    val = make_double2(xv.x + yv.y, -xv.y + yv.x);
+                v[q] = val;
+            }
+            dft<R1>(v, +1.0);
+            double2 *dst = lds + t * (PX + 1) + j * R1;
+#pragma unroll
+            for (int q = 0; q < R1; ++q) dst[q] = v[q];
         }
     } else {
         const int tot = ZB * nseq * Px;
@@ -643,7 +654,7 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
     }
     __syncthreads();
     if constexpr (PX > 0)
-        lds_fft_c<PX, ZBC * nseq, PX + 1, 1>(lds, twl, +1.0);
+        lds_fft_c<PX, ZBC * nseq, PX + 1, 1, 1, plan_radix(PX, 0)>(lds, twl, +1.0);
     else
         lds_fft(lds, ZB * nseq, SP, 1, px, twl, +1.0);
     // pointwise stage: read all fields of a point, compute, then (after a
