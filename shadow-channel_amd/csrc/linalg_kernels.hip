@@ -1053,9 +1053,18 @@ template <bool ADJ, int NKS>
 struct ForceParShape {
     static constexpr int NB = ADJ ? 5 : 3;
     static constexpr int NF = ADJ ? 5 : 3;
-    static constexpr int CKMAX = ADJ ? 2 : 4;
-    static constexpr int pick(int ck) { return (NKS % ck == 0 && (NKS / ck) % 2 == 0) ? ck : pick(ck - 1); }
-    static constexpr int CK = pick(CKMAX), NCH = NKS / CK;
+    // KS waves share a task's k-steps (partial sums added through LDS)
+    // where the exchange area fits beside the staged bases
+    static constexpr int XCH1 = 4 * 4 * 64;                  // doubles of one wave's partials
+    static constexpr int KS = (NKS % 2 == 0 &&
+                               (NKS * 2 * NB * 64 + (FR_THREADS / 128) * XCH1) * 8 <= 160 * 1024) ? 2 : 1;
+    static constexpr int KW = NKS / KS;                      // k-steps per wave
+    static constexpr int TPR = FR_THREADS / 64 / KS;         // tasks per round of the block
+    static constexpr int XCH = KS > 1 ? TPR * (KS - 1) * XCH1 : 0;
+    static constexpr int CKMAX = ADJ ? 2 : 5;
+    static constexpr int pick(int ck) { return KW % ck == 0 ? ck : pick(ck - 1); }
+    static constexpr int CK = pick(CKMAX), NCH = KW / CK;
+    static constexpr size_t lds_bytes() { return sizeof(double) * ((size_t) NKS * 2 * NB * 64 + XCH); }
 };
 template <bool ADJ, int NKS>
 __global__ __launch_bounds__(FR_THREADS) void k_force_par(ForceArgs a, int bpi, int nprob,
@@ -1063,11 +1072,17 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_par(ForceArgs a, int bpi, 
                                                           const cplx *U1, cplx *Fa1, cplx *Fb1)
 {
     using SH = ForceParShape<ADJ, NKS>;
-    constexpr int NB = SH::NB, NF = SH::NF, CK = SH::CK, NCH = SH::NCH;
-    static_assert(NCH % 2 == 0 && CK >= 1, "chunks alternate between two buffers");
+    constexpr int NB = SH::NB, NF = SH::NF, CK = SH::CK, NCH = SH::NCH, KS = SH::KS;
+    constexpr int KW = SH::KW, TPR = SH::TPR, XCH1 = SH::XCH1;
     extern __shared__ double bsf[];
     if (*a.abort_flag) return;
-    const int tp = blockIdx.x / bpi, lb = blockIdx.x - tp * bpi;
+    // XCD-aware: block b runs task range lb = (b / 8 / npair) 8 + b % 8 of
+    // row-tile pair tp = (b / 8) % npair, so the pairs' blocks of one task
+    // range (which read the same fields) share an XCD and its L2; bpi is a
+    // multiple of 8
+    const int npair = a.nit;
+    const int rest = blockIdx.x >> 3, tp = rest % npair;
+    const int lb = (rest / npair) * 8 + (blockIdx.x & 7);
     const int lane = threadIdx.x & 63, wave = wave_id();
     const int kq = lane >> 4, nl = lane & 15;
     const int Mx = a.Mx, q = a.qpts, nR = a.dimR, nQ = a.dimQ;
@@ -1139,89 +1154,120 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_par(ForceArgs a, int bpi, 
             }
         }
     };
-    int t = t0 + wave;
-    if (t < t1) load(task_of(t), 0, 0);
-    while (t < t1) {
-        const Task k = task_of(t);
-        const bool mean = k.z == 0 && k.xcol == 0;
-        const double kz = wave_kz(k.z, a.Nz, a.az);
-        const double kx = wave_kx(k.xcol, a.ax), k2 = kz * kz + kx * kx;
-        const double sg = k.part ? 1.0 : -1.0;
+    // Rounds of TPR tasks: task t + wave / KS, k-steps [kb, kb + KW) of it
+    // with kb = (wave % KS) KW.  The partial sums of the waves kb > 0 go
+    // through LDS to the wave kb = 0, which adds them in order and stores.
+    const int part = wave % KS, slot = wave / KS, kb = part * KW;
+    double *xch = bsf + (size_t) NKS * 2 * NB * 64;
+    for (int t = t0; t < t1; t += TPR) {
+        const int tk = t + slot;
+        const bool act = tk < t1;
         mf4 aAe = mf4_zero(), aBe = mf4_zero(), aAo = mf4_zero(), aBo = mf4_zero();
-        const int tn = t + 8;
-        auto chunk = [&](int ch, int buf) {
-            const double *bc = bsf + (size_t) ch * CK * 2 * NB * 64 + lane;
+        const Task k = task_of(act ? tk : t0);
+        if (act) {
+            const bool mean = k.z == 0 && k.xcol == 0;
+            const double kz = wave_kz(k.z, a.Nz, a.az);
+            const double kx = wave_kx(k.xcol, a.ax), k2 = kz * kz + kx * kx;
+            const double sg = k.part ? 1.0 : -1.0;
+            auto chunk = [&](int ch, int buf) {
+                const double *bc = bsf + (size_t) (kb + ch * CK) * 2 * NB * 64 + lane;
 #pragma unroll
-            for (int u = 0; u < CK; ++u) {
-                double vP[2], vY[2], vP2[2], vZ[2] = {0.0, 0.0}, vZ2[2] = {0.0, 0.0};
+                for (int u = 0; u < CK; ++u) {
+                    double vP[2], vY[2], vP2[2], vZ[2] = {0.0, 0.0}, vZ2[2] = {0.0, 0.0};
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const double hx = r[buf][h][0][u], hz = r[buf][h][1][u], hy = r[buf][h][2][u];
-                    if (mean) {
-                        vP[h] = 0.0;
-                        vY[h] = 0.0;
-                        vP2[h] = k.part ? hz : hx;
-                        if (ADJ) vZ2[h] = k.part ? -r[buf][h][3 % NF][u] : r[buf][h][4 % NF][u];
-                    } else {
-                        vP[h] = sg * (kx * hx + kz * hz);
-                        vY[h] = -k2 * hy;
-                        vP2[h] = sg * (kz * hx - kx * hz);
-                        if (ADJ) {
-                            vZ[h] = -sg * (kz * r[buf][h][3 % NF][u] - kx * r[buf][h][4 % NF][u]);
-                            vZ2[h] = sg * (kz * r[buf][h][4 % NF][u] + kx * r[buf][h][3 % NF][u]);
+                    for (int h = 0; h < 2; ++h) {
+                        const double hx = r[buf][h][0][u], hz = r[buf][h][1][u], hy = r[buf][h][2][u];
+                        if (mean) {
+                            vP[h] = 0.0;
+                            vY[h] = 0.0;
+                            vP2[h] = k.part ? hz : hx;
+                            if (ADJ) vZ2[h] = k.part ? -r[buf][h][3 % NF][u] : r[buf][h][4 % NF][u];
+                        } else {
+                            vP[h] = sg * (kx * hx + kz * hz);
+                            vY[h] = -k2 * hy;
+                            vP2[h] = sg * (kz * hx - kx * hz);
+                            if (ADJ) {
+                                vZ[h] = -sg * (kz * r[buf][h][3 % NF][u] - kx * r[buf][h][4 % NF][u]);
+                                vZ2[h] = sg * (kz * r[buf][h][4 % NF][u] + kx * r[buf][h][3 % NF][u]);
+                            }
                         }
                     }
+                    const double sP = vP[0] + vP[1], dP = vP[0] - vP[1];
+                    const double sY = vY[0] + vY[1], dY = vY[0] - vY[1];
+                    const double sP2 = vP2[0] + vP2[1], dP2 = vP2[0] - vP2[1];
+                    const double *be = bc + u * 2 * NB * 64, *bo = be + NB * 64;
+                    aAe = mfma_f64(dP, be[0], aAe);
+                    aAo = mfma_f64(sP, bo[0], aAo);
+                    aBe = mfma_f64(sP2, be[(ADJ ? 3 : 2) * 64], aBe);
+                    aBo = mfma_f64(dP2, bo[(ADJ ? 3 : 2) * 64], aBo);
+                    aAe = mfma_f64(sY, be[64], aAe);
+                    aAo = mfma_f64(dY, bo[64], aAo);
+                    if (ADJ) {
+                        const double sZ = vZ[0] + vZ[1], dZ = vZ[0] - vZ[1];
+                        const double sZ2 = vZ2[0] + vZ2[1], dZ2 = vZ2[0] - vZ2[1];
+                        aAe = mfma_f64(sZ, be[2 * 64], aAe);
+                        aAo = mfma_f64(dZ, bo[2 * 64], aAo);
+                        aBe = mfma_f64(dZ2, be[4 * 64], aBe);
+                        aBo = mfma_f64(sZ2, bo[4 * 64], aBo);
+                    }
                 }
-                const double sP = vP[0] + vP[1], dP = vP[0] - vP[1];
-                const double sY = vY[0] + vY[1], dY = vY[0] - vY[1];
-                const double sP2 = vP2[0] + vP2[1], dP2 = vP2[0] - vP2[1];
-                const double *be = bc + u * 2 * NB * 64, *bo = be + NB * 64;
-                aAe = mfma_f64(dP, be[0], aAe);
-                aAo = mfma_f64(sP, bo[0], aAo);
-                aBe = mfma_f64(sP2, be[(ADJ ? 3 : 2) * 64], aBe);
-                aBo = mfma_f64(dP2, bo[(ADJ ? 3 : 2) * 64], aBo);
-                aAe = mfma_f64(sY, be[64], aAe);
-                aAo = mfma_f64(dY, bo[64], aAo);
-                if (ADJ) {
-                    const double sZ = vZ[0] + vZ[1], dZ = vZ[0] - vZ[1];
-                    const double sZ2 = vZ2[0] + vZ2[1], dZ2 = vZ2[0] - vZ2[1];
-                    aAe = mfma_f64(sZ, be[2 * 64], aAe);
-                    aAo = mfma_f64(dZ, bo[2 * 64], aAo);
-                    aBe = mfma_f64(dZ2, be[4 * 64], aBe);
-                    aBo = mfma_f64(sZ2, bo[4 * 64], aBo);
-                }
-            }
-        };
-#pragma unroll 1
-        for (int ch = 0; ch < NCH; ch += 2) {
-            load(k, ch + 1, 1);
-            __builtin_amdgcn_sched_barrier(0);
-            chunk(ch, 0);
-            __builtin_amdgcn_sched_barrier(0);
-            if (ch + 2 < NCH)
-                load(k, ch + 2, 0);
-            else if (tn < t1)
-                load(task_of(tn), 0, 0);
-            __builtin_amdgcn_sched_barrier(0);
-            chunk(ch + 1, 1);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        cplx *Fa = k.p ? Fa1 : Fa0, *Fb = k.p ? Fb1 : Fb0;
+            };
+            // chunks of CK k-steps through two register buffers, the loads of
+            // chunk ch + 1 issued before the products of chunk ch
+            load(k, kb / CK, 0);
 #pragma unroll
-        for (int par = 0; par < 2; ++par) {
-            const int i = 2 * (tp * 16 + nl) + par;
-            if (i >= nR) continue;
-            const mf4 &A = par ? aAo : aAe, &B = par ? aBo : aBe;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int x = k.nt * 8 + 2 * kq + h;
-                if (x >= Mx) continue;
-                if (i < nQ && !(k.z == 0 && x == 0))
-                    Fa[((size_t) k.z * nQ + i) * Mx + x] = make_double2(A[2 * h], A[2 * h + 1]);
-                Fb[((size_t) k.z * nR + i) * Mx + x] = make_double2(B[2 * h], B[2 * h + 1]);
+            for (int ch = 0; ch < NCH; ++ch) {
+                if (ch + 1 < NCH) load(k, kb / CK + ch + 1, (ch + 1) & 1);
+                __builtin_amdgcn_sched_barrier(0);
+                chunk(ch, ch & 1);
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
-        t = tn;
+        if (KS > 1) {
+            double *xw = xch + (size_t) (slot * (KS - 1) + part - 1) * XCH1 + lane;
+            if (act && part > 0) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    xw[(0 * 4 + e) * 64] = aAe[e];
+                    xw[(1 * 4 + e) * 64] = aBe[e];
+                    xw[(2 * 4 + e) * 64] = aAo[e];
+                    xw[(3 * 4 + e) * 64] = aBo[e];
+                }
+            }
+            __syncthreads();
+            if (act && part == 0) {
+#pragma unroll
+                for (int p2 = 1; p2 < KS; ++p2) {
+                    const double *xr = xch + (size_t) (slot * (KS - 1) + p2 - 1) * XCH1 + lane;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        aAe[e] += xr[(0 * 4 + e) * 64];
+                        aBe[e] += xr[(1 * 4 + e) * 64];
+                        aAo[e] += xr[(2 * 4 + e) * 64];
+                        aBo[e] += xr[(3 * 4 + e) * 64];
+                    }
+                }
+            }
+        }
+        if (act && part == 0) {
+            cplx *Fa = k.p ? Fa1 : Fa0, *Fb = k.p ? Fb1 : Fb0;
+#pragma unroll
+            for (int par = 0; par < 2; ++par) {
+                const int i = 2 * (tp * 16 + nl) + par;
+                if (i >= nR) continue;
+                const mf4 &A = par ? aAo : aAe, &B = par ? aBo : aBe;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int x = k.nt * 8 + 2 * kq + h;
+                    if (x >= Mx) continue;
+                    if (i < nQ && !(k.z == 0 && x == 0))
+                        Fa[((size_t) k.z * nQ + i) * Mx + x] = make_double2(A[2 * h], A[2 * h + 1]);
+                    Fb[((size_t) k.z * nR + i) * Mx + x] = make_double2(B[2 * h], B[2 * h + 1]);
+                }
+            }
+        }
+        // the exchange area is reused by the next round
+        if (KS > 1) __syncthreads();
     }
 }
 
@@ -1231,14 +1277,15 @@ static void launch_force_par(const ForceArgs &a, int npair, int ntask, int nprob
                              hipStream_t st)
 {
     using SH = ForceParShape<ADJ, NKS>;
-    const size_t lds = sizeof(double) * NKS * 2 * SH::NB * 64;
+    const size_t lds = SH::lds_bytes();
     static bool attr = false;
     if (!attr) {
         (void) hipFuncSetAttribute((const void *) k_force_par<ADJ, NKS>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
         attr = true;
     }
-    const int bpi = std::max(1, std::min(ntask * nprob, 256 / npair));
+    // blocks per pair: a multiple of 8 (XCD-aware mapping), one block per CU
+    const int bpi = std::max(8, std::min((ntask * nprob + 7) / 8 * 8, 256 / npair / 8 * 8));
     hipLaunchKernelGGL((k_force_par<ADJ, NKS>), dim3(npair * bpi), dim3(FR_THREADS), lds, st, a, bpi,
                        nprob, U0, Fa0, Fb0, U1, Fa1, Fb1);
 }
