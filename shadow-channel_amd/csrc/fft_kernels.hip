@@ -481,6 +481,57 @@ __device__ __forceinline__ void zinv_stage(const ZinvArgs &a, const cplx *src, i
     }
 }
 
+// Fixed lengths: the stage-in is the first pass of the transform.  Thread
+// b < ZXB NB1 takes butterfly j = b / ZXB of column xl = b % ZXB (the pass's
+// own thread mapping), forms its R1 inputs (rows zp = j + q NB1) straight
+// from U, transforms them in registers and writes the pass's output; groups
+// q whose rows all lie in the zero band (the Nyquist row and the padding)
+// are not loaded.
+template <int KIND, int PZ>
+__device__ __forceinline__ void zinv_first_pass(const ZinvArgs &a, const cplx *src, int slot, int y,
+                                                int x0, double2 *lds)
+{
+    constexpr int Nz = 2 * PZ / 3, R1 = plan_radix(PZ, 0), NB1 = PZ / R1, NBUT = ZXB * NB1;
+    static_assert(NBUT <= BLK, "one first-pass butterfly per thread");
+    const int b = threadIdx.x;
+    if (b >= NBUT) return;
+    const int j = b / ZXB, xl = b - j * ZXB, Mx = a.Mx;
+    const size_t zs = (size_t) 5 * a.qpts * Mx, ss = (size_t) a.qpts * Mx;
+    const cplx *col = src + (size_t) y * Mx + x0 + xl;
+    const double kx = (x0 + xl) * a.ax;
+    double2 v[R1];
+#pragma unroll
+    for (int q = 0; q < R1; ++q) {
+        if (q * NB1 >= Nz / 2 && q * NB1 + NB1 - 1 <= Nz) {
+            v[q] = make_double2(0.0, 0.0);
+            continue;
+        }
+        const int zp = j + q * NB1;
+        const int kz = zp < Nz / 2 ? zp : (zp > Nz ? zp - Nz / 2 : -1);
+        const int kzi = max(kz, 0);
+        const cplx *bp = col + kzi * zs;
+        const double kzv = (kzi <= Nz / 2 ? kzi : kzi - Nz) * a.az;
+        double2 val;
+        if (KIND == FK_SLOT) {
+            val = bp[slot * ss];
+        } else if (KIND == FK_OMEGA_Z) {                // -du/dy + i kx v
+            const double2 du = bp[3 * ss], vv = bp[1 * ss];
+            val = make_double2(-(du.x + kx * vv.y), -du.y + kx * vv.x);
+        } else if (KIND == FK_OMEGA_X) {                // dw/dy - i kz v
+            const double2 dw = bp[4 * ss], vv = bp[1 * ss];
+            val = make_double2(dw.x + kzv * vv.y, dw.y - kzv * vv.x);
+        } else {                                        // i kz u - i kx w
+            const double2 u = bp[0], w = bp[2 * ss];
+            val = make_double2(kx * w.y - kzv * u.y, kzv * u.x - kx * w.x);
+        }
+        v[q] = kz >= 0 ? val : make_double2(0.0, 0.0);
+    }
+    dft<R1>(v, +1.0);
+    double2 *dst = lds + xl + j * R1 * ZXB;
+#pragma unroll
+    for (int q = 0; q < R1; ++q) dst[q * ZXB] = v[q];
+}
+
 // grid (ceil(Mx/XB), nf, qpts); PZ > 0: Pz == PZ, XB == ZXB and Mx % ZXB == 0
 template <int PZ>
 __global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, cplx *W1)
@@ -495,16 +546,22 @@ __global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, cplx *W1)
     const int kind = a.fs.kind[f], slot = a.fs.slot[f];
     double2 *twl = lds + XB0 * Pz;
     stage_twiddles(pz, twl);
-    if (kind == FK_SLOT) zinv_stage<FK_SLOT, PZ>(a, src, slot, y, x0, XB, lds);
-    else if (kind == FK_OMEGA_Z) zinv_stage<FK_OMEGA_Z, PZ>(a, src, slot, y, x0, XB, lds);
-    else if (kind == FK_OMEGA_X) zinv_stage<FK_OMEGA_X, PZ>(a, src, slot, y, x0, XB, lds);
-    else zinv_stage<FK_OMEGA_Y, PZ>(a, src, slot, y, x0, XB, lds);
-    __syncthreads();
     // columns interleaved in LDS exactly as in memory: no transpose
-    if constexpr (PZ > 0)
-        lds_fft_c<PZ, ZXB, 1, ZXB>(lds, twl, +1.0);
-    else
+    if constexpr (PZ > 0) {
+        if (kind == FK_SLOT) zinv_first_pass<FK_SLOT, PZ>(a, src, slot, y, x0, lds);
+        else if (kind == FK_OMEGA_Z) zinv_first_pass<FK_OMEGA_Z, PZ>(a, src, slot, y, x0, lds);
+        else if (kind == FK_OMEGA_X) zinv_first_pass<FK_OMEGA_X, PZ>(a, src, slot, y, x0, lds);
+        else zinv_first_pass<FK_OMEGA_Y, PZ>(a, src, slot, y, x0, lds);
+        __syncthreads();
+        lds_fft_c<PZ, ZXB, 1, ZXB, 1, plan_radix(PZ, 0)>(lds, twl, +1.0);
+    } else {
+        if (kind == FK_SLOT) zinv_stage<FK_SLOT, PZ>(a, src, slot, y, x0, XB, lds);
+        else if (kind == FK_OMEGA_Z) zinv_stage<FK_OMEGA_Z, PZ>(a, src, slot, y, x0, XB, lds);
+        else if (kind == FK_OMEGA_X) zinv_stage<FK_OMEGA_X, PZ>(a, src, slot, y, x0, XB, lds);
+        else zinv_stage<FK_OMEGA_Y, PZ>(a, src, slot, y, x0, XB, lds);
+        __syncthreads();
         lds_fft(lds, XB, 1, XB, pz, twl, +1.0);
+    }
     cplx *dst = W1 + (((size_t) y * gridDim.y + f) * Pz) * Mx;
     Walk wk(XB);
     for (int e = threadIdx.x; e < XB * Pz; e += BLK, wk.next())
