@@ -238,15 +238,15 @@ static void pass2_adjoint(Context &c, const cplx *Ub, const cplx *AUb, cplx *Uds
 // products (channel/channel.c:704-780); device arrays, vout may be vin.
 // Result in scal[2].  With acc, also acc -= a vout.
 static void ddt_project_dev(Context &c, int i, const cplx *vin, cplx *vout,
-                            cplx *acc = nullptr, double a = 0.0)
+                            cplx *acc = nullptr, double a = 0.0, int twice = 0)
 {
     const Dims &d = c.d;
     long ip = i == 0 ? 3 : 3L * i, im = i == 0 ? 0 : 3L * (i - 1);
     launch_diff_scale(c, c.Ctmp2, mc(c, ip), mc(c, im), 1.0 / d.dt, c.csize);
     launch_rebuild2(c, c.Ctmp2, c.Ut, vin, c.Ut2);
     // inner products from the spectra at the Gauss points (exact grid sums)
-    launch_ddt_reduce(c, launch_ddt_dot(c, c.Ut, c.Ut2));
-    launch_ddt_apply(c, vout, vin, c.Ctmp2, c.csize, acc, a);
+    const int nparts = launch_ddt_dot(c, c.Ut, c.Ut2);
+    launch_ddt_apply(c, vout, vin, c.Ctmp2, c.csize, acc, a, nparts, twice);
 }
 
 static int check_abort(Context &c, const char *what)
@@ -676,11 +676,9 @@ static int adjoint_run(int start, int end, sc_complex *AC_given, double strength
     const double s = strength * d.dt;
     for (int n = start; n > end; --n) {
         // The projection of MIC[3n] at step n was made at the end of the
-        // previous iteration (the reference computes it twice): reuse it.
-        if (n == start)
-            ddt_project_dev(c, n, mic(c, 3L * n), c.Ctmp, c.AC, 0.5 * s);
-        else
-            launch_axpy_c(c, c.AC, c.Ctmp, 0.5 * s, c.csize);
+        // previous iteration (the reference computes it twice), which also
+        // applied it for this step's start.
+        if (n == start) ddt_project_dev(c, n, mic(c, 3L * n), c.Ctmp, c.AC, 0.5 * s);
         for (int k = 0; k < 3; ++k) {
             long count = 3L * n - k - 1;
             // with both states cached, lambda is transformed once for the pair
@@ -701,7 +699,9 @@ static int adjoint_run(int start, int end, sc_complex *AC_given, double strength
             launch_solve(c, c.AC, c.Csol, k, 1, c.FLa, c.FLb, 0.0);
             launch_rebuild(c, c.Csol, c.AU);
         }
-        ddt_project_dev(c, n - 1, mic(c, 3L * (n - 1)), c.Ctmp, c.AC, 0.5 * s);
+        // forcing of MIC[3(n-1)]: end of this step and, when there is a next
+        // step, its start (the two subtractions back to back)
+        ddt_project_dev(c, n - 1, mic(c, 3L * (n - 1)), c.Ctmp, c.AC, 0.5 * s, n - 1 > end);
     }
     int rc = stream_sync(c);
     if (rc) return rc;

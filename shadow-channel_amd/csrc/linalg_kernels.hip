@@ -2258,47 +2258,12 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
 }
 
 // ------------------------------------------------------------ vectors
-__global__ void k_axpy(cplx *y, const cplx *x, double a, size_t n, const int *abort_flag)
-{
-    if (*abort_flag) return;
-    for (size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x; e < n;
-         e += (size_t) gridDim.x * blockDim.x) {
-        y[e].x -= a * x[e].x;
-        y[e].y -= a * x[e].y;
-    }
-}
-
 __global__ void k_diff_scale(cplx *o, const cplx *p, const cplx *m, double s, size_t n)
 {
     for (size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x; e < n;
          e += (size_t) gridDim.x * blockDim.x) {
         o[e].x = s * (p[e].x - m[e].x);
         o[e].y = s * (p[e].y - m[e].y);
-    }
-}
-
-__global__ void k_ddt_reduce(const double *part, int nparts, double *scal)
-{
-    __shared__ double r0[256], r1[256];
-    double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += 256) {
-        a += part[2 * i];
-        b += part[2 * i + 1];
-    }
-    r0[threadIdx.x] = a;
-    r1[threadIdx.x] = b;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
-            r0[threadIdx.x] += r0[threadIdx.x + o];
-            r1[threadIdx.x] += r1[threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        scal[0] = r0[0];
-        scal[1] = r1[0];
-        scal[2] = r0[0] / r1[0];
     }
 }
 
@@ -2367,11 +2332,37 @@ __global__ __launch_bounds__(256) void k_ddt_dot(const cplx *A, const cplx *B, c
     }
 }
 
-// vout = vin - r dC (r = scal[2]); with acc, also acc -= a vout
-__global__ void k_ddt_apply(cplx *vout, const cplx *vin, const cplx *dC, const double *scal,
-                            size_t n, cplx *acc, double a)
+// vout = vin - r dC with r = num / den from the nparts partial pairs of
+// k_ddt_dot, summed by every block in the same order (strided over 256
+// threads, then a tree); block 0 leaves num, den, r in scal.  With acc, also
+// acc -= a vout, twice when `twice` (the adjoint's trapezoid forcing at the
+// end of one step and the start of the next, applied back to back).
+__global__ __launch_bounds__(256) void k_ddt_apply(cplx *vout, const cplx *vin, const cplx *dC,
+                                                   const double *part, int nparts, double *scal,
+                                                   size_t n, cplx *acc, double a, int twice)
 {
-    double r = scal[2];
+    __shared__ double r0[256], r1[256];
+    double sa = 0.0, sb = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 256) {
+        sa += part[2 * i];
+        sb += part[2 * i + 1];
+    }
+    r0[threadIdx.x] = sa;
+    r1[threadIdx.x] = sb;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            r0[threadIdx.x] += r0[threadIdx.x + o];
+            r1[threadIdx.x] += r1[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    const double r = r0[0] / r1[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[0] = r0[0];
+        scal[1] = r1[0];
+        scal[2] = r;
+    }
     for (size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x; e < n;
          e += (size_t) gridDim.x * blockDim.x) {
         cplx v = vin[e];
@@ -2379,8 +2370,14 @@ __global__ void k_ddt_apply(cplx *vout, const cplx *vin, const cplx *dC, const d
         v.y -= r * dC[e].y;
         vout[e] = v;
         if (acc) {
-            acc[e].x -= a * v.x;
-            acc[e].y -= a * v.y;
+            cplx w = acc[e];
+            w.x -= a * v.x;
+            w.y -= a * v.y;
+            if (twice) {
+                w.x -= a * v.x;
+                w.y -= a * v.y;
+            }
+            acc[e] = w;
         }
     }
 }
@@ -2479,11 +2476,6 @@ void launch_tangent_forcing(Context &c, const cplx *S, cplx *fa, cplx *fb, cplx 
                        d.dimQ, d.dimR, fa, fb, f0);
 }
 
-void launch_axpy_c(Context &c, cplx *y, const cplx *x, double a, size_t n)
-{
-    hipLaunchKernelGGL(k_axpy, dim3(nblocks(n)), dim3(256), 0, c.stream, y, x, a, n, c.abort_flag);
-}
-
 void launch_diff_scale(Context &c, cplx *out, const cplx *p, const cplx *m, double s, size_t n)
 {
     hipLaunchKernelGGL(k_diff_scale, dim3(nblocks(n)), dim3(256), 0, c.stream, out, p, m, s, n);
@@ -2499,16 +2491,11 @@ int launch_ddt_dot(Context &c, const cplx *A, const cplx *B)
     return nb;
 }
 
-void launch_ddt_reduce(Context &c, int nparts)
-{
-    hipLaunchKernelGGL(k_ddt_reduce, dim3(1), dim3(256), 0, c.stream, c.dotpart, nparts, c.scal);
-}
-
 void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *dC, size_t n,
-                      cplx *acc, double a)
+                      cplx *acc, double a, int nparts, int twice)
 {
-    hipLaunchKernelGGL(k_ddt_apply, dim3(nblocks(n)), dim3(256), 0, c.stream, vout, vin, dC,
-                       c.scal, n, acc, a);
+    hipLaunchKernelGGL(k_ddt_apply, dim3(256), dim3(256), 0, c.stream, vout, vin, dC, c.dotpart,
+                       nparts, c.scal, n, acc, a, twice);
 }
 
 void launch_cfl_check(Context &c, int slot)

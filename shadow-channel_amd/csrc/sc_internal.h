@@ -194,12 +194,10 @@ int solve_columns_per_block(int dimR);
 // Re-sensitivity forcing of state S: fa [Nz][dimQ][Mx], fb [Nz][dimR][Mx], f0 [2][dimR]
 void launch_tangent_forcing(Context &c, const cplx *S, cplx *fa, cplx *fb, cplx *f0);
 int xmix_zblock(Context &c, int nf);
-void launch_axpy_c(Context &c, cplx *y, const cplx *x, double a, size_t n);
 void launch_diff_scale(Context &c, cplx *out, const cplx *a, const cplx *b, double s, size_t n);
 // vout = vin - (num/den) dC; with acc, also acc -= a vout
 void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *dC, size_t n,
-                      cplx *acc = nullptr, double a = 0.0);
-void launch_ddt_reduce(Context &c, int nparts);
+                      cplx *acc, double a, int nparts, int twice);
 // partial inner products of ddt_project from Gauss-point spectra; returns the
 // number of partial pairs written to dotpart
 int launch_ddt_dot(Context &c, const cplx *A, const cplx *B);
