@@ -173,8 +173,7 @@ static void pass1_state(Context &c, cplx *Ub, int check_cfl, cplx *wdst = nullpt
     cplx *w = wdst ? wdst : c.W1;
     launch_zinv(c, velocity_and_vorticity(Ub), w);
     launch_xmix(c, XM_PRIMAL, w, c.W2, check_cfl, nullptr, nullptr);
-    if (check_cfl) launch_cfl_check(c, slot);
-    launch_zfwd(c, c.W2, 3, Ub, ZF_USLOTS);
+    launch_zfwd(c, c.W2, 3, Ub, ZF_USLOTS, check_cfl ? slot : -1);
 }
 
 // IH = u x I omega + I u x omega into the H slots of IUb (channel/pass1.c).
@@ -196,8 +195,7 @@ static void pass1_tangent(Context &c, const cplx *Ub, cplx *IUb, int check_cfl,
         launch_zinv(c, a, c.W1);
         launch_xmix(c, XM_TANGENT, c.W1, c.W2, check_cfl, nullptr, nullptr);
     }
-    if (check_cfl) launch_cfl_check(c, 0);
-    launch_zfwd(c, c.W2, 3, IUb, ZF_USLOTS);
+    launch_zfwd(c, c.W2, 3, IUb, ZF_USLOTS, check_cfl ? 0 : -1);
 }
 
 // adjoint convective operator (channel/pass2.c, adjoint side) into Udst.

@@ -849,6 +849,12 @@ struct ZfwdArgs {
     int nout, Nz, Mx, Pz, qpts, XB, dst_kind;
     double norm;
     const int *abort_flag;
+    // cfl_slot >= 0: the first block runs the CFL check of the products
+    // kernel's maxima first (in place of a separate launch)
+    int cfl_slot;
+    double *cflmax;
+    double dt;
+    int *abort_rw;
 };
 
 // grid (ceil(Mx/XB), nout, qpts); PZ > 0: Pz == PZ, XB == ZXB and Mx % ZXB == 0
@@ -856,6 +862,8 @@ template <int PZ>
 __global__ __launch_bounds__(BLK) void k_zfwd(ZfwdArgs a, PlanDev pz, const cplx *W2, cplx *dst)
 {
     extern __shared__ double2 lds[];
+    if (a.cfl_slot >= 0 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0)
+        cfl_check_block(a.cflmax, a.qpts, a.dt, a.abort_rw, a.cfl_slot);
     if (*a.abort_flag) return;
     const int XB0 = PZ > 0 ? ZXB : a.XB;
     const int x0 = blockIdx.x * XB0, g = blockIdx.y, y = blockIdx.z;
@@ -1227,10 +1235,14 @@ int xmix_zblock(Context &c, int nf)
     return zb;
 }
 
-void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind)
+void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind, int cfl_slot)
 {
     const Dims &d = c.d;
     ZfwdArgs a;
+    a.cfl_slot = cfl_slot;
+    a.cflmax = c.cflmax;
+    a.dt = d.dt;
+    a.abort_rw = c.abort_flag;
     a.nout = nout;
     a.Nz = d.Nz;
     a.Mx = d.Mx;

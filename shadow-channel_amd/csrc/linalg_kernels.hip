@@ -2382,32 +2382,6 @@ __global__ __launch_bounds__(256) void k_ddt_apply(cplx *vout, const cplx *vin, 
     }
 }
 
-// The first plane whose CFL number reaches the limit sets abort_flag[0] =
-// 1 + plane, abort_flag[1] = slot (the first stored state the stopped step
-// would have written) and leaves its maximum in cflmax[qpts]; the per-plane
-// maxima are cleared for the next check.  One block of CFL_THREADS.
-constexpr int CFL_THREADS = 256;
-__global__ __launch_bounds__(CFL_THREADS) void k_cfl_check(double *cflmax, int qpts, double dt,
-                                                           int *abort_flag, int slot)
-{
-    __shared__ int first;
-    if (threadIdx.x == 0) first = qpts;
-    __syncthreads();
-    const bool live = *abort_flag == 0;
-    for (int y = threadIdx.x; y < qpts; y += CFL_THREADS) {
-        if (live && cflmax[y] * dt >= 2.8) atomicMin(&first, y);
-    }
-    __syncthreads();
-    const int y0 = first;
-    if (y0 < qpts && threadIdx.x == 0) {
-        cflmax[qpts] = cflmax[y0];
-        abort_flag[1] = slot;
-        abort_flag[0] = 1 + y0;
-    }
-    __syncthreads();
-    for (int y = threadIdx.x; y < qpts; y += CFL_THREADS) cflmax[y] = 0.0;
-}
-
 static int nblocks(size_t n) { return (int) std::min<size_t>((n + 255) / 256, 8192); }
 
 // Reynolds-number sensitivity forcing of a stored state S = MC[count]
@@ -2496,12 +2470,6 @@ void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *dC, s
 {
     hipLaunchKernelGGL(k_ddt_apply, dim3(256), dim3(256), 0, c.stream, vout, vin, dC, c.dotpart,
                        nparts, c.scal, n, acc, a, twice);
-}
-
-void launch_cfl_check(Context &c, int slot)
-{
-    hipLaunchKernelGGL(k_cfl_check, dim3(1), dim3(CFL_THREADS), 0, c.stream, c.cflmax, c.d.qpts,
-                       c.d.dt, c.abort_flag, slot);
 }
 
 }  // namespace sc

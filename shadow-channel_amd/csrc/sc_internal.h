@@ -173,7 +173,9 @@ void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1);
 // ([qpts][nf-nf1][Pz][Mx]); W1b null: all fields from W1.
 void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
                  double *flow, double *dot_out, const cplx *W1b = nullptr, int nf1 = 0);
-void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind);
+// cfl_slot >= 0: the first block runs the CFL check (cfl_check_block) with
+// that slot: the first stored state of the step being checked
+void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind, int cfl_slot = -1);
 void launch_assemble_adj(Context &c, const cplx *SP, cplx *Udst);
 // forward z transform of the six adjoint products fused with the assembly
 // into the five state slots of Udst; false when the tile does not fit
@@ -188,8 +190,6 @@ void launch_force2(Context &c, int adj, const cplx *U0, cplx *Fa0, cplx *Fb0, co
                    cplx *Fa1, cplx *Fb1);
 void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cplx *Fa2,
                   const cplx *Fb2, double target);
-// slot: first stored state of the step being checked (kept when it stops)
-void launch_cfl_check(Context &c, int slot);
 int solve_columns_per_block(int dimR);
 // Re-sensitivity forcing of state S: fa [Nz][dimQ][Mx], fb [Nz][dimR][Mx], f0 [2][dimR]
 void launch_tangent_forcing(Context &c, const cplx *S, cplx *fa, cplx *fb, cplx *f0);
@@ -214,6 +214,32 @@ void prof_end(Context &c, hipStream_t s, int kernel_id,
 
 std::string &last_error();
 }  // namespace sc
+
+// CFL check by one block (any size): the first plane whose CFL number
+// reaches the limit sets abort_flag[0] = 1 + plane, abort_flag[1] = slot
+// (the first stored state the stopped step would have written) and leaves
+// its maximum in cflmax[qpts]; the per-plane maxima are cleared for the next
+// check (channel/pass1.c CFL test)
+__device__ __forceinline__ void cfl_check_block(double *cflmax, int qpts, double dt, int *abort_flag,
+                                                int slot)
+{
+    __shared__ int first;
+    if (threadIdx.x == 0) first = qpts;
+    __syncthreads();
+    const bool live = *abort_flag == 0;
+    for (int y = threadIdx.x; y < qpts; y += blockDim.x) {
+        if (live && cflmax[y] * dt >= 2.8) atomicMin(&first, y);
+    }
+    __syncthreads();
+    const int y0 = first;
+    if (y0 < qpts && threadIdx.x == 0) {
+        cflmax[qpts] = cflmax[y0];
+        abort_flag[1] = slot;
+        abort_flag[0] = 1 + y0;
+    }
+    __syncthreads();
+    for (int y = threadIdx.x; y < qpts; y += blockDim.x) cflmax[y] = 0.0;
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs in launch order, and
 // each XCD has its own L2.  With a 1-D grid of 8*ceil(n/8) blocks, block b
