@@ -89,6 +89,24 @@ int c_save_solution(const char *filename, sc_complex *C_ptr,
 int sc_tangent_forcing(int count, sc_complex *f_a, sc_complex *f_b, sc_complex *f0_a,
                        sc_complex *f0_b);
 
+/* channel/initAlphaBeta2.c:70-81, 210-223 (the wall-plane part of
+   initAlphaBeta2, which the reference's read_solution runs at
+   channel/channel.c:868): Uxb, Uzb [Nz][Nx/2] of the coefficient array C
+   [Nz][2][dimR][Nx/2].  (0,0): Uxb = Rp0.alpha, Uzb = Rp0.beta; other
+   columns: Uxb = -i kz/k2 Rp0.beta, Uzb = i kx/k2 Rp0.beta, with
+   Rp0_j = R_j'(-1).  The reference keeps them in globals; here the caller
+   owns both planes. */
+int sc_wall_plane(const sc_complex *C_ptr, sc_complex *Uxb, sc_complex *Uzb);
+
+/* channel/increBoundary.c:20-162 (int increBoundary(void), called at
+   channel/channel.c:872): in place on the wall planes Uxb, Uzb [Nz][Nx/2]:
+   the dealiased products Uxb (w.n) and Uzb (w.n) with the fixed single-mode
+   plane w.n (Re = 1 at kz row 1, kx 1: channel/increBoundary.c:66), then
+   Uxb <- i kz Uxb - i kx Uzb and Uzb <- -i kx Uxb - i kz Uzb at every
+   (z,x) != (0,0); row Nz/2 of the result is zero.  SC_EINVAL for Nx or
+   Nz < 4. */
+int sc_incre_boundary(sc_complex *Uxb, sc_complex *Uzb);
+
 /* channel/channel.i:206-218: Gauss points and weights (library-owned) */
 int c_quad(double **y_ptr, int *qpts_ptr1, double **w_ptr, int *qpts_ptr2);
 

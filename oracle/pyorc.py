@@ -65,6 +65,9 @@ def load():
     L.orc_set_faithful_cost.argtypes = [c.c_int]
     L.orc_srand.argtypes = [c.c_uint]
     L.orc_tangent_forcing.argtypes = [c.c_int, c.c_void_p, c.c_void_p, c.c_void_p, c.c_void_p]
+    L.orc_wall_plane.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p]
+    L.orc_incre_boundary.argtypes = [c.c_void_p, c.c_void_p]
+    L.orc_incre_boundary.restype = c.c_int
     L.orc_rand.restype = c.c_int
     for f in ("orc_qpts", "orc_dimR", "orc_dimQ", "orc_Nx", "orc_Nz",
               "orc_nsteps", "orc_last_status"):
@@ -192,6 +195,22 @@ class Oracle:
         f0b = np.zeros(nR, complex)
         assert self.L.orc_tangent_forcing(count, _ptr(fa), _ptr(fb), _ptr(f0a), _ptr(f0b)) == 0
         return fa, fb, f0a, f0b
+
+    def wall_plane(self, C):
+        """Wall plane of C (initAlphaBeta2): (Uxb, Uzb), each [Nz][Nx/2]."""
+        C = _check_c(np.ascontiguousarray(C))
+        ub = np.zeros((self.Nz, self.Nx // 2), complex)
+        wb = np.zeros((self.Nz, self.Nx // 2), complex)
+        self.L.orc_wall_plane(_ptr(C), _ptr(ub), _ptr(wb))
+        return ub, wb
+
+    def incre_boundary(self, Uxb, Uzb):
+        """increBoundary on copies of two wall planes; returns (Uxb, Uzb)."""
+        ub = np.array(Uxb, dtype=np.complex128, order="C")
+        wb = np.array(Uzb, dtype=np.complex128, order="C")
+        assert ub.shape == wb.shape == (self.Nz, self.Nx // 2)
+        assert self.L.orc_incre_boundary(_ptr(ub), _ptr(wb)) == 0
+        return ub, wb
 
     def last_status(self):
         return self.L.orc_last_status()

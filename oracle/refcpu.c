@@ -717,6 +717,63 @@ static void coef_to_gauss(const cx *Ca, cx *Ua, int wall)
 
 void orc_coef_to_gauss(const orc_cplx *Ca, orc_cplx *Ua) { coef_to_gauss(Ca, Ua, 0); }
 
+/* Wall plane of a coefficient array (channel/initAlphaBeta2.c:70-81,
+   210-223): Uxb = Rp0.alpha, Uzb = Rp0.beta at (0,0); elsewhere
+   Uxb = -i kz/k2 Rp0.beta, Uzb = i kx/k2 Rp0.beta (every row, Nz/2 too). */
+void orc_wall_plane(const orc_cplx *Ca, orc_cplx *Uxb, orc_cplx *Uzb)
+{
+    wall_mean(Ca);
+    for (int z = 0; z < G.Nz; ++z)
+        for (int x = (z == 0 ? 1 : 0); x < G.Mx; ++x) wall_accumulate(Ca, z, x);
+    memcpy(Uxb, G.Uxb, sizeof(cx) * G.Nz * G.Mx);
+    memcpy(Uzb, G.Uzb, sizeof(cx) * G.Nz * G.Mx);
+}
+
+/* increBoundary (channel/increBoundary.c:20-162), in place on two wall
+   planes [Nz][Nx/2]: both planes and the fixed plane w.n (a single mode,
+   Re = 1 at kz row 1, kx 1: channel/increBoundary.c:66) go through the
+   padded inverse transform, the products Uxb w.n and Uzb w.n through the
+   forward transform (scaled by 1/(Px Pz), row Nz/2 left zero), then every
+   (z, x) != (0, 0) is rewritten as
+       Uxb <- i kz Uxb - i kx Uzb,   Uzb <- -i kx Uxb - i kz Uzb
+   (channel/increBoundary.c:137-158). */
+int orc_incre_boundary(orc_cplx *Uxb, orc_cplx *Uzb)
+{
+    const int Nz = G.Nz, Mx = G.Mx;
+    const long np = (long) G.Pz * G.Px;
+    if (Nz < 4 || Mx < 2) return -1;
+    cx *wn = calloc((size_t) Nz * Mx, sizeof(cx));
+    if (!wn) return -1;
+    wn[1 * Mx + 1].re = 1.0;
+    double *px = G.phys, *pz = G.phys + np, *pw = G.phys + 2 * np;
+    inv2d(Uxb, Mx, px);
+    inv2d(Uzb, Mx, pz);
+    inv2d(wn, Mx, pw);
+    free(wn);
+    for (long i = 0; i < np; ++i) {
+        px[i] = px[i] * pw[i];
+        pz[i] = pz[i] * pw[i];
+    }
+    fwd2d(px, Uxb, Mx);
+    fwd2d(pz, Uzb, Mx);
+    memset(Uxb + (long) (Nz / 2) * Mx, 0, sizeof(cx) * Mx);
+    memset(Uzb + (long) (Nz / 2) * Mx, 0, sizeof(cx) * Mx);
+    for (int z = 0; z < Nz; ++z)
+        for (int x = 0; x < Mx; ++x) {
+            if (z * z + x * x == 0) continue;
+            const double kx = G.Kx[x], kz = G.Kz[z];
+            cx *a = Uxb + (long) z * Mx + x, *b = Uzb + (long) z * Mx + x;
+            const double t1 = -kz * a->im + kx * b->im;
+            const double t2 = kz * a->re - kx * b->re;
+            const double t3 = kx * a->im + kz * b->im;
+            b->im = -kx * a->re - kz * b->re;
+            b->re = t3;
+            a->re = t1;
+            a->im = t2;
+        }
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* Galerkin projection of the nonlinear term (channel/project.c:95-115, */
 /* 176-191; channel/adjproject.c:108-134, 213-234)                      */

@@ -85,6 +85,10 @@ void   orc_pass2_state(orc_cplx *U, const orc_cplx *AU);
 void   orc_coef_to_gauss(const orc_cplx *C, orc_cplx *U);
 /* Copy the current working state C / U of the solver. */
 void   orc_get_state(orc_cplx *C, orc_cplx *U);
+/* Wall plane of C (initAlphaBeta2's Uxb, Uzb): two [Nz][Nx/2] planes. */
+void   orc_wall_plane(const orc_cplx *C, orc_cplx *Uxb, orc_cplx *Uzb);
+/* increBoundary in place on two wall planes; -1 if Nz < 4 or Nx < 4. */
+int    orc_incre_boundary(orc_cplx *Uxb, orc_cplx *Uzb);
 
 #ifdef __cplusplus
 }

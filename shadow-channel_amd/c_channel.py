@@ -76,6 +76,10 @@ _L.sc_field_cache_active.argtypes = []
 _L.sc_field_cache_active.restype = _i
 _L.sc_tangent_forcing.argtypes = [_i, _c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p]
 _L.sc_tangent_forcing.restype = _i
+_L.sc_wall_plane.argtypes = [_c.c_void_p, _c.c_void_p, _c.c_void_p]
+_L.sc_wall_plane.restype = _i
+_L.sc_incre_boundary.argtypes = [_c.c_void_p, _c.c_void_p]
+_L.sc_incre_boundary.restype = _i
 _L.sc_primal_dev.argtypes = [_i, _c.c_void_p]
 _L.sc_primal_dev.restype = _i
 _L.sc_adjoint_dev.argtypes = [_i, _i, _c.c_void_p, _d]
@@ -325,3 +329,29 @@ def sc_tangent_forcing(count):
     _check(_L.sc_tangent_forcing(int(count), fa.ctypes.data, fb.ctypes.data, f0a.ctypes.data,
                                  f0b.ctypes.data), "sc_tangent_forcing")
     return fa, fb, f0a, f0b
+
+
+def sc_wall_plane(C_ptr):
+    """Wall plane of a coefficient array (the Uxb, Uzb of initAlphaBeta2,
+    channel/initAlphaBeta2.c:70-81, 210-223): returns (Uxb, Uzb), each
+    [Nz, Nx/2] complex128."""
+    C = _inplace(C_ptr, np.complex128, "C")
+    Nz, Mx, nR = _L.c_Nz(), _L.c_Nx() // 2, _L.c_dimR()
+    assert C.shape == (Nz, 2, nR, Mx), "C must be (Nz, 2, dimR, Nx/2)"
+    ub = np.zeros((Nz, Mx), complex)
+    wb = np.zeros((Nz, Mx), complex)
+    _check(_L.sc_wall_plane(C.ctypes.data, ub.ctypes.data, wb.ctypes.data), "sc_wall_plane")
+    return ub, wb
+
+
+def sc_incre_boundary(Uxb, Uzb):
+    """increBoundary (channel/increBoundary.c:20-162) in place on two wall
+    planes, each a C-contiguous complex128 [Nz, Nx/2] array."""
+    shape = (_L.c_Nz(), _L.c_Nx() // 2)
+    for a, name in ((Uxb, "Uxb"), (Uzb, "Uzb")):
+        if not isinstance(a, np.ndarray) or a.dtype != np.complex128:
+            raise TypeError("%s must be a complex128 numpy array" % name)
+        if a.shape != shape or not a.flags.c_contiguous or not a.flags.writeable:
+            raise TypeError("%s must be C-contiguous, writeable, of shape (Nz, Nx/2)" % name)
+    ub, wb = Uxb, Uzb
+    _check(_L.sc_incre_boundary(ub.ctypes.data, wb.ctypes.data), "sc_incre_boundary")

@@ -905,6 +905,42 @@ int sc_tangent_forcing(int count, sc_complex *f_a, sc_complex *f_b, sc_complex *
     return SC_OK;
 }
 
+int sc_wall_plane(const sc_complex *C_ptr, sc_complex *Uxb, sc_complex *Uzb)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!C_ptr || !Uxb || !Uzb) {
+        last_error() = "wall_plane requires C and two output planes";
+        return SC_EINVAL;
+    }
+    const Dims &d = c.d;
+    const size_t np = (size_t) d.Nz * d.Mx;
+    SC_HIP(hipMemcpyAsync(c.Ctmp, C_ptr, sizeof(cplx) * c.csize, hipMemcpyHostToDevice, c.stream));
+    launch_wall_plane(c, c.Ctmp, c.Ctmp2, c.Ctmp2 + np);
+    SC_HIP(hipMemcpyAsync(Uxb, c.Ctmp2, sizeof(cplx) * np, hipMemcpyDeviceToHost, c.stream));
+    SC_HIP(hipMemcpyAsync(Uzb, c.Ctmp2 + np, sizeof(cplx) * np, hipMemcpyDeviceToHost, c.stream));
+    return stream_sync(c);
+}
+
+int sc_incre_boundary(sc_complex *Uxb, sc_complex *Uzb)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    const Dims &d = c.d;
+    if (!Uxb || !Uzb || d.Nz < 4 || d.Mx < 2) {
+        last_error() = "incre_boundary requires two wall planes and Nx, Nz >= 4";
+        return SC_EINVAL;
+    }
+    const size_t np = (size_t) d.Nz * d.Mx;
+    // planes in Ctmp2 (2 Nz Mx <= one coefficient array), transforms in W1
+    SC_HIP(hipMemcpyAsync(c.Ctmp2, Uxb, sizeof(cplx) * np, hipMemcpyHostToDevice, c.stream));
+    SC_HIP(hipMemcpyAsync(c.Ctmp2 + np, Uzb, sizeof(cplx) * np, hipMemcpyHostToDevice, c.stream));
+    launch_incre_boundary(c, c.Ctmp2, c.Ctmp2 + np, c.W1);
+    SC_HIP(hipMemcpyAsync(Uxb, c.Ctmp2, sizeof(cplx) * np, hipMemcpyDeviceToHost, c.stream));
+    SC_HIP(hipMemcpyAsync(Uzb, c.Ctmp2 + np, sizeof(cplx) * np, hipMemcpyDeviceToHost, c.stream));
+    return stream_sync(c);
+}
+
 int c_qpts(void) { return ctx().d.qpts; }
 int c_dimR(void) { return ctx().d.dimR; }
 int c_dimQ(void) { return ctx().d.dimQ; }

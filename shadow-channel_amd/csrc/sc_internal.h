@@ -195,6 +195,11 @@ int solve_columns_per_block(int dimR);
 void launch_tangent_forcing(Context &c, const cplx *S, cplx *fa, cplx *fb, cplx *f0);
 int xmix_zblock(Context &c, int nf);
 void launch_diff_scale(Context &c, cplx *out, const cplx *a, const cplx *b, double s, size_t n);
+// wall plane of C: Uxb, Uzb [Nz][Mx]
+void launch_wall_plane(Context &c, const cplx *C, cplx *Uxb, cplx *Uzb);
+// incremental boundary condition in place on Uxb, Uzb; scratch holds
+// 5 Pz Mx complex values
+void launch_incre_boundary(Context &c, cplx *Uxb, cplx *Uzb, cplx *scratch);
 // vout = vin - (num/den) dC; with acc, also acc -= a vout
 void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *dC, size_t n,
                       cplx *acc, double a, int nparts, int twice);

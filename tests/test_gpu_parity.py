@@ -289,6 +289,35 @@ def test_tangent_forcing(cc):
             assert np.linalg.norm(g - r) <= 1e-12 * np.linalg.norm(r)
 
 
+@pytest.mark.parametrize("cfg", [SMALL, G0, BENCH], ids=["16x33x12", "32x65x32", "64x97x64"])
+def test_wall_plane_and_incre_boundary(cc, cfg):
+    """The wall plane of a state (channel/initAlphaBeta2.c:70-81, 210-223) and
+    increBoundary on it and on random planes (channel/increBoundary.c:20-162)
+    against the oracle.  The random planes carry a nonzero row Nz/2 (not
+    read) and a non-Hermitian kx = 0 column (read through its real part in
+    z).  Both are short sums and products, so agreement is at rounding
+    level: 1e-13 (wall plane), 1e-12 (increBoundary)."""
+    print("case: wall plane + increBoundary, %dx%dx%d" % (cfg["Nx"], cfg["Ny"], cfg["Nz"]),
+          flush=True)
+    o, C0 = start(cc, cfg, 1, amp=0.1)
+    ub, wb = cc.sc_wall_plane(C0)
+    rub, rwb = o.wall_plane(C0)
+    assert rel(ub, rub) <= 1e-13 and rel(wb, rwb) <= 1e-13
+    rng = np.random.default_rng(21)
+    Nz, Mx = cfg["Nz"], cfg["Nx"] // 2
+    planes = [(ub, wb)]
+    planes.append(tuple(rng.standard_normal((Nz, Mx)) + 1j * rng.standard_normal((Nz, Mx))
+                        for _ in range(2)))
+    for x0, z0 in planes:
+        gx, gz = x0.copy(), z0.copy()
+        cc.sc_incre_boundary(gx, gz)
+        rx, rz = o.incre_boundary(x0, z0)
+        assert rel(gx, rx) <= 1e-12 and rel(gz, rz) <= 1e-12
+        assert np.all(gx[Nz // 2] == 0) and np.all(gz[Nz // 2] == 0)
+    with pytest.raises(TypeError):
+        cc.sc_incre_boundary(np.zeros((Nz, Mx + 1), complex), np.zeros((Nz, Mx), complex))
+
+
 def test_device_resident_entry_points(cc):
     """sc_primal_dev / sc_adjoint_dev (states in device memory) run the same
     launches as c_primal / c_adjoint on host arrays: bitwise equal results,
