@@ -391,10 +391,13 @@ static_assert(ZXB * 96 + 96 <= FFT_TILE, "z tile of the length-96 kernels");
 
 // rows per block of the fixed-length x kernel: the largest power of 2 with
 // the tile (rows * pairs * (PX + 1) + PX twiddles) within FFT_TILE
+#ifndef SC_XMIX_ZBMAX
+#define SC_XMIX_ZBMAX 64
+#endif
 constexpr int xmix_rows(int px, int nseq)
 {
     int zb = 1;
-    while (2 * zb * nseq * (px + 1) + px <= FFT_TILE) zb *= 2;
+    while (2 * zb <= SC_XMIX_ZBMAX && 2 * zb * nseq * (px + 1) + px <= FFT_TILE) zb *= 2;
     return zb;
 }
 
@@ -1229,7 +1232,7 @@ void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
 int xmix_zblock(Context &c, int nf)
 {
     int nseq = (nf + 1) / 2, zb = 1;
-    while (zb * 2 <= c.d.Pz && c.d.Pz % (zb * 2) == 0 &&
+    while (zb * 2 <= c.d.Pz && zb * 2 <= SC_XMIX_ZBMAX && c.d.Pz % (zb * 2) == 0 &&
            (size_t) zb * 2 * nseq * (c.d.Px + 1) + c.d.Px <= (size_t) XMIX_TILE)
         zb *= 2;
     return zb;
