@@ -240,11 +240,11 @@ static void ddt_project_dev(Context &c, int i, const cplx *vin, cplx *vout,
 {
     const Dims &d = c.d;
     long ip = i == 0 ? 3 : 3L * i, im = i == 0 ? 0 : 3L * (i - 1);
-    launch_diff_scale(c, c.Ctmp2, mc(c, ip), mc(c, im), 1.0 / d.dt, c.csize);
-    launch_rebuild2(c, c.Ctmp2, c.Ut, vin, c.Ut2);
-    // inner products from the spectra at the Gauss points (exact grid sums)
-    const int nparts = launch_ddt_dot(c, c.Ut, c.Ut2);
-    launch_ddt_apply(c, vout, vin, c.Ctmp2, c.csize, acc, a, nparts, twice);
+    // dudt = (MC[ip] - MC[im]) / dt; the inner products in closed form from
+    // the coefficients (equal to the grid sums of the rebuilt fields)
+    const int nparts = launch_ddt_gram(c, mc(c, ip), mc(c, im), 1.0 / d.dt, vin);
+    launch_ddt_apply(c, vout, vin, mc(c, ip), mc(c, im), 1.0 / d.dt, c.csize, acc, a, nparts,
+                     twice);
 }
 
 static int check_abort(Context &c, const char *what)

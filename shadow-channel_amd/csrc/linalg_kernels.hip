@@ -2258,62 +2258,113 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
 }
 
 // ------------------------------------------------------------ vectors
-__global__ void k_diff_scale(cplx *o, const cplx *p, const cplx *m, double s, size_t n)
+// Inner products of ddt_project (channel/channel.c:704-780) straight from the
+// coefficients.  The reference forms <f, g> = sum_y W_y sum_{x,z} f g over the
+// velocities of f and g rebuilt on the dealiased Px x Pz x qpts grid.  Those
+// sums are exact integrals: Parseval in x and z (the fields are band-limited
+// below the padded grid's Nyquist bins), and in y the Gauss rule with qpts
+// points integrates the products of the rebuilt fields exactly (degree at
+// most 2 dimQ + 6 < 2 qpts).  With u = (i kx v' - i kz w_y)/k2 and
+// w = (i kz v' + i kx w_y)/k2, u conj(u~) + w conj(w~) = (v' conj(v~') +
+// w_y conj(w~_y))/k2, so per column (kz, kx) != (0, 0)
+//     <f, g>_col = c Re[ a_f^H (Qs + Qps/k2) a_g + b_f^H (Rs/k2) b_g ]
+// with the Gauss-rule Gram bands Qs = sum W Q Q, Qps = sum W Q' Q',
+// Rs = sum W R R of the setup, alpha rows a < dimQ, beta rows b < dimR,
+// c = 2 for kx > 0 (the conjugate column) and c = 1 at kx = 0, where the
+// coefficients are replaced by their Hermitian part (c(kz) + conj c(-kz))/2,
+// the only part the complex-to-real x transform sees.  The mean column holds
+// the u and w profiles in the R basis: Re a_f^T Rs Re a_g + Re b_f^T Rs Re b_g
+// (the Hermitian part of column (0,0) is its real part).  Row Nz/2 is not
+// transformed and counts nothing.  The sums differ from the grid sums only
+// in rounding.  num = <dudt, V>, den = <dudt, dudt> (times Px Pz, the grid's
+// Parseval factor) with dudt = s (P - M) formed on load.
+//
+// Block: kz row z and ncol columns; dudt and V of the block's columns in
+// LDS ([var][row][column]), then one (var, row, column) item per thread and
+// pass; one partial pair per block.
+__global__ __launch_bounds__(256) void k_ddt_gram(const cplx *P, const cplx *M, const cplx *V,
+                                                  double s, const double *Qs, const double *Qps,
+                                                  const double *Rs, const double *Kx,
+                                                  const double *Kz, int Nz, int Mx, int dimR,
+                                                  int dimQ, int ncol, double scale, double *part,
+                                                  const int *abort_flag)
 {
-    for (size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x; e < n;
-         e += (size_t) gridDim.x * blockDim.x) {
-        o[e].x = s * (p[e].x - m[e].x);
-        o[e].y = s * (p[e].y - m[e].y);
-    }
-}
-
-// Weighted inner products of ddt_project (channel/channel.c:704-780) from the
-// Gauss-point spectra: num = <A, B>, den = <A, A> over u, v, w (slots 0..2),
-// where <f, g> = sum_y W_y sum_{x,z} f g on the dealiased Px x Pz grid.  The
-// fields are band-limited below the grid's Nyquist bins in x and z (3/2
-// rule), so by Parseval the grid sums are exact in spectral space:
-//     sum_{x,z} f g = Px Pz (sum_kz Re(Hf conj Hg) + 2 sum_{kx>0,kz} Re(F conj G)),
-// with Hf(kz) = (F(0,kz) + conj F(0,-kz)) / 2 the kx = 0 column as the
-// complex-to-real x transform uses it (real part of its z transform).  The
-// Nyquist row kz = Nz/2 carries nothing.  One partial pair per block.
-__global__ __launch_bounds__(256) void k_ddt_dot(const cplx *A, const cplx *B, const double *W,
-                                                 double scale, int Nz, int qpts, int Mx,
-                                                 double *part, const int *abort_flag)
-{
+    extern __shared__ double2 sh_ddt[];
     __shared__ double r0[256], r1[256];
-    double num = 0.0, den = 0.0;
-    if (!*abort_flag) {
-        const size_t n = (size_t) Nz * qpts * Mx, ss = (size_t) qpts * Mx;
-        for (size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x; e < n;
-             e += (size_t) gridDim.x * blockDim.x) {
-            const int x = (int) (e % Mx), y = (int) ((e / Mx) % qpts), z = (int) (e / ss);
-            if (z == Nz / 2) continue;
-            const size_t o = (size_t) z * 5 * ss + (size_t) y * Mx + x;
-            double sn = 0.0, sd = 0.0;
-            if (x > 0) {
-#pragma unroll
-                for (int s = 0; s < 3; ++s) {
-                    const cplx a = A[o + s * ss], b = B[o + s * ss];
-                    sn += a.x * b.x + a.y * b.y;
-                    sd += a.x * a.x + a.y * a.y;
-                }
-                sn *= 2.0;
-                sd *= 2.0;
-            } else {
-                const int zm = z == 0 ? 0 : Nz - z;
-                const size_t om = (size_t) zm * 5 * ss + (size_t) y * Mx;
-#pragma unroll
-                for (int s = 0; s < 3; ++s) {
-                    const cplx a = A[o + s * ss], am = A[om + s * ss];
-                    const cplx b = B[o + s * ss], bm = B[om + s * ss];
-                    const double har = 0.5 * (a.x + am.x), hai = 0.5 * (a.y - am.y);
-                    const double hbr = 0.5 * (b.x + bm.x), hbi = 0.5 * (b.y - bm.y);
-                    sn += har * hbr + hai * hbi;
-                    sd += har * har + hai * hai;
+    const int nxb = (Mx + ncol - 1) / ncol;
+    const int z = blockIdx.x / nxb, x0 = (blockIdx.x - z * nxb) * ncol;
+    const bool live = !*abort_flag && z != Nz / 2;
+    const int nit = 2 * dimR * ncol;
+    double2 *sd = sh_ddt, *sv = sh_ddt + nit;
+    if (live) {
+        const int zm = z == 0 ? 0 : Nz - z;
+        for (int e = threadIdx.x; e < nit; e += 256) {
+            const int col = e % ncol, j = (e / ncol) % dimR, var = e / (ncol * dimR);
+            const int x = x0 + col;
+            double2 d = make_double2(0.0, 0.0), v = d;
+            if (x < Mx) {
+                const size_t o = (((size_t) z * 2 + var) * dimR + j) * Mx + x;
+                const cplx p = P[o], m = M[o];
+                v = V[o];
+                d = make_double2(s * (p.x - m.x), s * (p.y - m.y));
+                if (x == 0) {
+                    const size_t om = (((size_t) zm * 2 + var) * dimR + j) * Mx;
+                    const cplx pm = P[om], mm = M[om], vm = V[om];
+                    const double dmx = s * (pm.x - mm.x), dmy = s * (pm.y - mm.y);
+                    d = make_double2(0.5 * (d.x + dmx), 0.5 * (d.y - dmy));
+                    v = make_double2(0.5 * (v.x + vm.x), 0.5 * (v.y - vm.y));
                 }
             }
-            num += W[y] * sn;
-            den += W[y] * sd;
+            sd[e] = d;
+            sv[e] = v;
+        }
+    }
+    __syncthreads();
+    double num = 0.0, den = 0.0;
+    if (live) {
+        const double kz = Kz[z];
+        for (int e = threadIdx.x; e < nit; e += 256) {
+            const int col = e % ncol, j = (e / ncol) % dimR, var = e / (ncol * dimR);
+            const int x = x0 + col;
+            if (x >= Mx) continue;
+            const bool mean = z == 0 && x == 0;
+            const bool qrow = var == 0 && !mean;             // v in the Q basis
+            if (qrow && j >= dimQ) continue;
+            const double kx = Kx[x], k2 = kx * kx + kz * kz;
+            const double2 *dc = sd + var * dimR * ncol + col;
+            const double2 *vc = sv + var * dimR * ncol + col;
+            double gdx = 0.0, gdy = 0.0, gvx = 0.0, gvy = 0.0;   // (G d)_j, (G v)_j
+            if (qrow) {
+                const double ik2 = 1.0 / k2;
+#pragma unroll
+                for (int m = -4; m <= 4; ++m) {
+                    const int i = j + 2 * m;
+                    if (i < 0 || i >= dimQ) continue;
+                    const double g = Qs[j * 9 + 4 + m] + Qps[j * 9 + 4 + m] * ik2;
+                    const double2 di = dc[i * ncol], vi = vc[i * ncol];
+                    gdx += g * di.x;
+                    gdy += g * di.y;
+                    gvx += g * vi.x;
+                    gvy += g * vi.y;
+                }
+            } else {
+                const double f = mean ? 1.0 : 1.0 / k2;
+#pragma unroll
+                for (int m = -2; m <= 2; ++m) {
+                    const int i = j + 2 * m;
+                    if (i < 0 || i >= dimR) continue;
+                    const double g = Rs[j * 5 + 2 + m] * f;
+                    const double2 di = dc[i * ncol], vi = vc[i * ncol];
+                    gdx += g * di.x;
+                    gdy += g * di.y;
+                    gvx += g * vi.x;
+                    gvy += g * vi.y;
+                }
+            }
+            const double2 dj = dc[j * ncol];
+            const double w = x > 0 ? 2.0 : 1.0;
+            num += w * (dj.x * gvx + dj.y * gvy);
+            den += w * (dj.x * gdx + dj.y * gdy);
         }
     }
     r0[threadIdx.x] = num * scale;
@@ -2332,14 +2383,16 @@ __global__ __launch_bounds__(256) void k_ddt_dot(const cplx *A, const cplx *B, c
     }
 }
 
-// vout = vin - r dC with r = num / den from the nparts partial pairs of
-// k_ddt_dot, summed by every block in the same order (strided over 256
-// threads, then a tree); block 0 leaves num, den, r in scal.  With acc, also
-// acc -= a vout, twice when `twice` (the adjoint's trapezoid forcing at the
-// end of one step and the start of the next, applied back to back).
-__global__ __launch_bounds__(256) void k_ddt_apply(cplx *vout, const cplx *vin, const cplx *dC,
-                                                   const double *part, int nparts, double *scal,
-                                                   size_t n, cplx *acc, double a, int twice)
+// vout = vin - r dC, dC = s (P - M), with r = num / den from the nparts
+// partial pairs of k_ddt_gram, summed by every block in the same order
+// (strided over 256 threads, then a tree); block 0 leaves num, den, r in
+// scal.  With acc, also acc -= a vout, twice when `twice` (the adjoint's
+// trapezoid forcing at the end of one step and the start of the next,
+// applied back to back).
+__global__ __launch_bounds__(256) void k_ddt_apply(cplx *vout, const cplx *vin, const cplx *P,
+                                                   const cplx *M, double s, const double *part,
+                                                   int nparts, double *scal, size_t n, cplx *acc,
+                                                   double a, int twice)
 {
     __shared__ double r0[256], r1[256];
     double sa = 0.0, sb = 0.0;
@@ -2365,9 +2418,11 @@ __global__ __launch_bounds__(256) void k_ddt_apply(cplx *vout, const cplx *vin, 
     }
     for (size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x; e < n;
          e += (size_t) gridDim.x * blockDim.x) {
+        const cplx p = P[e], m = M[e];
+        const double dx = s * (p.x - m.x), dy = s * (p.y - m.y);
         cplx v = vin[e];
-        v.x -= r * dC[e].x;
-        v.y -= r * dC[e].y;
+        v.x -= r * dx;
+        v.y -= r * dy;
         vout[e] = v;
         if (acc) {
             cplx w = acc[e];
@@ -2381,8 +2436,6 @@ __global__ __launch_bounds__(256) void k_ddt_apply(cplx *vout, const cplx *vin, 
         }
     }
 }
-
-static int nblocks(size_t n) { return (int) std::min<size_t>((n + 255) / 256, 8192); }
 
 // Reynolds-number sensitivity forcing of a stored state S = MC[count]
 // (channel/tangent_forcing.c:10-75, channel/tangent_forcing0.c:10-44), band
@@ -2450,26 +2503,30 @@ void launch_tangent_forcing(Context &c, const cplx *S, cplx *fa, cplx *fb, cplx 
                        d.dimQ, d.dimR, fa, fb, f0);
 }
 
-void launch_diff_scale(Context &c, cplx *out, const cplx *p, const cplx *m, double s, size_t n)
-{
-    hipLaunchKernelGGL(k_diff_scale, dim3(nblocks(n)), dim3(256), 0, c.stream, out, p, m, s, n);
-}
-
-int launch_ddt_dot(Context &c, const cplx *A, const cplx *B)
+int launch_ddt_gram(Context &c, const cplx *P, const cplx *M, double s, const cplx *V)
 {
     const Dims &d = c.d;
-    const size_t n = (size_t) d.Nz * d.qpts * d.Mx;
-    const int nb = (int) std::min<size_t>({(n + 255) / 256, 1024, (size_t) d.qpts * d.Pz});
-    hipLaunchKernelGGL(k_ddt_dot, dim3(nb), dim3(256), 0, c.stream, A, B, c.dW,
-                       (double) d.Px * (double) d.Pz, d.Nz, d.qpts, d.Mx, c.dotpart, c.abort_flag);
+    // columns per block: 8, fewer when dudt and V of 8 columns exceed 64 KB
+    int ncol = 8;
+    while (ncol > 1 && sizeof(double2) * 4 * (size_t) d.dimR * ncol > 65536) ncol /= 2;
+    int nb = d.Nz * ((d.Mx + ncol - 1) / ncol);
+    // the partial pairs fit in dotpart (2 qpts Pz doubles)
+    while (nb > d.qpts * d.Pz && ncol < d.Mx) {
+        ncol *= 2;
+        nb = d.Nz * ((d.Mx + ncol - 1) / ncol);
+    }
+    const size_t lds = sizeof(double2) * 4 * (size_t) d.dimR * ncol;
+    hipLaunchKernelGGL(k_ddt_gram, dim3(nb), dim3(256), lds, c.stream, P, M, V, s, c.dQs, c.dQps,
+                       c.dRs, c.dKx, c.dKz, d.Nz, d.Mx, d.dimR, d.dimQ, ncol,
+                       (double) d.Px * (double) d.Pz, c.dotpart, c.abort_flag);
     return nb;
 }
 
-void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *dC, size_t n,
-                      cplx *acc, double a, int nparts, int twice)
+void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, const cplx *M,
+                      double s, size_t n, cplx *acc, double a, int nparts, int twice)
 {
-    hipLaunchKernelGGL(k_ddt_apply, dim3(256), dim3(256), 0, c.stream, vout, vin, dC, c.dotpart,
-                       nparts, c.scal, n, acc, a, twice);
+    hipLaunchKernelGGL(k_ddt_apply, dim3(256), dim3(256), 0, c.stream, vout, vin, P, M, s,
+                       c.dotpart, nparts, c.scal, n, acc, a, twice);
 }
 
 }  // namespace sc

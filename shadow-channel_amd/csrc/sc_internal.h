@@ -194,18 +194,17 @@ int solve_columns_per_block(int dimR);
 // Re-sensitivity forcing of state S: fa [Nz][dimQ][Mx], fb [Nz][dimR][Mx], f0 [2][dimR]
 void launch_tangent_forcing(Context &c, const cplx *S, cplx *fa, cplx *fb, cplx *f0);
 int xmix_zblock(Context &c, int nf);
-void launch_diff_scale(Context &c, cplx *out, const cplx *a, const cplx *b, double s, size_t n);
 // wall plane of C: Uxb, Uzb [Nz][Mx]
 void launch_wall_plane(Context &c, const cplx *C, cplx *Uxb, cplx *Uzb);
 // incremental boundary condition in place on Uxb, Uzb; scratch holds
 // 5 Pz Mx complex values
 void launch_incre_boundary(Context &c, cplx *Uxb, cplx *Uzb, cplx *scratch);
-// vout = vin - (num/den) dC; with acc, also acc -= a vout
-void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *dC, size_t n,
-                      cplx *acc, double a, int nparts, int twice);
-// partial inner products of ddt_project from Gauss-point spectra; returns the
-// number of partial pairs written to dotpart
-int launch_ddt_dot(Context &c, const cplx *A, const cplx *B);
+// partial inner products of ddt_project from the coefficients of
+// dudt = s (P - M) and V; returns the number of partial pairs in dotpart
+int launch_ddt_gram(Context &c, const cplx *P, const cplx *M, double s, const cplx *V);
+// vout = vin - (num/den) s (P - M); with acc, also acc -= a vout (twice)
+void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, const cplx *M,
+                      double s, size_t n, cplx *acc, double a, int nparts, int twice);
 
 // dst_kind for launch_zfwd
 enum { ZF_USLOTS = 0, ZF_SCRATCH = 1 };
