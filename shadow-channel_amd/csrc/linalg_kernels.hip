@@ -2282,6 +2282,8 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
 // Block: kz row z and ncol columns; dudt and V of the block's columns in
 // LDS ([var][row][column]), then one (var, row, column) item per thread and
 // pass; one partial pair per block.
+constexpr int DDT_IT = 8;        // items per thread of k_ddt_gram's load phase
+
 __global__ __launch_bounds__(256) void k_ddt_gram(const cplx *P, const cplx *M, const cplx *V,
                                                   double s, const double *Qs, const double *Qps,
                                                   const double *Rs, const double *Kx,
@@ -2294,29 +2296,83 @@ __global__ __launch_bounds__(256) void k_ddt_gram(const cplx *P, const cplx *M, 
     const int nxb = (Mx + ncol - 1) / ncol;
     const int z = blockIdx.x / nxb, x0 = (blockIdx.x - z * nxb) * ncol;
     const bool live = !*abort_flag && z != Nz / 2;
-    const int nit = 2 * dimR * ncol;
+    const int nit = 2 * dimR * ncol;       // at most 256 DDT_IT (host)
     double2 *sd = sh_ddt, *sv = sh_ddt + nit;
+    double2 *smd = sv + nit, *smv = smd + 2 * dimR;   // kx = 0 column of row -kz
+    // the Gram bands: Qs, Qps [dimQ][9], Rs [dimR][5]
+    double *bq = (double *) (smv + 2 * dimR), *bqp = bq + 9 * dimQ, *br = bqp + 9 * dimQ;
+    const bool mirror = live && x0 == 0;
     if (live) {
-        const int zm = z == 0 ? 0 : Nz - z;
-        for (int e = threadIdx.x; e < nit; e += 256) {
-            const int col = e % ncol, j = (e / ncol) % dimR, var = e / (ncol * dimR);
-            const int x = x0 + col;
-            double2 d = make_double2(0.0, 0.0), v = d;
-            if (x < Mx) {
-                const size_t o = (((size_t) z * 2 + var) * dimR + j) * Mx + x;
-                const cplx p = P[o], m = M[o];
-                v = V[o];
-                d = make_double2(s * (p.x - m.x), s * (p.y - m.y));
-                if (x == 0) {
-                    const size_t om = (((size_t) zm * 2 + var) * dimR + j) * Mx;
-                    const cplx pm = P[om], mm = M[om], vm = V[om];
-                    const double dmx = s * (pm.x - mm.x), dmy = s * (pm.y - mm.y);
-                    d = make_double2(0.5 * (d.x + dmx), 0.5 * (d.y - dmy));
-                    v = make_double2(0.5 * (v.x + vm.x), 0.5 * (v.y - vm.y));
-                }
+        const int nbq = 9 * dimQ, nbt = 18 * dimQ + 5 * dimR;
+        for (int i0 = threadIdx.x; i0 < nbt; i0 += 4 * 256) {
+            double t[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = i0 + u * 256;
+                t[u] = i < nbq ? Qs[i] : i < 2 * nbq ? Qps[min(i - nbq, nbq - 1)]
+                                                     : Rs[min(max(i - 2 * nbq, 0), 5 * dimR - 1)];
             }
-            sd[e] = d;
-            sv[e] = v;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (i0 + u * 256 < nbt) bq[i0 + u * 256] = t[u];
+        }
+        // every load of the thread in flight before the first store
+        double2 rp[DDT_IT], rm[DDT_IT], rv[DDT_IT], qp, qm, qv;
+#pragma unroll
+        for (int t = 0; t < DDT_IT; ++t) {
+            const int e = threadIdx.x + t * 256;
+            const int col = e % ncol, j = (e / ncol) % dimR, var = e / (ncol * dimR);
+            const int x = min(x0 + col, Mx - 1);
+            const size_t o = (((size_t) z * 2 + min(var, 1)) * dimR + j) * Mx + x;
+            if (e < nit) {
+                rp[t] = P[o];
+                rm[t] = M[o];
+                rv[t] = V[o];
+            }
+        }
+        const bool mload = mirror && (int) threadIdx.x < 2 * dimR;
+        if (mload) {
+            const int zm = z == 0 ? 0 : Nz - z;
+            const int j = threadIdx.x % dimR, var = threadIdx.x / dimR;
+            const size_t om = (((size_t) zm * 2 + var) * dimR + j) * Mx;
+            qp = P[om];
+            qm = M[om];
+            qv = V[om];
+        }
+#pragma unroll
+        for (int t = 0; t < DDT_IT; ++t) {
+            const int e = threadIdx.x + t * 256;
+            if (e >= nit) continue;
+            const int col = e % ncol;
+            const bool in = x0 + col < Mx;
+            sd[e] = in ? make_double2(s * (rp[t].x - rm[t].x), s * (rp[t].y - rm[t].y))
+                       : make_double2(0.0, 0.0);
+            sv[e] = in ? rv[t] : make_double2(0.0, 0.0);
+        }
+        if (mload) {
+            smd[threadIdx.x] = make_double2(s * (qp.x - qm.x), s * (qp.y - qm.y));
+            smv[threadIdx.x] = qv;
+        }
+        if (mirror) {
+            // rows beyond the first 256 (var, j) pairs (dimR > 128)
+            const int zm = z == 0 ? 0 : Nz - z;
+            for (int q = threadIdx.x + 256; q < 2 * dimR; q += 256) {
+                const int j = q % dimR, var = q / dimR;
+                const size_t om = (((size_t) zm * 2 + var) * dimR + j) * Mx;
+                const cplx pm = P[om], mm = M[om];
+                smd[q] = make_double2(s * (pm.x - mm.x), s * (pm.y - mm.y));
+                smv[q] = V[om];
+            }
+        }
+    }
+    __syncthreads();
+    if (mirror) {
+        // Hermitian part (c(kz) + conj c(-kz)) / 2 of the kx = 0 column
+        for (int q = threadIdx.x; q < 2 * dimR; q += 256) {
+            const int e = q * ncol;                // (var, j) = q, column 0
+            const double2 d = sd[e], v = sv[e], dm = smd[q], vm = smv[q];
+            sd[e] = make_double2(0.5 * (d.x + dm.x), 0.5 * (d.y - dm.y));
+            sv[e] = make_double2(0.5 * (v.x + vm.x), 0.5 * (v.y - vm.y));
         }
     }
     __syncthreads();
@@ -2340,7 +2396,7 @@ __global__ __launch_bounds__(256) void k_ddt_gram(const cplx *P, const cplx *M, 
                 for (int m = -4; m <= 4; ++m) {
                     const int i = j + 2 * m;
                     if (i < 0 || i >= dimQ) continue;
-                    const double g = Qs[j * 9 + 4 + m] + Qps[j * 9 + 4 + m] * ik2;
+                    const double g = bq[j * 9 + 4 + m] + bqp[j * 9 + 4 + m] * ik2;
                     const double2 di = dc[i * ncol], vi = vc[i * ncol];
                     gdx += g * di.x;
                     gdy += g * di.y;
@@ -2353,7 +2409,7 @@ __global__ __launch_bounds__(256) void k_ddt_gram(const cplx *P, const cplx *M, 
                 for (int m = -2; m <= 2; ++m) {
                     const int i = j + 2 * m;
                     if (i < 0 || i >= dimR) continue;
-                    const double g = Rs[j * 5 + 2 + m] * f;
+                    const double g = br[j * 5 + 2 + m] * f;
                     const double2 di = dc[i * ncol], vi = vc[i * ncol];
                     gdx += g * di.x;
                     gdy += g * di.y;
@@ -2506,16 +2562,19 @@ void launch_tangent_forcing(Context &c, const cplx *S, cplx *fa, cplx *fb, cplx 
 int launch_ddt_gram(Context &c, const cplx *P, const cplx *M, double s, const cplx *V)
 {
     const Dims &d = c.d;
-    // columns per block: 8, fewer when dudt and V of 8 columns exceed 64 KB
-    int ncol = 8;
-    while (ncol > 1 && sizeof(double2) * 4 * (size_t) d.dimR * ncol > 65536) ncol /= 2;
-    int nb = d.Nz * ((d.Mx + ncol - 1) / ncol);
-    // the partial pairs fit in dotpart (2 qpts Pz doubles)
-    while (nb > d.qpts * d.Pz && ncol < d.Mx) {
-        ncol *= 2;
-        nb = d.Nz * ((d.Mx + ncol - 1) / ncol);
-    }
-    const size_t lds = sizeof(double2) * 4 * (size_t) d.dimR * ncol;
+    // columns per block: 4, fewer while the block's items exceed 256 DDT_IT
+    // or its LDS 64 KB; grids beyond dimR = 128 with one column ask for more
+    auto lds_of = [&](int nc) {
+        return sizeof(double2) * 4 * (size_t) d.dimR * (nc + 1) +
+               sizeof(double) * (18 * (size_t) d.dimQ + 5 * (size_t) d.dimR);
+    };
+    int ncol = 4;
+    while (ncol > 1 && (2 * d.dimR * ncol > 256 * DDT_IT || lds_of(ncol) > 65536)) ncol /= 2;
+    const int nb = d.Nz * ((d.Mx + ncol - 1) / ncol);
+    const size_t lds = lds_of(ncol);
+    if (lds > 65536)
+        (void) hipFuncSetAttribute((const void *) k_ddt_gram,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
     hipLaunchKernelGGL(k_ddt_gram, dim3(nb), dim3(256), lds, c.stream, P, M, V, s, c.dQs, c.dQps,
                        c.dRs, c.dKx, c.dKz, d.Nz, d.Mx, d.dimR, d.dimQ, ncol,
                        (double) d.Px * (double) d.Pz, c.dotpart, c.abort_flag);
@@ -2525,7 +2584,7 @@ int launch_ddt_gram(Context &c, const cplx *P, const cplx *M, double s, const cp
 void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, const cplx *M,
                       double s, size_t n, cplx *acc, double a, int nparts, int twice)
 {
-    hipLaunchKernelGGL(k_ddt_apply, dim3(256), dim3(256), 0, c.stream, vout, vin, P, M, s,
+    hipLaunchKernelGGL(k_ddt_apply, dim3(1024), dim3(256), 0, c.stream, vout, vin, P, M, s,
                        c.dotpart, nparts, c.scal, n, acc, a, twice);
 }
 
