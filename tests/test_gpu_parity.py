@@ -157,6 +157,32 @@ def test_ddt_project_and_spec2phys(cc):
     assert rel(flow, fo) <= SHORT
 
 
+RAGGED = dict(Nx=20, Ny=17, Nz=10, Lx=np.pi, Lz=0.3 * np.pi, Re=2000.0, flux=4.0 / 3.0, dt=0.005)
+
+
+def test_ragged_grid_ddt_project_and_adjoint(cc):
+    """Nx/2 = 10 columns (not a multiple of the inner-product kernel's 4
+    columns per block, so its last block is partly empty), Nz = 10, and
+    transform lengths 30 and 15 (generic-length FFT kernels): ddt_project and
+    a forced adjoint step against the oracle."""
+    print("case: ragged grid 20x17x10, ddt_project + adjoint", flush=True)
+    o, C0 = start(cc, RAGGED, 2, amp=0.1)
+    o.primal(0, C0.copy())
+    cc.c_primal(0, C0.copy())
+    for i in (0, 1, 2):
+        v = direction(RAGGED, 20 + i)
+        vo, vg = v.copy(), v.copy()
+        ro = o.ddt_project(i, vo)
+        rg = cc.c_ddt_project(i, vg)
+        assert abs(rg - ro) <= PROJ * max(1.0, abs(ro))
+        assert rel(vg, vo) <= PROJ
+    a = direction(RAGGED, 30)
+    Ao, Ag = a.copy(), a.copy()
+    o.adjoint(2, 0, Ao, 0, 1.0)
+    cc.c_adjoint(2, 0, Ag, 0, 1.0)
+    assert rel(Ag, Ao) <= SHORT
+
+
 def test_field_cache_is_bitwise(cc):
     """The tangent and adjoint read the primal's transformed velocity and
     vorticity from the per-state cache instead of rebuilding and transforming
