@@ -670,7 +670,7 @@ static int adjoint_run(int start, int end, sc_complex *AC_given, double strength
     const size_t cb = sizeof(cplx) * c.csize;
     SC_HIP(hipMemcpyAsync(c.AC, AC_given, cb, in, c.stream));
     SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
-    launch_rebuild(c, c.AC, c.AU);
+    launch_rebuild(c, c.AC, c.AU, true);      // lambda: velocity slots only
     const double s = strength * d.dt;
     for (int n = start; n > end; --n) {
         // The projection of MIC[3n] at step n was made at the end of the
@@ -695,7 +695,7 @@ static int adjoint_run(int start, int end, sc_complex *AC_given, double strength
             else
                 launch_force(c, c.U, 1);
             launch_solve(c, c.AC, c.Csol, k, 1, c.FLa, c.FLb, 0.0);
-            launch_rebuild(c, c.Csol, c.AU);
+            launch_rebuild(c, c.Csol, c.AU, true);
         }
         // forcing of MIC[3(n-1)]: end of this step and, when there is a next
         // step, its start (the two subtractions back to back)
@@ -792,7 +792,7 @@ int c_spec2phys(sc_complex *C_ptr, int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_
     }
     SC_HIP(hipMemcpyAsync(c.Ctmp, C_ptr, sizeof(cplx) * c.csize, hipMemcpyHostToDevice, c.stream));
     SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
-    launch_rebuild(c, c.Ctmp, c.Ut);
+    launch_rebuild(c, c.Ctmp, c.Ut, true);    // spec2phys reads u, v, w only
     FieldSpec fs;
     fs.nf = 3;
     for (int s = 0; s < 3; ++s) {

@@ -255,6 +255,8 @@ __device__ __forceinline__ RebuildTask rebuild_task(const RebuildArgs &a, int zr
 }
 
 // Gauss-point values of one task from its five accumulators (D[n][y] layout)
+// VEL: the velocity slots u, v, w only (acc[2], acc[4] unused)
+template <bool VEL = false>
 __device__ __forceinline__ void rebuild_store(const RebuildArgs &a, int zrow, int nt, int y, int kq,
                                               const mf4 *acc, cplx *U)
 {
@@ -267,7 +269,7 @@ __device__ __forceinline__ void rebuild_store(const RebuildArgs &a, int zrow, in
         double *Ur = (double *) (U + (size_t) y * Mx) + (kq & 1);
         const int su = kq < 2 ? 0 : 2;
         Ur[(size_t) su * ss * 2] = acc[3][0];
-        Ur[(size_t) (su == 0 ? 3 : 4) * ss * 2] = acc[4][0];
+        if (!VEL) Ur[(size_t) (su == 0 ? 3 : 4) * ss * 2] = acc[4][0];
         Ur[(size_t) 1 * ss * 2] = 0.0;
         return;
     }
@@ -282,12 +284,13 @@ __device__ __forceinline__ void rebuild_store(const RebuildArgs &a, int zrow, in
         const double kx = wave_kx(x, a.ax), r2 = 1.0 / (kz * kz + kx * kx);
         const double2 v = make_double2(acc[0][2 * h], acc[0][2 * h + 1]);
         const double2 f = make_double2(-acc[1][2 * h], -acc[1][2 * h + 1]);
-        const double2 q2 = make_double2(acc[2][2 * h], acc[2][2 * h + 1]);
         const double2 g = make_double2(acc[3][2 * h], acc[3][2 * h + 1]);
-        const double2 gp = make_double2(acc[4][2 * h], acc[4][2 * h + 1]);
         Uz[o + 1 * ss] = v;
         Uz[o + 0 * ss] = make_double2((kx * f.y + kz * g.y) * r2, -(kx * f.x + kz * g.x) * r2);
         Uz[o + 2 * ss] = make_double2((-kx * g.y + kz * f.y) * r2, (kx * g.x - kz * f.x) * r2);
+        if (VEL) continue;
+        const double2 q2 = make_double2(acc[2][2 * h], acc[2][2 * h + 1]);
+        const double2 gp = make_double2(acc[4][2 * h], acc[4][2 * h + 1]);
         Uz[o + 3 * ss] = make_double2((-kx * q2.y + kz * gp.y) * r2, (kx * q2.x - kz * gp.x) * r2);
         Uz[o + 4 * ss] = make_double2(-(kx * gp.y + kz * q2.y) * r2, (kx * gp.x + kz * q2.x) * r2);
     }
@@ -498,7 +501,7 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_rows(RebuildArgs a, const cp
 // statically: wave w of the bpq blocks of an (XCD, y-tile) takes the tasks
 // w, w + 4 bpq, ..., and issues its first coefficient loads before it
 // stages the basis.
-template <int NKS>
+template <int NKS, bool VEL>
 __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cplx *C, cplx *U)
 {
     __shared__ double bs[NKS * 5 * 64];
@@ -557,14 +560,15 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cpl
 #pragma unroll
             for (int t = 0; t < CK; ++t) {
                 acc[3] = mfma_f64(xa[t], bs[((sb + t) * 5 + 3) * 64 + lane], acc[3]);
-                acc[4] = mfma_f64(xa[t], bs[((sb + t) * 5 + 4) * 64 + lane], acc[4]);
+                if (!VEL) acc[4] = mfma_f64(xa[t], bs[((sb + t) * 5 + 4) * 64 + lane], acc[4]);
             }
             return;
         }
         double p0[10], p1[10];
         auto ldb = [&](int s0, double *p) {
 #pragma unroll
-            for (int e = 0; e < 10; ++e) p[e] = bs[(s0 * 5 + e) * 64 + lane];
+            for (int e = 0; e < 10; ++e)
+                if (!VEL || (e % 5 != 2 && e % 5 != 4)) p[e] = bs[(s0 * 5 + e) * 64 + lane];
         };
         auto mm = [&](int t0, const double *p) {
 #pragma unroll
@@ -572,9 +576,9 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cpl
                 const int t = t0 + u;
                 acc[0] = mfma_f64(xa[t], p[u * 5 + 0], acc[0]);
                 acc[1] = mfma_f64(xa[t], p[u * 5 + 1], acc[1]);
-                acc[2] = mfma_f64(xa[t], p[u * 5 + 2], acc[2]);
+                if (!VEL) acc[2] = mfma_f64(xa[t], p[u * 5 + 2], acc[2]);
                 acc[3] = mfma_f64(xb[t], p[u * 5 + 3], acc[3]);
-                acc[4] = mfma_f64(xb[t], p[u * 5 + 4], acc[4]);
+                if (!VEL) acc[4] = mfma_f64(xb[t], p[u * 5 + 4], acc[4]);
             }
         };
         ldb(sb, p0);
@@ -652,8 +656,8 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cpl
                 ae[m] = e + o;
                 ao[m] = (m == 1 || m == 4) ? o - e : e - o;
             }
-            rebuild_store(a, zr, nt, y, kq, ae, Ud);
-            if (ym != y) rebuild_store(a, zr, nt, ym, kq, ao, Ud);
+            rebuild_store<VEL>(a, zr, nt, y, kq, ae, Ud);
+            if (ym != y) rebuild_store<VEL>(a, zr, nt, ym, kq, ao, Ud);
         }
         t = tn;
     }
@@ -661,7 +665,7 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cpl
 
 // one or two rebuilds (C1 null: one); the queued kernel runs both in one
 // launch (they share the staged basis), the chunked kernel one launch each
-void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx *U1)
+void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx *U1, bool vel_only)
 {
     const Dims &d = c.d;
     // segments of kz rows for the chunked kernel
@@ -690,13 +694,20 @@ void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx 
     }
     const dim3 gpar(8 * c.nht * ap.bpq);
     auto go = [=]() {
-        if (par) {
+        if (par && vel_only) {
             if (nksp == 8)
-                hipLaunchKernelGGL(k_rebuild_par<8>, gpar, dim3(256), 0, st, ap, C0, U0);
+                hipLaunchKernelGGL((k_rebuild_par<8, true>), gpar, dim3(256), 0, st, ap, C0, U0);
             else if (nksp == 16)
-                hipLaunchKernelGGL(k_rebuild_par<16>, gpar, dim3(256), 0, st, ap, C0, U0);
+                hipLaunchKernelGGL((k_rebuild_par<16, true>), gpar, dim3(256), 0, st, ap, C0, U0);
             else
-                hipLaunchKernelGGL(k_rebuild_par<24>, gpar, dim3(256), 0, st, ap, C0, U0);
+                hipLaunchKernelGGL((k_rebuild_par<24, true>), gpar, dim3(256), 0, st, ap, C0, U0);
+        } else if (par) {
+            if (nksp == 8)
+                hipLaunchKernelGGL((k_rebuild_par<8, false>), gpar, dim3(256), 0, st, ap, C0, U0);
+            else if (nksp == 16)
+                hipLaunchKernelGGL((k_rebuild_par<16, false>), gpar, dim3(256), 0, st, ap, C0, U0);
+            else
+                hipLaunchKernelGGL((k_rebuild_par<24, false>), gpar, dim3(256), 0, st, ap, C0, U0);
         } else if (nks == 8)
             hipLaunchKernelGGL(k_rebuild_rows<8>, grows, dim3(256), 0, st, a, C0, U0);
         else if (nks == 16)
@@ -712,9 +723,9 @@ void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx 
     prof_end(c, st, 4, go);
 }
 
-void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst)
+void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst, bool vel_only)
 {
-    launch_rebuild2(c, Csrc, Udst, nullptr, nullptr);
+    launch_rebuild2(c, Csrc, Udst, nullptr, nullptr, vel_only);
 }
 
 // ------------------------------------------------------------ projection

@@ -180,9 +180,12 @@ void launch_assemble_adj(Context &c, const cplx *SP, cplx *Udst);
 // forward z transform of the six adjoint products fused with the assembly
 // into the five state slots of Udst; false when the tile does not fit
 bool launch_zfwd_adj(Context &c, const cplx *W2, cplx *Udst);
-void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst);
+// vel_only: only the velocity slots u, v, w are needed (du/dy and dw/dy may
+// be left unwritten where the kernel allows it)
+void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst, bool vel_only = false);
 // two rebuilds in one launch where the kernel allows it
-void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx *U1);
+void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx *U1,
+                     bool vel_only = false);
 void launch_force(Context &c, const cplx *Usrc, int adj);
 void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb);
 // two projections in one launch where the kernel allows it
