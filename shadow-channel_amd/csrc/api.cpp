@@ -107,7 +107,7 @@ static void free_all(Context &c)
     void *ptrs[] = {c.dKx, c.dKz, c.dcfl2, c.dW, c.dQ, c.dQp, c.dQpp, c.dR, c.dRp, c.dQw,
                     c.dQpw, c.dQppw, c.dRw, c.dRpw, c.dQs, c.dQps, c.dQpps, c.dRs, c.dRps,
                     c.dRowsum, c.C, c.IC, c.AC, c.U, c.IU, c.AU, c.LU, c.Ut, c.Ut2, c.MC,
-                    c.MIC, c.Csol, c.Ctmp, c.Ctmp2, c.Fa, c.Fb, c.FLa, c.FLb, c.W1, c.W2,
+                    c.MIC, c.Csol, c.Ctmp, c.Ctmp2, c.Fa, c.Fb, c.FLa, c.FLb, c.W1, c.W2, c.W2b,
                     c.SP, c.flowdev, c.cflmax, c.dotpart, c.scal, c.abort_flag, c.pz.tw,
                     c.px.tw, c.dFragR, c.dFragF, c.dFragP, c.dFragQ, c.wcache, c.rb_queue};
     for (void *p : ptrs)
@@ -245,6 +245,29 @@ static void ddt_project_dev(Context &c, int i, const cplx *vin, cplx *vout,
     const int nparts = launch_ddt_gram(c, mc(c, ip), mc(c, im), 1.0 / d.dt, vin);
     launch_ddt_apply(c, vout, vin, mc(c, ip), mc(c, im), 1.0 / d.dt, c.csize, acc, a, nparts,
                      twice);
+}
+
+// the two adjoint passes of one lambda (AUb) with the cached states wa
+// (into Uda) and wb (into Udb): lambda transformed once, one x-kernel launch
+static void pass2_adjoint_pair(Context &c, const cplx *AUb, const cplx *wa, const cplx *wb,
+                               cplx *Uda, cplx *Udb)
+{
+    FieldSpec lam;
+    lam.nf = 3;
+    for (int s = 0; s < 3; ++s) {
+        lam.src[s] = AUb;
+        lam.kind[s] = FK_SLOT;
+        lam.slot[s] = s;
+    }
+    launch_zinv(c, lam, c.W1);
+    launch_xmix_adj2(c, wa, wb, c.W1, c.W2, c.W2b);
+    const cplx *W2s[2] = {c.W2, c.W2b};
+    cplx *Ud[2] = {Uda, Udb};
+    for (int p = 0; p < 2; ++p)
+        if (!launch_zfwd_adj(c, W2s[p], Ud[p])) {
+            launch_zfwd(c, W2s[p], 6, c.SP, ZF_SCRATCH);
+            launch_assemble_adj(c, c.SP, Ud[p]);
+        }
 }
 
 static int check_abort(Context &c, const char *what)
@@ -473,6 +496,7 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
         {(void **) &c.Fa, fb}, {(void **) &c.Fb, fb}, {(void **) &c.FLa, fb}, {(void **) &c.FLb, fb},
         {(void **) &c.W1, sizeof(cplx) * (size_t) d.qpts * MAXF * d.Pz * d.Mx},
         {(void **) &c.W2, sizeof(cplx) * (size_t) d.qpts * 6 * d.Pz * d.Mx},
+        {(void **) &c.W2b, sizeof(cplx) * (size_t) d.qpts * 6 * d.Pz * d.Mx},
         {(void **) &c.SP, sizeof(cplx) * 6 * (size_t) d.Nz * d.qpts * d.Mx},
         {(void **) &c.flowdev, sizeof(double) * 3 * (size_t) d.Px * d.qpts * d.Pz},
         {(void **) &c.cflmax, sizeof(double) * (d.qpts + 1)},
@@ -680,16 +704,24 @@ static int adjoint_run(int start, int end, sc_complex *AC_given, double strength
         for (int k = 0; k < 3; ++k) {
             long count = 3L * n - k - 1;
             // with both states cached, lambda is transformed once for the pair
-            bool lam_ready = false;
+            bool lam_ready = false, paired = false;
             if (k < 2) {
-                const cplx *w = wcached(c, count - 1);
-                if (!w) launch_rebuild(c, mc(c, count - 1), c.U);
-                pass2_adjoint(c, c.U, c.AU, c.LU, w);
-                lam_ready = w != nullptr;
+                const cplx *wa = wcached(c, count - 1), *wb = wcached(c, count);
+                if (wa && wb && xmix_adj2_available(c)) {
+                    // both states cached: one merged pass for the pair
+                    pass2_adjoint_pair(c, c.AU, wa, wb, c.LU, c.U);
+                    paired = true;
+                } else {
+                    if (!wa) launch_rebuild(c, mc(c, count - 1), c.U);
+                    pass2_adjoint(c, c.U, c.AU, c.LU, wa);
+                    lam_ready = wa != nullptr;
+                }
             }
-            const cplx *w = wcached(c, count);
-            if (!w) launch_rebuild(c, mc(c, count), c.U);
-            pass2_adjoint(c, c.U, c.AU, c.U, w, lam_ready && w != nullptr);
+            if (!paired) {
+                const cplx *w = wcached(c, count);
+                if (!w) launch_rebuild(c, mc(c, count), c.U);
+                pass2_adjoint(c, c.U, c.AU, c.U, w, lam_ready && w != nullptr);
+            }
             if (k < 2)
                 launch_force2(c, 1, c.U, c.Fa, c.Fb, c.LU, c.FLa, c.FLb);
             else

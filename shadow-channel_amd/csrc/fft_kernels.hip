@@ -575,6 +575,9 @@ struct XmixArgs {
     int mode, nf, nout, Mx, Px, Pz, qpts, ZB, check_cfl;
     int nf1;                   // fields 0..nf1-1 from the first source, the rest from W1b
     const cplx *W1b;
+    int nf2;                   // XM_ADJ2: fields nf1..nf2-1 from W1b, nf2..nf-1 from W1c
+    const cplx *W1c;
+    cplx *W2b;                 // XM_ADJ2: the second product set
     double cfl1, cfl3;
     const double *cfl2, *W;
     double *cflmax;        // [qpts], as ordered bit patterns
@@ -594,6 +597,7 @@ template <> struct XmixShape<XM_TANGENT> { static constexpr int NF = 12, NOUT = 
 template <> struct XmixShape<XM_ADJOINT> { static constexpr int NF = 9, NOUT = 6; };
 template <> struct XmixShape<XM_PHYS> { static constexpr int NF = 3, NOUT = 0; };
 template <> struct XmixShape<XM_DOT> { static constexpr int NF = 6, NOUT = 0; };
+template <> struct XmixShape<XM_ADJ2> { static constexpr int NF = 15, NOUT = 12; };
 
 // grid (Pz/ZB, qpts); LDS: ZB * nseq sequences of Px complex at a stride of
 // Px + 1 (odd, so lanes on neighbouring sequences use different banks),
@@ -608,6 +612,7 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
     constexpr int NOUT = XmixShape<MODE>::NOUT;
     constexpr int nseq = (nf + 1) / 2;
     constexpr int nso = (NOUT + 1) / 2;
+    static_assert(MODE != XM_ADJ2 || PX > 0, "the merged adjoint mode has fixed-length kernels only");
     constexpr int ZBC = PX > 0 ? xmix_rows(PX, nseq) : 1;
     const int zb = blockIdx.x, y = blockIdx.y;
     const int Px = PX > 0 ? PX : a.Px, Mx = PX > 0 ? PX / 3 : a.Mx, Pz = a.Pz;
@@ -636,10 +641,12 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
         const int b = threadIdx.x;
         if (b < NBUT) {
             const int j = b / NSQ, t = b - j * NSQ, r = t / nseq, s2 = 2 * (t - r * nseq);
-            const bool first = s2 < nf1;
-            const cplx *X = first ? W1 + ((size_t) y * nf1 * Pz + z0) * Mx
-                                  : a.W1b + ((size_t) y * (nf - nf1) * Pz + z0) * Mx;
-            X += ((first ? s2 : s2 - nf1) * Pz + r) * Mx;
+            const int nf2 = MODE == XM_ADJ2 ? a.nf2 : nf;
+            const int src = s2 < nf1 ? 0 : s2 < nf2 ? 1 : 2;
+            const int f0 = src == 0 ? 0 : src == 1 ? nf1 : nf2;
+            const int fn = src == 0 ? nf1 : src == 1 ? nf2 - nf1 : nf - nf2;
+            const cplx *B = src == 0 ? W1 : src == 1 ? a.W1b : a.W1c;
+            const cplx *X = B + ((size_t) y * fn * Pz + z0) * Mx + ((s2 - f0) * Pz + r) * Mx;
             const bool hasy = s2 + 1 < nf;
             const int yo = hasy ? PM : 0;
             double2 v[R1];
@@ -734,9 +741,9 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
         const int n = wp.r, r = wp.q;
         wp.next();
         const double2 *row = lds + (size_t) r * nseq * SP + n;
-        double f[12];
+        double f[16];
 #pragma unroll
-        for (int s = 0; s < 6; ++s) {
+        for (int s = 0; s < 8; ++s) {
             if (s < nseq) {
                 double2 v = row[s * SP];
                 f[2 * s] = v.x;
@@ -768,6 +775,21 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
             outv[i][3] = ly * oz - lz * oy;         // Tx
             outv[i][4] = lz * ox - lx * oz;         // Ty
             outv[i][5] = lx * oy - ly * ox;         // Tz
+        } else if constexpr (MODE == XM_ADJ2) {
+            // the adjoint products of lambda with two states: (u, omega) in
+            // f[0..5] and f[6..11], lambda in f[12..14]
+            double lx = f[12], ly = f[13], lz = f[14];
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const double pu = f[6 * p], pv = f[6 * p + 1], pw = f[6 * p + 2];
+                const double qz = f[6 * p + 3], qx = f[6 * p + 4], qy = f[6 * p + 5];
+                outv[i][6 * p + 0] = pv * lz - pw * ly;     // Sx
+                outv[i][6 * p + 1] = pw * lx - pu * lz;     // Sy
+                outv[i][6 * p + 2] = pu * ly - pv * lx;     // Sz
+                outv[i][6 * p + 3] = ly * qz - lz * qy;     // Tx
+                outv[i][6 * p + 4] = lz * qx - lx * qz;     // Ty
+                outv[i][6 * p + 5] = lx * qy - ly * qx;     // Tz
+            }
         } else if constexpr (MODE == XM_PHYS) {
             int zp = z0 + r;
             size_t plane = (size_t) Px * a.qpts * Pz;
@@ -818,7 +840,7 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
         const int n = wp.r, r = wp.q;
         wp.next();
 #pragma unroll
-        for (int t = 0; t < 3; ++t) {
+        for (int t = 0; t < 6; ++t) {
             if (t < nso) {
                 double re = outv[i][2 * t < NO ? 2 * t : 0];
                 double im = (2 * t + 1 < NOUT) ? outv[i][2 * t + 1 < NO ? 2 * t + 1 : 0] : 0.0;
@@ -842,8 +864,16 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
         double2 Yv = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
         int zp = z0 + r;
         int g0 = 2 * t, g1 = 2 * t + 1;
-        W2[(((size_t) y * NOUT + g0) * Pz + zp) * Mx + k] = X;
-        if (g1 < NOUT) W2[(((size_t) y * NOUT + g1) * Pz + zp) * Mx + k] = Yv;
+        if constexpr (MODE == XM_ADJ2) {
+            // pairs t < 3 to W2, the others to W2b, six fields each
+            cplx *D = t < 3 ? W2 : a.W2b;
+            const int h0 = t < 3 ? g0 : g0 - 6;
+            D[(((size_t) y * 6 + h0) * Pz + zp) * Mx + k] = X;
+            D[(((size_t) y * 6 + h0 + 1) * Pz + zp) * Mx + k] = Yv;
+        } else {
+            W2[(((size_t) y * NOUT + g0) * Pz + zp) * Mx + k] = X;
+            if (g1 < NOUT) W2[(((size_t) y * NOUT + g1) * Pz + zp) * Mx + k] = Yv;
+        }
     }
     }
 }
@@ -1191,6 +1221,9 @@ void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
     a.nout = mode == XM_ADJOINT ? 6 : (mode == XM_PRIMAL || mode == XM_TANGENT) ? 3 : 0;
     a.nf1 = W1b ? nf1 : a.nf;
     a.W1b = W1b ? W1b : W1;
+    a.nf2 = a.nf;
+    a.W1c = nullptr;
+    a.W2b = nullptr;
     a.Mx = d.Mx;
     a.Px = d.Px;
     a.Pz = d.Pz;
@@ -1227,6 +1260,52 @@ void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
         prof_end(c, st, 2, again);
     else
         prof_end(c, st, 2);
+}
+
+bool xmix_adj2_available(Context &c)
+{
+    const int zb = xmix_zblock(c, 15);
+    return (c.d.Px == 96 && zb == xmix_rows(96, 8)) || (c.d.Px == 48 && zb == xmix_rows(48, 8));
+}
+
+void launch_xmix_adj2(Context &c, const cplx *Wa, const cplx *Wb, const cplx *Wl, cplx *W2,
+                      cplx *W2b)
+{
+    const Dims &d = c.d;
+    XmixArgs a;
+    a.mode = XM_ADJ2;
+    a.nf = 15;
+    a.nout = 12;
+    a.nf1 = 6;
+    a.W1b = Wb;
+    a.nf2 = 12;
+    a.W1c = Wl;
+    a.W2b = W2b;
+    a.Mx = d.Mx;
+    a.Px = d.Px;
+    a.Pz = d.Pz;
+    a.qpts = d.qpts;
+    const int zb = xmix_zblock(c, a.nf);
+    a.ZB = zb;
+    a.check_cfl = 0;
+    a.cfl1 = d.cfl1;
+    a.cfl3 = d.cfl3;
+    a.cfl2 = c.dcfl2;
+    a.W = c.dW;
+    a.cflmax = c.cflmax;
+    a.flow = nullptr;
+    a.dotpart = nullptr;
+    a.abort_flag = c.abort_flag;
+    const dim3 grid(d.Pz / zb, d.qpts);
+    const size_t lds = sizeof(double2) * (zb * 8 * (d.Px + 1) + d.Px);
+    const PlanDev pd = to_dev(c.px);
+    const XmixKernel kern = d.Px == 96 ? k_xmix<XM_ADJ2, 96> : k_xmix<XM_ADJ2, 48>;
+    const double bytes = 16.0 * (double) d.qpts * d.Pz * d.Mx * (a.nf + a.nout);
+    const hipStream_t st = c.stream;
+    auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, Wa, W2); };
+    prof_begin(c, st, 7, bytes);
+    go();
+    prof_end(c, st, 7, go);
 }
 
 int xmix_zblock(Context &c, int nf)

@@ -70,6 +70,8 @@ enum XMode : int {
     XM_ADJOINT = 2,  // 9 in (u,v,w,oz,ox,oy,lx,ly,lz) -> 6 out (Sx,Sy,Sz,Tx,Ty,Tz)
     XM_PHYS = 3,     // 3 in -> physical flow[3][Px][qpts][Pz]
     XM_DOT = 4,      // 6 in (dudt xyz, v xyz) -> weighted dot products
+    XM_ADJ2 = 5,     // 15 in (two states' 6 fields, lambda) -> 2 x 6 out (the two
+                     // adjoint product sets of one lambda, into W2 and W2b)
 };
 
 struct Dims {
@@ -129,7 +131,7 @@ struct Context {
     cplx *Csol = nullptr, *Ctmp = nullptr, *Ctmp2 = nullptr;
     cplx *Fa = nullptr, *Fb = nullptr;          // [Nz][dimQ][Mx], [Nz][dimR][Mx]
     cplx *FLa = nullptr, *FLb = nullptr;        // adjoint next-stage projection
-    cplx *W1 = nullptr, *W2 = nullptr, *SP = nullptr;
+    cplx *W1 = nullptr, *W2 = nullptr, *W2b = nullptr, *SP = nullptr;
     // Per stage state j = 3n + k of the stored trajectory: the six
     // velocity/vorticity fields after the z inverse transform, as written by
     // the primal's first nonlinear pass ([qpts][6][Pz][Mx] each).  The
@@ -176,6 +178,12 @@ void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
 // cfl_slot >= 0: the first block runs the CFL check (cfl_check_block) with
 // that slot: the first stored state of the step being checked
 void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind, int cfl_slot = -1);
+// the merged adjoint x-kernel exists for this grid (fixed transform length)
+bool xmix_adj2_available(Context &c);
+// both adjoint product sets of one lambda: states Wa, Wb ([qpts][6][Pz][Mx]
+// each), lambda Wl ([qpts][3][Pz][Mx]) -> W2 (with Wa), W2b (with Wb)
+void launch_xmix_adj2(Context &c, const cplx *Wa, const cplx *Wb, const cplx *Wl, cplx *W2,
+                      cplx *W2b);
 void launch_assemble_adj(Context &c, const cplx *SP, cplx *Udst);
 // forward z transform of the six adjoint products fused with the assembly
 // into the five state slots of Udst; false when the tile does not fit
