@@ -131,7 +131,8 @@ void sc_srand(unsigned seed);
    class is bracketed by HIP events on its stream; sc_prof_read returns the
    number of launches and the summed milliseconds, and clears the record.
    Classes: 1 z-inverse FFT, 2 x transform/products, 3 z-forward FFT,
-   4 y-transform to Gauss points, 5 Galerkin projection, 6 band solves. */
+   4 y-transform to Gauss points, 5 Galerkin projection, 6 band solves,
+   7 the merged adjoint x-kernel (two product sets of one lambda). */
 int sc_prof_enable(int kernel_class);
 int sc_prof_read(double *total_ms);
 /* Algorithmic bytes (compulsory reads + writes, each element once) of the
@@ -143,6 +144,11 @@ double sc_prof_bytes(void);
    algorithmic bytes.  The replayed kernels overwrite their outputs: use after
    the results are read back. */
 int sc_prof_replay(int reps, double *avg_ms, double *bytes);
+/* The same replay with each launch preceded by a fill of flush_mb MB of
+   scratch memory (evicting the last-level caches, as between the launches of
+   a step); the fills' own time, measured alone, is subtracted.  No
+   counterpart in the reference (benchmark hook). */
+int sc_prof_replay_cold(int reps, double flush_mb, double *avg_ms, double *bytes);
 /* Wait until all work queued by the library has finished. */
 int sc_synchronize(void);
 /* c_primal / c_adjoint for callers that keep the state in device memory:

@@ -68,6 +68,8 @@ _L.sc_prof_bytes.argtypes = []
 _L.sc_prof_bytes.restype = _d
 _L.sc_prof_replay.argtypes = [_i, _c.POINTER(_d), _c.POINTER(_d)]
 _L.sc_prof_replay.restype = _i
+_L.sc_prof_replay_cold.argtypes = [_i, _d, _c.POINTER(_d), _c.POINTER(_d)]
+_L.sc_prof_replay_cold.restype = _i
 _L.sc_synchronize.argtypes = []
 _L.sc_synchronize.restype = _i
 _L.sc_set_field_cache.argtypes = [_i]
@@ -300,6 +302,15 @@ def sc_prof_replay(reps):
     re-issued reps times back to back (overwrites that kernel's outputs)."""
     ms, nb = _d(0.0), _d(0.0)
     _check(_L.sc_prof_replay(int(reps), _c.byref(ms), _c.byref(nb)), "sc_prof_replay")
+    return ms.value, nb.value
+
+
+def sc_prof_replay_cold(reps, flush_mb=320.0):
+    """Like sc_prof_replay, each launch preceded by a cache-evicting fill of
+    flush_mb MB whose own time is subtracted: (avg_ms, bytes)."""
+    ms, nb = _c.c_double(), _c.c_double()
+    _check(_L.sc_prof_replay_cold(int(reps), float(flush_mb), _c.byref(ms), _c.byref(nb)),
+           "sc_prof_replay_cold")
     return ms.value, nb.value
 
 

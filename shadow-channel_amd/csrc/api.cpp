@@ -411,6 +411,39 @@ int sc_prof_replay(int reps, double *avg_ms, double *bytes)
     return SC_OK;
 }
 
+int sc_prof_replay_cold(int reps, double flush_mb, double *avg_ms, double *bytes)
+{
+    Context &c = ctx();
+    if (!c.stream) return SC_ENOTREADY;
+    if (!c.replay || reps < 1 || flush_mb <= 0.0 || !avg_ms || !bytes) return SC_EINVAL;
+    *bytes = c.replay_bytes;
+    const size_t fb = (size_t) (flush_mb * 1048576.0);
+    void *flush = nullptr;
+    SC_HIP(hipMalloc(&flush, fb));
+    hipEvent_t e[4];
+    for (auto &x : e) SC_HIP(hipEventCreate(&x));
+    c.replay();   // warm the code
+    // the fills alone, then each fill followed by the launch: every launch
+    // finds its data evicted from the caches, as inside a step
+    SC_HIP(hipEventRecord(e[0], c.stream));
+    for (int i = 0; i < reps; ++i) SC_HIP(hipMemsetAsync(flush, i & 0xff, fb, c.stream));
+    SC_HIP(hipEventRecord(e[1], c.stream));
+    SC_HIP(hipEventRecord(e[2], c.stream));
+    for (int i = 0; i < reps; ++i) {
+        SC_HIP(hipMemsetAsync(flush, i & 0xff, fb, c.stream));
+        c.replay();
+    }
+    SC_HIP(hipEventRecord(e[3], c.stream));
+    SC_HIP(hipEventSynchronize(e[3]));
+    float m0 = 0.f, m1 = 0.f;
+    SC_HIP(hipEventElapsedTime(&m0, e[0], e[1]));
+    SC_HIP(hipEventElapsedTime(&m1, e[2], e[3]));
+    for (auto &x : e) (void) hipEventDestroy(x);
+    (void) hipFree(flush);
+    *avg_ms = (m1 - m0) / reps;
+    return SC_OK;
+}
+
 double sc_prof_bytes(void) { return ctx().prof_bytes; }
 
 int sc_synchronize(void)

@@ -2242,10 +2242,15 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
     a.cn[1] = k != 2 ? -(d.re * A[k + 1] * d.dt) : 0.0;
     a.target = target;
     a.abort_flag = c.abort_flag;
-    // RHS in, projections in, solution and next right-hand side out, factors in
+    // RHS in, projections in (the next stage's only when they are other
+    // arrays: the primal's are the same), solution and next right-hand side
+    // out, and the factor images in (each stored image once: mirror rows
+    // share one)
+    const bool f2 = k != 2 && Fa2 != c.Fa;
+    const double fcols = (double) (c.factor_mirror ? d.Nz / 2 + 1 : d.Nz) * d.Mx;
     double bytes = (double) (d.Nz - 1) * d.Mx *
-                   (16.0 * (2.0 * d.dimR) * 3.0 + 16.0 * (d.dimQ + d.dimR) * (k != 2 ? 2.0 : 1.0) +
-                    8.0 * (9.0 * d.dimQ + 5.0 * d.dimR));
+                       (16.0 * (2.0 * d.dimR) * 3.0 + 16.0 * (d.dimQ + d.dimR) * (f2 ? 2.0 : 1.0)) +
+                   fcols * 8.0 * (9.0 * d.dimQ + 5.0 * d.dimR);
     const int sbt = c.sbt;
     const int ncol = d.Nz * d.Mx;
     const size_t lds = solve_lds_bytes(d.dimR, sbt);
