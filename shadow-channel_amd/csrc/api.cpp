@@ -261,13 +261,14 @@ static void pass2_adjoint_pair(Context &c, const cplx *AUb, const cplx *wa, cons
     }
     launch_zinv(c, lam, c.W1);
     launch_xmix_adj2(c, wa, wb, c.W1, c.W2, c.W2b);
-    const cplx *W2s[2] = {c.W2, c.W2b};
-    cplx *Ud[2] = {Uda, Udb};
-    for (int p = 0; p < 2; ++p)
-        if (!launch_zfwd_adj(c, W2s[p], Ud[p])) {
+    if (!launch_zfwd_adj2(c, c.W2, Uda, c.W2b, Udb)) {
+        const cplx *W2s[2] = {c.W2, c.W2b};
+        cplx *Ud[2] = {Uda, Udb};
+        for (int p = 0; p < 2; ++p) {
             launch_zfwd(c, W2s[p], 6, c.SP, ZF_SCRATCH);
             launch_assemble_adj(c, c.SP, Ud[p]);
         }
+    }
 }
 
 static int check_abort(Context &c, const char *what)

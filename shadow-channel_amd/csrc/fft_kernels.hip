@@ -963,6 +963,10 @@ struct ZadjArgs {
     double norm;
     const double *Kx, *Kz;
     const int *abort_flag;
+    // k_zfwd_adj2 with nprob = 2: a second problem (W2b -> Ub) in the same grid
+    int nprob;
+    const cplx *W2b;
+    cplx *Ub;
 };
 
 template <int PZ>
@@ -1051,9 +1055,15 @@ __global__ __launch_bounds__(BLK) void k_zfwd_adj2(ZadjArgs a, PlanDev pz, const
     static_assert(ZA * (Nz - 1) <= BLK, "one output column-row per thread");
     extern __shared__ double2 lds[];
     if (*a.abort_flag) return;
-    const int Mx = a.Mx, nxg = Mx / ZA;
-    const int tile = xcd_tile(nxg * a.qpts);
+    const int Mx = a.Mx, nxg = Mx / ZA, n1 = nxg * a.qpts;
+    int tile = xcd_tile(a.nprob * n1);
     if (tile < 0) return;
+    if (tile >= n1) {
+        // the second problem's tiles
+        tile -= n1;
+        W2 = a.W2b;
+        U = a.Ub;
+    }
     const int y = tile / nxg, x0 = (tile - y * nxg) * ZA;
     double2 *twl = lds + TOT;
     stage_twiddles(pz, twl);
@@ -1351,9 +1361,22 @@ void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind, 
 // caller then transforms into scratch and assembles separately)
 bool launch_zfwd_adj(Context &c, const cplx *W2, cplx *Udst)
 {
+    return launch_zfwd_adj2(c, W2, Udst, nullptr, nullptr);
+}
+
+bool launch_zfwd_adj2(Context &c, const cplx *W2, cplx *Udst, const cplx *W2b, cplx *Udstb)
+{
     const Dims &d = c.d;
     const int fixed = z_fixed(d);
+    // the pair in one grid only for the fixed-length two-phase kernel
+    if (W2b && !fixed) {
+        if (!launch_zfwd_adj2(c, W2, Udst, nullptr, nullptr)) return false;
+        return launch_zfwd_adj2(c, W2b, Udstb, nullptr, nullptr);
+    }
     ZadjArgs a;
+    a.nprob = W2b ? 2 : 1;
+    a.W2b = W2b;
+    a.Ub = Udstb;
     a.Nz = d.Nz;
     a.Mx = d.Mx;
     a.Pz = d.Pz;
@@ -1365,10 +1388,11 @@ bool launch_zfwd_adj(Context &c, const cplx *W2, cplx *Udst)
     a.Kx = c.dKx;
     a.Kz = c.dKz;
     a.abort_flag = c.abort_flag;
-    const dim3 grid = fixed ? dim3(xcd_grid(d.Mx / ZA * d.qpts)) : dim3((d.Mx + a.XA - 1) / a.XA, d.qpts);
+    const dim3 grid = fixed ? dim3(xcd_grid(a.nprob * (d.Mx / ZA) * d.qpts))
+                            : dim3((d.Mx + a.XA - 1) / a.XA, d.qpts);
     // fixed lengths: the two-phase kernel on a 3-field tile
     size_t lds = sizeof(double2) * ((fixed ? 3 : 6) * (size_t) a.XA * d.Pz + d.Pz);
-    double bytes = 16.0 * (double) d.qpts * d.Mx * (6.0 * d.Pz + 5.0 * (d.Nz - 1));
+    double bytes = a.nprob * 16.0 * (double) d.qpts * d.Mx * (6.0 * d.Pz + 5.0 * (d.Nz - 1));
     const hipStream_t st = c.stream;
     const PlanDev pd = to_dev(c.pz);
     typedef void (*ZadjKernel)(ZadjArgs, PlanDev, const cplx *, cplx *);

@@ -188,6 +188,10 @@ void launch_assemble_adj(Context &c, const cplx *SP, cplx *Udst);
 // forward z transform of the six adjoint products fused with the assembly
 // into the five state slots of Udst; false when the tile does not fit
 bool launch_zfwd_adj(Context &c, const cplx *W2, cplx *Udst);
+// the same for two product sets (W2 -> Udst, W2b -> Udstb) in one launch
+// where the kernel allows it (two launches otherwise); false when the tile
+// does not fit
+bool launch_zfwd_adj2(Context &c, const cplx *W2, cplx *Udst, const cplx *W2b, cplx *Udstb);
 // vel_only: only the velocity slots u, v, w are needed (du/dy and dw/dy may
 // be left unwritten where the kernel allows it)
 void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst, bool vel_only = false);
