@@ -2298,7 +2298,8 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
 // Block: kz row z and ncol columns; dudt and V of the block's columns in
 // LDS ([var][row][column]), then one (var, row, column) item per thread and
 // pass; one partial pair per block.
-constexpr int DDT_IT = 8;        // items per thread of k_ddt_gram's load phase
+constexpr int DDT_IT = 4;        // items per thread of k_ddt_gram's load phase
+constexpr int DDT_BT = 12;       // Gram band values per thread loaded in one batch
 
 __global__ __launch_bounds__(256) void k_ddt_gram(const cplx *P, const cplx *M, const cplx *V,
                                                   double s, const double *Qs, const double *Qps,
@@ -2319,20 +2320,8 @@ __global__ __launch_bounds__(256) void k_ddt_gram(const cplx *P, const cplx *M, 
     double *bq = (double *) (smv + 2 * dimR), *bqp = bq + 9 * dimQ, *br = bqp + 9 * dimQ;
     const bool mirror = live && x0 == 0;
     if (live) {
-        const int nbq = 9 * dimQ, nbt = 18 * dimQ + 5 * dimR;
-        for (int i0 = threadIdx.x; i0 < nbt; i0 += 4 * 256) {
-            double t[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int i = i0 + u * 256;
-                t[u] = i < nbq ? Qs[i] : i < 2 * nbq ? Qps[min(i - nbq, nbq - 1)]
-                                                     : Rs[min(max(i - 2 * nbq, 0), 5 * dimR - 1)];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (i0 + u * 256 < nbt) bq[i0 + u * 256] = t[u];
-        }
-        // every load of the thread in flight before the first store
+        // every load of the thread in flight before the first store: the
+        // block's items, the mirror column, the Gram bands
         double2 rp[DDT_IT], rm[DDT_IT], rv[DDT_IT], qp, qm, qv;
 #pragma unroll
         for (int t = 0; t < DDT_IT; ++t) {
@@ -2355,6 +2344,23 @@ __global__ __launch_bounds__(256) void k_ddt_gram(const cplx *P, const cplx *M, 
             qm = M[om];
             qv = V[om];
         }
+        const int nbq = 9 * dimQ, nbt = 18 * dimQ + 5 * dimR;
+        auto band = [&](int i) {
+            return i < nbq ? Qs[i] : i < 2 * nbq ? Qps[min(i - nbq, nbq - 1)]
+                                                 : Rs[min(max(i - 2 * nbq, 0), 5 * dimR - 1)];
+        };
+        double tb[DDT_BT];
+#pragma unroll
+        for (int u = 0; u < DDT_BT; ++u) {
+            const int i = threadIdx.x + u * 256;
+            if (i < nbt) tb[u] = band(i);
+        }
+#pragma unroll
+        for (int u = 0; u < DDT_BT; ++u) {
+            const int i = threadIdx.x + u * 256;
+            if (i < nbt) bq[i] = tb[u];
+        }
+        for (int i = threadIdx.x + DDT_BT * 256; i < nbt; i += 256) bq[i] = band(i);
 #pragma unroll
         for (int t = 0; t < DDT_IT; ++t) {
             const int e = threadIdx.x + t * 256;
