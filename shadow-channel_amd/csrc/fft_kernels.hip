@@ -803,13 +803,20 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
         }
     }
     if (a.check_cfl) {
-        red[0][threadIdx.x] = cmax;
+        // block maximum: within each wave by lane exchanges, then the waves'
+        // maxima through LDS (the maximum is exact in any order)
+        __shared__ double cw[BLK / 64];
+        double m = cmax;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+        if ((threadIdx.x & 63) == 0) cw[threadIdx.x >> 6] = m;
         __syncthreads();
-        for (int o = BLK / 2; o > 0; o >>= 1) {
-            if (threadIdx.x < o) red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + o]);
-            __syncthreads();
+        if (threadIdx.x == 0) {
+            double b = cw[0];
+#pragma unroll
+            for (int w = 1; w < BLK / 64; ++w) b = fmax(b, cw[w]);
+            atomic_max_nonneg(&a.cflmax[y], b);
         }
-        if (threadIdx.x == 0) atomic_max_nonneg(&a.cflmax[y], red[0][0]);
     }
     if constexpr (MODE == XM_DOT) {
         red[0][threadIdx.x] = acc0;
