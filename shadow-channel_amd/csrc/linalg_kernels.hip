@@ -2298,6 +2298,25 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
 // Block: kz row z and ncol columns; dudt and V of the block's columns in
 // LDS ([var][row][column]), then one (var, row, column) item per thread and
 // pass; one partial pair per block.
+// Sums of a and b over the 256 threads of a block, in the same order in every
+// block: lane exchanges within each wave (distances 32 .. 1; both partners
+// add the same two values, so every lane holds the same bits), then the four
+// wave sums in wave order.  Every thread gets both sums.  sh: 8 doubles.
+__device__ __forceinline__ double2 block_sum2_256(double a, double b, double *sh)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[threadIdx.x >> 6] = a;
+        sh[4 + (threadIdx.x >> 6)] = b;
+    }
+    __syncthreads();
+    return make_double2((sh[0] + sh[1]) + (sh[2] + sh[3]), (sh[4] + sh[5]) + (sh[6] + sh[7]));
+}
+
 constexpr int DDT_IT = 4;        // items per thread of k_ddt_gram's load phase
 constexpr int DDT_BT = 12;       // Gram band values per thread loaded in one batch
 
@@ -2309,7 +2328,7 @@ __global__ __launch_bounds__(256) void k_ddt_gram(const cplx *P, const cplx *M, 
                                                   const int *abort_flag)
 {
     extern __shared__ double2 sh_ddt[];
-    __shared__ double r0[256], r1[256];
+    __shared__ double rs[8];
     const int nxb = (Mx + ncol - 1) / ncol;
     const int z = blockIdx.x / nxb, x0 = (blockIdx.x - z * nxb) * ncol;
     const bool live = !*abort_flag && z != Nz / 2;
@@ -2445,25 +2464,16 @@ __global__ __launch_bounds__(256) void k_ddt_gram(const cplx *P, const cplx *M, 
             den += w * (dj.x * gdx + dj.y * gdy);
         }
     }
-    r0[threadIdx.x] = num * scale;
-    r1[threadIdx.x] = den * scale;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
-            r0[threadIdx.x] += r0[threadIdx.x + o];
-            r1[threadIdx.x] += r1[threadIdx.x + o];
-        }
-        __syncthreads();
-    }
+    const double2 t = block_sum2_256(num * scale, den * scale, rs);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = r0[0];
-        part[2 * blockIdx.x + 1] = r1[0];
+        part[2 * blockIdx.x] = t.x;
+        part[2 * blockIdx.x + 1] = t.y;
     }
 }
 
 // vout = vin - r dC, dC = s (P - M), with r = num / den from the nparts
 // partial pairs of k_ddt_gram, summed by every block in the same order
-// (strided over 256 threads, then a tree); block 0 leaves num, den, r in
+// (strided over 256 threads, then block_sum2_256); block 0 leaves num, den, r in
 // scal.  With acc, also acc -= a vout, twice when `twice` (the adjoint's
 // trapezoid forcing at the end of one step and the start of the next,
 // applied back to back).
@@ -2472,26 +2482,17 @@ __global__ __launch_bounds__(256) void k_ddt_apply(cplx *vout, const cplx *vin, 
                                                    int nparts, double *scal, size_t n, cplx *acc,
                                                    double a, int twice)
 {
-    __shared__ double r0[256], r1[256];
+    __shared__ double rs[8];
     double sa = 0.0, sb = 0.0;
     for (int i = threadIdx.x; i < nparts; i += 256) {
         sa += part[2 * i];
         sb += part[2 * i + 1];
     }
-    r0[threadIdx.x] = sa;
-    r1[threadIdx.x] = sb;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
-            r0[threadIdx.x] += r0[threadIdx.x + o];
-            r1[threadIdx.x] += r1[threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    const double r = r0[0] / r1[0];
+    const double2 t = block_sum2_256(sa, sb, rs);
+    const double r = t.x / t.y;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        scal[0] = r0[0];
-        scal[1] = r1[0];
+        scal[0] = t.x;
+        scal[1] = t.y;
         scal[2] = r;
     }
     for (size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x; e < n;
