@@ -172,7 +172,7 @@ static void pass1_state(Context &c, cplx *Ub, int check_cfl, cplx *wdst = nullpt
 {
     cplx *w = wdst ? wdst : c.W1;
     launch_zinv(c, velocity_and_vorticity(Ub), w);
-    launch_xmix(c, XM_PRIMAL, w, c.W2, check_cfl, nullptr, nullptr);
+    launch_xmix(c, XM_PRIMAL, w, c.W2, check_cfl, nullptr);
     launch_zfwd(c, c.W2, 3, Ub, ZF_USLOTS, check_cfl ? slot : -1);
 }
 
@@ -183,7 +183,7 @@ static void pass1_tangent(Context &c, const cplx *Ub, cplx *IUb, int check_cfl,
 {
     if (wbase) {
         launch_zinv(c, velocity_and_vorticity(IUb), c.W1);
-        launch_xmix(c, XM_TANGENT, wbase, c.W2, check_cfl, nullptr, nullptr, c.W1, 6);
+        launch_xmix(c, XM_TANGENT, wbase, c.W2, check_cfl, nullptr, c.W1, 6);
     } else {
         FieldSpec a = velocity_and_vorticity(Ub), b = velocity_and_vorticity(IUb);
         for (int f = 0; f < 6; ++f) {
@@ -193,7 +193,7 @@ static void pass1_tangent(Context &c, const cplx *Ub, cplx *IUb, int check_cfl,
         }
         a.nf = 12;
         launch_zinv(c, a, c.W1);
-        launch_xmix(c, XM_TANGENT, c.W1, c.W2, check_cfl, nullptr, nullptr);
+        launch_xmix(c, XM_TANGENT, c.W1, c.W2, check_cfl, nullptr);
     }
     launch_zfwd(c, c.W2, 3, IUb, ZF_USLOTS, check_cfl ? 0 : -1);
 }
@@ -214,7 +214,7 @@ static void pass2_adjoint(Context &c, const cplx *Ub, const cplx *AUb, cplx *Uds
     }
     if (wbase) {
         if (!lam_ready) launch_zinv(c, lam, c.W1);
-        launch_xmix(c, XM_ADJOINT, wbase, c.W2, 0, nullptr, nullptr, c.W1, 6);
+        launch_xmix(c, XM_ADJOINT, wbase, c.W2, 0, nullptr, c.W1, 6);
     } else {
         FieldSpec a = velocity_and_vorticity(Ub);
         for (int s = 0; s < 3; ++s) {
@@ -224,7 +224,7 @@ static void pass2_adjoint(Context &c, const cplx *Ub, const cplx *AUb, cplx *Uds
         }
         a.nf = 9;
         launch_zinv(c, a, c.W1);
-        launch_xmix(c, XM_ADJOINT, c.W1, c.W2, 0, nullptr, nullptr);
+        launch_xmix(c, XM_ADJOINT, c.W1, c.W2, 0, nullptr);
     }
     if (!launch_zfwd_adj(c, c.W2, Udst)) {
         launch_zfwd(c, c.W2, 6, c.SP, ZF_SCRATCH);
@@ -867,7 +867,7 @@ int c_spec2phys(sc_complex *C_ptr, int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_
         fs.slot[s] = s;
     }
     launch_zinv(c, fs, c.W1);
-    launch_xmix(c, XM_PHYS, c.W1, nullptr, 0, c.flowdev, nullptr);
+    launch_xmix(c, XM_PHYS, c.W1, nullptr, 0, c.flowdev);
     SC_HIP(hipMemcpyAsync(flow_ptr, c.flowdev, sizeof(double) * 3 * (size_t) d.Px * d.qpts * d.Pz,
                           hipMemcpyDeviceToHost, c.stream));
     SC_HIP(hipStreamSynchronize(c.stream));

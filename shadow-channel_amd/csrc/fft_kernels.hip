@@ -582,7 +582,6 @@ struct XmixArgs {
     const double *cfl2, *W;
     double *cflmax;        // [qpts], as ordered bit patterns
     double *flow;          // XM_PHYS output
-    double *dotpart;       // XM_DOT partials [qpts][nzb][2]
     const int *abort_flag;
 };
 
@@ -596,7 +595,6 @@ template <> struct XmixShape<XM_PRIMAL> { static constexpr int NF = 6, NOUT = 3;
 template <> struct XmixShape<XM_TANGENT> { static constexpr int NF = 12, NOUT = 3; };
 template <> struct XmixShape<XM_ADJOINT> { static constexpr int NF = 9, NOUT = 6; };
 template <> struct XmixShape<XM_PHYS> { static constexpr int NF = 3, NOUT = 0; };
-template <> struct XmixShape<XM_DOT> { static constexpr int NF = 6, NOUT = 0; };
 template <> struct XmixShape<XM_ADJ2> { static constexpr int NF = 15, NOUT = 12; };
 
 // grid (Pz/ZB, qpts); LDS: ZB * nseq sequences of Px complex at a stride of
@@ -606,7 +604,6 @@ template <int MODE, int PX>
 __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx *W1, cplx *W2)
 {
     extern __shared__ double2 lds[];
-    __shared__ double red[2][BLK];
     if (*a.abort_flag) return;
     constexpr int nf = XmixShape<MODE>::NF;
     constexpr int NOUT = XmixShape<MODE>::NOUT;
@@ -731,7 +728,7 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
                                 : (XMIX_TILE / nseq + BLK - 1) / BLK;
     constexpr int NO = NOUT > 0 ? NOUT : 1;
     double outv[MAXP][NO];
-    double cmax = 0.0, acc0 = 0.0, acc1 = 0.0;
+    double cmax = 0.0;
     const int npt = ZB * Px;
     Walk wp(Px);                                       // (z row r, point n)
 #pragma unroll
@@ -797,9 +794,6 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
             a.flow[o] = u;
             a.flow[plane + o] = v;
             a.flow[2 * plane + o] = w;
-        } else {   // XM_DOT: f[0..2] = du/dt, f[3..5] = v
-            acc0 += f[0] * f[3] + f[1] * f[4] + f[2] * f[5];
-            acc1 += f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
         }
     }
     if (a.check_cfl) {
@@ -817,24 +811,6 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
             for (int w = 1; w < BLK / 64; ++w) b = fmax(b, cw[w]);
             atomic_max_nonneg(&a.cflmax[y], b);
         }
-    }
-    if constexpr (MODE == XM_DOT) {
-        red[0][threadIdx.x] = acc0;
-        red[1][threadIdx.x] = acc1;
-        __syncthreads();
-        for (int o = BLK / 2; o > 0; o >>= 1) {
-            if (threadIdx.x < o) {
-                red[0][threadIdx.x] += red[0][threadIdx.x + o];
-                red[1][threadIdx.x] += red[1][threadIdx.x + o];
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            size_t slot = ((size_t) y * gridDim.x + zb) * 2;
-            a.dotpart[slot] = red[0][0] * a.W[y];
-            a.dotpart[slot + 1] = red[1][0] * a.W[y];
-        }
-        return;
     }
     if constexpr (MODE == XM_PHYS) return;
     if constexpr (NOUT > 0) {
@@ -1222,19 +1198,17 @@ static XmixKernel xmix_kernel(int mode, int Px, int nseq, int zb)
     case XM_PRIMAL: return xmix_kernel_mode<XM_PRIMAL>(Px, nseq, zb);
     case XM_TANGENT: return xmix_kernel_mode<XM_TANGENT>(Px, nseq, zb);
     case XM_ADJOINT: return xmix_kernel_mode<XM_ADJOINT>(Px, nseq, zb);
-    case XM_PHYS: return xmix_kernel_mode<XM_PHYS>(Px, nseq, zb);
-    default: return xmix_kernel_mode<XM_DOT>(Px, nseq, zb);
+    default: return xmix_kernel_mode<XM_PHYS>(Px, nseq, zb);
     }
 }
 
 void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
-                 double *flow, double *dot_out, const cplx *W1b, int nf1)
+                 double *flow, const cplx *W1b, int nf1)
 {
     const Dims &d = c.d;
     XmixArgs a;
     a.mode = mode;
-    a.nf = mode == XM_PRIMAL ? 6 : mode == XM_TANGENT ? 12 : mode == XM_ADJOINT ? 9
-         : mode == XM_PHYS ? 3 : 6;
+    a.nf = mode == XM_PRIMAL ? 6 : mode == XM_TANGENT ? 12 : mode == XM_ADJOINT ? 9 : 3;
     a.nout = mode == XM_ADJOINT ? 6 : (mode == XM_PRIMAL || mode == XM_TANGENT) ? 3 : 0;
     a.nf1 = W1b ? nf1 : a.nf;
     a.W1b = W1b ? W1b : W1;
@@ -1255,7 +1229,6 @@ void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
     a.W = c.dW;
     a.cflmax = c.cflmax;
     a.flow = flow;
-    a.dotpart = dot_out;
     a.abort_flag = c.abort_flag;
     dim3 grid(d.Pz / zb, d.qpts);
     size_t lds = sizeof(double2) * (zb * nseq * (d.Px + 1) + d.Px);
@@ -1311,7 +1284,6 @@ void launch_xmix_adj2(Context &c, const cplx *Wa, const cplx *Wb, const cplx *Wl
     a.W = c.dW;
     a.cflmax = c.cflmax;
     a.flow = nullptr;
-    a.dotpart = nullptr;
     a.abort_flag = c.abort_flag;
     const dim3 grid(d.Pz / zb, d.qpts);
     const size_t lds = sizeof(double2) * (zb * 8 * (d.Px + 1) + d.Px);

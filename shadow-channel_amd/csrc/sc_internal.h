@@ -69,7 +69,6 @@ enum XMode : int {
     XM_TANGENT = 1,  // 12 in (base, tangent) -> 3 out (IHz, IHx, IHy)
     XM_ADJOINT = 2,  // 9 in (u,v,w,oz,ox,oy,lx,ly,lz) -> 6 out (Sx,Sy,Sz,Tx,Ty,Tz)
     XM_PHYS = 3,     // 3 in -> physical flow[3][Px][qpts][Pz]
-    XM_DOT = 4,      // 6 in (dudt xyz, v xyz) -> weighted dot products
     XM_ADJ2 = 5,     // 15 in (two states' 6 fields, lambda) -> 2 x 6 out (the two
                      // adjoint product sets of one lambda, into W2 and W2b)
 };
@@ -174,7 +173,7 @@ void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1);
 // Fields 0..nf1-1 come from W1 ([qpts][nf1][Pz][Mx]), the rest from W1b
 // ([qpts][nf-nf1][Pz][Mx]); W1b null: all fields from W1.
 void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
-                 double *flow, double *dot_out, const cplx *W1b = nullptr, int nf1 = 0);
+                 double *flow, const cplx *W1b = nullptr, int nf1 = 0);
 // cfl_slot >= 0: the first block runs the CFL check (cfl_check_block) with
 // that slot: the first stored state of the step being checked
 void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind, int cfl_slot = -1);
