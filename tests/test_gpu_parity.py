@@ -373,3 +373,29 @@ def test_device_resident_entry_points(cc):
         assert rel(Cs[i], o.getsoln(i)) <= 1e-9
     with pytest.raises(AssertionError):
         cc.sc_adjoint_dev(3, 0, 0, 1.0)
+
+
+G4 = dict(Nx=128, Ny=129, Nz=128, Lx=2 * np.pi, Lz=np.pi, Re=5000.0, flux=4.0 / 3.0, dt=0.002)
+
+
+def test_largest_grid_one_step(cc):
+    """The largest configuration, 128x128x129 (dimR = 127): the chunked
+    rebuild, the 28-k-step projections and generic 192-point transforms.
+    One primal and one forced adjoint step against the oracle.  The rounding
+    floor at this grid (the oracle with and without FMA contraction,
+    tests/test_oracle_rounding_floor.py) is 9.8e-10 (step) and 1.75e-8
+    (adjoint); the device lands at about 1.1x those, and the bounds allow 3x."""
+    print("case: largest grid 128x128x129, primal + adjoint 1 step", flush=True)
+    o, C0 = start(cc, G4, 1)
+    Co, Cg = C0.copy(), C0.copy()
+    o.primal(0, Co)
+    cc.c_primal(0, Cg)
+    ep = rel(Cg, Co)
+    a = direction(G4, 4)
+    Ao, Ag = a.copy(), a.copy()
+    o.adjoint(1, 0, Ao, 0, 1.0)
+    cc.c_adjoint(1, 0, Ag, 0, 1.0)
+    ea = rel(Ag, Ao)
+    print("largest grid: step %.2e adjoint %.2e" % (ep, ea), flush=True)
+    assert ep <= 3e-9
+    assert ea <= 6e-8

@@ -118,3 +118,39 @@ def test_rounding_floor_at_benchmark_grid():
     adj = _both_builds(run("adjoint"))
     assert 1e-12 < step < 2e-10
     assert 1e-11 < adj < 3e-9
+
+
+@needs_fma
+def test_rounding_floor_at_largest_grid():
+    """128x128x129 channel (Lx = 2 pi, Lz = pi, Re = 5000, dt = 0.002): one
+    primal step and one adjoint step (strength 1), about 100 s.  Measured
+    floors 9.8e-10 (step) and 1.75e-8 (adjoint); the GPU tolerances at this
+    grid (3e-9, 6e-8) stay above them."""
+    c = dict(Nx=128, Ny=129, Nz=128, Lx=2 * np.pi, Lz=np.pi, Re=5000.0, flux=4.0 / 3.0, dt=0.002)
+
+    def run():
+        o = pyorc.Oracle(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
+                         c["dt"], 1)
+        C = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=0)
+        o.primal(0, C)
+        A = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=4,
+                            unit=True)
+        o.adjoint(1, 0, A, 0, 1.0)
+        o.destroy()
+        return np.concatenate([C.ravel(), A.ravel()])
+
+    outs = []
+    for path in (pyorc.lib_path(), FMA_LIB):
+        pyorc._LIB = None
+        orig = pyorc.lib_path
+        pyorc.lib_path = lambda p=path: p
+        try:
+            outs.append(run())
+        finally:
+            pyorc.lib_path = orig
+            pyorc._LIB = None
+    n = outs[0].size // 2
+    step = np.linalg.norm(outs[0][:n] - outs[1][:n]) / np.linalg.norm(outs[0][:n])
+    adj = np.linalg.norm(outs[0][n:] - outs[1][n:]) / np.linalg.norm(outs[0][n:])
+    assert 1e-12 < step < 1.5e-9, step
+    assert 1e-11 < adj < 3e-8, adj
