@@ -107,6 +107,26 @@ int sc_wall_plane(const sc_complex *C_ptr, sc_complex *Uxb, sc_complex *Uzb);
    Nz < 4. */
 int sc_incre_boundary(sc_complex *Uxb, sc_complex *Uzb);
 
+/* channel/__init__.py:217-299 (velAvg, IvelAvg and the profiles of
+   uavgGrad) and the wall shear Uxb(0,0) of channel/project0.c:227-234, as
+   device reductions over stored steps start_step .. end_step - 1 of the
+   primal (incre 0, MC[3i]) or the tangent (incre 1, MIC[3i]):
+     profile[y] = weight sum_i <u>(y_j, step i)   (y = 0 .. qpts-1; <u> is the
+                  x-z plane mean of the streamwise velocity, Re u_00)
+     shear[0]   = weight sum_i d<u>/dy at y = -1, shear[1] the same at y = +1.
+   The reference forms the profile from a full spec2phys per step; here it is
+   R (sum_i Re alpha_i(0,0)) weight, equal up to rounding.  Requires
+   0 <= start_step <= end_step <= nsteps + 1. */
+int sc_mean_profile(int incre, int start_step, int end_step, double weight, double *profile,
+                    double *shear);
+
+/* channel/comp_stat.c:9-230: plane statistics of the stored state at step
+   i_step (primal MC[3i], or with incre the tangent MIC[3i]) at every Gauss
+   point, stats[q][y] (q < 10, y < qpts): plane means of u, v, w, then the
+   plane means of uv, uw, vw, uu, vv, ww, then of du/dy.  Formed from the
+   spectra (Parseval on the dealiased grid) instead of transformed planes. */
+int sc_step_statistics(int incre, int i_step, double *stats);
+
 /* channel/channel.i:206-218: Gauss points and weights (library-owned) */
 int c_quad(double **y_ptr, int *qpts_ptr1, double **w_ptr, int *qpts_ptr2);
 

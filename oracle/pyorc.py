@@ -14,7 +14,9 @@ _LIB = None
 
 
 def lib_path():
-    return os.path.join(_HERE, "liborc.so")
+    """The oracle library; ORC_LIB names another build of the same sources
+    (the planted-mistake tests load mutated copies this way)."""
+    return os.environ.get("ORC_LIB") or os.path.join(_HERE, "liborc.so")
 
 
 def build():
@@ -69,6 +71,9 @@ def load():
     L.orc_incre_boundary.argtypes = [c.c_void_p, c.c_void_p]
     L.orc_incre_boundary.restype = c.c_int
     L.orc_rand.restype = c.c_int
+    L.orc_flux_trace.argtypes = [c.c_void_p, c.c_int]
+    L.orc_flux_trace.restype = c.c_int
+    L.orc_set_adjoint_refresh.argtypes = [c.c_int]
     for f in ("orc_qpts", "orc_dimR", "orc_dimQ", "orc_Nx", "orc_Nz",
               "orc_nsteps", "orc_last_status"):
         getattr(L, f).restype = c.c_int
@@ -211,6 +216,19 @@ class Oracle:
         assert ub.shape == wb.shape == (self.Nz, self.Nx // 2)
         assert self.L.orc_incre_boundary(_ptr(ub), _ptr(wb)) == 0
         return ub, wb
+
+    def flux_trace_start(self, cap=4096):
+        """Record the mean-mode flux correction of every later stage."""
+        self._ftrace = np.zeros((cap, 5))
+        self.L.orc_flux_trace(_ptr(self._ftrace), cap)
+
+    def flux_trace_stop(self):
+        """Records {mode, k, target, error before, error after} since
+        flux_trace_start (mode 0 primal, 1 tangent, 2 adjoint)."""
+        n = self.L.orc_flux_trace(None, 0)
+        out = self._ftrace[:n].copy()
+        self._ftrace = None
+        return out
 
     def last_status(self):
         return self.L.orc_last_status()

@@ -946,6 +946,20 @@ static void mean_rhs(const cx *Ua, int adj, double *fa, double *fb)
     }
 }
 
+/* flux-correction trace (test hook): per mean-mode stage, the record
+   {mode, stage k, target, flux error before the correction, after it} */
+static double *g_ftrace = NULL;
+static int g_ftrace_n = 0, g_ftrace_cap = 0;
+
+int orc_flux_trace(double *buf, int cap_records)
+{
+    int n = g_ftrace_n;
+    g_ftrace = buf;
+    g_ftrace_cap = buf ? cap_records : 0;
+    g_ftrace_n = 0;
+    return n;
+}
+
 static void stage_mean(cx *Ca, const cx *Ua, const cx *LUa, cx *Uout,
                        cx *store, int k, int mode, double target, int wall)
 {
@@ -992,9 +1006,18 @@ static void stage_mean(cx *Ca, const cx *Ua, const cx *LUa, cx *Uout,
     for (int i = 0; i < nR; ++i)
         for (int j = 0; j < q; ++j) flux_t += G.Rw[i * q + j] * al[i * s].re;
     flux_t = target - flux_t;
+    const double err_before = flux_t;
     for (int j = 0; j < nR; ++j) {
         al[j * s].re = al[j * s].re + flux_t * add[j] * 0.5;
         al[j * s].im = al[j * s].im + flux_t * 0.0 * 0.5;
+    }
+    if (g_ftrace && g_ftrace_n < g_ftrace_cap) {
+        /* flux error target - sum_ij Rw_ij Re alpha_i after the correction */
+        double after = 0;
+        for (int i = 0; i < nR; ++i)
+            for (int j = 0; j < q; ++j) after += G.Rw[i * q + j] * al[i * s].re;
+        double *r = g_ftrace + 5 * g_ftrace_n++;
+        r[0] = mode; r[1] = k; r[2] = target; r[3] = err_before; r[4] = target - after;
     }
     rebuild_mean(Ca, Uout);
     if (wall) wall_mean(Ca);
@@ -1372,6 +1395,16 @@ double orc_ddt_project(int i_step, orc_cplx *v)
     return num / den;
 }
 
+/* The reference subtracts the trapezoid forcing from AC only: the adjoint
+   velocities AU, rebuilt once before the loop (adj_initAlphaBeta2,
+   channel/channel.c:617) and after every stage solve, are not rebuilt after
+   a forcing (channel/channel.c:622-631, 682-689), so the forcing reaches the
+   convective term of pass2 one stage late.  Test hook: with the refresh on,
+   AU is rebuilt from AC after each forcing, which makes the forced adjoint
+   exactly the superposition of unforced adjoints of the forcing terms. */
+static int g_adj_refresh = 0;
+void orc_set_adjoint_refresh(int on) { g_adj_refresh = on; }
+
 int orc_adjoint(int start, int end, orc_cplx *ACg, int inhomo, double strength)
 {
     (void) inhomo;
@@ -1390,6 +1423,7 @@ int orc_adjoint(int start, int end, orc_cplx *ACg, int inhomo, double strength)
             G.AC[e].re -= 0.5 * strength * tmp[e].re;
             G.AC[e].im -= 0.5 * strength * tmp[e].im;
         }
+        if (g_adj_refresh) coef_to_gauss(G.AC, G.AU, 0);
         for (int k = 0; k < 3; ++k) {
             int count = n * 3 - k - 1;
             if (k < 2) {
@@ -1409,6 +1443,7 @@ int orc_adjoint(int start, int end, orc_cplx *ACg, int inhomo, double strength)
             G.AC[e].re -= 0.5 * strength * tmp[e].re;
             G.AC[e].im -= 0.5 * strength * tmp[e].im;
         }
+        if (g_adj_refresh) coef_to_gauss(G.AC, G.AU, 0);
     }
     memcpy(ACg, G.AC, sizeof(cx) * cs);
     free(tmp);

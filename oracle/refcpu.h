@@ -89,6 +89,16 @@ void   orc_get_state(orc_cplx *C, orc_cplx *U);
 void   orc_wall_plane(const orc_cplx *C, orc_cplx *Uxb, orc_cplx *Uzb);
 /* increBoundary in place on two wall planes; -1 if Nz < 4 or Nx < 4. */
 int    orc_incre_boundary(orc_cplx *Uxb, orc_cplx *Uzb);
+/* Flux-correction trace of the mean-mode stages: with buf non-NULL, each
+   later stage appends {mode (0 primal, 1 tangent, 2 adjoint), stage k,
+   flux target, flux error before the correction, error after it} while
+   fewer than cap_records are stored; NULL stops it.  Returns the number of
+   records stored since the previous call. */
+int    orc_flux_trace(double *buf, int cap_records);
+/* Test hook: 1 rebuilds the adjoint velocities after each trapezoid forcing
+   of orc_adjoint (the reference does not: the forcing reaches the
+   convective term one stage late); 0 (default) follows the reference. */
+void   orc_set_adjoint_refresh(int on);
 
 #ifdef __cplusplus
 }

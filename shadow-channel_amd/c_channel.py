@@ -87,6 +87,11 @@ _L.sc_primal_dev.restype = _i
 _L.sc_adjoint_dev.argtypes = [_i, _i, _c.c_void_p, _d]
 _L.sc_adjoint_dev.restype = _i
 
+_L.sc_mean_profile.argtypes = [_i, _i, _i, _d, _vp, _vp]
+_L.sc_mean_profile.restype = _i
+_L.sc_step_statistics.argtypes = [_i, _i, _vp]
+_L.sc_step_statistics.restype = _i
+
 SC_EINVAL = -3
 SC_ENOTREADY = -2
 
@@ -207,8 +212,15 @@ def sc_adjoint_dev(start_step, end_step, AC_dev, strength):
 def c_ddt_project(i_step, IC_given):
     IC = _inplace(IC_given, np.complex128, "IC_given")
     r = _L.c_ddt_project(int(i_step), IC.ctypes.data, *IC.shape)
-    if np.isnan(r) and last_error():
-        pass
+    if np.isnan(r):
+        # NaN with an error text is a failed call (an invalid one raises
+        # AssertionError, as the reference's assert would abort); NaN
+        # without one is the projection's own 0/0 (a constant trajectory)
+        msg = last_error()
+        if msg.startswith("ddt_project requires"):
+            raise AssertionError("c_ddt_project: " + msg)
+        if msg:
+            raise RuntimeError("c_ddt_project: " + msg)
     return r
 
 
@@ -366,3 +378,23 @@ def sc_incre_boundary(Uxb, Uzb):
             raise TypeError("%s must be C-contiguous, writeable, of shape (Nz, Nx/2)" % name)
     ub, wb = Uxb, Uzb
     _check(_L.sc_incre_boundary(ub.ctypes.data, wb.ctypes.data), "sc_incre_boundary")
+
+
+def sc_mean_profile(incre, start_step, end_step, weight):
+    """weight * (sum over stored steps start_step .. end_step - 1 of the x-z
+    plane-mean streamwise velocity profile at the Gauss points) of the primal
+    (incre 0) or tangent (incre 1); returns (profile[qpts], shear[2]) with the
+    same sums of d<u>/dy at y = -1 and y = +1."""
+    prof = np.zeros(c_qpts())
+    shear = np.zeros(2)
+    _check(_L.sc_mean_profile(int(incre), int(start_step), int(end_step), float(weight),
+                              prof.ctypes.data, shear.ctypes.data), "sc_mean_profile")
+    return prof, shear
+
+
+def sc_step_statistics(incre, i_step):
+    """Plane statistics [10, qpts] of the stored state at i_step: means of u,
+    v, w, then of uv, uw, vw, uu, vv, ww, then of du/dy."""
+    st = np.zeros((10, c_qpts()))
+    _check(_L.sc_step_statistics(int(incre), int(i_step), st.ctypes.data), "sc_step_statistics")
+    return st

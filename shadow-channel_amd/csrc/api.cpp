@@ -109,7 +109,8 @@ static void free_all(Context &c)
                     c.dRowsum, c.C, c.IC, c.AC, c.U, c.IU, c.AU, c.LU, c.Ut, c.Ut2, c.MC,
                     c.MIC, c.Csol, c.Ctmp, c.Ctmp2, c.Fa, c.Fb, c.FLa, c.FLb, c.W1, c.W2, c.W2b,
                     c.SP, c.flowdev, c.cflmax, c.dotpart, c.scal, c.abort_flag, c.pz.tw,
-                    c.px.tw, c.dFragR, c.dFragF, c.dFragP, c.dFragQ, c.wcache, c.rb_queue};
+                    c.px.tw, c.dFragR, c.dFragF, c.dFragP, c.dFragQ, c.wcache, c.rb_queue,
+                    c.objscr, c.dRwall};
     for (void *p : ptrs)
         if (p) (void) hipFree(p);
     for (int a = 0; a < 2; ++a)
@@ -235,16 +236,18 @@ static void pass2_adjoint(Context &c, const cplx *Ub, const cplx *AUb, cplx *Uds
 // vout = vin - (num/den) dC/dt with the Gauss-weighted physical inner
 // products (channel/channel.c:704-780); device arrays, vout may be vin.
 // Result in scal[2].  With acc, also acc -= a vout.
-static void ddt_project_dev(Context &c, int i, const cplx *vin, cplx *vout,
-                            cplx *acc = nullptr, double a = 0.0, int twice = 0)
+static int ddt_project_dev(Context &c, int i, const cplx *vin, cplx *vout,
+                           cplx *acc = nullptr, double a = 0.0, int twice = 0)
 {
     const Dims &d = c.d;
     long ip = i == 0 ? 3 : 3L * i, im = i == 0 ? 0 : 3L * (i - 1);
     // dudt = (MC[ip] - MC[im]) / dt; the inner products in closed form from
     // the coefficients (equal to the grid sums of the rebuilt fields)
     const int nparts = launch_ddt_gram(c, mc(c, ip), mc(c, im), 1.0 / d.dt, vin);
+    if (nparts <= 0) return SC_EDEVICE;
     launch_ddt_apply(c, vout, vin, mc(c, ip), mc(c, im), 1.0 / d.dt, c.csize, acc, a, nparts,
                      twice);
+    return SC_OK;
 }
 
 // the two adjoint passes of one lambda (AUb) with the cached states wa
@@ -269,6 +272,20 @@ static void pass2_adjoint_pair(Context &c, const cplx *AUb, const cplx *wa, cons
             launch_assemble_adj(c, c.SP, Ud[p]);
         }
     }
+}
+
+// Kernel error bits (abort_flag[2]) left by the last run: reported once
+// through last_error() and cleared.
+static int check_kernel_error(Context &c)
+{
+    int e = 0;
+    SC_HIP(hipMemcpy(&e, c.abort_flag + 2, sizeof(int), hipMemcpyDeviceToHost));
+    if (!e) return SC_OK;
+    SC_HIP(hipMemset(c.abort_flag + 2, 0, sizeof(int)));
+    last_error() = "kernel error " + std::to_string(e) +
+                   ((e & SC_KERR_SOLVE_WAIT) ? ": band solve factor staging never signalled" : "");
+    fprintf(stderr, "shadow-channel: %s\n", last_error().c_str());
+    return SC_EDEVICE;
 }
 
 static int check_abort(Context &c, const char *what)
@@ -521,6 +538,10 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
     const size_t cb = sizeof(cplx) * c.csize, ub = sizeof(cplx) * c.usize;
     const size_t fb = sizeof(cplx) * (size_t) d.Nz * d.dimR * d.Mx;
     const size_t mcb = cb * (3 * (size_t) nstep + 1);
+    // ddt_project's partial sums: one pair per block of its inner-product
+    // kernel, at most one block per (kz, kx) column
+    const size_t ndot = 2 * (size_t) d.Nz * d.Mx;
+    c.ndotpart = ndot;
     struct { void **p; size_t n; } al[] = {
         {(void **) &c.C, cb}, {(void **) &c.IC, cb}, {(void **) &c.AC, cb},
         {(void **) &c.Csol, cb}, {(void **) &c.Ctmp, cb}, {(void **) &c.Ctmp2, cb},
@@ -534,12 +555,29 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
         {(void **) &c.SP, sizeof(cplx) * 6 * (size_t) d.Nz * d.qpts * d.Mx},
         {(void **) &c.flowdev, sizeof(double) * 3 * (size_t) d.Px * d.qpts * d.Pz},
         {(void **) &c.cflmax, sizeof(double) * (d.qpts + 1)},
-        {(void **) &c.dotpart, sizeof(double) * 2 * (size_t) d.qpts * d.Pz},
+        {(void **) &c.dotpart, sizeof(double) * ndot},
         {(void **) &c.scal, sizeof(double) * 8},
-        {(void **) &c.abort_flag, 2 * sizeof(int)},
+        {(void **) &c.abort_flag, 4 * sizeof(int)},
+        {(void **) &c.objscr, sizeof(double) * objective_scratch_doubles(d)},
     };
     for (auto &e : al) {
         if (hipMalloc(e.p, e.n) != hipSuccess || hipMemset(*e.p, 0, e.n) != hipSuccess) {
+            last_error() = "device allocation failed";
+            free_all(c);
+            return SC_EINIT;
+        }
+    }
+    {
+        // wall derivatives of the R basis: R_j'(-1) = Rp0_j and, R_j having
+        // the parity of j, R_j'(+1) = -(-1)^j Rp0_j
+        std::vector<double> rw(2 * (size_t) d.dimR);
+        for (int j = 0; j < d.dimR; ++j) {
+            rw[j] = c.Rp0[j];
+            rw[d.dimR + j] = (j % 2 == 0 ? -1.0 : 1.0) * c.Rp0[j];
+        }
+        if (hipMalloc(&c.dRwall, sizeof(double) * rw.size()) != hipSuccess ||
+            hipMemcpy(c.dRwall, rw.data(), sizeof(double) * rw.size(), hipMemcpyHostToDevice) !=
+                hipSuccess) {
             last_error() = "device allocation failed";
             free_all(c);
             return SC_EINIT;
@@ -629,6 +667,7 @@ static int primal_run(int ru_steps, sc_complex *C_given, hipMemcpyKind in, hipMe
         }
     }
     int rc = stream_sync(c);
+    if (!rc) rc = check_kernel_error(c);
     if (rc) return rc;
     // after an abort the queued launches did nothing: no cached field is
     // valid, and the states from the stopped step on were not written
@@ -702,6 +741,7 @@ int c_tangent(int start, int end, sc_complex *IC_given, int Nz_dup, int Nvar_dup
         }
     }
     int rc = stream_sync(c);
+    if (!rc) rc = check_kernel_error(c);
     if (rc) return rc;
     check_abort(c, "Pass1");
     SC_HIP(hipMemcpy(IC_given, c.IC, cb, hipMemcpyDeviceToHost));
@@ -768,6 +808,7 @@ static int adjoint_run(int start, int end, sc_complex *AC_given, double strength
         ddt_project_dev(c, n - 1, mic(c, 3L * (n - 1)), c.Ctmp, c.AC, 0.5 * s, n - 1 > end);
     }
     int rc = stream_sync(c);
+    if (!rc) rc = check_kernel_error(c);
     if (rc) return rc;
     SC_HIP(hipMemcpy(AC_given, c.AC, cb, out));
     return SC_OK;
@@ -799,17 +840,27 @@ double c_ddt_project(int i_step, sc_complex *IC_given, int Nz_dup, int Nvar_dup,
                      int Nx_dup)
 {
     Context &c = ctx();
+    last_error().clear();
     if (!c.ok || !IC_given || !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup) || i_step < 0 ||
-        i_step > c.d.nsteps || c.d.nsteps < 1)
+        i_step > c.d.nsteps || c.d.nsteps < 1) {
+        last_error() = "ddt_project requires an initialised run, 0 <= i_step <= nsteps, "
+                       "nsteps >= 1 and a (Nz, 2, dimR, Nx/2) array";
         return NAN;
+    }
     const size_t cb = sizeof(cplx) * c.csize;
-    if (hipMemcpyAsync(c.Ctmp, IC_given, cb, hipMemcpyHostToDevice, c.stream) != hipSuccess) return NAN;
-    SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream)) ;
-    ddt_project_dev(c, i_step, c.Ctmp, c.Ctmp);
+    if (hipMemcpyAsync(c.Ctmp, IC_given, cb, hipMemcpyHostToDevice, c.stream) != hipSuccess ||
+        hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream) != hipSuccess) {
+        last_error() = "ddt_project: HIP transfer failed";
+        return NAN;
+    }
+    if (ddt_project_dev(c, i_step, c.Ctmp, c.Ctmp) != SC_OK) return NAN;
     double scal[3] = {0, 0, 0};
     if (hipMemcpyAsync(IC_given, c.Ctmp, cb, hipMemcpyDeviceToHost, c.stream) != hipSuccess) return NAN;
     if (hipMemcpyAsync(scal, c.scal, sizeof(scal), hipMemcpyDeviceToHost, c.stream) != hipSuccess) return NAN;
-    if (hipStreamSynchronize(c.stream) != hipSuccess) return NAN;
+    if (hipStreamSynchronize(c.stream) != hipSuccess) {
+        last_error() = "ddt_project: HIP synchronisation failed";
+        return NAN;
+    }
     return scal[2];
 }
 

@@ -1435,6 +1435,7 @@ struct SolveArgs {
     double ce0[2], cn[2];              // explicit-operator coefficients per field (stage 0, next)
     double target;                     // mean-mode flux target
     const int *abort_flag;
+    int *err;                          // device error word (SC_KERR_* bits)
 };
 
 // explicit band operator row entry A(i, i+2m)
@@ -1745,15 +1746,32 @@ __device__ __forceinline__ void bwd_step_r(const double *u, double x, double *w,
 struct StagedRows {
     const int *ready;
     int cr, nchunk, nwriters;
+    int *err;          // device error word (SC_KERR_* bits)
 };
 
-// wait until the U parts of rows >= row are in LDS (chunks in order)
+// Polls of one chunk's counter before the wait gives up: each poll sleeps
+// 64 clocks, so the limit is ~30 ms, against waits of a few microseconds.
+constexpr int SOLVE_SPIN_LIMIT = 1 << 20;
+
+// wait until the U parts of rows >= row are in LDS (chunks in order).  A
+// counter that never reaches the writer count (a staging wave that did not
+// signal) sets SC_KERR_SOLVE_WAIT in the error word and ends the wait, so
+// the kernel still exits; the host reports the error after the run.
 __device__ __forceinline__ void wait_rows(const StagedRows &sr, int row, int &low, int &next)
 {
     while (row < low && next < sr.nchunk) {
+        int spins = 0;
         while (__hip_atomic_load(&sr.ready[next], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) <
-               sr.nwriters)
+               sr.nwriters) {
+            if (++spins > SOLVE_SPIN_LIMIT) {
+                __hip_atomic_fetch_or(sr.err, SC_KERR_SOLVE_WAIT, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+                next = sr.nchunk;
+                low = 0;
+                return;
+            }
             __builtin_amdgcn_s_sleep(1);
+        }
         ++next;
         low = max(low - sr.cr, 0);
     }
@@ -1935,8 +1953,11 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx 
     // The other waves stage the U parts meanwhile, chunk by chunk, and signal.
     const int cl = tid & 63, cw = chain_wave[0];
     const int cr = (nr + NCHU - 1) / NCHU;
+    // Every wave but the chain wave stages: each signals every chunk exactly
+    // once (both staging paths below), so a chunk is complete at NWR signals.
     constexpr int NWR = SOLVE_THREADS / 64 - 1;          // staging waves
-    const StagedRows sr{ready, cr, NCHU, NWR};
+    static_assert(NWR >= 1 && SOLVE_THREADS % 64 == 0, "one chain wave, the rest stage");
+    const StagedRows sr{ready, cr, NCHU, NWR, a.err};
     if ((tid >> 6) == cw) {
         // lane cl < 2 SBT: part cl / SBT (real, imaginary) of column cl % SBT
         if (cl < 2 * SBT && co.chain && nr > 0) {
@@ -2242,6 +2263,7 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
     a.cn[1] = k != 2 ? -(d.re * A[k + 1] * d.dt) : 0.0;
     a.target = target;
     a.abort_flag = c.abort_flag;
+    a.err = c.abort_flag + 2;
     // RHS in, projections in (the next stage's only when they are other
     // arrays: the primal's are the same), solution and next right-hand side
     // out, and the factor images in (each stored image once: mirror rows
@@ -2594,6 +2616,11 @@ int launch_ddt_gram(Context &c, const cplx *P, const cplx *M, double s, const cp
     int ncol = 4;
     while (ncol > 1 && (2 * d.dimR * ncol > 256 * DDT_IT || lds_of(ncol) > 65536)) ncol /= 2;
     const int nb = d.Nz * ((d.Mx + ncol - 1) / ncol);
+    if (2 * (size_t) nb > c.ndotpart) {
+        // c_init sizes dotpart for one column per block; never reached
+        last_error() = "ddt_project: partial-sum buffer too small";
+        return 0;
+    }
     const size_t lds = lds_of(ncol);
     if (lds > 65536)
         (void) hipFuncSetAttribute((const void *) k_ddt_gram,

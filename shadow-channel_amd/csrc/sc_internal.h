@@ -23,6 +23,9 @@ namespace sc {
 
 typedef double2 cplx;
 
+// kernel error bits in abort_flag[2]
+constexpr int SC_KERR_SOLVE_WAIT = 1;   // band solve: factor staging never signalled
+
 constexpr int MAXF = 12;        // fields in one z-inverse launch
 constexpr int FFT_TILE = 2048;  // complex values (data + twiddles) in one transform block
 constexpr int FFT_BLOCK = 256;  // threads of a transform block
@@ -144,8 +147,13 @@ struct Context {
     // checks), then the maximum of the plane that stopped a run
     double *cflmax = nullptr;
     double *dotpart = nullptr;                  // partial sums
+    size_t ndotpart = 0;                        // doubles in dotpart
     double *scal = nullptr;                     // [8] device scalars
-    int *abort_flag = nullptr;                  // [0] CFL stop (1 + plane), [1] first unwritten slot
+    // [0] CFL stop (1 + plane), [1] first unwritten slot, [2] kernel error
+    // bits (SC_KERR_*, sticky until reported), [3] unused
+    int *abort_flag = nullptr;
+    double *objscr = nullptr;                   // objectives and statistics (objective_scratch_doubles)
+    double *dRwall = nullptr;                   // [2][dimR]: R_j'(-1), R_j'(+1)
     FftPlan pz, px;
     size_t csize = 0, usize = 0;                // element counts
     int status = 0;                             // -1 after a CFL failure
@@ -219,6 +227,9 @@ int launch_ddt_gram(Context &c, const cplx *P, const cplx *M, double s, const cp
 // vout = vin - (num/den) s (P - M); with acc, also acc -= a vout (twice)
 void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, const cplx *M,
                       double s, size_t n, cplx *acc, double a, int nparts, int twice);
+
+// doubles of the objective/statistics scratch buffer for a grid and window
+size_t objective_scratch_doubles(const Dims &d);
 
 // dst_kind for launch_zfwd
 enum { ZF_USLOTS = 0, ZF_SCRATCH = 1 };

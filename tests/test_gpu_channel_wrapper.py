@@ -134,3 +134,58 @@ def test_double_init_is_refused(run):
     ch, o = run
     with pytest.raises(AssertionError):
         ch.para_init(N, ch.get_solution(0))
+
+
+def _mean_profile_poly(o, C):
+    """The plane-mean u profile of C as a Legendre series in y (degree
+    qpts - 1, exact for the profile's polynomial of degree dimR + 1), fitted
+    through the oracle's physical field at the Gauss points."""
+    y, w = o.quad()
+    u = oracle_mean_u(o, C)
+    return np.polynomial.legendre.Legendre.fit(y, u, o.qpts - 1, domain=[-1, 1])
+
+
+def test_wall_shear_against_the_profile_derivative(run):
+    """wallShear (the Uxb(0,0) of channel/project0.c:227-234 at y = -1, and
+    the same at y = +1) time-averaged over the window, against the
+    derivative at the walls of the Legendre series through the oracle's
+    physical plane means; also the tangent's (incre 1)."""
+    print("case: wrapper wallShear", flush=True)
+    ch, o = run
+    T = N * CFG["dt"]
+    ref = np.zeros(2)
+    iref = np.zeros(2)
+    for i in range(N):
+        d = _mean_profile_poly(o, o.getsoln(i).copy()).deriv()
+        ref += (CFG["dt"] / T) * np.array([d(-1.0), d(1.0)])
+        di = _mean_profile_poly(o, o.getincresoln(i).copy()).deriv()
+        iref += (CFG["dt"] / T) * np.array([di(-1.0), di(1.0)])
+    got = ch.wallShear(0, N, T)
+    assert ref[0] > 0 and ref[1] < 0          # drag on both walls
+    assert rel(got, ref) <= 1e-9
+    assert rel(ch.wallShear(0, N, T, incre=1), iref) <= 1e-9
+    # the wall plane of one state (sc_wall_plane) gives the same lower-wall value
+    import c_channel
+    ub, _ = c_channel.sc_wall_plane(ch.get_solution(2))
+    one = ch.wallShear(2, 3, CFG["dt"])
+    assert abs(one[0] - ub[0, 0].real) <= 1e-12 * abs(ub[0, 0].real)
+
+
+def test_plane_statistics_against_physical_moments(run):
+    """statistics (channel/comp_stat.c:9-230: plane means of u, v, w, their
+    second moments and du/dy) from the spectra, against the same moments of
+    the oracle's physical field (mean over the dealiased x-z grid)."""
+    print("case: wrapper statistics", flush=True)
+    ch, o = run
+    for i, incre in ((1, 0), (N, 0), (2, 1)):
+        C = (o.getincresoln(i) if incre else o.getsoln(i)).copy()
+        f = o.spec2phys(C)
+        pm = lambda a: a.mean(axis=2).mean(axis=0)
+        ref = [pm(f[0]), pm(f[1]), pm(f[2]), pm(f[0] * f[1]), pm(f[0] * f[2]), pm(f[1] * f[2]),
+               pm(f[0] * f[0]), pm(f[1] * f[1]), pm(f[2] * f[2]),
+               o.coef_to_gauss(C)[0, 3, :, 0].real]
+        got = ch.statistics(i, incre)
+        assert got.shape == (10, o.qpts)
+        for q in range(10):
+            scale = max(np.abs(ref[q]).max(), 1e-3 * np.abs(ref[6]).max())
+            assert np.abs(got[q] - ref[q]).max() <= 1e-11 * scale, q

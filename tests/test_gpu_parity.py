@@ -176,6 +176,10 @@ def test_ragged_grid_ddt_project_and_adjoint(cc):
         rg = cc.c_ddt_project(i, vg)
         assert abs(rg - ro) <= PROJ * max(1.0, abs(ro))
         assert rel(vg, vo) <= PROJ
+    # a tangent first: MIC nonzero, so the adjoint's forcing is active
+    d = direction(RAGGED, 29)
+    o.tangent(0, 2, d.copy(), 0)
+    cc.c_tangent(0, 2, d.copy(), 0)
     a = direction(RAGGED, 30)
     Ao, Ag = a.copy(), a.copy()
     o.adjoint(2, 0, Ao, 0, 1.0)
@@ -244,18 +248,43 @@ def test_fixed_length_transforms(cc):
     assert rel(Ag, Ao) <= SHORT
 
 
+def _forced_adjoint_step(cc, o, cfg):
+    """One tangent step (so that MIC[0], MIC[3] and the adjoint's forcing
+    are nonzero), ddt_project of a direction at steps 0 and 1, and one
+    forced adjoint step, device against oracle: the relative errors."""
+    d = direction(cfg, 3)
+    Io, Ig = d.copy(), d.copy()
+    o.tangent(0, 1, Io, 0)
+    cc.c_tangent(0, 1, Ig, 0)
+    errs = {"tangent": rel(Ig, Io)}
+    for i in (0, 1):
+        v = direction(cfg, 10 + i)
+        vo, vg = v.copy(), v.copy()
+        ro, rg = o.ddt_project(i, vo), cc.c_ddt_project(i, vg)
+        errs["ddt%d" % i] = max(rel(vg, vo), abs(rg - ro) / max(1.0, abs(ro)))
+    a = direction(cfg, 4)
+    Ao, Ag = a.copy(), a.copy()
+    o.adjoint(1, 0, Ao, 0, 1.0)
+    cc.c_adjoint(1, 0, Ag, 0, 1.0)
+    errs["adjoint"] = rel(Ag, Ao)
+    print(" ".join("%s %.2e" % kv for kv in errs.items()), flush=True)
+    return errs
+
+
 def test_benchmark_grid_one_step(cc):
-    print("case: benchmark grid 64x64x97, primal + adjoint 1 step", flush=True)
+    """64x64x97: one primal, tangent and forced adjoint step and ddt_project.
+    The FMA/plain floor of the oracle here is 9.7e-11 (step) and 1.8e-9
+    (forced adjoint; tests/test_oracle_rounding_floor.py)."""
+    print("case: benchmark grid 64x64x97, primal + tangent + adjoint 1 step", flush=True)
     o, C0 = start(cc, BENCH, 1)
     Co, Cg = C0.copy(), C0.copy()
     o.primal(0, Co)
     cc.c_primal(0, Cg)
     assert rel(Cg, Co) <= 3e-10
-    a = direction(BENCH, 4)
-    Ao, Ag = a.copy(), a.copy()
-    o.adjoint(1, 0, Ao, 0, 1.0)
-    cc.c_adjoint(1, 0, Ag, 0, 1.0)
-    assert rel(Ag, Ao) <= 5e-9
+    e = _forced_adjoint_step(cc, o, BENCH)
+    assert e["tangent"] <= 3e-10
+    assert e["ddt0"] <= PROJ and e["ddt1"] <= PROJ
+    assert e["adjoint"] <= 5e-9
 
 
 def test_duality_on_device(cc):
@@ -385,17 +414,15 @@ def test_largest_grid_one_step(cc):
     floor at this grid (the oracle with and without FMA contraction,
     tests/test_oracle_rounding_floor.py) is 9.8e-10 (step) and 1.75e-8
     (adjoint); the device lands at about 1.1x those, and the bounds allow 3x."""
-    print("case: largest grid 128x128x129, primal + adjoint 1 step", flush=True)
+    print("case: largest grid 128x128x129, primal + tangent + adjoint 1 step", flush=True)
     o, C0 = start(cc, G4, 1)
     Co, Cg = C0.copy(), C0.copy()
     o.primal(0, Co)
     cc.c_primal(0, Cg)
     ep = rel(Cg, Co)
-    a = direction(G4, 4)
-    Ao, Ag = a.copy(), a.copy()
-    o.adjoint(1, 0, Ao, 0, 1.0)
-    cc.c_adjoint(1, 0, Ag, 0, 1.0)
-    ea = rel(Ag, Ao)
-    print("largest grid: step %.2e adjoint %.2e" % (ep, ea), flush=True)
+    print("largest grid: step %.2e" % ep, flush=True)
+    e = _forced_adjoint_step(cc, o, G4)
     assert ep <= 3e-9
-    assert ea <= 6e-8
+    assert e["tangent"] <= 3e-9
+    assert e["ddt0"] <= 3e-9 and e["ddt1"] <= 3e-9
+    assert e["adjoint"] <= 6e-8

@@ -107,6 +107,9 @@ def test_rounding_floor_at_benchmark_grid():
             if part == "primal":
                 o.destroy()
                 return C
+            # a tangent first, so that the adjoint's forcing P(MIC[3n]) is active
+            o.tangent(0, 1, synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"],
+                                            c["flux"], seed=3, unit=True), 0)
             A = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=4,
                                 unit=True)
             o.adjoint(1, 0, A, 0, 1.0)
@@ -133,6 +136,9 @@ def test_rounding_floor_at_largest_grid():
                          c["dt"], 1)
         C = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=0)
         o.primal(0, C)
+        # a tangent first, so that the adjoint's forcing P(MIC[3n]) is active
+        o.tangent(0, 1, synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"],
+                                        seed=3, unit=True), 0)
         A = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=4,
                             unit=True)
         o.adjoint(1, 0, A, 0, 1.0)
