@@ -52,6 +52,65 @@ def fwd_adj_dev(cc, Cd, ACd, nsteps):
     cc.sc_adjoint_dev(nsteps, 0, ACd, 1.0)
 
 
+def unit_direction(shape, seed):
+    """A seeded tangent direction of the state's spectral shape (unit norm)."""
+    from channel.synthetic import synthetic_state
+    c = CFG
+    return synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=seed,
+                           unit=True)
+
+
+def fwd_adj_members(cc, Cd, ACd, nsteps):
+    """All members in every launch: their primal runs, then their forced
+    adjoints along their own trajectories (states resident in HBM)."""
+    cc.sc_primal_members(0, Cd)
+    cc.sc_adjoint_members(nsteps, 0, ACd, 1.0, False)
+
+
+def member_rate(cc, states, K, W):
+    """fwd+adj member-steps per second of len(states) independent members
+    advanced together: W untimed steps, then K timed."""
+    c = CFG
+    nm = len(states)
+    cc.sc_set_members(nm)
+    try:
+        if W > 0:
+            assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
+                             c["dt"], W) == 0
+            Cd = cc.DeviceArray(np.stack(states))
+            ACd = cc.DeviceArray(np.zeros((nm,) + states[0].shape, complex))
+            fwd_adj_members(cc, Cd, ACd, W)
+            Cd.free()
+            ACd.free()
+            cc.c_destroy()
+        assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
+                         c["dt"], K) == 0
+        # untimed: each member's trajectory and a tangent along it, so that
+        # the timed adjoints' forcing (their stored tangents) is not zero; the
+        # timed primal runs recompute the same trajectories
+        Cd = cc.DeviceArray(np.stack(states))
+        cc.sc_primal_members(0, Cd)
+        Cd.upload(np.stack(states))
+        Id = cc.DeviceArray(np.stack([unit_direction(s.shape, 100 + m)
+                                      for m, s in enumerate(states)]))
+        cc.sc_tangent_members(0, K, Id, 0, False)
+        Id.free()
+        ACd = cc.DeviceArray(np.zeros((nm,) + states[0].shape, complex))
+        cc.sc_synchronize()
+        t0 = time.perf_counter()
+        fwd_adj_members(cc, Cd, ACd, K)
+        cc.sc_synchronize()
+        elapsed = time.perf_counter() - t0
+        C, AC = Cd.download(), ACd.download()
+        Cd.free()
+        ACd.free()
+        assert cc.last_status() == 0 and np.all(np.isfinite(C)) and np.all(np.isfinite(AC))
+        cc.c_destroy()
+    finally:
+        cc.sc_set_members(1)
+    return nm * K / elapsed, elapsed
+
+
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs
 # (scaled by a copy of known size), per kernel class of the roofline replay
 FLUSH_MB = 320       # above the 256 MB Infinity Cache
@@ -121,6 +180,8 @@ def run():
     ap.add_argument("--roofline-class", type=int, default=6)
     ap.add_argument("--replays", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--members", type=int, default=8,
+                    help="independent members advanced together in the headline figure")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver runs one simulation per device and "
@@ -142,8 +203,14 @@ def run():
     assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
                      c["dt"], K) == 0
     log("timing %d fwd+adj steps" % K)
+    # untimed: the trajectory and a tangent along it (the adjoint's forcing);
+    # the timed primal recomputes the same trajectory
+    Cd = cc.DeviceArray(C0)
+    cc.sc_primal_dev(0, Cd)
+    Cd.upload(C0)
+    cc.c_tangent(0, K, unit_direction(C0.shape, 100), 0)
     # inputs resident in HBM before the timed region
-    Cd, ACd = cc.DeviceArray(C0), cc.DeviceArray(np.zeros_like(C0))
+    ACd = cc.DeviceArray(np.zeros_like(C0))
     cc.sc_synchronize()
     t0 = time.perf_counter()
     fwd_adj_dev(cc, Cd, ACd, K)
@@ -161,6 +228,14 @@ def run():
     elapsed_h = time.perf_counter() - t0
     assert np.array_equal(Ch, C) and np.array_equal(ACh, AC)
     cc.c_destroy()
+
+    # the headline: members advanced together (each its own seeded state)
+    nm = args.members
+    log("timing %d members x %d fwd+adj steps" % (nm, K))
+    states = [synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=s)
+              for s in range(nm)]
+    mrate, melapsed = member_rate(cc, states, K, W)
+    plain = K / elapsed
 
     # roofline of one kernel class: one fwd+adj step records its launches,
     # then the last of them is re-issued back to back between two events on
@@ -191,15 +266,19 @@ def run():
             "step_avg_launch_us": round(1e3 * ms / n, 2),
             "step_bytes_per_launch": round(nbytes / n)}
     line = {
-        "metric": METRIC, "value": round(K / elapsed, 3), "unit": "steps/s",
-        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(1e3 * elapsed / K, 3),
+        "metric": METRIC, "value": round(mrate, 3), "unit": "steps/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(1e3 * melapsed / K, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "fwd+adj step: 1 primal RK3/CN step + 1 adjoint step "
-                               "(strength 1), state resident in HBM",
+                               "(strength 1) per member, %d independent members advanced "
+                               "together in every kernel launch, states resident in HBM; "
+                               "value = member-steps/s" % nm,
                    "grid": "64x64x97 (96x145x96 dealiased)", "Lx": "2pi", "Lz": "pi",
                    "Re": c["Re"], "dt": c["dt"], "meanU": c["flux"] / 2,
-                   "window_steps": K, "precision": "fp64",
+                   "members": nm, "window_steps": K, "precision": "fp64",
+                   "plain_fp64_steps_per_s": round(plain, 3),
+                   "ratio_vs_plain_fp64": round(mrate / plain, 3),
                    "host_array_steps_per_s": round(K / elapsed_h, 3)},
         "roofline": roof,
     }

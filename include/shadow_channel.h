@@ -187,6 +187,50 @@ int sc_adjoint_dev(int start_step, int end_step, sc_complex *AC_dev, double stre
 int sc_set_field_cache(int enable);
 int sc_field_cache_active(void);
 
+/* Members (no counterpart in the reference: its drivers run one solution per
+   process, drivers/wrapper.py:14-30 loops over them).  sc_set_members(n)
+   makes the next c_init allocate n members (1 .. 32, default 1): n copies
+   of every state array and stored trajectory.  The c_* entry points above
+   act on member 0.  The *_members calls advance all n in every kernel
+   launch, their arrays on the device, member m at m Nz 2 dimR Nx/2 complex
+   values:
+     sc_primal_members: n independent trajectories from C_dev;
+     sc_tangent_members / sc_adjoint_members: with shared_base 0, member m
+       along its own trajectory; with shared_base 1, every member along
+       member 0's (the least-squares-shadowing segments' tangent directions);
+       bit m of inhomo_mask gives member m the mean-velocity source.
+   A CFL violation in any member stops all.  sc_member_state copies MC[3i]
+   (incre 0) or MIC[3i] (incre 1) of one member to host memory. */
+int sc_set_members(int n);
+int sc_members(void);
+int sc_primal_members(int ru_steps, sc_complex *C_dev);
+int sc_tangent_members(int start_step, int end_step, sc_complex *IC_dev, unsigned inhomo_mask,
+                       int shared_base);
+int sc_adjoint_members(int start_step, int end_step, sc_complex *AC_dev, double strength,
+                       int shared_base);
+int sc_member_state(int incre, int member, int i_step, sc_complex *out);
+
+/* Sensitivities (no counterpart in the reference; the building blocks of the
+   time-averaged wall-shear gradient):
+   sc_tangent_sources: c_tangent with the mean-velocity source (inhomo, as
+     c_tangent) and/or the Reynolds-number source (re_source): the
+     derivative of the stage operators in Re from the forcing of
+     channel/tangent_forcing.c and tangent_forcing0.c (declared but never
+     wired into the tangent by the reference, channel/channel.c:519-522), so
+     that a tangent from IC = 0 is dC/dRe.  IC: host array, in place.
+   sc_adjoint_objective: the adjoint of J = weight sum_{m=end}^{start-1}
+     tau(C_m), tau = sum_j R_j'(-1) Re alpha_j(0,0) the lower-wall shear
+     (channel/project0.c:227-234), from AC (host, in place): the exact
+     transpose of the tangent in the Gauss-weighted physical inner product
+     (drivers/testAdj.py:55-56), including the mean column's flux
+     correction, where the reference's c_adjoint is exact only to O(re dt).
+   sc_gradient_euclid: the gradient in the real and imaginary parts of the
+     coefficients, g, of d -> <lam, d>_phys (g = its derivative in Re d plus
+     i times that in Im d). */
+int sc_tangent_sources(int start_step, int end_step, sc_complex *IC, int inhomo, int re_source);
+int sc_adjoint_objective(int start_step, int end_step, sc_complex *AC, double weight);
+int sc_gradient_euclid(const sc_complex *lam, sc_complex *g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,11 @@ def load():
     L.orc_flux_trace.argtypes = [c.c_void_p, c.c_int]
     L.orc_flux_trace.restype = c.c_int
     L.orc_set_adjoint_refresh.argtypes = [c.c_int]
+    L.orc_tangent_src.argtypes = [c.c_int, c.c_int, c.c_void_p, c.c_int, c.c_int]
+    L.orc_tangent_src.restype = c.c_int
+    L.orc_adjoint_objective.argtypes = [c.c_int, c.c_int, c.c_void_p, c.c_double]
+    L.orc_adjoint_objective.restype = c.c_int
+    L.orc_set_exact_mean_adjoint.argtypes = [c.c_int]
     for f in ("orc_qpts", "orc_dimR", "orc_dimQ", "orc_Nx", "orc_Nz",
               "orc_nsteps", "orc_last_status"):
         getattr(L, f).restype = c.c_int
@@ -116,6 +121,12 @@ class Oracle:
 
     def adjoint(self, start, end, AC, inhomo, strength):
         return self.L.orc_adjoint(start, end, _ptr(_check_c(AC)), inhomo, strength)
+
+    def tangent_src(self, start, end, IC, inhomo, re_src):
+        return self.L.orc_tangent_src(start, end, _ptr(_check_c(IC)), inhomo, re_src)
+
+    def adjoint_objective(self, start, end, AC, weight):
+        return self.L.orc_adjoint_objective(start, end, _ptr(_check_c(AC)), weight)
 
     def ddt_project(self, i, v):
         return self.L.orc_ddt_project(i, _ptr(_check_c(v)))

@@ -829,6 +829,11 @@ static void forcing_modes(const cx *Ua, int z, int x0, int adj)
     }
 }
 
+/* Optional right-hand-side source of the current stage (C layout), added
+   after the explicit part and the forcing, before the solve (the
+   Reynolds-number source of orc_tangent_src); NULL: none. */
+static const cx *g_src = NULL;
+
 /* ------------------------------------------------------------------ */
 /* one Runge-Kutta stage for the modes of row z, x >= x0                */
 /* project (channel/project.c:24-405), increproject                     */
@@ -860,6 +865,10 @@ static void stage_modes(cx *Ca, const cx *Ua, const cx *LUa, cx *Uout,
         for (int x = x0; x < Mx; ++x) {
             Ca[ci(z, 0, i, x)].re += dt * c[k] * G.Fa[i * Mx + x].re;
             Ca[ci(z, 0, i, x)].im += dt * c[k] * G.Fa[i * Mx + x].im;
+            if (g_src) {
+                Ca[ci(z, 0, i, x)].re += g_src[ci(z, 0, i, x)].re;
+                Ca[ci(z, 0, i, x)].im += g_src[ci(z, 0, i, x)].im;
+            }
         }
     for (int x = x0; x < Mx; ++x) {
         op_alpha(M, G.K2[z * Mx + x], -(re * b[k] * dt));
@@ -879,6 +888,10 @@ static void stage_modes(cx *Ca, const cx *Ua, const cx *LUa, cx *Uout,
         for (int x = x0; x < Mx; ++x) {
             Ca[ci(z, 1, i, x)].re += dt * c[k] * G.Fb[i * Mx + x].re;
             Ca[ci(z, 1, i, x)].im += dt * c[k] * G.Fb[i * Mx + x].im;
+            if (g_src) {
+                Ca[ci(z, 1, i, x)].re += g_src[ci(z, 1, i, x)].re;
+                Ca[ci(z, 1, i, x)].im += g_src[ci(z, 1, i, x)].im;
+            }
         }
     for (int x = x0; x < Mx; ++x) {
         op_beta(M, G.K2[z * Mx + x], re * b[k] * dt);
@@ -960,6 +973,42 @@ int orc_flux_trace(double *buf, int cap_records)
     return n;
 }
 
+/* 1: the adjoint's mean mode is the exact transpose of the tangent's (the
+   objective adjoint's mode); 0: the reference's (channel/adjproject0.c) */
+static int g_exact_mean = 0;
+
+/* The exact transpose of the tangent's flux correction x <- P x,
+   P = I - add w1^T / 2 (add = Rs^{-1} w1), in the pairing a^T Rs b of the
+   mean column: P is self-adjoint there, and in the reversed stage order of
+   the adjoint it falls on the stage's right-hand side before its solve
+   (rhs <- rhs - w1 (add^T rhs) / 2, at adjoint stages 0 and 1) and on the
+   adjoint state at the start of every step, before the step's adjoint
+   velocities are formed (lambda <- P lambda: the driver's part); the real
+   parts only, as in the tangent. */
+static void mean_flux_transpose(cx *al, long s, int on_rhs)
+{
+    const int nR = G.dimR, q = G.qpts;
+    double M[5 * 1024], w1[1024];
+    cx add[1024];
+    memcpy(M, G.Rs, sizeof(double) * nR * 5);
+    for (int i = 0; i < nR; ++i) {
+        w1[i] = 0.0;
+        for (int j = 0; j < q; ++j) w1[i] += G.Rw[i * q + j];
+        add[i].re = w1[i];
+        add[i].im = 0.0;
+    }
+    band_factor(M, nR, 2);
+    band_subst(M, add, 1, nR, 2);
+    double t = 0.0;
+    if (on_rhs) {
+        for (int i = 0; i < nR; ++i) t += add[i].re * al[i * s].re;
+        for (int i = 0; i < nR; ++i) al[i * s].re -= 0.5 * t * w1[i];
+    } else {
+        for (int i = 0; i < nR; ++i) t += w1[i] * al[i * s].re;
+        for (int i = 0; i < nR; ++i) al[i * s].re -= 0.5 * t * add[i].re;
+    }
+}
+
 static void stage_mean(cx *Ca, const cx *Ua, const cx *LUa, cx *Uout,
                        cx *store, int k, int mode, double target, int wall)
 {
@@ -973,6 +1022,7 @@ static void stage_mean(cx *Ca, const cx *Ua, const cx *LUa, cx *Uout,
     cx tm[1024];
     cx *al = Ca + ci(0, 0, 0, 0), *be = Ca + ci(0, 1, 0, 0);
     long s = G.Mx;
+    const int exact = adj && g_exact_mean;
     if (k == 0) {
         op_mean(M, -(re * a[0] * dt));
         band_mult(M, al, s, tm, 1, nR, 2);
@@ -999,18 +1049,30 @@ static void stage_mean(cx *Ca, const cx *Ua, const cx *LUa, cx *Uout,
     op_mean(M, re * b[k] * dt);
     band_factor(M, nR, 2);
     for (int i = 0; i < nR; ++i) al[i * s].re += dt * c[k] * fa[i];
+    if (g_src)
+        for (int i = 0; i < nR; ++i) {
+            al[i * s].re += g_src[ci(0, 0, i, 0)].re;
+            al[i * s].im += g_src[ci(0, 0, i, 0)].im;
+        }
+    if (exact && k < 2) mean_flux_transpose(al, s, 1);
     band_subst(M, al, s, nR, 2);
     for (int i = 0; i < nR; ++i) be[i * s].re += dt * c[k] * fb[i];
+    if (g_src)
+        for (int i = 0; i < nR; ++i) {
+            be[i * s].re += g_src[ci(0, 1, i, 0)].re;
+            be[i * s].im += g_src[ci(0, 1, i, 0)].im;
+        }
     band_subst(M, be, s, nR, 2);
     double flux_t = 0;
     for (int i = 0; i < nR; ++i)
         for (int j = 0; j < q; ++j) flux_t += G.Rw[i * q + j] * al[i * s].re;
     flux_t = target - flux_t;
     const double err_before = flux_t;
-    for (int j = 0; j < nR; ++j) {
-        al[j * s].re = al[j * s].re + flux_t * add[j] * 0.5;
-        al[j * s].im = al[j * s].im + flux_t * 0.0 * 0.5;
-    }
+    if (!exact)
+        for (int j = 0; j < nR; ++j) {
+            al[j * s].re = al[j * s].re + flux_t * add[j] * 0.5;
+            al[j * s].im = al[j * s].im + flux_t * 0.0 * 0.5;
+        }
     if (g_ftrace && g_ftrace_n < g_ftrace_cap) {
         /* flux error target - sum_ij Rw_ij Re alpha_i after the correction */
         double after = 0;
@@ -1417,6 +1479,10 @@ int orc_adjoint(int start, int end, orc_cplx *ACg, int inhomo, double strength)
     coef_to_gauss(G.AC, G.AU, 0);
     strength *= G.dt;
     for (int n = start; n > end; --n) {
+        if (g_exact_mean) {
+            mean_flux_transpose(G.AC, G.Mx, 0);
+            coef_to_gauss(G.AC, G.AU, 0);
+        }
         memcpy(tmp, G.MIC + cs * (3L * n), sizeof(cx) * cs);
         orc_ddt_project(n, tmp);
         for (long e = 0; e < cs; ++e) {
@@ -1522,6 +1588,165 @@ const orc_cplx *orc_getincresoln(int i)
 {
     if (!G.ok || i < 0 || i > G.nsteps) return NULL;
     return G.MIC + csize() * 3L * i;
+}
+
+/* Reynolds-number source of the tangent at stage k of step n, per unit
+   change of Re: the implicit and explicit operators M -+ re b/a dt D depend
+   on re = 1/Re, so with TF(X) = (dre/dRe) D X = -re^2 D X (the forcing of
+   channel/tangent_forcing.c, tangent_forcing0.c) the stage's right-hand side
+   gains dt (a_k TF(Y_k) + b_k TF(X_k)), Y_k the state the stage's explicit
+   part acts on (MC[3n+k]: the step's start at k = 0, the previous stage's
+   solution after) and X_k its own solution (MC[3n+k+1]).  src: C layout. */
+static void re_source(int n, int k, cx *src)
+{
+    const int Nz = G.Nz, Mx = G.Mx, nQ = G.dimQ, nR = G.dimR;
+    const long na = (long) Nz * nQ * Mx, nb = (long) Nz * nR * Mx;
+    cx *fa = (cx *) calloc(na, sizeof(cx)), *fb = (cx *) calloc(nb, sizeof(cx));
+    cx *f0a = (cx *) calloc(nR, sizeof(cx)), *f0b = (cx *) calloc(nR, sizeof(cx));
+    memset(src, 0, sizeof(cx) * csize());
+    const double w[2] = {G.dt * FA[k], G.dt * FB[k]};
+    for (int t = 0; t < 2; ++t) {
+        orc_tangent_forcing(3 * n + k + t, (orc_cplx *) fa, (orc_cplx *) fb, (orc_cplx *) f0a,
+                            (orc_cplx *) f0b);
+        for (int z = 0; z < Nz; ++z)
+            for (int x = 0; x < Mx; ++x) {
+                if (z == 0 && x == 0) continue;
+                for (int i = 0; i < nQ; ++i) {
+                    src[ci(z, 0, i, x)].re += w[t] * fa[((long) z * nQ + i) * Mx + x].re;
+                    src[ci(z, 0, i, x)].im += w[t] * fa[((long) z * nQ + i) * Mx + x].im;
+                }
+                for (int i = 0; i < nR; ++i) {
+                    src[ci(z, 1, i, x)].re += w[t] * fb[((long) z * nR + i) * Mx + x].re;
+                    src[ci(z, 1, i, x)].im += w[t] * fb[((long) z * nR + i) * Mx + x].im;
+                }
+            }
+        if (t == 1) {
+            /* The implicit operator acts on the mean solve's own result, before
+               the flux correction alpha += e add / 2 (add = Rs^{-1} w1, w1_i =
+               sum_j Rw_ij) that the stored state has had: X_pre = X - e add / 2
+               with e = (flux - w1.X) / (1 - w1.add / 2) recovered from X.  TF is
+               linear, so TF(X_pre) = TF(X) - e/2 re^2 Rps add. */
+            const int q = G.qpts;
+            const cx *X = G.MC + (long) (3 * n + k + 1) * csize();
+            double M[5 * 1024], w1[1024], wx = 0.0, wa = 0.0;
+            cx add[1024], rps_add[1024];
+            memcpy(M, G.Rs, sizeof(double) * nR * 5);
+            for (int i = 0; i < nR; ++i) {
+                w1[i] = 0.0;
+                for (int j = 0; j < q; ++j) w1[i] += G.Rw[i * q + j];
+                add[i].re = w1[i];
+                add[i].im = 0.0;
+            }
+            band_factor(M, nR, 2);
+            band_subst(M, add, 1, nR, 2);
+            for (int i = 0; i < nR; ++i) {
+                wx += w1[i] * X[ci(0, 0, i, 0)].re;
+                wa += w1[i] * add[i].re;
+            }
+            const double e = (G.flux - wx) / (1.0 - 0.5 * wa);
+            band_mult(G.Rps, add, 1, rps_add, 1, nR, 2);
+            for (int i = 0; i < nR; ++i) f0a[i].re -= 0.5 * e * G.re * G.re * rps_add[i].re;
+        }
+        for (int i = 0; i < nR; ++i) {
+            src[ci(0, 0, i, 0)].re += w[t] * f0a[i].re;
+            src[ci(0, 0, i, 0)].im += w[t] * f0a[i].im;
+            src[ci(0, 1, i, 0)].re += w[t] * f0b[i].re;
+            src[ci(0, 1, i, 0)].im += w[t] * f0b[i].im;
+        }
+    }
+    free(fa); free(fb); free(f0a); free(f0b);
+}
+
+int orc_tangent_src(int start, int end, orc_cplx *ICg, int inhomo, int re_src)
+{
+    if (!G.ok) return -2;
+    if (!(end > start) || start < 0 || end > G.nsteps || !ICg) return -3;
+    long cs = csize();
+    cx *src = re_src ? (cx *) malloc(sizeof(cx) * cs) : NULL;
+    memcpy(G.C, G.MC + cs * (3L * start), sizeof(cx) * cs);
+    memcpy(G.IC, ICg, sizeof(cx) * cs);
+    coef_to_gauss(G.C, G.U, 1);
+    coef_to_gauss(G.IC, G.IU, 0);
+    memcpy(G.MIC + cs * (3L * start), G.IC, sizeof(cx) * cs);
+    G.status = 0;
+    for (int n = start; n < end; ++n) {
+        for (int k = 0; k < 3; ++k) {
+            if (pass1(G.U, G.IU, k == 0) != 0) {
+                printf("Pass1 failure\n");
+                G.status = -1;
+                goto done;
+            }
+            int count = n * 3 + k + 1;
+            if (src) re_source(n, k, src);
+            g_src = src;
+            stage_all(G.IC, G.IU, NULL, G.IU, G.MIC + cs * count, k, 1, inhomo ? 2.0 : 0.0);
+            g_src = NULL;
+            stage_all(G.C, G.U, NULL, G.U, NULL, k, 0, G.flux);
+        }
+    }
+done:
+    g_src = NULL;
+    memcpy(ICg, G.IC, sizeof(cx) * cs);
+    free(src);
+    return 0;
+}
+
+/* The adjoint (no trapezoid forcing) of the objective
+     J = weight sum_{m = end}^{start - 1} tau(C_m),  tau(C) = sum_j R_j'(-1) Re alpha_j(0,0)
+   (the lower-wall shear of the mean profile, channel/project0.c:227-234):
+   after each step n -> n - 1, AC += weight G with G the representer of tau in
+   the Gauss-weighted physical inner product <a, b> of drivers/testAdj.py:55-56:
+   only its mean streamwise column is nonzero, alpha = Rs^{-1} Rp0 / (Px Pz)
+   (the plane sums of a mean column are Px Pz times its value, and the Gauss
+   rule makes <a, b> of two mean columns Px Pz alpha_a^T Rs alpha_b).  The
+   adjoint velocities are rebuilt after each addition.  On return,
+   <AC, d>_phys = <AC_given, T d> + weight sum_m tau(T_m d) for a tangent d
+   from step end. */
+void orc_set_exact_mean_adjoint(int on) { g_exact_mean = on; }
+
+int orc_adjoint_objective(int start, int end, orc_cplx *ACg, double weight)
+{
+    if (!G.ok) return -2;
+    if (!(end < start) || end < 0 || start > G.nsteps || !ACg) return -3;
+    const long cs = csize(), us = usize();
+    const int nR = G.dimR;
+    double gv[1024], M[5 * 1024];
+    {
+        cx tmp[1024];
+        memcpy(M, G.Rs, sizeof(double) * nR * 5);
+        for (int i = 0; i < nR; ++i) {
+            tmp[i].re = G.Rp0[i] / ((double) G.Px * G.Pz);
+            tmp[i].im = 0.0;
+        }
+        band_factor(M, nR, 2);
+        band_subst(M, tmp, 1, nR, 2);
+        for (int i = 0; i < nR; ++i) gv[i] = tmp[i].re;
+    }
+    memcpy(G.C, G.MC + cs * (3L * start), sizeof(cx) * cs);
+    memcpy(G.AC, ACg, sizeof(cx) * cs);
+    const int was_exact = g_exact_mean;
+    g_exact_mean = 1;
+    for (int n = start; n > end; --n) {
+        mean_flux_transpose(G.AC, G.Mx, 0);
+        coef_to_gauss(G.AC, G.AU, 0);
+        for (int k = 0; k < 3; ++k) {
+            int count = n * 3 - k - 1;
+            if (k < 2) {
+                memcpy(G.C, G.MC + cs * (count - 1), sizeof(cx) * cs);
+                coef_to_gauss(G.C, G.U, 1);
+                pass2(G.U, G.AU, 0);
+                memcpy(G.LU, G.U, sizeof(cx) * us);
+            }
+            memcpy(G.C, G.MC + cs * count, sizeof(cx) * cs);
+            coef_to_gauss(G.C, G.U, 1);
+            pass2(G.U, G.AU, 0);
+            stage_all(G.AC, G.U, G.LU, G.AU, NULL, k, 2, 0.0);
+        }
+        for (int i = 0; i < nR; ++i) G.AC[ci(0, 0, i, 0)].re += weight * gv[i];
+    }
+    g_exact_mean = was_exact;
+    memcpy(ACg, G.AC, sizeof(cx) * cs);
+    return 0;
 }
 
 void orc_quad(double *y, double *w)

@@ -41,6 +41,20 @@ double orc_ddt_project(int i_step, orc_cplx *IC);
    channel/tangent_forcing0.c): f_a [Nz][dimQ][Nx/2], f_b [Nz][dimR][Nx/2],
    f0_a, f0_b [dimR]. */
 int orc_tangent_forcing(int count, orc_cplx *fa, orc_cplx *fb, orc_cplx *f0a, orc_cplx *f0b);
+/* Tangent with optional sources: inhomo 1 the mean-velocity (flux) source as
+   orc_tangent; re_src 1 adds the Reynolds-number source (the derivative of
+   the stage operators in Re, from the forcing of tangent_forcing at the
+   stage's stored states), so that the tangent from IC = 0 is dC/dRe. */
+int orc_tangent_src(int start_step, int end_step, orc_cplx *IC, int inhomo, int re_src);
+/* Adjoint of the time-averaged lower-wall shear objective: no trapezoid
+   forcing; after each step n -> n-1, AC += weight G(tau), G the representer
+   of tau(C) = sum_j R_j'(-1) Re alpha_j(0,0) in the physical inner product. */
+int orc_adjoint_objective(int start_step, int end_step, orc_cplx *AC, double weight);
+/* 1: orc_adjoint's mean column is the exact transpose of the tangent's (the
+   flux correction as in orc_adjoint_objective); 0 (default): the
+   reference's (channel/adjproject0.c:82-95, 153-167: corrected after each
+   solve through the left-hand-side matrix), exact only to O(re dt). */
+void   orc_set_exact_mean_adjoint(int on);
 void   orc_spec2phys(const orc_cplx *C, double *flow);
 const orc_cplx *orc_getsoln(int i_step);
 const orc_cplx *orc_getincresoln(int i_step);

@@ -87,6 +87,18 @@ _L.sc_primal_dev.restype = _i
 _L.sc_adjoint_dev.argtypes = [_i, _i, _c.c_void_p, _d]
 _L.sc_adjoint_dev.restype = _i
 
+_L.sc_set_members.argtypes = [_i]
+_L.sc_set_members.restype = _i
+_L.sc_members.argtypes = []
+_L.sc_members.restype = _i
+_L.sc_primal_members.argtypes = [_i, _vp]
+_L.sc_primal_members.restype = _i
+_L.sc_tangent_members.argtypes = [_i, _i, _vp, _c.c_uint, _i]
+_L.sc_tangent_members.restype = _i
+_L.sc_adjoint_members.argtypes = [_i, _i, _vp, _d, _i]
+_L.sc_adjoint_members.restype = _i
+_L.sc_member_state.argtypes = [_i, _i, _i, _vp]
+_L.sc_member_state.restype = _i
 _L.sc_mean_profile.argtypes = [_i, _i, _i, _d, _vp, _vp]
 _L.sc_mean_profile.restype = _i
 _L.sc_step_statistics.argtypes = [_i, _i, _vp]
@@ -398,3 +410,40 @@ def sc_step_statistics(incre, i_step):
     st = np.zeros((10, c_qpts()))
     _check(_L.sc_step_statistics(int(incre), int(i_step), st.ctypes.data), "sc_step_statistics")
     return st
+
+
+def sc_set_members(n):
+    """Members the next c_init allocates (1 .. 32)."""
+    _check(_L.sc_set_members(int(n)), "sc_set_members")
+
+
+def sc_members():
+    return _L.sc_members()
+
+
+def sc_primal_members(ru_steps, C_dev):
+    """Primal runs of all members from their states in C_dev (a DeviceArray
+    of shape (members, Nz, 2, dimR, Nx/2)), overwritten with the final states."""
+    _check(_L.sc_primal_members(int(ru_steps), _devptr(C_dev)), "sc_primal_members")
+
+
+def sc_tangent_members(start_step, end_step, IC_dev, inhomo_mask=0, shared_base=False):
+    """Tangents of all members in one pass of every kernel; bit m of
+    inhomo_mask adds the mean-velocity source to member m; shared_base: all
+    along member 0's trajectory."""
+    _check(_L.sc_tangent_members(int(start_step), int(end_step), _devptr(IC_dev),
+                                 int(inhomo_mask), int(bool(shared_base))), "sc_tangent_members")
+
+
+def sc_adjoint_members(start_step, end_step, AC_dev, strength, shared_base=False):
+    """Adjoints of all members (forced by their own stored tangents)."""
+    _check(_L.sc_adjoint_members(int(start_step), int(end_step), _devptr(AC_dev),
+                                 float(strength), int(bool(shared_base))), "sc_adjoint_members")
+
+
+def sc_member_state(incre, member, i_step):
+    """MC[3 i_step] (incre 0) or MIC[3 i_step] (incre 1) of one member."""
+    out = np.zeros((_L.c_Nz(), 2, _L.c_dimR(), _L.c_Nx() // 2), complex)
+    _check(_L.sc_member_state(int(incre), int(member), int(i_step), out.ctypes.data),
+           "sc_member_state")
+    return out

@@ -36,6 +36,32 @@ std::string &last_error()
     return e;
 }
 
+size_t mstride_bytes(const Context &c, const void *p)
+{
+    if (c.nrun <= 1 || !p) return 0;
+    const char *q = (const char *) p;
+    for (const MemberArray &m : c.marrays)
+        if (q >= m.lo && q < m.lo + m.per * (size_t) c.nmem)
+            return (m.trajectory && c.shared_base) ? 0 : m.per;
+    return 0;
+}
+
+// The members a run advances: nrun members, sharing member 0's trajectory
+// or not; restores the single-member setting when the run ends.
+struct MemberRun {
+    Context &c;
+    MemberRun(Context &cc, int n, bool shared) : c(cc)
+    {
+        c.nrun = n;
+        c.shared_base = shared;
+    }
+    ~MemberRun()
+    {
+        c.nrun = 1;
+        c.shared_base = false;
+    }
+};
+
 // ---------------------------------------------------------------- timing
 void prof_begin(Context &c, hipStream_t s, int id, double bytes)
 {
@@ -66,6 +92,8 @@ void prof_end(Context &c, hipStream_t s, int id, const std::function<void()> &re
 
 // whether c_init allocates the transformed-field cache (sc_set_field_cache)
 static int g_field_cache = 1;
+// members c_init allocates (sc_set_members)
+static int g_members = 1;
 
 // ---------------------------------------------------------------- rand()
 // glibc random() TYPE_3: r_i = r_{i-3} + r_{i-31} (mod 2^32) after the
@@ -110,7 +138,7 @@ static void free_all(Context &c)
                     c.MIC, c.Csol, c.Ctmp, c.Ctmp2, c.Fa, c.Fb, c.FLa, c.FLb, c.W1, c.W2, c.W2b,
                     c.SP, c.flowdev, c.cflmax, c.dotpart, c.scal, c.abort_flag, c.pz.tw,
                     c.px.tw, c.dFragR, c.dFragF, c.dFragP, c.dFragQ, c.wcache, c.rb_queue,
-                    c.objscr, c.dRwall};
+                    c.objscr, c.dRwall, c.dGwall};
     for (void *p : ptrs)
         if (p) (void) hipFree(p);
     for (int a = 0; a < 2; ++a)
@@ -538,33 +566,49 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
     const size_t cb = sizeof(cplx) * c.csize, ub = sizeof(cplx) * c.usize;
     const size_t fb = sizeof(cplx) * (size_t) d.Nz * d.dimR * d.Mx;
     const size_t mcb = cb * (3 * (size_t) nstep + 1);
+    const int nm = g_members;
+    c.nmem = nm;
+    c.mcbytes = mcb;
     // ddt_project's partial sums: one pair per block of its inner-product
-    // kernel, at most one block per (kz, kx) column
-    const size_t ndot = 2 * (size_t) d.Nz * d.Mx;
+    // kernel, at most one block per (kz, kx) column, per member
+    const size_t ndot = 2 * (size_t) d.Nz * d.Mx * nm;
     c.ndotpart = ndot;
-    struct { void **p; size_t n; } al[] = {
-        {(void **) &c.C, cb}, {(void **) &c.IC, cb}, {(void **) &c.AC, cb},
-        {(void **) &c.Csol, cb}, {(void **) &c.Ctmp, cb}, {(void **) &c.Ctmp2, cb},
-        {(void **) &c.U, ub}, {(void **) &c.IU, ub}, {(void **) &c.AU, ub},
-        {(void **) &c.LU, ub}, {(void **) &c.Ut, ub}, {(void **) &c.Ut2, ub},
-        {(void **) &c.MC, mcb}, {(void **) &c.MIC, mcb},
-        {(void **) &c.Fa, fb}, {(void **) &c.Fb, fb}, {(void **) &c.FLa, fb}, {(void **) &c.FLb, fb},
-        {(void **) &c.W1, sizeof(cplx) * (size_t) d.qpts * MAXF * d.Pz * d.Mx},
-        {(void **) &c.W2, sizeof(cplx) * (size_t) d.qpts * 6 * d.Pz * d.Mx},
-        {(void **) &c.W2b, sizeof(cplx) * (size_t) d.qpts * 6 * d.Pz * d.Mx},
-        {(void **) &c.SP, sizeof(cplx) * 6 * (size_t) d.Nz * d.qpts * d.Mx},
-        {(void **) &c.flowdev, sizeof(double) * 3 * (size_t) d.Px * d.qpts * d.Pz},
-        {(void **) &c.cflmax, sizeof(double) * (d.qpts + 1)},
-        {(void **) &c.dotpart, sizeof(double) * ndot},
-        {(void **) &c.scal, sizeof(double) * 8},
-        {(void **) &c.abort_flag, 4 * sizeof(int)},
-        {(void **) &c.objscr, sizeof(double) * objective_scratch_doubles(d)},
+    // per-member arrays (nm copies each; trajectory: the base trajectory)
+    struct { void **p; size_t n; bool member, trajectory; } al[] = {
+        {(void **) &c.C, cb, true, false}, {(void **) &c.IC, cb, true, false},
+        {(void **) &c.AC, cb, true, false}, {(void **) &c.Csol, cb, true, false},
+        {(void **) &c.Ctmp, cb, true, false}, {(void **) &c.Ctmp2, cb, true, false},
+        {(void **) &c.U, ub, true, false}, {(void **) &c.IU, ub, true, false},
+        {(void **) &c.AU, ub, true, false}, {(void **) &c.LU, ub, true, false},
+        {(void **) &c.Ut, ub, true, false}, {(void **) &c.Ut2, ub, true, false},
+        {(void **) &c.MC, mcb, true, true}, {(void **) &c.MIC, mcb, true, false},
+        {(void **) &c.Fa, fb, true, false}, {(void **) &c.Fb, fb, true, false},
+        {(void **) &c.FLa, fb, true, false}, {(void **) &c.FLb, fb, true, false},
+        {(void **) &c.W1, sizeof(cplx) * (size_t) d.qpts * MAXF * d.Pz * d.Mx, true, false},
+        {(void **) &c.W2, sizeof(cplx) * (size_t) d.qpts * 6 * d.Pz * d.Mx, true, false},
+        {(void **) &c.W2b, sizeof(cplx) * (size_t) d.qpts * 6 * d.Pz * d.Mx, true, false},
+        {(void **) &c.SP, sizeof(cplx) * 6 * (size_t) d.Nz * d.qpts * d.Mx, true, false},
+        {(void **) &c.flowdev, sizeof(double) * 3 * (size_t) d.Px * d.qpts * d.Pz, false, false},
+        {(void **) &c.cflmax, sizeof(double) * (d.qpts + 1), false, false},
+        {(void **) &c.dotpart, sizeof(double) * ndot, false, false},
+        {(void **) &c.scal, sizeof(double) * 8 * nm, false, false},
+        {(void **) &c.abort_flag, 4 * sizeof(int), false, false},
+        {(void **) &c.objscr, sizeof(double) * objective_scratch_doubles(d), false, false},
     };
+    c.marrays.clear();
     for (auto &e : al) {
-        if (hipMalloc(e.p, e.n) != hipSuccess || hipMemset(*e.p, 0, e.n) != hipSuccess) {
+        const size_t n = e.member ? e.n * nm : e.n;
+        if (hipMalloc(e.p, n) != hipSuccess || hipMemset(*e.p, 0, n) != hipSuccess) {
             last_error() = "device allocation failed";
             free_all(c);
             return SC_EINIT;
+        }
+        if (e.member) {
+            MemberArray m;
+            m.lo = (const char *) *e.p;
+            m.per = e.n;
+            m.trajectory = e.trajectory;
+            c.marrays.push_back(m);
         }
     }
     {
@@ -583,17 +627,37 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
             return SC_EINIT;
         }
     }
+    {
+        // representer of the lower-wall shear in the physical inner product:
+        // alpha(0,0) = Rs^{-1} Rp0 / (Px Pz)
+        std::vector<double> M(c.Rs), g(c.Rp0);
+        band_factor(M.data(), d.dimR, 2);
+        for (double &v : g) v /= (double) d.Px * d.Pz;
+        band_subst_real(M.data(), g.data(), d.dimR, 2);
+        if (hipMalloc(&c.dGwall, sizeof(double) * d.dimR) != hipSuccess ||
+            hipMemcpy(c.dGwall, g.data(), sizeof(double) * d.dimR, hipMemcpyHostToDevice) !=
+                hipSuccess) {
+            last_error() = "device allocation failed";
+            free_all(c);
+            return SC_EINIT;
+        }
+    }
     // transformed-field cache of the stored trajectory, if it fits in half of
     // the free memory (otherwise the tangent and adjoint recompute)
     {
         const size_t per = (size_t) d.qpts * 6 * d.Pz * d.Mx;
-        const size_t need = sizeof(cplx) * per * 3 * (size_t) nstep;
+        const size_t need = sizeof(cplx) * per * 3 * (size_t) nstep;   // one member
         size_t freeb = 0, totalb = 0;
         if (g_field_cache && need > 0 && hipMemGetInfo(&freeb, &totalb) == hipSuccess &&
-            need <= freeb / 2 &&
-            hipMalloc(&c.wcache, need) == hipSuccess) {
+            need * nm <= freeb / 2 &&
+            hipMalloc(&c.wcache, need * nm) == hipSuccess) {
             c.wslot = per;
             c.wvalid.assign(3 * (size_t) nstep, 0);
+            MemberArray m;
+            m.lo = (const char *) c.wcache;
+            m.per = need;
+            m.trajectory = true;
+            c.marrays.push_back(m);
         } else {
             c.wcache = nullptr;
             c.wvalid.clear();
@@ -607,16 +671,32 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
 
 }   // extern "C"
 
+// slot `slot` of a trajectory array (MC or MIC) of every run member <- the
+// member copies of a coefficient array src (one csize apart)
+static int copy_to_slots(Context &c, cplx *traj, long slot, const cplx *src)
+{
+    const size_t cb = sizeof(cplx) * c.csize;
+    cplx *dst = traj + c.csize * slot;
+    if (c.nrun == 1) {
+        SC_HIP(hipMemcpyAsync(dst, src, cb, hipMemcpyDeviceToDevice, c.stream));
+    } else {
+        SC_HIP(hipMemcpy2DAsync(dst, c.mcbytes, src, cb, cb, c.nrun, hipMemcpyDeviceToDevice,
+                                c.stream));
+    }
+    return SC_OK;
+}
+
 // primal run from a given state (host or device memory, per `in`/`out`) or
-// from the laminar start (C_given null)
+// from the laminar start (C_given null); c.nrun members, C_given holding
+// their states one after the other
 static int primal_run(int ru_steps, sc_complex *C_given, hipMemcpyKind in, hipMemcpyKind out)
 {
     Context &c = ctx();
     const Dims &d = c.d;
     const bool given = C_given != nullptr;
-    const size_t cb = sizeof(cplx) * c.csize;
+    const size_t cb = sizeof(cplx) * c.csize, cbm = cb * c.nrun;
     if (given) {
-        SC_HIP(hipMemcpyAsync(c.C, C_given, cb, in, c.stream));
+        SC_HIP(hipMemcpyAsync(c.C, C_given, cbm, in, c.stream));
         launch_rebuild(c, c.C, c.U);
     } else {
         // laminar Poiseuille + rand() perturbation at Gauss points; C by
@@ -638,7 +718,7 @@ static int primal_run(int ru_steps, sc_complex *C_given, hipMemcpyKind in, hipMe
     // Every stored state is written by a completed run; after a CFL stop the
     // states from the stopped step on are cleared (below), so the trajectory
     // reads as the reference's zeroed array (channel/channel.c:426-432).
-    SC_HIP(hipMemcpyAsync(c.MC, c.C, cb, hipMemcpyDeviceToDevice, c.stream));
+    if (int rc = copy_to_slots(c, c.MC, 0, c.C)) return rc;
     SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
     // each CFL check clears the maxima it read; a run starts from zero
     SC_HIP(hipMemsetAsync(c.cflmax, 0, sizeof(double) * c.d.qpts, c.stream));
@@ -677,9 +757,11 @@ static int primal_run(int ru_steps, sc_complex *C_given, hipMemcpyKind in, hipMe
         SC_HIP(hipMemcpy(&slot, c.abort_flag + 1, sizeof(int), hipMemcpyDeviceToHost));
         const long last = 3L * d.nsteps;
         if (slot >= 1 && slot <= last)
-            SC_HIP(hipMemset(mc(c, slot), 0, cb * (size_t) (last - slot + 1)));
+            for (int m = 0; m < c.nrun; ++m)
+                SC_HIP(hipMemset((char *) mc(c, slot) + m * c.mcbytes, 0,
+                                 cb * (size_t) (last - slot + 1)));
     }
-    if (given) SC_HIP(hipMemcpy(C_given, c.C, cb, out));
+    if (given) SC_HIP(hipMemcpy(C_given, c.C, cbm, out));
     return SC_OK;
 }
 
@@ -703,26 +785,30 @@ int sc_primal_dev(int ru_steps, sc_complex *C_dev)
     return primal_run(ru_steps, C_dev, hipMemcpyDeviceToDevice, hipMemcpyDeviceToDevice);
 }
 
-int c_tangent(int start, int end, sc_complex *IC_given, int Nz_dup, int Nvar_dup, int Ny_dup,
-              int Nx_dup, int inhomo)
+static int tangent_range_ok(const Context &c, int start, int end)
+{
+    if (!(end > start) || start < 0 || end > c.d.nsteps) {
+        last_error() = "tangent requires 0 <= start_step < end_step <= nsteps";
+        return 0;
+    }
+    return 1;
+}
+
+// tangent run of c.nrun members (IC_given: their states one after the
+// other, host or device memory per in / out); member m has the mean-velocity
+// source when bit m of inhomo_mask is set
+static int tangent_run(int start, int end, sc_complex *IC_given, unsigned inhomo_mask,
+                       hipMemcpyKind in, hipMemcpyKind out, int re_source = 0)
 {
     Context &c = ctx();
-    if (!c.ok) return SC_ENOTREADY;
-    const Dims &d = c.d;
-    if (!IC_given || !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup)) return SC_EINVAL;
-    if (!(end > start) || start < 0 || end > d.nsteps) {
-        last_error() = "tangent requires 0 <= start_step < end_step <= nsteps";
-        return SC_EINVAL;
-    }
-    const size_t cb = sizeof(cplx) * c.csize;
-    SC_HIP(hipMemcpyAsync(c.IC, IC_given, cb, hipMemcpyHostToDevice, c.stream));
-    SC_HIP(hipMemcpyAsync(mic(c, 3L * start), c.IC, cb, hipMemcpyDeviceToDevice, c.stream));
+    const size_t cbm = sizeof(cplx) * c.csize * c.nrun;
+    SC_HIP(hipMemcpyAsync(c.IC, IC_given, cbm, in, c.stream));
+    if (int rc = copy_to_slots(c, c.MIC, 3L * start, c.IC)) return rc;
     SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
     // each CFL check clears the maxima it read; a run starts from zero
     SC_HIP(hipMemsetAsync(c.cflmax, 0, sizeof(double) * c.d.qpts, c.stream));
     launch_rebuild(c, c.IC, c.IU);
     c.status = 0;
-    const double target = inhomo ? 2.0 : 0.0;
     for (int n = start; n < end; ++n) {
         for (int k = 0; k < 3; ++k) {
             const cplx *w = wcached(c, 3L * n + k);
@@ -731,7 +817,14 @@ int c_tangent(int start, int end, sc_complex *IC_given, int Nz_dup, int Nvar_dup
             long count = 3L * n + k + 1;
             cplx *Cs = mic(c, count);
             launch_force(c, c.IU, 0);
-            launch_solve(c, c.IC, Cs, k, 0, c.Fa, c.Fb, target);
+            // the Reynolds-number source: forward RK3/CN coefficients a, b of
+            // the stage (channel/project.c:44-47)
+            static const double RA[3] = {1. / 3., -1. / 2, 1. / 3.}, RB[3] = {1. / 6, 2. / 3, 0.};
+            if (re_source)
+                launch_re_source(c, mc(c, 3L * n + k), mc(c, 3L * n + k + 1), c.d.dt * RA[k],
+                                 c.d.dt * RB[k], c.Ctmp2);
+            launch_solve(c, c.IC, Cs, k, 0, c.Fa, c.Fb, 0.0, 2.0, inhomo_mask,
+                         re_source ? c.Ctmp2 : nullptr);
             launch_rebuild(c, Cs, c.IU);
         }
         if (((n - start) & 63) == 63) {
@@ -744,8 +837,28 @@ int c_tangent(int start, int end, sc_complex *IC_given, int Nz_dup, int Nvar_dup
     if (!rc) rc = check_kernel_error(c);
     if (rc) return rc;
     check_abort(c, "Pass1");
-    SC_HIP(hipMemcpy(IC_given, c.IC, cb, hipMemcpyDeviceToHost));
+    SC_HIP(hipMemcpy(IC_given, c.IC, cbm, out));
     return SC_OK;
+}
+
+extern "C" int sc_tangent_sources(int start, int end, sc_complex *IC, int inhomo, int re_source)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!IC || !tangent_range_ok(c, start, end)) return SC_EINVAL;
+    return tangent_run(start, end, IC, inhomo ? 1u : 0u, hipMemcpyHostToDevice,
+                       hipMemcpyDeviceToHost, re_source != 0);
+}
+
+extern "C" int c_tangent(int start, int end, sc_complex *IC_given, int Nz_dup, int Nvar_dup,
+                         int Ny_dup, int Nx_dup, int inhomo)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!IC_given || !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup)) return SC_EINVAL;
+    if (!tangent_range_ok(c, start, end)) return SC_EINVAL;
+    return tangent_run(start, end, IC_given, inhomo ? 1u : 0u, hipMemcpyHostToDevice,
+                       hipMemcpyDeviceToHost);
 }
 
 }   // extern "C"
@@ -760,21 +873,10 @@ static int adjoint_range_ok(const Context &c, int start, int end)
 }
 
 // adjoint run; AC_given in host or device memory per `in`/`out`
-static int adjoint_run(int start, int end, sc_complex *AC_given, double strength, hipMemcpyKind in,
-                       hipMemcpyKind out)
+// the three stages of adjoint step n (channel/channel.c:633-679) on AC, AU;
+// exact_mean: the mean column as the exact transpose of the tangent's
+static void adjoint_stages(Context &c, int n, int exact_mean)
 {
-    Context &c = ctx();
-    const Dims &d = c.d;
-    const size_t cb = sizeof(cplx) * c.csize;
-    SC_HIP(hipMemcpyAsync(c.AC, AC_given, cb, in, c.stream));
-    SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
-    launch_rebuild(c, c.AC, c.AU, true);      // lambda: velocity slots only
-    const double s = strength * d.dt;
-    for (int n = start; n > end; --n) {
-        // The projection of MIC[3n] at step n was made at the end of the
-        // previous iteration (the reference computes it twice), which also
-        // applied it for this step's start.
-        if (n == start) ddt_project_dev(c, n, mic(c, 3L * n), c.Ctmp, c.AC, 0.5 * s);
         for (int k = 0; k < 3; ++k) {
             long count = 3L * n - k - 1;
             // with both states cached, lambda is transformed once for the pair
@@ -800,21 +902,76 @@ static int adjoint_run(int start, int end, sc_complex *AC_given, double strength
                 launch_force2(c, 1, c.U, c.Fa, c.Fb, c.LU, c.FLa, c.FLb);
             else
                 launch_force(c, c.U, 1);
-            launch_solve(c, c.AC, c.Csol, k, 1, c.FLa, c.FLb, 0.0);
+            launch_solve(c, c.AC, c.Csol, k, 1, c.FLa, c.FLb, 0.0, 0.0, 0, nullptr, exact_mean);
             launch_rebuild(c, c.Csol, c.AU, true);
         }
+}
+
+static int adjoint_run(int start, int end, sc_complex *AC_given, double strength, hipMemcpyKind in,
+                       hipMemcpyKind out)
+{
+    Context &c = ctx();
+    const Dims &d = c.d;
+    const size_t cbm = sizeof(cplx) * c.csize * c.nrun;
+    SC_HIP(hipMemcpyAsync(c.AC, AC_given, cbm, in, c.stream));
+    SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
+    launch_rebuild(c, c.AC, c.AU, true);      // lambda: velocity slots only
+    const double s = strength * d.dt;
+    for (int n = start; n > end; --n) {
+        // The projection of MIC[3n] at step n was made at the end of the
+        // previous iteration (the reference computes it twice), which also
+        // applied it for this step's start.
+        if (n == start && ddt_project_dev(c, n, mic(c, 3L * n), c.Ctmp, c.AC, 0.5 * s))
+            return SC_EDEVICE;
+        adjoint_stages(c, n, 0);
         // forcing of MIC[3(n-1)]: end of this step and, when there is a next
         // step, its start (the two subtractions back to back)
-        ddt_project_dev(c, n - 1, mic(c, 3L * (n - 1)), c.Ctmp, c.AC, 0.5 * s, n - 1 > end);
+        if (ddt_project_dev(c, n - 1, mic(c, 3L * (n - 1)), c.Ctmp, c.AC, 0.5 * s, n - 1 > end))
+            return SC_EDEVICE;
     }
     int rc = stream_sync(c);
     if (!rc) rc = check_kernel_error(c);
     if (rc) return rc;
-    SC_HIP(hipMemcpy(AC_given, c.AC, cb, out));
+    SC_HIP(hipMemcpy(AC_given, c.AC, cbm, out));
+    return SC_OK;
+}
+
+// The adjoint of J = weight sum_{m = end}^{start-1} tau(C_m), tau the lower
+// wall shear (sum_j R_j'(-1) Re alpha_j(0,0), channel/project0.c:227-234):
+// no trapezoid forcing; the mean column as the exact transpose of the
+// tangent's (its flux correction P at each step's start, before the step's
+// adjoint velocities); after each step the representer of tau in the
+// physical inner product, weight G (alpha(0,0) = Rs^{-1} Rp0 / (Px Pz)).
+static int adjoint_objective_run(int start, int end, sc_complex *AC_given, double weight,
+                                 hipMemcpyKind in, hipMemcpyKind out)
+{
+    Context &c = ctx();
+    const size_t cbm = sizeof(cplx) * c.csize * c.nrun;
+    SC_HIP(hipMemcpyAsync(c.AC, AC_given, cbm, in, c.stream));
+    SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
+    for (int n = start; n > end; --n) {
+        launch_mean_project(c, c.AC);
+        launch_rebuild(c, c.AC, c.AU, true);
+        adjoint_stages(c, n, 1);
+        launch_add_mean(c, c.AC, c.dGwall, weight);
+    }
+    int rc = stream_sync(c);
+    if (!rc) rc = check_kernel_error(c);
+    if (rc) return rc;
+    SC_HIP(hipMemcpy(AC_given, c.AC, cbm, out));
     return SC_OK;
 }
 
 extern "C" {
+
+int sc_adjoint_objective(int start, int end, sc_complex *AC, double weight)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!AC || !adjoint_range_ok(c, start, end)) return SC_EINVAL;
+    return adjoint_objective_run(start, end, AC, weight, hipMemcpyHostToDevice,
+                                 hipMemcpyDeviceToHost);
+}
 
 int c_adjoint(int start, int end, sc_complex *AC_given, int Nz_dup, int Nvar_dup, int Ny_dup,
               int Nx_dup, int inhomo, double strength)
@@ -1066,3 +1223,141 @@ int c_Nz(void) { return ctx().d.Nz; }
 int c_nsteps(void) { return ctx().d.nsteps; }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- members
+extern "C" {
+
+int sc_set_members(int n)
+{
+    if (n < 1 || n > 32) {
+        last_error() = "members: 1 .. 32";
+        return SC_EINVAL;
+    }
+    g_members = n;
+    return SC_OK;
+}
+
+int sc_members(void) { return ctx().ok ? ctx().nmem : g_members; }
+
+int sc_primal_members(int ru_steps, sc_complex *C_dev)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!C_dev || ru_steps < 0) return SC_EINVAL;
+    MemberRun run(c, c.nmem, false);
+    return primal_run(ru_steps, C_dev, hipMemcpyDeviceToDevice, hipMemcpyDeviceToDevice);
+}
+
+int sc_tangent_members(int start_step, int end_step, sc_complex *IC_dev, unsigned inhomo_mask,
+                       int shared_base)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!IC_dev || !tangent_range_ok(c, start_step, end_step)) return SC_EINVAL;
+    MemberRun run(c, c.nmem, shared_base != 0);
+    return tangent_run(start_step, end_step, IC_dev, inhomo_mask, hipMemcpyDeviceToDevice,
+                       hipMemcpyDeviceToDevice);
+}
+
+int sc_adjoint_members(int start_step, int end_step, sc_complex *AC_dev, double strength,
+                       int shared_base)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!AC_dev || !adjoint_range_ok(c, start_step, end_step)) return SC_EINVAL;
+    MemberRun run(c, c.nmem, shared_base != 0);
+    return adjoint_run(start_step, end_step, AC_dev, strength, hipMemcpyDeviceToDevice,
+                       hipMemcpyDeviceToDevice);
+}
+
+int sc_member_state(int incre, int member, int i_step, sc_complex *out)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!out || member < 0 || member >= c.nmem || i_step < 0 || i_step > c.d.nsteps) {
+        last_error() = "member_state requires 0 <= member < members and 0 <= i_step <= nsteps";
+        return SC_EINVAL;
+    }
+    const char *src = (const char *) ((incre ? c.MIC : c.MC) + 3 * c.csize * (size_t) i_step) +
+                      (size_t) member * c.mcbytes;
+    SC_HIP(hipStreamSynchronize(c.stream));
+    SC_HIP(hipMemcpy(out, src, sizeof(cplx) * c.csize, hipMemcpyDeviceToHost));
+    return SC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- gradients
+extern "C" int sc_gradient_euclid(const sc_complex *lam, sc_complex *g)
+{
+    // <lam, d> in the Gauss-weighted physical inner product of
+    // drivers/testAdj.py:55-56 is, by Parseval on the dealiased grid and the
+    // exactness of the Gauss rule (as in ddt_project's inner products),
+    //   Px Pz sum over columns c Re[lam_h^H G d_h]
+    // with G = Qs + Qps/k2 on alpha (rows < dimQ) and Rs/k2 on beta, c = 2
+    // for kx > 0, c = 1 at kx = 0 where both enter through their Hermitian
+    // parts (f(kz) + conj f(-kz)) / 2, and the mean column's real parts with
+    // Rs; row Nz/2 counts nothing.  Its gradient in the real and imaginary
+    // parts of d: 2 Px Pz G lam (kx > 0), Px Pz G lam_h (kx = 0), Px Pz Rs
+    // Re lam (mean, real), 0 elsewhere.
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!lam || !g) return SC_EINVAL;
+    const Dims &d = c.d;
+    const int Nz = d.Nz, Mx = d.Mx, nR = d.dimR, nQ = d.dimQ;
+    const double pp = (double) d.Px * d.Pz;
+    const cplx *L = (const cplx *) lam;
+    cplx *G = (cplx *) g;
+    auto at = [&](int z, int v, int i, int x) { return (((size_t) z * 2 + v) * nR + i) * Mx + x; };
+    std::fill(G, G + c.csize, make_double2(0.0, 0.0));
+    // band mat-vec y = A lam over rows of one column (A: N x (2dl+1), entry
+    // (i, i + 2m) at A[i (2dl+1) + dl + m])
+    auto band = [&](const double *A, int N, int dl, auto value, auto put) {
+        for (int i = 0; i < N; ++i) {
+            double sr = 0.0, si = 0.0;
+            for (int m = -std::min(dl, i / 2); m <= std::min(dl, (N - i + 1) / 2 - 1); ++m) {
+                const double a = A[(size_t) i * (2 * dl + 1) + dl + m];
+                const cplx v = value(i + 2 * m);
+                sr += a * v.x;
+                si += a * v.y;
+            }
+            put(i, make_double2(sr, si));
+        }
+    };
+    for (int z = 0; z < Nz; ++z) {
+        if (z == Nz / 2) continue;
+        for (int x = 0; x < Mx; ++x) {
+            if (z == 0 && x == 0) {
+                for (int v = 0; v < 2; ++v)
+                    band(c.Rs.data(), nR, 2,
+                         [&](int i) { return make_double2(L[at(0, v, i, 0)].x, 0.0); },
+                         [&](int i, cplx y) { G[at(0, v, i, 0)] = make_double2(pp * y.x, 0.0); });
+                continue;
+            }
+            const double k2 = c.Kz[z] * c.Kz[z] + c.Kx[x] * c.Kx[x];
+            const int zm = (Nz - z) % Nz;
+            auto lv = [&](int v, int i) {
+                const cplx a = L[at(z, v, i, x)];
+                if (x > 0) return a;
+                const cplx b = L[at(zm, v, i, 0)];
+                return make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+            };
+            const double cw = (x > 0 ? 2.0 : 1.0) * pp;
+            // alpha: (Qs + Qps / k2) on rows < dimQ
+            for (int i = 0; i < nQ; ++i) {
+                double sr = 0.0, si = 0.0;
+                for (int m = -std::min(4, i / 2); m <= std::min(4, (nQ - i + 1) / 2 - 1); ++m) {
+                    const size_t e = (size_t) i * 9 + 4 + m;
+                    const double a = c.Qs[e] + c.Qps[e] / k2;
+                    const cplx v = lv(0, i + 2 * m);
+                    sr += a * v.x;
+                    si += a * v.y;
+                }
+                G[at(z, 0, i, x)] = make_double2(cw * sr, cw * si);
+            }
+            band(c.Rs.data(), nR, 2, [&](int i) { return lv(1, i); },
+                 [&](int i, cplx y) { G[at(z, 1, i, x)] = make_double2(cw * y.x / k2, cw * y.y / k2); });
+        }
+    }
+    return SC_OK;
+}

@@ -407,6 +407,7 @@ struct ZinvArgs {
     int Nz, Mx, qpts, Pz, XB;
     double ax, az;                 // 2 pi / Lx, 2 pi / Lz
     const int *abort_flag;
+    size_t ms_w1;                  // member stride of W1
 };
 
 constexpr int MAXE = FFT_TILE / BLK;   // LDS elements per thread of a full tile
@@ -535,17 +536,19 @@ __device__ __forceinline__ void zinv_first_pass(const ZinvArgs &a, const cplx *s
     for (int q = 0; q < R1; ++q) dst[q * ZXB] = v[q];
 }
 
-// grid (ceil(Mx/XB), nf, qpts); PZ > 0: Pz == PZ, XB == ZXB and Mx % ZXB == 0
+// grid (ceil(Mx/XB), nf * members, qpts): blockIdx.y = member nf + field;
+// PZ > 0: Pz == PZ, XB == ZXB and Mx % ZXB == 0
 template <int PZ>
 __global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, cplx *W1)
 {
     extern __shared__ double2 lds[];
     if (*a.abort_flag) return;
     const int XB0 = PZ > 0 ? ZXB : a.XB;
-    const int x0 = blockIdx.x * XB0, f = blockIdx.y, y = blockIdx.z;
+    const int nf = a.fs.nf, m = blockIdx.y / nf, f = blockIdx.y - m * nf;
+    const int x0 = blockIdx.x * XB0, y = blockIdx.z;
     const int XB = PZ > 0 ? ZXB : min(a.XB, a.Mx - x0);
     const int Pz = PZ > 0 ? PZ : a.Pz, Mx = a.Mx;
-    const cplx *src = a.fs.src[f];
+    const cplx *src = a.fs.src[f] + m * a.fs.ms[f];
     const int kind = a.fs.kind[f], slot = a.fs.slot[f];
     double2 *twl = lds + XB0 * Pz;
     stage_twiddles(pz, twl);
@@ -565,7 +568,7 @@ __global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, cplx *W1)
         __syncthreads();
         lds_fft(lds, XB, 1, XB, pz, twl, +1.0);
     }
-    cplx *dst = W1 + (((size_t) y * gridDim.y + f) * Pz) * Mx;
+    cplx *dst = W1 + m * a.ms_w1 + (((size_t) y * nf + f) * Pz) * Mx;
     Walk wk(XB);
     for (int e = threadIdx.x; e < XB * Pz; e += BLK, wk.next())
         dst[(size_t) wk.q * Mx + x0 + wk.r] = lds[e];
@@ -583,6 +586,8 @@ struct XmixArgs {
     double *cflmax;        // [qpts], as ordered bit patterns
     double *flow;          // XM_PHYS output
     const int *abort_flag;
+    // member strides of W1, W1b, W1c, W2, W2b (grid z = member)
+    size_t ms_in, ms_inb, ms_inc, ms_out, ms_outb;
 };
 
 __device__ __forceinline__ void atomic_max_nonneg(double *addr, double v)
@@ -597,7 +602,7 @@ template <> struct XmixShape<XM_ADJOINT> { static constexpr int NF = 9, NOUT = 6
 template <> struct XmixShape<XM_PHYS> { static constexpr int NF = 3, NOUT = 0; };
 template <> struct XmixShape<XM_ADJ2> { static constexpr int NF = 15, NOUT = 12; };
 
-// grid (Pz/ZB, qpts); LDS: ZB * nseq sequences of Px complex at a stride of
+// grid (Pz/ZB, qpts, members); LDS: ZB * nseq sequences of Px complex at a stride of
 // Px + 1 (odd, so lanes on neighbouring sequences use different banks),
 // nseq = ceil(nf/2).  PX > 0: Px == PX and ZB == xmix_rows(PX, nseq).
 template <int MODE, int PX>
@@ -612,6 +617,13 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
     static_assert(MODE != XM_ADJ2 || PX > 0, "the merged adjoint mode has fixed-length kernels only");
     constexpr int ZBC = PX > 0 ? xmix_rows(PX, nseq) : 1;
     const int zb = blockIdx.x, y = blockIdx.y;
+    if (const int m = blockIdx.z) {
+        W1 += m * a.ms_in;
+        W2 += m * a.ms_out;
+        a.W1b += m * a.ms_inb;
+        a.W1c += m * a.ms_inc;
+        a.W2b += m * a.ms_outb;
+    }
     const int Px = PX > 0 ? PX : a.Px, Mx = PX > 0 ? PX / 3 : a.Mx, Pz = a.Pz;
     const int ZB = PX > 0 ? ZBC : a.ZB;
     const int SP = Px + 1;
@@ -871,9 +883,11 @@ struct ZfwdArgs {
     double *cflmax;
     double dt;
     int *abort_rw;
+    size_t ms_in, ms_out;      // member strides of W2 and dst
 };
 
-// grid (ceil(Mx/XB), nout, qpts); PZ > 0: Pz == PZ, XB == ZXB and Mx % ZXB == 0
+// grid (ceil(Mx/XB), nout * members, qpts): blockIdx.y = member nout + field;
+// PZ > 0: Pz == PZ, XB == ZXB and Mx % ZXB == 0
 template <int PZ>
 __global__ __launch_bounds__(BLK) void k_zfwd(ZfwdArgs a, PlanDev pz, const cplx *W2, cplx *dst)
 {
@@ -882,10 +896,12 @@ __global__ __launch_bounds__(BLK) void k_zfwd(ZfwdArgs a, PlanDev pz, const cplx
         cfl_check_block(a.cflmax, a.qpts, a.dt, a.abort_rw, a.cfl_slot);
     if (*a.abort_flag) return;
     const int XB0 = PZ > 0 ? ZXB : a.XB;
-    const int x0 = blockIdx.x * XB0, g = blockIdx.y, y = blockIdx.z;
+    const int m = blockIdx.y / a.nout, g = blockIdx.y - m * a.nout;
+    const int x0 = blockIdx.x * XB0, y = blockIdx.z;
     const int XB = PZ > 0 ? ZXB : min(a.XB, a.Mx - x0);
     const int Nz = PZ > 0 ? 2 * PZ / 3 : a.Nz, Pz = PZ > 0 ? PZ : a.Pz, Mx = a.Mx;
-    const cplx *src = W2 + (((size_t) y * a.nout + g) * Pz) * Mx;
+    const cplx *src = W2 + m * a.ms_in + (((size_t) y * a.nout + g) * Pz) * Mx;
+    dst += m * a.ms_out;
     double2 *twl = lds + XB0 * Pz;
     stage_twiddles(pz, twl);
     {
@@ -950,6 +966,10 @@ struct ZadjArgs {
     int nprob;
     const cplx *W2b;
     cplx *Ub;
+    // members: k_zfwd_adj grid z = member; k_zfwd_adj2 tiles of member m
+    // follow those of member m - 1 (strides of W2 / W2b and U / Ub)
+    int nm;
+    size_t ms_in, ms_out;
 };
 
 template <int PZ>
@@ -959,6 +979,8 @@ __global__ __launch_bounds__(BLK) void k_zfwd_adj(ZadjArgs a, PlanDev pz, const 
     if (*a.abort_flag) return;
     const int XA = PZ > 0 ? ZA : a.XA;
     const int x0 = blockIdx.x * XA, y = blockIdx.y;
+    W2 += blockIdx.z * a.ms_in;
+    U += blockIdx.z * a.ms_out;
     const int XB = PZ > 0 ? ZA : min(a.XA, a.Mx - x0);   // columns of this block
     const int Nz = PZ > 0 ? 2 * PZ / 3 : a.Nz, Pz = PZ > 0 ? PZ : a.Pz, Mx = a.Mx;
     const int NS = 6 * XA;                               // interleaved sequences
@@ -1039,14 +1061,17 @@ __global__ __launch_bounds__(BLK) void k_zfwd_adj2(ZadjArgs a, PlanDev pz, const
     extern __shared__ double2 lds[];
     if (*a.abort_flag) return;
     const int Mx = a.Mx, nxg = Mx / ZA, n1 = nxg * a.qpts;
-    int tile = xcd_tile(a.nprob * n1);
+    int tile = xcd_tile(a.nm * a.nprob * n1);
     if (tile < 0) return;
-    if (tile >= n1) {
+    const int prob = tile / n1, m = prob / a.nprob;
+    tile -= prob * n1;
+    if (prob - m * a.nprob) {
         // the second problem's tiles
-        tile -= n1;
         W2 = a.W2b;
         U = a.Ub;
     }
+    W2 += m * a.ms_in;
+    U += m * a.ms_out;
     const int y = tile / nxg, x0 = (tile - y * nxg) * ZA;
     double2 *twl = lds + TOT;
     stage_twiddles(pz, twl);
@@ -1099,10 +1124,14 @@ __global__ __launch_bounds__(BLK) void k_zfwd_adj2(ZadjArgs a, PlanDev pz, const
 
 // adjoint assembly (channel/pass2.c:474-566):
 // Hz = -Tz - i kx Sy, Hx = -Tx + i kz Sy, Hy = -Ty - i kz Sx + i kx Sz
+// grid (blocks, members): SP and U of member blockIdx.y at the given strides
 __global__ void k_assemble_adj(const cplx *SP, cplx *U, const double *Kx, const double *Kz,
-                               int Nz, int qpts, int Mx, const int *abort_flag)
+                               int Nz, int qpts, int Mx, const int *abort_flag, size_t ms_sp,
+                               size_t ms_u)
 {
     if (*abort_flag) return;
+    SP += blockIdx.y * ms_sp;
+    U += blockIdx.y * ms_u;
     size_t n = (size_t) Nz * qpts * Mx;
     for (size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x; e < n;
          e += (size_t) gridDim.x * blockDim.x) {
@@ -1160,7 +1189,10 @@ void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1)
     a.ax = 2. * M_PI / d.Lx;
     a.az = 2. * M_PI / d.Lz;
     a.abort_flag = c.abort_flag;
-    dim3 grid((d.Mx + a.XB - 1) / a.XB, fs.nf, d.qpts);
+    for (int f = 0; f < fs.nf; ++f) a.fs.ms[f] = mstride(c, fs.src[f]);
+    a.ms_w1 = mstride(c, W1);
+    const int nm = c.nrun;
+    dim3 grid((d.Mx + a.XB - 1) / a.XB, fs.nf * nm, d.qpts);
     size_t lds = sizeof(double2) * (a.XB * d.Pz + d.Pz);
     const double plane = 16.0 * (double) d.qpts * d.Mx;
     int slots = 0;
@@ -1171,7 +1203,7 @@ void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1)
     }
     if (fs.nf == 9) slots = 8;            // adjoint: 5 state slots + 3 adjoint
     if (fs.nf == 6 && fs.kind[3] == FK_SLOT) slots = 6;   // two velocity triples
-    double bytes = plane * ((double) slots * (d.Nz - 1) + (double) fs.nf * d.Pz);
+    double bytes = nm * plane * ((double) slots * (d.Nz - 1) + (double) fs.nf * d.Pz);
     const hipStream_t st = c.stream;
     const PlanDev pd = to_dev(c.pz);
     const ZinvKernel kern = fixed == 96 ? k_zinv<96> : fixed == 48 ? k_zinv<48> : k_zinv<0>;
@@ -1230,11 +1262,18 @@ void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
     a.cflmax = c.cflmax;
     a.flow = flow;
     a.abort_flag = c.abort_flag;
-    dim3 grid(d.Pz / zb, d.qpts);
+    a.ms_in = mstride(c, W1);
+    a.ms_inb = mstride(c, a.W1b);
+    a.ms_inc = 0;
+    a.ms_out = mstride(c, W2);
+    a.ms_outb = 0;
+    // the physical output exists for one member
+    const int nm = mode == XM_PHYS ? 1 : c.nrun;
+    dim3 grid(d.Pz / zb, d.qpts, nm);
     size_t lds = sizeof(double2) * (zb * nseq * (d.Px + 1) + d.Px);
     PlanDev pd = to_dev(c.px);
     const double rows = 16.0 * (double) d.qpts * d.Pz * d.Mx;
-    double bytes = rows * (a.nf + a.nout);
+    double bytes = nm * rows * (a.nf + a.nout);
     if (mode == XM_PHYS) bytes += 8.0 * 3 * d.Px * d.qpts * d.Pz;
     const hipStream_t st = c.stream;
     const XmixKernel kern = xmix_kernel(mode, d.Px, nseq, zb);
@@ -1285,11 +1324,16 @@ void launch_xmix_adj2(Context &c, const cplx *Wa, const cplx *Wb, const cplx *Wl
     a.cflmax = c.cflmax;
     a.flow = nullptr;
     a.abort_flag = c.abort_flag;
-    const dim3 grid(d.Pz / zb, d.qpts);
+    a.ms_in = mstride(c, Wa);
+    a.ms_inb = mstride(c, Wb);
+    a.ms_inc = mstride(c, Wl);
+    a.ms_out = mstride(c, W2);
+    a.ms_outb = mstride(c, W2b);
+    const dim3 grid(d.Pz / zb, d.qpts, c.nrun);
     const size_t lds = sizeof(double2) * (zb * 8 * (d.Px + 1) + d.Px);
     const PlanDev pd = to_dev(c.px);
     const XmixKernel kern = d.Px == 96 ? k_xmix<XM_ADJ2, 96> : k_xmix<XM_ADJ2, 48>;
-    const double bytes = 16.0 * (double) d.qpts * d.Pz * d.Mx * (a.nf + a.nout);
+    const double bytes = c.nrun * 16.0 * (double) d.qpts * d.Pz * d.Mx * (a.nf + a.nout);
     const hipStream_t st = c.stream;
     auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, Wa, W2); };
     prof_begin(c, st, 7, bytes);
@@ -1324,9 +1368,11 @@ void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind, 
     a.dst_kind = dst_kind;
     a.norm = 1.0 / ((3. * d.Nx / 2.) * (3. * d.Nz / 2.));
     a.abort_flag = c.abort_flag;
-    dim3 grid((d.Mx + a.XB - 1) / a.XB, nout, d.qpts);
+    a.ms_in = mstride(c, W2);
+    a.ms_out = mstride(c, dst);
+    dim3 grid((d.Mx + a.XB - 1) / a.XB, nout * c.nrun, d.qpts);
     size_t lds = sizeof(double2) * (a.XB * d.Pz + d.Pz);
-    double bytes = 16.0 * (double) d.qpts * d.Mx * nout * (d.Pz + d.Nz - 1);
+    double bytes = c.nrun * 16.0 * (double) d.qpts * d.Mx * nout * (d.Pz + d.Nz - 1);
     const hipStream_t st = c.stream;
     const PlanDev pd = to_dev(c.pz);
     const ZfwdKernel kern = fixed == 96 ? k_zfwd<96> : fixed == 48 ? k_zfwd<48> : k_zfwd<0>;
@@ -1367,11 +1413,15 @@ bool launch_zfwd_adj2(Context &c, const cplx *W2, cplx *Udst, const cplx *W2b, c
     a.Kx = c.dKx;
     a.Kz = c.dKz;
     a.abort_flag = c.abort_flag;
-    const dim3 grid = fixed ? dim3(xcd_grid(a.nprob * (d.Mx / ZA) * d.qpts))
-                            : dim3((d.Mx + a.XA - 1) / a.XA, d.qpts);
+    a.nm = c.nrun;
+    a.ms_in = mstride(c, W2);
+    a.ms_out = mstride(c, Udst);
+    if (W2b && (mstride(c, W2b) != a.ms_in || mstride(c, Udstb) != a.ms_out)) return false;
+    const dim3 grid = fixed ? dim3(xcd_grid(a.nm * a.nprob * (d.Mx / ZA) * d.qpts))
+                            : dim3((d.Mx + a.XA - 1) / a.XA, d.qpts, a.nm);
     // fixed lengths: the two-phase kernel on a 3-field tile
     size_t lds = sizeof(double2) * ((fixed ? 3 : 6) * (size_t) a.XA * d.Pz + d.Pz);
-    double bytes = a.nprob * 16.0 * (double) d.qpts * d.Mx * (6.0 * d.Pz + 5.0 * (d.Nz - 1));
+    double bytes = a.nm * a.nprob * 16.0 * (double) d.qpts * d.Mx * (6.0 * d.Pz + 5.0 * (d.Nz - 1));
     const hipStream_t st = c.stream;
     const PlanDev pd = to_dev(c.pz);
     typedef void (*ZadjKernel)(ZadjArgs, PlanDev, const cplx *, cplx *);
@@ -1388,8 +1438,9 @@ void launch_assemble_adj(Context &c, const cplx *SP, cplx *Udst)
     const Dims &d = c.d;
     size_t n = (size_t) d.Nz * d.qpts * d.Mx;
     int blocks = (int) std::min<size_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_assemble_adj, dim3(blocks), dim3(256), 0, c.stream, SP, Udst,
-                       c.dKx, c.dKz, d.Nz, d.qpts, d.Mx, c.abort_flag);
+    hipLaunchKernelGGL(k_assemble_adj, dim3(blocks, c.nrun), dim3(256), 0, c.stream, SP, Udst,
+                       c.dKx, c.dKz, d.Nz, d.qpts, d.Mx, c.abort_flag, mstride(c, SP),
+                       mstride(c, Udst));
 }
 
 }  // namespace sc

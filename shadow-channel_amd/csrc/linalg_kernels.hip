@@ -74,7 +74,23 @@ struct RebuildArgs {
     int nprob;                          // 2: a second rebuild C1 -> U1 in the same launch
     const cplx *C1;
     cplx *U1;
+    // members: problem p = m nprob + sub (sub 1: C1 -> U1) reads C (or C1)
+    // + m ms_c and writes U (or U1) + m ms_u; k_rebuild: grid y = member
+    int nm;
+    size_t ms_c, ms_u;
 };
+
+// the coefficient and Gauss-point arrays of problem p (member p / nprob)
+__device__ __forceinline__ const cplx *rb_src(const RebuildArgs &a, const cplx *C, int p)
+{
+    const int m = p / a.nprob;
+    return (p - m * a.nprob ? a.C1 : C) + m * a.ms_c;
+}
+__device__ __forceinline__ cplx *rb_dst(const RebuildArgs &a, cplx *U, int p)
+{
+    const int m = p / a.nprob;
+    return (p - m * a.nprob ? a.U1 : U) + m * a.ms_u;
+}
 
 // U[z][0..4][y][x] from C[z][0..1][j][x]; 1-D grid over the (kz row, y-tile)
 // tiles, y-tile fastest, XCD-aware (xcd_tile); 256 threads.
@@ -97,6 +113,8 @@ __global__ __launch_bounds__(256, SC_YT_WAVES) void k_rebuild(RebuildArgs a, con
     if (*a.abort_flag) return;
     const int tile = xcd_tile(a.nyt * a.Nz);
     if (tile < 0) return;
+    C += blockIdx.y * a.ms_c;
+    U += blockIdx.y * a.ms_u;
     const int zrow = tile / a.nyt, yt = tile - zrow * a.nyt;
     const bool meanb = zrow == a.Nz / 2;
     const int z = meanb ? 0 : zrow;
@@ -330,13 +348,13 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_rows(RebuildArgs a, const cp
         for (int i = 0; i < PT; ++i) bs[threadIdx.x + 256 * i] = st[i];
     }
     // task t of the queue: rows z0..z1-1 in order, nnt n-tiles each (one in
-    // the mean row); with two rebuilds, tasks ntask1.. belong to the second
+    // the mean row); tasks p ntask1 .. belong to problem p (rb_src)
     const int zm = a.Nz / 2;
     const bool has_mean = zm >= z0 && zm < z1;
     const int ntask1 = (z1 - z0) * nnt - (has_mean ? nnt - 1 : 0);
-    const int ntask = ntask1 * a.nprob;
+    const int ntask = ntask1 * a.nprob * a.nm;
     auto task_of = [&](int t, int &zrow, int &nt) {
-        if (t >= ntask1) t -= ntask1;
+        t -= (t / ntask1) * ntask1;
         const int tm = has_mean ? (zm - z0) * nnt : ntask1;  // first task of the mean row
         if (t < tm) {
             zrow = z0 + t / nnt;
@@ -433,7 +451,7 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_rows(RebuildArgs a, const cp
     // operands of task t: offsets and the coefficient array of its rebuild
     auto task_ops = [&](int t, int zr, int nt) {
         RebuildTask r = rebuild_task(a, zr, nt, nl);
-        r.cd = (const double *) (t >= ntask1 ? a.C1 : C);
+        r.cd = (const double *) rb_src(a, C, t / ntask1);
         return r;
     };
     int t = grab();
@@ -469,7 +487,7 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_rows(RebuildArgs a, const cp
             chunk(mean, ch, acc);
             __builtin_amdgcn_sched_barrier(0);
         }
-        rebuild_store(a, zr, nt, yt * 16 + nl, kq, acc, t >= ntask1 ? a.U1 : U);
+        rebuild_store(a, zr, nt, yt * 16 + nl, kq, acc, rb_dst(a, U, t / ntask1));
         t = tn;
     }
     // every wave of the queue's blocks exits here once; the last clears it
@@ -516,9 +534,9 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cpl
     const int zm = a.Nz / 2;
     const bool has_mean = zm >= z0 && zm < z1;
     const int ntask1 = (z1 - z0) * nnt - (has_mean ? nnt - 1 : 0);
-    const int ntask = ntask1 * a.nprob;
+    const int ntask = ntask1 * a.nprob * a.nm;
     auto task_of = [&](int t, int &zrow, int &nt) {
-        if (t >= ntask1) t -= ntask1;
+        t -= (t / ntask1) * ntask1;
         const int tm = has_mean ? (zm - z0) * nnt : ntask1;
         if (t < tm) {
             zrow = z0 + t / nnt;
@@ -597,7 +615,7 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cpl
     };
     auto task_ops = [&](int t, int zr, int nt) {
         RebuildTask r = rebuild_task(a, zr, nt, nl);
-        r.cd = (const double *) (t >= ntask1 ? a.C1 : C);
+        r.cd = (const double *) rb_src(a, C, t / ntask1);
         return r;
     };
     const int hq = (a.qpts + 1) / 2;
@@ -648,7 +666,7 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cpl
             __builtin_amdgcn_sched_barrier(0);
         }
         if (y < hq) {
-            cplx *Ud = t >= ntask1 ? a.U1 : U;
+            cplx *Ud = rb_dst(a, U, t / ntask1);
             // values at y (E + O), then at the mirror point (s (E - O))
 #pragma unroll
             for (int m = 0; m < 5; ++m) {
@@ -672,14 +690,22 @@ void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx 
     const int nseg = std::max(1, std::min(d.Nz, 512 / c.nyt));
     // queued form: blocks per (XCD, y-tile), about two blocks per CU
     const int bpq = std::max(1, 64 / c.nyt);
-    const int nprob = C1 ? 2 : 1;
+    int nprob = C1 ? 2 : 1;
+    const int nm = c.nrun;
+    const size_t ms_c = mstride(c, C0), ms_u = mstride(c, U0);
+    if (C1 && (mstride(c, C1) != ms_c || mstride(c, U1) != ms_u)) {
+        // the two problems' arrays have other member strides: two launches
+        launch_rebuild2(c, C0, U0, nullptr, nullptr, vel_only);
+        launch_rebuild2(c, C1, U1, nullptr, nullptr, vel_only);
+        return;
+    }
     RebuildArgs a{c.dFragR, c.dKx, c.dKz, d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, c.nksR, c.nyt,
                   nseg, 2. * M_PI / d.Lx, 2. * M_PI / d.Lz, c.abort_flag, c.rb_queue, bpq,
-                  nprob, C1, U1};
-    double bytes = nprob * 16.0 * (double) (d.Nz - 1) * d.Mx * (2.0 * d.dimR + 5.0 * d.qpts);
+                  nprob, C1, U1, nm, ms_c, ms_u};
+    double bytes = nm * nprob * 16.0 * (double) (d.Nz - 1) * d.Mx * (2.0 * d.dimR + 5.0 * d.qpts);
     const hipStream_t st = c.stream;
     const int nks = c.nksR;
-    const dim3 grows(8 * c.nyt * bpq), gchunk(xcd_grid(c.nyt * d.Nz));
+    const dim3 grows(8 * c.nyt * bpq), gchunk(xcd_grid(c.nyt * d.Nz), nm);
     const int nksp = c.nksP;
     const bool par = c.ypar && (nksp == 8 || nksp == 16 || nksp == 24);
     RebuildArgs ap = a;
@@ -687,9 +713,9 @@ void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx 
     ap.nyt = c.nht;
     ap.nks = nksp;
     {
-        // enough waves for one task each (of one rebuild), at most about
-        // two blocks per CU
-        const int ntmax = (d.Nz + 7) / 8 * ((d.Mx + 7) / 8);
+        // enough waves for one task each (of one rebuild of all members), at
+        // most about two blocks per CU
+        const int ntmax = nm * ((d.Nz + 7) / 8 * ((d.Mx + 7) / 8));
         ap.bpq = std::max(1, std::min((ntmax + 3) / 4, SC_RB_BLOCKS / c.nht));
     }
     const dim3 gpar(8 * c.nht * ap.bpq);
@@ -734,6 +760,11 @@ struct ForceArgs {
     int Nz, Mx, qpts, dimR, dimQ, nks, nit;
     double ax, az;                      // 2 pi / Lx, 2 pi / Lz
     const int *abort_flag;
+    // members: problem p = m nprob + sub of the staged-basis kernels reads
+    // U (sub 0: U0, 1: U1) + m ms_u and writes Fa, Fb + m ms_fa, m ms_fb;
+    // k_force: grid y = member
+    int nm;
+    size_t ms_u, ms_fa, ms_fb;
 };
 
 // Fa[z][i][x] (i < dimQ), Fb[z][i][x] (i < dimR); 1-D grid over the (kz row,
@@ -758,6 +789,9 @@ __global__ __launch_bounds__(256, SC_YT_WAVES) void k_force(ForceArgs a, const c
     if (*a.abort_flag) return;
     const int tile = xcd_tile(a.nit * a.Nz);
     if (tile < 0) return;
+    U += blockIdx.y * a.ms_u;
+    Fa += blockIdx.y * a.ms_fa;
+    Fb += blockIdx.y * a.ms_fb;
     const int z = tile / a.nit, it = tile - z * a.nit;
     if (z == a.Nz / 2) return;
     const int lane = threadIdx.x & 63, wave = wave_id();
@@ -911,7 +945,7 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_rows(ForceArgs a, int bpi,
     const int Mx = a.Mx, q = a.qpts, nR = a.dimR, nQ = a.dimQ;
     const int nnt = (Mx + 7) / 8;
     const int ntask = (a.Nz - 1) * nnt;                       // every row but the Nyquist row
-    const int nall = ntask * nprob;
+    const int nall = ntask * nprob * a.nm;
     const int t0 = lb * nall / bpi, t1 = (lb + 1) * nall / bpi;
     {
         // bases of the i-tile: fragF [nit][nksF][5][64] -> [NKS][NB][64]
@@ -934,13 +968,15 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_rows(ForceArgs a, int bpi,
     __syncthreads();
     // staged bases per k-step: Qpw, Qw, Qppw, Rw, Rpw (adjoint); Qpw, Qw, Rw (primal)
     struct Task {
-        int z, nt, xcol, part, p;
+        int z, nt, xcol, part, p, m;
         size_t off;
     };
     auto task_of = [&](int t) {
         Task k;
-        k.p = t >= ntask;                                     // which projection
-        t -= k.p * ntask;
+        const int pr = t / ntask;                             // problem: member, projection
+        k.m = pr / nprob;
+        k.p = pr - k.m * nprob;
+        t -= pr * ntask;
         const int zr = t / nnt;
         k.z = zr < a.Nz / 2 ? zr : zr + 1;
         k.nt = t - zr * nnt;
@@ -952,7 +988,7 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_rows(ForceArgs a, int bpi,
     const size_t ss2 = (size_t) q * Mx * 2;                   // slot stride in doubles
     double r[2][NF][CK];
     auto load = [&](const Task &k, int ch, int buf) {
-        const double *Ud = (const double *) (k.p ? U1 : U0);
+        const double *Ud = (const double *) ((k.p ? U1 : U0) + k.m * a.ms_u);
         const bool mean = k.z == 0 && k.xcol == 0;
         const int other = mean ? 0 : 1 - k.part;
 #pragma unroll
@@ -1028,7 +1064,7 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_rows(ForceArgs a, int bpi,
         }
         const int i = it * 16 + nl;
         if (i < nR) {
-            cplx *Fa = k.p ? Fa1 : Fa0, *Fb = k.p ? Fb1 : Fb0;
+            cplx *Fa = (k.p ? Fa1 : Fa0) + k.m * a.ms_fa, *Fb = (k.p ? Fb1 : Fb0) + k.m * a.ms_fb;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int x = k.nt * 8 + 2 * kq + h;
@@ -1099,7 +1135,7 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_par(ForceArgs a, int bpi, 
     const int Mx = a.Mx, q = a.qpts, nR = a.dimR, nQ = a.dimQ;
     const int nnt = (Mx + 7) / 8;
     const int ntask = (a.Nz - 1) * nnt;
-    const int nall = ntask * nprob;
+    const int nall = ntask * nprob * a.nm;
     const int t0 = lb * nall / bpi, t1 = (lb + 1) * nall / bpi;
     {
         // fragQ [npair][NKS][2][5][64] -> [NKS][2][NB][64]
@@ -1121,13 +1157,15 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_par(ForceArgs a, int bpi, 
     }
     __syncthreads();
     struct Task {
-        int z, nt, xcol, part, p;
+        int z, nt, xcol, part, p, m;
         unsigned off;                                         // bytes: row z, column xcol
     };
     auto task_of = [&](int t) {
         Task k;
-        k.p = t >= ntask;
-        t -= k.p * ntask;
+        const int pr = t / ntask;                             // problem: member, projection
+        k.m = pr / nprob;
+        k.p = pr - k.m * nprob;
+        t -= pr * ntask;
         const int zr = t / nnt;
         k.z = zr < a.Nz / 2 ? zr : zr + 1;
         k.nt = t - zr * nnt;
@@ -1142,7 +1180,7 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_par(ForceArgs a, int bpi, 
     double r[2][2][NF][CK];
     auto load = [&](const Task &k, int ch, int buf) {
         __amdgpu_buffer_rsrc_t rs =
-            __builtin_amdgcn_make_buffer_rsrc((void *) (k.p ? U1 : U0), 0, ubytes, 0x00020000);
+            __builtin_amdgcn_make_buffer_rsrc((void *) ((k.p ? U1 : U0) + k.m * a.ms_u), 0, ubytes, 0x00020000);
         const bool mean = k.z == 0 && k.xcol == 0;
         const unsigned other = k.off + (mean ? 0 : 1 - k.part) * 8, own = k.off + k.part * 8;
         // lane offsets for the point and the part, uniform slot offsets
@@ -1261,7 +1299,7 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_par(ForceArgs a, int bpi, 
             }
         }
         if (act && part == 0) {
-            cplx *Fa = k.p ? Fa1 : Fa0, *Fb = k.p ? Fb1 : Fb0;
+            cplx *Fa = (k.p ? Fa1 : Fa0) + k.m * a.ms_fa, *Fb = (k.p ? Fb1 : Fb0) + k.m * a.ms_fb;
 #pragma unroll
             for (int par = 0; par < 2; ++par) {
                 const int i = 2 * (tp * 16 + nl) + par;
@@ -1296,7 +1334,7 @@ static void launch_force_par(const ForceArgs &a, int npair, int ntask, int nprob
         attr = true;
     }
     // blocks per pair: a multiple of 8 (XCD-aware mapping), one block per CU
-    const int bpi = std::max(8, std::min((ntask * nprob + 7) / 8 * 8, 256 / npair / 8 * 8));
+    const int bpi = std::max(8, std::min((ntask * nprob * a.nm + 7) / 8 * 8, 256 / npair / 8 * 8));
     hipLaunchKernelGGL((k_force_par<ADJ, NKS>), dim3(npair * bpi), dim3(FR_THREADS), lds, st, a, bpi,
                        nprob, U0, Fa0, Fb0, U1, Fa1, Fb1);
 }
@@ -1317,7 +1355,7 @@ static void launch_force_rows(const ForceArgs &a, int nit, int ntask, int nprob,
     // blocks per i-tile: one 8-wave block per CU in total (the registers of
     // the wave's two field buffers allow no second block), never more than
     // the tasks
-    const int bpi = std::max(1, std::min(ntask * nprob, 256 / nit));
+    const int bpi = std::max(1, std::min(ntask * nprob * a.nm, 256 / nit));
     hipLaunchKernelGGL((k_force_rows<ADJ, NKS>), dim3(nit * bpi), dim3(FR_THREADS), lds, st, a, bpi,
                        nprob, U0, Fa0, Fb0, U1, Fa1, Fb1);
 }
@@ -1328,11 +1366,19 @@ void launch_force2(Context &c, int adj, const cplx *U0, cplx *Fa0, cplx *Fb0, co
                    cplx *Fa1, cplx *Fb1)
 {
     const Dims &d = c.d;
+    const int nm = c.nrun;
+    const size_t ms_u = mstride(c, U0), ms_fa = mstride(c, Fa0), ms_fb = mstride(c, Fb0);
+    if (U1 && (mstride(c, U1) != ms_u || mstride(c, Fa1) != ms_fa || mstride(c, Fb1) != ms_fb)) {
+        // the two problems' arrays have other member strides: two launches
+        launch_force2(c, adj, U0, Fa0, Fb0, nullptr, nullptr, nullptr);
+        launch_force2(c, adj, U1, Fa1, Fb1, nullptr, nullptr, nullptr);
+        return;
+    }
     ForceArgs a{c.dFragF, c.dKx, c.dKz, d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, c.nksF, c.nit,
-                2. * M_PI / d.Lx, 2. * M_PI / d.Lz, c.abort_flag};
-    dim3 grid(xcd_grid(c.nit * d.Nz));
+                2. * M_PI / d.Lx, 2. * M_PI / d.Lz, c.abort_flag, nm, ms_u, ms_fa, ms_fb};
+    dim3 grid(xcd_grid(c.nit * d.Nz), nm);
     const int nprob = U1 ? 2 : 1;
-    double bytes = nprob * 16.0 * (double) (d.Nz - 1) * d.Mx *
+    double bytes = nm * nprob * 16.0 * (double) (d.Nz - 1) * d.Mx *
                    ((adj ? 5.0 : 3.0) * d.qpts + d.dimQ + d.dimR);
     const hipStream_t st = c.stream;
     const int nks = c.nksF, nit = c.nit, ntask = (d.Nz - 1) * ((d.Mx + 7) / 8);
@@ -1434,8 +1480,21 @@ struct SolveArgs {
     double dtc, dtd;                   // dt c_k, dt d_{k+1}
     double ce0[2], cn[2];              // explicit-operator coefficients per field (stage 0, next)
     double target;                     // mean-mode flux target
+    double target_alt;                 // ... of the members m with bit m of alt_mask set
+    unsigned alt_mask;
     const int *abort_flag;
     int *err;                          // device error word (SC_KERR_* bits)
+    // member strides (grid z = member) of C, Cs, Fa, Fb, Fa2, Fb2, Src
+    size_t ms_c, ms_cs, ms_fa, ms_fb, ms_fa2, ms_fb2, ms_src;
+    // optional right-hand-side source of the stage (C layout), added after
+    // the explicit part and the forcing, before the solve (all modes)
+    const cplx *Src;
+    // adjoint only: the mean column as the exact transpose of the tangent's
+    // flux correction: at stages 0 and 1 the right-hand side's real alpha
+    // loses w1 (add_mass . rhs) / 2 before the solve, and no correction
+    // follows the solve (the driver applies P at each step's start)
+    int exact_mean;
+    const double *add_mass;            // Rs^{-1} w1
 };
 
 // explicit band operator row entry A(i, i+2m)
@@ -1873,13 +1932,14 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx 
     const size_t rs = (size_t) 2 * Mx;                                          // next chain row
 
     // right-hand side rows and the next stage's projection, loads first
-    double2 cv[VR], fv[VR], f2v[VR];
+    double2 cv[VR], fv[VR], f2v[VR], sv[VR];
 #pragma unroll
     for (int t = 0; t < VR; ++t) {
         const int r = min(rt + t * RSTEP, nr - 1);
         cv[t] = C[cbase + r * rs];
         fv[t] = F[fbase + r * rs];
         f2v[t] = a.has_next ? F2[fbase + r * rs] : make_double2(0.0, 0.0);
+        sv[t] = a.Src ? a.Src[cbase + r * rs] : make_double2(0.0, 0.0);
     }
     // The factor image ([row][W][SBT], 16-byte loads): the L parts (columns
     // j < DL) of every row before the barrier, for the forward sweep; the U
@@ -1922,6 +1982,8 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx 
             double2 v = cv[t];
             v.x += a.dtc * fv[t].x;
             v.y += a.dtc * fv[t].y;
+            v.x += sv[t].x;
+            v.y += sv[t].y;
             vec[r * SBT + cc] = v;
         }
     }
@@ -1943,6 +2005,8 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx 
                 }
             acc.x += a.dtc * fv[t].x;
             acc.y += a.dtc * fv[t].y;
+            acc.x += sv[t].x;
+            acc.y += sv[t].y;
             vec[r * SBT + cc] = acc;
         }
         __syncthreads();
@@ -2129,9 +2193,31 @@ __device__ void solve_mean(const SolveArgs &a, cplx *C, cplx *Cs, double *lds)
         if (i < nR) {
             al[i].x += a.dtc * fr[t].x;
             be[i].x += a.dtc * fr[t].y;
+            if (a.Src) {
+                const double2 sa = a.Src[(size_t) i * Mx], sb = a.Src[(size_t) (nR + i) * Mx];
+                al[i].x += sa.x;
+                al[i].y += sa.y;
+                be[i].x += sb.x;
+                be[i].y += sb.y;
+            }
         }
     }
     __syncthreads();
+    if (a.exact_mean && a.k < 2) {
+        // the transpose of the flux correction on the right-hand side
+        double q = 0.0;
+        for (int i = tid; i < nR; i += T) q += a.add_mass[i] * al[i].x;
+        red[tid] = q;
+        __syncthreads();
+        for (int o = T / 2; o > 0; o >>= 1) {
+            if (tid < o) red[tid] += red[tid + o];
+            __syncthreads();
+        }
+        const double h = 0.5 * red[0];
+        __syncthreads();
+        for (int i = tid; i < nR; i += T) al[i].x -= h * a.rowsum[i];
+        __syncthreads();
+    }
     const int n0 = (nR + 1) / 2, n1 = nR / 2;
     if (tid == 0) chain_subst<2>(lm, 10, 1, al, 2, n0);
     if (tid == 64) chain_subst<2>(lm + 5, 10, 1, al + 1, 2, n1);
@@ -2148,8 +2234,10 @@ __device__ void solve_mean(const SolveArgs &a, cplx *C, cplx *Cs, double *lds)
     }
     const double flux_t = a.target - red[0];
     for (int i = tid; i < nR; i += T) {
-        al[i].x = al[i].x + flux_t * a.add[i] * 0.5;
-        al[i].y = al[i].y + flux_t * 0.0 * 0.5;
+        if (!a.exact_mean) {
+            al[i].x = al[i].x + flux_t * a.add[i] * 0.5;
+            al[i].y = al[i].y + flux_t * 0.0 * 0.5;
+        }
         Cs[(size_t) i * Mx] = al[i];
         Cs[(size_t) (nR + i) * Mx] = be[i];
     }
@@ -2182,8 +2270,8 @@ __device__ void nyquist_clear(const SolveArgs &a, int var, int par, int blk, cpl
         C[base + (size_t) (e / SBT) * 2 * Mx + e % SBT] = make_double2(0.0, 0.0);
 }
 
-// grid (nblk, 4), or (nblk + 1, 4) when the mean mode needs a block of its
-// own: blockIdx.y = field*2 + parity.  With Mx % SBT == 0 the blocks of the
+// grid (nblk, 4, members), or (nblk + 1, 4, members) when the mean mode
+// needs a block of its own: blockIdx.y = field*2 + parity.  With Mx % SBT == 0 the blocks of the
 // Nyquist row hold no chains, and the first of them with blockIdx.y == 0 runs
 // the mean mode (so the grid is exactly the column blocks); otherwise the
 // extra x-block nblk with blockIdx.y == 0 does.
@@ -2192,6 +2280,17 @@ __global__ __launch_bounds__(SOLVE_THREADS, 4) void k_solve(SolveArgs a, cplx *C
 {
     extern __shared__ double lds_solve[];
     if (*a.abort_flag) return;
+    if (const int m = blockIdx.z) {
+        if ((a.alt_mask >> m) & 1u) a.target = a.target_alt;
+        C += m * a.ms_c;
+        Cs += m * a.ms_cs;
+        a.Fa += m * a.ms_fa;
+        a.Fb += m * a.ms_fb;
+        a.Fa2 += m * a.ms_fa2;
+        a.Fb2 += m * a.ms_fb2;
+        if (a.Src) a.Src += m * a.ms_src;
+    }
+    if (blockIdx.z == 0 && (a.alt_mask & 1u)) a.target = a.target_alt;
     const int var = blockIdx.y >> 1, par = blockIdx.y & 1, blk = blockIdx.x;
     if (blk == a.nblk) {
         if (blockIdx.y == 0) solve_mean(a, C, Cs, lds_solve);
@@ -2237,7 +2336,8 @@ int solve_columns_per_block(int dimR)
 }
 
 void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cplx *Fa2,
-                  const cplx *Fb2, double target)
+                  const cplx *Fb2, double target, double target_alt, unsigned alt_mask,
+                  const cplx *Src, int exact_mean)
 {
     const Dims &d = c.d;
     const double *A = adj ? BA_ : FA_, *Cc = adj ? BC_ : FC_, *D = adj ? BD_ : FD_;
@@ -2262,15 +2362,27 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
     a.cn[0] = k != 2 ? d.re * A[k + 1] * d.dt : 0.0;
     a.cn[1] = k != 2 ? -(d.re * A[k + 1] * d.dt) : 0.0;
     a.target = target;
+    a.target_alt = target_alt;
+    a.alt_mask = alt_mask;
+    a.Src = Src;
+    a.ms_src = mstride(c, Src);
+    a.exact_mean = adj && exact_mean;
+    a.add_mass = c.dAdd[0][k];
     a.abort_flag = c.abort_flag;
     a.err = c.abort_flag + 2;
+    a.ms_c = mstride(c, Ccarry);
+    a.ms_cs = mstride(c, Csol);
+    a.ms_fa = mstride(c, a.Fa);
+    a.ms_fb = mstride(c, a.Fb);
+    a.ms_fa2 = mstride(c, Fa2);
+    a.ms_fb2 = mstride(c, Fb2);
     // RHS in, projections in (the next stage's only when they are other
     // arrays: the primal's are the same), solution and next right-hand side
     // out, and the factor images in (each stored image once: mirror rows
     // share one)
     const bool f2 = k != 2 && Fa2 != c.Fa;
     const double fcols = (double) (c.factor_mirror ? d.Nz / 2 + 1 : d.Nz) * d.Mx;
-    double bytes = (double) (d.Nz - 1) * d.Mx *
+    double bytes = c.nrun * (double) (d.Nz - 1) * d.Mx *
                        (16.0 * (2.0 * d.dimR) * 3.0 + 16.0 * (d.dimQ + d.dimR) * (f2 ? 2.0 : 1.0)) +
                    fcols * 8.0 * (9.0 * d.dimQ + 5.0 * d.dimR);
     const int sbt = c.sbt;
@@ -2279,7 +2391,7 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
     a.nblk = (ncol + sbt - 1) / sbt;
     a.mirror = c.factor_mirror;
     a.nyq_mean = d.Mx % sbt == 0;
-    const dim3 grid(a.nblk + (a.nyq_mean ? 0 : 1), 4);
+    const dim3 grid(a.nblk + (a.nyq_mean ? 0 : 1), 4, c.nrun);
     const hipStream_t st = c.stream;
     // row slots per thread: the chains have at most (dimR + 1) / 2 rows
     const bool two = (d.dimR + 1) / 2 <= 2 * (SOLVE_THREADS / sbt);
@@ -2347,10 +2459,16 @@ __global__ __launch_bounds__(256) void k_ddt_gram(const cplx *P, const cplx *M, 
                                                   const double *Rs, const double *Kx,
                                                   const double *Kz, int Nz, int Mx, int dimR,
                                                   int dimQ, int ncol, double scale, double *part,
-                                                  const int *abort_flag)
+                                                  const int *abort_flag, size_t ms_p, size_t ms_m,
+                                                  size_t ms_v)
 {
     extern __shared__ double2 sh_ddt[];
     __shared__ double rs[8];
+    // member blockIdx.y: its partial pairs follow those of the members before
+    P += blockIdx.y * ms_p;
+    M += blockIdx.y * ms_m;
+    V += blockIdx.y * ms_v;
+    part += (size_t) blockIdx.y * 2 * gridDim.x;
     const int nxb = (Mx + ncol - 1) / ncol;
     const int z = blockIdx.x / nxb, x0 = (blockIdx.x - z * nxb) * ncol;
     const bool live = !*abort_flag && z != Nz / 2;
@@ -2499,12 +2617,25 @@ __global__ __launch_bounds__(256) void k_ddt_gram(const cplx *P, const cplx *M, 
 // scal.  With acc, also acc -= a vout, twice when `twice` (the adjoint's
 // trapezoid forcing at the end of one step and the start of the next,
 // applied back to back).
+// grid (blocks, members): member blockIdx.y sums its own nparts partial
+// pairs and writes its scalars to scal + 8 blockIdx.y
 __global__ __launch_bounds__(256) void k_ddt_apply(cplx *vout, const cplx *vin, const cplx *P,
                                                    const cplx *M, double s, const double *part,
                                                    int nparts, double *scal, size_t n, cplx *acc,
-                                                   double a, int twice)
+                                                   double a, int twice, size_t ms_vo, size_t ms_vi,
+                                                   size_t ms_p, size_t ms_m, size_t ms_acc)
 {
     __shared__ double rs[8];
+    {
+        const int m = blockIdx.y;
+        vout += m * ms_vo;
+        vin += m * ms_vi;
+        P += m * ms_p;
+        M += m * ms_m;
+        if (acc) acc += m * ms_acc;
+        part += (size_t) m * 2 * nparts;
+        scal += 8 * m;
+    }
     double sa = 0.0, sb = 0.0;
     for (int i = threadIdx.x; i < nparts; i += 256) {
         sa += part[2 * i];
@@ -2604,38 +2735,69 @@ void launch_tangent_forcing(Context &c, const cplx *S, cplx *fa, cplx *fb, cplx 
                        d.dimQ, d.dimR, fa, fb, f0);
 }
 
+// LDS bytes of a k_ddt_gram block of nc columns
+static size_t ddt_gram_lds(const Dims &d, int nc)
+{
+    return sizeof(double2) * 4 * (size_t) d.dimR * (nc + 1) +
+           sizeof(double) * (18 * (size_t) d.dimQ + 5 * (size_t) d.dimR);
+}
+
+// columns per k_ddt_gram block: 4, fewer while the block's items exceed
+// 256 DDT_IT or its LDS 64 KB (grids beyond dimR = 128 with one column ask
+// for more)
+static int ddt_gram_cols(const Dims &d)
+{
+    int ncol = 4;
+    while (ncol > 1 && (2 * d.dimR * ncol > 256 * DDT_IT || ddt_gram_lds(d, ncol) > 65536))
+        ncol /= 2;
+    return ncol;
+}
+
+int ddt_gram_blocks(const Dims &d)
+{
+    const int ncol = ddt_gram_cols(d);
+    return d.Nz * ((d.Mx + ncol - 1) / ncol);
+}
+
 int launch_ddt_gram(Context &c, const cplx *P, const cplx *M, double s, const cplx *V)
 {
+    return launch_ddt_gram(c, P, M, s, V, c.dotpart, c.ndotpart, c.abort_flag);
+}
+
+int launch_ddt_gram(Context &c, const cplx *P, const cplx *M, double s, const cplx *V,
+                    double *part, size_t part_cap, const int *stop)
+{
     const Dims &d = c.d;
-    // columns per block: 4, fewer while the block's items exceed 256 DDT_IT
-    // or its LDS 64 KB; grids beyond dimR = 128 with one column ask for more
-    auto lds_of = [&](int nc) {
-        return sizeof(double2) * 4 * (size_t) d.dimR * (nc + 1) +
-               sizeof(double) * (18 * (size_t) d.dimQ + 5 * (size_t) d.dimR);
-    };
-    int ncol = 4;
-    while (ncol > 1 && (2 * d.dimR * ncol > 256 * DDT_IT || lds_of(ncol) > 65536)) ncol /= 2;
-    const int nb = d.Nz * ((d.Mx + ncol - 1) / ncol);
-    if (2 * (size_t) nb > c.ndotpart) {
-        // c_init sizes dotpart for one column per block; never reached
+    const int ncol = ddt_gram_cols(d), nb = ddt_gram_blocks(d);
+    if (2 * (size_t) nb * c.nrun > part_cap) {
         last_error() = "ddt_project: partial-sum buffer too small";
         return 0;
     }
-    const size_t lds = lds_of(ncol);
+    const size_t lds = ddt_gram_lds(d, ncol);
     if (lds > 65536)
         (void) hipFuncSetAttribute((const void *) k_ddt_gram,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
-    hipLaunchKernelGGL(k_ddt_gram, dim3(nb), dim3(256), lds, c.stream, P, M, V, s, c.dQs, c.dQps,
-                       c.dRs, c.dKx, c.dKz, d.Nz, d.Mx, d.dimR, d.dimQ, ncol,
-                       (double) d.Px * (double) d.Pz, c.dotpart, c.abort_flag);
+    hipLaunchKernelGGL(k_ddt_gram, dim3(nb, c.nrun), dim3(256), lds, c.stream, P, M, V, s, c.dQs,
+                       c.dQps, c.dRs, c.dKx, c.dKz, d.Nz, d.Mx, d.dimR, d.dimQ, ncol,
+                       (double) d.Px * (double) d.Pz, part, stop, mstride(c, P), mstride(c, M),
+                       mstride(c, V));
     return nb;
 }
 
 void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, const cplx *M,
                       double s, size_t n, cplx *acc, double a, int nparts, int twice)
 {
-    hipLaunchKernelGGL(k_ddt_apply, dim3(1024), dim3(256), 0, c.stream, vout, vin, P, M, s,
-                       c.dotpart, nparts, c.scal, n, acc, a, twice);
+    launch_ddt_apply(c, vout, vin, P, M, s, n, acc, a, nparts, twice, c.dotpart, c.scal);
+}
+
+void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, const cplx *M,
+                      double s, size_t n, cplx *acc, double a, int nparts, int twice,
+                      const double *part, double *scal)
+{
+    const int nm = c.nrun;
+    hipLaunchKernelGGL(k_ddt_apply, dim3(std::max(128, 1024 / nm), nm), dim3(256), 0, c.stream,
+                       vout, vin, P, M, s, part, nparts, scal, n, acc, a, twice, mstride(c, vout),
+                       mstride(c, vin), mstride(c, P), mstride(c, M), acc ? mstride(c, acc) : 0);
 }
 
 }  // namespace sc

@@ -122,6 +122,157 @@ __global__ __launch_bounds__(OBJ_THREADS) void k_plane_stats(const cplx *U, int 
     }
 }
 
+// ------------------------------------------------------------ sensitivities
+// Reynolds-number source of the tangent's stage (per unit change of Re).  The
+// stage operators M + re a_k dt D (explicit) and M - re b_k dt D (implicit)
+// depend on re = 1/Re, so the right-hand side of the tangent's stage k gains
+//   Src = TF(dt a_k Y + dt b_k X),  TF(S) = (dre/dRe) D S = -re^2 D S,
+// TF being the forcing of channel/tangent_forcing.c and tangent_forcing0.c:
+// Y = MC[3n+k] (what the stage's explicit part acts on), X = MC[3n+k+1] (its
+// solution).  In the mean column the implicit operator acts on the solve's
+// own result, before the flux correction alpha += e add / 2 the stored X
+// carries: X_pre = X - e add / 2 with e = (flux - w1.Re X) / (1 - w1.add / 2).
+// Non-mean columns: one thread per element of Src (C layout); the mean column
+// is k_re_source_mean's.  Grid y: member (strides ms_y, ms_x, ms_s).
+__global__ void k_re_source(const cplx *Y, const cplx *X, double ca, double cb, const double *Qs,
+                            const double *Qps, const double *Qpps, const double *Rs,
+                            const double *Rps, const double *Kx, const double *Kz, double re,
+                            int Nz, int Mx, int nQ, int nR, cplx *Src, size_t ms_y, size_t ms_x,
+                            size_t ms_s)
+{
+    const size_t n = (size_t) Nz * 2 * nR * Mx;
+    const size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    Y += blockIdx.y * ms_y;
+    X += blockIdx.y * ms_x;
+    Src += blockIdx.y * ms_s;
+    const int x = (int) (e % Mx), i = (int) ((e / Mx) % nR), v = (int) ((e / ((size_t) Mx * nR)) % 2);
+    const int z = (int) (e / ((size_t) Mx * nR * 2));
+    if ((z == 0 && x == 0) || (v == 0 && i >= nQ)) {
+        if (!(z == 0 && x == 0)) Src[e] = make_double2(0.0, 0.0);
+        return;
+    }
+    const double k2 = Kz[z] * Kz[z] + Kx[x] * Kx[x], s2 = k2 * k2;
+    const size_t col = ((size_t) (z * 2 + v) * nR) * Mx + x;
+    double sr = 0.0, si = 0.0;
+    const int N = v == 0 ? nQ : nR, dl = v == 0 ? 4 : 2;
+    for (int m = -min(dl, i / 2); m <= min(dl, (N - i + 1) / 2 - 1); ++m) {
+        const size_t o = col + (size_t) (i + 2 * m) * Mx;
+        const double yr = ca * Y[o].x + cb * X[o].x, yi = ca * Y[o].y + cb * X[o].y;
+        double g;
+        if (v == 0) {
+            const int b = i * 9 + 4 + m;
+            g = -re * re * (s2 * Qs[b] + 2. * k2 * Qps[b] + Qpps[b]);
+        } else {
+            const int b = i * 5 + 2 + m;
+            g = re * re * (Rps[b] + k2 * Rs[b]);
+        }
+        sr += g * yr;
+        si += g * yi;
+    }
+    Src[e] = make_double2(sr, si);
+}
+
+// the mean column of the source: one block per member (grid x)
+__global__ __launch_bounds__(OBJ_THREADS) void k_re_source_mean(
+    const cplx *Y, const cplx *X, double ca, double cb, const double *Rps, const double *w1,
+    const double *add, double flux, double re, int nR, int Mx, cplx *Src, size_t ms_y,
+    size_t ms_x, size_t ms_s)
+{
+    Y += blockIdx.x * ms_y;
+    X += blockIdx.x * ms_x;
+    Src += blockIdx.x * ms_s;
+    __shared__ double red[2][OBJ_THREADS];
+    __shared__ double2 comb[2][512];
+    double wx = 0.0, wa = 0.0;
+    for (int i = threadIdx.x; i < nR; i += OBJ_THREADS) {
+        wx += w1[i] * X[(size_t) i * Mx].x;
+        wa += w1[i] * add[i];
+    }
+    red[0][threadIdx.x] = wx;
+    red[1][threadIdx.x] = wa;
+    __syncthreads();
+    for (int o = OBJ_THREADS / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + o];
+            red[1][threadIdx.x] += red[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    const double e = (flux - red[0][0]) / (1.0 - 0.5 * red[1][0]);
+    for (int i = threadIdx.x; i < nR; i += OBJ_THREADS) {
+        const cplx ya = Y[(size_t) i * Mx], xa = X[(size_t) i * Mx];
+        const cplx yb = Y[(size_t) (nR + i) * Mx], xb = X[(size_t) (nR + i) * Mx];
+        comb[0][i] = make_double2(ca * ya.x + cb * (xa.x - 0.5 * e * add[i]), ca * ya.y + cb * xa.y);
+        comb[1][i] = make_double2(ca * yb.x + cb * xb.x, ca * yb.y + cb * xb.y);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 2 * nR; t += OBJ_THREADS) {
+        const int v = t / nR, i = t - v * nR;
+        double sr = 0.0, si = 0.0;
+        for (int m = -min(2, i / 2); m <= min(2, (nR - i + 1) / 2 - 1); ++m) {
+            const double g = re * re * Rps[i * 5 + 2 + m];
+            sr += g * comb[v][i + 2 * m].x;
+            si += g * comb[v][i + 2 * m].y;
+        }
+        Src[(size_t) (v * nR + i) * Mx] = make_double2(sr, si);
+    }
+}
+
+// alpha(0,0) <- P alpha(0,0) on the real part, P = I - add w1^T / 2 (the
+// tangent's flux correction, self-adjoint in the mean column's pairing): the
+// exact adjoint's step start.  One block per member.
+__global__ __launch_bounds__(OBJ_THREADS) void k_mean_project(cplx *A, const double *w1,
+                                                              const double *add, int nR, int Mx,
+                                                              size_t ms)
+{
+    A += blockIdx.x * ms;
+    __shared__ double red[OBJ_THREADS];
+    double t = 0.0;
+    for (int i = threadIdx.x; i < nR; i += OBJ_THREADS) t += w1[i] * A[(size_t) i * Mx].x;
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (int o = OBJ_THREADS / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    const double h = 0.5 * red[0];
+    for (int i = threadIdx.x; i < nR; i += OBJ_THREADS) A[(size_t) i * Mx].x -= h * add[i];
+}
+
+// alpha(0,0) += weight g on the real part (the objective's representer).
+// One block per member.
+__global__ void k_add_mean(cplx *A, const double *g, double weight, int nR, int Mx, size_t ms)
+{
+    A += blockIdx.x * ms;
+    for (int i = threadIdx.x; i < nR; i += blockDim.x) A[(size_t) i * Mx].x += weight * g[i];
+}
+
+void launch_re_source(Context &c, const cplx *Y, const cplx *X, double ca, double cb, cplx *Src)
+{
+    const Dims &d = c.d;
+    const size_t n = (size_t) d.Nz * 2 * d.dimR * d.Mx;
+    const size_t my = mstride(c, Y), mx = mstride(c, X), ms = mstride(c, Src);
+    hipLaunchKernelGGL(k_re_source, dim3((unsigned) ((n + 255) / 256), c.nrun), dim3(256), 0,
+                       c.stream, Y, X, ca, cb, c.dQs, c.dQps, c.dQpps, c.dRs, c.dRps, c.dKx,
+                       c.dKz, d.re, d.Nz, d.Mx, d.dimQ, d.dimR, Src, my, mx, ms);
+    hipLaunchKernelGGL(k_re_source_mean, dim3(c.nrun), dim3(OBJ_THREADS), 0, c.stream, Y, X, ca,
+                       cb, c.dRps, c.dRowsum, c.dAdd[0][0], d.flux, d.re, d.dimR, d.Mx, Src, my,
+                       mx, ms);
+}
+
+void launch_mean_project(Context &c, cplx *A)
+{
+    hipLaunchKernelGGL(k_mean_project, dim3(c.nrun), dim3(OBJ_THREADS), 0, c.stream, A,
+                       c.dRowsum, c.dAdd[0][0], c.d.dimR, c.d.Mx, mstride(c, A));
+}
+
+void launch_add_mean(Context &c, cplx *A, const double *g, double weight)
+{
+    hipLaunchKernelGGL(k_add_mean, dim3(c.nrun), dim3(256), 0, c.stream, A, g, weight, c.d.dimR,
+                       c.d.Mx, mstride(c, A));
+}
+
 size_t objective_scratch_doubles(const Dims &d)
 {
     const size_t nchunk = (size_t) std::max(1, (d.nsteps + 1 + OBJ_CHUNK - 1) / OBJ_CHUNK);

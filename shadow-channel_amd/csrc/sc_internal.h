@@ -64,6 +64,7 @@ struct FieldSpec {
     int kind[MAXF];
     int slot[MAXF];
     int nf;
+    size_t ms[MAXF];    // member strides of the sources (set by launch_zinv)
 };
 
 // Pointwise stage of the x kernel.
@@ -81,10 +82,27 @@ struct Dims {
     double Lx, Lz, dt, re, flux, cfl1, cfl3;
 };
 
+// One per-member allocation: nmem copies of per bytes, one after the other.
+struct MemberArray {
+    const char *lo = nullptr;
+    size_t per = 0;
+    bool trajectory = false;   // the base trajectory (MC, field cache): shared in a shared-base run
+};
+
 struct Context {
     bool ok = false;
     Dims d{};
     hipStream_t stream = nullptr;
+
+    // Members: every per-member array holds nmem copies, member m at
+    // m * (its size); the kernels take the member from their grid and each
+    // pointer's member stride from mstride().  A run advances nrun members
+    // (1 for the c_* entry points: member 0).  In a shared-base run the
+    // members are tangent or adjoint solutions along member 0's trajectory.
+    int nmem = 1;
+    int nrun = 1;
+    bool shared_base = false;
+    std::vector<MemberArray> marrays;
 
     // host copies of the setup (also used for the laminar start)
     std::vector<double> y, w, cfl2, Kx, Kz;
@@ -130,6 +148,7 @@ struct Context {
     cplx *C = nullptr, *IC = nullptr, *AC = nullptr;
     cplx *U = nullptr, *IU = nullptr, *AU = nullptr, *LU = nullptr, *Ut = nullptr, *Ut2 = nullptr;
     cplx *MC = nullptr, *MIC = nullptr;
+    size_t mcbytes = 0;                         // bytes of one member's MC (and MIC)
     cplx *Csol = nullptr, *Ctmp = nullptr, *Ctmp2 = nullptr;
     cplx *Fa = nullptr, *Fb = nullptr;          // [Nz][dimQ][Mx], [Nz][dimR][Mx]
     cplx *FLa = nullptr, *FLb = nullptr;        // adjoint next-stage projection
@@ -150,10 +169,11 @@ struct Context {
     size_t ndotpart = 0;                        // doubles in dotpart
     double *scal = nullptr;                     // [8] device scalars
     // [0] CFL stop (1 + plane), [1] first unwritten slot, [2] kernel error
-    // bits (SC_KERR_*, sticky until reported), [3] unused
+    // bits (SC_KERR_*, sticky until reported), [3] always zero (never written)
     int *abort_flag = nullptr;
     double *objscr = nullptr;                   // objectives and statistics (objective_scratch_doubles)
     double *dRwall = nullptr;                   // [2][dimR]: R_j'(-1), R_j'(+1)
+    double *dGwall = nullptr;                   // [dimR]: Rs^{-1} Rp0 / (Px Pz), the wall shear's representer
     FftPlan pz, px;
     size_t csize = 0, usize = 0;                // element counts
     int status = 0;                             // -1 after a CFL failure
@@ -172,9 +192,22 @@ struct Context {
 
 Context &ctx();
 
+// member stride of p in bytes: the size of one copy of the per-member
+// array p points into; 0 when p is in no per-member array (constants), when
+// the run advances one member, or for the base trajectory of a shared-base
+// run (every member reads member 0's)
+size_t mstride_bytes(const Context &c, const void *p);
+template <class T>
+inline size_t mstride(const Context &c, const T *p)
+{
+    return mstride_bytes(c, (const void *) p) / sizeof(T);
+}
+
 // ---- setup (host) ----
 int setup_host(Context &c);
 void band_factor(double *A, int N, int dl);
+// forward and back substitution of a real right-hand side with band_factor's LU
+void band_subst_real(const double *A, double *b, int N, int dl);
 
 // ---- launches ----
 void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1);
@@ -210,8 +243,11 @@ void launch_force(Context &c, const cplx *Usrc, int adj, cplx *Fa, cplx *Fb);
 // two projections in one launch where the kernel allows it
 void launch_force2(Context &c, int adj, const cplx *U0, cplx *Fa0, cplx *Fb0, const cplx *U1,
                    cplx *Fa1, cplx *Fb1);
+// target: the mean mode's flux target; members m with bit m of alt_mask set
+// use target_alt instead
 void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cplx *Fa2,
-                  const cplx *Fb2, double target);
+                  const cplx *Fb2, double target, double target_alt = 0.0,
+                  unsigned alt_mask = 0, const cplx *Src = nullptr, int exact_mean = 0);
 int solve_columns_per_block(int dimR);
 // Re-sensitivity forcing of state S: fa [Nz][dimQ][Mx], fb [Nz][dimR][Mx], f0 [2][dimR]
 void launch_tangent_forcing(Context &c, const cplx *S, cplx *fa, cplx *fb, cplx *f0);
@@ -222,12 +258,31 @@ void launch_wall_plane(Context &c, const cplx *C, cplx *Uxb, cplx *Uzb);
 // 5 Pz Mx complex values
 void launch_incre_boundary(Context &c, cplx *Uxb, cplx *Uzb, cplx *scratch);
 // partial inner products of ddt_project from the coefficients of
-// dudt = s (P - M) and V; returns the number of partial pairs in dotpart
+// dudt = s (P - M) and V; returns the number of partial pairs written to
+// dotpart (or to part, of part_cap doubles), 0 if they do not fit
 int launch_ddt_gram(Context &c, const cplx *P, const cplx *M, double s, const cplx *V);
-// vout = vin - (num/den) s (P - M); with acc, also acc -= a vout (twice)
+// (stop: the word whose nonzero value skips the work, abort_flag by default)
+int launch_ddt_gram(Context &c, const cplx *P, const cplx *M, double s, const cplx *V,
+                    double *part, size_t part_cap, const int *stop);
+// blocks (partial pairs) of the inner-product launch for a grid
+int ddt_gram_blocks(const Dims &d);
+// vout = vin - (num/den) s (P - M); with acc, also acc -= a vout (twice);
+// num, den, num/den into scal (c.scal unless given), partials from dotpart
+// (or part)
 void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, const cplx *M,
                       double s, size_t n, cplx *acc, double a, int nparts, int twice);
+void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, const cplx *M,
+                      double s, size_t n, cplx *acc, double a, int nparts, int twice,
+                      const double *part, double *scal);
 
+// Reynolds-number source of a tangent stage: Src = TF(ca Y + cb X_pre)
+// (k_re_source: TF the tangent forcing, X_pre the mean solve before its
+// flux correction)
+void launch_re_source(Context &c, const cplx *Y, const cplx *X, double ca, double cb, cplx *Src);
+// the flux correction P on alpha(0,0) (the exact adjoint's step start)
+void launch_mean_project(Context &c, cplx *A);
+// alpha(0,0) += weight g (the wall-shear objective's representer)
+void launch_add_mean(Context &c, cplx *A, const double *g, double weight);
 // doubles of the objective/statistics scratch buffer for a grid and window
 size_t objective_scratch_doubles(const Dims &d);
 

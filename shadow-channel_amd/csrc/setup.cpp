@@ -102,7 +102,7 @@ static void device_factors(double *A, int N, int dl)
     }
 }
 
-static void band_subst_real(const double *A, double *b, int N, int dl)
+void band_subst_real(const double *A, double *b, int N, int dl)
 {
     int W = 2 * dl + 1;
     for (int i = 2; i < N; ++i)

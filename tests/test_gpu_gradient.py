@@ -1,0 +1,140 @@
+"""The wall-shear gradient on the device.
+
+* sc_tangent_sources (mean-velocity and Reynolds-number sources) and
+  sc_adjoint_objective against the oracle's orc_tangent_src and
+  orc_adjoint_objective (pinned in tests/test_oracle_gradient.py), 16x33x12,
+  at the one-step bar of tests/test_gpu_parity.py (1e-10).
+* sc_gradient_euclid: sum Re(conj(g) d) = <lam, d> in the physical inner
+  product formed from spec2phys fields, for directions with mean, kx = 0
+  and Nyquist-row content.
+* The 64x64x97 workload with the time-averaged lower-wall shear over a
+  1000-step window checkpointed every 50 steps (channel.gradient.WallShear,
+  a torch.autograd.Function): J equal to the unwindowed run's; dJ/dmeanU,
+  dJ/dRe and dJ/dC0 (along a random direction) against central differences
+  (eps 1e-6, relative to each quantity's size); the adjoint gradient against
+  the forward-mode (tangent) directional derivative.  1e-4.
+"""
+import numpy as np
+import pytest
+
+import pyorc
+from channel.synthetic import synthetic_state
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(Nx=16, Ny=33, Nz=12, Lx=np.pi, Lz=0.3 * np.pi, Re=2000.0, flux=4.0 / 3.0, dt=0.005)
+BENCH = dict(Nx=64, Ny=97, Nz=64, Lx=2 * np.pi, Lz=np.pi, Re=3000.0, meanU=2.0 / 3.0, dt=0.004)
+
+
+@pytest.fixture(scope="module")
+def cc():
+    import c_channel
+    yield c_channel
+    c_channel.c_destroy()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _st(cfg, seed, flux=4.0 / 3.0, **kw):
+    c = cfg
+    return synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], flux, seed=seed, **kw)
+
+
+def test_sources_and_objective_adjoint_against_oracle(cc):
+    print("case: tangent sources and objective adjoint vs oracle, 16x33x12", flush=True)
+    c, n = SMALL, 3
+    o = pyorc.Oracle(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"], n)
+    assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"],
+                     n) == 0
+    C0 = _st(c, 0, amp=0.1)
+    o.primal(0, C0.copy())
+    cc.c_primal(0, C0.copy())
+    for inhomo, re_src, seed in ((0, 1, None), (1, 1, 2), (1, 0, None)):
+        d = np.zeros_like(C0) if seed is None else _st(c, seed, unit=True)
+        Io, Ig = d.copy(), d.copy()
+        o.tangent_src(0, n, Io, inhomo, re_src)
+        cc.sc_tangent_sources(0, n, Ig, inhomo, re_src)
+        assert rel(Ig, Io) <= 1e-10, (inhomo, re_src)
+    A0 = _st(c, 4, unit=True)
+    A0[0, 0, :, 0] += np.linspace(0.1, -0.2, A0.shape[2])   # a mean column as well
+    Ao, Ag = A0.copy(), A0.copy()
+    o.adjoint_objective(n, 0, Ao, 0.7)
+    cc.sc_adjoint_objective(n, 0, Ag, 0.7)
+    assert rel(Ag, Ao) <= 1e-10
+    o.destroy()
+
+
+def test_euclidean_gradient_map(cc):
+    print("case: Euclidean gradient map, 16x33x12", flush=True)
+    c = SMALL
+    assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"],
+                     1) == 0
+    y, w = cc.c_quad()
+    rng = np.random.default_rng(7)
+    shape = (c["Nz"], 2, c["Ny"] - 2, c["Nx"] // 2)
+
+    def rand():
+        return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+    def phys(C):
+        f = np.zeros((3, 3 * c["Nx"] // 2, len(y), 3 * c["Nz"] // 2))
+        cc.c_spec2phys(np.ascontiguousarray(C), f)
+        return f
+    for _ in range(2):
+        lam, d = rand(), rand()
+        g = cc.sc_gradient_euclid(lam)
+        ip = float(np.sum(phys(lam) * phys(d) * w[None, None, :, None]))
+        assert abs(np.sum(g.real * d.real + g.imag * d.imag) - ip) <= 1e-12 * abs(ip) + 1e-12
+
+
+def test_checkpointed_wall_shear_gradient(cc):
+    """1000 steps at 64x64x97, checkpoints every 50."""
+    import torch
+    from channel.gradient import WallShearWindow, wall_shear
+    print("case: wall-shear gradient, 64x64x97, 1000 steps, checkpoints every 50", flush=True)
+    c = BENCH
+    N = 1000
+    win = WallShearWindow(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["dt"], N, every=50)
+    try:
+        C0 = _st(c, 0, flux=2 * c["meanU"])
+        Ct = torch.tensor(C0, requires_grad=True)
+        mu = torch.tensor(c["meanU"], dtype=torch.float64, requires_grad=True)
+        Re = torch.tensor(c["Re"], dtype=torch.float64, requires_grad=True)
+        J = wall_shear(win, Ct, mu, Re)
+        J.backward()
+        gC, gmu, gRe = Ct.grad.numpy(), float(mu.grad), float(Re.grad)
+
+        def Jof(C, m, r):
+            return win.forward(C, m, r, params=False)[0]
+        # checkpointing changes nothing: one window of 1000 steps
+        whole = WallShearWindow(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["dt"], N, every=N)
+        J_whole = whole.forward(C0, c["meanU"], c["Re"], params=False)[0]
+        whole.release()
+        assert abs(float(J) - J_whole) <= 1e-13 * abs(J_whole)
+        em, er = 1e-6 * c["meanU"], 1e-6 * c["Re"]
+        fd_mu = (Jof(C0, c["meanU"] + em, c["Re"]) - Jof(C0, c["meanU"] - em, c["Re"])) / (2 * em)
+        fd_re = (Jof(C0, c["meanU"], c["Re"] + er) - Jof(C0, c["meanU"], c["Re"] - er)) / (2 * er)
+        d = _st(c, 9, unit=True)
+        ed = 1e-6 * np.linalg.norm(C0)
+        fd_C = (Jof(C0 + ed * d, c["meanU"], c["Re"]) -
+                Jof(C0 - ed * d, c["meanU"], c["Re"])) / (2 * ed)
+        gd = float(np.sum(gC.real * d.real + gC.imag * d.imag))
+        # forward mode along d: the tangent from d, then the objective's tangent
+        win._init(c["meanU"], c["Re"])
+        I = d.copy()
+        tan = 0.0
+        for Cw in win.forward(C0, c["meanU"], c["Re"], params=False)[1]:
+            cc.c_primal(0, Cw.copy())
+            cc.sc_tangent_sources(0, win.every, I, 0, 0)
+            tan += float(cc.sc_mean_profile(1, 0, win.every, 1.0 / N)[1][0])
+        errs = {"meanU": abs(gmu - fd_mu) / abs(fd_mu), "Re": abs(gRe - fd_re) / abs(fd_re),
+                "C0": abs(gd - fd_C) / abs(fd_C), "C0_vs_tangent": abs(gd - tan) / abs(tan)}
+        print("J %.10e dJ/dmeanU %.6e (fd %.6e) dJ/dRe %.6e (fd %.6e) dJ/dC0.d %.6e (fd %.6e, "
+              "tangent %.6e)" % (float(J), gmu, fd_mu, gRe, fd_re, gd, fd_C, tan), flush=True)
+        print(" ".join("%s %.2e" % kv for kv in errs.items()), flush=True)
+        for k, e in errs.items():
+            assert e <= (1e-9 if k == "C0_vs_tangent" else 1e-4), (k, e)
+    finally:
+        win.release()

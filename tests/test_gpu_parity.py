@@ -273,8 +273,10 @@ def _forced_adjoint_step(cc, o, cfg):
 
 def test_benchmark_grid_one_step(cc):
     """64x64x97: one primal, tangent and forced adjoint step and ddt_project.
-    The FMA/plain floor of the oracle here is 9.7e-11 (step) and 1.8e-9
-    (forced adjoint; tests/test_oracle_rounding_floor.py)."""
+    The FMA/plain floors of the oracle here (tests/test_oracle_rounding_floor.py)
+    are 9.7e-11 (step), 1.7e-9 (tangent), 7.3e-10 and 4.1e-9 (ddt_project at
+    steps 0 and 1), 1.8e-9 (forced adjoint); the device lands at 1.0-1.05x
+    those, and the bounds allow 3x."""
     print("case: benchmark grid 64x64x97, primal + tangent + adjoint 1 step", flush=True)
     o, C0 = start(cc, BENCH, 1)
     Co, Cg = C0.copy(), C0.copy()
@@ -282,9 +284,9 @@ def test_benchmark_grid_one_step(cc):
     cc.c_primal(0, Cg)
     assert rel(Cg, Co) <= 3e-10
     e = _forced_adjoint_step(cc, o, BENCH)
-    assert e["tangent"] <= 3e-10
-    assert e["ddt0"] <= PROJ and e["ddt1"] <= PROJ
-    assert e["adjoint"] <= 5e-9
+    assert e["tangent"] <= 5e-9
+    assert e["ddt0"] <= 2.2e-9 and e["ddt1"] <= 1.2e-8
+    assert e["adjoint"] <= 5.4e-9
 
 
 def test_duality_on_device(cc):
@@ -410,10 +412,12 @@ G4 = dict(Nx=128, Ny=129, Nz=128, Lx=2 * np.pi, Lz=np.pi, Re=5000.0, flux=4.0 / 
 def test_largest_grid_one_step(cc):
     """The largest configuration, 128x128x129 (dimR = 127): the chunked
     rebuild, the 28-k-step projections and generic 192-point transforms.
-    One primal and one forced adjoint step against the oracle.  The rounding
-    floor at this grid (the oracle with and without FMA contraction,
-    tests/test_oracle_rounding_floor.py) is 9.8e-10 (step) and 1.75e-8
-    (adjoint); the device lands at about 1.1x those, and the bounds allow 3x."""
+    One primal, tangent and forced adjoint step and ddt_project against the
+    oracle.  The rounding floors at this grid (the oracle with and without
+    FMA contraction, tests/test_oracle_rounding_floor.py) are 9.8e-10
+    (step), 1.9e-8 (tangent), 8.1e-9 and 1.2e-9 (ddt_project at steps 0 and
+    1), 1.8e-8 (forced adjoint); the device lands at about 1.05x those, and
+    the bounds allow 3x."""
     print("case: largest grid 128x128x129, primal + tangent + adjoint 1 step", flush=True)
     o, C0 = start(cc, G4, 1)
     Co, Cg = C0.copy(), C0.copy()
@@ -423,6 +427,6 @@ def test_largest_grid_one_step(cc):
     print("largest grid: step %.2e" % ep, flush=True)
     e = _forced_adjoint_step(cc, o, G4)
     assert ep <= 3e-9
-    assert e["tangent"] <= 3e-9
-    assert e["ddt0"] <= 3e-9 and e["ddt1"] <= 3e-9
+    assert e["tangent"] <= 6e-8
+    assert e["ddt0"] <= 2.5e-8 and e["ddt1"] <= 4e-9
     assert e["adjoint"] <= 6e-8

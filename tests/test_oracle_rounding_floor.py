@@ -89,74 +89,67 @@ def test_rounding_floor_of_one_step():
     assert 1e-12 < floor < 1e-10
 
 
-@needs_fma
-def test_rounding_floor_at_benchmark_grid():
-    """64x64x97 channel (Lx = 2 pi, Lz = pi, Re = 3000, dt = 0.004): one primal
-    step and one adjoint step (strength 1).  The wall-normal resolution makes
-    the solves far less well conditioned than at Ny = 65: measured floors
-    9.7e-11 (step) and 1.8e-9 (adjoint); the GPU tolerances at this grid
-    (3e-10, 5e-9) stay above them."""
-    c = dict(Nx=64, Ny=97, Nz=64, Lx=2 * np.pi, Lz=np.pi, Re=3000.0, flux=4.0 / 3.0, dt=0.004)
-
-    def run(part):
-        def f():
-            o = pyorc.Oracle(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
-                             c["dt"], 1)
-            C = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=0)
-            o.primal(0, C)
-            if part == "primal":
-                o.destroy()
-                return C
-            # a tangent first, so that the adjoint's forcing P(MIC[3n]) is active
-            o.tangent(0, 1, synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"],
-                                            c["flux"], seed=3, unit=True), 0)
-            A = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=4,
-                                unit=True)
-            o.adjoint(1, 0, A, 0, 1.0)
-            o.destroy()
-            return A
-        return f
-
-    step = _both_builds(run("primal"))
-    adj = _both_builds(run("adjoint"))
-    assert 1e-12 < step < 2e-10
-    assert 1e-11 < adj < 3e-9
+def _one_step_parts(c):
+    """One primal step, a one-step tangent, ddt_project of two directions at
+    steps 0 and 1, and a forced adjoint step, from the loaded oracle build."""
+    st = lambda seed, **kw: synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"],
+                                            c["flux"], seed=seed, **kw)
+    o = pyorc.Oracle(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"], 1)
+    C = st(0)
+    o.primal(0, C)
+    out = {"step": C}
+    I = st(3, unit=True)
+    o.tangent(0, 1, I, 0)                 # also makes the adjoint's forcing active
+    out["tangent"] = I
+    for i in (0, 1):
+        v = st(10 + i, unit=True)
+        o.ddt_project(i, v)
+        out["ddt%d" % i] = v
+    A = st(4, unit=True)
+    o.adjoint(1, 0, A, 0, 1.0)
+    out["adjoint"] = A
+    o.destroy()
+    return out
 
 
-@needs_fma
-def test_rounding_floor_at_largest_grid():
-    """128x128x129 channel (Lx = 2 pi, Lz = pi, Re = 5000, dt = 0.002): one
-    primal step and one adjoint step (strength 1), about 100 s.  Measured
-    floors 9.8e-10 (step) and 1.75e-8 (adjoint); the GPU tolerances at this
-    grid (3e-9, 6e-8) stay above them."""
-    c = dict(Nx=128, Ny=129, Nz=128, Lx=2 * np.pi, Lz=np.pi, Re=5000.0, flux=4.0 / 3.0, dt=0.002)
-
-    def run():
-        o = pyorc.Oracle(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
-                         c["dt"], 1)
-        C = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=0)
-        o.primal(0, C)
-        # a tangent first, so that the adjoint's forcing P(MIC[3n]) is active
-        o.tangent(0, 1, synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"],
-                                        seed=3, unit=True), 0)
-        A = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=4,
-                            unit=True)
-        o.adjoint(1, 0, A, 0, 1.0)
-        o.destroy()
-        return np.concatenate([C.ravel(), A.ravel()])
-
+def _floors(c):
     outs = []
     for path in (pyorc.lib_path(), FMA_LIB):
         pyorc._LIB = None
         orig = pyorc.lib_path
         pyorc.lib_path = lambda p=path: p
         try:
-            outs.append(run())
+            outs.append(_one_step_parts(c))
         finally:
             pyorc.lib_path = orig
             pyorc._LIB = None
-    n = outs[0].size // 2
-    step = np.linalg.norm(outs[0][:n] - outs[1][:n]) / np.linalg.norm(outs[0][:n])
-    adj = np.linalg.norm(outs[0][n:] - outs[1][n:]) / np.linalg.norm(outs[0][n:])
-    assert 1e-12 < step < 1.5e-9, step
-    assert 1e-11 < adj < 3e-8, adj
+    return {k: np.linalg.norm(outs[0][k] - outs[1][k]) / np.linalg.norm(outs[0][k])
+            for k in outs[0]}
+
+
+@needs_fma
+def test_rounding_floor_at_benchmark_grid():
+    """64x64x97 channel (Lx = 2 pi, Lz = pi, Re = 3000, dt = 0.004).  The
+    wall-normal resolution makes the solves far less well conditioned than at
+    Ny = 65: measured floors 9.7e-11 (step), 1.7e-9 (tangent), 7.3e-10 and
+    4.1e-9 (ddt_project at steps 0 and 1), 1.8e-9 (forced adjoint); the GPU
+    bounds at this grid (tests/test_gpu_parity.py: 3e-10, 5e-9, 2.2e-9,
+    1.2e-8, 5.4e-9) stay at 3x them."""
+    c = dict(Nx=64, Ny=97, Nz=64, Lx=2 * np.pi, Lz=np.pi, Re=3000.0, flux=4.0 / 3.0, dt=0.004)
+    f = _floors(c)
+    gpu = {"step": 3e-10, "tangent": 5e-9, "ddt0": 2.2e-9, "ddt1": 1.2e-8, "adjoint": 5.4e-9}
+    for k, b in gpu.items():
+        assert 1e-13 < f[k] <= b / 2.5, (k, f[k])
+
+
+@needs_fma
+def test_rounding_floor_at_largest_grid():
+    """128x128x129 channel (Lx = 2 pi, Lz = pi, Re = 5000, dt = 0.002), about
+    two minutes.  Measured floors 9.8e-10 (step), 1.9e-8 (tangent), 8.1e-9
+    and 1.2e-9 (ddt_project at steps 0 and 1), 1.8e-8 (forced adjoint); the
+    GPU bounds at this grid (3e-9, 6e-8, 2.5e-8, 4e-9, 6e-8) stay at 3x them."""
+    c = dict(Nx=128, Ny=129, Nz=128, Lx=2 * np.pi, Lz=np.pi, Re=5000.0, flux=4.0 / 3.0, dt=0.002)
+    f = _floors(c)
+    gpu = {"step": 3e-9, "tangent": 6e-8, "ddt0": 2.5e-8, "ddt1": 4e-9, "adjoint": 6e-8}
+    for k, b in gpu.items():
+        assert 1e-13 < f[k] <= b / 2.5, (k, f[k])
