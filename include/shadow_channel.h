@@ -231,6 +231,16 @@ int sc_tangent_sources(int start_step, int end_step, sc_complex *IC, int inhomo,
 int sc_adjoint_objective(int start_step, int end_step, sc_complex *AC, double weight);
 int sc_gradient_euclid(const sc_complex *lam, sc_complex *g);
 
+/* No counterpart in the reference: c_ddt_project for callers that cannot
+   allocate device memory (code under HIP graph capture, serving loops).
+   v_dev: device array [Nz][2][dimR][Nx/2] projected in place; r_dev: device
+   array of 3 doubles that receives num, den and num/den.  Queued on the
+   library's stream without any allocation or host synchronisation (wait
+   with sc_synchronize).  Its partial sums live in a fixed 65536-double
+   device array per GPU (set up by c_init): grids needing more are refused
+   with SC_EINVAL.  Calls that overlap on one GPU share that array and race. */
+int sc_ddt_project_noalloc(int i_step, sc_complex *v_dev, double *r_dev);
+
 #ifdef __cplusplus
 }
 #endif

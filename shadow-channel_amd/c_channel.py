@@ -87,6 +87,8 @@ _L.sc_primal_dev.restype = _i
 _L.sc_adjoint_dev.argtypes = [_i, _i, _c.c_void_p, _d]
 _L.sc_adjoint_dev.restype = _i
 
+_L.sc_ddt_project_noalloc.argtypes = [_i, _vp, _vp]
+_L.sc_ddt_project_noalloc.restype = _i
 _L.sc_set_members.argtypes = [_i]
 _L.sc_set_members.restype = _i
 _L.sc_members.argtypes = []
@@ -410,6 +412,16 @@ def sc_step_statistics(incre, i_step):
     st = np.zeros((10, c_qpts()))
     _check(_L.sc_step_statistics(int(incre), int(i_step), st.ctypes.data), "sc_step_statistics")
     return st
+
+
+def sc_ddt_project_noalloc(i_step, v_dev, r_dev):
+    """c_ddt_project on device memory without any allocation: v_dev (a
+    DeviceArray or device address of a [Nz][2][dimR][Nx/2] complex128
+    array) is projected in place, r_dev (device address of 3 doubles)
+    receives num, den and num/den.  Asynchronous on the library's stream
+    (sc_synchronize waits for it)."""
+    _check(_L.sc_ddt_project_noalloc(int(i_step), _devptr(v_dev), _devptr(r_dev)),
+           "sc_ddt_project_noalloc")
 
 
 def sc_set_members(n):

@@ -557,7 +557,7 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
         last_error() = "no HIP device";
         return SC_EINIT;
     }
-    if (setup_host(c) != 0) {
+    if (setup_host(c) != 0 || noalloc_setup() != SC_OK) {
         free_all(c);
         return SC_EINIT;
     }

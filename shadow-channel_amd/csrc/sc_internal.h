@@ -274,6 +274,9 @@ void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, co
 void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, const cplx *M,
                       double s, size_t n, cplx *acc, double a, int nparts, int twice,
                       const double *part, double *scal);
+// the no-allocation entry's scratch array: its device address for the
+// current device, looked up once per device (0 on success)
+int noalloc_setup();
 
 // Reynolds-number source of a tangent stage: Src = TF(ca Y + cb X_pre)
 // (k_re_source: TF the tangent forcing, X_pre the mean solve before its
