@@ -187,6 +187,19 @@ int sc_adjoint_dev(int start_step, int end_step, sc_complex *AC_dev, double stre
 int sc_set_field_cache(int enable);
 int sc_field_cache_active(void);
 
+/* HIP graphs (no counterpart in the reference, which launches from C loops
+   on the CPU).  With sc_set_graphs(1), a stepping loop of c_primal,
+   c_tangent, c_adjoint and their device and member forms is recorded as a
+   HIP graph on its first call with a given signature (entry point, step
+   range, member layout, sources, forcing strength, cached-field layout) and
+   replayed on later calls with the same signature: the host no longer
+   issues thousands of launches per window.  The CFL test still stops a
+   primal run (its flag is read after the replay instead of every 64
+   steps).  sc_set_graphs(0) (the default) launches kernel by kernel and
+   drops the recorded graphs; kernel timing (sc_prof_enable) also bypasses
+   them.  Results are identical either way. */
+int sc_set_graphs(int enable);
+
 /* Members (no counterpart in the reference: its drivers run one solution per
    process, drivers/wrapper.py:14-30 loops over them).  sc_set_members(n)
    makes the next c_init allocate n members (1 .. 32, default 1): n copies

@@ -105,6 +105,14 @@ _L.sc_mean_profile.argtypes = [_i, _i, _i, _d, _vp, _vp]
 _L.sc_mean_profile.restype = _i
 _L.sc_step_statistics.argtypes = [_i, _i, _vp]
 _L.sc_step_statistics.restype = _i
+_L.sc_tangent_sources.argtypes = [_i, _i, _vp, _i, _i]
+_L.sc_tangent_sources.restype = _i
+_L.sc_adjoint_objective.argtypes = [_i, _i, _vp, _d]
+_L.sc_adjoint_objective.restype = _i
+_L.sc_gradient_euclid.argtypes = [_vp, _vp]
+_L.sc_gradient_euclid.restype = _i
+_L.sc_set_graphs.argtypes = [_i]
+_L.sc_set_graphs.restype = _i
 
 SC_EINVAL = -3
 SC_ENOTREADY = -2
@@ -459,3 +467,41 @@ def sc_member_state(incre, member, i_step):
     _check(_L.sc_member_state(int(incre), int(member), int(i_step), out.ctypes.data),
            "sc_member_state")
     return out
+
+
+def _coef_array(a, name):
+    a = _inplace(a, np.complex128, name)
+    shape = (_L.c_Nz(), 2, _L.c_dimR(), _L.c_Nx() // 2)
+    if a.shape != shape:
+        raise TypeError("%s must have shape %s" % (name, shape))
+    return a
+
+
+def sc_tangent_sources(start_step, end_step, IC, inhomo, re_source):
+    """c_tangent with the mean-velocity source (inhomo) and/or the
+    Reynolds-number source (re_source), in place on IC."""
+    IC = _coef_array(IC, "IC")
+    _check(_L.sc_tangent_sources(int(start_step), int(end_step), IC.ctypes.data, int(inhomo),
+                                 int(re_source)), "sc_tangent_sources")
+
+
+def sc_adjoint_objective(start_step, end_step, AC, weight):
+    """Adjoint of weight * (sum of the lower-wall shear over steps end_step ..
+    start_step - 1), in place on AC (the exact transpose of the tangent)."""
+    AC = _coef_array(AC, "AC")
+    _check(_L.sc_adjoint_objective(int(start_step), int(end_step), AC.ctypes.data,
+                                   float(weight)), "sc_adjoint_objective")
+
+
+def sc_gradient_euclid(lam):
+    """g with sum Re(conj(g) d) = <lam, d> in the physical inner product."""
+    lam = np.ascontiguousarray(lam, dtype=np.complex128)
+    g = np.zeros_like(lam)
+    _check(_L.sc_gradient_euclid(lam.ctypes.data, g.ctypes.data), "sc_gradient_euclid")
+    return g
+
+
+def sc_set_graphs(enable):
+    """Record each stepping loop once as a HIP graph and replay it on later
+    calls with the same arguments (1), or launch kernel by kernel (0)."""
+    _check(_L.sc_set_graphs(int(enable)), "sc_set_graphs")

@@ -90,6 +90,47 @@ void prof_end(Context &c, hipStream_t s, int id, const std::function<void()> &re
     c.ev_used += 2;
 }
 
+static void drop_graphs(Context &c)
+{
+    for (auto &g : c.graphs) (void) hipGraphExecDestroy(g.exec);
+    c.graphs.clear();
+}
+
+// The launches of a run's stepping loop (loop(capturing) queues them on the
+// library's stream).  With graphs on (and no kernel timing), the loop is
+// captured into a HIP graph at the first call of its signature `key` and
+// the instantiated graph is replayed afterwards; the loop then must not
+// synchronise while capturing.
+static int stepping(Context &c, const std::vector<double> &key,
+                    const std::function<int(bool)> &loop)
+{
+    if (!c.use_graphs || c.prof_class) return loop(false);
+    for (auto &g : c.graphs)
+        if (g.key == key) {
+            SC_HIP(hipGraphLaunch(g.exec, c.stream));
+            return SC_OK;
+        }
+    hipGraph_t graph = nullptr;
+    SC_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    const int rc = loop(true);
+    const hipError_t e = hipStreamEndCapture(c.stream, &graph);
+    if (rc != SC_OK || e != hipSuccess) {
+        if (graph) (void) hipGraphDestroy(graph);
+        if (rc == SC_OK) last_error() = std::string("graph capture: ") + hipGetErrorString(e);
+        return rc != SC_OK ? rc : SC_EDEVICE;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void) hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        last_error() = std::string("graph instantiation: ") + hipGetErrorString(ei);
+        return SC_EDEVICE;
+    }
+    c.graphs.push_back({key, exec});
+    SC_HIP(hipGraphLaunch(exec, c.stream));
+    return SC_OK;
+}
+
 // whether c_init allocates the transformed-field cache (sc_set_field_cache)
 static int g_field_cache = 1;
 // members c_init allocates (sc_set_members)
@@ -149,8 +190,11 @@ static void free_all(Context &c)
             if (c.dAdd[a][k]) (void) hipFree(c.dAdd[a][k]);
         }
     for (hipEvent_t e : c.ev_pool) (void) hipEventDestroy(e);
+    for (auto &g : c.graphs) (void) hipGraphExecDestroy(g.exec);
     if (c.stream) (void) hipStreamDestroy(c.stream);
+    const bool graphs = c.use_graphs;
     c = Context();
+    c.use_graphs = graphs;
 }
 
 static int dims_ok(const Context &c, int Nz, int Nvar, int Ny, int Nx)
@@ -723,36 +767,49 @@ static int primal_run(int ru_steps, sc_complex *C_given, hipMemcpyKind in, hipMe
     // each CFL check clears the maxima it read; a run starts from zero
     SC_HIP(hipMemsetAsync(c.cflmax, 0, sizeof(double) * c.d.qpts, c.stream));
     c.status = 0;
-    std::fill(c.wvalid.begin(), c.wvalid.end(), 0);
-    const int total = d.nsteps + ru_steps;
-    for (int n = 0; n < total; ++n) {
-        for (int k = 0; k < 3; ++k) {
-            // the stage works on stored state j = 3(n - ru) + k; its fields
-            // equal those of rebuild(MC[j]) except for state 0 after a
-            // laminar start or a run-up (MC[0] is the state before either)
-            const long j = (long) (n - ru_steps) * 3 + k;
-            cplx *w = (j > 0 || (j == 0 && given && ru_steps == 0)) ? wslot(c, j) : nullptr;
-            pass1_state(c, c.U, k == 0, w, (int) std::max<long>(1, (long) (n - ru_steps) * 3 + 1));
-            if (w) c.wvalid[j] = 1;
-            long count = (long) (n - ru_steps) * 3 + k + 1;
-            cplx *Cs = count >= 0 ? mc(c, count) : c.Csol;
-            launch_force(c, c.U, 0);
-            launch_solve(c, c.C, Cs, k, 0, c.Fa, c.Fb, d.flux);
-            launch_rebuild(c, Cs, c.U);
-        }
-        if ((n & 63) == 63) {
-            int flag = 0;
-            SC_HIP(hipMemcpy(&flag, c.abort_flag, sizeof(int), hipMemcpyDeviceToHost));
-            if (flag) break;
-        }
+    // the stage working on stored state j = 3(n - ru) + k writes the cached
+    // fields of that state (its fields equal those of rebuild(MC[j])), except
+    // for state 0 after a laminar start or a run-up (MC[0] is the state
+    // before either)
+    {
+        std::vector<char> valid(c.wvalid.size(), 0);
+        for (size_t j = 0; j < valid.size(); ++j)
+            valid[j] = wslot(c, (long) j) && (j > 0 || (given && ru_steps == 0));
+        if (valid != c.wvalid) drop_graphs(c);
+        c.wvalid = valid;
     }
-    int rc = stream_sync(c);
+    const int total = d.nsteps + ru_steps;
+    const std::vector<double> key = {0, (double) ru_steps, (double) given, (double) c.nrun};
+    int rc = stepping(c, key, [&](bool capturing) {
+        for (int n = 0; n < total; ++n) {
+            for (int k = 0; k < 3; ++k) {
+                const long j = (long) (n - ru_steps) * 3 + k;
+                cplx *w = (j >= 0 && j < (long) c.wvalid.size() && c.wvalid[j]) ? wslot(c, j)
+                                                                                 : nullptr;
+                pass1_state(c, c.U, k == 0, w,
+                            (int) std::max<long>(1, (long) (n - ru_steps) * 3 + 1));
+                long count = (long) (n - ru_steps) * 3 + k + 1;
+                cplx *Cs = count >= 0 ? mc(c, count) : c.Csol;
+                launch_force(c, c.U, 0);
+                launch_solve(c, c.C, Cs, k, 0, c.Fa, c.Fb, d.flux);
+                launch_rebuild(c, Cs, c.U);
+            }
+            if ((n & 63) == 63 && !capturing) {
+                int flag = 0;
+                SC_HIP(hipMemcpy(&flag, c.abort_flag, sizeof(int), hipMemcpyDeviceToHost));
+                if (flag) break;
+            }
+        }
+        return SC_OK;
+    });
+    if (!rc) rc = stream_sync(c);
     if (!rc) rc = check_kernel_error(c);
     if (rc) return rc;
     // after an abort the queued launches did nothing: no cached field is
     // valid, and the states from the stopped step on were not written
     if (check_abort(c, "Pass1")) {
         std::fill(c.wvalid.begin(), c.wvalid.end(), 0);
+        drop_graphs(c);
         int slot = 1;
         SC_HIP(hipMemcpy(&slot, c.abort_flag + 1, sizeof(int), hipMemcpyDeviceToHost));
         const long last = 3L * d.nsteps;
@@ -809,6 +866,10 @@ static int tangent_run(int start, int end, sc_complex *IC_given, unsigned inhomo
     SC_HIP(hipMemsetAsync(c.cflmax, 0, sizeof(double) * c.d.qpts, c.stream));
     launch_rebuild(c, c.IC, c.IU);
     c.status = 0;
+    const std::vector<double> key = {1, (double) start, (double) end, (double) c.nrun,
+                                     (double) c.shared_base, (double) inhomo_mask,
+                                     (double) re_source};
+    int rc = stepping(c, key, [&](bool capturing) {
     for (int n = start; n < end; ++n) {
         for (int k = 0; k < 3; ++k) {
             const cplx *w = wcached(c, 3L * n + k);
@@ -827,13 +888,15 @@ static int tangent_run(int start, int end, sc_complex *IC_given, unsigned inhomo
                          re_source ? c.Ctmp2 : nullptr);
             launch_rebuild(c, Cs, c.IU);
         }
-        if (((n - start) & 63) == 63) {
+        if (((n - start) & 63) == 63 && !capturing) {
             int flag = 0;
             SC_HIP(hipMemcpy(&flag, c.abort_flag, sizeof(int), hipMemcpyDeviceToHost));
             if (flag) break;
         }
     }
-    int rc = stream_sync(c);
+    return SC_OK;
+    });
+    if (!rc) rc = stream_sync(c);
     if (!rc) rc = check_kernel_error(c);
     if (rc) return rc;
     check_abort(c, "Pass1");
@@ -917,19 +980,24 @@ static int adjoint_run(int start, int end, sc_complex *AC_given, double strength
     SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
     launch_rebuild(c, c.AC, c.AU, true);      // lambda: velocity slots only
     const double s = strength * d.dt;
-    for (int n = start; n > end; --n) {
-        // The projection of MIC[3n] at step n was made at the end of the
-        // previous iteration (the reference computes it twice), which also
-        // applied it for this step's start.
-        if (n == start && ddt_project_dev(c, n, mic(c, 3L * n), c.Ctmp, c.AC, 0.5 * s))
-            return SC_EDEVICE;
-        adjoint_stages(c, n, 0);
-        // forcing of MIC[3(n-1)]: end of this step and, when there is a next
-        // step, its start (the two subtractions back to back)
-        if (ddt_project_dev(c, n - 1, mic(c, 3L * (n - 1)), c.Ctmp, c.AC, 0.5 * s, n - 1 > end))
-            return SC_EDEVICE;
-    }
-    int rc = stream_sync(c);
+    const std::vector<double> key = {2, (double) start, (double) end, (double) c.nrun,
+                                     (double) c.shared_base, s};
+    int rc = stepping(c, key, [&](bool) {
+        for (int n = start; n > end; --n) {
+            // The projection of MIC[3n] at step n was made at the end of the
+            // previous iteration (the reference computes it twice), which also
+            // applied it for this step's start.
+            if (n == start && ddt_project_dev(c, n, mic(c, 3L * n), c.Ctmp, c.AC, 0.5 * s))
+                return SC_EDEVICE;
+            adjoint_stages(c, n, 0);
+            // forcing of MIC[3(n-1)]: end of this step and, when there is a
+            // next step, its start (the two subtractions back to back)
+            if (ddt_project_dev(c, n - 1, mic(c, 3L * (n - 1)), c.Ctmp, c.AC, 0.5 * s, n - 1 > end))
+                return SC_EDEVICE;
+        }
+        return SC_OK;
+    });
+    if (!rc) rc = stream_sync(c);
     if (!rc) rc = check_kernel_error(c);
     if (rc) return rc;
     SC_HIP(hipMemcpy(AC_given, c.AC, cbm, out));
@@ -1226,6 +1294,14 @@ int c_nsteps(void) { return ctx().d.nsteps; }
 
 // ---------------------------------------------------------------- members
 extern "C" {
+
+int sc_set_graphs(int on)
+{
+    Context &c = ctx();
+    c.use_graphs = on != 0;
+    if (!c.use_graphs) drop_graphs(c);
+    return SC_OK;
+}
 
 int sc_set_members(int n)
 {

@@ -104,6 +104,16 @@ struct Context {
     bool shared_base = false;
     std::vector<MemberArray> marrays;
 
+    // HIP graphs of the runs' stepping loops (sc_set_graphs): one per run
+    // signature, captured at its first call and replayed after; dropped when
+    // the trajectory's cached fields change or the context is freed
+    bool use_graphs = false;
+    struct GraphEntry {
+        std::vector<double> key;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+
     // host copies of the setup (also used for the laminar start)
     std::vector<double> y, w, cfl2, Kx, Kz;
     std::vector<double> Q, Qp, Qpp, R, Rp, Qw, Qpw, Qppw, Rw, Rpw;
