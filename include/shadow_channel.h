@@ -71,13 +71,37 @@ int c_spec2phys(sc_complex *C_ptr, int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_
                 double *flow_ptr, int Nd_flow, int Nx_flow, int Ny_flow, int Nz_flow);
 
 /* channel/channel.i:172-203 -> read_solution / save_solution
-   (channel/channel.c:791-1028).  Datasets alpha, beta, ialpha, ibeta as
-   [dimR][Nz][Nx/2] complex plus Nx, Ny, Nz, dt, Re, mpg, in the container
-   format described in INTEGRATION.md. */
+   (channel/channel.c:791-1028).  HDF5 files in the reference's layout,
+   written and read without libhdf5: /data_alpha, /data_beta, /data_ialpha,
+   /data_ibeta as [dimR][Nz][Nx/2] compound {double real; double imaginary}
+   and data_Nx, data_Ny (= 2 qpts / 3), data_Nz (int32), data_dt, data_Re
+   (= 1/Re: the reference stores its global re), data_mpg (float64) of
+   shape [1].  ialpha/ibeta hold the library's
+   tangent state (the last tangent's final state), which read_solution
+   also restores.  SC_EIO with sc_last_error() on an unreadable file or a
+   grid other than c_init's. */
 int c_read_solution(const char *filename, sc_complex *C_ptr,
                     int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_dup);
 int c_save_solution(const char *filename, sc_complex *C_ptr,
                     int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_dup);
+
+/* Host-side HDF5 access (no counterpart in the reference, which calls
+   libhdf5; no device work and no c_init needed).  sc_h5_dataset_shape:
+   rank of a root-group dataset (dims[0 .. rank-1] filled, kind 0 integer,
+   1 floating point, 2 complex) or a negative SC_* code.  sc_h5_read_doubles:
+   the dataset converted to double (complex as re, im pairs) into out of
+   cap doubles; returns the count or a negative SC_* code.  Files of the
+   libhdf5 family the reference writes: superblock 0 or 1 (after a
+   userblock or not), symbol-table groups, version-1 object headers,
+   contiguous or compact data; no compression or chunking. */
+int sc_h5_dataset_shape(const char *path, const char *dataset, long long *dims, int max_rank,
+                        int *kind);
+long long sc_h5_read_doubles(const char *path, const char *dataset, double *out, long long cap);
+/* save_solution's file from host arrays C and IC [Nz][2][Ny-2][Nx/2] of the
+   grid (Nx, Ny, Nz) with dt and Re, without a c_init (same datasets as
+   c_save_solution; qpts = 3 Ny / 2 as in c_init). */
+int sc_h5_write_solution(const char *path, int Nx, int Ny, int Nz, double dt, double Re,
+                         const sc_complex *C, const sc_complex *IC);
 
 /* channel/channel.h:17-18 -> tangent_forcing0 / tangent_forcing
    (channel/tangent_forcing0.c:10-44, channel/tangent_forcing.c:10-75):

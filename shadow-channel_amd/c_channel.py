@@ -111,6 +111,13 @@ _L.sc_adjoint_objective.argtypes = [_i, _i, _vp, _d]
 _L.sc_adjoint_objective.restype = _i
 _L.sc_gradient_euclid.argtypes = [_vp, _vp]
 _L.sc_gradient_euclid.restype = _i
+_L.sc_h5_dataset_shape.argtypes = [_c.c_char_p, _c.c_char_p, _c.POINTER(_c.c_longlong), _i,
+                                   _ip]
+_L.sc_h5_dataset_shape.restype = _i
+_L.sc_h5_read_doubles.argtypes = [_c.c_char_p, _c.c_char_p, _vp, _c.c_longlong]
+_L.sc_h5_read_doubles.restype = _c.c_longlong
+_L.sc_h5_write_solution.argtypes = [_c.c_char_p, _i, _i, _i, _d, _d, _vp, _vp]
+_L.sc_h5_write_solution.restype = _i
 _L.sc_set_graphs.argtypes = [_i]
 _L.sc_set_graphs.restype = _i
 
@@ -505,3 +512,35 @@ def sc_set_graphs(enable):
     """Record each stepping loop once as a HIP graph and replay it on later
     calls with the same arguments (1), or launch kernel by kernel (0)."""
     _check(_L.sc_set_graphs(int(enable)), "sc_set_graphs")
+
+
+def h5_read(path, dataset):
+    """A root-group dataset of an HDF5 file as a numpy array of its shape
+    (complex128 for the reference's {real, imaginary} compound, float64
+    otherwise), read by the library without libhdf5."""
+    dims = (_c.c_longlong * 32)()
+    kind = _c.c_int()
+    rank = _L.sc_h5_dataset_shape(str(path).encode(), dataset.encode(), dims, 32,
+                                  _c.byref(kind))
+    _check(rank if rank < 0 else 0, "sc_h5_dataset_shape")
+    shape = tuple(int(d) for d in dims[:rank])
+    n = int(np.prod(shape)) * (2 if kind.value == 2 else 1)
+    out = np.zeros(n)
+    got = _L.sc_h5_read_doubles(str(path).encode(), dataset.encode(), out.ctypes.data, n)
+    _check(int(got) if got < 0 else 0, "sc_h5_read_doubles")
+    if kind.value == 2:
+        return out.view(np.complex128).reshape(shape)
+    return out.reshape(shape)
+
+
+def h5_write_solution(path, Nx, Ny, Nz, dt, Re, C, IC):
+    """save_solution's HDF5 file from host arrays C and IC [Nz][2][Ny-2][Nx/2]
+    (no c_init needed)."""
+    shape = (int(Nz), 2, int(Ny) - 2, int(Nx) // 2)
+    C = np.ascontiguousarray(C, dtype=np.complex128)
+    IC = np.ascontiguousarray(IC, dtype=np.complex128)
+    if C.shape != shape or IC.shape != shape:
+        raise TypeError("C and IC must have shape %s" % (shape,))
+    _check(_L.sc_h5_write_solution(str(path).encode(), int(Nx), int(Ny), int(Nz), float(dt),
+                                   float(Re), C.ctypes.data, IC.ctypes.data),
+           "sc_h5_write_solution")

@@ -1150,35 +1150,23 @@ int c_spec2phys(sc_complex *C_ptr, int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_
     return SC_OK;
 }
 
-// Container for save/read_solution: magic "SCSOL001", int32 Nx, Ny, Nz, dimR,
-// float64 dt, Re, mpg, then alpha, beta, ialpha, ibeta as [dimR][Nz][Nx/2]
-// complex128 (the dataset layout of channel/channel.c:889-1028).
-static const char MAGIC[8] = {'S', 'C', 'S', 'O', 'L', '0', '0', '1'};
-
+// save_solution / read_solution (channel/channel.c:791-1028) in the
+// reference's HDF5 layout (h5file.cpp): C from the caller, IC from the
+// library's tangent state (the reference's global IC: the last tangent's
+// final state); read_solution fills the caller's C and the tangent state.
+// data_mpg is 0 as in the reference's channel module, which never sets mpg.
 int c_save_solution(const char *filename, sc_complex *C_ptr, int Nz_dup, int Nvar_dup, int Ny_dup,
                     int Nx_dup)
 {
     Context &c = ctx();
     if (!c.ok) return SC_ENOTREADY;
     if (!filename || !C_ptr || !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup)) return SC_EINVAL;
-    const Dims &d = c.d;
     std::vector<cplx> IC(c.csize);
     SC_HIP(hipMemcpy(IC.data(), c.IC, sizeof(cplx) * c.csize, hipMemcpyDeviceToHost));
-    std::ofstream f(filename, std::ios::binary);
-    if (!f) return SC_EIO;
-    int32_t hdr[4] = {d.Nx, d.Ny, d.Nz, d.dimR};
-    double par[3] = {d.dt, d.re, 0.0};
-    f.write(MAGIC, 8);
-    f.write((const char *) hdr, sizeof(hdr));
-    f.write((const char *) par, sizeof(par));
-    const cplx *src[2] = {(const cplx *) C_ptr, IC.data()};
-    for (int a = 0; a < 2; ++a)
-        for (int v = 0; v < 2; ++v)
-            for (int y = 0; y < d.dimR; ++y)
-                for (int z = 0; z < d.Nz; ++z)
-                    f.write((const char *) (src[a] + (((size_t) z * 2 + v) * d.dimR + y) * d.Mx),
-                            sizeof(cplx) * d.Mx);
-    return f ? SC_OK : SC_EIO;
+    std::string err;
+    const int rc = h5_write_solution(filename, c.d, 0.0, (const cplx *) C_ptr, IC.data(), err);
+    if (rc != SC_OK) last_error() = err;
+    return rc;
 }
 
 int c_read_solution(const char *filename, sc_complex *C_ptr, int Nz_dup, int Nvar_dup, int Ny_dup,
@@ -1187,30 +1175,14 @@ int c_read_solution(const char *filename, sc_complex *C_ptr, int Nz_dup, int Nva
     Context &c = ctx();
     if (!c.ok) return SC_ENOTREADY;
     if (!filename || !C_ptr || !dims_ok(c, Nz_dup, Nvar_dup, Ny_dup, Nx_dup)) return SC_EINVAL;
-    const Dims &d = c.d;
-    std::ifstream f(filename, std::ios::binary);
-    char magic[8];
-    int32_t hdr[4];
-    double par[3];
-    if (!f.read(magic, 8) || memcmp(magic, MAGIC, 8) != 0 || !f.read((char *) hdr, sizeof(hdr)) ||
-        !f.read((char *) par, sizeof(par))) {
-        last_error() = "not a solution file";
+    std::vector<cplx> C(c.csize), IC(c.csize);
+    std::string err;
+    if (h5_read_solution(filename, c.d, C.data(), IC.data(), err) != SC_OK) {
+        last_error() = err;
         return SC_EIO;
     }
-    if (hdr[0] != d.Nx || hdr[2] != d.Nz || hdr[3] != d.dimR) {
-        last_error() = "solution file grid does not match c_init";
-        return SC_EIO;
-    }
-    std::vector<cplx> IC(c.csize);
-    cplx *dst[2] = {(cplx *) C_ptr, IC.data()};
-    for (int a = 0; a < 2; ++a)
-        for (int v = 0; v < 2; ++v)
-            for (int y = 0; y < d.dimR; ++y)
-                for (int z = 0; z < d.Nz; ++z)
-                    if (!f.read((char *) (dst[a] + (((size_t) z * 2 + v) * d.dimR + y) * d.Mx),
-                                sizeof(cplx) * d.Mx))
-                        return SC_EIO;
-    SC_HIP(hipMemcpy(c.C, C_ptr, sizeof(cplx) * c.csize, hipMemcpyHostToDevice));
+    memcpy(C_ptr, C.data(), sizeof(cplx) * c.csize);
+    SC_HIP(hipMemcpy(c.C, C.data(), sizeof(cplx) * c.csize, hipMemcpyHostToDevice));
     SC_HIP(hipMemcpy(c.IC, IC.data(), sizeof(cplx) * c.csize, hipMemcpyHostToDevice));
     return SC_OK;
 }

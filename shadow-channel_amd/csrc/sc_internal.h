@@ -288,6 +288,12 @@ void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, co
 // current device, looked up once per device (0 on success)
 int noalloc_setup();
 
+// solution files in the reference's HDF5 layout (h5file.cpp); host arrays
+// C, IC of [Nz][2][dimR][Nx/2]; SC_OK or SC_EIO with err set
+int h5_write_solution(const std::string &path, const Dims &d, double mpg, const cplx *C,
+                      const cplx *IC, std::string &err);
+int h5_read_solution(const std::string &path, const Dims &d, cplx *C, cplx *IC, std::string &err);
+
 // Reynolds-number source of a tangent stage: Src = TF(ca Y + cb X_pre)
 // (k_re_source: TF the tangent forcing, X_pre the mean solve before its
 // flux correction)

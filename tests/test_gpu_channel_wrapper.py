@@ -125,6 +125,12 @@ def test_save_read_round_trip(run, tmp_path):
     base = os.path.join(str(tmp_path), "state")
     ch.save_solution(base, C)
     assert os.path.exists(base + ".hd5")
+    # the reference's HDF5 datasets, read back on the host
+    import c_channel
+    alpha = c_channel.h5_read(base + ".hd5", "/data_alpha")
+    assert np.array_equal(alpha, np.transpose(C[:, 0], (1, 0, 2)))
+    assert c_channel.h5_read(base + ".hd5", "data_Nx").tolist() == [C.shape[3] * 2]
+    assert c_channel.h5_read(base + ".hd5", "data_Nz").tolist() == [C.shape[0]]
     back = ch.read_solution(base)
     assert np.array_equal(back, C)
     assert np.array_equal(ch.spec2phys(base), ch.spec2phys(C))

@@ -3,7 +3,9 @@
 * sc_tangent_sources (mean-velocity and Reynolds-number sources) and
   sc_adjoint_objective against the oracle's orc_tangent_src and
   orc_adjoint_objective (pinned in tests/test_oracle_gradient.py), 16x33x12,
-  at the one-step bar of tests/test_gpu_parity.py (1e-10).
+  at the one-step bar of tests/test_gpu_parity.py (1e-10), or 3x the
+  rounding floor where that is higher (the Reynolds-number tangent from
+  zero).
 * sc_gradient_euclid: sum Re(conj(g) d) = <lam, d> in the physical inner
   product formed from spec2phys fields, for directions with mean, kx = 0
   and Nyquist-row content.
@@ -51,12 +53,18 @@ def test_sources_and_objective_adjoint_against_oracle(cc):
     C0 = _st(c, 0, amp=0.1)
     o.primal(0, C0.copy())
     cc.c_primal(0, C0.copy())
-    for inhomo, re_src, seed in ((0, 1, None), (1, 1, 2), (1, 0, None)):
+    # bounds: 3x the rounding floor (tests/test_oracle_rounding_floor.py),
+    # 1e-10 where that is lower; the Reynolds-number tangent from zero is a
+    # small difference of O(1) stage terms (floor 8.1e-10)
+    for inhomo, re_src, seed, bound in ((0, 1, None, 2.5e-9), (1, 1, 2, 1e-10),
+                                        (1, 0, None, 1e-10)):
         d = np.zeros_like(C0) if seed is None else _st(c, seed, unit=True)
         Io, Ig = d.copy(), d.copy()
         o.tangent_src(0, n, Io, inhomo, re_src)
         cc.sc_tangent_sources(0, n, Ig, inhomo, re_src)
-        assert rel(Ig, Io) <= 1e-10, (inhomo, re_src)
+        err = rel(Ig, Io)
+        print("inhomo %d re_source %d: rel err %.2e" % (inhomo, re_src, err), flush=True)
+        assert err <= bound, (inhomo, re_src, err)
     A0 = _st(c, 4, unit=True)
     A0[0, 0, :, 0] += np.linspace(0.1, -0.2, A0.shape[2])   # a mean column as well
     Ao, Ag = A0.copy(), A0.copy()
