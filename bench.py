@@ -5,8 +5,18 @@ One step of the workload is one primal RK3/CN step followed by one adjoint
 step (strength 1, with the two trapezoid ddt_project forcings per step), the
 unit the least-squares-shadowing drivers repeat per time step
 (drivers/wrapper.py:14-30).  Timed: K primal steps then the K adjoint steps
-back over the same window, with the state resident in device memory; the
+back over the same window, with the states resident in device memory; the
 host only launches.  Prints one JSON line.
+
+The headline value is the batched mixed-precision rate: 8 independent
+members (seeded states) advanced together in every kernel launch, with the
+x/z transforms and nonlinear products in FP32 (solves, y-transforms, flux
+correction and projections FP64), in member-steps per second.  Beside it:
+the plain FP64 single-solution rate (the metric's denominator), the FP64
+batched rate, the rate without the transformed-field cache, and the wall-
+shear gradient of a 1000-step window (checkpoints every 50) in mixed
+precision against FP64 and the FP64 gradient against fourth-order central
+differences (both untimed).
 
 Box: Lx = 2 pi, Lz = pi, Re = 3000, dt = 0.004, meanU = 2/3 (flux 4/3),
 synthetic seeded state (laminar profile + k^-2 perturbation, amplitude 0.02).
@@ -67,12 +77,17 @@ def fwd_adj_members(cc, Cd, ACd, nsteps):
     cc.sc_adjoint_members(nsteps, 0, ACd, 1.0, False)
 
 
-def member_rate(cc, states, K, W):
+def member_rate(cc, states, K, W, mixed, roofline_class=0, replays=0):
     """fwd+adj member-steps per second of len(states) independent members
-    advanced together: W untimed steps, then K timed."""
+    advanced together (mixed: FP32 transforms): W untimed steps, then K
+    timed.  With roofline_class, one more fwd+adj step of the same members
+    records that kernel class's launches and the last one is replayed cold:
+    (rate, elapsed, roofline tuple or None)."""
     c = CFG
     nm = len(states)
     cc.sc_set_members(nm)
+    cc.sc_set_precision(mixed)
+    roof = None
     try:
         if W > 0:
             assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
@@ -102,21 +117,78 @@ def member_rate(cc, states, K, W):
         cc.sc_synchronize()
         elapsed = time.perf_counter() - t0
         C, AC = Cd.download(), ACd.download()
+        assert cc.last_status() == 0 and np.all(np.isfinite(C)) and np.all(np.isfinite(AC))
         Cd.free()
         ACd.free()
-        assert cc.last_status() == 0 and np.all(np.isfinite(C)) and np.all(np.isfinite(AC))
         cc.c_destroy()
+        if roofline_class:
+            # one fwd+adj step of the same members records the class's
+            # launches; the last is re-issued `replays` times, each after a
+            # cache-evicting fill whose own time is subtracted
+            assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
+                             c["dt"], 1) == 0
+            Cd = cc.DeviceArray(np.stack(states))
+            ACd = cc.DeviceArray(np.zeros((nm,) + states[0].shape, complex))
+            cc.sc_prof_enable(roofline_class)
+            fwd_adj_members(cc, Cd, ACd, 1)
+            n, ms, nbytes = cc.sc_prof_read()
+            avg_ms, bytes1 = cc.sc_prof_replay_cold(replays, FLUSH_MB)
+            cc.sc_prof_enable(0)
+            Cd.free()
+            ACd.free()
+            cc.c_destroy()
+            roof = (n, ms, nbytes, avg_ms, bytes1)
     finally:
         cc.sc_set_members(1)
-    return nm * K / elapsed, elapsed
+        cc.sc_set_precision(0)
+    return nm * K / elapsed, elapsed, roof
+
+
+def gradient_errors(cc):
+    """Wall-shear gradient of a 1000-step window (checkpoints every 50) at the
+    benchmark grid from the seed-0 state: mixed precision against FP64 (J,
+    dJ/dmeanU, dJ/dRe, dJ/dC0), and the FP64 adjoint along a unit direction
+    against a fourth-order central difference at 1e-3 of the state's size
+    (the chaotic growth of rounding differences over 1000 steps makes
+    smaller steps noisy)."""
+    from channel.gradient import WallShearWindow
+    from channel.synthetic import synthetic_state
+    c = CFG
+    N, meanU = 1000, c["flux"] / 2
+    C0 = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=0)
+    d = unit_direction(C0.shape, 9)
+    out = {}
+    for mixed in (0, 1):
+        cc.sc_set_precision(mixed)
+        win = WallShearWindow(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["dt"], N, every=50)
+        try:
+            J, cps, dmu, dre = win.forward(C0, meanU, c["Re"], params=True)
+            g = win.state_gradient(cps, meanU, c["Re"])
+            fd = None
+            if not mixed:
+                f = lambda t: win.forward(C0 + t * d, meanU, c["Re"], params=False)[0]
+                h = 1e-3 * np.linalg.norm(C0)
+                fd = (8 * (f(h) - f(-h)) - (f(2 * h) - f(-2 * h))) / (12 * h)
+            out[mixed] = (J, dmu, dre, g, fd)
+        finally:
+            win.release()
+            cc.sc_set_precision(0)
+    (J0, mu0, re0, g0, fd), (J1, mu1, re1, g1, _) = out[0], out[1]
+    gd = float(np.sum(g0.real * d.real + g0.imag * d.imag))
+    rel = lambda a, b: float(abs(a - b) / abs(b))
+    return {"window_steps": N, "checkpoint_every": 50,
+            "mixed_vs_fp64": {"J": rel(J1, J0), "dJ_dmeanU": rel(mu1, mu0),
+                              "dJ_dRe": rel(re1, re0),
+                              "dJ_dC0": float(np.linalg.norm(g1 - g0) / np.linalg.norm(g0))},
+            "fp64_vs_fd": {"dJ_dC0_along_unit_direction": rel(gd, fd)}}
 
 
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-# (scaled by a copy of known size), per kernel class of the roofline replay
+# (scaled by a copy of known size), per kernel class of the roofline replay in
+# the headline configuration (8 members, mixed precision)
 FLUSH_MB = 320       # above the 256 MB Infinity Cache
 
-TRAFFIC_PROFILES = {2: "profiles/xmix_adjoint_64x64x97_pmc.md",
-                    6: "profiles/solve_64x64x97_pmc.md"}
+TRAFFIC_PROFILES = {6: "profiles/solve_8members_mixed_64x64x97_pmc.md"}
 
 
 def measured_traffic(kernel_class):
@@ -147,7 +219,16 @@ def cpu_baseline():
     t = time.perf_counter() - t0
     L.orc_set_faithful_cost(0)
     o.destroy()
+    model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": 1.0 / t, "unit": "steps/s", "cores": 1, "kind": "port",
+            "cpu_model": model, "nproc": os.cpu_count(),
             "sample": "1 primal + 1 adjoint step of the 64x64x97 workload, oracle (FP64 "
                       "restatement, single thread, incl. the reference's discarded "
                       "tangent/incremental-adjoint transforms), %.1f s" % t}
@@ -180,6 +261,8 @@ def run():
     ap.add_argument("--roofline-class", type=int, default=6)
     ap.add_argument("--replays", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-gradient", action="store_true",
+                    help="skip the untimed gradient-error check")
     ap.add_argument("--members", type=int, default=8,
                     help="independent members advanced together in the headline figure")
     args = ap.parse_args()
@@ -194,65 +277,64 @@ def run():
     K, W = args.steps, args.warmup
     C0 = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=0)
 
-    # warm-up: a window of W steps
-    if W > 0:
-        assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
-                         c["dt"], W) == 0
-        fwd_adj(cc, C0, W)
-        cc.c_destroy()
-    assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
-                     c["dt"], K) == 0
-    log("timing %d fwd+adj steps" % K)
-    # untimed: the trajectory and a tangent along it (the adjoint's forcing);
-    # the timed primal recomputes the same trajectory
-    Cd = cc.DeviceArray(C0)
-    cc.sc_primal_dev(0, Cd)
-    Cd.upload(C0)
-    cc.c_tangent(0, K, unit_direction(C0.shape, 100), 0)
-    # inputs resident in HBM before the timed region
-    ACd = cc.DeviceArray(np.zeros_like(C0))
-    cc.sc_synchronize()
-    t0 = time.perf_counter()
-    fwd_adj_dev(cc, Cd, ACd, K)
-    cc.sc_synchronize()
-    elapsed = time.perf_counter() - t0
-    C, AC = Cd.download(), ACd.download()
-    Cd.free()
-    ACd.free()
-    assert cc.last_status() == 0 and np.all(np.isfinite(C)) and np.all(np.isfinite(AC))
-    # the same window through the host-array interface (PCIe transfers of
-    # the states included), reported beside the resident figure
-    cc.sc_synchronize()
-    t0 = time.perf_counter()
-    Ch, ACh = fwd_adj(cc, C0, K)
-    elapsed_h = time.perf_counter() - t0
-    assert np.array_equal(Ch, C) and np.array_equal(ACh, AC)
-    cc.c_destroy()
+    def single_rate(field_cache):
+        """plain FP64, one solution: device-resident fwd+adj rate (and the
+        host-array rate with the cache on)"""
+        cc.sc_set_field_cache(field_cache)
+        try:
+            if W > 0:
+                assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"],
+                                 c["flux"], c["dt"], W) == 0
+                fwd_adj(cc, C0, W)
+                cc.c_destroy()
+            assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
+                             c["dt"], K) == 0
+            assert cc.sc_field_cache_active() == field_cache
+            # untimed: the trajectory and a tangent along it (the adjoint's
+            # forcing); the timed primal recomputes the same trajectory
+            Cd = cc.DeviceArray(C0)
+            cc.sc_primal_dev(0, Cd)
+            Cd.upload(C0)
+            cc.c_tangent(0, K, unit_direction(C0.shape, 100), 0)
+            ACd = cc.DeviceArray(np.zeros_like(C0))
+            cc.sc_synchronize()
+            t0 = time.perf_counter()
+            fwd_adj_dev(cc, Cd, ACd, K)
+            cc.sc_synchronize()
+            elapsed = time.perf_counter() - t0
+            C, AC = Cd.download(), ACd.download()
+            Cd.free()
+            ACd.free()
+            assert cc.last_status() == 0 and np.all(np.isfinite(C)) and np.all(np.isfinite(AC))
+            host = None
+            if field_cache:
+                # the same window through the host-array interface (PCIe
+                # transfers of the states included)
+                cc.sc_synchronize()
+                t0 = time.perf_counter()
+                Ch, ACh = fwd_adj(cc, C0, K)
+                host = K / (time.perf_counter() - t0)
+                assert np.array_equal(Ch, C) and np.array_equal(ACh, AC)
+            cc.c_destroy()
+        finally:
+            cc.sc_set_field_cache(1)
+        return K / elapsed, host
 
-    # the headline: members advanced together (each its own seeded state)
+    log("timing plain FP64 fwd+adj, %d steps" % K)
+    plain, host_rate = single_rate(1)
+    log("timing plain FP64 without the field cache")
+    nocache, _ = single_rate(0)
     nm = args.members
-    log("timing %d members x %d fwd+adj steps" % (nm, K))
     states = [synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=s)
               for s in range(nm)]
-    mrate, melapsed = member_rate(cc, states, K, W)
-    plain = K / elapsed
-
-    # roofline of one kernel class: one fwd+adj step records its launches,
-    # then the last of them is re-issued back to back between two events on
-    # the library's stream (the timed window above runs without profiling)
-    log("roofline: kernel class %d, %d replays" % (args.roofline_class, args.replays))
-    assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"],
-                     c["dt"], 1) == 0
-    cc.sc_prof_enable(args.roofline_class)
-    fwd_adj(cc, C0, 1)
-    n, ms, nbytes = cc.sc_prof_read()
-    # the step's last launch of the class re-issued, each time after a fill
-    # of 320 MB that evicts the caches as the other kernels of a step do (a
-    # back-to-back replay would find the latency-bound kernels' data in the
-    # Infinity Cache); the fills' own time, measured alone, is subtracted
-    avg_ms, bytes1 = cc.sc_prof_replay_cold(args.replays, FLUSH_MB)
-    cc.sc_prof_enable(0)
-    cc.c_destroy()
+    log("timing %d FP64 members x %d fwd+adj steps" % (nm, K))
+    rate64, _, _ = member_rate(cc, states, K, W, 0)
+    # the headline: members in mixed precision, and the roofline of one
+    # kernel class in that same configuration
+    log("timing %d mixed-precision members x %d fwd+adj steps (+ roofline class %d)" % (
+        nm, K, args.roofline_class))
+    mrate, melapsed, rf = member_rate(cc, states, K, W, 1, args.roofline_class, args.replays)
+    n, ms, nbytes, avg_ms, bytes1 = rf
     achieved = bytes1 / (avg_ms * 1e-3) / 1e9
     traffic = measured_traffic(args.roofline_class)
     roof = {"kernel": CLASS_NAMES.get(args.roofline_class, str(args.roofline_class)),
@@ -260,28 +342,40 @@ def run():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic, "avg_launch_us": round(1e3 * avg_ms, 2),
             "bytes_per_launch": round(bytes1), "launches_per_step": n,
+            "members_per_launch": nm,
             "timing": "HIP events on the library's stream around %d launches of the step's "
                       "last one, each after a %d MB cache-evicting fill (fill time measured "
                       "alone and subtracted)" % (args.replays, FLUSH_MB),
             "step_avg_launch_us": round(1e3 * ms / n, 2),
-            "step_bytes_per_launch": round(nbytes / n)}
+            "step_bytes_per_launch": round(nbytes / n),
+            "step_avg_frac": round((nbytes / n) / (ms / n * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    grad = None
+    if not args.no_gradient:
+        log("gradient errors: 1000-step window, mixed vs FP64, FP64 vs finite differences")
+        grad = gradient_errors(cc)
     line = {
         "metric": METRIC, "value": round(mrate, 3), "unit": "steps/s",
         "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(1e3 * melapsed / K, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": "fwd+adj step: 1 primal RK3/CN step + 1 adjoint step "
                                "(strength 1) per member, %d independent members advanced "
-                               "together in every kernel launch, states resident in HBM; "
+                               "together in every kernel launch, mixed precision (x/z "
+                               "transforms and products FP32; solves, y-transforms, flux "
+                               "correction, projections FP64), states resident in HBM; "
                                "value = member-steps/s" % nm,
                    "grid": "64x64x97 (96x145x96 dealiased)", "Lx": "2pi", "Lz": "pi",
                    "Re": c["Re"], "dt": c["dt"], "meanU": c["flux"] / 2,
-                   "members": nm, "window_steps": K, "precision": "fp64",
+                   "members": nm, "window_steps": K, "precision": "mixed",
                    "plain_fp64_steps_per_s": round(plain, 3),
                    "ratio_vs_plain_fp64": round(mrate / plain, 3),
-                   "host_array_steps_per_s": round(K / elapsed_h, 3)},
+                   "fp64_members_steps_per_s": round(rate64, 3),
+                   "plain_fp64_no_field_cache_steps_per_s": round(nocache, 3),
+                   "host_array_steps_per_s": round(host_rate, 3)},
         "roofline": roof,
     }
+    if grad is not None:
+        line["config"]["gradient_errors"] = grad
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, 1 thread)")
         line["cpu_baseline"] = cpu_baseline()

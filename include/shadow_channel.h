@@ -224,6 +224,17 @@ int sc_field_cache_active(void);
    them.  Results are identical either way. */
 int sc_set_graphs(int enable);
 
+/* Precision (no counterpart in the reference, which is FP64 throughout).
+   sc_set_precision(1) makes the next c_init a mixed-precision context: the
+   x/z transforms of the time steps, the nonlinear products and the arrays
+   between the transforms (and the cached fields, at half the memory) in
+   FP32; the y-transforms, the band solves, the mean-mode flux correction,
+   the trapezoid projection and every state array stay FP64.  0 (default):
+   FP64 everywhere.  spec2phys is FP64 in both.  sc_precision() reports the
+   current context's setting. */
+int sc_set_precision(int mixed);
+int sc_precision(void);
+
 /* Members (no counterpart in the reference: its drivers run one solution per
    process, drivers/wrapper.py:14-30 loops over them).  sc_set_members(n)
    makes the next c_init allocate n members (1 .. 32, default 1): n copies

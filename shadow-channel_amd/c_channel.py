@@ -118,6 +118,10 @@ _L.sc_h5_read_doubles.argtypes = [_c.c_char_p, _c.c_char_p, _vp, _c.c_longlong]
 _L.sc_h5_read_doubles.restype = _c.c_longlong
 _L.sc_h5_write_solution.argtypes = [_c.c_char_p, _i, _i, _i, _d, _d, _vp, _vp]
 _L.sc_h5_write_solution.restype = _i
+_L.sc_set_precision.argtypes = [_i]
+_L.sc_set_precision.restype = _i
+_L.sc_precision.argtypes = []
+_L.sc_precision.restype = _i
 _L.sc_set_graphs.argtypes = [_i]
 _L.sc_set_graphs.restype = _i
 
@@ -544,3 +548,13 @@ def h5_write_solution(path, Nx, Ny, Nz, dt, Re, C, IC):
     _check(_L.sc_h5_write_solution(str(path).encode(), int(Nx), int(Ny), int(Nz), float(dt),
                                    float(Re), C.ctypes.data, IC.ctypes.data),
            "sc_h5_write_solution")
+
+
+def sc_set_precision(mixed):
+    """Precision of the next c_init: 0 FP64, 1 mixed (transforms and products
+    of the time steps in FP32; solves, y-transforms and states FP64)."""
+    _check(_L.sc_set_precision(int(mixed)), "sc_set_precision")
+
+
+def sc_precision():
+    return _L.sc_precision()

@@ -135,6 +135,8 @@ static int stepping(Context &c, const std::vector<double> &key,
 static int g_field_cache = 1;
 // members c_init allocates (sc_set_members)
 static int g_members = 1;
+// precision of the next c_init's time steps (sc_set_precision)
+static int g_mixed = 0;
 
 // ---------------------------------------------------------------- rand()
 // glibc random() TYPE_3: r_i = r_{i-3} + r_{i-31} (mod 2^32) after the
@@ -469,6 +471,18 @@ int sc_set_field_cache(int enable)
 
 int sc_field_cache_active(void) { return ctx().wcache != nullptr; }
 
+int sc_set_precision(int mixed)
+{
+    if (mixed != 0 && mixed != 1) {
+        last_error() = "precision: 0 (FP64) or 1 (mixed)";
+        return SC_EINVAL;
+    }
+    g_mixed = mixed;
+    return SC_OK;
+}
+
+int sc_precision(void) { return ctx().mixed ? 1 : 0; }
+
 int sc_prof_enable(int kernel_class)
 {
     Context &c = ctx();
@@ -580,6 +594,7 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
         last_error() = "grid requires Nx % 4 == 0, Nz % 2 == 0, Ny >= 4, nsteps >= 0";
         return SC_EINIT;
     }
+    c.mixed = g_mixed != 0;
     Dims &d = c.d;
     d.Nx = Nx; d.Ny = Ny; d.Nz = Nz; d.Lx = Lx; d.Lz = Lz; d.dt = dt; d.nsteps = nstep;
     d.flux = flux;
@@ -689,7 +704,9 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
     // transformed-field cache of the stored trajectory, if it fits in half of
     // the free memory (otherwise the tangent and adjoint recompute)
     {
-        const size_t per = (size_t) d.qpts * 6 * d.Pz * d.Mx;
+        // six fields per stored stage: FP64 complex, or FP32 complex (half the
+        // bytes) when the transforms run in FP32
+        const size_t per = (size_t) d.qpts * 6 * d.Pz * d.Mx / (c.mixed ? 2 : 1);
         const size_t need = sizeof(cplx) * per * 3 * (size_t) nstep;   // one member
         size_t freeb = 0, totalb = 0;
         if (g_field_cache && need > 0 && hipMemGetInfo(&freeb, &totalb) == hipSuccess &&
@@ -1142,8 +1159,9 @@ int c_spec2phys(sc_complex *C_ptr, int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_
         fs.kind[s] = FK_SLOT;
         fs.slot[s] = s;
     }
-    launch_zinv(c, fs, c.W1);
-    launch_xmix(c, XM_PHYS, c.W1, nullptr, 0, c.flowdev);
+    // physical fields in FP64 whatever the precision of the time steps
+    fp64::launch_zinv(c, fs, c.W1);
+    fp64::launch_xmix(c, XM_PHYS, c.W1, nullptr, 0, c.flowdev, nullptr, 0);
     SC_HIP(hipMemcpyAsync(flow_ptr, c.flowdev, sizeof(double) * 3 * (size_t) d.Px * d.qpts * d.Pz,
                           hipMemcpyDeviceToHost, c.stream));
     SC_HIP(hipStreamSynchronize(c.stream));

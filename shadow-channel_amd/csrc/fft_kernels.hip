@@ -19,7 +19,39 @@
 // barrier.
 #include "sc_internal.h"
 
+// The file is compiled twice: as is (sc::fp64, FP64 transforms) and with
+// SC_FFT_F32 defined (sc::fp32: transforms, products and the intermediate
+// arrays W1, W2 and the cached fields in FP32).  The state arrays read at
+// the start (U, FP64) and the results written at the end (the Gauss-point
+// slots, FP64) are FP64 in both; values are converted on load and store.
+#ifdef SC_FFT_F32
+#define SC_FFT_NS fp32
+#else
+#define SC_FFT_NS fp64
+#endif
+
 namespace sc {
+namespace SC_FFT_NS {
+
+#ifdef SC_FFT_F32
+using real = float;
+using real2 = float2;
+#else
+using real = double;
+using real2 = double2;
+#endif
+
+__host__ __device__ __forceinline__ real2 mk2(real a, real b)
+{
+    real2 v;
+    v.x = a;
+    v.y = b;
+    return v;
+}
+__device__ __forceinline__ real2 to_real2(const double2 &v) { return mk2((real) v.x, (real) v.y); }
+__device__ __forceinline__ double2 to_cplx(const real2 &v) { return make_double2(v.x, v.y); }
+// member stride of an array viewed as real2 values
+static inline size_t rstride(const Context &c, const void *p) { return mstride_bytes(c, p) / sizeof(real2); }
 
 constexpr int BLK = FFT_BLOCK;   // threads per block of the transform kernels
 // complex values staged in LDS by one x-kernel block, data and twiddles
@@ -31,7 +63,7 @@ struct PlanDev {
     // per pass: ceil(2^32 / Ns), Ns = product of the earlier radices, so that
     // x / Ns = (x * nsm) >> 32 exactly for x <= 8192, Ns <= 4096
     unsigned long long nsm[MAX_FACTORS];
-    const double2 *tw;
+    const double2 *tw;         // the plan's FP64 twiddles (converted when staged)
 };
 
 static PlanDev to_dev(const FftPlan &p)
@@ -77,66 +109,66 @@ struct Walk {
     }
 };
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+__device__ __forceinline__ real2 cmul(real2 a, real2 b)
 {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return mk2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
 // In-place DFTs in registers: v[q] <- sum_m v[m] exp(sign 2 pi i m q / R).
-template <int R> __device__ __forceinline__ void dft(double2 *v, double sign);
+template <int R> __device__ __forceinline__ void dft(real2 *v, real sign);
 
-template <> __device__ __forceinline__ void dft<2>(double2 *v, double sign)
+template <> __device__ __forceinline__ void dft<2>(real2 *v, real sign)
 {
-    double2 a = v[0], b = v[1];
-    v[0] = make_double2(a.x + b.x, a.y + b.y);
-    v[1] = make_double2(a.x - b.x, a.y - b.y);
+    real2 a = v[0], b = v[1];
+    v[0] = mk2(a.x + b.x, a.y + b.y);
+    v[1] = mk2(a.x - b.x, a.y - b.y);
 }
 
-template <> __device__ __forceinline__ void dft<4>(double2 *v, double sign)
+template <> __device__ __forceinline__ void dft<4>(real2 *v, real sign)
 {
-    double2 a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
-    double2 s02 = make_double2(a0.x + a2.x, a0.y + a2.y);
-    double2 d02 = make_double2(a0.x - a2.x, a0.y - a2.y);
-    double2 s13 = make_double2(a1.x + a3.x, a1.y + a3.y);
-    double2 d13 = make_double2(a1.x - a3.x, a1.y - a3.y);
+    real2 a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
+    real2 s02 = mk2(a0.x + a2.x, a0.y + a2.y);
+    real2 d02 = mk2(a0.x - a2.x, a0.y - a2.y);
+    real2 s13 = mk2(a1.x + a3.x, a1.y + a3.y);
+    real2 d13 = mk2(a1.x - a3.x, a1.y - a3.y);
     // w d13 with w = sign * i
-    double2 wd = make_double2(-sign * d13.y, sign * d13.x);
-    v[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
-    v[1] = make_double2(d02.x + wd.x, d02.y + wd.y);
-    v[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
-    v[3] = make_double2(d02.x - wd.x, d02.y - wd.y);
+    real2 wd = mk2(-sign * d13.y, sign * d13.x);
+    v[0] = mk2(s02.x + s13.x, s02.y + s13.y);
+    v[1] = mk2(d02.x + wd.x, d02.y + wd.y);
+    v[2] = mk2(s02.x - s13.x, s02.y - s13.y);
+    v[3] = mk2(d02.x - wd.x, d02.y - wd.y);
 }
 
-template <> __device__ __forceinline__ void dft<3>(double2 *v, double sign)
+template <> __device__ __forceinline__ void dft<3>(real2 *v, real sign)
 {
-    const double h = 0.86602540378443864676;   // sqrt(3)/2
-    double2 a0 = v[0], a1 = v[1], a2 = v[2];
-    double2 t1 = make_double2(a1.x + a2.x, a1.y + a2.y);
-    double2 t2 = make_double2(a0.x - 0.5 * t1.x, a0.y - 0.5 * t1.y);
-    double2 t3 = make_double2(sign * h * (a1.x - a2.x), sign * h * (a1.y - a2.y));
-    v[0] = make_double2(a0.x + t1.x, a0.y + t1.y);
-    v[1] = make_double2(t2.x - t3.y, t2.y + t3.x);
-    v[2] = make_double2(t2.x + t3.y, t2.y - t3.x);
+    const real h = 0.86602540378443864676;   // sqrt(3)/2
+    real2 a0 = v[0], a1 = v[1], a2 = v[2];
+    real2 t1 = mk2(a1.x + a2.x, a1.y + a2.y);
+    real2 t2 = mk2(a0.x - (real) 0.5 * t1.x, a0.y - (real) 0.5 * t1.y);
+    real2 t3 = mk2(sign * h * (a1.x - a2.x), sign * h * (a1.y - a2.y));
+    v[0] = mk2(a0.x + t1.x, a0.y + t1.y);
+    v[1] = mk2(t2.x - t3.y, t2.y + t3.x);
+    v[2] = mk2(t2.x + t3.y, t2.y - t3.x);
 }
 
-template <> __device__ __forceinline__ void dft<5>(double2 *v, double sign)
+template <> __device__ __forceinline__ void dft<5>(real2 *v, real sign)
 {
-    const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
-    const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
-    double2 a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3], a4 = v[4];
-    double2 b1 = make_double2(a1.x + a4.x, a1.y + a4.y);
-    double2 b2 = make_double2(a2.x + a3.x, a2.y + a3.y);
-    double2 d1 = make_double2(a1.x - a4.x, a1.y - a4.y);
-    double2 d2 = make_double2(a2.x - a3.x, a2.y - a3.y);
-    double2 t1 = make_double2(a0.x + c1 * b1.x + c2 * b2.x, a0.y + c1 * b1.y + c2 * b2.y);
-    double2 t2 = make_double2(a0.x + c2 * b1.x + c1 * b2.x, a0.y + c2 * b1.y + c1 * b2.y);
-    double2 u1 = make_double2(sign * (s1 * d1.x + s2 * d2.x), sign * (s1 * d1.y + s2 * d2.y));
-    double2 u2 = make_double2(sign * (s2 * d1.x - s1 * d2.x), sign * (s2 * d1.y - s1 * d2.y));
-    v[0] = make_double2(a0.x + b1.x + b2.x, a0.y + b1.y + b2.y);
-    v[1] = make_double2(t1.x - u1.y, t1.y + u1.x);
-    v[4] = make_double2(t1.x + u1.y, t1.y - u1.x);
-    v[2] = make_double2(t2.x - u2.y, t2.y + u2.x);
-    v[3] = make_double2(t2.x + u2.y, t2.y - u2.x);
+    const real c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+    const real s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+    real2 a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3], a4 = v[4];
+    real2 b1 = mk2(a1.x + a4.x, a1.y + a4.y);
+    real2 b2 = mk2(a2.x + a3.x, a2.y + a3.y);
+    real2 d1 = mk2(a1.x - a4.x, a1.y - a4.y);
+    real2 d2 = mk2(a2.x - a3.x, a2.y - a3.y);
+    real2 t1 = mk2(a0.x + c1 * b1.x + c2 * b2.x, a0.y + c1 * b1.y + c2 * b2.y);
+    real2 t2 = mk2(a0.x + c2 * b1.x + c1 * b2.x, a0.y + c2 * b1.y + c1 * b2.y);
+    real2 u1 = mk2(sign * (s1 * d1.x + s2 * d2.x), sign * (s1 * d1.y + s2 * d2.y));
+    real2 u2 = mk2(sign * (s2 * d1.x - s1 * d2.x), sign * (s2 * d1.y - s1 * d2.y));
+    v[0] = mk2(a0.x + b1.x + b2.x, a0.y + b1.y + b2.y);
+    v[1] = mk2(t1.x - u1.y, t1.y + u1.x);
+    v[4] = mk2(t1.x + u1.y, t1.y - u1.x);
+    v[2] = mk2(t2.x - u2.y, t2.y + u2.x);
+    v[3] = mk2(t2.x + u2.y, t2.y - u2.x);
 }
 
 // cos and sin of 2 pi j / R, j < R (correctly rounded; exact where the value
@@ -175,13 +207,13 @@ template <> struct RootTab<16> {
 // DFT_R = (R1 DFTs of length R2) . twiddles exp(sign 2 pi i m2 k1 / R) .
 // (R2 DFTs of length R1).  The twiddles are compile-time constants.
 template <int R1, int R2>
-__device__ __forceinline__ void dft_comp(double2 *v, double sign)
+__device__ __forceinline__ void dft_comp(real2 *v, real sign)
 {
     constexpr int R = R1 * R2;
-    double2 a[R];
+    real2 a[R];
 #pragma unroll
     for (int m2 = 0; m2 < R2; ++m2) {
-        double2 t[R1];
+        real2 t[R1];
 #pragma unroll
         for (int m1 = 0; m1 < R1; ++m1) t[m1] = v[R2 * m1 + m2];
         dft<R1>(t, sign);
@@ -191,14 +223,14 @@ __device__ __forceinline__ void dft_comp(double2 *v, double sign)
             if (e == 0) {
                 a[m2 * R1 + k1] = t[k1];
             } else {
-                const double2 w = make_double2(RootTab<R>::c[e], sign * RootTab<R>::s[e]);
+                const real2 w = mk2((real) RootTab<R>::c[e], sign * (real) RootTab<R>::s[e]);
                 a[m2 * R1 + k1] = cmul(t[k1], w);
             }
         }
     }
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) {
-        double2 t[R2];
+        real2 t[R2];
 #pragma unroll
         for (int m2 = 0; m2 < R2; ++m2) t[m2] = a[m2 * R1 + k1];
         dft<R2>(t, sign);
@@ -207,13 +239,13 @@ __device__ __forceinline__ void dft_comp(double2 *v, double sign)
     }
 }
 
-template <> __device__ __forceinline__ void dft<6>(double2 *v, double s) { dft_comp<3, 2>(v, s); }
-template <> __device__ __forceinline__ void dft<8>(double2 *v, double s) { dft_comp<4, 2>(v, s); }
-template <> __device__ __forceinline__ void dft<9>(double2 *v, double s) { dft_comp<3, 3>(v, s); }
-template <> __device__ __forceinline__ void dft<10>(double2 *v, double s) { dft_comp<5, 2>(v, s); }
-template <> __device__ __forceinline__ void dft<12>(double2 *v, double s) { dft_comp<4, 3>(v, s); }
-template <> __device__ __forceinline__ void dft<15>(double2 *v, double s) { dft_comp<5, 3>(v, s); }
-template <> __device__ __forceinline__ void dft<16>(double2 *v, double s) { dft_comp<4, 4>(v, s); }
+template <> __device__ __forceinline__ void dft<6>(real2 *v, real s) { dft_comp<3, 2>(v, s); }
+template <> __device__ __forceinline__ void dft<8>(real2 *v, real s) { dft_comp<4, 2>(v, s); }
+template <> __device__ __forceinline__ void dft<9>(real2 *v, real s) { dft_comp<3, 3>(v, s); }
+template <> __device__ __forceinline__ void dft<10>(real2 *v, real s) { dft_comp<5, 2>(v, s); }
+template <> __device__ __forceinline__ void dft<12>(real2 *v, real s) { dft_comp<4, 3>(v, s); }
+template <> __device__ __forceinline__ void dft<15>(real2 *v, real s) { dft_comp<5, 3>(v, s); }
+template <> __device__ __forceinline__ void dft<16>(real2 *v, real s) { dft_comp<4, 4>(v, s); }
 
 // One Stockham pass of radix R over nseq sequences of length n, in place in
 // LDS: element e of sequence s at data[s*sstride + e*estride].  Lanes run
@@ -224,14 +256,14 @@ template <> __device__ __forceinline__ void dft<16>(double2 *v, double s) { dft_
 // threads transforms at most FFT_TILE complex values, so a thread owns at most
 // ceil(FFT_TILE / (BLK R)) butterflies.
 template <int R>
-__device__ __forceinline__ void fft_pass(double2 *data, int nseq, int sstride, int estride, int n,
-                                         int Ns, unsigned long long nsm, const double2 *tw,
-                                         double sign)
+__device__ __forceinline__ void fft_pass(real2 *data, int nseq, int sstride, int estride, int n,
+                                         int Ns, unsigned long long nsm, const real2 *tw,
+                                         real sign)
 {
     constexpr int NB = (FFT_TILE / BLK + R - 1) / R;
     const int nb = n / R;
     const int twstep = magic_div(nb, nsm);      // n / (Ns R)
-    double2 v[NB][R];
+    real2 v[NB][R];
     int off[NB];                                // output position of the butterfly
     Walk w(nseq);                               // w.q = butterfly index j, w.r = sequence s
 #pragma unroll
@@ -239,14 +271,14 @@ __device__ __forceinline__ void fft_pass(double2 *data, int nseq, int sstride, i
         const int j = w.q, s = w.r;
         off[i] = -1;
         if (j < nb) {
-            const double2 *src = data + s * sstride;
+            const real2 *src = data + s * sstride;
             const int jq = magic_div(j, nsm), k = j - jq * Ns;
             off[i] = s * sstride + (jq * Ns * R + k) * estride;
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                double2 a = src[(j + q * nb) * estride];
+                real2 a = src[(j + q * nb) * estride];
                 if (q > 0) {
-                    double2 t = tw[q * k * twstep];
+                    real2 t = tw[q * k * twstep];
                     t.y *= sign;
                     a = cmul(a, t);
                 }
@@ -260,7 +292,7 @@ __device__ __forceinline__ void fft_pass(double2 *data, int nseq, int sstride, i
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
         if (off[i] >= 0) {
-            double2 *dst = data + off[i];
+            real2 *dst = data + off[i];
 #pragma unroll
             for (int q = 0; q < R; ++q) dst[q * Ns * estride] = v[i][q];
         }
@@ -269,8 +301,8 @@ __device__ __forceinline__ void fft_pass(double2 *data, int nseq, int sstride, i
 }
 
 // tw: the plan's twiddles exp(2 pi i m / n), m < n, staged in LDS
-__device__ __forceinline__ void lds_fft(double2 *data, int nseq, int sstride, int estride,
-                                        const PlanDev &p, const double2 *tw, double sign)
+__device__ __forceinline__ void lds_fft(real2 *data, int nseq, int sstride, int estride,
+                                        const PlanDev &p, const real2 *tw, real sign)
 {
     int Ns = 1;
     for (int pass = 0; pass < p.nf; ++pass) {
@@ -291,9 +323,9 @@ __device__ __forceinline__ void lds_fft(double2 *data, int nseq, int sstride, in
 }
 
 // copy the plan's twiddle table into LDS (visible after the caller's barrier)
-__device__ __forceinline__ void stage_twiddles(const PlanDev &p, double2 *twl)
+__device__ __forceinline__ void stage_twiddles(const PlanDev &p, real2 *twl)
 {
-    for (int m = threadIdx.x; m < p.n; m += BLK) twl[m] = p.tw[m];
+    for (int m = threadIdx.x; m < p.n; m += BLK) twl[m] = to_real2(p.tw[m]);
 }
 
 // ---------------------------------------------------------------- fixed lengths
@@ -333,13 +365,13 @@ constexpr int plan_radix(int n, int p)
 
 // one pass over NSEQ sequences, element e of sequence s at s*SSTR + e*ESTR
 template <int R, int N, int NS, int NSEQ, int SSTR, int ESTR>
-__device__ __forceinline__ void fft_pass_c(double2 *data, const double2 *tw, double sign)
+__device__ __forceinline__ void fft_pass_c(real2 *data, const real2 *tw, real sign)
 {
     constexpr int NBS = N / R;                       // butterflies per sequence
     constexpr int TOTAL = NSEQ * NBS;
     constexpr int NBT = (TOTAL + BLK - 1) / BLK;     // butterflies per thread
     constexpr int TWS = N / (NS * R);
-    double2 v[NBT][R];
+    real2 v[NBT][R];
     int off[NBT];
 #pragma unroll
     for (int i = 0; i < NBT; ++i) {
@@ -347,14 +379,14 @@ __device__ __forceinline__ void fft_pass_c(double2 *data, const double2 *tw, dou
         const int j = b / NSEQ, s = b - j * NSEQ;
         off[i] = -1;
         if (b < TOTAL) {
-            const double2 *src = data + s * SSTR;
+            const real2 *src = data + s * SSTR;
             const int jq = j / NS, k = j - jq * NS;
             off[i] = s * SSTR + (jq * NS * R + k) * ESTR;
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                double2 a = src[(j + q * NBS) * ESTR];
+                real2 a = src[(j + q * NBS) * ESTR];
                 if (q > 0 && NS > 1) {
-                    double2 t = tw[q * k * TWS];
+                    real2 t = tw[q * k * TWS];
                     t.y *= sign;
                     a = cmul(a, t);
                 }
@@ -367,7 +399,7 @@ __device__ __forceinline__ void fft_pass_c(double2 *data, const double2 *tw, dou
 #pragma unroll
     for (int i = 0; i < NBT; ++i) {
         if (off[i] >= 0) {
-            double2 *dst = data + off[i];
+            real2 *dst = data + off[i];
 #pragma unroll
             for (int q = 0; q < R; ++q) dst[q * NS * ESTR] = v[i][q];
         }
@@ -376,7 +408,7 @@ __device__ __forceinline__ void fft_pass_c(double2 *data, const double2 *tw, dou
 }
 
 template <int N, int NSEQ, int SSTR, int ESTR, int P = 0, int NS = 1>
-__device__ __forceinline__ void lds_fft_c(double2 *data, const double2 *tw, double sign)
+__device__ __forceinline__ void lds_fft_c(real2 *data, const real2 *tw, real sign)
 {
     if constexpr (P < plan_best(N).passes) {
         constexpr int R = plan_radix(N, P);
@@ -421,7 +453,7 @@ static_assert(MAXE % SBAT == 0, "stage-in batches must tile the per-thread eleme
 // sits behind a branch.
 template <int KIND, int PZ>
 __device__ __forceinline__ void zinv_stage(const ZinvArgs &a, const cplx *src, int slot, int y,
-                                           int x0, int XB, double2 *lds)
+                                           int x0, int XB, real2 *lds)
 {
     // PZ > 0: the tile is ZXB x PZ and every thread stages NE values in one batch
     constexpr int NE = PZ > 0 ? (ZXB * PZ + BLK - 1) / BLK : MAXE;
@@ -467,6 +499,7 @@ __device__ __forceinline__ void zinv_stage(const ZinvArgs &a, const cplx *src, i
             const double kx = xi[i] * a.ax;
             const int kz = kzi[i];
             const double kzv = (kz <= Nz / 2 ? kz : kz - Nz) * a.az;
+            // formed in FP64 from the FP64 state, then stored as real
             double2 val;
             if (KIND == FK_SLOT) {
                 val = p[i];
@@ -480,7 +513,7 @@ __device__ __forceinline__ void zinv_stage(const ZinvArgs &a, const cplx *src, i
                 const double2 u = p[i], w = q[i];
                 val = make_double2(kx * w.y - kzv * u.y, kzv * u.x - kx * w.x);
             }
-            lds[e] = ok[i] ? val : make_double2(0.0, 0.0);
+            lds[e] = ok[i] ? to_real2(val) : mk2(0.0, 0.0);
         }
     }
 }
@@ -493,7 +526,7 @@ __device__ __forceinline__ void zinv_stage(const ZinvArgs &a, const cplx *src, i
 // are not loaded.
 template <int KIND, int PZ>
 __device__ __forceinline__ void zinv_first_pass(const ZinvArgs &a, const cplx *src, int slot, int y,
-                                                int x0, double2 *lds)
+                                                int x0, real2 *lds)
 {
     constexpr int Nz = 2 * PZ / 3, R1 = plan_radix(PZ, 0), NB1 = PZ / R1, NBUT = ZXB * NB1;
     static_assert(NBUT <= BLK, "one first-pass butterfly per thread");
@@ -503,11 +536,11 @@ __device__ __forceinline__ void zinv_first_pass(const ZinvArgs &a, const cplx *s
     const size_t zs = (size_t) 5 * a.qpts * Mx, ss = (size_t) a.qpts * Mx;
     const cplx *col = src + (size_t) y * Mx + x0 + xl;
     const double kx = (x0 + xl) * a.ax;
-    double2 v[R1];
+    real2 v[R1];
 #pragma unroll
     for (int q = 0; q < R1; ++q) {
         if (q * NB1 >= Nz / 2 && q * NB1 + NB1 - 1 <= Nz) {
-            v[q] = make_double2(0.0, 0.0);
+            v[q] = mk2(0.0, 0.0);
             continue;
         }
         const int zp = j + q * NB1;
@@ -528,10 +561,10 @@ __device__ __forceinline__ void zinv_first_pass(const ZinvArgs &a, const cplx *s
             const double2 u = bp[0], w = bp[2 * ss];
             val = make_double2(kx * w.y - kzv * u.y, kzv * u.x - kx * w.x);
         }
-        v[q] = kz >= 0 ? val : make_double2(0.0, 0.0);
+        v[q] = kz >= 0 ? to_real2(val) : mk2(0.0, 0.0);
     }
     dft<R1>(v, +1.0);
-    double2 *dst = lds + xl + j * R1 * ZXB;
+    real2 *dst = lds + xl + j * R1 * ZXB;
 #pragma unroll
     for (int q = 0; q < R1; ++q) dst[q * ZXB] = v[q];
 }
@@ -539,9 +572,9 @@ __device__ __forceinline__ void zinv_first_pass(const ZinvArgs &a, const cplx *s
 // grid (ceil(Mx/XB), nf * members, qpts): blockIdx.y = member nf + field;
 // PZ > 0: Pz == PZ, XB == ZXB and Mx % ZXB == 0
 template <int PZ>
-__global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, cplx *W1)
+__global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, real2 *W1)
 {
-    extern __shared__ double2 lds[];
+    extern __shared__ real2 lds[];
     if (*a.abort_flag) return;
     const int XB0 = PZ > 0 ? ZXB : a.XB;
     const int nf = a.fs.nf, m = blockIdx.y / nf, f = blockIdx.y - m * nf;
@@ -550,7 +583,7 @@ __global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, cplx *W1)
     const int Pz = PZ > 0 ? PZ : a.Pz, Mx = a.Mx;
     const cplx *src = a.fs.src[f] + m * a.fs.ms[f];
     const int kind = a.fs.kind[f], slot = a.fs.slot[f];
-    double2 *twl = lds + XB0 * Pz;
+    real2 *twl = lds + XB0 * Pz;
     stage_twiddles(pz, twl);
     // columns interleaved in LDS exactly as in memory: no transpose
     if constexpr (PZ > 0) {
@@ -568,7 +601,7 @@ __global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, cplx *W1)
         __syncthreads();
         lds_fft(lds, XB, 1, XB, pz, twl, +1.0);
     }
-    cplx *dst = W1 + m * a.ms_w1 + (((size_t) y * nf + f) * Pz) * Mx;
+    real2 *dst = W1 + m * a.ms_w1 + (((size_t) y * nf + f) * Pz) * Mx;
     Walk wk(XB);
     for (int e = threadIdx.x; e < XB * Pz; e += BLK, wk.next())
         dst[(size_t) wk.q * Mx + x0 + wk.r] = lds[e];
@@ -577,10 +610,10 @@ __global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, cplx *W1)
 struct XmixArgs {
     int mode, nf, nout, Mx, Px, Pz, qpts, ZB, check_cfl;
     int nf1;                   // fields 0..nf1-1 from the first source, the rest from W1b
-    const cplx *W1b;
+    const real2 *W1b;
     int nf2;                   // XM_ADJ2: fields nf1..nf2-1 from W1b, nf2..nf-1 from W1c
-    const cplx *W1c;
-    cplx *W2b;                 // XM_ADJ2: the second product set
+    const real2 *W1c;
+    real2 *W2b;                 // XM_ADJ2: the second product set
     double cfl1, cfl3;
     const double *cfl2, *W;
     double *cflmax;        // [qpts], as ordered bit patterns
@@ -606,9 +639,9 @@ template <> struct XmixShape<XM_ADJ2> { static constexpr int NF = 15, NOUT = 12;
 // Px + 1 (odd, so lanes on neighbouring sequences use different banks),
 // nseq = ceil(nf/2).  PX > 0: Px == PX and ZB == xmix_rows(PX, nseq).
 template <int MODE, int PX>
-__global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx *W1, cplx *W2)
+__global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const real2 *W1, real2 *W2)
 {
-    extern __shared__ double2 lds[];
+    extern __shared__ real2 lds[];
     if (*a.abort_flag) return;
     constexpr int nf = XmixShape<MODE>::NF;
     constexpr int NOUT = XmixShape<MODE>::NOUT;
@@ -628,7 +661,7 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
     const int ZB = PX > 0 ? ZBC : a.ZB;
     const int SP = Px + 1;
     const int z0 = zb * ZB;
-    double2 *twl = lds + ZB * nseq * SP;
+    real2 *twl = lds + ZB * nseq * SP;
     stage_twiddles(px, twl);
     // load packed pairs with Hermitian extension (c2r uses Re only at kx=0):
     // bin n < Mx takes X[n] + i Y[n], bin n > Px - Mx the conjugates at
@@ -654,32 +687,32 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
             const int src = s2 < nf1 ? 0 : s2 < nf2 ? 1 : 2;
             const int f0 = src == 0 ? 0 : src == 1 ? nf1 : nf2;
             const int fn = src == 0 ? nf1 : src == 1 ? nf2 - nf1 : nf - nf2;
-            const cplx *B = src == 0 ? W1 : src == 1 ? a.W1b : a.W1c;
-            const cplx *X = B + ((size_t) y * fn * Pz + z0) * Mx + ((s2 - f0) * Pz + r) * Mx;
+            const real2 *B = src == 0 ? W1 : src == 1 ? a.W1b : a.W1c;
+            const real2 *X = B + ((size_t) y * fn * Pz + z0) * Mx + ((s2 - f0) * Pz + r) * Mx;
             const bool hasy = s2 + 1 < nf;
             const int yo = hasy ? PM : 0;
-            double2 v[R1];
+            real2 v[R1];
 #pragma unroll
             for (int q = 0; q < R1; ++q) {
                 if (q * NB1 >= MX && q * NB1 + NB1 - 1 <= PX - MX) {
-                    v[q] = make_double2(0.0, 0.0);             // zero band for every j
+                    v[q] = mk2(0.0, 0.0);             // zero band for every j
                     continue;
                 }
                 const int n = j + q * NB1;
                 const int k = min(n < MX ? n : PX - n, MX - 1);
-                const double2 xv = X[k], yl = X[yo + k];
-                const double2 yv = hasy ? yl : make_double2(0.0, 0.0);
-                double2 val = make_double2(0.0, 0.0);
+                const real2 xv = X[k], yl = X[yo + k];
+                const real2 yv = hasy ? yl : mk2(0.0, 0.0);
+                real2 val = mk2(0.0, 0.0);
                 if (n == 0)
-                    val = make_double2(xv.x, yv.x);
+                    val = mk2(xv.x, yv.x);
                 else if (n < MX)
-                    val = make_double2(xv.x - yv.y, xv.y + yv.x);
+                    val = mk2(xv.x - yv.y, xv.y + yv.x);
                 else if (n > PX - MX)
-                    val = make_double2(xv.x + yv.y, -xv.y + yv.x);
+                    val = mk2(xv.x + yv.y, -xv.y + yv.x);
                 v[q] = val;
             }
             dft<R1>(v, +1.0);
-            double2 *dst = lds + t * (PX + 1) + j * R1;
+            real2 *dst = lds + t * (PX + 1) + j * R1;
 #pragma unroll
             for (int q = 0; q < R1; ++q) dst[q] = v[q];
         }
@@ -691,7 +724,7 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
 #pragma unroll 1
         for (int i0 = 0; i0 < MAXL; i0 += LB) {
             if ((int) threadIdx.x + i0 * BLK >= tot) break;
-            double2 lx[LB], ly[LB];
+            real2 lx[LB], ly[LB];
             int tn[LB];
 #pragma unroll
             for (int i = 0; i < LB; ++i) {
@@ -701,7 +734,7 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
                 const int s = t % nseq, r = t / nseq;
                 const int k = min(n < Mx ? n : Px - n, Mx - 1);
                 const int f0 = 2 * s, nf1 = a.nf1;
-                const cplx *X = f0 < nf1 ? W1 + (((size_t) y * nf1 + f0) * Pz + z0 + r) * Mx
+                const real2 *X = f0 < nf1 ? W1 + (((size_t) y * nf1 + f0) * Pz + z0 + r) * Mx
                                          : a.W1b + (((size_t) y * (nf - nf1) + f0 - nf1) * Pz + z0 + r) * Mx;
                 const int sy = min(2 * s + 1, nf - 1) - 2 * s;     // 1, or 0 for a lone field
                 lx[i] = X[k];
@@ -714,16 +747,16 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
                 const int t = tn[i] / Px, n = tn[i] - t * Px;
                 const int s = t % nseq, r = t / nseq;
                 const bool hasy = 2 * s + 1 < nf;
-                double2 xv = lx[i], yv = hasy ? ly[i] : make_double2(0.0, 0.0);
-                double2 val;
+                real2 xv = lx[i], yv = hasy ? ly[i] : mk2(0.0, 0.0);
+                real2 val;
                 if (n == 0)
-                    val = make_double2(xv.x, yv.x);
+                    val = mk2(xv.x, yv.x);
                 else if (n < Mx)
-                    val = make_double2(xv.x - yv.y, xv.y + yv.x);
+                    val = mk2(xv.x - yv.y, xv.y + yv.x);
                 else if (n > Px - Mx)
-                    val = make_double2(xv.x + yv.y, -xv.y + yv.x);
+                    val = mk2(xv.x + yv.y, -xv.y + yv.x);
                 else
-                    val = make_double2(0.0, 0.0);
+                    val = mk2(0.0, 0.0);
                 lds[(r * nseq + s) * SP + n] = val;
             }
         }
@@ -739,7 +772,7 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
     constexpr int MAXP = PX > 0 ? (ZBC * PX + BLK - 1) / BLK          // points per thread
                                 : (XMIX_TILE / nseq + BLK - 1) / BLK;
     constexpr int NO = NOUT > 0 ? NOUT : 1;
-    double outv[MAXP][NO];
+    real outv[MAXP][NO];
     double cmax = 0.0;
     const int npt = ZB * Px;
     Walk wp(Px);                                       // (z row r, point n)
@@ -749,12 +782,12 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
         if (e >= npt) break;
         const int n = wp.r, r = wp.q;
         wp.next();
-        const double2 *row = lds + (size_t) r * nseq * SP + n;
-        double f[16];
+        const real2 *row = lds + (size_t) r * nseq * SP + n;
+        real f[16];
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             if (s < nseq) {
-                double2 v = row[s * SP];
+                real2 v = row[s * SP];
                 f[2 * s] = v.x;
                 f[2 * s + 1] = v.y;
             } else {
@@ -762,7 +795,7 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
                 f[2 * s + 1] = 0.0;
             }
         }
-        double u = f[0], v = f[1], w = f[2], oz = f[3], ox = f[4], oy = f[5];
+        real u = f[0], v = f[1], w = f[2], oz = f[3], ox = f[4], oy = f[5];
         if (a.check_cfl) {
             double t = fabs(u) * a.cfl1 + fabs(v) * a.cfl2[y] + fabs(w) * a.cfl3;
             cmax = fmax(cmax, t);
@@ -772,12 +805,12 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
             outv[i][2] = w * ox - u * oz;           // Hy
             outv[i][0] = u * oy - v * ox;           // Hz
         } else if constexpr (MODE == XM_TANGENT) {
-            double Iu = f[6], Iv = f[7], Iw = f[8], Ioz = f[9], Iox = f[10], Ioy = f[11];
+            real Iu = f[6], Iv = f[7], Iw = f[8], Ioz = f[9], Iox = f[10], Ioy = f[11];
             outv[i][1] = Iv * oz - Iw * oy + v * Ioz - w * Ioy;
             outv[i][2] = Iw * ox - Iu * oz + w * Iox - u * Ioz;
             outv[i][0] = Iu * oy - Iv * ox + u * Ioy - v * Iox;
         } else if constexpr (MODE == XM_ADJOINT) {
-            double lx = f[6], ly = f[7], lz = f[8];
+            real lx = f[6], ly = f[7], lz = f[8];
             outv[i][0] = v * lz - w * ly;           // Sx
             outv[i][1] = w * lx - u * lz;           // Sy
             outv[i][2] = u * ly - v * lx;           // Sz
@@ -787,11 +820,11 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
         } else if constexpr (MODE == XM_ADJ2) {
             // the adjoint products of lambda with two states: (u, omega) in
             // f[0..5] and f[6..11], lambda in f[12..14]
-            double lx = f[12], ly = f[13], lz = f[14];
+            real lx = f[12], ly = f[13], lz = f[14];
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
-                const double pu = f[6 * p], pv = f[6 * p + 1], pw = f[6 * p + 2];
-                const double qz = f[6 * p + 3], qx = f[6 * p + 4], qy = f[6 * p + 5];
+                const real pu = f[6 * p], pv = f[6 * p + 1], pw = f[6 * p + 2];
+                const real qz = f[6 * p + 3], qx = f[6 * p + 4], qy = f[6 * p + 5];
                 outv[i][6 * p + 0] = pv * lz - pw * ly;     // Sx
                 outv[i][6 * p + 1] = pw * lx - pu * lz;     // Sy
                 outv[i][6 * p + 2] = pu * ly - pv * lx;     // Sz
@@ -837,9 +870,9 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
 #pragma unroll
         for (int t = 0; t < 6; ++t) {
             if (t < nso) {
-                double re = outv[i][2 * t < NO ? 2 * t : 0];
-                double im = (2 * t + 1 < NOUT) ? outv[i][2 * t + 1 < NO ? 2 * t + 1 : 0] : 0.0;
-                lds[(size_t) (r * nso + t) * SP + n] = make_double2(re, im);
+                real re = outv[i][2 * t < NO ? 2 * t : 0];
+                real im = (2 * t + 1 < NOUT) ? outv[i][2 * t + 1 < NO ? 2 * t + 1 : 0] : (real) 0;
+                lds[(size_t) (r * nso + t) * SP + n] = mk2(re, im);
             }
         }
     }
@@ -853,15 +886,15 @@ __global__ __launch_bounds__(BLK) void k_xmix(XmixArgs a, PlanDev px, const cplx
     Walk wu(Mx);                                       // (packed row, bin k)
     for (int e = threadIdx.x; e < tot2; e += BLK, wu.next()) {
         const int k = wu.r, t = wu.q % nso, r = wu.q / nso;
-        const double2 *Z = lds + (size_t) (r * nso + t) * SP;
-        double2 zk = Z[k], zm = Z[k == 0 ? 0 : Px - k];
-        double2 X = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
-        double2 Yv = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+        const real2 *Z = lds + (size_t) (r * nso + t) * SP;
+        real2 zk = Z[k], zm = Z[k == 0 ? 0 : Px - k];
+        real2 X = mk2((real) 0.5 * (zk.x + zm.x), (real) 0.5 * (zk.y - zm.y));
+        real2 Yv = mk2((real) 0.5 * (zk.y + zm.y), (real) -0.5 * (zk.x - zm.x));
         int zp = z0 + r;
         int g0 = 2 * t, g1 = 2 * t + 1;
         if constexpr (MODE == XM_ADJ2) {
             // pairs t < 3 to W2, the others to W2b, six fields each
-            cplx *D = t < 3 ? W2 : a.W2b;
+            real2 *D = t < 3 ? W2 : a.W2b;
             const int h0 = t < 3 ? g0 : g0 - 6;
             D[(((size_t) y * 6 + h0) * Pz + zp) * Mx + k] = X;
             D[(((size_t) y * 6 + h0 + 1) * Pz + zp) * Mx + k] = Yv;
@@ -889,9 +922,9 @@ struct ZfwdArgs {
 // grid (ceil(Mx/XB), nout * members, qpts): blockIdx.y = member nout + field;
 // PZ > 0: Pz == PZ, XB == ZXB and Mx % ZXB == 0
 template <int PZ>
-__global__ __launch_bounds__(BLK) void k_zfwd(ZfwdArgs a, PlanDev pz, const cplx *W2, cplx *dst)
+__global__ __launch_bounds__(BLK) void k_zfwd(ZfwdArgs a, PlanDev pz, const real2 *W2, cplx *dst)
 {
-    extern __shared__ double2 lds[];
+    extern __shared__ real2 lds[];
     if (a.cfl_slot >= 0 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0)
         cfl_check_block(a.cflmax, a.qpts, a.dt, a.abort_rw, a.cfl_slot);
     if (*a.abort_flag) return;
@@ -900,9 +933,9 @@ __global__ __launch_bounds__(BLK) void k_zfwd(ZfwdArgs a, PlanDev pz, const cplx
     const int x0 = blockIdx.x * XB0, y = blockIdx.z;
     const int XB = PZ > 0 ? ZXB : min(a.XB, a.Mx - x0);
     const int Nz = PZ > 0 ? 2 * PZ / 3 : a.Nz, Pz = PZ > 0 ? PZ : a.Pz, Mx = a.Mx;
-    const cplx *src = W2 + m * a.ms_in + (((size_t) y * a.nout + g) * Pz) * Mx;
+    const real2 *src = W2 + m * a.ms_in + (((size_t) y * a.nout + g) * Pz) * Mx;
     dst += m * a.ms_out;
-    double2 *twl = lds + XB0 * Pz;
+    real2 *twl = lds + XB0 * Pz;
     stage_twiddles(pz, twl);
     {
         // a batch's loads all issued before its LDS stores
@@ -913,7 +946,7 @@ __global__ __launch_bounds__(BLK) void k_zfwd(ZfwdArgs a, PlanDev pz, const cplx
 #pragma unroll 1
         for (int i0 = 0; i0 < NE; i0 += SB) {
             if ((int) threadIdx.x + i0 * BLK >= total) break;
-            double2 v[SB];
+            real2 v[SB];
 #pragma unroll
             for (int i = 0; i < SB; ++i) {
                 const int zp = min(wk.q, Pz - 1), xl = wk.r;
@@ -937,15 +970,15 @@ __global__ __launch_bounds__(BLK) void k_zfwd(ZfwdArgs a, PlanDev pz, const cplx
         const int r = wt.q, xl = wt.r;
         int kz = r < Nz / 2 ? r : r + 1;             // skip the Nyquist row
         int zp = kz < Nz / 2 ? kz : kz + Nz / 2;
-        double2 v = lds[zp * XB + xl];
-        v.x *= a.norm;
-        v.y *= a.norm;
+        real2 v = lds[zp * XB + xl];
+        v.x *= (real) a.norm;
+        v.y *= (real) a.norm;
         size_t o;
         if (a.dst_kind == ZF_USLOTS)
             o = (((size_t) kz * 5 + g) * a.qpts + y) * Mx + x0 + xl;
         else
             o = (((size_t) g * Nz + kz) * a.qpts + y) * Mx + x0 + xl;
-        dst[o] = v;
+        dst[o] = to_cplx(v);
     }
 }
 
@@ -964,7 +997,7 @@ struct ZadjArgs {
     const int *abort_flag;
     // k_zfwd_adj2 with nprob = 2: a second problem (W2b -> Ub) in the same grid
     int nprob;
-    const cplx *W2b;
+    const real2 *W2b;
     cplx *Ub;
     // members: k_zfwd_adj grid z = member; k_zfwd_adj2 tiles of member m
     // follow those of member m - 1 (strides of W2 / W2b and U / Ub)
@@ -973,9 +1006,9 @@ struct ZadjArgs {
 };
 
 template <int PZ>
-__global__ __launch_bounds__(BLK) void k_zfwd_adj(ZadjArgs a, PlanDev pz, const cplx *W2, cplx *U)
+__global__ __launch_bounds__(BLK) void k_zfwd_adj(ZadjArgs a, PlanDev pz, const real2 *W2, cplx *U)
 {
-    extern __shared__ double2 lds[];
+    extern __shared__ real2 lds[];
     if (*a.abort_flag) return;
     const int XA = PZ > 0 ? ZA : a.XA;
     const int x0 = blockIdx.x * XA, y = blockIdx.y;
@@ -984,19 +1017,19 @@ __global__ __launch_bounds__(BLK) void k_zfwd_adj(ZadjArgs a, PlanDev pz, const 
     const int XB = PZ > 0 ? ZA : min(a.XA, a.Mx - x0);   // columns of this block
     const int Nz = PZ > 0 ? 2 * PZ / 3 : a.Nz, Pz = PZ > 0 ? PZ : a.Pz, Mx = a.Mx;
     const int NS = 6 * XA;                               // interleaved sequences
-    double2 *twl = lds + NS * Pz;
+    real2 *twl = lds + NS * Pz;
     stage_twiddles(pz, twl);
     {
         // element (zp, field f, column xl) at (zp * 6 + f) * XA + xl
         const int total = 6 * Pz * XA;
-        const cplx *src = W2 + (size_t) y * 6 * Pz * Mx;
+        const real2 *src = W2 + (size_t) y * 6 * Pz * Mx;
         Walk wk(XA);                                     // (zp * 6 + f, xl)
         constexpr int NE = PZ > 0 ? (6 * ZA * PZ + BLK - 1) / BLK : MAXE;
         constexpr int SB = PZ > 0 ? NE : SBAT;
 #pragma unroll 1
         for (int i0 = 0; i0 < NE; i0 += SB) {
             if ((int) threadIdx.x + i0 * BLK >= total) break;
-            double2 v[SB];
+            real2 v[SB];
 #pragma unroll
             for (int i = 0; i < SB; ++i) {
                 const int q = min(wk.q, 6 * Pz - 1), xl = min(wk.r, XB - 1);
@@ -1023,22 +1056,22 @@ __global__ __launch_bounds__(BLK) void k_zfwd_adj(ZadjArgs a, PlanDev pz, const 
         if (xl >= XB) continue;
         const int kz = r < Nz / 2 ? r : r + 1;           // skip the Nyquist row
         const int zp = kz < Nz / 2 ? kz : kz + Nz / 2;
-        double2 S[6];
+        real2 S[6];
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
             S[f] = lds[(zp * 6 + f) * XA + xl];
-            S[f].x *= a.norm;
-            S[f].y *= a.norm;
+            S[f].x *= (real) a.norm;
+            S[f].y *= (real) a.norm;
         }
         const int x = x0 + xl;
         const double kx = a.Kx[x], kzv = a.Kz[kz];
-        const double2 Sx = S[0], Sy = S[1], Sz = S[2], Tx = S[3], Ty = S[4], Tz = S[5];
+        const real2 Sx = S[0], Sy = S[1], Sz = S[2], Tx = S[3], Ty = S[4], Tz = S[5];
         cplx *b = U + ((size_t) kz * 5 * a.qpts + y) * Mx + x;
         b[0 * ss] = make_double2(-Tz.x + kx * Sy.y, -Tz.y - kx * Sy.x);
         b[1 * ss] = make_double2(-Tx.x - kzv * Sy.y, -Tx.y + kzv * Sy.x);
         b[2 * ss] = make_double2(-Ty.x + kzv * Sx.y - kx * Sz.y, -Ty.y - kzv * Sx.x + kx * Sz.x);
-        b[3 * ss] = Sx;
-        b[4 * ss] = Sz;
+        b[3 * ss] = to_cplx(Sx);
+        b[4 * ss] = to_cplx(Sz);
     }
 }
 
@@ -1052,13 +1085,13 @@ __global__ __launch_bounds__(BLK) void k_zfwd_adj(ZadjArgs a, PlanDev pz, const 
 // column groups of a y share an XCD, so the two halves of a 128-byte line
 // (ZA = 4 columns are 64 bytes) meet in one L2.
 template <int PZ>
-__global__ __launch_bounds__(BLK) void k_zfwd_adj2(ZadjArgs a, PlanDev pz, const cplx *W2, cplx *U)
+__global__ __launch_bounds__(BLK) void k_zfwd_adj2(ZadjArgs a, PlanDev pz, const real2 *W2, cplx *U)
 {
     static_assert(PZ > 0, "fixed lengths only");
     constexpr int Pz = PZ, Nz = 2 * PZ / 3, NS = 3 * ZA, TOT = NS * Pz;
     constexpr int NE = (TOT + BLK - 1) / BLK;
     static_assert(ZA * (Nz - 1) <= BLK, "one output column-row per thread");
-    extern __shared__ double2 lds[];
+    extern __shared__ real2 lds[];
     if (*a.abort_flag) return;
     const int Mx = a.Mx, nxg = Mx / ZA, n1 = nxg * a.qpts;
     int tile = xcd_tile(a.nm * a.nprob * n1);
@@ -1073,18 +1106,18 @@ __global__ __launch_bounds__(BLK) void k_zfwd_adj2(ZadjArgs a, PlanDev pz, const
     W2 += m * a.ms_in;
     U += m * a.ms_out;
     const int y = tile / nxg, x0 = (tile - y * nxg) * ZA;
-    double2 *twl = lds + TOT;
+    real2 *twl = lds + TOT;
     stage_twiddles(pz, twl);
     // element (zp, f, xl) of a phase at (zp * 3 + f) * ZA + xl; phase h
     // holds fields 3h .. 3h + 2 of W2[y]
-    const cplx *src = W2 + (size_t) y * 6 * Pz * Mx;
+    const real2 *src = W2 + (size_t) y * 6 * Pz * Mx;
     const int r = threadIdx.x / ZA, xl = threadIdx.x - r * ZA;
     const bool own = r < Nz - 1;
     const int kz = r < Nz / 2 ? r : r + 1;               // skip the Nyquist row
     const int zp = kz < Nz / 2 ? kz : kz + Nz / 2;
     // phase h: fields 3h .. 3h + 2 into LDS, transform, this thread's values
-    auto phase = [&](int h, double2 *P) {
-        double2 v[NE];
+    auto phase = [&](int h, real2 *P) {
+        real2 v[NE];
 #pragma unroll
         for (int i = 0; i < NE; ++i) {
             const int e = min((int) threadIdx.x + i * BLK, TOT - 1);
@@ -1100,26 +1133,26 @@ __global__ __launch_bounds__(BLK) void k_zfwd_adj2(ZadjArgs a, PlanDev pz, const
         lds_fft_c<PZ, NS, 1, NS>(lds, twl, -1.0);
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
-            P[f] = own ? lds[(zp * 3 + f) * ZA + xl] : make_double2(0.0, 0.0);
-            P[f].x *= a.norm;
-            P[f].y *= a.norm;
+            P[f] = own ? lds[(zp * 3 + f) * ZA + xl] : mk2(0.0, 0.0);
+            P[f].x *= (real) a.norm;
+            P[f].y *= (real) a.norm;
         }
     };
-    double2 S[3], T[3];
+    real2 S[3], T[3];
     phase(0, S);
     __syncthreads();                                     // S read before T is stored
     phase(1, T);
     if (!own) return;
     const int x = x0 + xl;
     const double kx = a.Kx[x], kzv = a.Kz[kz];
-    const double2 Sx = S[0], Sy = S[1], Sz = S[2], Tx = T[0], Ty = T[1], Tz = T[2];
+    const real2 Sx = S[0], Sy = S[1], Sz = S[2], Tx = T[0], Ty = T[1], Tz = T[2];
     const size_t ss = (size_t) a.qpts * Mx;
     cplx *b = U + ((size_t) kz * 5 * a.qpts + y) * Mx + x;
     b[0 * ss] = make_double2(-Tz.x + kx * Sy.y, -Tz.y - kx * Sy.x);
     b[1 * ss] = make_double2(-Tx.x - kzv * Sy.y, -Tx.y + kzv * Sy.x);
     b[2 * ss] = make_double2(-Ty.x + kzv * Sx.y - kx * Sz.y, -Ty.y - kzv * Sx.x + kx * Sz.x);
-    b[3 * ss] = Sx;
-    b[4 * ss] = Sz;
+    b[3 * ss] = to_cplx(Sx);
+    b[4 * ss] = to_cplx(Sz);
 }
 
 // adjoint assembly (channel/pass2.c:474-566):
@@ -1155,8 +1188,8 @@ __global__ void k_assemble_adj(const cplx *SP, cplx *U, const double *Kx, const 
 int xmix_zblock(Context &c, int nf);
 
 // fixed-length instances of the z kernels: Pz of 48 or 96, Nx/2 a multiple of ZXB
-typedef void (*ZinvKernel)(ZinvArgs, PlanDev, cplx *);
-typedef void (*ZfwdKernel)(ZfwdArgs, PlanDev, const cplx *, cplx *);
+typedef void (*ZinvKernel)(ZinvArgs, PlanDev, real2 *);
+typedef void (*ZfwdKernel)(ZfwdArgs, PlanDev, const real2 *, cplx *);
 
 static int z_fixed(const Dims &d)
 {
@@ -1190,11 +1223,11 @@ void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1)
     a.az = 2. * M_PI / d.Lz;
     a.abort_flag = c.abort_flag;
     for (int f = 0; f < fs.nf; ++f) a.fs.ms[f] = mstride(c, fs.src[f]);
-    a.ms_w1 = mstride(c, W1);
+    a.ms_w1 = rstride(c, W1);
     const int nm = c.nrun;
     dim3 grid((d.Mx + a.XB - 1) / a.XB, fs.nf * nm, d.qpts);
-    size_t lds = sizeof(double2) * (a.XB * d.Pz + d.Pz);
-    const double plane = 16.0 * (double) d.qpts * d.Mx;
+    size_t lds = sizeof(real2) * (a.XB * d.Pz + d.Pz);
+    const double plane = (double) d.qpts * d.Mx;
     int slots = 0;
     for (int f = 0; f < fs.nf; ++f) {
         bool seen = false;
@@ -1203,17 +1236,18 @@ void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1)
     }
     if (fs.nf == 9) slots = 8;            // adjoint: 5 state slots + 3 adjoint
     if (fs.nf == 6 && fs.kind[3] == FK_SLOT) slots = 6;   // two velocity triples
-    double bytes = nm * plane * ((double) slots * (d.Nz - 1) + (double) fs.nf * d.Pz);
+    double bytes = nm * plane * (16.0 * slots * (d.Nz - 1) + (double) sizeof(real2) * fs.nf * d.Pz);
     const hipStream_t st = c.stream;
     const PlanDev pd = to_dev(c.pz);
     const ZinvKernel kern = fixed == 96 ? k_zinv<96> : fixed == 48 ? k_zinv<48> : k_zinv<0>;
-    auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, W1); };
+    real2 *W1r = (real2 *) W1;
+    auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, W1r); };
     prof_begin(c, st, 1, bytes);
     go();
     prof_end(c, st, 1, go);
 }
 
-typedef void (*XmixKernel)(XmixArgs, PlanDev, const cplx *, cplx *);
+typedef void (*XmixKernel)(XmixArgs, PlanDev, const real2 *, real2 *);
 
 template <int MODE>
 static XmixKernel xmix_kernel_mode(int Px, int nseq, int zb)
@@ -1243,7 +1277,7 @@ void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
     a.nf = mode == XM_PRIMAL ? 6 : mode == XM_TANGENT ? 12 : mode == XM_ADJOINT ? 9 : 3;
     a.nout = mode == XM_ADJOINT ? 6 : (mode == XM_PRIMAL || mode == XM_TANGENT) ? 3 : 0;
     a.nf1 = W1b ? nf1 : a.nf;
-    a.W1b = W1b ? W1b : W1;
+    a.W1b = (const real2 *) (W1b ? W1b : W1);
     a.nf2 = a.nf;
     a.W1c = nullptr;
     a.W2b = nullptr;
@@ -1252,7 +1286,7 @@ void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
     a.Pz = d.Pz;
     a.qpts = d.qpts;
     int nseq = (a.nf + 1) / 2;
-    int zb = xmix_zblock(c, a.nf);
+    int zb = SC_FFT_NS::xmix_zblock(c, a.nf);
     a.ZB = zb;
     a.check_cfl = check_cfl;
     a.cfl1 = d.cfl1;
@@ -1262,26 +1296,28 @@ void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
     a.cflmax = c.cflmax;
     a.flow = flow;
     a.abort_flag = c.abort_flag;
-    a.ms_in = mstride(c, W1);
-    a.ms_inb = mstride(c, a.W1b);
+    a.ms_in = rstride(c, W1);
+    a.ms_inb = rstride(c, a.W1b);
     a.ms_inc = 0;
-    a.ms_out = mstride(c, W2);
+    a.ms_out = rstride(c, W2);
     a.ms_outb = 0;
     // the physical output exists for one member
     const int nm = mode == XM_PHYS ? 1 : c.nrun;
     dim3 grid(d.Pz / zb, d.qpts, nm);
-    size_t lds = sizeof(double2) * (zb * nseq * (d.Px + 1) + d.Px);
+    size_t lds = sizeof(real2) * (zb * nseq * (d.Px + 1) + d.Px);
     PlanDev pd = to_dev(c.px);
-    const double rows = 16.0 * (double) d.qpts * d.Pz * d.Mx;
+    const double rows = (double) sizeof(real2) * d.qpts * d.Pz * d.Mx;
     double bytes = nm * rows * (a.nf + a.nout);
     if (mode == XM_PHYS) bytes += 8.0 * 3 * d.Px * d.qpts * d.Pz;
     const hipStream_t st = c.stream;
     const XmixKernel kern = xmix_kernel(mode, d.Px, nseq, zb);
-    auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, W1, W2); };
+    const real2 *W1r = (const real2 *) W1;
+    real2 *W2r = (real2 *) W2;
+    auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, W1r, W2r); };
     // the CFL maximum accumulates across launches, so a replay skips it
     XmixArgs ar = a;
     ar.check_cfl = 0;
-    auto again = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, ar, pd, W1, W2); };
+    auto again = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, ar, pd, W1r, W2r); };
     prof_begin(c, st, 2, bytes);
     go();
     // replays only the modes whose launches are side-effect free
@@ -1293,7 +1329,7 @@ void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,
 
 bool xmix_adj2_available(Context &c)
 {
-    const int zb = xmix_zblock(c, 15);
+    const int zb = SC_FFT_NS::xmix_zblock(c, 15);
     return (c.d.Px == 96 && zb == xmix_rows(96, 8)) || (c.d.Px == 48 && zb == xmix_rows(48, 8));
 }
 
@@ -1306,15 +1342,15 @@ void launch_xmix_adj2(Context &c, const cplx *Wa, const cplx *Wb, const cplx *Wl
     a.nf = 15;
     a.nout = 12;
     a.nf1 = 6;
-    a.W1b = Wb;
+    a.W1b = (const real2 *) Wb;
     a.nf2 = 12;
-    a.W1c = Wl;
-    a.W2b = W2b;
+    a.W1c = (const real2 *) Wl;
+    a.W2b = (real2 *) W2b;
     a.Mx = d.Mx;
     a.Px = d.Px;
     a.Pz = d.Pz;
     a.qpts = d.qpts;
-    const int zb = xmix_zblock(c, a.nf);
+    const int zb = SC_FFT_NS::xmix_zblock(c, a.nf);
     a.ZB = zb;
     a.check_cfl = 0;
     a.cfl1 = d.cfl1;
@@ -1324,18 +1360,20 @@ void launch_xmix_adj2(Context &c, const cplx *Wa, const cplx *Wb, const cplx *Wl
     a.cflmax = c.cflmax;
     a.flow = nullptr;
     a.abort_flag = c.abort_flag;
-    a.ms_in = mstride(c, Wa);
-    a.ms_inb = mstride(c, Wb);
-    a.ms_inc = mstride(c, Wl);
-    a.ms_out = mstride(c, W2);
-    a.ms_outb = mstride(c, W2b);
+    a.ms_in = rstride(c, Wa);
+    a.ms_inb = rstride(c, Wb);
+    a.ms_inc = rstride(c, Wl);
+    a.ms_out = rstride(c, W2);
+    a.ms_outb = rstride(c, W2b);
     const dim3 grid(d.Pz / zb, d.qpts, c.nrun);
-    const size_t lds = sizeof(double2) * (zb * 8 * (d.Px + 1) + d.Px);
+    const size_t lds = sizeof(real2) * (zb * 8 * (d.Px + 1) + d.Px);
     const PlanDev pd = to_dev(c.px);
     const XmixKernel kern = d.Px == 96 ? k_xmix<XM_ADJ2, 96> : k_xmix<XM_ADJ2, 48>;
-    const double bytes = c.nrun * 16.0 * (double) d.qpts * d.Pz * d.Mx * (a.nf + a.nout);
+    const double bytes = c.nrun * (double) sizeof(real2) * d.qpts * d.Pz * d.Mx * (a.nf + a.nout);
     const hipStream_t st = c.stream;
-    auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, Wa, W2); };
+    const real2 *War = (const real2 *) Wa;
+    real2 *W2r = (real2 *) W2;
+    auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, War, W2r); };
     prof_begin(c, st, 7, bytes);
     go();
     prof_end(c, st, 7, go);
@@ -1368,15 +1406,16 @@ void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind, 
     a.dst_kind = dst_kind;
     a.norm = 1.0 / ((3. * d.Nx / 2.) * (3. * d.Nz / 2.));
     a.abort_flag = c.abort_flag;
-    a.ms_in = mstride(c, W2);
+    a.ms_in = rstride(c, W2);
     a.ms_out = mstride(c, dst);
     dim3 grid((d.Mx + a.XB - 1) / a.XB, nout * c.nrun, d.qpts);
-    size_t lds = sizeof(double2) * (a.XB * d.Pz + d.Pz);
-    double bytes = c.nrun * 16.0 * (double) d.qpts * d.Mx * nout * (d.Pz + d.Nz - 1);
+    size_t lds = sizeof(real2) * (a.XB * d.Pz + d.Pz);
+    double bytes = c.nrun * (double) d.qpts * d.Mx * nout * (sizeof(real2) * d.Pz + 16.0 * (d.Nz - 1));
     const hipStream_t st = c.stream;
     const PlanDev pd = to_dev(c.pz);
     const ZfwdKernel kern = fixed == 96 ? k_zfwd<96> : fixed == 48 ? k_zfwd<48> : k_zfwd<0>;
-    auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, W2, dst); };
+    const real2 *W2r = (const real2 *) W2;
+    auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, W2r, dst); };
     prof_begin(c, st, 3, bytes);
     go();
     prof_end(c, st, 3, go);
@@ -1386,7 +1425,7 @@ void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind, 
 // caller then transforms into scratch and assembles separately)
 bool launch_zfwd_adj(Context &c, const cplx *W2, cplx *Udst)
 {
-    return launch_zfwd_adj2(c, W2, Udst, nullptr, nullptr);
+    return SC_FFT_NS::launch_zfwd_adj2(c, W2, Udst, nullptr, nullptr);
 }
 
 bool launch_zfwd_adj2(Context &c, const cplx *W2, cplx *Udst, const cplx *W2b, cplx *Udstb)
@@ -1395,12 +1434,12 @@ bool launch_zfwd_adj2(Context &c, const cplx *W2, cplx *Udst, const cplx *W2b, c
     const int fixed = z_fixed(d);
     // the pair in one grid only for the fixed-length two-phase kernel
     if (W2b && !fixed) {
-        if (!launch_zfwd_adj2(c, W2, Udst, nullptr, nullptr)) return false;
-        return launch_zfwd_adj2(c, W2b, Udstb, nullptr, nullptr);
+        if (!SC_FFT_NS::launch_zfwd_adj2(c, W2, Udst, nullptr, nullptr)) return false;
+        return SC_FFT_NS::launch_zfwd_adj2(c, W2b, Udstb, nullptr, nullptr);
     }
     ZadjArgs a;
     a.nprob = W2b ? 2 : 1;
-    a.W2b = W2b;
+    a.W2b = (const real2 *) W2b;
     a.Ub = Udstb;
     a.Nz = d.Nz;
     a.Mx = d.Mx;
@@ -1414,19 +1453,20 @@ bool launch_zfwd_adj2(Context &c, const cplx *W2, cplx *Udst, const cplx *W2b, c
     a.Kz = c.dKz;
     a.abort_flag = c.abort_flag;
     a.nm = c.nrun;
-    a.ms_in = mstride(c, W2);
+    a.ms_in = rstride(c, W2);
     a.ms_out = mstride(c, Udst);
-    if (W2b && (mstride(c, W2b) != a.ms_in || mstride(c, Udstb) != a.ms_out)) return false;
+    if (W2b && (rstride(c, W2b) != a.ms_in || mstride(c, Udstb) != a.ms_out)) return false;
     const dim3 grid = fixed ? dim3(xcd_grid(a.nm * a.nprob * (d.Mx / ZA) * d.qpts))
                             : dim3((d.Mx + a.XA - 1) / a.XA, d.qpts, a.nm);
     // fixed lengths: the two-phase kernel on a 3-field tile
-    size_t lds = sizeof(double2) * ((fixed ? 3 : 6) * (size_t) a.XA * d.Pz + d.Pz);
-    double bytes = a.nm * a.nprob * 16.0 * (double) d.qpts * d.Mx * (6.0 * d.Pz + 5.0 * (d.Nz - 1));
+    size_t lds = sizeof(real2) * ((fixed ? 3 : 6) * (size_t) a.XA * d.Pz + d.Pz);
+    double bytes = a.nm * a.nprob * (double) d.qpts * d.Mx * (sizeof(real2) * 6.0 * d.Pz + 16.0 * 5.0 * (d.Nz - 1));
     const hipStream_t st = c.stream;
     const PlanDev pd = to_dev(c.pz);
-    typedef void (*ZadjKernel)(ZadjArgs, PlanDev, const cplx *, cplx *);
+    typedef void (*ZadjKernel)(ZadjArgs, PlanDev, const real2 *, cplx *);
     const ZadjKernel kern = fixed == 96 ? k_zfwd_adj2<96> : fixed == 48 ? k_zfwd_adj2<48> : k_zfwd_adj<0>;
-    auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, W2, Udst); };
+    const real2 *W2r = (const real2 *) W2;
+    auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, W2r, Udst); };
     prof_begin(c, st, 3, bytes);
     go();
     prof_end(c, st, 3, go);
@@ -1443,4 +1483,5 @@ void launch_assemble_adj(Context &c, const cplx *SP, cplx *Udst)
                        mstride(c, Udst));
 }
 
+}  // namespace SC_FFT_NS
 }  // namespace sc

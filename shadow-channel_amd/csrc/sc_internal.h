@@ -108,6 +108,9 @@ struct Context {
     // signature, captured at its first call and replayed after; dropped when
     // the trajectory's cached fields change or the context is freed
     bool use_graphs = false;
+    // mixed precision (sc_set_precision): the transforms of the time steps in
+    // FP32, everything else FP64; fixed at c_init
+    bool mixed = false;
     struct GraphEntry {
         std::vector<double> key;
         hipGraphExec_t exec;
@@ -242,6 +245,27 @@ bool launch_zfwd_adj(Context &c, const cplx *W2, cplx *Udst);
 // where the kernel allows it (two launches otherwise); false when the tile
 // does not fit
 bool launch_zfwd_adj2(Context &c, const cplx *W2, cplx *Udst, const cplx *W2b, cplx *Udstb);
+// The transforms above dispatch on Context::mixed to one of two builds of
+// fft_kernels.hip: fp64 (all FP64) and fp32 (transforms, products, W1, W2,
+// W2b and the cached fields in FP32; U and the slots written stay FP64).
+#define SC_FFT_LAUNCHERS                                                                        \
+    void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1);                                \
+    void launch_xmix(Context &c, int mode, const cplx *W1, cplx *W2, int check_cfl,             \
+                     double *flow, const cplx *W1b, int nf1);                                   \
+    void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind,             \
+                     int cfl_slot);                                                             \
+    bool xmix_adj2_available(Context &c);                                                       \
+    void launch_xmix_adj2(Context &c, const cplx *Wa, const cplx *Wb, const cplx *Wl, cplx *W2, \
+                          cplx *W2b);                                                           \
+    void launch_assemble_adj(Context &c, const cplx *SP, cplx *Udst);                           \
+    bool launch_zfwd_adj(Context &c, const cplx *W2, cplx *Udst);                               \
+    bool launch_zfwd_adj2(Context &c, const cplx *W2, cplx *Udst, const cplx *W2b, cplx *Udstb);
+namespace fp64 {
+SC_FFT_LAUNCHERS
+}
+namespace fp32 {
+SC_FFT_LAUNCHERS
+}
 // vel_only: only the velocity slots u, v, w are needed (du/dy and dw/dy may
 // be left unwritten where the kernel allows it)
 void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst, bool vel_only = false);
@@ -261,7 +285,6 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
 int solve_columns_per_block(int dimR);
 // Re-sensitivity forcing of state S: fa [Nz][dimQ][Mx], fb [Nz][dimR][Mx], f0 [2][dimR]
 void launch_tangent_forcing(Context &c, const cplx *S, cplx *fa, cplx *fb, cplx *f0);
-int xmix_zblock(Context &c, int nf);
 // wall plane of C: Uxb, Uzb [Nz][Mx]
 void launch_wall_plane(Context &c, const cplx *C, cplx *Uxb, cplx *Uzb);
 // incremental boundary condition in place on Uxb, Uzb; scratch holds

