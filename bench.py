@@ -82,9 +82,14 @@ def member_rate(cc, states, K, W, mixed, roofline_class=0, replays=0):
     advanced together (mixed: FP32 transforms): W untimed steps, then K
     timed.  With roofline_class, one more fwd+adj step of the same members
     records that kernel class's launches and the last one is replayed cold:
-    (rate, elapsed, roofline tuple or None)."""
+    (rate, elapsed, roofline tuple or None, the launch shapes the autotuner
+    chose for the configuration before timing)."""
+    from channel import tuning
     c = CFG
     nm = len(states)
+    # untimed: the launch shapes of this configuration, chosen by timing
+    shapes = tuning.autotune((c["Nx"], c["Ny"], c["Nz"]), (c["Lx"], c["Lz"]), c["Re"], c["flux"],
+                             c["dt"], np.stack(states), mixed=bool(mixed))
     cc.sc_set_members(nm)
     cc.sc_set_precision(mixed)
     roof = None
@@ -141,7 +146,8 @@ def member_rate(cc, states, K, W, mixed, roofline_class=0, replays=0):
     finally:
         cc.sc_set_members(1)
         cc.sc_set_precision(0)
-    return nm * K / elapsed, elapsed, roof
+        tuning.reset()
+    return nm * K / elapsed, elapsed, roof, shapes
 
 
 def gradient_errors(cc):
@@ -328,12 +334,12 @@ def run():
     states = [synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=s)
               for s in range(nm)]
     log("timing %d FP64 members x %d fwd+adj steps" % (nm, K))
-    rate64, _, _ = member_rate(cc, states, K, W, 0)
+    rate64, _, _, shapes64 = member_rate(cc, states, K, W, 0)
     # the headline: members in mixed precision, and the roofline of one
     # kernel class in that same configuration
     log("timing %d mixed-precision members x %d fwd+adj steps (+ roofline class %d)" % (
         nm, K, args.roofline_class))
-    mrate, melapsed, rf = member_rate(cc, states, K, W, 1, args.roofline_class, args.replays)
+    mrate, melapsed, rf, shapes = member_rate(cc, states, K, W, 1, args.roofline_class, args.replays)
     n, ms, nbytes, avg_ms, bytes1 = rf
     achieved = bytes1 / (avg_ms * 1e-3) / 1e9
     traffic = measured_traffic(args.roofline_class)
@@ -371,6 +377,7 @@ def run():
                    "ratio_vs_plain_fp64": round(mrate / plain, 3),
                    "fp64_members_steps_per_s": round(rate64, 3),
                    "plain_fp64_no_field_cache_steps_per_s": round(nocache, 3),
+                   "launch_shapes": shapes, "launch_shapes_fp64_members": shapes64,
                    "host_array_steps_per_s": round(host_rate, 3)},
         "roofline": roof,
     }

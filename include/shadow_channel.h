@@ -235,6 +235,20 @@ int sc_set_graphs(int enable);
 int sc_set_precision(int mixed);
 int sc_precision(void);
 
+/* Launch shapes (no counterpart in the reference).  sc_set_tuning(name,
+   value) sets, for all later launches of the process, one of
+     "solve_members_per_block"  1, 2 or 4 members whose band solves share a
+                                block and its staged factor image (0: 4, or
+                                fewer where they do not fit);
+     "rebuild_blocks"           blocks per XCD of the parity rebuild (<= 256);
+     "force_blocks"             blocks per pair of i-tiles of the parity
+                                projection (multiple of 8, <= 256);
+   0 restores the built-in rule.  Results are the same bit for bit whatever
+   the shapes; channel/tuning.py times candidates and keeps the fastest.
+   sc_get_tuning(name) returns the current value. */
+int sc_set_tuning(const char *name, int value);
+int sc_get_tuning(const char *name);
+
 /* Members (no counterpart in the reference: its drivers run one solution per
    process, drivers/wrapper.py:14-30 loops over them).  sc_set_members(n)
    makes the next c_init allocate n members (1 .. 32, default 1): n copies
@@ -257,6 +271,11 @@ int sc_tangent_members(int start_step, int end_step, sc_complex *IC_dev, unsigne
 int sc_adjoint_members(int start_step, int end_step, sc_complex *AC_dev, double strength,
                        int shared_base);
 int sc_member_state(int incre, int member, int i_step, sc_complex *out);
+/* c_ddt_project of the host array v [Nz][2][dimR][Nx/2] at step i_step of
+   member `member`'s stored trajectory (the least-squares-shadowing segments
+   run as members project at their own ends); returns num/den, NaN on an
+   invalid call. */
+double sc_member_ddt_project(int member, int i_step, sc_complex *v);
 
 /* Sensitivities (no counterpart in the reference; the building blocks of the
    time-averaged wall-shear gradient):

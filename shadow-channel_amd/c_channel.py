@@ -122,6 +122,12 @@ _L.sc_set_precision.argtypes = [_i]
 _L.sc_set_precision.restype = _i
 _L.sc_precision.argtypes = []
 _L.sc_precision.restype = _i
+_L.sc_set_tuning.argtypes = [_c.c_char_p, _i]
+_L.sc_set_tuning.restype = _i
+_L.sc_get_tuning.argtypes = [_c.c_char_p]
+_L.sc_get_tuning.restype = _i
+_L.sc_member_ddt_project.argtypes = [_i, _i, _vp]
+_L.sc_member_ddt_project.restype = _d
 _L.sc_set_graphs.argtypes = [_i]
 _L.sc_set_graphs.restype = _i
 
@@ -558,3 +564,25 @@ def sc_set_precision(mixed):
 
 def sc_precision():
     return _L.sc_precision()
+
+
+def sc_set_tuning(name, value):
+    """Launch shape `name` (solve_members_per_block, rebuild_blocks,
+    force_blocks) for later launches; 0 restores the built-in rule."""
+    _check(_L.sc_set_tuning(str(name).encode(), int(value)), "sc_set_tuning")
+
+
+def sc_get_tuning(name):
+    v = _L.sc_get_tuning(str(name).encode())
+    _check(v if v < 0 else 0, "sc_get_tuning")
+    return v
+
+
+def sc_member_ddt_project(member, i_step, v):
+    """c_ddt_project of v (in place) on member `member`'s trajectory at
+    i_step; returns num/den."""
+    v = _coef_array(v, "v")
+    r = _L.sc_member_ddt_project(int(member), int(i_step), v.ctypes.data)
+    if np.isnan(r) and last_error():
+        raise AssertionError("sc_member_ddt_project failed: %s" % last_error())
+    return r

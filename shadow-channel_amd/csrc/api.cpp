@@ -131,6 +131,12 @@ static int stepping(Context &c, const std::vector<double> &key,
     return SC_OK;
 }
 
+Tuning &sc::tuning()
+{
+    static Tuning t;
+    return t;
+}
+
 // whether c_init allocates the transformed-field cache (sc_set_field_cache)
 static int g_field_cache = 1;
 // members c_init allocates (sc_set_members)
@@ -482,6 +488,37 @@ int sc_set_precision(int mixed)
 }
 
 int sc_precision(void) { return ctx().mixed ? 1 : 0; }
+
+int sc_set_tuning(const char *name, int value)
+{
+    if (!name || value < 0) return SC_EINVAL;
+    Tuning &t = tuning();
+    const std::string n(name);
+    if (n == "solve_members_per_block" && (value == 0 || value == 1 || value == 2 || value == 4))
+        t.solve_mb = value;
+    else if (n == "rebuild_blocks" && value <= 256)
+        t.rebuild_blocks = value;
+    else if (n == "force_blocks" && value % 8 == 0 && value <= 256)
+        t.force_blocks = value;
+    else {
+        last_error() = "tuning: unknown name or value out of range";
+        return SC_EINVAL;
+    }
+    // recorded graphs hold the old launch shapes
+    drop_graphs(ctx());
+    return SC_OK;
+}
+
+int sc_get_tuning(const char *name)
+{
+    const Tuning &t = tuning();
+    const std::string n(name ? name : "");
+    if (n == "solve_members_per_block") return t.solve_mb;
+    if (n == "rebuild_blocks") return t.rebuild_blocks;
+    if (n == "force_blocks") return t.force_blocks;
+    last_error() = "tuning: unknown name";
+    return SC_EINVAL;
+}
 
 int sc_prof_enable(int kernel_class)
 {
@@ -1349,6 +1386,40 @@ int sc_member_state(int incre, int member, int i_step, sc_complex *out)
     SC_HIP(hipStreamSynchronize(c.stream));
     SC_HIP(hipMemcpy(out, src, sizeof(cplx) * c.csize, hipMemcpyDeviceToHost));
     return SC_OK;
+}
+
+// c_ddt_project on member `member`'s stored trajectory (its own states;
+// the least-squares-shadowing segments run as members use it at their ends)
+double sc_member_ddt_project(int member, int i_step, sc_complex *v)
+{
+    Context &c = ctx();
+    last_error().clear();
+    if (!c.ok || !v || member < 0 || member >= c.nmem || i_step < 0 || i_step > c.d.nsteps ||
+        c.d.nsteps < 1) {
+        last_error() = "member_ddt_project requires 0 <= member < members, 0 <= i_step <= "
+                       "nsteps and nsteps >= 1";
+        return NAN;
+    }
+    const size_t cb = sizeof(cplx) * c.csize;
+    const long ip = i_step == 0 ? 3 : 3L * i_step, im = i_step == 0 ? 0 : 3L * (i_step - 1);
+    const cplx *P = (const cplx *) ((const char *) mc(c, ip) + (size_t) member * c.mcbytes);
+    const cplx *M = (const cplx *) ((const char *) mc(c, im) + (size_t) member * c.mcbytes);
+    if (hipMemcpyAsync(c.Ctmp, v, cb, hipMemcpyHostToDevice, c.stream) != hipSuccess ||
+        hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream) != hipSuccess) {
+        last_error() = "member_ddt_project: HIP transfer failed";
+        return NAN;
+    }
+    const int nparts = launch_ddt_gram(c, P, M, 1.0 / c.d.dt, c.Ctmp);
+    if (nparts <= 0) return NAN;
+    launch_ddt_apply(c, c.Ctmp, c.Ctmp, P, M, 1.0 / c.d.dt, c.csize, nullptr, 0.0, nparts, 0);
+    double scal[3] = {0, 0, 0};
+    if (hipMemcpyAsync(v, c.Ctmp, cb, hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+        hipMemcpyAsync(scal, c.scal, sizeof(scal), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+        hipStreamSynchronize(c.stream) != hipSuccess) {
+        last_error() = "member_ddt_project: HIP transfer failed";
+        return NAN;
+    }
+    return scal[2];
 }
 
 }  // extern "C"

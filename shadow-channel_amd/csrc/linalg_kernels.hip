@@ -716,7 +716,8 @@ void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx 
         // enough waves for one task each (of one rebuild of all members), at
         // most about two blocks per CU
         const int ntmax = nm * ((d.Nz + 7) / 8 * ((d.Mx + 7) / 8));
-        ap.bpq = std::max(1, std::min((ntmax + 3) / 4, SC_RB_BLOCKS / c.nht));
+        const int rbb = tuning().rebuild_blocks > 0 ? tuning().rebuild_blocks : SC_RB_BLOCKS;
+        ap.bpq = std::max(1, std::min((ntmax + 3) / 4, rbb / c.nht));
     }
     const dim3 gpar(8 * c.nht * ap.bpq);
     auto go = [=]() {
@@ -1334,7 +1335,8 @@ static void launch_force_par(const ForceArgs &a, int npair, int ntask, int nprob
         attr = true;
     }
     // blocks per pair: a multiple of 8 (XCD-aware mapping), one block per CU
-    const int bpi = std::max(8, std::min((ntask * nprob * a.nm + 7) / 8 * 8, 256 / npair / 8 * 8));
+    const int cap = tuning().force_blocks > 0 ? tuning().force_blocks : 256 / npair / 8 * 8;
+    const int bpi = std::max(8, std::min((ntask * nprob * a.nm + 7) / 8 * 8, cap));
     hipLaunchKernelGGL((k_force_par<ADJ, NKS>), dim3(npair * bpi), dim3(FR_THREADS), lds, st, a, bpi,
                        nprob, U0, Fa0, Fb0, U1, Fa1, Fb1);
 }
@@ -1495,6 +1497,7 @@ struct SolveArgs {
     // follows the solve (the driver applies P at each step's start)
     int exact_mean;
     const double *add_mass;            // Rs^{-1} w1
+    int nm;                            // members of the launch (grid z: groups of them)
 };
 
 // explicit band operator row entry A(i, i+2m)
@@ -1891,27 +1894,40 @@ constexpr int SOLVE_VR_MAX = 4;
 // factor values in flight per thread and batch (16-byte loads)
 constexpr int SOLVE_FBAT = 8;
 
-// One (field, parity) of SBT consecutive columns.  The factors of the block
-// are one contiguous image [nr][W][SBT] (laid out so at init) copied to LDS
-// with 16-byte loads; every thread owns the column tid % SBT, so its rows are
-// tid / SBT + t * (SOLVE_THREADS / SBT).  lds: [nr][W][SBT] factors, then
-// [nr][SBT] right-hand side, then [nr][SBT] stage-0 copy of C.
-template <int DL, int SBT, int VR>
-__device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx *C, cplx *Cs,
-                            double *lds)
+// One (field, parity) of SBT consecutive columns, for the members m0 ..
+// m0 + nmb - 1 (nmb <= MB), which all solve with the same operator.  The
+// factors of the block are one contiguous image [nr][W][SBT] (laid out so at
+// init) copied to LDS once with 16-byte loads and shared by the members;
+// every thread owns the column tid % SBT, so its rows are tid / SBT + t *
+// (SOLVE_THREADS / SBT).  lds: [nr][W][SBT] factors, then [MB][nr][SBT]
+// right-hand sides, then [MB][nr][SBT] stage-0 copies of C.  The chain wave
+// runs 2 SBT lanes per member (real and imaginary part of each column): with
+// several members one staged image feeds up to 64 chains at once.
+template <int DL, int SBT, int VR, int MB>
+__device__ void solve_block(const SolveArgs &a, int var, int par, int blk, int m0, int nmb,
+                            cplx *C, cplx *Cs, double *lds)
 {
     constexpr int W = 2 * DL + 1;
     constexpr int RSTEP = SOLVE_THREADS / SBT;
+    static_assert(2 * SBT * MB <= 64, "one chain lane per member, part and column");
     const int Mx = a.Mx, nR = a.dimR, nQ = a.dimQ;
     const int N = var == 0 ? nQ : nR;
     const int nr = (N - par + 1) / 2, nrmax = (N + 1) / 2;
     const int tid = threadIdx.x;
     double *lf = lds;
     double2 *vec = (double2 *) (lds + (size_t) nr * W * SBT);
-    double2 *raw = vec + nr * SBT;
-    const cplx *F = var == 0 ? a.Fa : a.Fb;
-    const cplx *F2 = var == 0 ? a.Fa2 : a.Fb2;
+    double2 *raw = vec + MB * nr * SBT;
     const double ce0 = a.ce0[var], cn = a.cn[var];
+    // the members' arrays (member m0 + mb; uniform across the block)
+    auto Cm = [&](int mb) { return C + (size_t) (m0 + mb) * a.ms_c; };
+    auto Csm = [&](int mb) { return Cs + (size_t) (m0 + mb) * a.ms_cs; };
+    auto Fm = [&](int mb) {
+        return var == 0 ? a.Fa + (size_t) (m0 + mb) * a.ms_fa : a.Fb + (size_t) (m0 + mb) * a.ms_fb;
+    };
+    auto F2m = [&](int mb) {
+        return var == 0 ? a.Fa2 + (size_t) (m0 + mb) * a.ms_fa2 : a.Fb2 + (size_t) (m0 + mb) * a.ms_fb2;
+    };
+    auto Sm = [&](int mb) { return a.Src + (size_t) (m0 + mb) * a.ms_src; };
 
     constexpr int NCHU = 4;                 // chunks of rows whose U parts stream in
     __shared__ int chain_wave[1];
@@ -1931,16 +1947,36 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx 
     const size_t fbase = ((size_t) co.z * N + par) * Mx + co.x;                 // F row par
     const size_t rs = (size_t) 2 * Mx;                                          // next chain row
 
-    // right-hand side rows and the next stage's projection, loads first
-    double2 cv[VR], fv[VR], f2v[VR], sv[VR];
+    // right-hand side rows: member 0's loads first (before the factor copy);
+    // the other members' after it, one member at a time
+    double2 cv[VR], fv[VR], sv[VR];
+    auto load_rows = [&](int mb) {
 #pragma unroll
-    for (int t = 0; t < VR; ++t) {
-        const int r = min(rt + t * RSTEP, nr - 1);
-        cv[t] = C[cbase + r * rs];
-        fv[t] = F[fbase + r * rs];
-        f2v[t] = a.has_next ? F2[fbase + r * rs] : make_double2(0.0, 0.0);
-        sv[t] = a.Src ? a.Src[cbase + r * rs] : make_double2(0.0, 0.0);
-    }
+        for (int t = 0; t < VR; ++t) {
+            const int r = min(rt + t * RSTEP, nr - 1);
+            cv[t] = Cm(mb)[cbase + r * rs];
+            fv[t] = Fm(mb)[fbase + r * rs];
+            sv[t] = a.Src ? Sm(mb)[cbase + r * rs] : make_double2(0.0, 0.0);
+        }
+    };
+    auto store_rows = [&](int mb) {
+#pragma unroll
+        for (int t = 0; t < VR; ++t) {
+            const int r = rt + t * RSTEP;
+            if (r >= nr || !co.chain) continue;
+            if (a.k == 0) {
+                raw[(mb * nr + r) * SBT + cc] = cv[t];
+            } else {
+                double2 v = cv[t];
+                v.x += a.dtc * fv[t].x;
+                v.y += a.dtc * fv[t].y;
+                v.x += sv[t].x;
+                v.y += sv[t].y;
+                vec[(mb * nr + r) * SBT + cc] = v;
+            }
+        }
+    };
+    load_rows(0);
     // The factor image ([row][W][SBT], 16-byte loads): the L parts (columns
     // j < DL) of every row before the barrier, for the forward sweep; the U
     // parts (j >= DL) after it, last rows first in NCHU chunks, each wave
@@ -1972,42 +2008,55 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx 
                 if (b0 + tid + i * SOLVE_THREADS < nl) dst[o[i]] = t[i];
         }
     }
+    store_rows(0);
+    // member 0's forcing rows kept for the stage-0 operator (with one member
+    // they stay where they are)
+    constexpr int KV = MB > 1 ? VR : 1;
+    double2 fv0[KV], sv0[KV];
+    if constexpr (MB > 1) {
 #pragma unroll
-    for (int t = 0; t < VR; ++t) {
-        const int r = rt + t * RSTEP;
-        if (r >= nr || !co.chain) continue;
-        if (a.k == 0) {
-            raw[r * SBT + cc] = cv[t];
-        } else {
-            double2 v = cv[t];
-            v.x += a.dtc * fv[t].x;
-            v.y += a.dtc * fv[t].y;
-            v.x += sv[t].x;
-            v.y += sv[t].y;
-            vec[r * SBT + cc] = v;
+        for (int t = 0; t < VR; ++t) {
+            fv0[t] = fv[t];
+            sv0[t] = sv[t];
         }
+    }
+    for (int mb = 1; mb < nmb; ++mb) {
+        load_rows(mb);
+        store_rows(mb);
     }
     __syncthreads();
     if (a.k == 0) {
-        // E C + dt c_0 F with the stage-0 explicit operator
+        // E C + dt c_0 F with the stage-0 explicit operator (member 0's
+        // forcing rows still in registers, the others' loaded again)
+        for (int mb = 0; mb < nmb; ++mb) {
+            if (mb > 0) load_rows(mb);
 #pragma unroll
-        for (int t = 0; t < VR; ++t) {
-            const int r = rt + t * RSTEP, i = par + 2 * r;
-            if (r >= nr || !co.chain) continue;
-            double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-            for (int m = -DL; m <= DL; ++m)
-                if (r + m >= 0 && r + m < nr) {
-                    double ev = op_entry(a, var, i, m, k2, ce0);
-                    double2 y = raw[(r + m) * SBT + cc];
-                    acc.x += ev * y.x;
-                    acc.y += ev * y.y;
+            for (int t = 0; t < VR; ++t) {
+                const int r = rt + t * RSTEP, i = par + 2 * r;
+                if (r >= nr || !co.chain) continue;
+                double2 f = fv[t], sf = sv[t];
+                if constexpr (MB > 1) {
+                    if (mb == 0) {
+                        f = fv0[t];
+                        sf = sv0[t];
+                    }
                 }
-            acc.x += a.dtc * fv[t].x;
-            acc.y += a.dtc * fv[t].y;
-            acc.x += sv[t].x;
-            acc.y += sv[t].y;
-            vec[r * SBT + cc] = acc;
+                const double2 *rm = raw + mb * nr * SBT;
+                double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                for (int m = -DL; m <= DL; ++m)
+                    if (r + m >= 0 && r + m < nr) {
+                        double ev = op_entry(a, var, i, m, k2, ce0);
+                        double2 y = rm[(r + m) * SBT + cc];
+                        acc.x += ev * y.x;
+                        acc.y += ev * y.y;
+                    }
+                acc.x += a.dtc * f.x;
+                acc.y += a.dtc * f.y;
+                acc.x += sf.x;
+                acc.y += sf.y;
+                vec[(mb * nr + r) * SBT + cc] = acc;
+            }
         }
         __syncthreads();
     }
@@ -2023,10 +2072,11 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx 
     static_assert(NWR >= 1 && SOLVE_THREADS % 64 == 0, "one chain wave, the rest stage");
     const StagedRows sr{ready, cr, NCHU, NWR, a.err};
     if ((tid >> 6) == cw) {
-        // lane cl < 2 SBT: part cl / SBT (real, imaginary) of column cl % SBT
-        if (cl < 2 * SBT && co.chain && nr > 0) {
-            const int col = cl % SBT, part = cl / SBT;
-            double *vd = (double *) vec + 2 * col + part;
+        // lane cl < 2 SBT nmb: member cl / (2 SBT), part (cl / SBT) % 2
+        // (real, imaginary) of column cl % SBT
+        if (cl < 2 * SBT * nmb && co.chain && nr > 0) {
+            const int mb = cl / (2 * SBT), col = cl % SBT, part = (cl / SBT) & 1;
+            double *vd = (double *) (vec + mb * nr * SBT) + 2 * col + part;
             chain_forward_r<DL, 3>(lf + col, W * SBT, SBT, vd, 2 * SBT, nr);
             chain_backward_staged<DL, 3>(lf + col, W * SBT, SBT, vd, 2 * SBT, nr, sr);
         }
@@ -2091,42 +2141,47 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, cplx 
         }
     }
     __syncthreads();
-    // write the solution and the next stage's right-hand side
+    // write the solutions and the next stage's right-hand sides
 #pragma unroll
-    for (int t = 0; t < VR; ++t) {
-        const int r = rt + t * RSTEP, i = par + 2 * r;
-        if (r >= nr || !co.chain) continue;
-        const size_t gi = cbase + r * rs;
-        const double2 sol = vec[r * SBT + cc];
-        Cs[gi] = sol;
-        if (a.has_next) {
-            double2 acc = make_double2(0.0, 0.0);
+    for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-            for (int m = -DL; m <= DL; ++m)
-                if (r + m >= 0 && r + m < nr) {
-                    double ev = op_entry(a, var, i, m, k2, cn);
-                    double2 y = vec[(r + m) * SBT + cc];
-                    acc.x += ev * y.x;
-                    acc.y += ev * y.y;
-                }
-            acc.x += a.dtd * f2v[t].x;
-            acc.y += a.dtd * f2v[t].y;
-            C[gi] = acc;
-        } else {
-            C[gi] = sol;
+        for (int t = 0; t < VR; ++t) {
+            const int r = rt + t * RSTEP, i = par + 2 * r;
+            if (mb >= nmb || r >= nr || !co.chain) continue;
+            const double2 *vm = vec + mb * nr * SBT;
+            const size_t gi = cbase + r * rs;
+            const double2 sol = vm[r * SBT + cc];
+            Csm(mb)[gi] = sol;
+            if (a.has_next) {
+                const double2 f2 = F2m(mb)[fbase + r * rs];
+                double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                for (int m = -DL; m <= DL; ++m)
+                    if (r + m >= 0 && r + m < nr) {
+                        double ev = op_entry(a, var, i, m, k2, cn);
+                        double2 y = vm[(r + m) * SBT + cc];
+                        acc.x += ev * y.x;
+                        acc.y += ev * y.y;
+                    }
+                acc.x += a.dtd * f2.x;
+                acc.y += a.dtd * f2.y;
+                Cm(mb)[gi] = acc;
+            } else {
+                Cm(mb)[gi] = sol;
+            }
         }
-    }
     // rows of v beyond dimQ are carried unchanged; the Nyquist row is zeroed
     const int nrest = (nR - par + 1) / 2;
     if (!co.valid || (co.z == 0 && co.x == 0)) return;
-    for (int r = rt; r < nrest; r += RSTEP) {
-        const int i = par + 2 * r;
-        const size_t gi = cbase + r * rs;
-        if (co.z == a.Nz / 2)
-            C[gi] = make_double2(0.0, 0.0);
-        else if (var == 0 && i >= nQ)
-            Cs[gi] = C[gi];
-    }
+    for (int mb = 0; mb < nmb; ++mb)
+        for (int r = rt; r < nrest; r += RSTEP) {
+            const int i = par + 2 * r;
+            const size_t gi = cbase + r * rs;
+            if (co.z == a.Nz / 2)
+                Cm(mb)[gi] = make_double2(0.0, 0.0);
+            else if (var == 0 && i >= nQ)
+                Csm(mb)[gi] = Cm(mb)[gi];
+        }
 }
 
 // explicit mean operator Rs + coef Rps applied to v (row i)
@@ -2270,41 +2325,48 @@ __device__ void nyquist_clear(const SolveArgs &a, int var, int par, int blk, cpl
         C[base + (size_t) (e / SBT) * 2 * Mx + e % SBT] = make_double2(0.0, 0.0);
 }
 
-// grid (nblk, 4, members), or (nblk + 1, 4, members) when the mean mode
-// needs a block of its own: blockIdx.y = field*2 + parity.  With Mx % SBT == 0 the blocks of the
-// Nyquist row hold no chains, and the first of them with blockIdx.y == 0 runs
-// the mean mode (so the grid is exactly the column blocks); otherwise the
-// extra x-block nblk with blockIdx.y == 0 does.
-template <int SBT, int VR>
-__global__ __launch_bounds__(SOLVE_THREADS, 4) void k_solve(SolveArgs a, cplx *C, cplx *Cs)
+// grid (nblk, 4, member groups), or (nblk + 1, 4, member groups) when the
+// mean mode needs a block of its own: blockIdx.y = field*2 + parity,
+// blockIdx.z = group of MB members (the last group may hold fewer: a.nm
+// members in all).  With Mx % SBT == 0 the blocks of the Nyquist row hold no
+// chains, and the first of them with blockIdx.y == 0 runs the mean mode (so
+// the grid is exactly the column blocks); otherwise the extra x-block nblk
+// with blockIdx.y == 0 does.  The mean mode runs member after member.
+template <int SBT, int VR, int MB>
+__global__ __launch_bounds__(SOLVE_THREADS, MB == 1 ? 4 : MB == 2 ? 3 : 2) void k_solve(SolveArgs a, cplx *C, cplx *Cs)
 {
     extern __shared__ double lds_solve[];
     if (*a.abort_flag) return;
-    if (const int m = blockIdx.z) {
-        if ((a.alt_mask >> m) & 1u) a.target = a.target_alt;
-        C += m * a.ms_c;
-        Cs += m * a.ms_cs;
-        a.Fa += m * a.ms_fa;
-        a.Fb += m * a.ms_fb;
-        a.Fa2 += m * a.ms_fa2;
-        a.Fb2 += m * a.ms_fb2;
-        if (a.Src) a.Src += m * a.ms_src;
-    }
-    if (blockIdx.z == 0 && (a.alt_mask & 1u)) a.target = a.target_alt;
+    const int m0 = blockIdx.z * MB, nmb = min(MB, a.nm - m0);
     const int var = blockIdx.y >> 1, par = blockIdx.y & 1, blk = blockIdx.x;
+    auto mean = [&]() {
+        for (int mb = 0; mb < nmb; ++mb) {
+            const int m = m0 + mb;
+            SolveArgs b = a;
+            if ((a.alt_mask >> m) & 1u) b.target = a.target_alt;
+            b.Fa += m * a.ms_fa;
+            b.Fb += m * a.ms_fb;
+            b.Fa2 += m * a.ms_fa2;
+            b.Fb2 += m * a.ms_fb2;
+            if (b.Src) b.Src += m * a.ms_src;
+            if (mb) __syncthreads();                 // the previous member's LDS reads done
+            solve_mean(b, C + m * a.ms_c, Cs + m * a.ms_cs, lds_solve);
+        }
+    };
     if (blk == a.nblk) {
-        if (blockIdx.y == 0) solve_mean(a, C, Cs, lds_solve);
+        if (blockIdx.y == 0) mean();
         return;
     }
     if (a.nyq_mean && blk * SBT / a.Mx == a.Nz / 2) {
-        if (blockIdx.y == 0 && blk * SBT == (a.Nz / 2) * a.Mx) solve_mean(a, C, Cs, lds_solve);
-        nyquist_clear<SBT>(a, var, par, blk, C);
+        if (blockIdx.y == 0 && blk * SBT == (a.Nz / 2) * a.Mx) mean();
+        for (int mb = 0; mb < nmb; ++mb)
+            nyquist_clear<SBT>(a, var, par, blk, C + (m0 + mb) * a.ms_c);
         return;
     }
     if (var == 0)
-        solve_block<4, SBT, VR>(a, 0, par, blk, C, Cs, lds_solve);
+        solve_block<4, SBT, VR, MB>(a, 0, par, blk, m0, nmb, C, Cs, lds_solve);
     else
-        solve_block<2, SBT, VR>(a, 1, par, blk, C, Cs, lds_solve);
+        solve_block<2, SBT, VR, MB>(a, 1, par, blk, m0, nmb, C, Cs, lds_solve);
 }
 
 // coefficient tables for the stages
@@ -2315,10 +2377,10 @@ static const double BA_[3] = {1. / 3., -1. / 2, 1. / 3.};
 static const double BC_[3] = {1., 1. / 3., 1. / 2.};
 static const double BD_[3] = {0., -2. / 3, -1. / 6};
 
-size_t solve_lds_bytes(int dimR, int sbt)
+size_t solve_lds_bytes(int dimR, int sbt, int mb)
 {
     const size_t nr = (size_t) (dimR + 1) / 2;
-    const size_t blk = nr * sbt * (9 * 8 + 16 + 16);
+    const size_t blk = nr * sbt * (9 * 8 + 32 * (size_t) mb);
     const size_t mean = (size_t) dimR * (4 * 16 + 5 * 8) + SOLVE_THREADS * 8;
     return blk > mean ? blk : mean;
 }
@@ -2330,9 +2392,24 @@ int solve_columns_per_block(int dimR)
 {
     const int nr = (dimR + 1) / 2;
     for (int sbt = 8; sbt >= 2; sbt >>= 1)
-        if (solve_lds_bytes(dimR, sbt) <= 64 * 1024 && nr <= SOLVE_VR_MAX * (SOLVE_THREADS / sbt))
+        if (solve_lds_bytes(dimR, sbt, 1) <= 64 * 1024 && nr <= SOLVE_VR_MAX * (SOLVE_THREADS / sbt))
             return sbt;
     return 0;
+}
+
+// Members per solve block (1, 2 or 4): the tuning's choice when set, else 4;
+// fewer when the run has fewer members or their vectors and the factor image
+// would not fit twice in the LDS of a CU.
+int solve_members_per_block(const Context &c)
+{
+    int mb = tuning().solve_mb > 0 ? tuning().solve_mb : 4;
+    // several members only with two row slots per thread (the four-slot
+    // form would spill its registers)
+    const bool two = (c.d.dimR + 1) / 2 <= 2 * (SOLVE_THREADS / c.sbt);
+    if (!two) mb = 1;
+    while (mb > 1 && (mb > c.nrun || 2 * solve_lds_bytes(c.d.dimR, c.sbt, mb) > 160 * 1024))
+        mb /= 2;
+    return mb;
 }
 
 void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cplx *Fa2,
@@ -2387,18 +2464,31 @@ void launch_solve(Context &c, cplx *Ccarry, cplx *Csol, int k, int adj, const cp
                    fcols * 8.0 * (9.0 * d.dimQ + 5.0 * d.dimR);
     const int sbt = c.sbt;
     const int ncol = d.Nz * d.Mx;
-    const size_t lds = solve_lds_bytes(d.dimR, sbt);
+    // members per block: the staged factor image feeds the chains of up to
+    // four members (2 sbt lanes each), within 160 KB of LDS per CU
+    const int mb = solve_members_per_block(c);
+    const size_t lds = solve_lds_bytes(d.dimR, sbt, mb);
+    a.nm = c.nrun;
     a.nblk = (ncol + sbt - 1) / sbt;
     a.mirror = c.factor_mirror;
     a.nyq_mean = d.Mx % sbt == 0;
-    const dim3 grid(a.nblk + (a.nyq_mean ? 0 : 1), 4, c.nrun);
+    const dim3 grid(a.nblk + (a.nyq_mean ? 0 : 1), 4, (c.nrun + mb - 1) / mb);
     const hipStream_t st = c.stream;
     // row slots per thread: the chains have at most (dimR + 1) / 2 rows
     const bool two = (d.dimR + 1) / 2 <= 2 * (SOLVE_THREADS / sbt);
     typedef void (*SolveKernel)(SolveArgs, cplx *, cplx *);
-    const SolveKernel kern = sbt == 8 ? (two ? k_solve<8, 2> : k_solve<8, 4>)
-                           : sbt == 4 ? (two ? k_solve<4, 2> : k_solve<4, 4>)
-                                      : (two ? k_solve<2, 2> : k_solve<2, 4>);
+    SolveKernel kern;
+    if (mb == 4)
+        kern = sbt == 8 ? k_solve<8, 2, 4> : sbt == 4 ? k_solve<4, 2, 4> : k_solve<2, 2, 4>;
+    else if (mb == 2)
+        kern = sbt == 8 ? k_solve<8, 2, 2> : sbt == 4 ? k_solve<4, 2, 2> : k_solve<2, 2, 2>;
+    else
+        kern = sbt == 8 ? (two ? k_solve<8, 2, 1> : k_solve<8, 4, 1>)
+             : sbt == 4 ? (two ? k_solve<4, 2, 1> : k_solve<4, 4, 1>)
+                        : (two ? k_solve<2, 2, 1> : k_solve<2, 4, 1>);
+    if (lds > 64 * 1024)
+        (void) hipFuncSetAttribute((const void *) kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int) lds);
     auto go = [=]() {
         hipLaunchKernelGGL(kern, grid, dim3(SOLVE_THREADS), lds, st, a, Ccarry, Csol);
     };

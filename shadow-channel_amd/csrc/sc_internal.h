@@ -216,6 +216,14 @@ inline size_t mstride(const Context &c, const T *p)
     return mstride_bytes(c, (const void *) p) / sizeof(T);
 }
 
+// Launch shapes the tuner may set (sc_set_tuning; 0 = the built-in rule).
+struct Tuning {
+    int solve_mb = 0;        // members per band-solve block: 1, 2 or 4
+    int rebuild_blocks = 0;  // blocks per XCD of the parity rebuild (at most 256)
+    int force_blocks = 0;    // blocks per pair of i-tiles of the parity projection (multiple of 8)
+};
+Tuning &tuning();
+
 // ---- setup (host) ----
 int setup_host(Context &c);
 void band_factor(double *A, int N, int dl);
