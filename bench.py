@@ -77,6 +77,9 @@ def fwd_adj_members(cc, Cd, ACd, nsteps):
     cc.sc_adjoint_members(nsteps, 0, ACd, 1.0, False)
 
 
+RETUNE = False      # --retune: time the launch shapes instead of reading them
+
+
 def member_rate(cc, states, K, W, mixed, roofline_class=0, replays=0):
     """fwd+adj member-steps per second of len(states) independent members
     advanced together (mixed: FP32 transforms): W untimed steps, then K
@@ -87,9 +90,11 @@ def member_rate(cc, states, K, W, mixed, roofline_class=0, replays=0):
     from channel import tuning
     c = CFG
     nm = len(states)
-    # untimed: the launch shapes of this configuration, chosen by timing
+    # untimed: the launch shapes of this configuration, as the autotuner
+    # chose them on an MI355X (channel/shapes.json), or timed again
     shapes = tuning.autotune((c["Nx"], c["Ny"], c["Nz"]), (c["Lx"], c["Lz"]), c["Re"], c["flux"],
-                             c["dt"], np.stack(states), mixed=bool(mixed))
+                             c["dt"], np.stack(states), mixed=bool(mixed),
+                             cache=None if RETUNE else tuning.SHAPES_FILE)
     cc.sc_set_members(nm)
     cc.sc_set_precision(mixed)
     roof = None
@@ -271,7 +276,11 @@ def run():
                     help="skip the untimed gradient-error check")
     ap.add_argument("--members", type=int, default=8,
                     help="independent members advanced together in the headline figure")
+    ap.add_argument("--retune", action="store_true",
+                    help="time the launch shapes instead of using channel/shapes.json")
     args = ap.parse_args()
+    global RETUNE
+    RETUNE = args.retune
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver runs one simulation per device and "
               "multi-GPU sharding is not part of this build (--gpus must be 1)",

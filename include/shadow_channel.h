@@ -273,9 +273,21 @@ int sc_adjoint_members(int start_step, int end_step, sc_complex *AC_dev, double 
 int sc_member_state(int incre, int member, int i_step, sc_complex *out);
 /* c_ddt_project of the host array v [Nz][2][dimR][Nx/2] at step i_step of
    member `member`'s stored trajectory (the least-squares-shadowing segments
-   run as members project at their own ends); returns num/den, NaN on an
-   invalid call. */
+   run as members project at their own ends; at step 0 of a member with a
+   previous state, sc_set_member_previous, the backward difference); returns
+   num/den, NaN on an invalid call. */
 double sc_member_ddt_project(int member, int i_step, sc_complex *v);
+
+/* Members as consecutive windows of one trajectory (no counterpart in the
+   reference; the least-squares-shadowing segments run as members): prev
+   holds, for each member, the state one step before its window (members one
+   after the other, [Nz][2][dimR][Nx/2] each; member 0's entry unused).  Then
+   a member's time derivative at its step 0 (the forced adjoint's last
+   forcing, sc_member_ddt_project at step 0) is the backward difference from
+   that state, as c_ddt_project's at that step of the long trajectory;
+   member 0 keeps the forward difference.  NULL switches it off; c_init
+   clears it. */
+int sc_set_member_previous(const sc_complex *prev);
 
 /* Sensitivities (no counterpart in the reference; the building blocks of the
    time-averaged wall-shear gradient):

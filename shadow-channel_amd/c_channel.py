@@ -128,6 +128,8 @@ _L.sc_get_tuning.argtypes = [_c.c_char_p]
 _L.sc_get_tuning.restype = _i
 _L.sc_member_ddt_project.argtypes = [_i, _i, _vp]
 _L.sc_member_ddt_project.restype = _d
+_L.sc_set_member_previous.argtypes = [_vp]
+_L.sc_set_member_previous.restype = _i
 _L.sc_set_graphs.argtypes = [_i]
 _L.sc_set_graphs.restype = _i
 
@@ -586,3 +588,13 @@ def sc_member_ddt_project(member, i_step, v):
     if np.isnan(r) and last_error():
         raise AssertionError("sc_member_ddt_project failed: %s" % last_error())
     return r
+
+
+def sc_set_member_previous(prev):
+    """The state one step before each member's window (members x [Nz][2]
+    [dimR][Nx/2]; member 0's entry unused), or None to switch it off."""
+    if prev is None:
+        _check(_L.sc_set_member_previous(None), "sc_set_member_previous")
+        return
+    prev = np.ascontiguousarray(prev, dtype=np.complex128)
+    _check(_L.sc_set_member_previous(prev.ctypes.data), "sc_set_member_previous")

@@ -18,6 +18,10 @@ import numpy as np
 
 import c_channel
 
+# the autotuner's choices for the configurations bench.py measures, timed on
+# an MI355X (a cache in the format autotune() writes)
+SHAPES_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "shapes.json")
+
 SHAPES = {
     "solve_members_per_block": (1, 2, 4),
     "rebuild_blocks": (32, 64, 128),

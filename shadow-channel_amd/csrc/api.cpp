@@ -184,7 +184,7 @@ static void free_all(Context &c)
     void *ptrs[] = {c.dKx, c.dKz, c.dcfl2, c.dW, c.dQ, c.dQp, c.dQpp, c.dR, c.dRp, c.dQw,
                     c.dQpw, c.dQppw, c.dRw, c.dRpw, c.dQs, c.dQps, c.dQpps, c.dRs, c.dRps,
                     c.dRowsum, c.C, c.IC, c.AC, c.U, c.IU, c.AU, c.LU, c.Ut, c.Ut2, c.MC,
-                    c.MIC, c.Csol, c.Ctmp, c.Ctmp2, c.Fa, c.Fb, c.FLa, c.FLb, c.W1, c.W2, c.W2b,
+                    c.MIC, c.Csol, c.Ctmp, c.Ctmp2, c.Cprev, c.Fa, c.Fb, c.FLa, c.FLb, c.W1, c.W2, c.W2b,
                     c.SP, c.flowdev, c.cflmax, c.dotpart, c.scal, c.abort_flag, c.pz.tw,
                     c.px.tw, c.dFragR, c.dFragF, c.dFragP, c.dFragQ, c.wcache, c.rb_queue,
                     c.objscr, c.dRwall, c.dGwall};
@@ -323,10 +323,21 @@ static int ddt_project_dev(Context &c, int i, const cplx *vin, cplx *vout,
     long ip = i == 0 ? 3 : 3L * i, im = i == 0 ? 0 : 3L * (i - 1);
     // dudt = (MC[ip] - MC[im]) / dt; the inner products in closed form from
     // the coefficients (equal to the grid sums of the rebuilt fields)
-    const int nparts = launch_ddt_gram(c, mc(c, ip), mc(c, im), 1.0 / d.dt, vin);
+    const cplx *P = mc(c, ip), *M = mc(c, im), *P0 = nullptr, *M0 = nullptr;
+    if (i == 0 && c.use_prev) {
+        // members whose windows continue a trajectory: the backward
+        // difference from the state before the window, member 0 (the
+        // trajectory's start) keeping the forward difference
+        P = mc(c, 0);
+        M = c.Cprev;
+        P0 = mc(c, 3);
+        M0 = mc(c, 0);
+    }
+    const int nparts = launch_ddt_gram(c, P, M, 1.0 / d.dt, vin, c.dotpart, c.ndotpart,
+                                       c.abort_flag, P0, M0);
     if (nparts <= 0) return SC_EDEVICE;
-    launch_ddt_apply(c, vout, vin, mc(c, ip), mc(c, im), 1.0 / d.dt, c.csize, acc, a, nparts,
-                     twice);
+    launch_ddt_apply(c, vout, vin, P, M, 1.0 / d.dt, c.csize, acc, a, nparts, twice, c.dotpart,
+                     c.scal, P0, M0);
     return SC_OK;
 }
 
@@ -674,6 +685,7 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
         {(void **) &c.C, cb, true, false}, {(void **) &c.IC, cb, true, false},
         {(void **) &c.AC, cb, true, false}, {(void **) &c.Csol, cb, true, false},
         {(void **) &c.Ctmp, cb, true, false}, {(void **) &c.Ctmp2, cb, true, false},
+        {(void **) &c.Cprev, cb, true, false},
         {(void **) &c.U, ub, true, false}, {(void **) &c.IU, ub, true, false},
         {(void **) &c.AU, ub, true, false}, {(void **) &c.LU, ub, true, false},
         {(void **) &c.Ut, ub, true, false}, {(void **) &c.Ut2, ub, true, false},
@@ -1388,8 +1400,32 @@ int sc_member_state(int incre, int member, int i_step, sc_complex *out)
     return SC_OK;
 }
 
+// The state one step before each member's window (host, members one after
+// the other; member 0's entry is not used): afterwards the members' time
+// derivative at their step 0 (the adjoint's last forcing, sc_member_ddt_project
+// at step 0) is the backward difference from it, as on one long trajectory
+// cut into the members' windows.  NULL: the forward difference again.
+int sc_set_member_previous(const sc_complex *prev)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!prev) {
+        c.use_prev = false;
+        drop_graphs(c);
+        return SC_OK;
+    }
+    const size_t cb = sizeof(cplx) * c.csize;
+    for (int m = 0; m < c.nmem; ++m)
+        SC_HIP(hipMemcpy((char *) c.Cprev + (size_t) m * cb, prev + (size_t) m * c.csize, cb,
+                         hipMemcpyHostToDevice));
+    c.use_prev = true;
+    drop_graphs(c);
+    return SC_OK;
+}
+
 // c_ddt_project on member `member`'s stored trajectory (its own states;
-// the least-squares-shadowing segments run as members use it at their ends)
+// the least-squares-shadowing segments run as members use it at their ends;
+// at step 0 of a member with a previous state, the backward difference)
 double sc_member_ddt_project(int member, int i_step, sc_complex *v)
 {
     Context &c = ctx();
@@ -1404,6 +1440,11 @@ double sc_member_ddt_project(int member, int i_step, sc_complex *v)
     const long ip = i_step == 0 ? 3 : 3L * i_step, im = i_step == 0 ? 0 : 3L * (i_step - 1);
     const cplx *P = (const cplx *) ((const char *) mc(c, ip) + (size_t) member * c.mcbytes);
     const cplx *M = (const cplx *) ((const char *) mc(c, im) + (size_t) member * c.mcbytes);
+    if (i_step == 0 && c.use_prev && member > 0) {
+        // the backward difference from the state before the member's window
+        P = (const cplx *) ((const char *) mc(c, 0) + (size_t) member * c.mcbytes);
+        M = c.Cprev + (size_t) member * c.csize;
+    }
     if (hipMemcpyAsync(c.Ctmp, v, cb, hipMemcpyHostToDevice, c.stream) != hipSuccess ||
         hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream) != hipSuccess) {
         last_error() = "member_ddt_project: HIP transfer failed";

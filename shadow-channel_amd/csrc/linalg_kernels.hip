@@ -2550,13 +2550,18 @@ __global__ __launch_bounds__(256) void k_ddt_gram(const cplx *P, const cplx *M, 
                                                   const double *Kz, int Nz, int Mx, int dimR,
                                                   int dimQ, int ncol, double scale, double *part,
                                                   const int *abort_flag, size_t ms_p, size_t ms_m,
-                                                  size_t ms_v)
+                                                  size_t ms_v, const cplx *P0, const cplx *M0)
 {
     extern __shared__ double2 sh_ddt[];
     __shared__ double rs[8];
-    // member blockIdx.y: its partial pairs follow those of the members before
+    // member blockIdx.y: its partial pairs follow those of the members before;
+    // P0, M0 (when given) replace member 0's pair of states
     P += blockIdx.y * ms_p;
     M += blockIdx.y * ms_m;
+    if (blockIdx.y == 0 && P0) {
+        P = P0;
+        M = M0;
+    }
     V += blockIdx.y * ms_v;
     part += (size_t) blockIdx.y * 2 * gridDim.x;
     const int nxb = (Mx + ncol - 1) / ncol;
@@ -2713,7 +2718,8 @@ __global__ __launch_bounds__(256) void k_ddt_apply(cplx *vout, const cplx *vin, 
                                                    const cplx *M, double s, const double *part,
                                                    int nparts, double *scal, size_t n, cplx *acc,
                                                    double a, int twice, size_t ms_vo, size_t ms_vi,
-                                                   size_t ms_p, size_t ms_m, size_t ms_acc)
+                                                   size_t ms_p, size_t ms_m, size_t ms_acc,
+                                                   const cplx *P0, const cplx *M0)
 {
     __shared__ double rs[8];
     {
@@ -2722,6 +2728,10 @@ __global__ __launch_bounds__(256) void k_ddt_apply(cplx *vout, const cplx *vin, 
         vin += m * ms_vi;
         P += m * ms_p;
         M += m * ms_m;
+        if (m == 0 && P0) {
+            P = P0;
+            M = M0;
+        }
         if (acc) acc += m * ms_acc;
         part += (size_t) m * 2 * nparts;
         scal += 8 * m;
@@ -2855,7 +2865,8 @@ int launch_ddt_gram(Context &c, const cplx *P, const cplx *M, double s, const cp
 }
 
 int launch_ddt_gram(Context &c, const cplx *P, const cplx *M, double s, const cplx *V,
-                    double *part, size_t part_cap, const int *stop)
+                    double *part, size_t part_cap, const int *stop, const cplx *P0,
+                    const cplx *M0)
 {
     const Dims &d = c.d;
     const int ncol = ddt_gram_cols(d), nb = ddt_gram_blocks(d);
@@ -2870,7 +2881,7 @@ int launch_ddt_gram(Context &c, const cplx *P, const cplx *M, double s, const cp
     hipLaunchKernelGGL(k_ddt_gram, dim3(nb, c.nrun), dim3(256), lds, c.stream, P, M, V, s, c.dQs,
                        c.dQps, c.dRs, c.dKx, c.dKz, d.Nz, d.Mx, d.dimR, d.dimQ, ncol,
                        (double) d.Px * (double) d.Pz, part, stop, mstride(c, P), mstride(c, M),
-                       mstride(c, V));
+                       mstride(c, V), P0, M0);
     return nb;
 }
 
@@ -2882,12 +2893,13 @@ void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, co
 
 void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, const cplx *M,
                       double s, size_t n, cplx *acc, double a, int nparts, int twice,
-                      const double *part, double *scal)
+                      const double *part, double *scal, const cplx *P0, const cplx *M0)
 {
     const int nm = c.nrun;
     hipLaunchKernelGGL(k_ddt_apply, dim3(std::max(128, 1024 / nm), nm), dim3(256), 0, c.stream,
                        vout, vin, P, M, s, part, nparts, scal, n, acc, a, twice, mstride(c, vout),
-                       mstride(c, vin), mstride(c, P), mstride(c, M), acc ? mstride(c, acc) : 0);
+                       mstride(c, vin), mstride(c, P), mstride(c, M), acc ? mstride(c, acc) : 0,
+                       P0, M0);
 }
 
 }  // namespace sc

@@ -163,6 +163,11 @@ struct Context {
     cplx *MC = nullptr, *MIC = nullptr;
     size_t mcbytes = 0;                         // bytes of one member's MC (and MIC)
     cplx *Csol = nullptr, *Ctmp = nullptr, *Ctmp2 = nullptr;
+    // per member, the state one step before its window (sc_set_member_previous):
+    // with use_prev, a member's time derivative at its step 0 is the backward
+    // difference from it, except member 0's (the forward difference)
+    cplx *Cprev = nullptr;
+    bool use_prev = false;
     cplx *Fa = nullptr, *Fb = nullptr;          // [Nz][dimQ][Mx], [Nz][dimR][Mx]
     cplx *FLa = nullptr, *FLb = nullptr;        // adjoint next-stage projection
     cplx *W1 = nullptr, *W2 = nullptr, *W2b = nullptr, *SP = nullptr;
@@ -304,7 +309,8 @@ void launch_incre_boundary(Context &c, cplx *Uxb, cplx *Uzb, cplx *scratch);
 int launch_ddt_gram(Context &c, const cplx *P, const cplx *M, double s, const cplx *V);
 // (stop: the word whose nonzero value skips the work, abort_flag by default)
 int launch_ddt_gram(Context &c, const cplx *P, const cplx *M, double s, const cplx *V,
-                    double *part, size_t part_cap, const int *stop);
+                    double *part, size_t part_cap, const int *stop, const cplx *P0 = nullptr,
+                    const cplx *M0 = nullptr);
 // blocks (partial pairs) of the inner-product launch for a grid
 int ddt_gram_blocks(const Dims &d);
 // vout = vin - (num/den) s (P - M); with acc, also acc -= a vout (twice);
@@ -314,7 +320,8 @@ void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, co
                       double s, size_t n, cplx *acc, double a, int nparts, int twice);
 void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, const cplx *M,
                       double s, size_t n, cplx *acc, double a, int nparts, int twice,
-                      const double *part, double *scal);
+                      const double *part, double *scal, const cplx *P0 = nullptr,
+                      const cplx *M0 = nullptr);
 // the no-allocation entry's scratch array: its device address for the
 // current device, looked up once per device (0 on success)
 int noalloc_setup();
