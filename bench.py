@@ -10,8 +10,9 @@ host only launches.  Prints one JSON line.
 
 The headline value is the batched mixed-precision rate: 8 independent
 members (seeded states) advanced together in every kernel launch, with the
-x/z transforms and nonlinear products in FP32 (solves, y-transforms, flux
-correction and projections FP64), in member-steps per second.  Beside it:
+x/z transforms, the nonlinear products and the y-transform to the Gauss
+points in FP32 (Galerkin projections, solves, flux correction and the
+trapezoid projection FP64), in member-steps per second.  Beside it:
 the plain FP64 single-solution rate (the metric's denominator), the FP64
 batched rate, the rate without the transformed-field cache, and the wall-
 shear gradient of a 1000-step window (checkpoints every 50) in mixed
@@ -376,8 +377,9 @@ def run():
         "config": {"workload": "fwd+adj step: 1 primal RK3/CN step + 1 adjoint step "
                                "(strength 1) per member, %d independent members advanced "
                                "together in every kernel launch, mixed precision (x/z "
-                               "transforms and products FP32; solves, y-transforms, flux "
-                               "correction, projections FP64), states resident in HBM; "
+                               "transforms, products and the y-transform to the Gauss "
+                               "points FP32; Galerkin projections, solves, flux "
+                               "correction FP64), states resident in HBM; "
                                "value = member-steps/s" % nm,
                    "grid": "64x64x97 (96x145x96 dealiased)", "Lx": "2pi", "Lz": "pi",
                    "Re": c["Re"], "dt": c["dt"], "meanU": c["flux"] / 2,

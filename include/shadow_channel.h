@@ -226,12 +226,14 @@ int sc_set_graphs(int enable);
 
 /* Precision (no counterpart in the reference, which is FP64 throughout).
    sc_set_precision(1) makes the next c_init a mixed-precision context: the
-   x/z transforms of the time steps, the nonlinear products and the arrays
-   between the transforms (and the cached fields, at half the memory) in
-   FP32; the y-transforms, the band solves, the mean-mode flux correction,
-   the trapezoid projection and every state array stay FP64.  0 (default):
-   FP64 everywhere.  spec2phys is FP64 in both.  sc_precision() reports the
-   current context's setting. */
+   x/z transforms of the time steps, the nonlinear products, the arrays
+   between the transforms (and the cached fields, at half the memory) and
+   the y-transform to the Gauss points (FP32 matrix cores, FP32
+   accumulation; where dimR <= 96) in FP32; the Galerkin projections, the
+   band solves, the mean-mode flux correction, the trapezoid projection and
+   every state array stay FP64.  0 (default): FP64 everywhere.  spec2phys
+   is FP64 in both.  sc_precision() reports the current context's
+   setting. */
 int sc_set_precision(int mixed);
 int sc_precision(void);
 
@@ -288,6 +290,20 @@ double sc_member_ddt_project(int member, int i_step, sc_complex *v);
    member 0 keeps the forward difference.  NULL switches it off; c_init
    clears it. */
 int sc_set_member_previous(const sc_complex *prev);
+
+/* Windows of uneven length (no counterpart in the reference, whose drivers
+   run each least-squares-shadowing segment by a call of its own,
+   drivers/wrapper.py:14-54): len[m] (1 .. nsteps, one per member) is the
+   last step of member m's window.  sc_tangent_members then advances member
+   m over steps < len[m] only: its result is its state after step len[m]
+   (sc_member_state(1, m, len[m])); from there on the member is zero and
+   gets no mean-velocity source, so its later stored states are zero.
+   sc_adjoint_members starts member m at step min(len[m], start_step) from
+   its given state, with the forcing c_adjoint applies at a run's start;
+   before that the member is zero.  Each member's results equal a run of
+   its own over its window bit for bit.  sc_primal_members is not affected.
+   NULL: every member over the whole run; c_init clears it. */
+int sc_set_member_windows(const int *len);
 
 /* Sensitivities (no counterpart in the reference; the building blocks of the
    time-averaged wall-shear gradient):

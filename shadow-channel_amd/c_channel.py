@@ -130,6 +130,8 @@ _L.sc_member_ddt_project.argtypes = [_i, _i, _vp]
 _L.sc_member_ddt_project.restype = _d
 _L.sc_set_member_previous.argtypes = [_vp]
 _L.sc_set_member_previous.restype = _i
+_L.sc_set_member_windows.argtypes = [_vp]
+_L.sc_set_member_windows.restype = _i
 _L.sc_set_graphs.argtypes = [_i]
 _L.sc_set_graphs.restype = _i
 
@@ -598,3 +600,16 @@ def sc_set_member_previous(prev):
         return
     prev = np.ascontiguousarray(prev, dtype=np.complex128)
     _check(_L.sc_set_member_previous(prev.ctypes.data), "sc_set_member_previous")
+
+
+def sc_set_member_windows(lengths):
+    """Windows of their own for the members (the last step of member m's
+    window, one per member), or None: every member over the whole run."""
+    if lengths is None:
+        _check(_L.sc_set_member_windows(None), "sc_set_member_windows")
+        return
+    n = _L.sc_members()
+    w = np.ascontiguousarray(lengths, dtype=np.int32)
+    if w.shape != (n,):
+        raise TypeError("sc_set_member_windows needs one length per member (%d)" % n)
+    _check(_L.sc_set_member_windows(w.ctypes.data), "sc_set_member_windows")

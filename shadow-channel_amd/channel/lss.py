@@ -26,17 +26,20 @@ Two implementations of the same operator:
 * Segments: the reference's loop, one segment after another, on the single
   trajectory of the current run (any bounds; c_tangent, c_adjoint and
   c_ddt_project).
-* BatchedSegments: segments of equal length L as members of one batched
-  run.  Member s recomputes its segment of the base trajectory from the
-  state at step s L (bit for bit the stored one: the same kernels on the
-  same state), so every segment's tangent, forced adjoint and projections
-  advance in the same kernel launches.  The projection at a segment's
-  start is the one at the previous segment's end (the backward difference
-  at step s L uses steps s L - 1 and s L, both in member s-1's window);
-  segment 0 starts with the forward difference at step 0.  The operator is
-  the reference's bit for bit: each member's time derivative at its
-  window's first step is the backward difference from the state one step
-  before the window (sc_set_member_previous), as on the long trajectory.
+* BatchedSegments: the segments (of equal or uneven length) as members of
+  one batched run.  Member s recomputes its segment of the base trajectory
+  from the state at step bounds[s] (bit for bit the stored one: the same
+  kernels on the same state), so every segment's tangent, forced adjoint
+  and projections advance in the same kernel launches.  Every member's
+  window is as long as the longest segment; a shorter segment's tangent
+  ends inside it and its adjoint starts there (sc_set_member_windows).
+  The projection at a segment's start is the one at the previous
+  segment's end (the backward difference at step bounds[s] uses steps
+  bounds[s] - 1 and bounds[s], both in member s-1's window); segment 0
+  starts with the forward difference at step 0.  The operator is the
+  reference's bit for bit: each member's time derivative at its window's
+  first step is the backward difference from the state one step before
+  the window (sc_set_member_previous), as on the long trajectory.
 
 minres (Paige and Saunders' minimal-residual method) and cg take the
 operator as a callable and an inner product, like drivers/pariter.py:63-125.
@@ -103,17 +106,27 @@ class Segments:
 
 
 class BatchedSegments(Segments):
-    """Segments of equal length L advanced together as members.
+    """The segments advanced together as members.
 
-    setup(C0, ...) runs the base trajectory of n L steps once, keeps the
-    states at the segment starts, and makes a member context of n members
-    whose windows are the segments (c_init with nsteps L)."""
+    BatchedSegments(grid, box, Re, flux, dt, n_seg, seg_steps) cuts n_seg
+    segments of seg_steps steps; bounds=[0, b_1, ..., b_n] gives segments of
+    any lengths instead.  setup(C0) runs the base trajectory of b_n steps
+    once, keeps the states at the segment starts, and makes a member context
+    of n members whose windows are the segments (c_init with nsteps = the
+    longest segment)."""
 
-    def __init__(self, grid, box, Re, flux, dt, n_seg, seg_steps, mixed=False):
-        Segments.__init__(self, grid, [seg_steps * s for s in range(n_seg + 1)])
+    def __init__(self, grid, box, Re, flux, dt, n_seg=None, seg_steps=None, mixed=False,
+                 bounds=None):
+        if bounds is None:
+            bounds = [int(seg_steps) * s for s in range(int(n_seg) + 1)]
+        Segments.__init__(self, grid, bounds)
         self.box = tuple(float(b) for b in box)
         self.Re, self.flux, self.dt = float(Re), float(flux), float(dt)
-        self.L = int(seg_steps)
+        self.lens = [self.nb[i + 1] - self.nb[i] for i in range(self.n)]
+        if self.nb[0] != 0 or min(self.lens) < 1:
+            raise ValueError("bounds must start at 0 and increase")
+        self.L = max(self.lens)
+        self.uneven = min(self.lens) != self.L
         self.mixed = int(bool(mixed))
         self.starts = None
 
@@ -123,13 +136,13 @@ class BatchedSegments(Segments):
         grid = (self.Nx, self.Ny, self.Nz)
         c_channel.sc_set_precision(self.mixed)
         c_channel.sc_set_members(1)
-        assert c_channel.c_init(*grid, *self.box, self.Re, self.flux, self.dt, n * L) == 0
+        assert c_channel.c_init(*grid, *self.box, self.Re, self.flux, self.dt, self.nb[-1]) == 0
         C = np.ascontiguousarray(C0, dtype=np.complex128).copy()
         c_channel.c_primal(0, C)
-        self.starts = np.stack([c_channel.c_getsoln(s * L).copy() for s in range(n)])
+        self.starts = np.stack([c_channel.c_getsoln(self.nb[s]).copy() for s in range(n)])
         # the states one step before each window (segment 0 has none)
         prev = np.stack([np.zeros_like(self.starts[0])] +
-                        [c_channel.c_getsoln(s * L - 1).copy() for s in range(1, n)])
+                        [c_channel.c_getsoln(self.nb[s] - 1).copy() for s in range(1, n)])
         c_channel.c_destroy()
         c_channel.sc_set_members(n)
         assert c_channel.c_init(*grid, *self.box, self.Re, self.flux, self.dt, L) == 0
@@ -141,6 +154,8 @@ class BatchedSegments(Segments):
         # the members' step-0 time derivatives as on the long trajectory:
         # the backward difference (the adjoint's last forcing uses it)
         c_channel.sc_set_member_previous(prev)
+        # segments shorter than the window end inside it
+        c_channel.sc_set_member_windows(self.lens if self.uneven else None)
 
     def release(self):
         c_channel.c_destroy()
@@ -148,11 +163,13 @@ class BatchedSegments(Segments):
         c_channel.sc_set_precision(0)
 
     def _proj_start(self, s, v):
-        # step s L: member 0's step 0, or the end of member s - 1's window
-        return c_channel.sc_member_ddt_project(0 if s == 0 else s - 1, 0 if s == 0 else self.L, v)
+        # step bounds[s]: member 0's step 0, or the end of member s - 1's segment
+        if s == 0:
+            return c_channel.sc_member_ddt_project(0, 0, v)
+        return c_channel.sc_member_ddt_project(s - 1, self.lens[s - 1], v)
 
     def _proj_end(self, s, v):
-        return c_channel.sc_member_ddt_project(s, self.L, v)
+        return c_channel.sc_member_ddt_project(s, self.lens[s], v)
 
     def forward_all(self, V, inhomo):
         """forward(i, V[i]) of every segment: (end states, etas)."""
@@ -163,7 +180,12 @@ class BatchedSegments(Segments):
         d = c_channel.DeviceArray(V1)
         try:
             c_channel.sc_tangent_members(0, self.L, d, mask, False)
-            V1 = d.download()
+            if self.uneven:
+                # each member's state at the end of its own segment
+                V1 = np.stack([c_channel.sc_member_state(1, s, self.lens[s])
+                               for s in range(self.n)])
+            else:
+                V1 = d.download()
         finally:
             d.free()
         eta = np.array([self._proj_end(s, V1[s]) for s in range(self.n)])

@@ -184,7 +184,7 @@ static void free_all(Context &c)
     void *ptrs[] = {c.dKx, c.dKz, c.dcfl2, c.dW, c.dQ, c.dQp, c.dQpp, c.dR, c.dRp, c.dQw,
                     c.dQpw, c.dQppw, c.dRw, c.dRpw, c.dQs, c.dQps, c.dQpps, c.dRs, c.dRps,
                     c.dRowsum, c.C, c.IC, c.AC, c.U, c.IU, c.AU, c.LU, c.Ut, c.Ut2, c.MC,
-                    c.MIC, c.Csol, c.Ctmp, c.Ctmp2, c.Cprev, c.Fa, c.Fb, c.FLa, c.FLb, c.W1, c.W2, c.W2b,
+                    c.MIC, c.Csol, c.Ctmp, c.Ctmp2, c.Cprev, c.ACinit, c.Fa, c.Fb, c.FLa, c.FLb, c.W1, c.W2, c.W2b,
                     c.SP, c.flowdev, c.cflmax, c.dotpart, c.scal, c.abort_flag, c.pz.tw,
                     c.px.tw, c.dFragR, c.dFragF, c.dFragP, c.dFragQ, c.wcache, c.rb_queue,
                     c.objscr, c.dRwall, c.dGwall};
@@ -686,6 +686,7 @@ int c_init(int Nx, int Ny, int Nz, double Lx, double Lz, double Re, double flux,
         {(void **) &c.AC, cb, true, false}, {(void **) &c.Csol, cb, true, false},
         {(void **) &c.Ctmp, cb, true, false}, {(void **) &c.Ctmp2, cb, true, false},
         {(void **) &c.Cprev, cb, true, false},
+        {(void **) &c.ACinit, cb, true, false},
         {(void **) &c.U, ub, true, false}, {(void **) &c.IU, ub, true, false},
         {(void **) &c.AU, ub, true, false}, {(void **) &c.LU, ub, true, false},
         {(void **) &c.Ut, ub, true, false}, {(void **) &c.Ut2, ub, true, false},
@@ -794,6 +795,18 @@ static int copy_to_slots(Context &c, cplx *traj, long slot, const cplx *src)
                                 c.stream));
     }
     return SC_OK;
+}
+
+// member m's copy of a member array in the current member run
+static cplx *member_at(const Context &c, const cplx *p, int m)
+{
+    return (cplx *) ((const char *) p + (size_t) m * mstride_bytes(c, p));
+}
+
+// the member windows of sc_set_member_windows apply to this run
+static bool windowed(const Context &c)
+{
+    return c.nrun > 1 && (int) c.win.size() == c.nrun;
 }
 
 // primal run from a given state (host or device memory, per `in`/`out`) or
@@ -930,6 +943,14 @@ static int tangent_run(int start, int end, sc_complex *IC_given, unsigned inhomo
     SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
     // each CFL check clears the maxima it read; a run starts from zero
     SC_HIP(hipMemsetAsync(c.cflmax, 0, sizeof(double) * c.d.qpts, c.stream));
+    // member windows: a member is zero from the end of its window on (a
+    // zero tangent stays zero without the mean-velocity source), so the
+    // stored states after its window are zero and its result stays at
+    // MIC[3 win]
+    const bool windows = windowed(c);
+    if (windows)
+        for (int m = 0; m < c.nrun; ++m)
+            if (c.win[m] <= start) launch_zero(c, member_at(c, c.IC, m), c.csize);
     launch_rebuild(c, c.IC, c.IU);
     c.status = 0;
     const std::vector<double> key = {1, (double) start, (double) end, (double) c.nrun,
@@ -937,6 +958,10 @@ static int tangent_run(int start, int end, sc_complex *IC_given, unsigned inhomo
                                      (double) re_source};
     int rc = stepping(c, key, [&](bool capturing) {
     for (int n = start; n < end; ++n) {
+        unsigned mask = inhomo_mask;
+        if (windows)
+            for (int m = 0; m < c.nrun; ++m)
+                if (c.win[m] <= n) mask &= ~(1u << m);
         for (int k = 0; k < 3; ++k) {
             const cplx *w = wcached(c, 3L * n + k);
             if (!w) launch_rebuild(c, mc(c, 3L * n + k), c.U);
@@ -950,10 +975,16 @@ static int tangent_run(int start, int end, sc_complex *IC_given, unsigned inhomo
             if (re_source)
                 launch_re_source(c, mc(c, 3L * n + k), mc(c, 3L * n + k + 1), c.d.dt * RA[k],
                                  c.d.dt * RB[k], c.Ctmp2);
-            launch_solve(c, c.IC, Cs, k, 0, c.Fa, c.Fb, 0.0, 2.0, inhomo_mask,
+            launch_solve(c, c.IC, Cs, k, 0, c.Fa, c.Fb, 0.0, 2.0, mask,
                          re_source ? c.Ctmp2 : nullptr);
             launch_rebuild(c, Cs, c.IU);
         }
+        if (windows)
+            for (int m = 0; m < c.nrun; ++m)
+                if (c.win[m] == n + 1 && n + 1 < end) {
+                    launch_zero(c, member_at(c, c.IC, m), c.csize);
+                    launch_zero(c, member_at(c, c.IU, m), mstride_bytes(c, c.IU) / sizeof(cplx));
+                }
         if (((n - start) & 63) == 63 && !capturing) {
             int flag = 0;
             SC_HIP(hipMemcpy(&flag, c.abort_flag, sizeof(int), hipMemcpyDeviceToHost));
@@ -1036,6 +1067,29 @@ static void adjoint_stages(Context &c, int n, int exact_mean)
         }
 }
 
+// Member m of a windowed adjoint member run starts at step n: its given
+// state, lambda rebuilt from it, and the forcing c_adjoint applies at its
+// start_step (acc -= a v with v the projected MIC[3n]), on that member alone.
+// Whatever the member held before (zero, or forcing read from a tangent
+// continued past its window) is overwritten.
+static int start_member(Context &c, int m, int n, double a)
+{
+    cplx *ac = member_at(c, c.AC, m), *au = member_at(c, c.AU, m);
+    const cplx *init = member_at(c, c.ACinit, m), *vin = member_at(c, mic(c, 3L * n), m);
+    const cplx *P = member_at(c, mc(c, 3L * n), m), *M = member_at(c, mc(c, 3L * (n - 1)), m);
+    SC_HIP(hipMemcpyAsync(ac, init, sizeof(cplx) * c.csize, hipMemcpyDeviceToDevice, c.stream));
+    const int nrun = c.nrun;
+    c.nrun = 1;                 // the launches below see one member at these addresses
+    launch_rebuild(c, ac, au, true);
+    const int nparts = launch_ddt_gram(c, P, M, 1.0 / c.d.dt, vin, c.dotpart, c.ndotpart,
+                                       c.abort_flag);
+    if (nparts > 0)
+        launch_ddt_apply(c, c.Ctmp, vin, P, M, 1.0 / c.d.dt, c.csize, ac, a, nparts, 0,
+                         c.dotpart, c.scal);
+    c.nrun = nrun;
+    return nparts > 0 ? SC_OK : SC_EDEVICE;
+}
+
 static int adjoint_run(int start, int end, sc_complex *AC_given, double strength, hipMemcpyKind in,
                        hipMemcpyKind out)
 {
@@ -1044,12 +1098,24 @@ static int adjoint_run(int start, int end, sc_complex *AC_given, double strength
     const size_t cbm = sizeof(cplx) * c.csize * c.nrun;
     SC_HIP(hipMemcpyAsync(c.AC, AC_given, cbm, in, c.stream));
     SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
+    // member windows: a member whose window ends before `start` waits at
+    // zero and takes its given state at its window's end
+    const bool windows = windowed(c);
+    if (windows) {
+        SC_HIP(hipMemcpyAsync(c.ACinit, c.AC, cbm, hipMemcpyDeviceToDevice, c.stream));
+        for (int m = 0; m < c.nrun; ++m)
+            if (c.win[m] < start) launch_zero(c, member_at(c, c.AC, m), c.csize);
+    }
     launch_rebuild(c, c.AC, c.AU, true);      // lambda: velocity slots only
     const double s = strength * d.dt;
     const std::vector<double> key = {2, (double) start, (double) end, (double) c.nrun,
                                      (double) c.shared_base, s};
     int rc = stepping(c, key, [&](bool) {
         for (int n = start; n > end; --n) {
+            if (windows)
+                for (int m = 0; m < c.nrun; ++m)
+                    if (c.win[m] == n && n < start && start_member(c, m, n, 0.5 * s))
+                        return SC_EDEVICE;
             // The projection of MIC[3n] at step n was made at the end of the
             // previous iteration (the reference computes it twice), which also
             // applied it for this step's start.
@@ -1200,7 +1266,9 @@ int c_spec2phys(sc_complex *C_ptr, int Nz_dup, int Nvar_dup, int Ny_dup, int Nx_
     }
     SC_HIP(hipMemcpyAsync(c.Ctmp, C_ptr, sizeof(cplx) * c.csize, hipMemcpyHostToDevice, c.stream));
     SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
+    c.fp64_out = true;                          // an output: FP64 transforms
     launch_rebuild(c, c.Ctmp, c.Ut, true);    // spec2phys reads u, v, w only
+    c.fp64_out = false;
     FieldSpec fs;
     fs.nf = 3;
     for (int s = 0; s < 3; ++s) {
@@ -1419,6 +1487,28 @@ int sc_set_member_previous(const sc_complex *prev)
         SC_HIP(hipMemcpy((char *) c.Cprev + (size_t) m * cb, prev + (size_t) m * c.csize, cb,
                          hipMemcpyHostToDevice));
     c.use_prev = true;
+    drop_graphs(c);
+    return SC_OK;
+}
+
+// Windows of their own for the members of a run: member m advances over
+// steps < len[m] of the tangent member runs and starts at step len[m] of the
+// adjoint member runs (drivers/wrapper.py:14-54 runs each segment by a call
+// of its own; here segments of uneven length share the launches)
+int sc_set_member_windows(const int *len)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (len) {
+        for (int m = 0; m < c.nmem; ++m)
+            if (len[m] < 1 || len[m] > c.d.nsteps) {
+                last_error() = "member windows must lie in 1 .. nsteps";
+                return SC_EINVAL;
+            }
+        c.win.assign(len, len + c.nmem);
+    } else {
+        c.win.clear();
+    }
     drop_graphs(c);
     return SC_OK;
 }

@@ -28,6 +28,35 @@ __device__ __forceinline__ mf4 mfma_f64(double a, double b, mf4 c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// The FP32 form for the mixed-precision rebuild, v_mfma_f32_16x16x4f32: the
+// same operand layout (A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k),
+// but result row i sits in lane group i / 4, item i % 4, where the FP64 form
+// puts it in lane group i % 4, item i / 4 (both measured on gfx950).  The
+// FP32 instance therefore feeds A row i with the data of row
+// 4 (i % 4) + i / 4 (mfma_arow), which leaves every result where the FP64
+// instance, and the shared store code, have it.
+typedef float ff4 __attribute__((ext_vector_type(4)));
+template <class T> __device__ __forceinline__ int mfma_arow(int i)
+{
+    return sizeof(T) == sizeof(double) ? i : ((i & 3) << 2) | (i >> 2);
+}
+
+__device__ __forceinline__ mf4 mfma_t(double a, double b, mf4 c) { return mfma_f64(a, b, c); }
+__device__ __forceinline__ ff4 mfma_t(float a, float b, ff4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// accumulator type of the y-transform products in arithmetic type T
+template <class T> struct YAcc;
+template <> struct YAcc<double> { typedef mf4 type; };
+template <> struct YAcc<float> { typedef ff4 type; };
+template <class T> __device__ __forceinline__ typename YAcc<T>::type yacc_zero()
+{
+    typename YAcc<T>::type z = {0, 0, 0, 0};
+    return z;
+}
+
 // The 16 real columns of an n-tile are eight complex columns; real column n
 // is complex column 2(n%4) + n/8, part (n/4)%2.  Item r of lane l then holds
 // complex column 2(l/16) + r/2, part r%2: both parts of two neighbouring
@@ -274,9 +303,9 @@ __device__ __forceinline__ RebuildTask rebuild_task(const RebuildArgs &a, int zr
 
 // Gauss-point values of one task from its five accumulators (D[n][y] layout)
 // VEL: the velocity slots u, v, w only (acc[2], acc[4] unused)
-template <bool VEL = false>
+template <bool VEL = false, class ACC = mf4>
 __device__ __forceinline__ void rebuild_store(const RebuildArgs &a, int zrow, int nt, int y, int kq,
-                                              const mf4 *acc, cplx *U)
+                                              const ACC *acc, cplx *U)
 {
     const int Mx = a.Mx, q = a.qpts;
     if (y >= q) return;
@@ -519,10 +548,11 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_rows(RebuildArgs a, const cp
 // statically: wave w of the bpq blocks of an (XCD, y-tile) takes the tasks
 // w, w + 4 bpq, ..., and issues its first coefficient loads before it
 // stages the basis.
-template <int NKS, bool VEL>
+template <int NKS, bool VEL, class T>
 __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cplx *C, cplx *U)
 {
-    __shared__ double bs[NKS * 5 * 64];
+    using ACC = typename YAcc<T>::type;
+    __shared__ T bs[NKS * 5 * 64];
     if (*a.abort_flag) return;
     const int xcd = blockIdx.x & 7, ib = blockIdx.x >> 3;
     const int yt = ib % a.nyt;
@@ -552,7 +582,7 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cpl
     };
     constexpr int NKH = NKS / 2, CK = NKS / 4;
     static_assert(CK % 2 == 0, "a quarter must be whole pairs of k-steps");
-    double ca[2][CK], cb[2][CK];
+    T ca[2][CK], cb[2][CK];
     // quarter ch: parity ch / 2, k-steps (ch % 2) CK .. + CK - 1 of that
     // parity, as buffer loads over the whole coefficient array: a lane
     // offset plus a uniform k-step offset.  Rows past dimR - 1 (whose basis
@@ -567,36 +597,36 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cpl
 #pragma unroll
         for (int s = 0; s < CK; ++s) {
             const int so = (s0 + s) * 8 * M2 * 8;
-            ca[buf][s] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, va, so, 0));
-            cb[buf][s] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, vb, so, 0));
+            ca[buf][s] = (T) __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, va, so, 0));
+            cb[buf][s] = (T) __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, vb, so, 0));
         }
     };
-    auto chunk = [&](bool mean, int ch, int buf, mf4 *acc) {
-        const double *xa = ca[buf], *xb = cb[buf];
+    auto chunk = [&](bool mean, int ch, int buf, ACC *acc) {
+        const T *xa = ca[buf], *xb = cb[buf];
         const int sb = (ch >> 1) * NKH + (ch & 1) * CK;
         if (mean) {
 #pragma unroll
             for (int t = 0; t < CK; ++t) {
-                acc[3] = mfma_f64(xa[t], bs[((sb + t) * 5 + 3) * 64 + lane], acc[3]);
-                if (!VEL) acc[4] = mfma_f64(xa[t], bs[((sb + t) * 5 + 4) * 64 + lane], acc[4]);
+                acc[3] = mfma_t(xa[t], bs[((sb + t) * 5 + 3) * 64 + lane], acc[3]);
+                if (!VEL) acc[4] = mfma_t(xa[t], bs[((sb + t) * 5 + 4) * 64 + lane], acc[4]);
             }
             return;
         }
-        double p0[10], p1[10];
-        auto ldb = [&](int s0, double *p) {
+        T p0[10], p1[10];
+        auto ldb = [&](int s0, T *p) {
 #pragma unroll
             for (int e = 0; e < 10; ++e)
                 if (!VEL || (e % 5 != 2 && e % 5 != 4)) p[e] = bs[(s0 * 5 + e) * 64 + lane];
         };
-        auto mm = [&](int t0, const double *p) {
+        auto mm = [&](int t0, const T *p) {
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
                 const int t = t0 + u;
-                acc[0] = mfma_f64(xa[t], p[u * 5 + 0], acc[0]);
-                acc[1] = mfma_f64(xa[t], p[u * 5 + 1], acc[1]);
-                if (!VEL) acc[2] = mfma_f64(xa[t], p[u * 5 + 2], acc[2]);
-                acc[3] = mfma_f64(xb[t], p[u * 5 + 3], acc[3]);
-                if (!VEL) acc[4] = mfma_f64(xb[t], p[u * 5 + 4], acc[4]);
+                acc[0] = mfma_t(xa[t], p[u * 5 + 0], acc[0]);
+                acc[1] = mfma_t(xa[t], p[u * 5 + 1], acc[1]);
+                if (!VEL) acc[2] = mfma_t(xa[t], p[u * 5 + 2], acc[2]);
+                acc[3] = mfma_t(xb[t], p[u * 5 + 3], acc[3]);
+                if (!VEL) acc[4] = mfma_t(xb[t], p[u * 5 + 4], acc[4]);
             }
         };
         ldb(sb, p0);
@@ -614,7 +644,7 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cpl
         }
     };
     auto task_ops = [&](int t, int zr, int nt) {
-        RebuildTask r = rebuild_task(a, zr, nt, nl);
+        RebuildTask r = rebuild_task(a, zr, nt, mfma_arow<T>(nl));
         r.cd = (const double *) rb_src(a, C, t / ntask1);
         return r;
     };
@@ -632,9 +662,9 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cpl
         static_assert(NKS * 5 * 64 % 256 == 0, "basis must split over the block");
         double st[PT];
 #pragma unroll
-        for (int i = 0; i < PT; ++i) st[i] = frag[threadIdx.x + 256 * i];
+        for (int i = 0; i < PT; ++i) st[i] = frag[threadIdx.x + 256 * i];   // converted when stored
 #pragma unroll
-        for (int i = 0; i < PT; ++i) bs[threadIdx.x + 256 * i] = st[i];
+        for (int i = 0; i < PT; ++i) bs[threadIdx.x + 256 * i] = (T) st[i];
     }
     __syncthreads();
     while (t < ntask) {
@@ -642,11 +672,11 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cpl
         task_of(t, zr, nt);
         const RebuildTask cur = task_ops(t, zr, nt);
         const bool mean = zr == zm;
-        mf4 ae[5], ao[5];
+        ACC ae[5], ao[5];
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
-            ae[m] = mf4_zero();
-            ao[m] = mf4_zero();
+            ae[m] = yacc_zero<T>();
+            ao[m] = yacc_zero<T>();
         }
         int tn = ntask;
 #pragma unroll
@@ -670,7 +700,7 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cpl
             // values at y (E + O), then at the mirror point (s (E - O))
 #pragma unroll
             for (int m = 0; m < 5; ++m) {
-                const mf4 e = ae[m], o = ao[m];
+                const ACC e = ae[m], o = ao[m];
                 ae[m] = e + o;
                 ao[m] = (m == 1 || m == 4) ? o - e : e - o;
             }
@@ -720,21 +750,26 @@ void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx 
         ap.bpq = std::max(1, std::min((ntmax + 3) / 4, rbb / c.nht));
     }
     const dim3 gpar(8 * c.nht * ap.bpq);
+    // mixed precision: the parity rebuild's products on the FP32 matrix cores
+    // (the state slots stay FP64), except for outputs such as spec2phys
+    const bool f32 = c.mixed && !c.fp64_out;
+    typedef void (*RebuildKernel)(RebuildArgs, const cplx *, cplx *);
+#define SC_RB_PAR(N, V) (f32 ? (RebuildKernel) k_rebuild_par<N, V, float> : (RebuildKernel) k_rebuild_par<N, V, double>)
     auto go = [=]() {
         if (par && vel_only) {
             if (nksp == 8)
-                hipLaunchKernelGGL((k_rebuild_par<8, true>), gpar, dim3(256), 0, st, ap, C0, U0);
+                hipLaunchKernelGGL(SC_RB_PAR(8, true), gpar, dim3(256), 0, st, ap, C0, U0);
             else if (nksp == 16)
-                hipLaunchKernelGGL((k_rebuild_par<16, true>), gpar, dim3(256), 0, st, ap, C0, U0);
+                hipLaunchKernelGGL(SC_RB_PAR(16, true), gpar, dim3(256), 0, st, ap, C0, U0);
             else
-                hipLaunchKernelGGL((k_rebuild_par<24, true>), gpar, dim3(256), 0, st, ap, C0, U0);
+                hipLaunchKernelGGL(SC_RB_PAR(24, true), gpar, dim3(256), 0, st, ap, C0, U0);
         } else if (par) {
             if (nksp == 8)
-                hipLaunchKernelGGL((k_rebuild_par<8, false>), gpar, dim3(256), 0, st, ap, C0, U0);
+                hipLaunchKernelGGL(SC_RB_PAR(8, false), gpar, dim3(256), 0, st, ap, C0, U0);
             else if (nksp == 16)
-                hipLaunchKernelGGL((k_rebuild_par<16, false>), gpar, dim3(256), 0, st, ap, C0, U0);
+                hipLaunchKernelGGL(SC_RB_PAR(16, false), gpar, dim3(256), 0, st, ap, C0, U0);
             else
-                hipLaunchKernelGGL((k_rebuild_par<24, false>), gpar, dim3(256), 0, st, ap, C0, U0);
+                hipLaunchKernelGGL(SC_RB_PAR(24, false), gpar, dim3(256), 0, st, ap, C0, U0);
         } else if (nks == 8)
             hipLaunchKernelGGL(k_rebuild_rows<8>, grows, dim3(256), 0, st, a, C0, U0);
         else if (nks == 16)
@@ -749,6 +784,7 @@ void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx 
     go();
     prof_end(c, st, 4, go);
 }
+#undef SC_RB_PAR
 
 void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst, bool vel_only)
 {
@@ -2900,6 +2936,19 @@ void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, co
                        vout, vin, P, M, s, part, nparts, scal, n, acc, a, twice, mstride(c, vout),
                        mstride(c, vin), mstride(c, P), mstride(c, M), acc ? mstride(c, acc) : 0,
                        P0, M0);
+}
+
+__global__ __launch_bounds__(256) void k_zero(cplx *p, size_t n)
+{
+    for (size_t e = blockIdx.x * (size_t) blockDim.x + threadIdx.x; e < n;
+         e += (size_t) gridDim.x * blockDim.x)
+        p[e] = make_double2(0.0, 0.0);
+}
+
+void launch_zero(Context &c, cplx *p, size_t n)
+{
+    const size_t nb = std::min<size_t>(1024, (n + 255) / 256);
+    if (nb) hipLaunchKernelGGL(k_zero, dim3((unsigned) nb), dim3(256), 0, c.stream, p, n);
 }
 
 }  // namespace sc

@@ -108,9 +108,12 @@ struct Context {
     // signature, captured at its first call and replayed after; dropped when
     // the trajectory's cached fields change or the context is freed
     bool use_graphs = false;
-    // mixed precision (sc_set_precision): the transforms of the time steps in
-    // FP32, everything else FP64; fixed at c_init
+    // mixed precision (sc_set_precision): the x/z transforms and the rebuild
+    // of the time steps in FP32, everything else FP64; fixed at c_init
     bool mixed = false;
+    // set while an output in physical space is formed (spec2phys): its
+    // transforms run in FP64 whatever the precision of the steps
+    bool fp64_out = false;
     struct GraphEntry {
         std::vector<double> key;
         hipGraphExec_t exec;
@@ -168,6 +171,12 @@ struct Context {
     // difference from it, except member 0's (the forward difference)
     cplx *Cprev = nullptr;
     bool use_prev = false;
+    // per-member windows (sc_set_member_windows): member m of a member run
+    // advances over steps < win[m] only (empty: every member over the whole
+    // run); ACinit keeps the adjoint members' given states until their
+    // windows' ends
+    std::vector<int> win;
+    cplx *ACinit = nullptr;
     cplx *Fa = nullptr, *Fb = nullptr;          // [Nz][dimQ][Mx], [Nz][dimR][Mx]
     cplx *FLa = nullptr, *FLb = nullptr;        // adjoint next-stage projection
     cplx *W1 = nullptr, *W2 = nullptr, *W2b = nullptr, *SP = nullptr;
@@ -282,6 +291,8 @@ SC_FFT_LAUNCHERS
 // vel_only: only the velocity slots u, v, w are needed (du/dy and dw/dy may
 // be left unwritten where the kernel allows it)
 void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst, bool vel_only = false);
+// n complex values at p <- 0 (a kernel, so that graph capture records no memset)
+void launch_zero(Context &c, cplx *p, size_t n);
 // two rebuilds in one launch where the kernel allows it
 void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx *U1,
                      bool vel_only = false);

@@ -7,7 +7,9 @@
   mean-velocity source, at the one-window bar of tests/test_gpu_windows.py
   (1e-9).
 * BatchedSegments (every segment a member; one launch per stage for all
-  segments) equal to Segments bit for bit, in FP64 and in mixed precision.
+  segments) equal to Segments bit for bit, in FP64 and in mixed precision,
+  for segments of equal length and of uneven length (3, 5 and 4 steps:
+  the shorter segments end inside the members' common window).
 * The drivers' solve: the right-hand side -K(0, inhomo 1) - K(0, inhomo 0)
   and a few MINRES iterations on the batched operator reduce the KKT
   residual (drivers/wrapper.py:78-134)."""
@@ -58,13 +60,14 @@ class _OracleBackend:
         self.o.adjoint(start, end, AC, inhomo, strength)
 
 
-def _sequential(cc, x, inhomo):
+def _sequential(cc, x, inhomo, bounds=None):
     from channel.lss import Segments
     c = G
+    bounds = bounds or [L * s for s in range(NSEG + 1)]
     assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"],
-                     NSEG * L) == 0
+                     bounds[-1]) == 0
     cc.c_primal(0, _C0())
-    seg = Segments((c["Nx"], c["Ny"], c["Nz"]), [L * s for s in range(NSEG + 1)])
+    seg = Segments((c["Nx"], c["Ny"], c["Nz"]), bounds)
     r = seg.matvec(x, inhomo)
     cc.c_destroy()
     return r, seg.zeta.copy()
@@ -95,21 +98,28 @@ def test_kkt_operator_against_oracle(cc):
     o.destroy()
 
 
-@pytest.mark.parametrize("mixed", [0, 1])
-def test_batched_segments_equal_the_loop(cc, mixed):
+@pytest.mark.parametrize("mixed,bounds", [(0, None), (1, None), (0, [0, 3, 8, 12]),
+                                          (1, [0, 3, 8, 12])])
+def test_batched_segments_equal_the_loop(cc, mixed, bounds):
     from channel.lss import BatchedSegments, Segments
     c = G
-    print("case: batched segments vs the segment loop, mixed %d" % mixed, flush=True)
+    print("case: batched segments vs the segment loop, mixed %d, bounds %s" % (mixed, bounds),
+          flush=True)
     x = _x(Segments((c["Nx"], c["Ny"], c["Nz"]), [L * s for s in range(NSEG + 1)]), 2)
     ref = {}
     cc.sc_set_precision(mixed)
     try:
         for inhomo in (0, 1):
-            ref[inhomo] = _sequential(cc, x, inhomo)
+            ref[inhomo] = _sequential(cc, x, inhomo, bounds)
     finally:
         cc.sc_set_precision(0)
-    bs = BatchedSegments((c["Nx"], c["Ny"], c["Nz"]), (c["Lx"], c["Lz"]), c["Re"], c["flux"],
-                         c["dt"], NSEG, L, mixed=bool(mixed))
+    if bounds is None:
+        bs = BatchedSegments((c["Nx"], c["Ny"], c["Nz"]), (c["Lx"], c["Lz"]), c["Re"], c["flux"],
+                             c["dt"], NSEG, L, mixed=bool(mixed))
+    else:
+        bs = BatchedSegments((c["Nx"], c["Ny"], c["Nz"]), (c["Lx"], c["Lz"]), c["Re"], c["flux"],
+                             c["dt"], mixed=bool(mixed), bounds=bounds)
+        assert bs.uneven and bs.L == 5
     bs.setup(_C0())
     try:
         for inhomo in (0, 1):

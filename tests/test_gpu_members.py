@@ -104,6 +104,69 @@ def test_members_equal_single_runs(cc, grid):
         np.testing.assert_array_equal(As[m], A)
 
 
+WIN = [6, 2, 5, 3]
+
+
+@pytest.mark.parametrize("grid", ["16x33x12", "64x17x64"])
+def test_member_windows_equal_single_runs(cc, grid):
+    """Uneven windows along one base trajectory (sc_set_member_windows):
+    member m's tangent over steps 0 .. WIN[m] (members 1 and 2 with the
+    mean-velocity source) and its forced adjoint from WIN[m] back to 0,
+    against c_tangent(0, WIN[m]) and c_adjoint(WIN[m], 0) on member 0's
+    trajectory; eager, recorded and replayed graph runs alike."""
+    cfg = GRIDS[grid]
+    c = cfg
+    nm, n = len(WIN), max(WIN)
+    print("case: %d members with windows %s, %s" % (nm, WIN, grid), flush=True)
+    cc.sc_set_members(nm)
+    assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"],
+                     n) == 0
+    C0 = [_st(cfg, s, amp=0.1) for s in range(nm)]
+    D0 = [_st(cfg, 10 + s, unit=True) for s in range(nm)]
+    A0 = [_st(cfg, 20 + s, unit=True) for s in range(nm)]
+    _run_members(cc, cc.sc_primal_members, C0, 0, ())
+    with pytest.raises(AssertionError):
+        cc.sc_set_member_windows([0, 2, 5, 3])
+    with pytest.raises(AssertionError):
+        cc.sc_set_member_windows([n + 1, 2, 5, 3])
+    cc.sc_set_member_windows(WIN)
+    runs = []
+    try:
+        for graphs in (0, 1, 1):
+            cc.sc_set_graphs(graphs)
+            Iw = _run_members(cc, cc.sc_tangent_members, D0, 0, n, (0b0110, True))
+            ends = [cc.sc_member_state(1, m, WIN[m]) for m in range(nm)]
+            after = [[cc.sc_member_state(1, m, i) for i in range(WIN[m] + 1, n + 1)]
+                     for m in range(nm)]
+            Aw = _run_members(cc, cc.sc_adjoint_members, A0, n, 0, (1.0, True))
+            runs.append((Iw, ends, after, Aw))
+    finally:
+        cc.sc_set_graphs(0)
+    for Iw, ends, after, Aw in runs[1:]:
+        for m in range(nm):
+            np.testing.assert_array_equal(ends[m], runs[0][1][m])
+            np.testing.assert_array_equal(Aw[m], runs[0][3][m])
+    Iw, ends, after, Aw = runs[0]
+    for m in range(nm):
+        # a member past its window is zero, and so are its stored states
+        if WIN[m] < n:
+            assert not np.any(Iw[m])
+        for a in after[m]:
+            assert not np.any(a)
+    cc.sc_set_member_windows(None)
+    C = C0[0].copy()
+    cc.c_primal(0, C)                     # member 0's trajectory again
+    for m in range(nm):
+        I = D0[m].copy()
+        cc.c_tangent(0, WIN[m], I, 1 if m in (1, 2) else 0)
+        np.testing.assert_array_equal(ends[m], I)
+        A = A0[m].copy()
+        cc.c_adjoint(WIN[m], 0, A, 0, 1.0)   # forced by member m's tangent, stored just now
+        np.testing.assert_array_equal(Aw[m], A)
+    cc.c_destroy()
+    cc.sc_set_members(1)
+
+
 def test_member_count_is_checked(cc):
     with pytest.raises(AssertionError):
         cc.sc_set_members(0)
