@@ -8,8 +8,8 @@ unit the least-squares-shadowing drivers repeat per time step
 back over the same window, with the states resident in device memory; the
 host only launches.  Prints one JSON line.
 
-The headline value is the batched mixed-precision rate: 8 independent
-members (seeded states) advanced together in every kernel launch, with the
+The headline value is the batched mixed-precision rate: 32 independent
+members (seeded states; --members) advanced together in every kernel launch, with the
 x/z transforms, the nonlinear products and the y-transform to the Gauss
 points in FP32 (Galerkin projections, solves, flux correction and the
 trapezoid projection FP64), in member-steps per second.  Beside it:
@@ -196,15 +196,16 @@ def gradient_errors(cc):
 
 
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-# (scaled by a copy of known size), per kernel class of the roofline replay in
-# the headline configuration (8 members, mixed precision)
+# (scaled by a copy of known size), per kernel class of the roofline replay and
+# member count, mixed precision
 FLUSH_MB = 320       # above the 256 MB Infinity Cache
 
-TRAFFIC_PROFILES = {6: "profiles/solve_8members_mixed_64x64x97_pmc.md"}
+TRAFFIC_PROFILES = {(6, 8): "profiles/solve_8members_mixed_64x64x97_pmc.md",
+                    (6, 32): "profiles/solve_32members_mixed_64x64x97_pmc.md"}
 
 
-def measured_traffic(kernel_class):
-    path = TRAFFIC_PROFILES.get(kernel_class)
+def measured_traffic(kernel_class, members):
+    path = TRAFFIC_PROFILES.get((kernel_class, members))
     if not path or not os.path.exists(os.path.join(ROOT, path)):
         return None
     for ln in open(os.path.join(ROOT, path)):
@@ -275,7 +276,7 @@ def run():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gradient", action="store_true",
                     help="skip the untimed gradient-error check")
-    ap.add_argument("--members", type=int, default=8,
+    ap.add_argument("--members", type=int, default=32,
                     help="independent members advanced together in the headline figure")
     ap.add_argument("--retune", action="store_true",
                     help="time the launch shapes instead of using channel/shapes.json")
@@ -352,7 +353,7 @@ def run():
     mrate, melapsed, rf, shapes = member_rate(cc, states, K, W, 1, args.roofline_class, args.replays)
     n, ms, nbytes, avg_ms, bytes1 = rf
     achieved = bytes1 / (avg_ms * 1e-3) / 1e9
-    traffic = measured_traffic(args.roofline_class)
+    traffic = measured_traffic(args.roofline_class, nm)
     roof = {"kernel": CLASS_NAMES.get(args.roofline_class, str(args.roofline_class)),
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

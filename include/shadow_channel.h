@@ -293,14 +293,15 @@ int sc_set_member_previous(const sc_complex *prev);
 
 /* Windows of uneven length (no counterpart in the reference, whose drivers
    run each least-squares-shadowing segment by a call of its own,
-   drivers/wrapper.py:14-54): len[m] (1 .. nsteps, one per member) is the
+   drivers/wrapper.py:14-54): len[m] (0 .. nsteps, one per member) is the
    last step of member m's window.  sc_tangent_members then advances member
    m over steps < len[m] only: its result is its state after step len[m]
    (sc_member_state(1, m, len[m])); from there on the member is zero and
    gets no mean-velocity source, so its later stored states are zero.
    sc_adjoint_members starts member m at step min(len[m], start_step) from
    its given state, with the forcing c_adjoint applies at a run's start;
-   before that the member is zero.  Each member's results equal a run of
+   before that the member is zero (len[m] = 0: zero throughout; a tangent
+   member with len[m] <= start_step is zero from the start).  Each member's results equal a run of
    its own over its window bit for bit.  sc_primal_members is not affected.
    NULL: every member over the whole run; c_init clears it. */
 int sc_set_member_windows(const int *len);
@@ -324,6 +325,17 @@ int sc_set_member_windows(const int *len);
      i times that in Im d). */
 int sc_tangent_sources(int start_step, int end_step, sc_complex *IC, int inhomo, int re_source);
 int sc_adjoint_objective(int start_step, int end_step, sc_complex *AC, double weight);
+/* Members (no counterpart in the reference): sc_adjoint_objective of every
+   member in each launch, AC_dev on the device (members one after the
+   other), each member's own trajectory (shared_base 0) or member 0's; with
+   member windows (sc_set_member_windows) member m starts at step len[m]
+   from its given state and is zero before (len[m] = 0: zero throughout).
+   sc_member_mean_profile: sc_mean_profile of member `member`'s stored
+   trajectory. */
+int sc_adjoint_objective_members(int start_step, int end_step, sc_complex *AC_dev, double weight,
+                                 int shared_base);
+int sc_member_mean_profile(int member, int incre, int start_step, int end_step, double weight,
+                           double *profile, double *shear);
 int sc_gradient_euclid(const sc_complex *lam, sc_complex *g);
 
 /* No counterpart in the reference: c_ddt_project for callers that cannot

@@ -103,12 +103,16 @@ _L.sc_member_state.argtypes = [_i, _i, _i, _vp]
 _L.sc_member_state.restype = _i
 _L.sc_mean_profile.argtypes = [_i, _i, _i, _d, _vp, _vp]
 _L.sc_mean_profile.restype = _i
+_L.sc_member_mean_profile.argtypes = [_i, _i, _i, _i, _d, _vp, _vp]
+_L.sc_member_mean_profile.restype = _i
 _L.sc_step_statistics.argtypes = [_i, _i, _vp]
 _L.sc_step_statistics.restype = _i
 _L.sc_tangent_sources.argtypes = [_i, _i, _vp, _i, _i]
 _L.sc_tangent_sources.restype = _i
 _L.sc_adjoint_objective.argtypes = [_i, _i, _vp, _d]
 _L.sc_adjoint_objective.restype = _i
+_L.sc_adjoint_objective_members.argtypes = [_i, _i, _vp, _d, _i]
+_L.sc_adjoint_objective_members.restype = _i
 _L.sc_gradient_euclid.argtypes = [_vp, _vp]
 _L.sc_gradient_euclid.restype = _i
 _L.sc_h5_dataset_shape.argtypes = [_c.c_char_p, _c.c_char_p, _c.POINTER(_c.c_longlong), _i,
@@ -222,6 +226,16 @@ class DeviceArray:
         assert host.nbytes == self.nbytes
         if DeviceArray._hip.hipMemcpy(self.ptr, host.ctypes.data, self.nbytes, 1) != 0:
             raise RuntimeError("hipMemcpy host to device failed")
+
+    def clone(self):
+        """A second device array with the same contents (device-to-device)."""
+        out = DeviceArray.__new__(DeviceArray)
+        out.shape, out.nbytes, out.ptr = self.shape, self.nbytes, _vp()
+        if DeviceArray._hip.hipMalloc(_c.byref(out.ptr), self.nbytes) != 0:
+            raise MemoryError("hipMalloc of %d bytes failed" % self.nbytes)
+        if DeviceArray._hip.hipMemcpy(out.ptr, self.ptr, self.nbytes, 3) != 0:
+            raise RuntimeError("hipMemcpy device to device failed")
+        return out
 
     def download(self):
         out = np.empty(self.shape, dtype=np.complex128)
@@ -435,6 +449,16 @@ def sc_mean_profile(incre, start_step, end_step, weight):
     return prof, shear
 
 
+def sc_member_mean_profile(member, incre, start_step, end_step, weight):
+    """sc_mean_profile of one member's stored trajectory."""
+    prof = np.zeros(c_qpts())
+    shear = np.zeros(2)
+    _check(_L.sc_member_mean_profile(int(member), int(incre), int(start_step), int(end_step),
+                                     float(weight), prof.ctypes.data, shear.ctypes.data),
+           "sc_member_mean_profile")
+    return prof, shear
+
+
 def sc_step_statistics(incre, i_step):
     """Plane statistics [10, qpts] of the stored state at i_step: means of u,
     v, w, then of uv, uw, vw, uu, vv, ww, then of du/dy."""
@@ -504,6 +528,14 @@ def sc_tangent_sources(start_step, end_step, IC, inhomo, re_source):
     IC = _coef_array(IC, "IC")
     _check(_L.sc_tangent_sources(int(start_step), int(end_step), IC.ctypes.data, int(inhomo),
                                  int(re_source)), "sc_tangent_sources")
+
+
+def sc_adjoint_objective_members(start_step, end_step, AC_dev, weight, shared_base=False):
+    """sc_adjoint_objective of all members in every launch (AC_dev: a
+    DeviceArray of the members' states, in place); member windows apply."""
+    _check(_L.sc_adjoint_objective_members(int(start_step), int(end_step), _devptr(AC_dev),
+                                           float(weight), int(bool(shared_base))),
+           "sc_adjoint_objective_members")
 
 
 def sc_adjoint_objective(start_step, end_step, AC, weight):

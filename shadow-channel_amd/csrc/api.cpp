@@ -131,7 +131,7 @@ static int stepping(Context &c, const std::vector<double> &key,
     return SC_OK;
 }
 
-Tuning &sc::tuning()
+Tuning &tuning()
 {
     static Tuning t;
     return t;
@@ -1069,15 +1069,15 @@ static void adjoint_stages(Context &c, int n, int exact_mean)
 
 // Member m of a windowed adjoint member run starts at step n: its given
 // state, lambda rebuilt from it, and the forcing c_adjoint applies at its
-// start_step (acc -= a v with v the projected MIC[3n]), on that member alone.
-// Whatever the member held before (zero, or forcing read from a tangent
-// continued past its window) is overwritten.
+// start_step (acc -= a v with v the projected MIC[3n]), on that member alone
+// (a < 0: the given state only).
 static int start_member(Context &c, int m, int n, double a)
 {
     cplx *ac = member_at(c, c.AC, m), *au = member_at(c, c.AU, m);
     const cplx *init = member_at(c, c.ACinit, m), *vin = member_at(c, mic(c, 3L * n), m);
     const cplx *P = member_at(c, mc(c, 3L * n), m), *M = member_at(c, mc(c, 3L * (n - 1)), m);
     SC_HIP(hipMemcpyAsync(ac, init, sizeof(cplx) * c.csize, hipMemcpyDeviceToDevice, c.stream));
+    if (a < 0) return SC_OK;        // the objective adjoint: rebuilt at its step's start
     const int nrun = c.nrun;
     c.nrun = 1;                 // the launches below see one member at these addresses
     launch_rebuild(c, ac, au, true);
@@ -1088,6 +1088,27 @@ static int start_member(Context &c, int m, int n, double a)
                          c.dotpart, c.scal);
     c.nrun = nrun;
     return nparts > 0 ? SC_OK : SC_EDEVICE;
+}
+
+// Members of a windowed adjoint run whose windows end before step n are
+// still waiting: held at zero (whatever forcing or source reached them is
+// dropped), so that a member that never starts returns zero
+static void zero_pending(Context &c, int n)
+{
+    for (int m = 0; m < c.nrun; ++m)
+        if (c.win[m] < n) launch_zero(c, member_at(c, c.AC, m), c.csize);
+}
+
+// top of iteration n of a windowed adjoint run from `start`: the waiting
+// members held at zero, the members whose windows end at n < start started
+// (a >= 0: the forced adjoint's start forcing; a < 0: none, the objective
+// adjoint); members whose windows reach `start` begin with the run itself
+static int hold_pending(Context &c, int start, int n, double a)
+{
+    zero_pending(c, n);
+    for (int m = 0; m < c.nrun; ++m)
+        if (c.win[m] == n && n < start && start_member(c, m, n, a)) return SC_EDEVICE;
+    return SC_OK;
 }
 
 static int adjoint_run(int start, int end, sc_complex *AC_given, double strength, hipMemcpyKind in,
@@ -1103,8 +1124,7 @@ static int adjoint_run(int start, int end, sc_complex *AC_given, double strength
     const bool windows = windowed(c);
     if (windows) {
         SC_HIP(hipMemcpyAsync(c.ACinit, c.AC, cbm, hipMemcpyDeviceToDevice, c.stream));
-        for (int m = 0; m < c.nrun; ++m)
-            if (c.win[m] < start) launch_zero(c, member_at(c, c.AC, m), c.csize);
+        zero_pending(c, start);
     }
     launch_rebuild(c, c.AC, c.AU, true);      // lambda: velocity slots only
     const double s = strength * d.dt;
@@ -1112,10 +1132,7 @@ static int adjoint_run(int start, int end, sc_complex *AC_given, double strength
                                      (double) c.shared_base, s};
     int rc = stepping(c, key, [&](bool) {
         for (int n = start; n > end; --n) {
-            if (windows)
-                for (int m = 0; m < c.nrun; ++m)
-                    if (c.win[m] == n && n < start && start_member(c, m, n, 0.5 * s))
-                        return SC_EDEVICE;
+            if (windows && hold_pending(c, start, n, 0.5 * s)) return SC_EDEVICE;
             // The projection of MIC[3n] at step n was made at the end of the
             // previous iteration (the reference computes it twice), which also
             // applied it for this step's start.
@@ -1127,6 +1144,7 @@ static int adjoint_run(int start, int end, sc_complex *AC_given, double strength
             if (ddt_project_dev(c, n - 1, mic(c, 3L * (n - 1)), c.Ctmp, c.AC, 0.5 * s, n - 1 > end))
                 return SC_EDEVICE;
         }
+        if (windows) zero_pending(c, end + 1);   // never started: win <= end
         return SC_OK;
     });
     if (!rc) rc = stream_sync(c);
@@ -1149,12 +1167,20 @@ static int adjoint_objective_run(int start, int end, sc_complex *AC_given, doubl
     const size_t cbm = sizeof(cplx) * c.csize * c.nrun;
     SC_HIP(hipMemcpyAsync(c.AC, AC_given, cbm, in, c.stream));
     SC_HIP(hipMemsetAsync(c.abort_flag, 0, sizeof(int), c.stream));
+    // member windows: a member waits at zero until its window's end
+    const bool windows = windowed(c);
+    if (windows) {
+        SC_HIP(hipMemcpyAsync(c.ACinit, c.AC, cbm, hipMemcpyDeviceToDevice, c.stream));
+        zero_pending(c, start);
+    }
     for (int n = start; n > end; --n) {
+        if (windows && hold_pending(c, start, n, -1.0)) return SC_EDEVICE;
         launch_mean_project(c, c.AC);
         launch_rebuild(c, c.AC, c.AU, true);
         adjoint_stages(c, n, 1);
         launch_add_mean(c, c.AC, c.dGwall, weight);
     }
+    if (windows) zero_pending(c, end + 1);   // never started: win <= end
     int rc = stream_sync(c);
     if (!rc) rc = check_kernel_error(c);
     if (rc) return rc;
@@ -1442,6 +1468,17 @@ int sc_tangent_members(int start_step, int end_step, sc_complex *IC_dev, unsigne
                        hipMemcpyDeviceToDevice);
 }
 
+int sc_adjoint_objective_members(int start_step, int end_step, sc_complex *AC_dev,
+                                 double weight, int shared_base)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!AC_dev || !adjoint_range_ok(c, start_step, end_step)) return SC_EINVAL;
+    MemberRun run(c, c.nmem, shared_base != 0);
+    return adjoint_objective_run(start_step, end_step, AC_dev, weight, hipMemcpyDeviceToDevice,
+                                 hipMemcpyDeviceToDevice);
+}
+
 int sc_adjoint_members(int start_step, int end_step, sc_complex *AC_dev, double strength,
                        int shared_base)
 {
@@ -1501,8 +1538,8 @@ int sc_set_member_windows(const int *len)
     if (!c.ok) return SC_ENOTREADY;
     if (len) {
         for (int m = 0; m < c.nmem; ++m)
-            if (len[m] < 1 || len[m] > c.d.nsteps) {
-                last_error() = "member windows must lie in 1 .. nsteps";
+            if (len[m] < 0 || len[m] > c.d.nsteps) {
+                last_error() = "member windows must lie in 0 .. nsteps";
                 return SC_EINVAL;
             }
         c.win.assign(len, len + c.nmem);

@@ -291,21 +291,16 @@ static int obj_fail(const char *msg)
 
 extern "C" {
 
-int sc_mean_profile(int incre, int start_step, int end_step, double weight, double *profile,
-                    double *shear)
+// the profile and wall shear of the stored trajectory S (one member's MC or MIC)
+static int mean_profile(Context &c, const cplx *S, int start_step, int end_step, double weight,
+                        double *profile, double *shear)
 {
-    Context &c = ctx();
-    if (!c.ok) return SC_ENOTREADY;
     const Dims &d = c.d;
-    if (!profile || !shear || start_step < 0 || end_step < start_step || end_step > d.nsteps + 1)
-        return obj_fail("mean_profile requires 0 <= start_step <= end_step <= nsteps + 1 "
-                        "and two output arrays");
     const int count = end_step - start_step;
     const int nchunk = std::max(1, (count + OBJ_CHUNK - 1) / OBJ_CHUNK);
     // scratch (objective_scratch_doubles): the partials, then the profile and
     // the two wall values; the wall-derivative rows sit in c.dRwall
     double *scr = c.objscr, *out = scr + (size_t) nchunk * d.dimR;
-    const cplx *S = incre ? c.MIC : c.MC;
     // (an empty window writes zero partials)
     hipLaunchKernelGGL(k_mean_alpha_partial, dim3(nchunk), dim3(OBJ_THREADS), 0, c.stream, S,
                        3 * c.csize, start_step, count, d.dimR, d.Mx, scr);
@@ -319,6 +314,33 @@ int sc_mean_profile(int incre, int start_step, int end_step, double weight, doub
     shear[0] = h[d.qpts];
     shear[1] = h[d.qpts + 1];
     return SC_OK;
+}
+
+int sc_mean_profile(int incre, int start_step, int end_step, double weight, double *profile,
+                    double *shear)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    const Dims &d = c.d;
+    if (!profile || !shear || start_step < 0 || end_step < start_step || end_step > d.nsteps + 1)
+        return obj_fail("mean_profile requires 0 <= start_step <= end_step <= nsteps + 1 "
+                        "and two output arrays");
+    return mean_profile(c, incre ? c.MIC : c.MC, start_step, end_step, weight, profile, shear);
+}
+
+int sc_member_mean_profile(int member, int incre, int start_step, int end_step, double weight,
+                           double *profile, double *shear)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    const Dims &d = c.d;
+    if (!profile || !shear || member < 0 || member >= c.nmem || start_step < 0 ||
+        end_step < start_step || end_step > d.nsteps + 1)
+        return obj_fail("member_mean_profile requires 0 <= member < members, 0 <= start_step "
+                        "<= end_step <= nsteps + 1 and two output arrays");
+    const cplx *S = (const cplx *) ((const char *) (incre ? c.MIC : c.MC) +
+                                    (size_t) member * c.mcbytes);
+    return mean_profile(c, S, start_step, end_step, weight, profile, shear);
 }
 
 int sc_step_statistics(int incre, int i_step, double *stats)

@@ -18,6 +18,6 @@ def test_more_than_one_gpu_is_refused():
 def test_traffic_figures_parse_from_profiles():
     sys.path.insert(0, ROOT)
     import bench
-    for cls in bench.TRAFFIC_PROFILES:
-        t = bench.measured_traffic(cls)
-        assert isinstance(t, int) and t > 10_000_000, (cls, t)
+    for cls, members in bench.TRAFFIC_PROFILES:
+        t = bench.measured_traffic(cls, members)
+        assert isinstance(t, int) and t > 10_000_000, (cls, members, t)

@@ -149,3 +149,58 @@ def test_minres_on_the_batched_operator(cc):
         assert np.all(np.isfinite(x)) and res[-1] < 0.9 * np.linalg.norm(rhs)
     finally:
         bs.release()
+
+
+def test_uneven_segments_at_benchmark_grid(cc):
+    """Configuration 3 at the benchmark grid (64x64x97, Re 3000, dt 0.004),
+    scaled to 8 segments: lengths drawn from 100 .. 400 steps, every segment
+    a member (one launch per stage for all, HIP graphs on, launch shapes from
+    the autotuner's cache), 8 tangent directions and one inhomogeneous
+    tangent from zero per segment.  Each direction's end states and
+    segment-boundary projection coefficients equal the segment loop's bit
+    for bit; the projection residuals (the coefficient left after a second
+    projection at each boundary) are reported and held to 1e-12."""
+    from channel import tuning
+    from channel.lss import BatchedSegments, Segments
+    grid, box = (64, 97, 64), (2 * np.pi, np.pi)
+    Re, flux, dt = 3000.0, 4.0 / 3.0, 0.004
+    lens = [int(x) for x in np.random.default_rng(3).integers(100, 401, size=8)]
+    bounds = [0] + list(np.cumsum(lens))
+    print("case: 8 uneven segments %s at 64x64x97, 9 tangents each" % lens, flush=True)
+    C0 = synthetic_state(*grid, *box, flux, seed=0)
+    dirs = []
+    for d in range(9):
+        if d < 8:
+            V = np.stack([synthetic_state(*grid, *box, flux, seed=100 * d + s, unit=True)
+                          for s in range(len(lens))])
+        else:
+            V = np.zeros((len(lens), grid[2], 2, grid[1] - 2, grid[0] // 2), complex)
+        dirs.append((V, int(d == 8)))
+    bs = BatchedSegments(grid, box, Re, flux, dt, bounds=bounds)
+    batched, resid = [], []
+    cc.sc_set_graphs(1)
+    try:
+        bs.setup(C0)
+        tuning.autotune(grid, box, Re, flux, dt, bs.starts, cache=tuning.SHAPES_FILE)
+        for V, inhomo in dirs:
+            V1, eta = bs.forward_all(V, inhomo)
+            batched.append((V1, eta))
+            resid.append(max(abs(cc.sc_member_ddt_project(s, bs.lens[s], V1[s].copy())) /
+                             max(abs(eta[s]), 1e-300) for s in range(bs.n)))
+    finally:
+        cc.sc_set_graphs(0)
+        tuning.reset()
+        bs.release()
+    print("projection residuals per direction:", " ".join("%.1e" % r for r in resid), flush=True)
+    assert max(resid) <= 1e-12
+    assert cc.c_init(*grid, *box, Re, flux, dt, bounds[-1]) == 0
+    try:
+        cc.c_primal(0, C0.copy())
+        seg = Segments(grid, bounds)
+        for (V, inhomo), (V1, eta) in zip(dirs, batched):
+            for s in range(seg.n):
+                v, e = seg.forward(s, V[s], inhomo)
+                np.testing.assert_array_equal(v, V1[s])
+                assert e == eta[s], (s, e, eta[s])
+    finally:
+        cc.c_destroy()

@@ -126,7 +126,7 @@ def test_member_windows_equal_single_runs(cc, grid):
     A0 = [_st(cfg, 20 + s, unit=True) for s in range(nm)]
     _run_members(cc, cc.sc_primal_members, C0, 0, ())
     with pytest.raises(AssertionError):
-        cc.sc_set_member_windows([0, 2, 5, 3])
+        cc.sc_set_member_windows([-1, 2, 5, 3])
     with pytest.raises(AssertionError):
         cc.sc_set_member_windows([n + 1, 2, 5, 3])
     cc.sc_set_member_windows(WIN)
