@@ -836,7 +836,15 @@ static int primal_run(int ru_steps, sc_complex *C_given, hipMemcpyKind in, hipMe
         }
         host_gauss_to_coef(c, Uh, Ch);
         SC_HIP(hipMemcpy(c.C, Ch.data(), cb, hipMemcpyHostToDevice));
-        SC_HIP(hipMemcpy(c.U, Uh.data(), sizeof(cplx) * c.usize, hipMemcpyHostToDevice));
+        if (slots32(c)) {
+            // FP32 slots, as the rebuild writes them in this context
+            std::vector<float2> Uf(c.usize);
+            for (size_t i = 0; i < c.usize; ++i)
+                Uf[i] = make_float2((float) Uh[i].x, (float) Uh[i].y);
+            SC_HIP(hipMemcpy(c.U, Uf.data(), sizeof(float2) * c.usize, hipMemcpyHostToDevice));
+        } else {
+            SC_HIP(hipMemcpy(c.U, Uh.data(), sizeof(cplx) * c.usize, hipMemcpyHostToDevice));
+        }
     }
     // Every stored state is written by a completed run; after a CFL stop the
     // states from the stopped step on are cleared (below), so the trajectory

@@ -446,12 +446,25 @@ constexpr int MAXE = FFT_TILE / BLK;   // LDS elements per thread of a full tile
 constexpr int SBAT = 8;                // stage-in loads in flight per thread and batch
 static_assert(MAXE % SBAT == 0, "stage-in batches must tile the per-thread elements");
 
+// A Gauss-point slot value at element index i of src: FP64 complex, or
+// FP32 complex (S32) where the mixed-precision rebuild stored the slots.
+template <bool S32>
+__device__ __forceinline__ double2 ld_slot(const cplx *src, size_t i)
+{
+    if constexpr (S32) {
+        const float2 v = ((const float2 *) src)[i];
+        return make_double2(v.x, v.y);
+    } else {
+        return src[i];
+    }
+}
+
 // Stage a (Pz x XB) tile of one field into LDS in its memory layout
 // [zp][x], zero rows for the padding, the spectral curl formed on the fly.
 // A batch's loads are all issued before its first LDS store: indices are
 // clamped into the array and the padding masked arithmetically, so no load
 // sits behind a branch.
-template <int KIND, int PZ>
+template <int KIND, int PZ, bool S32>
 __device__ __forceinline__ void zinv_stage(const ZinvArgs &a, const cplx *src, int slot, int y,
                                            int x0, int XB, real2 *lds)
 {
@@ -478,18 +491,18 @@ __device__ __forceinline__ void zinv_stage(const ZinvArgs &a, const cplx *src, i
             ok[i] = e < total && kz >= 0;
             kzi[i] = max(kz, 0);
             xi[i] = x0 + xl;
-            const cplx *b = src + kzi[i] * zs + (size_t) y * Mx + xi[i];
+            const size_t b = kzi[i] * zs + (size_t) y * Mx + xi[i];
             if (KIND == FK_SLOT) {
-                p[i] = b[slot * ss];
+                p[i] = ld_slot<S32>(src, b + slot * ss);
             } else if (KIND == FK_OMEGA_Z) {
-                p[i] = b[3 * ss];
-                q[i] = b[1 * ss];
+                p[i] = ld_slot<S32>(src, b + 3 * ss);
+                q[i] = ld_slot<S32>(src, b + 1 * ss);
             } else if (KIND == FK_OMEGA_X) {
-                p[i] = b[4 * ss];
-                q[i] = b[1 * ss];
+                p[i] = ld_slot<S32>(src, b + 4 * ss);
+                q[i] = ld_slot<S32>(src, b + 1 * ss);
             } else {
-                p[i] = b[0];
-                q[i] = b[2 * ss];
+                p[i] = ld_slot<S32>(src, b);
+                q[i] = ld_slot<S32>(src, b + 2 * ss);
             }
         }
 #pragma unroll
@@ -524,7 +537,7 @@ __device__ __forceinline__ void zinv_stage(const ZinvArgs &a, const cplx *src, i
 // from U, transforms them in registers and writes the pass's output; groups
 // q whose rows all lie in the zero band (the Nyquist row and the padding)
 // are not loaded.
-template <int KIND, int PZ>
+template <int KIND, int PZ, bool S32>
 __device__ __forceinline__ void zinv_first_pass(const ZinvArgs &a, const cplx *src, int slot, int y,
                                                 int x0, real2 *lds)
 {
@@ -534,7 +547,7 @@ __device__ __forceinline__ void zinv_first_pass(const ZinvArgs &a, const cplx *s
     if (b >= NBUT) return;
     const int j = b / ZXB, xl = b - j * ZXB, Mx = a.Mx;
     const size_t zs = (size_t) 5 * a.qpts * Mx, ss = (size_t) a.qpts * Mx;
-    const cplx *col = src + (size_t) y * Mx + x0 + xl;
+    const size_t col = (size_t) y * Mx + x0 + xl;
     const double kx = (x0 + xl) * a.ax;
     real2 v[R1];
 #pragma unroll
@@ -546,19 +559,19 @@ __device__ __forceinline__ void zinv_first_pass(const ZinvArgs &a, const cplx *s
         const int zp = j + q * NB1;
         const int kz = zp < Nz / 2 ? zp : (zp > Nz ? zp - Nz / 2 : -1);
         const int kzi = max(kz, 0);
-        const cplx *bp = col + kzi * zs;
+        const size_t bp = col + kzi * zs;
         const double kzv = (kzi <= Nz / 2 ? kzi : kzi - Nz) * a.az;
         double2 val;
         if (KIND == FK_SLOT) {
-            val = bp[slot * ss];
+            val = ld_slot<S32>(src, bp + slot * ss);
         } else if (KIND == FK_OMEGA_Z) {                // -du/dy + i kx v
-            const double2 du = bp[3 * ss], vv = bp[1 * ss];
+            const double2 du = ld_slot<S32>(src, bp + 3 * ss), vv = ld_slot<S32>(src, bp + 1 * ss);
             val = make_double2(-(du.x + kx * vv.y), -du.y + kx * vv.x);
         } else if (KIND == FK_OMEGA_X) {                // dw/dy - i kz v
-            const double2 dw = bp[4 * ss], vv = bp[1 * ss];
+            const double2 dw = ld_slot<S32>(src, bp + 4 * ss), vv = ld_slot<S32>(src, bp + 1 * ss);
             val = make_double2(dw.x + kzv * vv.y, dw.y - kzv * vv.x);
         } else {                                        // i kz u - i kx w
-            const double2 u = bp[0], w = bp[2 * ss];
+            const double2 u = ld_slot<S32>(src, bp), w = ld_slot<S32>(src, bp + 2 * ss);
             val = make_double2(kx * w.y - kzv * u.y, kzv * u.x - kx * w.x);
         }
         v[q] = kz >= 0 ? to_real2(val) : mk2(0.0, 0.0);
@@ -571,7 +584,7 @@ __device__ __forceinline__ void zinv_first_pass(const ZinvArgs &a, const cplx *s
 
 // grid (ceil(Mx/XB), nf * members, qpts): blockIdx.y = member nf + field;
 // PZ > 0: Pz == PZ, XB == ZXB and Mx % ZXB == 0
-template <int PZ>
+template <int PZ, bool S32>
 __global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, real2 *W1)
 {
     extern __shared__ real2 lds[];
@@ -587,17 +600,17 @@ __global__ __launch_bounds__(BLK) void k_zinv(ZinvArgs a, PlanDev pz, real2 *W1)
     stage_twiddles(pz, twl);
     // columns interleaved in LDS exactly as in memory: no transpose
     if constexpr (PZ > 0) {
-        if (kind == FK_SLOT) zinv_first_pass<FK_SLOT, PZ>(a, src, slot, y, x0, lds);
-        else if (kind == FK_OMEGA_Z) zinv_first_pass<FK_OMEGA_Z, PZ>(a, src, slot, y, x0, lds);
-        else if (kind == FK_OMEGA_X) zinv_first_pass<FK_OMEGA_X, PZ>(a, src, slot, y, x0, lds);
-        else zinv_first_pass<FK_OMEGA_Y, PZ>(a, src, slot, y, x0, lds);
+        if (kind == FK_SLOT) zinv_first_pass<FK_SLOT, PZ, S32>(a, src, slot, y, x0, lds);
+        else if (kind == FK_OMEGA_Z) zinv_first_pass<FK_OMEGA_Z, PZ, S32>(a, src, slot, y, x0, lds);
+        else if (kind == FK_OMEGA_X) zinv_first_pass<FK_OMEGA_X, PZ, S32>(a, src, slot, y, x0, lds);
+        else zinv_first_pass<FK_OMEGA_Y, PZ, S32>(a, src, slot, y, x0, lds);
         __syncthreads();
         lds_fft_c<PZ, ZXB, 1, ZXB, 1, plan_radix(PZ, 0)>(lds, twl, +1.0);
     } else {
-        if (kind == FK_SLOT) zinv_stage<FK_SLOT, PZ>(a, src, slot, y, x0, XB, lds);
-        else if (kind == FK_OMEGA_Z) zinv_stage<FK_OMEGA_Z, PZ>(a, src, slot, y, x0, XB, lds);
-        else if (kind == FK_OMEGA_X) zinv_stage<FK_OMEGA_X, PZ>(a, src, slot, y, x0, XB, lds);
-        else zinv_stage<FK_OMEGA_Y, PZ>(a, src, slot, y, x0, XB, lds);
+        if (kind == FK_SLOT) zinv_stage<FK_SLOT, PZ, S32>(a, src, slot, y, x0, XB, lds);
+        else if (kind == FK_OMEGA_Z) zinv_stage<FK_OMEGA_Z, PZ, S32>(a, src, slot, y, x0, XB, lds);
+        else if (kind == FK_OMEGA_X) zinv_stage<FK_OMEGA_X, PZ, S32>(a, src, slot, y, x0, XB, lds);
+        else zinv_stage<FK_OMEGA_Y, PZ, S32>(a, src, slot, y, x0, XB, lds);
         __syncthreads();
         lds_fft(lds, XB, 1, XB, pz, twl, +1.0);
     }
@@ -1227,6 +1240,12 @@ void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1)
     const int nm = c.nrun;
     dim3 grid((d.Mx + a.XB - 1) / a.XB, fs.nf * nm, d.qpts);
     size_t lds = sizeof(real2) * (a.XB * d.Pz + d.Pz);
+    // the FP32 build reads FP32 slots where the rebuild wrote them
+#ifdef SC_FFT_F32
+    const bool s32 = slots32(c);
+#else
+    const bool s32 = false;
+#endif
     const double plane = (double) d.qpts * d.Mx;
     int slots = 0;
     for (int f = 0; f < fs.nf; ++f) {
@@ -1236,10 +1255,13 @@ void launch_zinv(Context &c, const FieldSpec &fs, cplx *W1)
     }
     if (fs.nf == 9) slots = 8;            // adjoint: 5 state slots + 3 adjoint
     if (fs.nf == 6 && fs.kind[3] == FK_SLOT) slots = 6;   // two velocity triples
-    double bytes = nm * plane * (16.0 * slots * (d.Nz - 1) + (double) sizeof(real2) * fs.nf * d.Pz);
+    double bytes = nm * plane * ((s32 ? 8.0 : 16.0) * slots * (d.Nz - 1) +
+                                 (double) sizeof(real2) * fs.nf * d.Pz);
     const hipStream_t st = c.stream;
     const PlanDev pd = to_dev(c.pz);
-    const ZinvKernel kern = fixed == 96 ? k_zinv<96> : fixed == 48 ? k_zinv<48> : k_zinv<0>;
+    const ZinvKernel kern =
+        s32 ? (fixed == 96 ? k_zinv<96, true> : fixed == 48 ? k_zinv<48, true> : k_zinv<0, true>)
+            : (fixed == 96 ? k_zinv<96, false> : fixed == 48 ? k_zinv<48, false> : k_zinv<0, false>);
     real2 *W1r = (real2 *) W1;
     auto go = [=]() { hipLaunchKernelGGL(kern, grid, dim3(BLK), lds, st, a, pd, W1r); };
     prof_begin(c, st, 1, bytes);

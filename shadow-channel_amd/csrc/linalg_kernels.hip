@@ -14,6 +14,8 @@
 // (bands sit on every other diagonal), so one thread solves one parity of
 // one (kz, kx, field) system.  Explicit band products are formed on the fly
 // from the Gram tables.
+#include <type_traits>
+
 #include "sc_internal.h"
 
 namespace sc {
@@ -302,25 +304,31 @@ __device__ __forceinline__ RebuildTask rebuild_task(const RebuildArgs &a, int zr
 }
 
 // Gauss-point values of one task from its five accumulators (D[n][y] layout)
-// VEL: the velocity slots u, v, w only (acc[2], acc[4] unused)
-template <bool VEL = false, class ACC = mf4>
+// VEL: the velocity slots u, v, w only (acc[2], acc[4] unused).  The slots
+// are stored as O: double2, or float2 for the FP32 instance (the mixed-
+// precision steps, whose z-inverse transform reads FP32 slots at the same
+// element index; the member base address stays that of the FP64 layout).
+template <bool VEL = false, class ACC = mf4, class O = double2>
 __device__ __forceinline__ void rebuild_store(const RebuildArgs &a, int zrow, int nt, int y, int kq,
                                               const ACC *acc, cplx *U)
 {
+    typedef decltype(O().x) S;
     const int Mx = a.Mx, q = a.qpts;
     if (y >= q) return;
     const size_t ss = (size_t) q * Mx;
+    O *Uo = (O *) U;
     if (zrow == a.Nz / 2) {
         // mean column: item 0 of lane group kq holds n = kq, Re/Im of
         // alpha (kq < 2: u, du/dy) or beta (kq >= 2: w, dw/dy)
-        double *Ur = (double *) (U + (size_t) y * Mx) + (kq & 1);
+        S *Ur = (S *) (Uo + (size_t) y * Mx) + (kq & 1);
         const int su = kq < 2 ? 0 : 2;
-        Ur[(size_t) su * ss * 2] = acc[3][0];
-        if (!VEL) Ur[(size_t) (su == 0 ? 3 : 4) * ss * 2] = acc[4][0];
-        Ur[(size_t) 1 * ss * 2] = 0.0;
+        Ur[(size_t) su * ss * 2] = (S) acc[3][0];
+        if (!VEL) Ur[(size_t) (su == 0 ? 3 : 4) * ss * 2] = (S) acc[4][0];
+        Ur[(size_t) 1 * ss * 2] = (S) 0;
         return;
     }
-    cplx *Uz = U + (size_t) zrow * 5 * ss;
+    O *Uz = Uo + (size_t) zrow * 5 * ss;
+    auto put = [&](size_t i, double re, double im) { Uz[i] = O{(S) re, (S) im}; };
     const double kz = wave_kz(zrow, a.Nz, a.az);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -332,14 +340,14 @@ __device__ __forceinline__ void rebuild_store(const RebuildArgs &a, int zrow, in
         const double2 v = make_double2(acc[0][2 * h], acc[0][2 * h + 1]);
         const double2 f = make_double2(-acc[1][2 * h], -acc[1][2 * h + 1]);
         const double2 g = make_double2(acc[3][2 * h], acc[3][2 * h + 1]);
-        Uz[o + 1 * ss] = v;
-        Uz[o + 0 * ss] = make_double2((kx * f.y + kz * g.y) * r2, -(kx * f.x + kz * g.x) * r2);
-        Uz[o + 2 * ss] = make_double2((-kx * g.y + kz * f.y) * r2, (kx * g.x - kz * f.x) * r2);
+        put(o + 1 * ss, v.x, v.y);
+        put(o + 0 * ss, (kx * f.y + kz * g.y) * r2, -(kx * f.x + kz * g.x) * r2);
+        put(o + 2 * ss, (-kx * g.y + kz * f.y) * r2, (kx * g.x - kz * f.x) * r2);
         if (VEL) continue;
         const double2 q2 = make_double2(acc[2][2 * h], acc[2][2 * h + 1]);
         const double2 gp = make_double2(acc[4][2 * h], acc[4][2 * h + 1]);
-        Uz[o + 3 * ss] = make_double2((-kx * q2.y + kz * gp.y) * r2, (kx * q2.x - kz * gp.x) * r2);
-        Uz[o + 4 * ss] = make_double2(-(kx * gp.y + kz * q2.y) * r2, (kx * gp.x + kz * q2.x) * r2);
+        put(o + 3 * ss, (-kx * q2.y + kz * gp.y) * r2, (kx * q2.x - kz * gp.x) * r2);
+        put(o + 4 * ss, -(kx * gp.y + kz * q2.y) * r2, (kx * gp.x + kz * q2.x) * r2);
     }
 }
 
@@ -704,8 +712,10 @@ __global__ __launch_bounds__(256, 2) void k_rebuild_par(RebuildArgs a, const cpl
                 ae[m] = e + o;
                 ao[m] = (m == 1 || m == 4) ? o - e : e - o;
             }
-            rebuild_store<VEL>(a, zr, nt, y, kq, ae, Ud);
-            if (ym != y) rebuild_store<VEL>(a, zr, nt, ym, kq, ao, Ud);
+            // FP32 products: FP32 slots
+            typedef typename std::conditional<sizeof(T) == sizeof(float), float2, double2>::type O;
+            rebuild_store<VEL, ACC, O>(a, zr, nt, y, kq, ae, Ud);
+            if (ym != y) rebuild_store<VEL, ACC, O>(a, zr, nt, ym, kq, ao, Ud);
         }
         t = tn;
     }
@@ -732,7 +742,9 @@ void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx 
     RebuildArgs a{c.dFragR, c.dKx, c.dKz, d.Nz, d.Mx, d.qpts, d.dimR, d.dimQ, c.nksR, c.nyt,
                   nseg, 2. * M_PI / d.Lx, 2. * M_PI / d.Lz, c.abort_flag, c.rb_queue, bpq,
                   nprob, C1, U1, nm, ms_c, ms_u};
-    double bytes = nm * nprob * 16.0 * (double) (d.Nz - 1) * d.Mx * (2.0 * d.dimR + 5.0 * d.qpts);
+    const double slot_bytes = slots32(c) ? 8.0 : 16.0;
+    double bytes = nm * nprob * (double) (d.Nz - 1) * d.Mx *
+                   (16.0 * 2.0 * d.dimR + slot_bytes * (vel_only ? 3.0 : 5.0) * d.qpts);
     const hipStream_t st = c.stream;
     const int nks = c.nksR;
     const dim3 grows(8 * c.nyt * bpq), gchunk(xcd_grid(c.nyt * d.Nz), nm);
@@ -751,8 +763,8 @@ void launch_rebuild2(Context &c, const cplx *C0, cplx *U0, const cplx *C1, cplx 
     }
     const dim3 gpar(8 * c.nht * ap.bpq);
     // mixed precision: the parity rebuild's products on the FP32 matrix cores
-    // (the state slots stay FP64), except for outputs such as spec2phys
-    const bool f32 = c.mixed && !c.fp64_out;
+    // and FP32 slots, except for outputs such as spec2phys
+    const bool f32 = slots32(c);
     typedef void (*RebuildKernel)(RebuildArgs, const cplx *, cplx *);
 #define SC_RB_PAR(N, V) (f32 ? (RebuildKernel) k_rebuild_par<N, V, float> : (RebuildKernel) k_rebuild_par<N, V, double>)
     auto go = [=]() {

@@ -354,7 +354,9 @@ int sc_step_statistics(int incre, int i_step, double *stats)
     double *dst = c.objscr;
     // the state's Gauss-point values in the spare field buffer (all five
     // slots: du/dy is one of the statistics)
+    c.fp64_out = true;                 // the statistics read FP64 slots
     launch_rebuild(c, S, c.Ut2);
+    c.fp64_out = false;
     hipLaunchKernelGGL(k_plane_stats, dim3(d.qpts), dim3(OBJ_THREADS), 0, c.stream, c.Ut2, d.Nz,
                        d.Mx, d.qpts, dst);
     SC_HIP(hipMemcpyAsync(stats, dst, sizeof(double) * NSTAT * d.qpts, hipMemcpyDeviceToHost,

@@ -290,6 +290,14 @@ SC_FFT_LAUNCHERS
 }
 // vel_only: only the velocity slots u, v, w are needed (du/dy and dw/dy may
 // be left unwritten where the kernel allows it)
+// Whether the Gauss-point slots the rebuild writes (and the z-inverse
+// transform reads) are FP32 complex: in mixed-precision contexts whose
+// rebuild runs in parity form on the FP32 matrix cores, except while an
+// output in physical space is formed.  Same element index as the FP64 layout.
+inline bool slots32(const Context &c)
+{
+    return c.mixed && !c.fp64_out && c.ypar && (c.nksP == 8 || c.nksP == 16 || c.nksP == 24);
+}
 void launch_rebuild(Context &c, const cplx *Csrc, cplx *Udst, bool vel_only = false);
 // n complex values at p <- 0 (a kernel, so that graph capture records no memset)
 void launch_zero(Context &c, cplx *p, size_t n);
