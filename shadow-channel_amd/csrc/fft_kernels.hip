@@ -930,7 +930,18 @@ struct ZfwdArgs {
     double dt;
     int *abort_rw;
     size_t ms_in, ms_out;      // member strides of W2 and dst
+    int p32;                   // ZF_USLOTS stored as FP32 complex (same element index)
 };
+
+// a product slot value at element index i of U: FP64 or FP32 complex
+template <class V>
+__device__ __forceinline__ void st_slot(cplx *U, size_t i, bool p32, V re, V im)
+{
+    if (p32)
+        ((float2 *) U)[i] = make_float2((float) re, (float) im);
+    else
+        U[i] = make_double2((double) re, (double) im);
+}
 
 // grid (ceil(Mx/XB), nout * members, qpts): blockIdx.y = member nout + field;
 // PZ > 0: Pz == PZ, XB == ZXB and Mx % ZXB == 0
@@ -986,12 +997,10 @@ __global__ __launch_bounds__(BLK) void k_zfwd(ZfwdArgs a, PlanDev pz, const real
         real2 v = lds[zp * XB + xl];
         v.x *= (real) a.norm;
         v.y *= (real) a.norm;
-        size_t o;
         if (a.dst_kind == ZF_USLOTS)
-            o = (((size_t) kz * 5 + g) * a.qpts + y) * Mx + x0 + xl;
+            st_slot(dst, (((size_t) kz * 5 + g) * a.qpts + y) * Mx + x0 + xl, a.p32, v.x, v.y);
         else
-            o = (((size_t) g * Nz + kz) * a.qpts + y) * Mx + x0 + xl;
-        dst[o] = to_cplx(v);
+            dst[(((size_t) g * Nz + kz) * a.qpts + y) * Mx + x0 + xl] = to_cplx(v);
     }
 }
 
@@ -1016,6 +1025,7 @@ struct ZadjArgs {
     // follow those of member m - 1 (strides of W2 / W2b and U / Ub)
     int nm;
     size_t ms_in, ms_out;
+    int p32;                   // the five slots stored as FP32 complex
 };
 
 template <int PZ>
@@ -1079,12 +1089,12 @@ __global__ __launch_bounds__(BLK) void k_zfwd_adj(ZadjArgs a, PlanDev pz, const 
         const int x = x0 + xl;
         const double kx = a.Kx[x], kzv = a.Kz[kz];
         const real2 Sx = S[0], Sy = S[1], Sz = S[2], Tx = S[3], Ty = S[4], Tz = S[5];
-        cplx *b = U + ((size_t) kz * 5 * a.qpts + y) * Mx + x;
-        b[0 * ss] = make_double2(-Tz.x + kx * Sy.y, -Tz.y - kx * Sy.x);
-        b[1 * ss] = make_double2(-Tx.x - kzv * Sy.y, -Tx.y + kzv * Sy.x);
-        b[2 * ss] = make_double2(-Ty.x + kzv * Sx.y - kx * Sz.y, -Ty.y - kzv * Sx.x + kx * Sz.x);
-        b[3 * ss] = to_cplx(Sx);
-        b[4 * ss] = to_cplx(Sz);
+        const size_t b = ((size_t) kz * 5 * a.qpts + y) * Mx + x;
+        st_slot(U, b + 0 * ss, a.p32, -Tz.x + kx * Sy.y, -Tz.y - kx * Sy.x);
+        st_slot(U, b + 1 * ss, a.p32, -Tx.x - kzv * Sy.y, -Tx.y + kzv * Sy.x);
+        st_slot(U, b + 2 * ss, a.p32, -Ty.x + kzv * Sx.y - kx * Sz.y, -Ty.y - kzv * Sx.x + kx * Sz.x);
+        st_slot(U, b + 3 * ss, a.p32, Sx.x, Sx.y);
+        st_slot(U, b + 4 * ss, a.p32, Sz.x, Sz.y);
     }
 }
 
@@ -1160,12 +1170,12 @@ __global__ __launch_bounds__(BLK) void k_zfwd_adj2(ZadjArgs a, PlanDev pz, const
     const double kx = a.Kx[x], kzv = a.Kz[kz];
     const real2 Sx = S[0], Sy = S[1], Sz = S[2], Tx = T[0], Ty = T[1], Tz = T[2];
     const size_t ss = (size_t) a.qpts * Mx;
-    cplx *b = U + ((size_t) kz * 5 * a.qpts + y) * Mx + x;
-    b[0 * ss] = make_double2(-Tz.x + kx * Sy.y, -Tz.y - kx * Sy.x);
-    b[1 * ss] = make_double2(-Tx.x - kzv * Sy.y, -Tx.y + kzv * Sy.x);
-    b[2 * ss] = make_double2(-Ty.x + kzv * Sx.y - kx * Sz.y, -Ty.y - kzv * Sx.x + kx * Sz.x);
-    b[3 * ss] = to_cplx(Sx);
-    b[4 * ss] = to_cplx(Sz);
+    const size_t b = ((size_t) kz * 5 * a.qpts + y) * Mx + x;
+    st_slot(U, b + 0 * ss, a.p32, -Tz.x + kx * Sy.y, -Tz.y - kx * Sy.x);
+    st_slot(U, b + 1 * ss, a.p32, -Tx.x - kzv * Sy.y, -Tx.y + kzv * Sy.x);
+    st_slot(U, b + 2 * ss, a.p32, -Ty.x + kzv * Sx.y - kx * Sz.y, -Ty.y - kzv * Sx.x + kx * Sz.x);
+    st_slot(U, b + 3 * ss, a.p32, Sx.x, Sx.y);
+    st_slot(U, b + 4 * ss, a.p32, Sz.x, Sz.y);
 }
 
 // adjoint assembly (channel/pass2.c:474-566):
@@ -1173,7 +1183,7 @@ __global__ __launch_bounds__(BLK) void k_zfwd_adj2(ZadjArgs a, PlanDev pz, const
 // grid (blocks, members): SP and U of member blockIdx.y at the given strides
 __global__ void k_assemble_adj(const cplx *SP, cplx *U, const double *Kx, const double *Kz,
                                int Nz, int qpts, int Mx, const int *abort_flag, size_t ms_sp,
-                               size_t ms_u)
+                               size_t ms_u, int p32)
 {
     if (*abort_flag) return;
     SP += blockIdx.y * ms_sp;
@@ -1189,11 +1199,11 @@ __global__ void k_assemble_adj(const cplx *SP, cplx *U, const double *Kx, const 
         double2 Sx = SP[e], Sy = SP[n + e], Sz = SP[2 * n + e];
         double2 Tx = SP[3 * n + e], Ty = SP[4 * n + e], Tz = SP[5 * n + e];
         size_t b = ((size_t) z * 5 * qpts + y) * Mx + x, ss = (size_t) qpts * Mx;
-        U[b + 0 * ss] = make_double2(-Tz.x + kx * Sy.y, -Tz.y - kx * Sy.x);
-        U[b + 1 * ss] = make_double2(-Tx.x - kz * Sy.y, -Tx.y + kz * Sy.x);
-        U[b + 2 * ss] = make_double2(-Ty.x + kz * Sx.y - kx * Sz.y, -Ty.y - kz * Sx.x + kx * Sz.x);
-        U[b + 3 * ss] = Sx;
-        U[b + 4 * ss] = Sz;
+        st_slot(U, b + 0 * ss, p32, -Tz.x + kx * Sy.y, -Tz.y - kx * Sy.x);
+        st_slot(U, b + 1 * ss, p32, -Tx.x - kz * Sy.y, -Tx.y + kz * Sy.x);
+        st_slot(U, b + 2 * ss, p32, -Ty.x + kz * Sx.y - kx * Sz.y, -Ty.y - kz * Sx.x + kx * Sz.x);
+        st_slot(U, b + 3 * ss, p32, Sx.x, Sx.y);
+        st_slot(U, b + 4 * ss, p32, Sz.x, Sz.y);
     }
 }
 
@@ -1430,9 +1440,11 @@ void launch_zfwd(Context &c, const cplx *W2, int nout, cplx *dst, int dst_kind, 
     a.abort_flag = c.abort_flag;
     a.ms_in = rstride(c, W2);
     a.ms_out = mstride(c, dst);
+    a.p32 = dst_kind == ZF_USLOTS && prods32(c);
     dim3 grid((d.Mx + a.XB - 1) / a.XB, nout * c.nrun, d.qpts);
     size_t lds = sizeof(real2) * (a.XB * d.Pz + d.Pz);
-    double bytes = c.nrun * (double) d.qpts * d.Mx * nout * (sizeof(real2) * d.Pz + 16.0 * (d.Nz - 1));
+    double bytes = c.nrun * (double) d.qpts * d.Mx * nout *
+                   (sizeof(real2) * d.Pz + (a.p32 ? 8.0 : 16.0) * (d.Nz - 1));
     const hipStream_t st = c.stream;
     const PlanDev pd = to_dev(c.pz);
     const ZfwdKernel kern = fixed == 96 ? k_zfwd<96> : fixed == 48 ? k_zfwd<48> : k_zfwd<0>;
@@ -1477,12 +1489,14 @@ bool launch_zfwd_adj2(Context &c, const cplx *W2, cplx *Udst, const cplx *W2b, c
     a.nm = c.nrun;
     a.ms_in = rstride(c, W2);
     a.ms_out = mstride(c, Udst);
+    a.p32 = prods32(c);
     if (W2b && (rstride(c, W2b) != a.ms_in || mstride(c, Udstb) != a.ms_out)) return false;
     const dim3 grid = fixed ? dim3(xcd_grid(a.nm * a.nprob * (d.Mx / ZA) * d.qpts))
                             : dim3((d.Mx + a.XA - 1) / a.XA, d.qpts, a.nm);
     // fixed lengths: the two-phase kernel on a 3-field tile
     size_t lds = sizeof(real2) * ((fixed ? 3 : 6) * (size_t) a.XA * d.Pz + d.Pz);
-    double bytes = a.nm * a.nprob * (double) d.qpts * d.Mx * (sizeof(real2) * 6.0 * d.Pz + 16.0 * 5.0 * (d.Nz - 1));
+    double bytes = a.nm * a.nprob * (double) d.qpts * d.Mx *
+                   (sizeof(real2) * 6.0 * d.Pz + (a.p32 ? 8.0 : 16.0) * 5.0 * (d.Nz - 1));
     const hipStream_t st = c.stream;
     const PlanDev pd = to_dev(c.pz);
     typedef void (*ZadjKernel)(ZadjArgs, PlanDev, const real2 *, cplx *);
@@ -1502,7 +1516,7 @@ void launch_assemble_adj(Context &c, const cplx *SP, cplx *Udst)
     int blocks = (int) std::min<size_t>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(k_assemble_adj, dim3(blocks, c.nrun), dim3(256), 0, c.stream, SP, Udst,
                        c.dKx, c.dKz, d.Nz, d.qpts, d.Mx, c.abort_flag, mstride(c, SP),
-                       mstride(c, Udst));
+                       mstride(c, Udst), (int) prods32(c));
 }
 
 }  // namespace SC_FFT_NS

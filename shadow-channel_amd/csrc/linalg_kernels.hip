@@ -1162,13 +1162,16 @@ struct ForceParShape {
     static constexpr int CK = pick(CKMAX), NCH = KW / CK;
     static constexpr size_t lds_bytes() { return sizeof(double) * ((size_t) NKS * 2 * NB * 64 + XCH); }
 };
-template <bool ADJ, int NKS>
+// P32: the product slots are FP32 complex (mixed precision; same element
+// index), converted to FP64 on load
+template <bool ADJ, int NKS, bool P32>
 __global__ __launch_bounds__(FR_THREADS) void k_force_par(ForceArgs a, int bpi, int nprob,
                                                           const cplx *U0, cplx *Fa0, cplx *Fb0,
                                                           const cplx *U1, cplx *Fa1, cplx *Fb1)
 {
     using SH = ForceParShape<ADJ, NKS>;
     constexpr int NB = SH::NB, NF = SH::NF, CK = SH::CK, NCH = SH::NCH, KS = SH::KS;
+    constexpr unsigned ESZ = P32 ? 8 : 16;                   // bytes of a slot value
     constexpr int KW = SH::KW, TPR = SH::TPR, XCH1 = SH::XCH1;
     extern __shared__ double bsf[];
     if (*a.abort_flag) return;
@@ -1220,21 +1223,25 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_par(ForceArgs a, int bpi, 
         k.nt = t - zr * nnt;
         k.xcol = min(k.nt * 8 + ncol_x(nl), Mx - 1);
         k.part = ncol_part(nl);
-        k.off = (unsigned) (((size_t) k.z * 5 * q * Mx + k.xcol) * 16);
+        k.off = (unsigned) (((size_t) k.z * 5 * q * Mx + k.xcol) * ESZ);
         return k;
     };
-    const unsigned ubytes = (unsigned) ((size_t) a.Nz * 5 * q * Mx * 16);
-    const unsigned rowb = (unsigned) Mx * 16, slotb = (unsigned) q * rowb;
+    const unsigned ubytes = (unsigned) ((size_t) a.Nz * 5 * q * Mx * ESZ);
+    const unsigned rowb = (unsigned) Mx * ESZ, slotb = (unsigned) q * rowb;
     // r[buf][0: y, 1: mirror][field][k-step]
     double r[2][2][NF][CK];
     auto load = [&](const Task &k, int ch, int buf) {
         __amdgpu_buffer_rsrc_t rs =
             __builtin_amdgcn_make_buffer_rsrc((void *) ((k.p ? U1 : U0) + k.m * a.ms_u), 0, ubytes, 0x00020000);
         const bool mean = k.z == 0 && k.xcol == 0;
-        const unsigned other = k.off + (mean ? 0 : 1 - k.part) * 8, own = k.off + k.part * 8;
+        const unsigned other = k.off + (mean ? 0 : 1 - k.part) * (ESZ / 2),
+                       own = k.off + k.part * (ESZ / 2);
         // lane offsets for the point and the part, uniform slot offsets
         auto ld = [&](unsigned v, unsigned slot) {
-            return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int) v, (int) (slot * slotb), 0));
+            if constexpr (P32)
+                return (double) __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int) v, (int) (slot * slotb), 0));
+            else
+                return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int) v, (int) (slot * slotb), 0));
         };
 #pragma unroll
         for (int s = 0; s < CK; ++s) {
@@ -1369,7 +1376,7 @@ __global__ __launch_bounds__(FR_THREADS) void k_force_par(ForceArgs a, int bpi, 
     }
 }
 
-template <bool ADJ, int NKS>
+template <bool ADJ, int NKS, bool P32>
 static void launch_force_par(const ForceArgs &a, int npair, int ntask, int nprob, const cplx *U0,
                              cplx *Fa0, cplx *Fb0, const cplx *U1, cplx *Fa1, cplx *Fb1,
                              hipStream_t st)
@@ -1378,14 +1385,14 @@ static void launch_force_par(const ForceArgs &a, int npair, int ntask, int nprob
     const size_t lds = SH::lds_bytes();
     static bool attr = false;
     if (!attr) {
-        (void) hipFuncSetAttribute((const void *) k_force_par<ADJ, NKS>,
+        (void) hipFuncSetAttribute((const void *) k_force_par<ADJ, NKS, P32>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
         attr = true;
     }
     // blocks per pair: a multiple of 8 (XCD-aware mapping), one block per CU
     const int cap = tuning().force_blocks > 0 ? tuning().force_blocks : 256 / npair / 8 * 8;
     const int bpi = std::max(8, std::min((ntask * nprob * a.nm + 7) / 8 * 8, cap));
-    hipLaunchKernelGGL((k_force_par<ADJ, NKS>), dim3(npair * bpi), dim3(FR_THREADS), lds, st, a, bpi,
+    hipLaunchKernelGGL((k_force_par<ADJ, NKS, P32>), dim3(npair * bpi), dim3(FR_THREADS), lds, st, a, bpi,
                        nprob, U0, Fa0, Fb0, U1, Fa1, Fb1);
 }
 
@@ -1428,8 +1435,10 @@ void launch_force2(Context &c, int adj, const cplx *U0, cplx *Fa0, cplx *Fb0, co
                 2. * M_PI / d.Lx, 2. * M_PI / d.Lz, c.abort_flag, nm, ms_u, ms_fa, ms_fb};
     dim3 grid(xcd_grid(c.nit * d.Nz), nm);
     const int nprob = U1 ? 2 : 1;
-    double bytes = nm * nprob * 16.0 * (double) (d.Nz - 1) * d.Mx *
-                   ((adj ? 5.0 : 3.0) * d.qpts + d.dimQ + d.dimR);
+    // mixed precision: FP32 product slots (the z-forward transforms wrote them)
+    const bool p32 = prods32(c);
+    double bytes = nm * nprob * (double) (d.Nz - 1) * d.Mx *
+                   ((p32 ? 8.0 : 16.0) * (adj ? 5.0 : 3.0) * d.qpts + 16.0 * (d.dimQ + d.dimR));
     const hipStream_t st = c.stream;
     const int nks = c.nksF, nit = c.nit, ntask = (d.Nz - 1) * ((d.Mx + 7) / 8);
     const int nksq = c.nksQ, npair = c.npair;
@@ -1440,8 +1449,9 @@ void launch_force2(Context &c, int adj, const cplx *U0, cplx *Fa0, cplx *Fb0, co
     ap.nit = npair;
     auto go = [=]() {
         if (par) {
-#define SC_FORCE_PAR(A, N) \
-    launch_force_par<A, N>(ap, npair, ntask, nprob, U0, Fa0, Fb0, U1, Fa1, Fb1, st)
+#define SC_FORCE_PAR(A, N)                                                                   \
+    (p32 ? launch_force_par<A, N, true>(ap, npair, ntask, nprob, U0, Fa0, Fb0, U1, Fa1, Fb1, st) \
+         : launch_force_par<A, N, false>(ap, npair, ntask, nprob, U0, Fa0, Fb0, U1, Fa1, Fb1, st))
             if (adj) {
                 if (nksq == 8) SC_FORCE_PAR(true, 8);
                 else if (nksq == 16) SC_FORCE_PAR(true, 16);

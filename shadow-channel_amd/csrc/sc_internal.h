@@ -290,6 +290,14 @@ SC_FFT_LAUNCHERS
 }
 // vel_only: only the velocity slots u, v, w are needed (du/dy and dw/dy may
 // be left unwritten where the kernel allows it)
+// Whether the product slots the z-forward transforms write (and the
+// Galerkin projections read) are FP32 complex: mixed-precision contexts whose
+// projections run in parity form.  Same element index as the FP64 layout.
+inline bool prods32(const Context &c)
+{
+    return c.mixed && c.ypar && (c.nksQ == 8 || c.nksQ == 16 || c.nksQ == 20 || c.nksQ == 28);
+}
+
 // Whether the Gauss-point slots the rebuild writes (and the z-inverse
 // transform reads) are FP32 complex: in mixed-precision contexts whose
 // rebuild runs in parity form on the FP32 matrix cores, except while an
