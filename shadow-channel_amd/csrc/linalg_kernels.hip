@@ -2084,36 +2084,77 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, int m
     }
     __syncthreads();
     if (a.k == 0) {
-        // E C + dt c_0 F with the stage-0 explicit operator (member 0's
-        // forcing rows still in registers, the others' loaded again)
-        for (int mb = 0; mb < nmb; ++mb) {
-            if (mb > 0) load_rows(mb);
+        // E C + dt c_0 F with the stage-0 explicit operator
+        if constexpr (MB <= 2) {
+            // each band entry formed once for the block's members; their
+            // forcing rows are still in registers (member 0's in fv0/sv0
+            // when MB > 1, the last member's in fv/sv)
 #pragma unroll
             for (int t = 0; t < VR; ++t) {
                 const int r = rt + t * RSTEP, i = par + 2 * r;
                 if (r >= nr || !co.chain) continue;
-                double2 f = fv[t], sf = sv[t];
-                if constexpr (MB > 1) {
+                double2 acc[MB];
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) acc[mb] = make_double2(0.0, 0.0);
+#pragma unroll
+                for (int m = -DL; m <= DL; ++m)
+                    if (r + m >= 0 && r + m < nr) {
+                        const double ev = op_entry(a, var, i, m, k2, ce0);
+#pragma unroll
+                        for (int mb = 0; mb < MB; ++mb)
+                            if (mb < nmb) {
+                                const double2 y = raw[(mb * nr + r + m) * SBT + cc];
+                                acc[mb].x += ev * y.x;
+                                acc[mb].y += ev * y.y;
+                            }
+                    }
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) {
+                    if (mb >= nmb) continue;
+                    double2 f = fv[t], sf = sv[t];
+                    if constexpr (MB > 1) {
+                        if (mb == 0) {
+                            f = fv0[t];
+                            sf = sv0[t];
+                        }
+                    }
+                    acc[mb].x += a.dtc * f.x;
+                    acc[mb].y += a.dtc * f.y;
+                    acc[mb].x += sf.x;
+                    acc[mb].y += sf.y;
+                    vec[(mb * nr + r) * SBT + cc] = acc[mb];
+                }
+            }
+        } else {
+            // member 0's forcing rows still in registers, the others'
+            // loaded again (one member's at a time)
+            for (int mb = 0; mb < nmb; ++mb) {
+                if (mb > 0) load_rows(mb);
+#pragma unroll
+                for (int t = 0; t < VR; ++t) {
+                    const int r = rt + t * RSTEP, i = par + 2 * r;
+                    if (r >= nr || !co.chain) continue;
+                    double2 f = fv[t], sf = sv[t];
                     if (mb == 0) {
                         f = fv0[t];
                         sf = sv0[t];
                     }
-                }
-                const double2 *rm = raw + mb * nr * SBT;
-                double2 acc = make_double2(0.0, 0.0);
+                    const double2 *rm = raw + mb * nr * SBT;
+                    double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
-                for (int m = -DL; m <= DL; ++m)
-                    if (r + m >= 0 && r + m < nr) {
-                        double ev = op_entry(a, var, i, m, k2, ce0);
-                        double2 y = rm[(r + m) * SBT + cc];
-                        acc.x += ev * y.x;
-                        acc.y += ev * y.y;
-                    }
-                acc.x += a.dtc * f.x;
-                acc.y += a.dtc * f.y;
-                acc.x += sf.x;
-                acc.y += sf.y;
-                vec[(mb * nr + r) * SBT + cc] = acc;
+                    for (int m = -DL; m <= DL; ++m)
+                        if (r + m >= 0 && r + m < nr) {
+                            double ev = op_entry(a, var, i, m, k2, ce0);
+                            double2 y = rm[(r + m) * SBT + cc];
+                            acc.x += ev * y.x;
+                            acc.y += ev * y.y;
+                        }
+                    acc.x += a.dtc * f.x;
+                    acc.y += a.dtc * f.y;
+                    acc.x += sf.x;
+                    acc.y += sf.y;
+                    vec[(mb * nr + r) * SBT + cc] = acc;
+                }
             }
         }
         __syncthreads();
@@ -2200,34 +2241,78 @@ __device__ void solve_block(const SolveArgs &a, int var, int par, int blk, int m
     }
     __syncthreads();
     // write the solutions and the next stage's right-hand sides
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb)
+    if constexpr (MB <= 2) {
+        // each band entry of the next stage's operator formed once for the
+        // block's members
 #pragma unroll
         for (int t = 0; t < VR; ++t) {
             const int r = rt + t * RSTEP, i = par + 2 * r;
-            if (mb >= nmb || r >= nr || !co.chain) continue;
-            const double2 *vm = vec + mb * nr * SBT;
+            if (r >= nr || !co.chain) continue;
             const size_t gi = cbase + r * rs;
-            const double2 sol = vm[r * SBT + cc];
-            Csm(mb)[gi] = sol;
+            double2 f2[MB], acc[MB];
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+                acc[mb] = make_double2(0.0, 0.0);
+                if (mb < nmb && a.has_next) f2[mb] = F2m(mb)[fbase + r * rs];
+            }
             if (a.has_next) {
-                const double2 f2 = F2m(mb)[fbase + r * rs];
-                double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
                 for (int m = -DL; m <= DL; ++m)
                     if (r + m >= 0 && r + m < nr) {
-                        double ev = op_entry(a, var, i, m, k2, cn);
-                        double2 y = vm[(r + m) * SBT + cc];
-                        acc.x += ev * y.x;
-                        acc.y += ev * y.y;
+                        const double ev = op_entry(a, var, i, m, k2, cn);
+#pragma unroll
+                        for (int mb = 0; mb < MB; ++mb)
+                            if (mb < nmb) {
+                                const double2 y = vec[(mb * nr + r + m) * SBT + cc];
+                                acc[mb].x += ev * y.x;
+                                acc[mb].y += ev * y.y;
+                            }
                     }
-                acc.x += a.dtd * f2.x;
-                acc.y += a.dtd * f2.y;
-                Cm(mb)[gi] = acc;
-            } else {
-                Cm(mb)[gi] = sol;
+            }
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+                if (mb >= nmb) continue;
+                const double2 sol = vec[(mb * nr + r) * SBT + cc];
+                Csm(mb)[gi] = sol;
+                if (a.has_next) {
+                    acc[mb].x += a.dtd * f2[mb].x;
+                    acc[mb].y += a.dtd * f2[mb].y;
+                    Cm(mb)[gi] = acc[mb];
+                } else {
+                    Cm(mb)[gi] = sol;
+                }
             }
         }
+    } else {
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int t = 0; t < VR; ++t) {
+                const int r = rt + t * RSTEP, i = par + 2 * r;
+                if (mb >= nmb || r >= nr || !co.chain) continue;
+                const double2 *vm = vec + mb * nr * SBT;
+                const size_t gi = cbase + r * rs;
+                const double2 sol = vm[r * SBT + cc];
+                Csm(mb)[gi] = sol;
+                if (a.has_next) {
+                    const double2 f2 = F2m(mb)[fbase + r * rs];
+                    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int m = -DL; m <= DL; ++m)
+                        if (r + m >= 0 && r + m < nr) {
+                            double ev = op_entry(a, var, i, m, k2, cn);
+                            double2 y = vm[(r + m) * SBT + cc];
+                            acc.x += ev * y.x;
+                            acc.y += ev * y.y;
+                        }
+                    acc.x += a.dtd * f2.x;
+                    acc.y += a.dtd * f2.y;
+                    Cm(mb)[gi] = acc;
+                } else {
+                    Cm(mb)[gi] = sol;
+                }
+            }
+    }
     // rows of v beyond dimQ are carried unchanged; the Nyquist row is zeroed
     const int nrest = (nR - par + 1) / 2;
     if (!co.valid || (co.z == 0 && co.x == 0)) return;
