@@ -51,7 +51,13 @@ def load():
     L.orc_getsoln.restype = c.c_void_p
     L.orc_getincresoln.argtypes = [c.c_int]
     L.orc_getincresoln.restype = c.c_void_p
+    L.orc_getincrestage.argtypes = [c.c_int]
+    L.orc_getincrestage.restype = c.c_void_p
     L.orc_quad.argtypes = [c.c_void_p, c.c_void_p]
+    L.orc_increadjproject.argtypes = [c.c_int, c.c_int] + [c.c_void_p] * 7
+    L.orc_increadjproject.restype = c.c_int
+    L.orc_adjproject_stage.argtypes = [c.c_int] + [c.c_void_p] * 4
+    L.orc_adjproject_stage.restype = c.c_int
     L.orc_get_array.argtypes = [c.c_char_p, dp, c.c_int]
     L.orc_get_array.restype = c.c_int
     L.orc_get_scalar.argtypes = [c.c_char_p]
@@ -146,10 +152,45 @@ class Oracle:
         assert p
         return self._view(p).copy()
 
+    def getincrestage(self, count):
+        p = self.L.orc_getincrestage(count)
+        assert p
+        return self._view(p).copy()
+
     def getincresoln(self, i):
         p = self.L.orc_getincresoln(i)
         assert p
         return self._view(p).copy()
+
+    def increadjproject(self, k, count, IAC, IU, LIU, walls, force=None):
+        """One stage k of the incremental adjoint (channel/increadjproject0.c,
+        channel/increadjproject.c) on copies of the inputs; returns
+        (IAC after the stage, IAU [Nz][5][qpts][Nx/2], IAUb [2][Nz][Nx/2])."""
+        IAC = _check_c(np.ascontiguousarray(IAC, dtype=np.complex128)).copy()
+        IU = np.ascontiguousarray(IU, dtype=np.complex128)
+        LIU = None if LIU is None else np.ascontiguousarray(LIU, dtype=np.complex128)
+        walls = np.ascontiguousarray(walls, dtype=np.complex128)
+        F = None if force is None else np.ascontiguousarray(force, dtype=np.complex128)
+        Nz, _, _, Mx = IAC.shape
+        IAU = np.zeros((Nz, 5, self.qpts, Mx), np.complex128)
+        IAUb = np.zeros((2, Nz, Mx), np.complex128)
+        rc = self.L.orc_increadjproject(k, count, _ptr(IAC), _ptr(IU),
+                                        None if LIU is None else _ptr(LIU), _ptr(walls),
+                                        None if F is None else _ptr(F), _ptr(IAU), _ptr(IAUb))
+        assert rc == 0, rc
+        return IAC, IAU, IAUb
+
+    def adjproject_stage(self, k, AC, U, LU):
+        """One adjoint stage k (channel/adjproject0.c, channel/adjproject.c)
+        on copies; returns (AC after the stage, AU)."""
+        AC = _check_c(np.ascontiguousarray(AC, dtype=np.complex128)).copy()
+        U = np.ascontiguousarray(U, dtype=np.complex128).copy()
+        LU = None if LU is None else np.ascontiguousarray(LU, dtype=np.complex128)
+        AU = np.zeros_like(U)
+        rc = self.L.orc_adjproject_stage(k, _ptr(AC), _ptr(U), None if LU is None else _ptr(LU),
+                                         _ptr(AU))
+        assert rc == 0, rc
+        return AC, AU
 
     def quad(self):
         y = np.zeros(self.qpts)

@@ -1590,6 +1590,12 @@ const orc_cplx *orc_getincresoln(int i)
     return G.MIC + csize() * 3L * i;
 }
 
+const orc_cplx *orc_getincrestage(int count)
+{
+    if (!G.ok || count < 0 || count > 3 * G.nsteps) return NULL;
+    return G.MIC + csize() * (long) count;
+}
+
 /* Reynolds-number source of the tangent at stage k of step n, per unit
    change of Re: the implicit and explicit operators M -+ re b/a dt D depend
    on re = 1/Re, so with TF(X) = (dre/dRe) D X = -re^2 D X (the forcing of
@@ -1746,6 +1752,357 @@ int orc_adjoint_objective(int start, int end, orc_cplx *ACg, double weight)
     }
     g_exact_mean = was_exact;
     memcpy(ACg, G.AC, sizeof(cx) * cs);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* incremental adjoint stage: increadjproject0                          */
+/* (channel/increadjproject0.c:14-357) for (0,0), then increadjproject  */
+/* (channel/increadjproject.c:10-645) for row 0 (x >= 1) and every row  */
+/* z != Nz/2, the order of the driver loop                              */
+/* (shan/increadjoint_test7.c:673-685).                                 */
+/* ------------------------------------------------------------------ */
+/* Galerkin projections of the wall lifts (channel/increadjproject.c:40-46,
+   the reference's table): LG[i] = 2 (R_i, (1-y)/2), LX[i] = (Q_i, lv) and
+   LR[i] = 2 (Q_i, lv'') with lv = (1-y)^2 (1+y)/4; the projections vanish
+   beyond these entries. */
+static const double LG[4] = { 4. / 3., -4. / 15., -4. / 15., 4. / 35. };
+static const double LR[8] = { -16. / 15., 16. / 35., 32. / 105., -32. / 105.,
+                              -16. / 315., 16. / 231., 0., 0. };
+static const double LX[8] = { 8. / 35., -8. / 315., -8. / 105., 8. / 385.,
+                              8. / 385., -8. / 1001., -8. / 3003., 8. / 6435. };
+
+/* viscous parts alone: K2^2 Qs + 2 K2 Qps + Qpps and -(Rps + K2 Rs) */
+static void visc_alpha(double *M, double K2)
+{
+    for (int e = 0; e < G.dimQ * 9; ++e)
+        M[e] = K2 * K2 * G.Qs[e] + 2. * K2 * G.Qps[e] + G.Qpps[e];
+}
+
+static void visc_beta(double *M, double K2)
+{
+    for (int e = 0; e < G.dimR * 5; ++e) M[e] = -(G.Rps[e] + K2 * G.Rs[e]);
+}
+
+/* one (z, x) column: y += coef * V * v (V banded, rows N, half width dl) */
+static void add_band(const double *V, const cx *v, long vs, cx *y, long ys, int N, int dl,
+                     double coef)
+{
+    cx t[1024];
+    band_mult(V, v, vs, t, 1, N, dl);
+    for (int i = 0; i < N; ++i) {
+        y[i * ys].re += t[i].re * coef;
+        y[i * ys].im += t[i].im * coef;
+    }
+}
+
+static inline cx wall_at(const cx *W, int f, int z, int x)
+{
+    return W[((long) f * G.Nz + z) * G.Mx + x];
+}
+
+/* forcing value: stage s (0: k, 1: k+1), side v (0 alpha, 1 beta) */
+static inline cx force_at(const cx *F, int s, int v, int z, int x, int i)
+{
+    cx zero = { 0.0, 0.0 };
+    if (!F) return zero;
+    return F[((((long) s * 2 + v) * G.Nz + z) * G.Mx + x) * G.dimR + i];
+}
+
+static void iadj_modes(cx *Ca, const cx *Ua, const cx *LUa, const cx *W, const cx *F,
+                       cx *Uout, cx *Ub, int k, int count, int z, int x0)
+{
+    int Mx = G.Mx, nQ = G.dimQ, nR = G.dimR, q = G.qpts;
+    const double *a = BA, *b = BB, *c = BC, *d = BD;
+    double dt = G.dt, re = G.re, kz = G.Kz[z];
+    double *M = G.band;
+    const cx *mic0 = G.MIC + csize() * (long) count;
+    const cx *mic1 = count > 0 ? G.MIC + csize() * (long) (count - 1) : NULL;
+    for (int x = x0; x < Mx; ++x) {
+        double k2 = G.K2[z * Mx + x];
+        cx *al = Ca + ci(z, 0, 0, x);
+        cx tm[1024];
+        if (k == 0) {
+            op_alpha(M, k2, re * a[0] * dt);
+            band_mult(M, al, Mx, tm, 1, nQ, 4);
+            for (int i = 0; i < nQ; ++i) al[i * Mx] = tm[i];
+            visc_alpha(M, k2);
+            add_band(M, mic0 + ci(z, 0, 0, x), Mx, al, Mx, nQ, 4, re * dt * (a[0] + b[0]));
+        }
+    }
+    forcing_modes(Ua, z, x0, 1);
+    for (int x = x0; x < Mx; ++x) {
+        double k2 = G.K2[z * Mx + x], k4 = k2 * k2;
+        cx *al = Ca + ci(z, 0, 0, x);
+        cx wb = wall_at(W, 1, z, x), wbt = wall_at(W, 3, z, x), awb = wall_at(W, 5, z, x);
+        for (int i = 0; i < nQ; ++i) {
+            cx f = G.Fa[i * Mx + x], g = force_at(F, 0, 0, z, x, i);
+            al[i * Mx].re += dt * c[k] * (f.re + g.re);
+            al[i * Mx].im += dt * c[k] * (f.im + g.im);
+        }
+        for (int i = 0; i < 8 && i < nQ; ++i) {
+            double m = LR[i] / 2. - k2 * LX[i], v = -k2 * LR[i] + k4 * LX[i];
+            al[i * Mx].re += m * (wbt.re - wb.re) + dt * re * (a[k] * wbt.re + b[k] * wb.re) * v +
+                             dt * re * (a[k] + b[k]) * awb.re * v;
+            al[i * Mx].im += m * (wbt.im - wb.im) + dt * re * (a[k] * wbt.im + b[k] * wb.im) * v +
+                             dt * re * (a[k] + b[k]) * awb.im * v;
+        }
+        op_alpha(M, k2, -(re * b[k] * dt));
+        band_factor(M, nQ, 4);
+        band_subst(M, al, Mx, nQ, 4);
+    }
+    for (int x = x0; x < Mx; ++x) {
+        double k2 = G.K2[z * Mx + x];
+        cx *be = Ca + ci(z, 1, 0, x);
+        cx ub = wall_at(W, 0, z, x), ubt = wall_at(W, 2, z, x), aub = wall_at(W, 4, z, x);
+        cx tm[1024];
+        if (k == 0) {
+            op_beta(M, k2, -(re * a[0] * dt));
+            band_mult(M, be, Mx, tm, 1, nR, 2);
+            for (int i = 0; i < nR; ++i) be[i * Mx] = tm[i];
+            visc_beta(M, k2);
+            add_band(M, mic0 + ci(z, 1, 0, x), Mx, be, Mx, nR, 2, re * dt * (a[0] + b[0]));
+        }
+        for (int i = 0; i < nR; ++i) {
+            cx f = G.Fb[i * Mx + x], g = force_at(F, 0, 1, z, x, i);
+            be[i * Mx].re += dt * c[k] * (f.re + g.re);
+            be[i * Mx].im += dt * c[k] * (f.im + g.im);
+        }
+        for (int i = 0; i < 4 && i < nR; ++i) {
+            double h = LG[i] / 2.;
+            be[i * Mx].re += h * (ubt.re - ub.re) + dt * re * (a[k] * ubt.re + b[k] * ub.re) * (-k2 * h) +
+                             dt * re * (a[k] + b[k]) * aub.re * (-k2 * h);
+            be[i * Mx].im += h * (ubt.im - ub.im) + dt * re * (a[k] * ubt.im + b[k] * ub.im) * (-k2 * h) +
+                             dt * re * (a[k] + b[k]) * aub.im * (-k2 * h);
+        }
+        op_beta(M, k2, re * b[k] * dt);
+        band_factor(M, nR, 2);
+        band_subst(M, be, Mx, nR, 2);
+    }
+    /* Gauss-point values with the wall lifts; the omega_y sums run over the
+       first dimQ coefficients (channel/increadjproject.c:371-395) */
+    for (int x = x0; x < Mx; ++x) {
+        double kx = G.Kx[x], k2 = G.K2[z * Mx + x];
+        const cx *al = Ca + ci(z, 0, 0, x), *be = Ca + ci(z, 1, 0, x);
+        cx ub = wall_at(W, 0, z, x), wb = wall_at(W, 1, z, x);
+        for (int i = 0; i < q; ++i) {
+            double y = G.y[i];
+            double vr = 0, vi = 0, fr = 0, fi = 0, qr = 0, qi = 0, gr = 0, gi = 0, pr = 0, pi = 0;
+            for (int j = 0; j < nQ; ++j) {
+                cx aj = al[j * Mx], bj = be[j * Mx];
+                double cq = G.Q[i * nQ + j], cp = G.Qp[i * nQ + j], c2 = G.Qpp[i * nQ + j];
+                double cr = G.R[i * nR + j], crp = G.Rp[i * nR + j];
+                vr += cq * aj.re; vi += cq * aj.im;
+                fr -= cp * aj.re; fi -= cp * aj.im;
+                qr += c2 * aj.re; qi += c2 * aj.im;
+                gr += cr * bj.re; gi += cr * bj.im;
+                pr += crp * bj.re; pi += crp * bj.im;
+            }
+            double lv = (1 - y) * (1 - y) * (1 + y) * 0.25, lv1 = (3 * y * y - 2 * y - 1) * 0.25;
+            double lv2 = (3 * y - 1) * 0.5, lg = (1 - y) * 0.5;
+            vr += wb.re * lv; vi += wb.im * lv;
+            fr -= wb.re * lv1; fi -= wb.im * lv1;
+            qr += wb.re * lv2; qi += wb.im * lv2;
+            gr += ub.re * lg; gi += ub.im * lg;
+            pr += -ub.re / 2.; pi += -ub.im / 2.;
+            cx *o = Uout;
+            o[ui(z, S_V, i, x)].re = vr;
+            o[ui(z, S_V, i, x)].im = vi;
+            o[ui(z, S_U, i, x)].re = (kx * fi + kz * gi) / k2;
+            o[ui(z, S_U, i, x)].im = -(kx * fr + kz * gr) / k2;
+            o[ui(z, S_W, i, x)].re = (-kx * gi + kz * fi) / k2;
+            o[ui(z, S_W, i, x)].im = (kx * gr - kz * fr) / k2;
+            o[ui(z, S_DU, i, x)].re = (-kx * qi + kz * pi) / k2;
+            o[ui(z, S_DU, i, x)].im = (kx * qr - kz * pr) / k2;
+            o[ui(z, S_DW, i, x)].re = -(kx * pi + kz * qi) / k2;
+            o[ui(z, S_DW, i, x)].im = (kx * pr + kz * qr) / k2;
+        }
+        /* wall values IAUxb, IAUzb (channel/increadjproject.c:446-485) */
+        cx *xb = Ub + z * Mx + x, *zb = Ub + ((long) G.Nz + z) * Mx + x;
+        for (int j = 0; j < nR; ++j) {
+            cx bj = be[j * Mx];
+            xb->re += (G.Rp0[j] * bj.im) * kz / k2;
+            xb->im += (-G.Rp0[j] * bj.re) * kz / k2;
+            zb->re += -(G.Rp0[j] * bj.im) * kx / k2;
+            zb->im += (G.Rp0[j] * bj.re) * kx / k2;
+        }
+        xb->re += -ub.im / 2. * kz / k2;
+        xb->im += ub.re / 2. * kz / k2;
+        zb->re += ub.im / 2. * kx / k2;
+        zb->im += -ub.re / 2. * kx / k2;
+        for (int j = 0; j < nQ; ++j) {
+            double sg = (j & 1) ? -1.0 : 1.0;
+            cx aj = al[j * Mx];
+            xb->re -= (sg * 8. * aj.im) * kx / k2;
+            xb->im -= (-sg * 8. * aj.re) * 2 * kx / k2;
+            zb->re -= (sg * 8. * aj.im) * kz / k2;
+            zb->im -= (-sg * 8. * aj.re) * 2 * kz / k2;
+        }
+        xb->re -= -wb.im * 2. * kx / k2;
+        xb->im -= wb.re * 2. * kx / k2;
+        zb->re -= -wb.im * 2. * kz / k2;
+        zb->im -= wb.re * 2 * kz / k2;
+    }
+    if (k == 2) return;
+    /* explicit part of the next stage: LIU, forcing k+1, MIC[count-1] */
+    forcing_modes(LUa, z, x0, 1);
+    for (int x = x0; x < Mx; ++x) {
+        double k2 = G.K2[z * Mx + x];
+        cx *al = Ca + ci(z, 0, 0, x), *be = Ca + ci(z, 1, 0, x);
+        cx tm[1024];
+        op_alpha(M, k2, re * a[k + 1] * dt);
+        band_mult(M, al, Mx, tm, 1, nQ, 4);
+        for (int i = 0; i < nQ; ++i) {
+            cx f = G.Fa[i * Mx + x], g = force_at(F, 1, 0, z, x, i);
+            al[i * Mx].re = tm[i].re + dt * d[k + 1] * (f.re + g.re);
+            al[i * Mx].im = tm[i].im + dt * d[k + 1] * (f.im + g.im);
+        }
+        visc_alpha(M, k2);
+        add_band(M, mic1 + ci(z, 0, 0, x), Mx, al, Mx, nQ, 4, re * dt * (a[k + 1] + b[k + 1]));
+        op_beta(M, k2, -(re * a[k + 1] * dt));
+        band_mult(M, be, Mx, tm, 1, nR, 2);
+        for (int i = 0; i < nR; ++i) {
+            cx f = G.Fb[i * Mx + x], g = force_at(F, 1, 1, z, x, i);
+            be[i * Mx].re = tm[i].re + dt * d[k + 1] * (f.re + g.re);
+            be[i * Mx].im = tm[i].im + dt * d[k + 1] * (f.im + g.im);
+        }
+        visc_beta(M, k2);
+        add_band(M, mic1 + ci(z, 1, 0, x), Mx, be, Mx, nR, 2, re * dt * (a[k + 1] + b[k + 1]));
+    }
+}
+
+static void iadj_mean(cx *Ca, const cx *Ua, const cx *LUa, const cx *W, const cx *F, cx *Uout,
+                      cx *Ub, int k, int count)
+{
+    int nR = G.dimR, q = G.qpts;
+    const double *a = BA, *b = BB, *c = BC, *d = BD;
+    double dt = G.dt, re = G.re;
+    double M[5 * 1024], fa[1024], fb[1024];
+    cx *al = Ca + ci(0, 0, 0, 0), *be = Ca + ci(0, 1, 0, 0);
+    long s = G.Mx;
+    const cx *mic0 = G.MIC + csize() * (long) count;
+    const cx *mic1 = count > 0 ? G.MIC + csize() * (long) (count - 1) : NULL;
+    cx ub = wall_at(W, 0, 0, 0), wb = wall_at(W, 1, 0, 0);
+    cx ubt = wall_at(W, 2, 0, 0), wbt = wall_at(W, 3, 0, 0);
+    if (k == 0) {
+        cx tm[1024];
+        op_mean(M, -(re * a[0] * dt));
+        band_mult(M, al, s, tm, 1, nR, 2);
+        for (int i = 0; i < nR; ++i) al[i * s] = tm[i];
+        add_band(G.Rps, mic0 + ci(0, 0, 0, 0), s, al, s, nR, 2, -(dt * re * (a[0] + b[0])));
+        band_mult(M, be, s, tm, 1, nR, 2);
+        for (int i = 0; i < nR; ++i) be[i * s] = tm[i];
+        add_band(G.Rps, mic0 + ci(0, 1, 0, 0), s, be, s, nR, 2, -(dt * re * (a[0] + b[0])));
+    }
+    mean_rhs(Ua, 1, fa, fb);
+    op_mean(M, re * b[k] * dt);
+    band_factor(M, nR, 2);
+    /* only the real part of the forcing reaches the stage's right-hand side
+       (channel/increadjproject0.c:130-141) */
+    for (int i = 0; i < nR; ++i) al[i * s].re += dt * c[k] * (fa[i] + force_at(F, 0, 0, 0, 0, i).re);
+    for (int i = 0; i < nR; ++i) {
+        double lr = 0, li = 0;
+        for (int j = 0; j < q; ++j) {
+            lr += G.Rw[i * q + j] * (1 - G.y[j]) * 0.5 * (ubt.re - ub.re);
+            li += G.Rw[i * q + j] * (1 - G.y[j]) * 0.5 * (ubt.im - ub.im);
+        }
+        al[i * s].re += lr;
+        al[i * s].im += li;
+    }
+    band_subst(M, al, s, nR, 2);
+    for (int i = 0; i < nR; ++i) be[i * s].re += dt * c[k] * (fb[i] + force_at(F, 0, 1, 0, 0, i).re);
+    for (int i = 0; i < nR; ++i) {
+        double lr = 0, li = 0;
+        for (int j = 0; j < q; ++j) {
+            lr += G.Rw[i * q + j] * (1 - G.y[j]) * 0.5 * (wbt.re - wb.re);
+            li += G.Rw[i * q + j] * (1 - G.y[j]) * 0.5 * (wbt.im - wb.im);
+        }
+        be[i * s].re += lr;
+        be[i * s].im += li;
+    }
+    band_subst(M, be, s, nR, 2);
+    for (int i = 0; i < q; ++i) {
+        double ur = 0, uim = 0, dr = 0, di = 0, wr = 0, wi = 0, er = 0, ei = 0;
+        for (int j = 0; j < nR; ++j) {
+            cx aj = al[j * s], bj = be[j * s];
+            double r = G.R[i * nR + j], rp = G.Rp[i * nR + j];
+            ur += r * aj.re; uim += r * aj.im;
+            dr += rp * aj.re; di += rp * aj.im;
+            wr += r * bj.re; wi += r * bj.im;
+            er += rp * bj.re; ei += rp * bj.im;
+        }
+        double lg = (1 - G.y[i]) * 0.5;
+        Uout[ui(0, S_U, i, 0)].re = ur + ub.re * lg;
+        Uout[ui(0, S_U, i, 0)].im = uim + ub.im * lg;
+        Uout[ui(0, S_DU, i, 0)].re = dr - ub.re / 2.;
+        Uout[ui(0, S_DU, i, 0)].im = di - ub.im / 2.;
+        Uout[ui(0, S_V, i, 0)].re = 0;
+        Uout[ui(0, S_V, i, 0)].im = 0;
+        Uout[ui(0, S_W, i, 0)].re = wr + wb.re * lg;
+        Uout[ui(0, S_W, i, 0)].im = wi + wb.im * lg;
+        Uout[ui(0, S_DW, i, 0)].re = er - wb.re / 2.;
+        Uout[ui(0, S_DW, i, 0)].im = ei - wb.im / 2.;
+    }
+    cx *xb = Ub, *zb = Ub + (long) G.Nz * G.Mx;
+    for (int j = 0; j < nR; ++j) {
+        xb->re += G.Rp0[j] * al[j * s].re;
+        xb->im += G.Rp0[j] * al[j * s].im;
+        zb->re += G.Rp0[j] * be[j * s].re;
+        zb->im += G.Rp0[j] * be[j * s].im;
+    }
+    xb->re -= ub.re / 2.;
+    xb->im -= ub.im / 2.;
+    zb->re -= wb.re / 2.;
+    zb->im -= wb.im / 2.;
+    if (k == 2) return;
+    mean_rhs(LUa, 1, fa, fb);
+    cx tm[1024];
+    op_mean(M, -(re * a[k + 1] * dt));
+    band_mult(M, al, s, tm, 1, nR, 2);
+    for (int i = 0; i < nR; ++i) {
+        cx g = force_at(F, 1, 0, 0, 0, i);
+        al[i * s].re = tm[i].re + dt * d[k + 1] * (fa[i] + g.re);
+        al[i * s].im = tm[i].im + dt * d[k + 1] * g.im;
+    }
+    add_band(G.Rps, mic1 + ci(0, 0, 0, 0), s, al, s, nR, 2, -(dt * re * (a[k + 1] + b[k + 1])));
+    band_mult(M, be, s, tm, 1, nR, 2);
+    for (int i = 0; i < nR; ++i) {
+        cx g = force_at(F, 1, 1, 0, 0, i);
+        be[i * s].re = tm[i].re + dt * d[k + 1] * (fb[i] + g.re);
+        be[i * s].im = tm[i].im + dt * d[k + 1] * g.im;
+    }
+    add_band(G.Rps, mic1 + ci(0, 1, 0, 0), s, be, s, nR, 2, -(dt * re * (a[k + 1] + b[k + 1])));
+}
+
+/* one adjoint stage over all modes (adjproject0 + adjproject, the loop of
+   channel/channel.c:668-679) on caller arrays, for pinning tests */
+int orc_adjproject_stage(int k, orc_cplx *AC, orc_cplx *U, const orc_cplx *LU, orc_cplx *AU)
+{
+    if (!G.ok || k < 0 || k > 2 || !AC || !U || (k < 2 && !LU) || !AU) return -1;
+    stage_all((cx *) AC, (cx *) U, (const cx *) LU, (cx *) AU, NULL, k, 2, 0.0);
+    return 0;
+}
+
+int orc_increadjproject(int k, int count, orc_cplx *IAC, const orc_cplx *IU,
+                        const orc_cplx *LIU, const orc_cplx *W, const orc_cplx *F,
+                        orc_cplx *IAU, orc_cplx *IAUb)
+{
+    if (!G.ok || k < 0 || k > 2 || count < (k < 2 ? 1 : 0) || count > 3 * G.nsteps) return -1;
+    if (!IAC || !IU || (k < 2 && !LIU) || !W || !IAU || !IAUb) return -1;
+    cx *Ca = (cx *) IAC, *Uout = (cx *) IAU, *Ub = (cx *) IAUb;
+    const cx *Ua = (const cx *) IU, *LUa = (const cx *) LIU, *Wc = (const cx *) W;
+    const cx *Fc = (const cx *) F;
+    memset(Ub, 0, sizeof(cx) * 2 * G.Nz * G.Mx);
+    iadj_mean(Ca, Ua, LUa, Wc, Fc, Uout, Ub, k, count);
+    iadj_modes(Ca, Ua, LUa, Wc, Fc, Uout, Ub, k, count, 0, 1);
+    for (int z = 1; z < G.Nz; ++z) {
+        if (z == G.Nz / 2) {
+            for (int sl = 0; sl < 5; ++sl)
+                memset(Uout + ui(z, sl, 0, 0), 0, sizeof(cx) * G.qpts * G.Mx);
+            continue;
+        }
+        iadj_modes(Ca, Ua, LUa, Wc, Fc, Uout, Ub, k, count, z, 0);
+    }
     return 0;
 }
 

@@ -58,6 +58,8 @@ void   orc_set_exact_mean_adjoint(int on);
 void   orc_spec2phys(const orc_cplx *C, double *flow);
 const orc_cplx *orc_getsoln(int i_step);
 const orc_cplx *orc_getincresoln(int i_step);
+/* MIC[count], the tangent state stored at stage slot count (0 .. 3 nsteps). */
+const orc_cplx *orc_getincrestage(int count);
 void   orc_quad(double *y, double *w);
 int    orc_qpts(void);
 int    orc_dimR(void);
@@ -75,7 +77,25 @@ void   orc_set_faithful_cost(int on);
 void   orc_srand(unsigned seed);
 int    orc_rand(void);
 
+/* One Runge-Kutta stage k of the incremental adjoint over all modes
+   (channel/increadjproject0.c:14-357, channel/increadjproject.c:10-645):
+   IAC [Nz][2][dimR][Nx/2] in/out (the next stage's right-hand side after
+   k < 2), IU / LIU [Nz][5][qpts][Nx/2] the incremental adjoint products of
+   this stage and of the next (LIU unused at k = 2), W [6][Nz][Nx/2] the wall
+   planes Uxb, Uzb, Uxbt, Uzbt, AUxb, AUzb, F [2][2][Nz][Nx/2][dimR] or NULL
+   the forcing of stages k and k+1 on the alpha and beta equations, IAU
+   [Nz][5][qpts][Nx/2] and IAUb [2][Nz][Nx/2] (IAUxb, IAUzb) out; the tangent
+   states MIC[count] and MIC[count-1] of the last orc_tangent.  Requires
+   1 <= count (0 at k = 2) <= 3 nsteps; -1 on bad arguments. */
+int orc_increadjproject(int k, int count, orc_cplx *IAC, const orc_cplx *IU,
+                        const orc_cplx *LIU, const orc_cplx *W, const orc_cplx *F,
+                        orc_cplx *IAU, orc_cplx *IAUb);
+
 /* ---- internals exposed for pinning tests ---- */
+/* One adjoint stage k over all modes (channel/adjproject0.c,
+   channel/adjproject.c) on caller arrays: AC in/out, U the adjoint products
+   of the stage, LU those of the next (k < 2), AU out. */
+int orc_adjproject_stage(int k, orc_cplx *AC, orc_cplx *U, const orc_cplx *LU, orc_cplx *AU);
 /* name: "Q","Qp","Qpp","R","Rp","Qw","Qpw","Qppw","Rw","Rpw" (row-major),
    "Qs","Qps","Qpps" (dimQ x 9), "Rs","Rps" (dimR x 5), "Rp0","Rpp0",
    "Kx","Kz","cfl2","y","w".  Returns the element count, or -1. */
