@@ -247,6 +247,95 @@ def cpu_baseline():
                       "tangent/incremental-adjoint transforms), %.1f s" % t}
 
 
+def iadj_flops(q, nQ, nR, k):
+    """Algorithmic FP64 operations of one mode of the incremental adjoint
+    stage: the Galerkin projections (alpha: Qpw, Qw, Qppw rows; beta: Rw,
+    Rpw; a real-by-complex multiply-add is 4 operations) of this stage's
+    products and, for k < 2, the next stage's; the five y-sums of the
+    Gauss-point values; the band products (9- and 5-point) of the explicit
+    parts and the tangent coupling."""
+    proj = 4 * q * (3 * nQ + 2 * nR)
+    band = 4 * 2 * (9 * nQ + 5 * nR)
+    f = proj + 4 * 5 * q * nQ + (band if k == 0 else 0)
+    if k < 2:
+        f += proj + band
+    return f
+
+
+def iadj_cpu_baseline(IAC, IU, LIU, W):
+    """The oracle's incremental adjoint stage (single thread) on the same
+    inputs at the same grid, after its own one-step primal and tangent."""
+    import pyorc
+    from channel.synthetic import synthetic_state
+    c = CFG
+    o = pyorc.Oracle(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"], 1)
+    C0 = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=0)
+    o.primal(0, C0.copy())
+    o.tangent(0, 1, unit_direction(C0.shape, 7), 0)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        o.increadjproject(0, 3, IAC, IU, LIU, W)
+    t = (time.perf_counter() - t0) / reps
+    o.destroy()
+    return {"value": round(1.0 / t, 3), "unit": "stages/s", "cores": 1, "kind": "port",
+            "sample": "%d stages (k = 0 with the next stage's explicit part) of the 64x64x97 "
+                      "grid on the same inputs, oracle (FP64 restatement, single thread), "
+                      "%.2f s each" % (reps, t)}
+
+
+def incremental_adjoint_line(cc, calls, cpu=True):
+    """Kernel time of the incremental adjoint stage (sc_increadjproject,
+    channel/increadjproject.c): stage 0 with the next stage's explicit part
+    at the benchmark grid, random products and wall planes, the tangent
+    states of a one-step tangent run; `calls` calls, each on fresh inputs,
+    the stage's launches (projections, right-hand sides and solves, the
+    y-transform, the wall lifts) bracketed by HIP events on the library's
+    stream."""
+    from channel.synthetic import synthetic_state
+    c = CFG
+    assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"],
+                     1) == 0
+    C0 = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=0)
+    cc.c_primal(0, C0.copy())
+    cc.c_tangent(0, 1, unit_direction(C0.shape, 7), 0)
+    Nz, Mx, q, nR, nQ = c["Nz"], c["Nx"] // 2, cc.c_qpts(), cc.c_dimR(), cc.c_dimQ()
+    rng = np.random.default_rng(0)
+
+    def rc(shape):
+        return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+    IAC, IU, LIU, W = rc((Nz, 2, nR, Mx)), rc((Nz, 5, q, Mx)), rc((Nz, 5, q, Mx)), rc((6, Nz, Mx))
+    cc.sc_increadjproject(0, 3, IAC.copy(), IU, LIU, W)      # untimed: constants, code
+    cc.sc_prof_enable(8)
+    for _ in range(calls):
+        cc.sc_increadjproject(0, 3, IAC.copy(), IU, LIU, W)
+    n, ms, nbytes = cc.sc_prof_read()
+    cc.sc_prof_enable(0)
+    cc.c_destroy()
+    base = iadj_cpu_baseline(IAC, IU, LIU, W) if cpu else None
+    avg_s = ms / n / 1000.0
+    modes = (Nz - 1) * Mx - 1
+    flops = modes * iadj_flops(q, nQ, nR, 0)
+    tf = flops / avg_s / 1e12
+    gbs = nbytes / n / avg_s / 1e9
+    return {"metric": "incremental adjoint stage (increadjproject) at 64x64x97",
+            "value": round(1.0 / avg_s, 1), "unit": "stages/s", "n_gpus": 1, "steps": n,
+            "warmup": 1, "ms_per_step": round(avg_s * 1e3, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "one incremental adjoint stage (k = 0 with the next stage's "
+                                   "explicit part) over every mode of the 64x64x97 grid, random "
+                                   "products and wall planes, tangent states of a one-step run",
+                       "modes": modes, "flops_per_launch": flops,
+                       "bytes_per_launch": round(nbytes / n)},
+            "roofline": {"kernel": "incremental adjoint stage (all launches)", "bound": "mfma", "achieved": round(tf, 3),
+                         "peak": 78.6, "unit": "TFLOP/s", "frac": round(tf / 78.6, 4),
+                         "traffic": None, "hbm_achieved_gbs": round(gbs, 1),
+                         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "avg_launch_us": round(avg_s * 1e6, 2)},
+            "cpu_baseline": base}
+
+
 def main():
     # the library prints the reference's grid banner on stdout at init: send
     # file descriptor 1 to stderr while it runs so that stdout carries only
@@ -280,6 +369,9 @@ def run():
                     help="independent members advanced together in the headline figure")
     ap.add_argument("--retune", action="store_true",
                     help="time the launch shapes instead of using channel/shapes.json")
+    ap.add_argument("--incremental-adjoint", type=int, default=0, metavar="CALLS",
+                    help="instead of the headline, time the incremental adjoint stage's "
+                         "kernel over CALLS calls and print its own line")
     args = ap.parse_args()
     global RETUNE
     RETUNE = args.retune
@@ -290,6 +382,8 @@ def run():
         return None
     import c_channel as cc
     from channel.synthetic import synthetic_state
+    if args.incremental_adjoint > 0:
+        return incremental_adjoint_line(cc, args.incremental_adjoint, not args.no_cpu_baseline)
     c = CFG
     K, W = args.steps, args.warmup
     C0 = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=0)

@@ -131,6 +131,35 @@ int sc_wall_plane(const sc_complex *C_ptr, sc_complex *Uxb, sc_complex *Uzb);
    Nz < 4. */
 int sc_incre_boundary(sc_complex *Uxb, sc_complex *Uzb);
 
+/* channel/increadjproject0.c:14-357 (void increadjproject0(int k, int n,
+   int count, func_force_t force)) for the mean mode and
+   channel/increadjproject.c:10-645 (void increadjproject(int n, int k, int z,
+   int x0, int count, func_force_t force)) for every other mode, in the
+   order of the reference's driver loop (shan/increadjoint_test7.c:673-685):
+   one Runge-Kutta stage k (0..2) of the incremental adjoint, on host arrays
+     IAC   [Nz][2][dimR][Nx/2]   in: the stage's right-hand side; out: the
+                                 next stage's (k < 2) or the solution (k = 2);
+                                 row Nz/2 is left as it is
+     IU    [Nz][5][qpts][Nx/2]   the stage's incremental adjoint products
+                                 (slots 0..2 H_z, H_x, H_y; slots 3, 4 S_x, S_z:
+                                 the IU of channel/pass2.c)
+     LIU   as IU                 the next stage's products (NULL allowed at k = 2)
+     walls [6][Nz][Nx/2]         Uxb, Uzb, Uxbt, Uzbt, AUxb, AUzb
+     force [2][2][Nz][Nx/2][dimR] or NULL: what the reference's force callback
+                                 returns for stages k and k+1, the alpha
+                                 equation's values (modes: its second array,
+                                 the mean: its first) then the beta equation's
+     IAU   [Nz][5][qpts][Nx/2]   out: u, v, w, du/dy, dw/dy at the Gauss points
+     IAUb  [2][Nz][Nx/2]         out: IAUxb, IAUzb
+   The tangent states MIC[count] and MIC[count-1] are those of the last
+   tangent run (c_tangent, or member 0 of sc_tangent_members).  The callback
+   argument n of the reference only reaches the callback and has no
+   counterpart here.  SC_EINVAL unless 0 <= k <= 2, 1 <= count <= 3 nsteps
+   (0 allowed at k = 2) and dimR <= 128. */
+int sc_increadjproject(int k, int count, sc_complex *IAC, const sc_complex *IU,
+                       const sc_complex *LIU, const sc_complex *walls, const sc_complex *force,
+                       sc_complex *IAU, sc_complex *IAUb);
+
 /* channel/__init__.py:217-299 (velAvg, IvelAvg and the profiles of
    uavgGrad) and the wall shear Uxb(0,0) of channel/project0.c:227-234, as
    device reductions over stored steps start_step .. end_step - 1 of the
@@ -176,7 +205,8 @@ void sc_srand(unsigned seed);
    number of launches and the summed milliseconds, and clears the record.
    Classes: 1 z-inverse FFT, 2 x transform/products, 3 z-forward FFT,
    4 y-transform to Gauss points, 5 Galerkin projection, 6 band solves,
-   7 the merged adjoint x-kernel (two product sets of one lambda). */
+   7 the merged adjoint x-kernel (two product sets of one lambda),
+   8 the incremental adjoint stage's launches (sc_increadjproject). */
 int sc_prof_enable(int kernel_class);
 int sc_prof_read(double *total_ms);
 /* Algorithmic bytes (compulsory reads + writes, each element once) of the

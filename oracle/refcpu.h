@@ -10,7 +10,7 @@
  * channel/RsQs.c, channel/waveNumsAndCFL.c), the nonlinear terms
  * (channel/pass1.c, channel/pass2.c), the Galerkin/Helmholtz solves
  * (channel/project*.c, channel/increproject*.c, channel/adjproject*.c,
- * channel/mvOps.c), the coefficient<->Gauss-point transforms
+ * channel/increadjproject*.c, channel/mvOps.c), the coefficient<->Gauss-point transforms
  * (channel/initAlphaBeta*.c, channel/spec2phys.c) and the drivers
  * (channel/channel.c: init, primal, tangent, adjoint, ddt_project).
  *

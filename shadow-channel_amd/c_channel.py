@@ -82,6 +82,8 @@ _L.sc_wall_plane.argtypes = [_c.c_void_p, _c.c_void_p, _c.c_void_p]
 _L.sc_wall_plane.restype = _i
 _L.sc_incre_boundary.argtypes = [_c.c_void_p, _c.c_void_p]
 _L.sc_incre_boundary.restype = _i
+_L.sc_increadjproject.argtypes = [_i, _i] + [_c.c_void_p] * 7
+_L.sc_increadjproject.restype = _i
 _L.sc_primal_dev.argtypes = [_i, _c.c_void_p]
 _L.sc_primal_dev.restype = _i
 _L.sc_adjoint_dev.argtypes = [_i, _i, _c.c_void_p, _d]
@@ -435,6 +437,38 @@ def sc_incre_boundary(Uxb, Uzb):
             raise TypeError("%s must be C-contiguous, writeable, of shape (Nz, Nx/2)" % name)
     ub, wb = Uxb, Uzb
     _check(_L.sc_incre_boundary(ub.ctypes.data, wb.ctypes.data), "sc_incre_boundary")
+
+
+def sc_increadjproject(k, count, IAC, IU, LIU, walls, force=None):
+    """One Runge-Kutta stage k of the incremental adjoint over every mode
+    (increadjproject0 and increadjproject, channel/increadjproject0.c:14-357,
+    channel/increadjproject.c:10-645): IAC [Nz, 2, dimR, Nx/2] is updated in
+    place (the next stage's right-hand side for k < 2, the solution at k = 2);
+    IU, LIU [Nz, 5, qpts, Nx/2] are the products of this stage and the next
+    (LIU may be None at k = 2); walls [6, Nz, Nx/2] holds Uxb, Uzb, Uxbt, Uzbt,
+    AUxb, AUzb; force [2, 2, Nz, Nx/2, dimR] or None.  Uses the tangent states
+    MIC[count], MIC[count - 1] of the last tangent run.  Returns (IAU
+    [Nz, 5, qpts, Nx/2], IAUb [2, Nz, Nx/2])."""
+    Nz, Mx, nR, q = _L.c_Nz(), _L.c_Nx() // 2, _L.c_dimR(), _L.c_qpts()
+    IAC = _inplace(IAC, np.complex128, "IAC")
+    assert IAC.shape == (Nz, 2, nR, Mx), "IAC must be (Nz, 2, dimR, Nx/2)"
+
+    def arr(a, shape, name):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        assert a.shape == shape, "%s must have shape %s" % (name, shape)
+        return a
+
+    IU = arr(IU, (Nz, 5, q, Mx), "IU")
+    LIU = None if LIU is None else arr(LIU, (Nz, 5, q, Mx), "LIU")
+    walls = arr(walls, (6, Nz, Mx), "walls")
+    force = None if force is None else arr(force, (2, 2, Nz, Mx, nR), "force")
+    IAU = np.zeros((Nz, 5, q, Mx), complex)
+    IAUb = np.zeros((2, Nz, Mx), complex)
+    _check(_L.sc_increadjproject(k, count, IAC.ctypes.data, IU.ctypes.data,
+                                 None if LIU is None else LIU.ctypes.data, walls.ctypes.data,
+                                 None if force is None else force.ctypes.data,
+                                 IAU.ctypes.data, IAUb.ctypes.data), "sc_increadjproject")
+    return IAU, IAUb
 
 
 def sc_mean_profile(incre, start_step, end_step, weight):

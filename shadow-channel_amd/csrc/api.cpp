@@ -187,7 +187,7 @@ static void free_all(Context &c)
                     c.MIC, c.Csol, c.Ctmp, c.Ctmp2, c.Cprev, c.ACinit, c.Fa, c.Fb, c.FLa, c.FLb, c.W1, c.W2, c.W2b,
                     c.SP, c.flowdev, c.cflmax, c.dotpart, c.scal, c.abort_flag, c.pz.tw,
                     c.px.tw, c.dFragR, c.dFragF, c.dFragP, c.dFragQ, c.wcache, c.rb_queue,
-                    c.objscr, c.dRwall, c.dGwall};
+                    c.objscr, c.dRwall, c.dGwall, c.dIadj, c.iadjbuf};
     for (void *p : ptrs)
         if (p) (void) hipFree(p);
     for (int a = 0; a < 2; ++a)
@@ -1421,6 +1421,44 @@ int sc_incre_boundary(sc_complex *Uxb, sc_complex *Uzb)
     launch_incre_boundary(c, c.Ctmp2, c.Ctmp2 + np, c.W1);
     SC_HIP(hipMemcpyAsync(Uxb, c.Ctmp2, sizeof(cplx) * np, hipMemcpyDeviceToHost, c.stream));
     SC_HIP(hipMemcpyAsync(Uzb, c.Ctmp2 + np, sizeof(cplx) * np, hipMemcpyDeviceToHost, c.stream));
+    return stream_sync(c);
+}
+
+int sc_increadjproject(int k, int count, sc_complex *IAC, const sc_complex *IU,
+                       const sc_complex *LIU, const sc_complex *walls, const sc_complex *force,
+                       sc_complex *IAU, sc_complex *IAUb)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    const Dims &d = c.d;
+    if (k < 0 || k > 2 || count < (k < 2 ? 1 : 0) || count > 3 * d.nsteps || !IAC || !IU ||
+        (k < 2 && !LIU) || !walls || !IAU || !IAUb) {
+        last_error() = "increadjproject requires 0 <= k <= 2, 1 <= count <= 3 nsteps (0 allowed "
+                       "at k = 2), and IAC, IU, LIU (k < 2), walls, IAU, IAUb";
+        return SC_EINVAL;
+    }
+    if (d.dimR > 128) {
+        last_error() = "increadjproject supports dimR <= 128 (Ny <= 130)";
+        return SC_EINVAL;
+    }
+    if (iadj_prepare(c)) return SC_EINIT;
+    const size_t np = (size_t) d.Nz * d.Mx, nf = 4 * np * d.dimR;
+    cplx *bIAC = c.iadjbuf, *bIU = bIAC + c.csize, *bLIU = bIU + c.usize, *bIAU = bLIU + c.usize;
+    cplx *bW = bIAU + c.usize, *bF = bW + 6 * np, *bIAUb = bF + nf;
+    SC_HIP(hipMemcpyAsync(bIAC, IAC, sizeof(cplx) * c.csize, hipMemcpyHostToDevice, c.stream));
+    SC_HIP(hipMemcpyAsync(bIU, IU, sizeof(cplx) * c.usize, hipMemcpyHostToDevice, c.stream));
+    if (k < 2)
+        SC_HIP(hipMemcpyAsync(bLIU, LIU, sizeof(cplx) * c.usize, hipMemcpyHostToDevice, c.stream));
+    SC_HIP(hipMemcpyAsync(bW, walls, sizeof(cplx) * 6 * np, hipMemcpyHostToDevice, c.stream));
+    if (force)
+        SC_HIP(hipMemcpyAsync(bF, force, sizeof(cplx) * nf, hipMemcpyHostToDevice, c.stream));
+    if (launch_iadj(c, k, count, bIAC, bIU, bLIU, bW, force ? bF : nullptr, bIAU, bIAUb)) {
+        last_error() = "increadjproject: launch failed";
+        return SC_EINIT;
+    }
+    SC_HIP(hipMemcpyAsync(IAC, bIAC, sizeof(cplx) * c.csize, hipMemcpyDeviceToHost, c.stream));
+    SC_HIP(hipMemcpyAsync(IAU, bIAU, sizeof(cplx) * c.usize, hipMemcpyDeviceToHost, c.stream));
+    SC_HIP(hipMemcpyAsync(IAUb, bIAUb, sizeof(cplx) * 2 * np, hipMemcpyDeviceToHost, c.stream));
     return stream_sync(c);
 }
 

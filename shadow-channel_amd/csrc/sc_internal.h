@@ -200,6 +200,10 @@ struct Context {
     int *abort_flag = nullptr;
     double *objscr = nullptr;                   // objectives and statistics (objective_scratch_doubles)
     double *dRwall = nullptr;                   // [2][dimR]: R_j'(-1), R_j'(+1)
+    // incremental adjoint stage (sc_increadjproject): transposed bases and
+    // lift values, and the staging of its host arrays; built at its first call
+    double *dIadj = nullptr;
+    cplx *iadjbuf = nullptr;
     double *dGwall = nullptr;                   // [dimR]: Rs^{-1} Rp0 / (Px Pz), the wall shear's representer
     FftPlan pz, px;
     size_t csize = 0, usize = 0;                // element counts
@@ -349,6 +353,13 @@ void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, co
                       double s, size_t n, cplx *acc, double a, int nparts, int twice,
                       const double *part, double *scal, const cplx *P0 = nullptr,
                       const cplx *M0 = nullptr);
+// incremental adjoint stage k over every mode (iadj_kernels.hip): device
+// arrays in the reference's layouts, the tangent states MIC[count] and
+// MIC[count - 1]; iadj_prepare builds the constants and the staging buffer
+// (IAC, IU, LIU, IAU, walls, forcing, IAUb, one after the other)
+int iadj_prepare(Context &c);
+int launch_iadj(Context &c, int k, int count, cplx *IAC, const cplx *IU, const cplx *LIU,
+                const cplx *W, const cplx *F, cplx *IAU, cplx *IAUb);
 // the no-allocation entry's scratch array: its device address for the
 // current device, looked up once per device (0 on success)
 int noalloc_setup();

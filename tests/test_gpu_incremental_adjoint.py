@@ -1,0 +1,124 @@
+"""GPU parity of the incremental adjoint stage (sc_increadjproject, the HIP
+kernels of increadjproject0 / increadjproject, channel/increadjproject0.c:
+14-357 and channel/increadjproject.c:10-645) against the oracle's
+restatement on the same seeded inputs, after the same primal and tangent
+runs (whose stored tangent states MIC[count], MIC[count - 1] enter the
+stage, each side's own).
+
+Tolerance: relative L2 per output array, a few times the stage's own
+rounding floor on these inputs (white noise in every mode, so the stiffest
+modes carry weight): the oracle built with FMA contraction differs from the
+plain oracle by at most 1.1e-11 (16x33x12), 2.0e-12 (12x25x10) and 1.7e-9
+(32x65x32, the Gauss-point values and wall values of stage 0) on the same
+inputs, each after its own primal and tangent runs.
+"""
+import numpy as np
+import pytest
+
+import pyorc
+from channel.synthetic import synthetic_state
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"16x33x12": 1e-10, "12x25x10": 1e-10, "32x65x32": 5e-9}
+GRIDS = {
+    "16x33x12": dict(Nx=16, Ny=33, Nz=12, Lx=np.pi, Lz=0.3 * np.pi, Re=2000.0, flux=4.0 / 3.0,
+                     dt=0.005),
+    "12x25x10": dict(Nx=12, Ny=25, Nz=10, Lx=2.0, Lz=1.3, Re=800.0, flux=4.0 / 3.0, dt=0.01),
+    "32x65x32": dict(Nx=32, Ny=65, Nz=32, Lx=np.pi, Lz=0.3 * np.pi, Re=2000.0, flux=4.0 / 3.0,
+                     dt=0.005),
+}
+NSTEPS = 2
+
+
+@pytest.fixture(scope="module")
+def cc():
+    import c_channel
+    yield c_channel
+    c_channel.c_destroy()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def setup(cc, name):
+    c = GRIDS[name]
+    o = pyorc.Oracle(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"], NSTEPS)
+    assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"],
+                     NSTEPS) == 0
+    C0 = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=3, amp=0.02)
+    D0 = synthetic_state(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["flux"], seed=4, unit=True)
+    Co, Cg = C0.copy(), C0.copy()
+    o.primal(0, Co)
+    cc.c_primal(0, Cg)
+    Io, Ig = D0.copy(), D0.copy()
+    o.tangent(0, NSTEPS, Io, 0)
+    cc.c_tangent(0, NSTEPS, Ig, 0)
+    # the stored tangent states of the two sides differ at the tangent's own
+    # rounding floor (1.1e-9 after two steps at 32x65x32); they enter the
+    # stage scaled by re dt
+    assert rel(Ig, Io) <= 1e-8
+    return o
+
+
+def inputs(o, seed, force):
+    rng = np.random.default_rng(seed)
+    Nz, Mx = o.cshape[0], o.cshape[3]
+
+    def rc(shape):
+        return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+    IAC, IU, LIU = rc(o.cshape), rc(o.ushape), rc(o.ushape)
+    W = rc((6, Nz, Mx))
+    F = rc((2, 2, Nz, Mx, o.dimR)) if force else None
+    return IAC, IU, LIU, W, F
+
+
+@pytest.mark.parametrize("name", list(GRIDS))
+def test_stages_match_the_oracle(cc, name):
+    o = setup(cc, name)
+    for k in (0, 1, 2):
+        for force in (False, True):
+            print("case: increadjproject k=%d force=%d %s" % (k, force, name), flush=True)
+            count = 5 - k
+            IAC, IU, LIU, W, F = inputs(o, 10 * k + force, force)
+            ref, IAUo, IAUbo = o.increadjproject(k, count, IAC, IU, LIU, W, F)
+            got = IAC.copy()
+            IAU, IAUb = cc.sc_increadjproject(k, count, got, IU, None if k == 2 else LIU, W, F)
+            tol = TOL[name]
+            assert rel(got, ref) <= tol, (k, force, rel(got, ref))
+            assert rel(IAU, IAUo) <= tol, (k, force, rel(IAU, IAUo))
+            assert rel(IAUb, IAUbo) <= tol, (k, force, rel(IAUb, IAUbo))
+
+
+def test_three_stages_in_a_row(cc):
+    """Stages 0, 1, 2 of one step (count 6, 5, 4), the IAC of each stage the
+    next one's input, the products of stage k + 1 as LIU of stage k."""
+    o = setup(cc, "16x33x12")
+    print("case: increadjproject stages 0..2 chained, 16x33x12", flush=True)
+    IAC, _, _, W, F = inputs(o, 99, True)
+    rng = np.random.default_rng(5)
+    prods = [rng.standard_normal(o.ushape) + 1j * rng.standard_normal(o.ushape) for _ in range(3)]
+    ro, rg = IAC.copy(), IAC.copy()
+    for k in range(3):
+        LIU = prods[k + 1] if k < 2 else None
+        ro, IAUo, IAUbo = o.increadjproject(k, 6 - k, ro, prods[k], LIU, W, F)
+        IAU, IAUb = cc.sc_increadjproject(k, 6 - k, rg, prods[k], LIU, W, F)
+        tol = TOL["16x33x12"]
+        assert rel(rg, ro) <= tol
+        assert rel(IAU, IAUo) <= tol
+        assert rel(IAUb, IAUbo) <= tol
+
+
+def test_arguments_are_checked(cc):
+    o = setup(cc, "12x25x10")
+    IAC, IU, LIU, W, _ = inputs(o, 1, False)
+    for k, count, liu in ((3, 3, LIU), (0, 0, LIU), (1, 3 * NSTEPS + 1, LIU), (0, 3, None)):
+        with pytest.raises(Exception):
+            cc.sc_increadjproject(k, count, IAC.copy(), IU, liu, W)
+    # count 0 is allowed at the last stage (no MIC[count - 1] is read)
+    got = IAC.copy()
+    cc.sc_increadjproject(2, 0, got, IU, None, W)
+    ref, _, _ = o.increadjproject(2, 0, IAC, IU, None, W)
+    assert rel(got, ref) <= TOL["12x25x10"]
