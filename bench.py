@@ -201,7 +201,8 @@ def gradient_errors(cc):
 FLUSH_MB = 320       # above the 256 MB Infinity Cache
 
 TRAFFIC_PROFILES = {(6, 8): "profiles/solve_8members_mixed_64x64x97_pmc.md",
-                    (6, 32): "profiles/solve_32members_mixed_64x64x97_pmc.md"}
+                    (6, 32): "profiles/solve_32members_mixed_64x64x97_pmc.md",
+                    (8, 1): "profiles/iadj_stage_64x64x97.md"}
 
 
 def measured_traffic(kernel_class, members):
@@ -330,7 +331,7 @@ def incremental_adjoint_line(cc, calls, cpu=True):
                        "bytes_per_launch": round(nbytes / n)},
             "roofline": {"kernel": "incremental adjoint stage (all launches)", "bound": "mfma", "achieved": round(tf, 3),
                          "peak": 78.6, "unit": "TFLOP/s", "frac": round(tf / 78.6, 4),
-                         "traffic": None, "hbm_achieved_gbs": round(gbs, 1),
+                         "traffic": measured_traffic(8, 1), "hbm_achieved_gbs": round(gbs, 1),
                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                          "avg_launch_us": round(avg_s * 1e6, 2)},
             "cpu_baseline": base}
