@@ -131,6 +131,19 @@ int sc_wall_plane(const sc_complex *C_ptr, sc_complex *Uxb, sc_complex *Uzb);
    Nz < 4. */
 int sc_incre_boundary(sc_complex *Uxb, sc_complex *Uzb);
 
+/* channel/pass2.c:120-566, the incremental side of int pass2(int dctr, int n)
+   (the reference computes it inside pass2 with the adjoint side; the
+   incremental adjoint's driver loop uses it, shan/increadjoint_test7.c:
+   640-700): from the Gauss-point arrays [Nz][5][qpts][Nx/2] of the state U,
+   the tangent IU, the adjoint AU and the incremental adjoint IAU (slots u, v,
+   w, du/dy, dw/dy), IS = u x I lambda + Iu x lambda and IT = lambda x I omega
+   + I lambda x omega on the dealiased grid, then IH <- -IT - the x,z curl of
+   IS in slots 0..2 (H_z, H_x, H_y) and IS_x, IS_z in slots 3, 4: the
+   products sc_increadjproject takes as IU.  Host arrays; row Nz/2 of IH is
+   zero.  FP64 whatever the precision setting. */
+int sc_pass2_incremental(const sc_complex *U, const sc_complex *IU, const sc_complex *AU,
+                         const sc_complex *IAU, sc_complex *IH);
+
 /* channel/increadjproject0.c:14-357 (void increadjproject0(int k, int n,
    int count, func_force_t force)) for the mean mode and
    channel/increadjproject.c:10-645 (void increadjproject(int n, int k, int z,

@@ -58,6 +58,8 @@ def load():
     L.orc_increadjproject.restype = c.c_int
     L.orc_adjproject_stage.argtypes = [c.c_int] + [c.c_void_p] * 4
     L.orc_adjproject_stage.restype = c.c_int
+    L.orc_pass2_incremental.argtypes = [c.c_void_p] * 5
+    L.orc_pass2_incremental.restype = c.c_int
     L.orc_get_array.argtypes = [c.c_char_p, dp, c.c_int]
     L.orc_get_array.restype = c.c_int
     L.orc_get_scalar.argtypes = [c.c_char_p]
@@ -179,6 +181,15 @@ class Oracle:
                                         None if F is None else _ptr(F), _ptr(IAU), _ptr(IAUb))
         assert rc == 0, rc
         return IAC, IAU, IAUb
+
+    def pass2_incremental(self, U, IU, AU, IAU):
+        """The incremental side of pass2 (channel/pass2.c:120-566): returns
+        IH [Nz][5][qpts][Nx/2] (H slots 0..2, IS_x, IS_z in slots 3, 4)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in (U, IU, AU, IAU)]
+        IH = np.zeros(self.ushape, np.complex128)
+        rc = self.L.orc_pass2_incremental(*[_ptr(a) for a in arrs], _ptr(IH))
+        assert rc == 0, rc
+        return IH
 
     def adjproject_stage(self, k, AC, U, LU):
         """One adjoint stage k (channel/adjproject0.c, channel/adjproject.c)

@@ -605,6 +605,84 @@ static void pass2(cx *Ua, const cx *AUa, int incremental_cost)
 
 void orc_pass2_state(orc_cplx *Ua, const orc_cplx *AUa) { pass2(Ua, AUa, 0); }
 
+/* The incremental side of the adjoint convective operator
+   (channel/pass2.c:120-566): per plane the state (u, omega) of U, the
+   tangent (Iu, I omega) of IU, the adjoint velocity lambda of AU and the
+   incremental adjoint velocity I lambda of IAU at the Gauss points;
+   IS = u x I lambda + Iu x lambda, IT = lambda x I omega + I lambda x omega
+   (channel/pass2.c:227-330); IH = -IT - the x,z curl of IS with IS_x, IS_z
+   left in the du/dy, dw/dy slots (channel/pass2.c:474-566), into IH. */
+int orc_pass2_incremental(const orc_cplx *U, const orc_cplx *IU, const orc_cplx *AU,
+                          const orc_cplx *IAU, orc_cplx *IH)
+{
+    if (!G.ok || !U || !IU || !AU || !IAU || !IH) return -1;
+    int Nz = G.Nz, Mx = G.Mx;
+    long fs = (long) Nz * Mx, ps = (long) G.Pz * G.Px;
+    cx *sp = calloc((size_t) 18 * fs, sizeof(cx));
+    double *ph = calloc((size_t) 18 * ps, sizeof(double));
+    if (!sp || !ph) { free(sp); free(ph); return -1; }
+    const cx *Ua = (const cx *) U, *IUa = (const cx *) IU;
+    const cx *AUa = (const cx *) AU, *IAUa = (const cx *) IAU;
+    cx *Ha = (cx *) IH;
+    for (int y = 0; y < G.qpts; ++y) {
+        load_curl(Ua, y, sp);
+        load_curl(IUa, y, sp + 6 * fs);
+        for (int s = 0; s < 3; ++s)
+            for (int z = 0; z < Nz; ++z) {
+                if (z == Nz / 2) continue;
+                for (int x = 0; x < Mx; ++x) {
+                    sp[(12 + s) * fs + (long) z * Mx + x] = AUa[ui(z, s, y, x)];
+                    sp[(15 + s) * fs + (long) z * Mx + x] = IAUa[ui(z, s, y, x)];
+                }
+            }
+        for (int f = 0; f < 18; ++f) inv2d(sp + f * fs, Mx, ph + f * ps);
+        const double *u = ph, *v = u + ps, *w = v + ps, *oz = w + ps, *ox = oz + ps, *oy = ox + ps;
+        const double *iu = oy + ps, *iv = iu + ps, *iw = iv + ps;
+        const double *ioz = iw + ps, *iox = ioz + ps, *ioy = iox + ps;
+        const double *lx = ioy + ps, *ly = lx + ps, *lz = ly + ps;
+        const double *ilx = lz + ps, *ily = ilx + ps, *ilz = ily + ps;
+        double tmp[6];
+        for (long p = 0; p < ps; ++p) {
+            tmp[0] = v[p] * ilz[p] - w[p] * ily[p] + iv[p] * lz[p] - iw[p] * ly[p];
+            tmp[1] = w[p] * ilx[p] - u[p] * ilz[p] + iw[p] * lx[p] - iu[p] * lz[p];
+            tmp[2] = u[p] * ily[p] - v[p] * ilx[p] + iu[p] * ly[p] - iv[p] * lx[p];
+            tmp[3] = ly[p] * ioz[p] - lz[p] * ioy[p] + ily[p] * oz[p] - ilz[p] * oy[p];
+            tmp[4] = lz[p] * iox[p] - lx[p] * ioz[p] + ilz[p] * ox[p] - ilx[p] * oz[p];
+            tmp[5] = lx[p] * ioy[p] - ly[p] * iox[p] + ilx[p] * oy[p] - ily[p] * ox[p];
+            /* planes 0..5 (the state's) are read only at p: overwrite in place */
+            for (int f = 0; f < 6; ++f) ph[f * ps + p] = tmp[f];
+        }
+        for (int f = 0; f < 6; ++f) fwd2d(ph + f * ps, sp + f * fs, Mx);
+        const cx *Sx = sp, *Sy = Sx + fs, *Sz = Sy + fs;
+        const cx *Tx = Sz + fs, *Ty = Tx + fs, *Tz = Ty + fs;
+        for (int z = 0; z < Nz; ++z) {
+            if (z == Nz / 2) {
+                for (int s = 0; s < 5; ++s)
+                    for (int x = 0; x < Mx; ++x) Ha[ui(z, s, y, x)].re = Ha[ui(z, s, y, x)].im = 0.0;
+                continue;
+            }
+            double kz = G.Kz[z];
+            for (int x = 0; x < Mx; ++x) {
+                double kx = G.Kx[x];
+                long o = (long) z * Mx + x;
+                cx *hx = Ha + ui(z, S_HX, y, x), *hy = Ha + ui(z, S_HY, y, x);
+                cx *hz = Ha + ui(z, S_HZ, y, x);
+                Ha[ui(z, S_DU, y, x)] = Sx[o];
+                Ha[ui(z, S_DW, y, x)] = Sz[o];
+                hx->re = -Tx[o].re - kz * Sy[o].im;
+                hx->im = -Tx[o].im + kz * Sy[o].re;
+                hy->re = -Ty[o].re + kz * Sx[o].im - kx * Sz[o].im;
+                hy->im = -Ty[o].im - kz * Sx[o].re + kx * Sz[o].re;
+                hz->re = -Tz[o].re + kx * Sy[o].im;
+                hz->im = -Tz[o].im - kx * Sy[o].re;
+            }
+        }
+    }
+    free(sp);
+    free(ph);
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* coefficients -> Gauss points (channel/initAlphaBeta2.c:12-240)       */
 /* ------------------------------------------------------------------ */

@@ -115,6 +115,13 @@ int    orc_pass1_state(orc_cplx *U, int check_cfl);
 /* Adjoint nonlinear term: U (state velocities) and AU (adjoint velocities)
    in; U[z][HZ,HX,HY] <- adjoint H, U[z][DX] <- S_x, U[z][DZ] <- S_z. */
 void   orc_pass2_state(orc_cplx *U, const orc_cplx *AU);
+/* The incremental side of the adjoint nonlinear term (channel/pass2.c:
+   120-566): U (state), IU (tangent), AU (adjoint), IAU (incremental
+   adjoint) Gauss-point arrays in; IH [Nz][5][qpts][Nx/2] <- slots 0..2 the
+   incremental adjoint H, slots 3, 4 IS_x, IS_z (what pass2 leaves in IU).
+   -1 on bad arguments. */
+int orc_pass2_incremental(const orc_cplx *U, const orc_cplx *IU, const orc_cplx *AU,
+                          const orc_cplx *IAU, orc_cplx *IH);
 /* C -> U at Gauss points (initAlphaBeta2). */
 void   orc_coef_to_gauss(const orc_cplx *C, orc_cplx *U);
 /* Copy the current working state C / U of the solver. */

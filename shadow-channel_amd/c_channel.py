@@ -84,6 +84,8 @@ _L.sc_incre_boundary.argtypes = [_c.c_void_p, _c.c_void_p]
 _L.sc_incre_boundary.restype = _i
 _L.sc_increadjproject.argtypes = [_i, _i] + [_c.c_void_p] * 7
 _L.sc_increadjproject.restype = _i
+_L.sc_pass2_incremental.argtypes = [_c.c_void_p] * 5
+_L.sc_pass2_incremental.restype = _i
 _L.sc_primal_dev.argtypes = [_i, _c.c_void_p]
 _L.sc_primal_dev.restype = _i
 _L.sc_adjoint_dev.argtypes = [_i, _i, _c.c_void_p, _d]
@@ -437,6 +439,23 @@ def sc_incre_boundary(Uxb, Uzb):
             raise TypeError("%s must be C-contiguous, writeable, of shape (Nz, Nx/2)" % name)
     ub, wb = Uxb, Uzb
     _check(_L.sc_incre_boundary(ub.ctypes.data, wb.ctypes.data), "sc_incre_boundary")
+
+
+def sc_pass2_incremental(U, IU, AU, IAU):
+    """The incremental side of pass2 (channel/pass2.c:120-566) from the
+    Gauss-point arrays [Nz, 5, qpts, Nx/2] of the state, tangent, adjoint and
+    incremental adjoint: returns IH (H slots 0..2, IS_x, IS_z in slots 3, 4),
+    the products sc_increadjproject takes."""
+    Nz, Mx, q = _L.c_Nz(), _L.c_Nx() // 2, _L.c_qpts()
+    arrs = []
+    for a, name in ((U, "U"), (IU, "IU"), (AU, "AU"), (IAU, "IAU")):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        assert a.shape == (Nz, 5, q, Mx), "%s must be (Nz, 5, qpts, Nx/2)" % name
+        arrs.append(a)
+    IH = np.zeros((Nz, 5, q, Mx), complex)
+    _check(_L.sc_pass2_incremental(*[a.ctypes.data for a in arrs], IH.ctypes.data),
+           "sc_pass2_incremental")
+    return IH
 
 
 def sc_increadjproject(k, count, IAC, IU, LIU, walls, force=None):

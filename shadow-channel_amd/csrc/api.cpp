@@ -187,7 +187,7 @@ static void free_all(Context &c)
                     c.MIC, c.Csol, c.Ctmp, c.Ctmp2, c.Cprev, c.ACinit, c.Fa, c.Fb, c.FLa, c.FLb, c.W1, c.W2, c.W2b,
                     c.SP, c.flowdev, c.cflmax, c.dotpart, c.scal, c.abort_flag, c.pz.tw,
                     c.px.tw, c.dFragR, c.dFragF, c.dFragP, c.dFragQ, c.wcache, c.rb_queue,
-                    c.objscr, c.dRwall, c.dGwall, c.dIadj, c.iadjbuf};
+                    c.objscr, c.dRwall, c.dGwall, c.dIadj, c.iadjbuf, c.p2ibuf};
     for (void *p : ptrs)
         if (p) (void) hipFree(p);
     for (int a = 0; a < 2; ++a)
@@ -1421,6 +1421,39 @@ int sc_incre_boundary(sc_complex *Uxb, sc_complex *Uzb)
     launch_incre_boundary(c, c.Ctmp2, c.Ctmp2 + np, c.W1);
     SC_HIP(hipMemcpyAsync(Uxb, c.Ctmp2, sizeof(cplx) * np, hipMemcpyDeviceToHost, c.stream));
     SC_HIP(hipMemcpyAsync(Uzb, c.Ctmp2 + np, sizeof(cplx) * np, hipMemcpyDeviceToHost, c.stream));
+    return stream_sync(c);
+}
+
+int sc_pass2_incremental(const sc_complex *U, const sc_complex *IU, const sc_complex *AU,
+                         const sc_complex *IAU, sc_complex *IH)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!U || !IU || !AU || !IAU || !IH) {
+        last_error() = "pass2_incremental requires U, IU, AU, IAU and IH";
+        return SC_EINVAL;
+    }
+    const size_t us = c.usize;
+    if (!c.p2ibuf && hipMalloc(&c.p2ibuf, sizeof(cplx) * 6 * us) != hipSuccess) {
+        c.p2ibuf = nullptr;
+        last_error() = "pass2_incremental: device allocation failed";
+        return SC_EINIT;
+    }
+    cplx *bU = c.p2ibuf, *bIU = bU + us, *bAU = bIU + us, *bIAU = bAU + us, *T1 = bIAU + us,
+         *T2 = T1 + us;
+    SC_HIP(hipMemcpyAsync(bU, U, sizeof(cplx) * us, hipMemcpyHostToDevice, c.stream));
+    SC_HIP(hipMemcpyAsync(bIU, IU, sizeof(cplx) * us, hipMemcpyHostToDevice, c.stream));
+    SC_HIP(hipMemcpyAsync(bAU, AU, sizeof(cplx) * us, hipMemcpyHostToDevice, c.stream));
+    SC_HIP(hipMemcpyAsync(bIAU, IAU, sizeof(cplx) * us, hipMemcpyHostToDevice, c.stream));
+    // bilinear in (state, lambda): the adjoint operator of (u, I lambda)
+    // plus that of (Iu, lambda), in FP64 whatever the context's precision
+    const bool mixed = c.mixed;
+    c.mixed = false;
+    pass2_adjoint(c, bU, bIAU, T1);
+    pass2_adjoint(c, bIU, bAU, T2);
+    launch_gauss_sum(c, T1, T2);
+    c.mixed = mixed;
+    SC_HIP(hipMemcpyAsync(IH, T1, sizeof(cplx) * us, hipMemcpyDeviceToHost, c.stream));
     return stream_sync(c);
 }
 

@@ -568,7 +568,23 @@ __global__ __launch_bounds__(256) void k_iadj_lift(IadjArgs a)
     o[4 * plane] = cadd(o[4 * plane], dw);
 }
 
+// a += b, row Nz/2 of a cleared (one thread per complex value)
+__global__ __launch_bounds__(256) void k_gauss_sum(cplx *a, const cplx *b, size_t n, size_t per_z,
+                                                   int zhalf)
+{
+    const size_t e = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    a[e] = (e / per_z == (size_t) zhalf) ? make_double2(0.0, 0.0) : cadd(a[e], b[e]);
+}
+
 }  // namespace
+
+void launch_gauss_sum(Context &c, cplx *a, const cplx *b)
+{
+    const size_t n = c.usize, per_z = (size_t) 5 * c.d.qpts * c.d.Mx;
+    hipLaunchKernelGGL(k_gauss_sum, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, c.stream, a, b,
+                       n, per_z, c.d.Nz / 2);
+}
 
 // Constants of the incremental adjoint on the device (the lift values at the
 // Gauss points, the mean mode's lift projection, Rw and Rpw transposed) and

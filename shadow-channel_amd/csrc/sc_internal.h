@@ -204,6 +204,7 @@ struct Context {
     // lift values, and the staging of its host arrays; built at its first call
     double *dIadj = nullptr;
     cplx *iadjbuf = nullptr;
+    cplx *p2ibuf = nullptr;     // sc_pass2_incremental: 4 inputs and 2 product sets
     double *dGwall = nullptr;                   // [dimR]: Rs^{-1} Rp0 / (Px Pz), the wall shear's representer
     FftPlan pz, px;
     size_t csize = 0, usize = 0;                // element counts
@@ -360,6 +361,8 @@ void launch_ddt_apply(Context &c, cplx *vout, const cplx *vin, const cplx *P, co
 int iadj_prepare(Context &c);
 int launch_iadj(Context &c, int k, int count, cplx *IAC, const cplx *IU, const cplx *LIU,
                 const cplx *W, const cplx *F, cplx *IAU, cplx *IAUb);
+// a += b over Gauss-point arrays [Nz][5][qpts][Mx], row Nz/2 of a cleared
+void launch_gauss_sum(Context &c, cplx *a, const cplx *b);
 // the no-allocation entry's scratch array: its device address for the
 // current device, looked up once per device (0 on success)
 int noalloc_setup();

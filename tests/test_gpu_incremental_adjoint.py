@@ -122,3 +122,32 @@ def test_arguments_are_checked(cc):
     cc.sc_increadjproject(2, 0, got, IU, None, W)
     ref, _, _ = o.increadjproject(2, 0, IAC, IU, None, W)
     assert rel(got, ref) <= TOL["12x25x10"]
+
+
+@pytest.mark.parametrize("name", ["16x33x12", "32x65x32"])
+def test_incremental_products_match_the_oracle(cc, name):
+    """sc_pass2_incremental (the incremental side of channel/pass2.c) against
+    the oracle's restatement on Gauss-point fields rebuilt from seeded
+    coefficient arrays; then those products through a stage on both sides."""
+    o = setup(cc, name)
+    print("case: pass2 incremental products, %s" % name, flush=True)
+    Nz = o.cshape[0]
+
+    def field(seed):
+        r = np.random.default_rng(seed)
+        C = r.standard_normal(o.cshape) + 1j * r.standard_normal(o.cshape)
+        C[Nz // 2] = 0
+        C[0, :, :, 0] = C[0, :, :, 0].real
+        return o.coef_to_gauss(C)
+
+    U, IU, AU, IAU = field(1), field(2), field(3), field(4)
+    ref = o.pass2_incremental(U, IU, AU, IAU)
+    got = cc.sc_pass2_incremental(U, IU, AU, IAU)
+    assert rel(got, ref) <= 1e-12, rel(got, ref)
+    assert not np.any(got[Nz // 2])
+    IAC, _, _, W, _ = inputs(o, 7, False)
+    r0, IAUo, _ = o.increadjproject(2, 3, IAC, ref, None, W)
+    g0 = IAC.copy()
+    IAUg, _ = cc.sc_increadjproject(2, 3, g0, got, None, W)
+    assert rel(g0, r0) <= TOL[name]
+    assert rel(IAUg, IAUo) <= TOL[name]

@@ -302,3 +302,30 @@ def test_mean_mode_forcing_parts(case):
     dt = o.scalar("dt")
     np.testing.assert_allclose(moved[0, :, :, 0] - base[0, :, :, 0], dt * D[2] * F[1, :, 0, 0, :],
                                rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("grid", [(8, 17, 8), (12, 21, 10)])
+def test_incremental_products_are_two_adjoint_product_sets(grid):
+    """pass2's incremental side (channel/pass2.c:227-330: IS = u x I lambda +
+    Iu x lambda, IT = lambda x I omega + I lambda x omega) is bilinear: it
+    equals the pinned adjoint operator with the state u and I lambda plus the
+    same with the tangent Iu and lambda; zero when I lambda and Iu vanish."""
+    Nx, Ny, Nz = grid
+    o = pyorc.Oracle(Nx, Ny, Nz, 2.0, 1.3, RE_NUM, 4.0 / 3.0, DT, 1)
+
+    def field(seed):
+        r = np.random.default_rng(seed)
+        C = r.standard_normal(o.cshape) + 1j * r.standard_normal(o.cshape)
+        C[Nz // 2] = 0
+        C[0, :, :, 0] = C[0, :, :, 0].real
+        return o.coef_to_gauss(C)
+
+    U, IU, AU, IAU = field(1), field(2), field(3), field(4)
+    IH = o.pass2_incremental(U, IU, AU, IAU)
+    ref = o.pass2(U, IAU) + o.pass2(IU, AU)
+    m = np.ones(o.ushape, bool)
+    m[Nz // 2] = False
+    np.testing.assert_allclose(IH[m], ref[m], rtol=0, atol=1e-12 * np.abs(ref[m]).max())
+    assert not np.any(IH[Nz // 2])
+    Z = np.zeros(o.ushape, complex)
+    assert np.abs(o.pass2_incremental(U, Z, AU, Z)).max() == 0.0
