@@ -151,3 +151,40 @@ def test_incremental_products_match_the_oracle(cc, name):
     IAUg, _ = cc.sc_increadjproject(2, 3, g0, got, None, W)
     assert rel(g0, r0) <= TOL[name]
     assert rel(IAUg, IAUo) <= TOL[name]
+
+
+def test_mixed_precision_context_runs_fp64(cc):
+    """In a mixed-precision context both entry points still compute in FP64:
+    the products and a last stage (which reads no tangent state) match the
+    oracle at the FP64 tolerance."""
+    c = GRIDS["16x33x12"]
+    o = pyorc.Oracle(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"], NSTEPS)
+    cc.sc_set_precision(1)
+    try:
+        assert cc.c_init(c["Nx"], c["Ny"], c["Nz"], c["Lx"], c["Lz"], c["Re"], c["flux"], c["dt"],
+                         NSTEPS) == 0
+        assert cc.sc_precision() == 1
+        print("case: mixed-precision context, 16x33x12", flush=True)
+        Nz = o.cshape[0]
+
+        def field(seed):
+            r = np.random.default_rng(seed)
+            C = r.standard_normal(o.cshape) + 1j * r.standard_normal(o.cshape)
+            C[Nz // 2] = 0
+            C[0, :, :, 0] = C[0, :, :, 0].real
+            return o.coef_to_gauss(C)
+
+        U, IU, AU, IAU = field(11), field(12), field(13), field(14)
+        ref = o.pass2_incremental(U, IU, AU, IAU)
+        got = cc.sc_pass2_incremental(U, IU, AU, IAU)
+        assert rel(got, ref) <= 1e-12, rel(got, ref)
+        IAC, _, _, W, F = inputs(o, 21, True)
+        r0, IAUo, IAUbo = o.increadjproject(2, 0, IAC, ref, None, W, F)
+        g0 = IAC.copy()
+        IAUg, IAUbg = cc.sc_increadjproject(2, 0, g0, got, None, W, F)
+        tol = TOL["16x33x12"]
+        assert rel(g0, r0) <= tol
+        assert rel(IAUg, IAUo) <= tol
+        assert rel(IAUbg, IAUbo) <= tol
+    finally:
+        cc.sc_set_precision(0)
