@@ -378,7 +378,7 @@ def run():
     RETUNE = args.retune
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver runs one simulation per device and "
-              "multi-GPU sharding is not part of this build (--gpus must be 1)",
+              "multi-GPU sharding is not implemented (--gpus must be 1)",
               file=sys.stderr)
         return None
     import c_channel as cc

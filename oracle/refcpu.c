@@ -1837,7 +1837,7 @@ int orc_adjoint_objective(int start, int end, orc_cplx *ACg, double weight)
 /* incremental adjoint stage: increadjproject0                          */
 /* (channel/increadjproject0.c:14-357) for (0,0), then increadjproject  */
 /* (channel/increadjproject.c:10-645) for row 0 (x >= 1) and every row  */
-/* z != Nz/2, the order of the driver loop                              */
+/* z != Nz/2, the order of the reference's calling loop               */
 /* (shan/increadjoint_test7.c:673-685).                                 */
 /* ------------------------------------------------------------------ */
 /* Galerkin projections of the wall lifts (channel/increadjproject.c:40-46,

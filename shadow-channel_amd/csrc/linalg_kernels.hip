@@ -1552,7 +1552,7 @@ struct SolveArgs {
     // adjoint only: the mean column as the exact transpose of the tangent's
     // flux correction: at stages 0 and 1 the right-hand side's real alpha
     // loses w1 (add_mass . rhs) / 2 before the solve, and no correction
-    // follows the solve (the driver applies P at each step's start)
+    // follows the solve (the host loop applies P at each step's start)
     int exact_mean;
     const double *add_mass;            // Rs^{-1} w1
     int nm;                            // members of the launch (grid z: groups of them)

@@ -13,7 +13,7 @@ for p in (ROOT, os.path.join(ROOT, "oracle"), PKG):
 
 def _ensure_built():
     """Build the oracle and the HIP library when they are missing (a fresh
-    checkout); the GPU box only receives prebuilt files."""
+    checkout); a machine that receives only the prebuilt files skips this."""
     if not os.path.exists(os.path.join(ROOT, "oracle", "liborc.so")):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
     if not os.path.exists(os.path.join(PKG, "libshadowchannel.so")):
