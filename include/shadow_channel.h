@@ -173,6 +173,17 @@ int sc_increadjproject(int k, int count, sc_complex *IAC, const sc_complex *IU,
                        const sc_complex *LIU, const sc_complex *walls, const sc_complex *force,
                        sc_complex *IAU, sc_complex *IAUb);
 
+/* sc_increadjproject and sc_pass2_incremental on arrays in device memory
+   (same layouts; no host transfer; IAC_dev in place; the output arrays must
+   not overlap the inputs), for callers that chain stages on resident
+   fields.  Results are bitwise those of the host-array calls. */
+int sc_increadjproject_dev(int k, int count, sc_complex *IAC_dev, const sc_complex *IU_dev,
+                           const sc_complex *LIU_dev, const sc_complex *walls_dev,
+                           const sc_complex *force_dev, sc_complex *IAU_dev, sc_complex *IAUb_dev);
+int sc_pass2_incremental_dev(const sc_complex *U_dev, const sc_complex *IU_dev,
+                             const sc_complex *AU_dev, const sc_complex *IAU_dev,
+                             sc_complex *IH_dev);
+
 /* channel/__init__.py:217-299 (velAvg, IvelAvg and the profiles of
    uavgGrad) and the wall shear Uxb(0,0) of channel/project0.c:227-234, as
    device reductions over stored steps start_step .. end_step - 1 of the

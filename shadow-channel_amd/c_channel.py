@@ -86,6 +86,10 @@ _L.sc_increadjproject.argtypes = [_i, _i] + [_c.c_void_p] * 7
 _L.sc_increadjproject.restype = _i
 _L.sc_pass2_incremental.argtypes = [_c.c_void_p] * 5
 _L.sc_pass2_incremental.restype = _i
+_L.sc_increadjproject_dev.argtypes = [_i, _i] + [_c.c_void_p] * 7
+_L.sc_increadjproject_dev.restype = _i
+_L.sc_pass2_incremental_dev.argtypes = [_c.c_void_p] * 5
+_L.sc_pass2_incremental_dev.restype = _i
 _L.sc_primal_dev.argtypes = [_i, _c.c_void_p]
 _L.sc_primal_dev.restype = _i
 _L.sc_adjoint_dev.argtypes = [_i, _i, _c.c_void_p, _d]
@@ -456,6 +460,22 @@ def sc_pass2_incremental(U, IU, AU, IAU):
     _check(_L.sc_pass2_incremental(*[a.ctypes.data for a in arrs], IH.ctypes.data),
            "sc_pass2_incremental")
     return IH
+
+
+def sc_pass2_incremental_dev(U_dev, IU_dev, AU_dev, IAU_dev, IH_dev):
+    """sc_pass2_incremental on DeviceArray operands (IH_dev written)."""
+    _check(_L.sc_pass2_incremental_dev(U_dev.ptr, IU_dev.ptr, AU_dev.ptr, IAU_dev.ptr, IH_dev.ptr),
+           "sc_pass2_incremental_dev")
+
+
+def sc_increadjproject_dev(k, count, IAC_dev, IU_dev, LIU_dev, walls_dev, force_dev, IAU_dev,
+                           IAUb_dev):
+    """sc_increadjproject on DeviceArray operands (LIU_dev and force_dev may
+    be None); IAC_dev in place, IAU_dev and IAUb_dev written."""
+    ptr = lambda a: None if a is None else a.ptr
+    _check(_L.sc_increadjproject_dev(k, count, IAC_dev.ptr, IU_dev.ptr, ptr(LIU_dev),
+                                     walls_dev.ptr, ptr(force_dev), IAU_dev.ptr, IAUb_dev.ptr),
+           "sc_increadjproject_dev")
 
 
 def sc_increadjproject(k, count, IAC, IU, LIU, walls, force=None):

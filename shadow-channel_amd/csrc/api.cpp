@@ -1495,6 +1495,58 @@ int sc_increadjproject(int k, int count, sc_complex *IAC, const sc_complex *IU,
     return stream_sync(c);
 }
 
+int sc_increadjproject_dev(int k, int count, sc_complex *IAC_dev, const sc_complex *IU_dev,
+                           const sc_complex *LIU_dev, const sc_complex *walls_dev,
+                           const sc_complex *force_dev, sc_complex *IAU_dev, sc_complex *IAUb_dev)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    const Dims &d = c.d;
+    if (k < 0 || k > 2 || count < (k < 2 ? 1 : 0) || count > 3 * d.nsteps || !IAC_dev ||
+        !IU_dev || (k < 2 && !LIU_dev) || !walls_dev || !IAU_dev || !IAUb_dev) {
+        last_error() = "increadjproject_dev requires 0 <= k <= 2, 1 <= count <= 3 nsteps (0 "
+                       "allowed at k = 2), and IAC, IU, LIU (k < 2), walls, IAU, IAUb";
+        return SC_EINVAL;
+    }
+    if (d.dimR > 128) {
+        last_error() = "increadjproject supports dimR <= 128 (Ny <= 130)";
+        return SC_EINVAL;
+    }
+    if (iadj_prepare(c)) return SC_EINIT;
+    if (launch_iadj(c, k, count, (cplx *) IAC_dev, (const cplx *) IU_dev, (const cplx *) LIU_dev,
+                    (const cplx *) walls_dev, (const cplx *) force_dev, (cplx *) IAU_dev,
+                    (cplx *) IAUb_dev)) {
+        last_error() = "increadjproject: launch failed";
+        return SC_EINIT;
+    }
+    return stream_sync(c);
+}
+
+int sc_pass2_incremental_dev(const sc_complex *U_dev, const sc_complex *IU_dev,
+                             const sc_complex *AU_dev, const sc_complex *IAU_dev, sc_complex *IH_dev)
+{
+    Context &c = ctx();
+    if (!c.ok) return SC_ENOTREADY;
+    if (!U_dev || !IU_dev || !AU_dev || !IAU_dev || !IH_dev) {
+        last_error() = "pass2_incremental_dev requires U, IU, AU, IAU and IH";
+        return SC_EINVAL;
+    }
+    const size_t us = c.usize;
+    if (!c.p2ibuf && hipMalloc(&c.p2ibuf, sizeof(cplx) * 6 * us) != hipSuccess) {
+        c.p2ibuf = nullptr;
+        last_error() = "pass2_incremental: device allocation failed";
+        return SC_EINIT;
+    }
+    cplx *T2 = c.p2ibuf + 5 * us;
+    const bool mixed = c.mixed;
+    c.mixed = false;
+    pass2_adjoint(c, (const cplx *) U_dev, (const cplx *) IAU_dev, (cplx *) IH_dev);
+    pass2_adjoint(c, (const cplx *) IU_dev, (const cplx *) AU_dev, T2);
+    launch_gauss_sum(c, (cplx *) IH_dev, T2);
+    c.mixed = mixed;
+    return stream_sync(c);
+}
+
 int c_qpts(void) { return ctx().d.qpts; }
 int c_dimR(void) { return ctx().d.dimR; }
 int c_dimQ(void) { return ctx().d.dimQ; }

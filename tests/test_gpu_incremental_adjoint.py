@@ -191,3 +191,41 @@ def test_mixed_precision_context_runs_fp64(cc):
         assert rel(IAUbg, IAUbo) <= tol
     finally:
         cc.sc_set_precision(0)
+
+
+def test_device_entries_equal_host_entries(cc):
+    """sc_pass2_incremental_dev and sc_increadjproject_dev on resident arrays
+    give the host-array entries' results bit for bit, chained: the products
+    formed on the device feed the stage without a host copy."""
+    o = setup(cc, "16x33x12")
+    print("case: device-array entries, 16x33x12", flush=True)
+    Nz = o.cshape[0]
+
+    def field(seed):
+        r = np.random.default_rng(seed)
+        C = r.standard_normal(o.cshape) + 1j * r.standard_normal(o.cshape)
+        C[Nz // 2] = 0
+        C[0, :, :, 0] = C[0, :, :, 0].real
+        return o.coef_to_gauss(C)
+
+    U, IU, AU, IAU = field(31), field(32), field(33), field(34)
+    LIU = field(35)
+    IAC, _, _, W, F = inputs(o, 36, True)
+    IH = cc.sc_pass2_incremental(U, IU, AU, IAU)
+    ref_c = IAC.copy()
+    ref_u, ref_b = cc.sc_increadjproject(0, 4, ref_c, IH, LIU, W, F)
+    dev = {n: cc.DeviceArray(a) for n, a in (("U", U), ("IU", IU), ("AU", AU), ("IAU", IAU),
+                                             ("LIU", LIU), ("IAC", IAC), ("W", W), ("F", F),
+                                             ("IH", np.zeros_like(U)), ("OU", np.zeros_like(U)),
+                                             ("OB", np.zeros((2,) + W.shape[1:], complex)))}
+    try:
+        cc.sc_pass2_incremental_dev(dev["U"], dev["IU"], dev["AU"], dev["IAU"], dev["IH"])
+        np.testing.assert_array_equal(dev["IH"].download(), IH)
+        cc.sc_increadjproject_dev(0, 4, dev["IAC"], dev["IH"], dev["LIU"], dev["W"], dev["F"],
+                                  dev["OU"], dev["OB"])
+        np.testing.assert_array_equal(dev["IAC"].download(), ref_c)
+        np.testing.assert_array_equal(dev["OU"].download(), ref_u)
+        np.testing.assert_array_equal(dev["OB"].download(), ref_b)
+    finally:
+        for a in dev.values():
+            a.free()
