@@ -490,6 +490,16 @@ def run():
     }
     if grad is not None:
         line["config"]["gradient_errors"] = grad
+    # the incremental adjoint stage at the same grid (untimed for the
+    # headline; its own line: --incremental-adjoint)
+    try:
+        ia = incremental_adjoint_line(cc, 10, cpu=False)
+        line["config"]["incremental_adjoint_stage"] = {
+            "stages_per_s": ia["value"], "avg_us": ia["roofline"]["avg_launch_us"],
+            "tflops": ia["roofline"]["achieved"], "frac_fp64_peak": ia["roofline"]["frac"],
+            "workload": ia["config"]["workload"]}
+    except Exception as e:   # reported, never fatal for the headline
+        line["config"]["incremental_adjoint_stage"] = {"error": str(e)}
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, 1 thread)")
         line["cpu_baseline"] = cpu_baseline()
