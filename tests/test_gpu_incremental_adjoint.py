@@ -125,6 +125,14 @@ def test_arguments_are_checked(cc):
     cc.sc_increadjproject(2, 0, got, IU, None, W)
     ref, _, _ = o.increadjproject(2, 0, IAC, IU, None, W)
     assert rel(got, ref) <= TOL["12x25x10"]
+    # more than 128 omega_y coefficients (Ny = 133) is refused
+    assert cc.c_init(8, 133, 8, 2.0, 1.3, 800.0, 4.0 / 3.0, 0.01, 1) == 0
+    nR, q = cc.c_dimR(), cc.c_qpts()
+    assert nR > 128
+    z = np.zeros((8, 2, nR, 4), complex)
+    u = np.zeros((8, 5, q, 4), complex)
+    with pytest.raises(Exception):
+        cc.sc_increadjproject(2, 0, z, u, None, np.zeros((6, 8, 4), complex))
 
 
 @pytest.mark.parametrize("name", ["16x33x12", "32x65x32"])
